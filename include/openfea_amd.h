@@ -1,0 +1,157 @@
+/*
+ * openfea_amd.h -- C ABI of the MI355X-native OpenFEA element hot path.
+ *
+ * libofea_hip.so exports exactly these symbols.  All indices are int32,
+ * node ids in element connectivity are 1-based (as in the reference's
+ * node_topos_), every other index is 0-based.  Functions return OFEA_OK (0)
+ * or a negative status; they never exit().  ofea_last_error() gives the
+ * message of the last failure on the calling thread.  One context per host
+ * thread; work on a context is ordered on the context's own HIP stream.
+ * Host buffers are caller-owned and only read/written during the call.
+ *
+ * Reference call surface replaced (citations relative to the reference tree):
+ *   ofea_inp_*            ReadInfo::read_ele_node_num/read_geo_mesh/
+ *                         read_load_bcs/read_dis_bcs/read_mat/read_time
+ *                         (src/data2cae.cpp:18-179, :315-492)
+ *   ofea_build_free_index set_BCs::build_free_index (src/set_bc.cpp:20-43)
+ *   ofea_build_single_load set_BCs::build_single_load (src/set_bc.cpp:46-77)
+ *   ofea_set_mesh         data_management fields coords_/node_topos_/
+ *                         ele_list_idx_/resort_free_nodes_
+ *                         (src/include/data_management.h:45-104)
+ *   ofea_pattern          assamble_stiffness::build_CSR (src/assemble.cpp:19-77)
+ *   ofea_assemble         assamble_stiffness::fill_CSR_sparse_mat
+ *                         (src/assemble.cpp:95-151)
+ *   ofea_solve            solution_api(..., solver_type) (src/linear_solution.cpp:17-64)
+ *                         with GPU PCG methods beside "Pardiso_*"/"SuperLU_*"
+ *   ofea_nl_assemble      assamble_nl_stiffness::fill_CSR_sparse_mat
+ *                         (src/nl_assemble.cpp:94-166)
+ *   ofea_lumped_mass      ele_base::build_ele_mass over all elements (src/cae.cpp:439-448)
+ *   ofea_explicit_run     CAE_process::explicit_analysis time loop (src/cae.cpp:420-574)
+ *   ofea_reset_displacement simulation_post::reset_displacement (src/post_process.cpp:18-40)
+ */
+#ifndef OPENFEA_AMD_H
+#define OPENFEA_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+#pragma GCC visibility push(default)
+
+#define OFEA_OK 0
+#define OFEA_NOT_CONVERGED 1   /* solver hit maxit; x holds the last iterate */
+#define OFEA_ERR_ARG (-1)      /* invalid argument or size                    */
+#define OFEA_ERR_HIP (-2)      /* HIP runtime failure (no device, OOM, ...)   */
+#define OFEA_ERR_STATE (-3)    /* call out of order (e.g. assemble before mesh) */
+#define OFEA_ERR_IO (-4)       /* file could not be read / parsed             */
+#define OFEA_ERR_LIMIT (-5)    /* mesh exceeds a documented kernel limit      */
+
+/* element type ids, as ELE_TYPES in src/include/data_management.h:74 */
+#define OFEA_C3D4 0
+#define OFEA_C3D8 1
+
+/* solver methods for ofea_solve */
+#define OFEA_PCG_JACOBI 0        /* FP64 values, point-Jacobi PCG                       */
+#define OFEA_PCG_BLOCK_JACOBI 1  /* FP64 values, 3x3 node-block Jacobi PCG              */
+#define OFEA_PCG_MIXED 2         /* FP32 inner block-Jacobi PCG + FP64 residual refinement */
+#define OFEA_PCG_LEARNED 3       /* as MIXED with the learned block-Jacobi correction   */
+
+/* explicit precision for ofea_explicit_run */
+#define OFEA_FP32 0
+#define OFEA_FP64 1
+
+typedef struct ofea_ctx ofea_ctx;
+typedef struct ofea_deck ofea_deck;
+
+const char *ofea_last_error(void);
+/* library version string, e.g. "openfea_amd 0.1 gfx950" */
+const char *ofea_version(void);
+
+/* ---- host drop-in: reference-dialect .inp reader (no GPU needed) ---- */
+/* Reads the deck the way src/data2cae.cpp does (line order defines node and
+ * element numbering, counts from the first *Nset and *Elset follow-on lines, set
+ * keywords "Set-load"/"Set-fix", first "*Cload" line, "*Elastic",
+ * "*Density", "*Dynamic, Explicit" time line). */
+int ofea_inp_open(const char *path, ofea_deck **deck);
+int ofea_inp_sizes(const ofea_deck *deck, int *nd, int *ne, int *nfix, int *nload);
+/* xyz nd*3, conn ne*8 (1-based, unused slots 0), etype ne, fix_set nfix,
+ * load_set nload; scalars may be NULL.  A missing *Elastic leaves E/nu as
+ * NaN (the reference keeps the constructor's material). */
+int ofea_inp_get(const ofea_deck *deck, double *xyz, int *conn, int *etype, int *fix_set, int *load_set,
+                 int *load_dof, double *load_value, double *E, double *nu, double *rho, double *dt,
+                 double *t_total);
+void ofea_inp_close(ofea_deck *deck);
+
+/* ---- host drop-in: boundary conditions (no GPU needed) ---- */
+/* set_BCs::build_free_index: sorts fix_set in place, fills resort (nd) with
+ * the free-node rank or -1; returns the number of free nodes (>= 0) or < 0. */
+int ofea_build_free_index(int nd, int *fix_set, int nfix, int *resort);
+/* set_BCs::build_single_load: F (n = 3*free nodes) zeroed, then F[3r+dof-1]
+ * = value for each load-set node (assignment, not sum). */
+int ofea_build_single_load(int n, const int *resort, const int *load_set, int nload, int dof, double value,
+                           double *F);
+/* build_CSR on the host, without a device context: pass col_ptr/row_idx NULL
+ * to get n and nnz, then arrays of n+1 and nnz entries for the CSC pattern
+ * (src/assemble.cpp:19-77: rows ascending per column, clamped DOFs dropped). */
+int ofea_build_csr(int nd, int ne, const int *conn, const int *etype, const int *resort, int *n, long long *nnz,
+                   int *col_ptr, int *row_idx);
+/* reset_displacement: u_full (3*nd) from the free-DOF u (3*free). */
+int ofea_reset_displacement(int nd, const int *resort, const double *u_free, double *u_full);
+
+/* ---- context ---- */
+int ofea_create(ofea_ctx **ctx, int device);
+int ofea_destroy(ofea_ctx *ctx);
+/* Copies the mesh, builds the CSC pattern and the gather plans on the host,
+ * uploads everything to HBM.  conn: ne*8 1-based ids (C3D4 uses slots 0..3),
+ * etype: 0 C3D4 / 1 C3D8, resort: nd free-node rank or -1. */
+int ofea_set_mesh(ofea_ctx *ctx, int nd, const double *xyz, int ne, const int *conn, const int *etype,
+                  const int *resort);
+/* build_CSR: n free DOFs and nnz; when col_ptr/row_idx are non-NULL they
+ * receive the CSC arrays (n+1, nnz), identical to the reference's. */
+int ofea_pattern(ofea_ctx *ctx, int *n, long long *nnz, int *col_ptr, int *row_idx);
+
+/* Per-element Young's modulus (ne), kept resident for later assemblies;
+ * NULL clears it (then the global E of ofea_assemble is used). */
+int ofea_set_elem_E(ofea_ctx *ctx, const double *E_elem);
+/* fill_CSR_sparse_mat on the GPU (FP64).  The values stay resident in HBM for
+ * ofea_solve; nz_val (nnz) receives a copy when non-NULL (then the call
+ * synchronises), otherwise the call is asynchronous on the context stream. */
+int ofea_assemble(ofea_ctx *ctx, double E, double nu, double *nz_val);
+
+/* y = K x with the resident K (FP64), host vectors of length n. */
+int ofea_spmv(ofea_ctx *ctx, const double *x, double *y);
+
+/* RHS upload (n) kept resident; ofea_solve with b == NULL uses it. */
+int ofea_set_rhs(ofea_ctx *ctx, const double *b);
+/* Solve K x = b with the resident K.  Stops when the true FP64 residual
+ * ||b - Kx||_2 <= rtol ||b||_2 (checked in FP64 for every method) or after
+ * maxit inner iterations.  b/x may be NULL (resident RHS / keep x on device).
+ * iters/relres may be NULL.  Returns OFEA_OK or OFEA_NOT_CONVERGED. */
+int ofea_solve(ofea_ctx *ctx, int method, const double *b, double *x, double rtol, int maxit, int *iters,
+               double *relres);
+int ofea_get_solution(ofea_ctx *ctx, double *x);
+
+/* ---- explicit dynamics ---- */
+/* Lumped nodal mass (nd) of all elements at density rho. */
+int ofea_lumped_mass(ofea_ctx *ctx, double rho, double *mass);
+/* Half-step central-difference loop with fixed dt for nsteps steps.
+ * mass nd, f_ext 3*nd, v0 3*nd (nullable), fixed_mask nd (nonzero = clamped),
+ * precision OFEA_FP32 / OFEA_FP64, u_out 3*nd (nullable: keep on device). */
+int ofea_explicit_run(ofea_ctx *ctx, double E, double nu, const double *mass, const double *f_ext,
+                      const double *v0, const int *fixed_mask, double dt, int nsteps, int precision,
+                      double *u_out);
+
+/* ---- geometric nonlinear (total Lagrangian St.Venant-Kirchhoff) ---- */
+/* Tangent K_T (resident, copied to nz_val when non-NULL) and internal force
+ * f_int (n, free DOFs) at the free-DOF displacement u (n). */
+int ofea_nl_assemble(ofea_ctx *ctx, double E, double nu, const double *u, double *nz_val, double *f_int);
+
+/* ---- timing on the context stream (HIP events) ---- */
+int ofea_timer_start(ofea_ctx *ctx);
+int ofea_timer_stop(ofea_ctx *ctx, float *ms);
+int ofea_sync(ofea_ctx *ctx);
+
+#pragma GCC visibility pop
+#ifdef __cplusplus
+}
+#endif
+#endif

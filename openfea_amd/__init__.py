@@ -1,0 +1,12 @@
+"""openfea_amd -- MI355X-native (gfx950) implementation of OpenFEA's element hot path.
+
+The numerical work lives in the C-ABI library libofea_hip.so (hand-written HIP
+kernels, include/openfea_amd.h); this package is its host-side mirror of the
+reference's call surface (fea.py) plus synthetic workload generators (meshgen.py).
+"""
+from . import _capi, meshgen  # noqa: F401
+from .fea import (AssambleStiffness, Context, Deck, build_csr, build_free_index, build_single_load,  # noqa: F401
+                  read_inp, reset_displacement, solution_api)
+
+__all__ = ["AssambleStiffness", "Context", "Deck", "build_csr", "build_free_index", "build_single_load", "read_inp",
+           "reset_displacement", "solution_api", "meshgen"]

@@ -1,0 +1,302 @@
+// capi.cpp -- C ABI entry points of libofea_hip.so that own device state.
+#include <algorithm>
+#include <cstring>
+
+#include "ofea_internal.h"
+
+using namespace ofea;
+
+namespace ofea {
+int plan_caps(int type);
+}
+
+static void free_plan(DevPlan &d)
+{
+    dev_free(d.conn);
+    dev_free(d.E);
+    dev_free(d.chunk_pair);
+    dev_free(d.chunk_blk);
+    dev_free(d.pair);
+    dev_free(d.blk_rk);
+    dev_free(d.contrib_ptr);
+    dev_free(d.contrib);
+    dev_free(d.node_pair);
+    dev_free(d.chunk_node);
+    d = DevPlan();
+}
+
+static void free_mesh(ofea_ctx *c)
+{
+    solver_free(c);
+    explicit_free(c);
+    dev_free(c->d_xyz);
+    dev_free(c->d_col_ptr);
+    dev_free(c->d_node_blk);
+    dev_free(c->d_bcol);
+    dev_free(c->d_resort);
+    dev_free(c->d_val);
+    dev_free(c->d_rhs);
+    dev_free(c->d_x);
+    dev_free(c->d_u);
+    dev_free(c->d_fint);
+    for (auto &d : c->dp) free_plan(d);
+    c->assembled = false;
+    c->rhs_set = false;
+}
+
+#define CHECK_CTX(c)                                                  \
+    do {                                                              \
+        if (!(c)) return fail(OFEA_ERR_ARG, "null context");         \
+        hipError_t _se = hipSetDevice((c)->device);                   \
+        if (_se != hipSuccess)                                        \
+            return fail(OFEA_ERR_HIP, hipGetErrorString(_se));        \
+    } while (0)
+
+extern "C" {
+
+int ofea_create(ofea_ctx **ctx, int device)
+{
+    if (!ctx) return fail(OFEA_ERR_ARG, "null ctx pointer");
+    *ctx = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(OFEA_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(OFEA_ERR_ARG, "device index out of range");
+    OFEA_HIP(hipSetDevice(device));
+    auto *c = new ofea_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return fail(OFEA_ERR_HIP, "stream/event creation failed");
+    }
+    *ctx = c;
+    return OFEA_OK;
+}
+
+int ofea_destroy(ofea_ctx *c)
+{
+    if (!c) return OFEA_OK;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    free_mesh(c);
+    hipEventDestroy(c->ev0);
+    hipEventDestroy(c->ev1);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return OFEA_OK;
+}
+
+int ofea_set_mesh(ofea_ctx *c, int nd, const double *xyz, int ne, const int *conn, const int *etype, const int *resort)
+{
+    CHECK_CTX(c);
+    if (nd <= 0 || ne <= 0 || !xyz || !conn || !etype || !resort) return fail(OFEA_ERR_ARG, "bad mesh arguments");
+    for (int e = 0; e < ne; ++e)
+        if (etype[e] != OFEA_C3D4 && etype[e] != OFEA_C3D8) return fail(OFEA_ERR_ARG, "unknown element type id");
+    for (int i = 0; i < nd; ++i)
+        if (resort[i] >= nd) return fail(OFEA_ERR_ARG, "resort entry out of range");
+    hipStreamSynchronize(c->stream);
+    free_mesh(c);
+    c->nd = nd;
+    c->ne = ne;
+    c->xyz.assign(xyz, xyz + 3 * (size_t)nd);
+    c->conn.assign(conn, conn + 8 * (size_t)ne);
+    c->etype.assign(etype, etype + ne);
+    c->resort.assign(resort, resort + nd);
+    c->E_elem.clear();
+    int rc = build_pattern(nd, ne, conn, etype, resort, c->P);
+    if (rc) return rc;
+    for (int t = 0; t < 2; ++t) {
+        c->has_type[t] = std::find(etype, etype + ne, t) != etype + ne;
+        if (!c->has_type[t]) continue;
+        rc = build_gather_plan(nd, ne, conn, etype, resort, c->P, t, plan_caps(t), c->G[t]);
+        if (rc) return rc;
+    }
+    // upload
+    if ((rc = dev_upload(c->d_xyz, c->xyz.data(), c->xyz.size()))) return rc;
+    if ((rc = dev_upload(c->d_col_ptr, c->P.col_ptr.data(), c->P.col_ptr.size()))) return rc;
+    if ((rc = dev_upload(c->d_node_blk, c->P.node_blk.data(), c->P.node_blk.size()))) return rc;
+    if ((rc = dev_upload(c->d_bcol, c->P.bcol.data(), c->P.bcol.size()))) return rc;
+    if ((rc = dev_upload(c->d_resort, c->resort.data(), c->resort.size()))) return rc;
+    if ((rc = dev_upload<double>(c->d_val, nullptr, (size_t)std::max(c->P.nnz, 1LL)))) return rc;
+    if ((rc = dev_upload<double>(c->d_rhs, nullptr, (size_t)std::max(c->P.n, 1)))) return rc;
+    if ((rc = dev_upload<double>(c->d_x, nullptr, (size_t)std::max(c->P.n, 1)))) return rc;
+    for (int t = 0; t < 2; ++t) {
+        if (!c->has_type[t]) continue;
+        const GatherPlan &G = c->G[t];
+        DevPlan &d = c->dp[t];
+        d.nn = G.nn;
+        d.ne = G.ne;
+        d.nchunks = G.nchunks();
+        d.nblk = (int)G.blk_rk.size();
+        if ((rc = dev_upload(d.conn, G.conn.data(), G.conn.size()))) return rc;
+        if ((rc = dev_upload(d.chunk_pair, G.chunk_pair.data(), G.chunk_pair.size()))) return rc;
+        if ((rc = dev_upload(d.chunk_blk, G.chunk_blk.data(), G.chunk_blk.size()))) return rc;
+        if ((rc = dev_upload(d.pair, G.pair.data(), std::max<size_t>(G.pair.size(), 1)))) return rc;
+        if ((rc = dev_upload(d.blk_rk, G.blk_rk.data(), std::max<size_t>(G.blk_rk.size(), 1)))) return rc;
+        if ((rc = dev_upload(d.contrib_ptr, G.contrib_ptr.data(), G.contrib_ptr.size()))) return rc;
+        if ((rc = dev_upload(d.contrib, G.contrib.data(), std::max<size_t>(G.contrib.size(), 1)))) return rc;
+        if ((rc = dev_upload(d.node_pair, G.node_pair.data(), G.node_pair.size()))) return rc;
+        if ((rc = dev_upload(d.chunk_node, G.chunk_node.data(), G.chunk_node.size()))) return rc;
+    }
+    return OFEA_OK;
+}
+
+int ofea_pattern(ofea_ctx *c, int *n, long long *nnz, int *col_ptr, int *row_idx)
+{
+    if (!c) return fail(OFEA_ERR_ARG, "null context");
+    if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    if (n) *n = c->P.n;
+    if (nnz) *nnz = c->P.nnz;
+    if (col_ptr) std::memcpy(col_ptr, c->P.col_ptr.data(), c->P.col_ptr.size() * sizeof(int));
+    if (row_idx) std::memcpy(row_idx, c->P.row_idx.data(), c->P.row_idx.size() * sizeof(int));
+    return OFEA_OK;
+}
+
+int ofea_set_elem_E(ofea_ctx *c, const double *E_elem)
+{
+    CHECK_CTX(c);
+    if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    hipStreamSynchronize(c->stream);
+    for (int t = 0; t < 2; ++t) dev_free(c->dp[t].E);
+    c->E_elem.clear();
+    if (!E_elem) return OFEA_OK;
+    c->E_elem.assign(E_elem, E_elem + c->ne);
+    for (int t = 0; t < 2; ++t) {
+        if (!c->has_type[t]) continue;
+        std::vector<double> loc(c->G[t].ne);
+        for (int i = 0; i < c->G[t].ne; ++i) loc[i] = E_elem[c->G[t].elem_global[i]];
+        int rc = dev_upload(c->dp[t].E, loc.data(), loc.size());
+        if (rc) return rc;
+    }
+    return OFEA_OK;
+}
+
+int ofea_assemble(ofea_ctx *c, double E, double nu, double *nz_val)
+{
+    CHECK_CTX(c);
+    if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    if (!(nu > -1.0 && nu < 0.5)) return fail(OFEA_ERR_ARG, "Poisson ratio must lie in (-1, 0.5)");
+    int rc = launch_assemble(c, E, nu, c->d_val, false);
+    if (rc) return rc;
+    c->assembled = true;
+    c->asm_E = E;
+    c->asm_nu = nu;
+    solver_invalidate(c);
+    if (nz_val) {
+        OFEA_HIP(hipMemcpyAsync(nz_val, c->d_val, sizeof(double) * c->P.nnz, hipMemcpyDeviceToHost, c->stream));
+        OFEA_HIP(hipStreamSynchronize(c->stream));
+    }
+    return OFEA_OK;
+}
+
+int ofea_spmv(ofea_ctx *c, const double *x, double *y)
+{
+    CHECK_CTX(c);
+    if (!c->assembled) return fail(OFEA_ERR_STATE, "ofea_assemble first");
+    if (!x || !y) return fail(OFEA_ERR_ARG, "null vector");
+    size_t n = c->P.n;
+    double *dx = nullptr, *dy = nullptr;
+    OFEA_HIP(hipMalloc((void **)&dx, sizeof(double) * std::max<size_t>(n, 1)));
+    OFEA_HIP(hipMalloc((void **)&dy, sizeof(double) * std::max<size_t>(n, 1)));
+    OFEA_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    int rc = solver_spmv(c, dx, dy);
+    if (rc == OFEA_OK) {
+        hipMemcpyAsync(y, dy, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+        hipStreamSynchronize(c->stream);
+    }
+    hipFree(dx);
+    hipFree(dy);
+    return rc;
+}
+
+int ofea_set_rhs(ofea_ctx *c, const double *b)
+{
+    CHECK_CTX(c);
+    if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    if (!b) return fail(OFEA_ERR_ARG, "null rhs");
+    OFEA_HIP(hipMemcpyAsync(c->d_rhs, b, sizeof(double) * c->P.n, hipMemcpyHostToDevice, c->stream));
+    OFEA_HIP(hipStreamSynchronize(c->stream));
+    c->rhs_set = true;
+    return OFEA_OK;
+}
+
+int ofea_solve(ofea_ctx *c, int method, const double *b, double *x, double rtol, int maxit, int *iters, double *relres)
+{
+    CHECK_CTX(c);
+    if (!c->assembled) return fail(OFEA_ERR_STATE, "ofea_assemble first");
+    if (!(rtol > 0.0) || maxit <= 0) return fail(OFEA_ERR_ARG, "rtol must be > 0 and maxit > 0");
+    if (b) {
+        int rc = ofea_set_rhs(c, b);
+        if (rc) return rc;
+    }
+    if (!c->rhs_set) return fail(OFEA_ERR_STATE, "no right-hand side (ofea_set_rhs)");
+    int rc = solver_solve(c, method, rtol, maxit, iters, relres);
+    if (rc < 0) return rc;
+    if (x) {
+        OFEA_HIP(hipMemcpyAsync(x, c->d_x, sizeof(double) * c->P.n, hipMemcpyDeviceToHost, c->stream));
+        OFEA_HIP(hipStreamSynchronize(c->stream));
+    }
+    return rc;
+}
+
+int ofea_get_solution(ofea_ctx *c, double *x)
+{
+    CHECK_CTX(c);
+    if (!x) return fail(OFEA_ERR_ARG, "null x");
+    OFEA_HIP(hipMemcpyAsync(x, c->d_x, sizeof(double) * c->P.n, hipMemcpyDeviceToHost, c->stream));
+    OFEA_HIP(hipStreamSynchronize(c->stream));
+    return OFEA_OK;
+}
+
+int ofea_lumped_mass(ofea_ctx *c, double rho, double *mass)
+{
+    CHECK_CTX(c);
+    if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    if (!mass) return fail(OFEA_ERR_ARG, "null mass");
+    return lumped_mass(c, rho, mass);
+}
+
+int ofea_explicit_run(ofea_ctx *c, double E, double nu, const double *mass, const double *f_ext, const double *v0,
+                      const int *fixed_mask, double dt, int nsteps, int precision, double *u_out)
+{
+    CHECK_CTX(c);
+    if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    if (!mass || !f_ext || !fixed_mask || nsteps < 0 || !(dt > 0.0)) return fail(OFEA_ERR_ARG, "bad explicit arguments");
+    if (precision != OFEA_FP32 && precision != OFEA_FP64) return fail(OFEA_ERR_ARG, "precision must be FP32 or FP64");
+    return explicit_run(c, E, nu, mass, f_ext, v0, fixed_mask, dt, nsteps, precision, u_out);
+}
+
+int ofea_nl_assemble(ofea_ctx *c, double E, double nu, const double *u, double *nz_val, double *f_int)
+{
+    CHECK_CTX(c);
+    if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    if (!u) return fail(OFEA_ERR_ARG, "null displacement");
+    return nl_assemble(c, E, nu, u, nz_val, f_int);
+}
+
+int ofea_timer_start(ofea_ctx *c)
+{
+    CHECK_CTX(c);
+    OFEA_HIP(hipEventRecord(c->ev0, c->stream));
+    return OFEA_OK;
+}
+
+int ofea_timer_stop(ofea_ctx *c, float *ms)
+{
+    CHECK_CTX(c);
+    OFEA_HIP(hipEventRecord(c->ev1, c->stream));
+    OFEA_HIP(hipEventSynchronize(c->ev1));
+    float t = 0.f;
+    OFEA_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
+    if (ms) *ms = t;
+    return OFEA_OK;
+}
+
+int ofea_sync(ofea_ctx *c)
+{
+    CHECK_CTX(c);
+    OFEA_HIP(hipStreamSynchronize(c->stream));
+    return OFEA_OK;
+}
+
+}  // extern "C"

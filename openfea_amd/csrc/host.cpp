@@ -1,0 +1,529 @@
+// host.cpp -- host-side drop-in pieces of libofea_hip.so: the reference-dialect
+// .inp reader, the boundary-condition numbering, the CSC pattern builder and
+// the column-owner gather plans that the assembly kernels consume.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <map>
+#include <sstream>
+
+#include "ofea_internal.h"
+
+namespace ofea {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+// ---------------------------------------------------------------- .inp reader
+// Line reader accepting \n, \r\n and \r endings (ReadInfo::safeGetline,
+// src/data2cae.cpp:821-851).  Returns false at end of file with no characters.
+static bool get_line(std::istream &is, std::string &t)
+{
+    t.clear();
+    std::streambuf *sb = is.rdbuf();
+    for (;;) {
+        int c = sb->sbumpc();
+        if (c == '\n') return true;
+        if (c == '\r') {
+            if (sb->sgetc() == '\n') sb->sbumpc();
+            return true;
+        }
+        if (c == std::streambuf::traits_type::eof()) {
+            is.setstate(std::ios::eofbit);
+            return !t.empty();
+        }
+        t += (char)c;
+    }
+}
+
+static std::string strip_blanks(std::string s)
+{
+    s.erase(std::remove(s.begin(), s.end(), ' '), s.end());
+    return s;
+}
+
+static void drop_last(std::string &s)
+{
+    if (!s.empty()) s.erase(s.end() - 1);
+}
+
+// ReadInfo::read_ele_node_num (src/data2cae.cpp:18-53): the second token of the
+// line after the first "*Nset" / "*Elset" keyword lines.
+static int read_counts(const std::string &path, int &nd, int &ne)
+{
+    std::ifstream in(path);
+    if (!in) return fail(OFEA_ERR_IO, "cannot open " + path);
+    std::string line;
+    nd = ne = -1;
+    while (get_line(in, line)) {
+        if (line.find("*Nset") != std::string::npos) {
+            get_line(in, line);
+            std::istringstream iss(line);
+            std::string a, b, c;
+            iss >> a >> b >> c;
+            nd = std::atoi(b.c_str());
+        }
+        if (line.find("*Elset") != std::string::npos) {
+            get_line(in, line);
+            std::istringstream iss(line);
+            std::string a, b, c;
+            iss >> a >> b >> c;
+            ne = std::atoi(b.c_str());
+            break;
+        }
+    }
+    if (nd <= 0 || ne <= 0) return fail(OFEA_ERR_IO, "missing *Nset/*Elset count lines in " + path);
+    return OFEA_OK;
+}
+
+// ReadInfo::read_geo_mesh (src/data2cae.cpp:77-179): node and element order is
+// line order; "id, x, y, z" with the trailing commas of x and y dropped;
+// element lines "id, n1, ..., nk"; the element type comes from "type=" and an
+// unknown name maps to C3D4 (the std::map operator[] default of ELE_TYPES).
+static int read_mesh(const std::string &path, ofea_deck &D)
+{
+    std::ifstream in(path);
+    if (!in) return fail(OFEA_ERR_IO, "cannot open " + path);
+    D.xyz.assign((size_t)D.nd * 3, 0.0);
+    D.conn.assign((size_t)D.ne * 8, 0);
+    D.etype.assign((size_t)D.ne, 0);
+    std::map<std::string, int> types = {{"C3D4", OFEA_C3D4}, {"C3D8", OFEA_C3D8}};
+    std::string line;
+    bool node_rec = false, elem_rec = false;
+    int id_node = 0, id_ele = 0, cur_type = 0;
+    while (get_line(in, line)) {
+        if (line.find("*Node") != std::string::npos) {
+            node_rec = true;
+            get_line(in, line);
+        }
+        if (node_rec && line.find('*') != std::string::npos) node_rec = false;
+        if (node_rec) {
+            std::istringstream iss(line);
+            std::string id, x, y, z;
+            iss >> id >> x >> y >> z;
+            drop_last(x);
+            drop_last(y);
+            if (id_node >= D.nd) return fail(OFEA_ERR_IO, "more node lines than the *Nset count");
+            try {
+                D.xyz[3 * id_node + 0] = std::stod(x);
+                D.xyz[3 * id_node + 1] = std::stod(y);
+                D.xyz[3 * id_node + 2] = std::stod(z);
+            } catch (...) {
+                return fail(OFEA_ERR_IO, "bad node line: " + line);
+            }
+            id_node++;
+        }
+        if (line.find("*Element") != std::string::npos) {
+            size_t eq = line.find('=');
+            std::string name = eq == std::string::npos ? std::string() : strip_blanks(line.substr(eq + 1));
+            cur_type = types[name];
+            elem_rec = true;
+            get_line(in, line);
+        }
+        if (elem_rec && line.find('*') != std::string::npos) elem_rec = false;
+        if (elem_rec) {
+            if (id_ele >= D.ne) return fail(OFEA_ERR_IO, "more element lines than the *Elset count");
+            D.etype[id_ele] = cur_type;
+            std::istringstream iss(line);
+            std::string tok;
+            int slot = -1;
+            while (std::getline(iss, tok, ',')) {
+                tok = strip_blanks(tok);
+                if (slot >= 0 && slot < 8) D.conn[(size_t)8 * id_ele + slot] = std::atoi(tok.c_str());
+                slot++;
+            }
+            id_ele++;
+        }
+        if (line.find("*End Assembly") != std::string::npos) break;
+    }
+    if (id_node != D.nd || id_ele != D.ne) return fail(OFEA_ERR_IO, "node/element line count differs from *Nset/*Elset");
+    return OFEA_OK;
+}
+
+// Node-set scan of ReadInfo::read_load_bcs / read_dis_bcs
+// (src/data2cae.cpp:315-414): ids on the lines after the set keyword line, up
+// to the next line holding '*'; the scan stops at "*End Assembly".
+static void read_set(std::istream &in, const std::string &key, std::vector<int> &out)
+{
+    std::string line;
+    while (get_line(in, line)) {
+        if (line.find(key) != std::string::npos) {
+            while (get_line(in, line)) {
+                if (line.find('*') != std::string::npos) break;
+                std::istringstream iss(line);
+                std::string tok;
+                while (iss >> tok) {
+                    if (tok.find(',') != std::string::npos) drop_last(tok);
+                    out.push_back(std::atoi(tok.c_str()));
+                }
+            }
+        }
+        if (line.find("*End Assembly") != std::string::npos) break;
+    }
+}
+
+static int read_bcs_mat_time(const std::string &path, ofea_deck &D)
+{
+    {
+        std::ifstream in(path);
+        if (!in) return fail(OFEA_ERR_IO, "cannot open " + path);
+        read_set(in, "Set-load", D.load_set);
+        // first "Cload" keyword after the sets: "set, dof, value" (src/data2cae.cpp:352-370)
+        std::string line;
+        while (get_line(in, line)) {
+            if (line.find("Cload") != std::string::npos) {
+                get_line(in, line);
+                std::istringstream iss(line);
+                std::string name, dof, val;
+                iss >> name >> dof >> val;
+                drop_last(dof);
+                D.load_dof = std::atoi(dof.c_str());
+                try {
+                    D.load_value = std::stod(val);
+                } catch (...) {
+                    return fail(OFEA_ERR_IO, "bad *Cload line: " + line);
+                }
+                break;
+            }
+        }
+    }
+    {
+        std::ifstream in(path);
+        read_set(in, "Set-fix", D.fix_set);
+    }
+    {
+        // ReadInfo::read_mat (src/data2cae.cpp:416-446)
+        std::ifstream in(path);
+        std::string line;
+        while (get_line(in, line)) {
+            if (line.find("*Density") != std::string::npos) {
+                get_line(in, line);
+                std::istringstream iss(line);
+                std::string rho;
+                iss >> rho;
+                try { D.rho = std::stod(rho); } catch (...) { return fail(OFEA_ERR_IO, "bad *Density line"); }
+            }
+            if (line.find("*Elastic") != std::string::npos) {
+                get_line(in, line);
+                std::istringstream iss(line);
+                std::string e, nu;
+                iss >> e >> nu;
+                drop_last(e);
+                try {
+                    D.E = std::stod(e);
+                    D.nu = std::stod(nu);
+                } catch (...) {
+                    return fail(OFEA_ERR_IO, "bad *Elastic line");
+                }
+            }
+        }
+    }
+    {
+        // ReadInfo::read_time (src/data2cae.cpp:448-492): "dt, T" after the
+        // "Explicit" keyword line; an omitted dt leaves 0 (automatic step).
+        std::ifstream in(path);
+        std::string line;
+        while (get_line(in, line)) {
+            if (line.find("Explicit") == std::string::npos) continue;
+            while (get_line(in, line)) {
+                if (line.size() >= 2 && line[0] == '*' && line[1] == '*') continue;
+                if (!line.empty() && line[0] == '*') break;
+                std::istringstream iss(line);
+                double dt;
+                if (iss >> dt)
+                    D.dt = dt;
+                else {
+                    iss.clear();
+                    iss.seekg(0);
+                }
+                char comma;
+                iss >> comma;
+                double T;
+                if (iss >> T) D.t_total = T;
+            }
+        }
+    }
+    return OFEA_OK;
+}
+
+// ---------------------------------------------------------------- pattern
+int build_pattern(int nd, int ne, const int *conn, const int *etype, const int *resort, Pattern &P)
+{
+    int nfree = 0;
+    for (int i = 0; i < nd; ++i)
+        if (resort[i] >= 0) nfree = std::max(nfree, resort[i] + 1);
+    std::vector<int> rank_node(nfree, -1);
+    for (int i = 0; i < nd; ++i)
+        if (resort[i] >= 0) rank_node[resort[i]] = i;
+    // node -> element incidence
+    std::vector<int> inc_ptr(nd + 1, 0);
+    for (int e = 0; e < ne; ++e) {
+        int nn = etype[e] == OFEA_C3D8 ? 8 : 4;
+        for (int a = 0; a < nn; ++a) {
+            int node = conn[8 * e + a] - 1;
+            if (node < 0 || node >= nd) return fail(OFEA_ERR_ARG, "element node id out of range");
+            inc_ptr[node + 1]++;
+        }
+    }
+    for (int i = 0; i < nd; ++i) inc_ptr[i + 1] += inc_ptr[i];
+    std::vector<int> inc(inc_ptr[nd]);
+    {
+        std::vector<int> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+        for (int e = 0; e < ne; ++e) {
+            int nn = etype[e] == OFEA_C3D8 ? 8 : 4;
+            for (int a = 0; a < nn; ++a) inc[fill[conn[8 * e + a] - 1]++] = e;
+        }
+    }
+    P.nfree = nfree;
+    P.n = 3 * nfree;
+    P.node_blk.assign(nfree + 1, 0);
+    std::vector<int> stamp(nfree, -1), nb;
+    std::vector<std::vector<int>> adj(nfree);
+    for (int r = 0; r < nfree; ++r) {
+        int node = rank_node[r];
+        nb.clear();
+        if (node >= 0)
+            for (int t = inc_ptr[node]; t < inc_ptr[node + 1]; ++t) {
+                int e = inc[t];
+                int nn = etype[e] == OFEA_C3D8 ? 8 : 4;
+                for (int b = 0; b < nn; ++b) {
+                    int s = resort[conn[8 * e + b] - 1];
+                    if (s >= 0 && stamp[s] != r) {
+                        stamp[s] = r;
+                        nb.push_back(s);
+                    }
+                }
+            }
+        std::sort(nb.begin(), nb.end());
+        adj[r] = nb;
+        P.node_blk[r + 1] = P.node_blk[r] + (int)nb.size();
+    }
+    long long nblk = P.node_blk[nfree];
+    if (9 * nblk > (long long)std::numeric_limits<int>::max())
+        return fail(OFEA_ERR_LIMIT, "nnz exceeds the int32 index range of the CSC arrays");
+    P.nnz = 9 * nblk;
+    P.col_ptr.assign(P.n + 1, 0);
+    P.row_idx.resize(P.nnz);
+    P.bcol.resize(nblk);
+    for (int r = 0; r < nfree; ++r) {
+        int deg = (int)adj[r].size();
+        int base = 9 * P.node_blk[r];
+        for (int k = 0; k < deg; ++k) P.bcol[P.node_blk[r] + k] = adj[r][k];
+        for (int c = 0; c < 3; ++c) {
+            int cb = base + c * 3 * deg;
+            P.col_ptr[3 * r + c] = cb;
+            for (int k = 0; k < deg; ++k)
+                for (int d = 0; d < 3; ++d) P.row_idx[cb + 3 * k + d] = 3 * adj[r][k] + d;
+        }
+    }
+    P.col_ptr[P.n] = (int)P.nnz;
+    return OFEA_OK;
+}
+
+int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const int *resort, const Pattern &P,
+                      int type, int pair_cap, GatherPlan &G)
+{
+    G = GatherPlan();
+    G.nn = type == OFEA_C3D8 ? 8 : 4;
+    const int nn = G.nn;
+    std::vector<int> local(ne, -1);
+    for (int e = 0; e < ne; ++e)
+        if (etype[e] == type) {
+            local[e] = G.ne++;
+            G.elem_global.push_back(e);
+            for (int a = 0; a < nn; ++a) G.conn.push_back(conn[8 * e + a] - 1);
+        }
+    const int nfree = P.nfree;
+    // per free node: incident elements of this type (ascending) with local index
+    std::vector<int> cnt(nfree + 1, 0);
+    for (int e = 0; e < ne; ++e) {
+        if (etype[e] != type) continue;
+        for (int a = 0; a < nn; ++a) {
+            int r = resort[conn[8 * e + a] - 1];
+            if (r >= 0) cnt[r + 1]++;
+        }
+    }
+    for (int r = 0; r < nfree; ++r) cnt[r + 1] += cnt[r];
+    std::vector<int> pairs(cnt[nfree]);
+    {
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (int e = 0; e < ne; ++e) {  // ascending e => ascending per node
+            if (etype[e] != type) continue;
+            for (int a = 0; a < nn; ++a) {
+                int r = resort[conn[8 * e + a] - 1];
+                if (r >= 0) pairs[fill[r]++] = (local[e] << 3) | a;
+            }
+        }
+    }
+    if ((long long)G.ne >= (1LL << 28)) return fail(OFEA_ERR_LIMIT, "too many elements for the packed pair index");
+    if (nfree >= (1 << 23)) return fail(OFEA_ERR_LIMIT, "too many free nodes for the packed block index");
+    G.pair = pairs;
+    G.node_pair = cnt;
+    G.chunk_node.push_back(0);
+    G.chunk_pair.push_back(0);
+    G.chunk_blk.push_back(0);
+    G.contrib_ptr.push_back(0);
+    int chunk_first_pair = 0;
+    int in_chunk = 0;
+    std::vector<std::vector<uint16_t>> lists;
+    for (int r = 0; r < nfree; ++r) {
+        int np = cnt[r + 1] - cnt[r];
+        if (np > pair_cap) return fail(OFEA_ERR_LIMIT, "a node has more incident elements than the gather chunk holds");
+        int deg = P.node_blk[r + 1] - P.node_blk[r];
+        if (deg > 255) return fail(OFEA_ERR_LIMIT, "a node has more than 255 neighbours");
+        if (in_chunk + np > pair_cap) {
+            G.chunk_pair.push_back(cnt[r]);
+            G.chunk_blk.push_back((int)G.blk_rk.size());
+            G.chunk_node.push_back(r);
+            chunk_first_pair = cnt[r];
+            in_chunk = 0;
+        }
+        in_chunk += np;
+        const int *nbr = P.bcol.data() + P.node_blk[r];
+        lists.assign(deg, std::vector<uint16_t>());
+        for (int t = cnt[r]; t < cnt[r + 1]; ++t) {
+            int el = pairs[t] >> 3;
+            for (int b = 0; b < nn; ++b) {
+                int s = resort[G.conn[(size_t)el * nn + b]];
+                if (s < 0) continue;
+                int k = (int)(std::lower_bound(nbr, nbr + deg, s) - nbr);
+                lists[k].push_back((uint16_t)((t - chunk_first_pair) * nn + b));
+            }
+        }
+        for (int k = 0; k < deg; ++k) {
+            G.blk_rk.push_back((r << 8) | k);
+            G.contrib.insert(G.contrib.end(), lists[k].begin(), lists[k].end());
+            G.contrib_ptr.push_back((int)G.contrib.size());
+        }
+    }
+    G.chunk_pair.push_back(cnt[nfree]);
+    G.chunk_blk.push_back((int)G.blk_rk.size());
+    G.chunk_node.push_back(nfree);
+    (void)nd;
+    return OFEA_OK;
+}
+
+}  // namespace ofea
+
+// ====================================================================== C ABI (host-only part)
+using namespace ofea;
+
+extern "C" {
+
+const char *ofea_last_error(void) { return g_err.c_str(); }
+const char *ofea_version(void) { return "openfea_amd 0.1 gfx950"; }
+
+int ofea_inp_open(const char *path, ofea_deck **deck)
+{
+    if (!path || !deck) return fail(OFEA_ERR_ARG, "null argument");
+    *deck = nullptr;
+    auto *D = new ofea_deck();
+    D->E = D->nu = D->rho = std::numeric_limits<double>::quiet_NaN();
+    int rc = read_counts(path, D->nd, D->ne);
+    if (rc == OFEA_OK) rc = read_mesh(path, *D);
+    if (rc == OFEA_OK) rc = read_bcs_mat_time(path, *D);
+    if (rc != OFEA_OK) {
+        delete D;
+        return rc;
+    }
+    *deck = D;
+    return OFEA_OK;
+}
+
+int ofea_inp_sizes(const ofea_deck *D, int *nd, int *ne, int *nfix, int *nload)
+{
+    if (!D) return fail(OFEA_ERR_ARG, "null deck");
+    if (nd) *nd = D->nd;
+    if (ne) *ne = D->ne;
+    if (nfix) *nfix = (int)D->fix_set.size();
+    if (nload) *nload = (int)D->load_set.size();
+    return OFEA_OK;
+}
+
+int ofea_inp_get(const ofea_deck *D, double *xyz, int *conn, int *etype, int *fix_set, int *load_set, int *load_dof,
+                 double *load_value, double *E, double *nu, double *rho, double *dt, double *t_total)
+{
+    if (!D) return fail(OFEA_ERR_ARG, "null deck");
+    if (xyz) std::memcpy(xyz, D->xyz.data(), D->xyz.size() * sizeof(double));
+    if (conn) std::memcpy(conn, D->conn.data(), D->conn.size() * sizeof(int));
+    if (etype) std::memcpy(etype, D->etype.data(), D->etype.size() * sizeof(int));
+    if (fix_set && !D->fix_set.empty()) std::memcpy(fix_set, D->fix_set.data(), D->fix_set.size() * sizeof(int));
+    if (load_set && !D->load_set.empty()) std::memcpy(load_set, D->load_set.data(), D->load_set.size() * sizeof(int));
+    if (load_dof) *load_dof = D->load_dof;
+    if (load_value) *load_value = D->load_value;
+    if (E) *E = D->E;
+    if (nu) *nu = D->nu;
+    if (rho) *rho = D->rho;
+    if (dt) *dt = D->dt;
+    if (t_total) *t_total = D->t_total;
+    return OFEA_OK;
+}
+
+void ofea_inp_close(ofea_deck *deck) { delete deck; }
+
+int ofea_build_free_index(int nd, int *fix_set, int nfix, int *resort)
+{
+    if (nd < 0 || nfix < 0 || !resort || (nfix > 0 && !fix_set)) return fail(OFEA_ERR_ARG, "bad argument");
+    std::sort(fix_set, fix_set + nfix);
+    int rank = 0, cur = 0;
+    for (int i = 0; i < nd; ++i) {
+        if (nfix > 0 && i == fix_set[cur] - 1) {
+            resort[i] = -1;
+            if (cur < nfix - 1) cur++;
+        } else {
+            resort[i] = rank++;
+        }
+    }
+    return rank;
+}
+
+int ofea_build_single_load(int n, const int *resort, const int *load_set, int nload, int dof, double value, double *F)
+{
+    if (n < 0 || !F || (nload > 0 && (!load_set || !resort))) return fail(OFEA_ERR_ARG, "bad argument");
+    std::fill(F, F + n, 0.0);
+    if (dof < 1 || dof > 3) return fail(OFEA_ERR_ARG, "the DOF of the load node must be 1, 2 or 3");
+    for (int i = 0; i < nload; ++i) {
+        int r = resort[load_set[i] - 1];
+        int idx = 3 * r + dof - 1;
+        if (r < 0 || idx >= n) continue;  // a clamped load node carries no free DOF
+        F[idx] = value;
+    }
+    return OFEA_OK;
+}
+
+int ofea_build_csr(int nd, int ne, const int *conn, const int *etype, const int *resort, int *n, long long *nnz,
+                   int *col_ptr, int *row_idx)
+{
+    if (nd <= 0 || ne <= 0 || !conn || !etype || !resort) return fail(OFEA_ERR_ARG, "bad mesh arguments");
+    for (int e = 0; e < ne; ++e)
+        if (etype[e] != OFEA_C3D4 && etype[e] != OFEA_C3D8) return fail(OFEA_ERR_ARG, "unknown element type id");
+    Pattern P;
+    int rc = build_pattern(nd, ne, conn, etype, resort, P);
+    if (rc) return rc;
+    if (n) *n = P.n;
+    if (nnz) *nnz = P.nnz;
+    if (col_ptr) std::memcpy(col_ptr, P.col_ptr.data(), P.col_ptr.size() * sizeof(int));
+    if (row_idx) std::memcpy(row_idx, P.row_idx.data(), P.row_idx.size() * sizeof(int));
+    return OFEA_OK;
+}
+
+int ofea_reset_displacement(int nd, const int *resort, const double *u_free, double *u_full)
+{
+    if (nd < 0 || !resort || !u_full || !u_free) return fail(OFEA_ERR_ARG, "bad argument");
+    for (int i = 0; i < nd; ++i) {
+        int r = resort[i];
+        for (int c = 0; c < 3; ++c) u_full[3 * i + c] = r >= 0 ? u_free[3 * r + c] : 0.0;
+    }
+    return OFEA_OK;
+}
+
+}  // extern "C"

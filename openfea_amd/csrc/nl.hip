@@ -1,0 +1,370 @@
+// nl.hip -- total-Lagrangian St.Venant-Kirchhoff tangent and internal force.
+//
+// Reference: build_GL_strain_gauss / build_ele_nl_stiff_mat_gauss
+// (src/Elements/C3D8_Gauss.cpp:110-266), build_ele_nl_stiff_mat_hammer
+// (src/Elements/C3D4_hammer.cpp:75-183), assembled by
+// assamble_nl_stiffness::fill_CSR_sparse_mat (src/nl_assemble.cpp:94-166).
+//
+// With F = I + grad u, E = (F^T F - I)/2, S = lambda tr(E) I + 2 mu E (the
+// reference's D applied to Voigt E with doubled shears), h_a = F g_a, the
+// reference's B_L^T D B_L + B_NL^T Sigma B_NL block and B_L^T S force are
+//   K_ab[c][d] = w (lambda h_a[c] h_b[d] + mu (F F^T)_cd g_a.g_b + mu h_a[d] h_b[c]
+//                   + delta_cd g_a^T S g_b)
+//   f_a = w F S g_a
+// The kernels reuse the column-owner gather plan of the linear assembly.
+#include <algorithm>
+
+#include "elem_device.h"
+#include "ofea_internal.h"
+
+namespace ofea {
+
+constexpr int NL_WG = 256;
+constexpr int NL_TET_CAP = 256;
+constexpr int NL_HEX_CAP = 32;
+
+struct NLPoint {
+    double F[3][3], S[3][3], FFt[3][3];
+};
+
+__device__ inline void nl_state(const double H[3][3], double lam, double mu, NLPoint &P)
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int J = 0; J < 3; ++J) P.F[c][J] = H[c][J] + (c == J ? 1.0 : 0.0);
+    double E[3][3];
+#pragma unroll
+    for (int I = 0; I < 3; ++I)
+#pragma unroll
+        for (int J = 0; J < 3; ++J) {
+            double acc = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc += P.F[c][I] * P.F[c][J];
+            E[I][J] = 0.5 * (acc - (I == J ? 1.0 : 0.0));
+        }
+    double tr = E[0][0] + E[1][1] + E[2][2];
+#pragma unroll
+    for (int I = 0; I < 3; ++I)
+#pragma unroll
+        for (int J = 0; J < 3; ++J) P.S[I][J] = 2.0 * mu * E[I][J] + (I == J ? lam * tr : 0.0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) P.FFt[c][d] = P.F[c][0] * P.F[d][0] + P.F[c][1] * P.F[d][1] + P.F[c][2] * P.F[d][2];
+}
+
+__device__ inline void nl_block(const NLPoint &P, const double ga[3], const double gb[3], double wl, double wm, double w,
+                                double out[9])
+{
+    double ha[3], hb[3], Sgb[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ha[c] = P.F[c][0] * ga[0] + P.F[c][1] * ga[1] + P.F[c][2] * ga[2];
+        hb[c] = P.F[c][0] * gb[0] + P.F[c][1] * gb[1] + P.F[c][2] * gb[2];
+        Sgb[c] = P.S[c][0] * gb[0] + P.S[c][1] * gb[1] + P.S[c][2] * gb[2];
+    }
+    double gg = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+    double gSg = w * (ga[0] * Sgb[0] + ga[1] * Sgb[1] + ga[2] * Sgb[2]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            out[3 * c + d] = wl * ha[c] * hb[d] + wm * P.FFt[c][d] * gg + wm * ha[d] * hb[c] + (c == d ? gSg : 0.0);
+}
+
+// element displacement from the free-DOF vector (clamped DOFs are 0,
+// src/nl_assemble.cpp:116-131)
+template <int NN>
+__device__ inline void gather_u(const int *cn, const int *__restrict__ resort, const double *__restrict__ u, double U[3 * NN])
+{
+#pragma unroll
+    for (int b = 0; b < NN; ++b) {
+        int r = resort[cn[b]];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) U[3 * b + c] = r >= 0 ? u[3 * r + c] : 0.0;
+    }
+}
+
+template <class Source>
+__device__ inline void nl_gather_blocks(int b0, int b1, const int *__restrict__ blk_rk,
+                                        const int *__restrict__ contrib_ptr, const uint16_t *__restrict__ contrib,
+                                        const int *__restrict__ col_ptr, double *__restrict__ val, int accumulate,
+                                        const Source &src)
+{
+    for (int bk = b0 + (int)threadIdx.x; bk < b1; bk += NL_WG) {
+        int rk = blk_rk[bk];
+        int r = rk >> 8, k = rk & 255;
+        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        int t1 = contrib_ptr[bk + 1];
+        for (int t = contrib_ptr[bk]; t < t1; ++t) src.add(contrib[t], acc);
+        int base = col_ptr[3 * r];
+        int stride = col_ptr[3 * r + 1] - base;
+        double *dst = val + base + 3 * k;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                double v = acc[3 * c + d];
+                if (accumulate) v += dst[c * stride + d];
+                dst[c * stride + d] = v;
+            }
+    }
+}
+
+// f_int of the chunk's nodes: sum of the pair forces in ascending element order
+template <class ForceFn>
+__device__ inline void nl_gather_force(int chunk, const int *__restrict__ chunk_node, const int *__restrict__ node_pair,
+                                       int p0, const ForceFn &pf, double *__restrict__ fint, int accumulate)
+{
+    const int r0 = chunk_node[chunk], r1 = chunk_node[chunk + 1];
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += NL_WG) {
+        double f[3] = {0.0, 0.0, 0.0};
+        for (int p = node_pair[r]; p < node_pair[r + 1]; ++p) pf(p - p0, f);
+        if (accumulate) {
+            f[0] += fint[3 * r];
+            f[1] += fint[3 * r + 1];
+            f[2] += fint[3 * r + 2];
+        }
+        fint[3 * r] = f[0];
+        fint[3 * r + 1] = f[1];
+        fint[3 * r + 2] = f[2];
+    }
+}
+
+// ------------------------------------------------------------------ C3D8
+struct NLHexSource {
+    const double (*blk)[9];
+    __device__ void add(int ci, double acc[9]) const
+    {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) acc[i] += blk[ci][i];
+    }
+};
+
+__global__ __launch_bounds__(NL_WG) void k_nl_hex(const double *__restrict__ xyz, const int *__restrict__ conn,
+                                                  const int *__restrict__ resort, const double *__restrict__ u, double lam,
+                                                  double mu, const int *__restrict__ chunk_pair,
+                                                  const int *__restrict__ chunk_blk, const int *__restrict__ chunk_node,
+                                                  const int *__restrict__ node_pair, const int *__restrict__ pair,
+                                                  const int *__restrict__ blk_rk, const int *__restrict__ contrib_ptr,
+                                                  const uint16_t *__restrict__ contrib, const int *__restrict__ col_ptr,
+                                                  double *__restrict__ val, double *__restrict__ fint, int accumulate,
+                                                  int nchunks)
+{
+    __shared__ double s_blk[NL_HEX_CAP * 8][9];
+    __shared__ double s_f[NL_HEX_CAP][3];
+    const int chunk = xcd_remap(blockIdx.x, nchunks);
+    const int p0 = chunk_pair[chunk], p1 = chunk_pair[chunk + 1];
+    if (p1 > p0) {
+        const int lp = threadIdx.x >> 3, gp = threadIdx.x & 7;
+        const int p = min(p0 + lp, p1 - 1);
+        const int pk = pair[p];
+        const int el = pk >> 3, a = pk & 7;
+        const int *cn = conn + 8 * el;
+        double X[24], U[24];
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) X[3 * b + c] = xyz[3 * cn[b] + c];
+        gather_u<8>(cn, resort, u, U);
+        double g[3][8];
+        const double w = hex_point<double>(X, gp, g);  // weight 1
+        double H[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int J = 0; J < 3; ++J) {
+                double acc = 0.0;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) acc += U[3 * b + c] * g[J][b];
+                H[c][J] = acc;
+            }
+        NLPoint P;
+        nl_state(H, lam, mu, P);
+        double ga[3];
+        pick_col<double, 8>(g, a, ga);
+        const double wl = w * lam, wm = w * mu;
+        auto blk = [&](int b, double out[9]) {
+            double gb[3] = {g[0][b], g[1][b], g[2][b]};
+            nl_block(P, ga, gb, wl, wm, w, out);
+        };
+        double mine[9];
+        reduce_scatter8<9, double>(gp, blk, mine);
+        // f_a = w F S g_a, summed over the 8 Gauss points of the group
+        double fa[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double acc = 0.0;
+#pragma unroll
+            for (int J = 0; J < 3; ++J) {
+                double SgJ = P.S[J][0] * ga[0] + P.S[J][1] * ga[1] + P.S[J][2] * ga[2];
+                acc += P.F[c][J] * SgJ;
+            }
+            fa[c] = group8_sum(w * acc);
+        }
+        if (p0 + lp < p1) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) s_blk[lp * 8 + gp][i] = mine[i];
+            if (gp == 0) {
+                s_f[lp][0] = fa[0];
+                s_f[lp][1] = fa[1];
+                s_f[lp][2] = fa[2];
+            }
+        }
+    }
+    __syncthreads();
+    NLHexSource src{s_blk};
+    nl_gather_blocks(chunk_blk[chunk], chunk_blk[chunk + 1], blk_rk, contrib_ptr, contrib, col_ptr, val, accumulate, src);
+    auto pf = [&](int lp, double f[3]) {
+        f[0] += s_f[lp][0];
+        f[1] += s_f[lp][1];
+        f[2] += s_f[lp][2];
+    };
+    nl_gather_force(chunk, chunk_node, node_pair, p0, pf, fint, accumulate);
+}
+
+// ------------------------------------------------------------------ C3D4
+__device__ inline void load_tet_state(const double *st, NLPoint &P)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) P.F[i][j] = st[3 * i + j];
+    P.S[0][0] = st[9];
+    P.S[1][1] = st[10];
+    P.S[2][2] = st[11];
+    P.S[0][1] = P.S[1][0] = st[12];
+    P.S[1][2] = P.S[2][1] = st[13];
+    P.S[0][2] = P.S[2][0] = st[14];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) P.FFt[c][d] = P.F[c][0] * P.F[d][0] + P.F[c][1] * P.F[d][1] + P.F[c][2] * P.F[d][2];
+}
+
+struct NLTetSource {
+    const double (*g)[12];
+    const double (*st)[16];  // F (9), S (xx yy zz xy yz xz), w
+    const int *a;
+    double lam, mu;
+    __device__ void add(int ci, double acc[9]) const
+    {
+        int lp = ci >> 2, b = ci & 3, aa = a[lp];
+        NLPoint P;
+        load_tet_state(st[lp], P);
+        double w = st[lp][15];
+        double ga[3] = {g[lp][3 * aa], g[lp][3 * aa + 1], g[lp][3 * aa + 2]};
+        double gb[3] = {g[lp][3 * b], g[lp][3 * b + 1], g[lp][3 * b + 2]};
+        double out[9];
+        nl_block(P, ga, gb, w * lam, w * mu, w, out);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) acc[i] += out[i];
+    }
+};
+
+__global__ __launch_bounds__(NL_WG) void k_nl_tet(const double *__restrict__ xyz, const int *__restrict__ conn,
+                                                  const int *__restrict__ resort, const double *__restrict__ u, double lam,
+                                                  double mu, const int *__restrict__ chunk_pair,
+                                                  const int *__restrict__ chunk_blk, const int *__restrict__ chunk_node,
+                                                  const int *__restrict__ node_pair, const int *__restrict__ pair,
+                                                  const int *__restrict__ blk_rk, const int *__restrict__ contrib_ptr,
+                                                  const uint16_t *__restrict__ contrib, const int *__restrict__ col_ptr,
+                                                  double *__restrict__ val, double *__restrict__ fint, int accumulate,
+                                                  int nchunks)
+{
+    __shared__ double s_g[NL_TET_CAP][12];
+    __shared__ double s_st[NL_TET_CAP][16];
+    __shared__ int s_a[NL_TET_CAP];
+    const int chunk = xcd_remap(blockIdx.x, nchunks);
+    const int p0 = chunk_pair[chunk], p1 = chunk_pair[chunk + 1];
+    for (int p = p0 + (int)threadIdx.x; p < p1; p += NL_WG) {
+        const int pk = pair[p], el = pk >> 3, a = pk & 7, lp = p - p0;
+        const int *cn = conn + 4 * el;
+        double X[12], U[12];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) X[3 * b + c] = xyz[3 * cn[b] + c];
+        gather_u<4>(cn, resort, u, U);
+        double g[3][4];
+        const double w = tet_grads<double>(X, g) / 6.0;  // Hammer weight 1/6
+        double H[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int J = 0; J < 3; ++J) H[c][J] = U[c] * g[J][0] + U[3 + c] * g[J][1] + U[6 + c] * g[J][2] + U[9 + c] * g[J][3];
+        NLPoint P;
+        nl_state(H, lam, mu, P);
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s_g[lp][3 * b + c] = g[c][b];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) s_st[lp][3 * i + j] = P.F[i][j];
+        s_st[lp][9] = P.S[0][0];
+        s_st[lp][10] = P.S[1][1];
+        s_st[lp][11] = P.S[2][2];
+        s_st[lp][12] = P.S[0][1];
+        s_st[lp][13] = P.S[1][2];
+        s_st[lp][14] = P.S[0][2];
+        s_st[lp][15] = w;
+        s_a[lp] = a;
+    }
+    __syncthreads();
+    NLTetSource src{s_g, s_st, s_a, lam, mu};
+    nl_gather_blocks(chunk_blk[chunk], chunk_blk[chunk + 1], blk_rk, contrib_ptr, contrib, col_ptr, val, accumulate, src);
+    // f_a = w F S g_a from the staged point state
+    auto pf = [&](int lp, double f[3]) {
+        NLPoint P;
+        load_tet_state(s_st[lp], P);
+        const int aa = s_a[lp];
+        const double ga[3] = {s_g[lp][3 * aa], s_g[lp][3 * aa + 1], s_g[lp][3 * aa + 2]};
+        const double w = s_st[lp][15];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double acc = 0.0;
+#pragma unroll
+            for (int J = 0; J < 3; ++J) acc += P.F[c][J] * (P.S[J][0] * ga[0] + P.S[J][1] * ga[1] + P.S[J][2] * ga[2]);
+            f[c] += w * acc;
+        }
+    };
+    nl_gather_force(chunk, chunk_node, node_pair, p0, pf, fint, accumulate);
+}
+
+int nl_plan_cap(int type) { return type == OFEA_C3D8 ? NL_HEX_CAP : NL_TET_CAP; }
+
+int nl_assemble(ofea_ctx *c, double E, double nu, const double *u, double *nz_val, double *f_int)
+{
+    const int n = c->P.n;
+    if (!c->d_u) OFEA_HIP(hipMalloc((void **)&c->d_u, sizeof(double) * std::max(n, 1)));
+    if (!c->d_fint) OFEA_HIP(hipMalloc((void **)&c->d_fint, sizeof(double) * std::max(n, 1)));
+    OFEA_HIP(hipMemcpyAsync(c->d_u, u, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    const bool mixed = c->has_type[0] && c->has_type[1];
+    if (mixed) {
+        OFEA_HIP(hipMemsetAsync(c->d_val, 0, sizeof(double) * c->P.nnz, c->stream));
+        OFEA_HIP(hipMemsetAsync(c->d_fint, 0, sizeof(double) * std::max(n, 1), c->stream));
+    }
+    double lam, mu;
+    lame<double>(E, nu, lam, mu);
+    for (int t = 0; t < 2; ++t) {
+        if (!c->has_type[t]) continue;
+        const DevPlan &d = c->dp[t];
+        if (d.nchunks == 0) continue;
+        auto kern = t == OFEA_C3D8 ? k_nl_hex : k_nl_tet;
+        hipLaunchKernelGGL(kern, dim3(d.nchunks), dim3(NL_WG), 0, c->stream, c->d_xyz, d.conn, c->d_resort, c->d_u, lam,
+                           mu, d.chunk_pair, d.chunk_blk, d.chunk_node, d.node_pair, d.pair, d.blk_rk, d.contrib_ptr,
+                           d.contrib, c->d_col_ptr, c->d_val, c->d_fint, mixed ? 1 : 0, d.nchunks);
+        OFEA_HIP(hipGetLastError());
+    }
+    c->assembled = true;
+    solver_invalidate(c);
+    if (nz_val) OFEA_HIP(hipMemcpyAsync(nz_val, c->d_val, sizeof(double) * c->P.nnz, hipMemcpyDeviceToHost, c->stream));
+    if (f_int) OFEA_HIP(hipMemcpyAsync(f_int, c->d_fint, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    OFEA_HIP(hipStreamSynchronize(c->stream));
+    return OFEA_OK;
+}
+
+}  // namespace ofea

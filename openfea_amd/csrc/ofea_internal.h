@@ -1,0 +1,152 @@
+// ofea_internal.h -- shared declarations of libofea_hip.so (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/openfea_amd.h"
+
+namespace ofea {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define OFEA_HIP(call)                                                                          \
+    do {                                                                                        \
+        hipError_t _e = (call);                                                                 \
+        if (_e != hipSuccess)                                                                   \
+            return ::ofea::fail(OFEA_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// ---------------------------------------------------------------- host mesh pieces
+// CSC-of-symmetric pattern, identical to assamble_stiffness::build_CSR
+// (src/assemble.cpp:19-77).  Built node-block-wise: the three columns of free
+// node r hold rows 3s+d for every free neighbour s (sorted) and d = 0..2.
+struct Pattern {
+    int nfree = 0;                 // free nodes
+    int n = 0;                     // 3 * nfree DOFs
+    long long nnz = 0;
+    std::vector<int> col_ptr;      // n + 1
+    std::vector<int> row_idx;      // nnz
+    std::vector<int> node_blk;     // nfree + 1: first 3x3 block of node r (= col_ptr[3r] / 9)
+    std::vector<int> bcol;         // nnz / 9: neighbour rank of each block, (r, k) order
+};
+
+// Column-owner gather plan for one element type.  The free nodes are cut into
+// chunks of consecutive ranks; one workgroup owns a chunk's columns, computes
+// the element data of every (node, incident element) pair of the chunk and
+// gathers each 3x3 block from its contributors in ascending element order
+// (the summation order of src/assemble.cpp:106-149).  Every CSC value is
+// written once and no atomics are used.
+struct GatherPlan {
+    int nn = 0;                          // nodes per element (4 or 8)
+    int ne = 0;                          // elements of this type
+    std::vector<int> elem_global;        // type-local -> global element index
+    std::vector<int> conn;               // ne * nn, 0-based node ids
+    std::vector<int> chunk_pair;         // nchunks + 1
+    std::vector<int> chunk_blk;          // nchunks + 1
+    std::vector<int> pair;               // (elem_local << 3) | a, pairs sorted by (node, elem)
+    std::vector<int> blk_rk;             // (r << 8) | k per block of the chunk list
+    std::vector<int> contrib_ptr;        // blocks + 1
+    std::vector<uint16_t> contrib;       // chunk-local pair * nn + b
+    std::vector<int> node_pair;          // nfree + 1: pairs of node r = [node_pair[r], node_pair[r+1])
+    std::vector<int> chunk_node;         // nchunks + 1: first free node of each chunk
+    int nchunks() const { return (int)chunk_pair.size() - 1; }
+};
+
+int build_pattern(int nd, int ne, const int *conn, const int *etype, const int *resort, Pattern &P);
+int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const int *resort, const Pattern &P,
+                      int type, int pair_cap, GatherPlan &G);
+
+// ---------------------------------------------------------------- device plan copy
+struct DevPlan {
+    int nn = 0, ne = 0, nchunks = 0, nblk = 0;
+    int *conn = nullptr;
+    double *E = nullptr;  // per-element modulus (type-local order) or null
+    int *chunk_pair = nullptr, *chunk_blk = nullptr, *pair = nullptr, *blk_rk = nullptr, *contrib_ptr = nullptr;
+    uint16_t *contrib = nullptr;
+    int *node_pair = nullptr, *chunk_node = nullptr;
+};
+
+}  // namespace ofea
+
+// --------------------------------------------------------------------- context
+struct ofea_deck {
+    int nd = 0, ne = 0;
+    std::vector<double> xyz;
+    std::vector<int> conn, etype, fix_set, load_set;
+    int load_dof = 0;
+    double load_value = 0.0;
+    double E, nu, rho, dt = 0.0, t_total = 0.0;
+};
+
+struct ofea_solver_ws;  // solver work space (solver.hip)
+
+struct ofea_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // mesh (host)
+    int nd = 0, ne = 0;
+    std::vector<double> xyz;
+    std::vector<int> conn, etype, resort;
+    ofea::Pattern P;
+    ofea::GatherPlan G[2];
+    bool has_type[2] = {false, false};
+    std::vector<double> E_elem;  // empty = global E
+    // device
+    double *d_xyz = nullptr;
+    int *d_col_ptr = nullptr, *d_node_blk = nullptr, *d_bcol = nullptr;
+    int *d_resort = nullptr;
+    double *d_val = nullptr;    // FP64 values, nnz
+    ofea::DevPlan dp[2];
+    bool assembled = false;
+    double asm_E = 0.0, asm_nu = 0.0;
+    // solver
+    ofea_solver_ws *ws = nullptr;
+    double *d_rhs = nullptr;
+    double *d_x = nullptr;
+    bool rhs_set = false;
+    // explicit / nl scratch
+    void *explicit_ws = nullptr;
+    double *d_u = nullptr, *d_fint = nullptr;  // TL displacement / internal force (n)
+    // learned preconditioner parameters (device), set by ofea_learned_*
+    void *learned = nullptr;
+};
+
+namespace ofea {
+// kernels (assemble.hip)
+int launch_assemble(ofea_ctx *ctx, double E, double nu, double *d_out, bool zero_first);
+// solver (solver.hip)
+int solver_spmv(ofea_ctx *ctx, const double *d_x, double *d_y);
+int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, double *relres);
+void solver_free(ofea_ctx *ctx);
+void solver_invalidate(ofea_ctx *ctx);  // K changed: preconditioner and FP32 copy are stale
+// explicit (explicit.hip)
+int explicit_run(ofea_ctx *ctx, double E, double nu, const double *mass, const double *f_ext, const double *v0,
+                 const int *fixed_mask, double dt, int nsteps, int precision, double *u_out);
+int lumped_mass(ofea_ctx *ctx, double rho, double *mass);
+void explicit_free(ofea_ctx *ctx);
+// nonlinear (nl.hip)
+int nl_assemble(ofea_ctx *ctx, double E, double nu, const double *u, double *nz_val, double *f_int);
+
+template <class T>
+int dev_upload(T *&d, const T *h, size_t count)
+{
+    if (d) { hipFree(d); d = nullptr; }
+    if (count == 0) return OFEA_OK;
+    OFEA_HIP(hipMalloc((void **)&d, count * sizeof(T)));
+    if (h) OFEA_HIP(hipMemcpy(d, h, count * sizeof(T), hipMemcpyHostToDevice));
+    return OFEA_OK;
+}
+template <class T>
+void dev_free(T *&d)
+{
+    if (d) hipFree(d);
+    d = nullptr;
+}
+}  // namespace ofea

@@ -1,0 +1,255 @@
+"""Host-side mirror of the reference's call surface over libofea_hip.so.
+
+Names follow the reference so that a driver written against it reads the same:
+  ReadInfo-style deck reading      -> read_inp()        (src/data2cae.cpp)
+  set_BCs::build_free_index        -> build_free_index() (src/set_bc.cpp:20-43)
+  set_BCs::build_single_load       -> build_single_load() (src/set_bc.cpp:46-77)
+  assamble_stiffness               -> AssambleStiffness  (src/include/assemble.h:35-72)
+  solution_api(..., solver_type)   -> solution_api()     (src/linear_solution.cpp:17-64)
+  simulation_post::reset_displacement -> reset_displacement() (src/post_process.cpp:18-40)
+All numerical work runs in the native library; this module only marshals arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi as A
+from ._capi import check, f64, i32, lib, ptr
+
+
+@dataclass
+class Deck:
+    """data_management fields filled by the reader (src/include/data_management.h:45-104)."""
+    xyz: np.ndarray            # nd x 3
+    conn: np.ndarray           # ne x 8, 1-based
+    etype: np.ndarray          # ne (0 C3D4, 1 C3D8)
+    fix_set: np.ndarray        # dis_bc_set_
+    load_set: np.ndarray       # load_set_
+    load_dof: int = 0
+    load_value: float = 0.0
+    E: float = float("nan")
+    nu: float = float("nan")
+    rho: float = float("nan")
+    dt: float = 0.0
+    t_total: float = 0.0
+    resort: np.ndarray = field(default=None)
+    F: np.ndarray = field(default=None)
+
+    @property
+    def nd(self):
+        return len(self.xyz)
+
+    @property
+    def ne(self):
+        return len(self.etype)
+
+
+def read_inp(path: str) -> Deck:
+    """Read a reference-dialect .inp deck through the native reader."""
+    L = lib()
+    h = C.c_void_p()
+    check(L.ofea_inp_open(path.encode(), C.byref(h)))
+    try:
+        nd, ne, nfix, nload = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(L.ofea_inp_sizes(h, C.byref(nd), C.byref(ne), C.byref(nfix), C.byref(nload)))
+        xyz = np.zeros((nd.value, 3))
+        conn = np.zeros((ne.value, 8), dtype=np.int32)
+        etype = np.zeros(ne.value, dtype=np.int32)
+        fix = np.zeros(nfix.value, dtype=np.int32)
+        load = np.zeros(nload.value, dtype=np.int32)
+        dof = C.c_int()
+        vals = [C.c_double() for _ in range(6)]
+        check(L.ofea_inp_get(h, ptr(xyz), ptr(conn), ptr(etype), ptr(fix), ptr(load), C.byref(dof),
+                             *[C.byref(v) for v in vals]))
+    finally:
+        L.ofea_inp_close(h)
+    lv, E, nu, rho, dt, T = [v.value for v in vals]
+    return Deck(xyz, conn, etype, fix, load, dof.value, lv, E, nu, rho, dt, T)
+
+
+def build_free_index(nd: int, fix_set) -> tuple[np.ndarray, int]:
+    fix = i32(fix_set).copy()
+    resort = np.zeros(nd, dtype=np.int32)
+    nfree = lib().ofea_build_free_index(nd, ptr(fix), len(fix), ptr(resort))
+    if nfree < 0:
+        check(nfree)
+    return resort, nfree
+
+
+def build_single_load(n: int, resort, load_set, dof: int, value: float) -> np.ndarray:
+    F = np.zeros(n)
+    ls = i32(load_set)
+    check(lib().ofea_build_single_load(n, ptr(i32(resort)), ptr(ls), len(ls), dof, value, ptr(F)))
+    return F
+
+
+def build_csr(conn, etype, resort):
+    """build_CSR on the host (no device): CSC col_ptr (n+1) and row_idx (nnz)."""
+    conn, etype, resort = i32(conn).reshape(-1, 8), i32(etype), i32(resort)
+    n = C.c_int()
+    nnz = C.c_longlong()
+    L = lib()
+    check(L.ofea_build_csr(len(resort), len(etype), ptr(conn), ptr(etype), ptr(resort), C.byref(n), C.byref(nnz),
+                           None, None))
+    col_ptr = np.zeros(n.value + 1, dtype=np.int32)
+    row_idx = np.zeros(nnz.value, dtype=np.int32)
+    check(L.ofea_build_csr(len(resort), len(etype), ptr(conn), ptr(etype), ptr(resort), None, None, ptr(col_ptr),
+                           ptr(row_idx)))
+    return col_ptr, row_idx
+
+
+def reset_displacement(resort, u_free) -> np.ndarray:
+    resort = i32(resort)
+    out = np.zeros(3 * len(resort))
+    check(lib().ofea_reset_displacement(len(resort), ptr(resort), ptr(f64(u_free)), ptr(out)))
+    return out
+
+
+class Context:
+    """One device context (ofea_ctx) holding a mesh, its CSC pattern and K."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        check(lib().ofea_create(C.byref(self._h), device))
+        self.n = 0
+        self.nnz = 0
+        self.nd = 0
+        self.ne = 0
+
+    def close(self):
+        if self._h:
+            lib().ofea_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- mesh / pattern
+    def set_mesh(self, xyz, conn, etype, resort):
+        xyz = f64(xyz).reshape(-1, 3)
+        conn = i32(conn).reshape(-1, 8)
+        etype = i32(etype)
+        resort = i32(resort)
+        self._keep = (xyz, conn, etype, resort)
+        check(lib().ofea_set_mesh(self._h, len(xyz), ptr(xyz), len(etype), ptr(conn), ptr(etype), ptr(resort)))
+        n = C.c_int()
+        nnz = C.c_longlong()
+        check(lib().ofea_pattern(self._h, C.byref(n), C.byref(nnz), None, None))
+        self.n, self.nnz, self.nd, self.ne = n.value, nnz.value, len(xyz), len(etype)
+
+    def pattern(self):
+        col_ptr = np.zeros(self.n + 1, dtype=np.int32)
+        row_idx = np.zeros(self.nnz, dtype=np.int32)
+        check(lib().ofea_pattern(self._h, None, None, ptr(col_ptr), ptr(row_idx)))
+        return col_ptr, row_idx
+
+    # ---- assembly
+    def set_elem_E(self, E_elem):
+        if E_elem is None:
+            check(lib().ofea_set_elem_E(self._h, None))
+        else:
+            check(lib().ofea_set_elem_E(self._h, ptr(f64(E_elem))))
+
+    def assemble(self, E: float, nu: float, copy: bool = True):
+        out = np.zeros(self.nnz) if copy else None
+        check(lib().ofea_assemble(self._h, E, nu, ptr(out)))
+        return out
+
+    def spmv(self, x):
+        y = np.zeros(self.n)
+        check(lib().ofea_spmv(self._h, ptr(f64(x)), ptr(y)))
+        return y
+
+    # ---- solve
+    def set_rhs(self, b):
+        check(lib().ofea_set_rhs(self._h, ptr(f64(b))))
+
+    def solve(self, b=None, method=A.PCG_JACOBI, rtol=1e-8, maxit=100000, copy=True):
+        x = np.zeros(self.n) if copy else None
+        it = C.c_int()
+        rr = C.c_double()
+        bb = None if b is None else f64(b)
+        rc = check(lib().ofea_solve(self._h, method, ptr(bb), ptr(x), rtol, maxit, C.byref(it), C.byref(rr)),
+                   allow=(A.OFEA_OK, A.OFEA_NOT_CONVERGED))
+        return x, it.value, rr.value, rc == A.OFEA_OK
+
+    # ---- explicit
+    def lumped_mass(self, rho: float):
+        m = np.zeros(self.nd)
+        check(lib().ofea_lumped_mass(self._h, rho, ptr(m)))
+        return m
+
+    def explicit_run(self, E, nu, mass, f_ext, v0, fixed_mask, dt, nsteps, precision=A.FP32, copy=True):
+        u = np.zeros(3 * self.nd) if copy else None
+        v0a = None if v0 is None else f64(v0).ravel()
+        check(lib().ofea_explicit_run(self._h, E, nu, ptr(f64(mass)), ptr(f64(f_ext).ravel()), ptr(v0a),
+                                      ptr(i32(fixed_mask)), dt, nsteps, precision, ptr(u)))
+        return u
+
+    # ---- geometric nonlinear
+    def nl_assemble(self, E, nu, u_free, copy=True):
+        val = np.zeros(self.nnz) if copy else None
+        f = np.zeros(self.n) if copy else None
+        check(lib().ofea_nl_assemble(self._h, E, nu, ptr(f64(u_free)), ptr(val), ptr(f)))
+        return val, f
+
+    # ---- timing on the context stream
+    def timer_start(self):
+        check(lib().ofea_timer_start(self._h))
+
+    def timer_stop(self) -> float:
+        ms = C.c_float()
+        check(lib().ofea_timer_stop(self._h, C.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        check(lib().ofea_sync(self._h))
+
+
+class AssambleStiffness:
+    """assamble_stiffness (src/include/assemble.h:35-72): build_CSR then
+    fill_CSR_sparse_mat, with the same public arrays."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or Context()
+        self.num_row_ = self.num_col_ = self.num_nz_val_ = 0
+        self.nz_val_ = None
+        self.row_idx_ = None
+        self.col_idx_ = None
+
+    def build_CSR(self, deck: Deck):
+        self.ctx.set_mesh(deck.xyz, deck.conn, deck.etype, deck.resort)
+        self.col_idx_, self.row_idx_ = self.ctx.pattern()
+        self.num_row_ = self.num_col_ = self.ctx.n
+        self.num_nz_val_ = self.ctx.nnz
+
+    def fill_CSR_sparse_mat(self, deck: Deck, E: float, nu: float):
+        self.nz_val_ = self.ctx.assemble(E, nu)
+
+
+SOLVERS = {"GPU_PCG": A.PCG_JACOBI, "GPU_PCG_BLOCK": A.PCG_BLOCK_JACOBI, "GPU_PCG_MIXED": A.PCG_MIXED,
+           "GPU_PCG_LEARNED": A.PCG_LEARNED}
+
+
+def solution_api(assam: AssambleStiffness, deck: Deck, solver_type: str = "GPU_PCG", rtol: float = 1e-10,
+                 maxit: int = 200000):
+    """solution_api (src/linear_solution.cpp:17-64) with the GPU solver
+    strings; returns False for an unknown solver string like the reference."""
+    if solver_type not in SOLVERS:
+        return False
+    x, it, rel, ok = assam.ctx.solve(deck.F, SOLVERS[solver_type], rtol, maxit)
+    deck.single_dis_vec_ = x
+    deck.solve_info = {"iters": it, "relres": rel, "converged": ok}
+    return True

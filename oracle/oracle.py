@@ -1,0 +1,198 @@
+"""ctypes front-end of the CPU checker (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker, never as the measured or
+shipped path.  Arrays follow the layout documented in ofea_oracle.c.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_d = C.c_double
+_i = C.c_int
+_ll = C.c_longlong
+_pd = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_pi = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (run `make -C oracle`)")
+        L = C.CDLL(path)
+        L.or_dmat.argtypes = [_d, _d, _pd]
+        L.or_c3d8_ke.argtypes = [_pd, _pd, _pd]
+        L.or_c3d4_ke.argtypes = [_pd, _pd, _pd]
+        L.or_c3d4_ke.restype = _d
+        L.or_c3d8_nl.argtypes = [_pd, _pd, _pd, _pd, _pd]
+        L.or_c3d4_nl.argtypes = [_pd, _pd, _pd, _pd, _pd]
+        L.or_c3d8_mass.argtypes = [_pd, _d, _pd]
+        L.or_c3d4_mass.argtypes = [_pd, _d, _pd]
+        L.or_free_index.argtypes = [_i, _pi, _i, _pi]
+        L.or_free_index.restype = _i
+        L.or_single_load.argtypes = [_i, _pi, _pi, _i, _i, _d, _pd]
+        L.or_build_csc.argtypes = [_i, _i, _pi, _pi, _pi, _i, _vp, _vp]
+        L.or_build_csc.restype = _ll
+        L.or_fill_csc.argtypes = [_i, _pi, _pi, _pd, _pi, _d, _d, _vp, _pi, _pi, _ll, _pd]
+        L.or_nl_fill.argtypes = [_i, _pi, _pi, _pd, _pi, _d, _d, _pd, _pi, _pi, _ll, _pd, _pd, _i]
+        L.or_spmv_csc.argtypes = [_i, _pi, _pi, _pd, _pd, _pd]
+        L.or_pcg_jacobi.argtypes = [_i, _pi, _pi, _pd, _pd, _pd, _d, _i, C.POINTER(_d)]
+        L.or_pcg_jacobi.restype = _i
+        L.or_reset_displacement.argtypes = [_i, _pi, _pd, _pd]
+        L.or_lumped_mass.argtypes = [_i, _i, _pi, _pi, _pd, _d, _pd]
+        L.or_explicit_run.argtypes = [_i, _i, _pi, _pi, _pd, _d, _d, _pd, _pd, _vp, _pi, _d, _i, _pd]
+        L.or_matfree_apply.argtypes = [_i, _pi, _pi, _pd, _pi, _d, _d, _vp, _pd, _pd, _i]
+        _LIB = L
+    return _LIB
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def dmat(E, nu):
+    D = np.zeros(36)
+    lib().or_dmat(E, nu, D)
+    return D.reshape(6, 6)
+
+
+def c3d8_ke(X, E, nu):
+    Ke = np.zeros(576)
+    lib().or_c3d8_ke(_f64(X).ravel(), _f64(dmat(E, nu)).ravel(), Ke)
+    return Ke.reshape(24, 24)
+
+
+def c3d4_ke(X, E, nu):
+    Ke = np.zeros(144)
+    lib().or_c3d4_ke(_f64(X).ravel(), _f64(dmat(E, nu)).ravel(), Ke)
+    return Ke.reshape(12, 12)
+
+
+def c3d8_nl(X, u, E, nu):
+    K = np.zeros(576)
+    f = np.zeros(24)
+    lib().or_c3d8_nl(_f64(X).ravel(), _f64(u).ravel(), _f64(dmat(E, nu)).ravel(), K, f)
+    return K.reshape(24, 24), f
+
+
+def c3d4_nl(X, u, E, nu):
+    K = np.zeros(144)
+    f = np.zeros(12)
+    lib().or_c3d4_nl(_f64(X).ravel(), _f64(u).ravel(), _f64(dmat(E, nu)).ravel(), K, f)
+    return K.reshape(12, 12), f
+
+
+def c3d8_mass(X, rho):
+    m = np.zeros(8)
+    lib().or_c3d8_mass(_f64(X).ravel(), rho, m)
+    return m
+
+
+def c3d4_mass(X, rho):
+    m = np.zeros(4)
+    lib().or_c3d4_mass(_f64(X).ravel(), rho, m)
+    return m
+
+
+def free_index(nd, fix_set):
+    fix = _i32(fix_set).copy()
+    resort = np.zeros(nd, dtype=np.int32)
+    nfree = lib().or_free_index(nd, fix, len(fix), resort)
+    return resort, nfree
+
+
+def single_load(n, resort, load_set, dof, value):
+    F = np.zeros(n)
+    ls = _i32(load_set)
+    lib().or_single_load(n, _i32(resort), ls, len(ls), dof, value, F)
+    return F
+
+
+def build_csc(conn, etype, resort, n):
+    conn = _i32(conn)
+    etype = _i32(etype)
+    resort = _i32(resort)
+    nd = len(resort)
+    ne = len(etype)
+    col_ptr = np.zeros(n + 1, dtype=np.int32)
+    nnz_upper = lib().or_build_csc(nd, ne, conn, etype, resort, n, None, None)
+    row_idx = np.zeros(max(nnz_upper, 1), dtype=np.int32)
+    nnz = lib().or_build_csc(nd, ne, conn, etype, resort, n, col_ptr.ctypes.data, row_idx.ctypes.data)
+    return col_ptr, row_idx[:nnz].copy()
+
+
+def fill_csc(conn, etype, xyz, resort, E, nu, col_ptr, row_idx, E_elem=None):
+    val = np.zeros(len(row_idx))
+    Ee = None if E_elem is None else _f64(E_elem)
+    lib().or_fill_csc(len(etype), _i32(conn), _i32(etype), _f64(xyz).ravel(), _i32(resort), E, nu,
+                      None if Ee is None else Ee.ctypes.data, _i32(col_ptr), _i32(row_idx), len(row_idx), val)
+    return val
+
+
+def nl_fill(conn, etype, xyz, resort, E, nu, u_free, col_ptr, row_idx):
+    val = np.zeros(len(row_idx))
+    n = len(col_ptr) - 1
+    f = np.zeros(n)
+    lib().or_nl_fill(len(etype), _i32(conn), _i32(etype), _f64(xyz).ravel(), _i32(resort), E, nu, _f64(u_free),
+                     _i32(col_ptr), _i32(row_idx), len(row_idx), val, f, n)
+    return val, f
+
+
+def spmv_csc(col_ptr, row_idx, val, x):
+    n = len(col_ptr) - 1
+    y = np.zeros(n)
+    lib().or_spmv_csc(n, _i32(col_ptr), _i32(row_idx), _f64(val), _f64(x), y)
+    return y
+
+
+def pcg_jacobi(col_ptr, row_idx, val, b, rtol=1e-8, maxit=100000):
+    n = len(col_ptr) - 1
+    x = np.zeros(n)
+    rr = C.c_double(0.0)
+    it = lib().or_pcg_jacobi(n, _i32(col_ptr), _i32(row_idx), _f64(val), _f64(b), x, rtol, maxit, C.byref(rr))
+    return x, it, rr.value
+
+
+def reset_displacement(resort, u_free):
+    nd = len(resort)
+    out = np.zeros(3 * nd)
+    lib().or_reset_displacement(nd, _i32(resort), _f64(u_free), out)
+    return out
+
+
+def lumped_mass(conn, etype, xyz, rho):
+    nd = len(xyz)
+    m = np.zeros(nd)
+    lib().or_lumped_mass(nd, len(etype), _i32(conn), _i32(etype), _f64(xyz).ravel(), rho, m)
+    return m
+
+
+def explicit_run(conn, etype, xyz, E, nu, mass, f_ext, v0, fixed_mask, dt, nsteps):
+    nd = len(xyz)
+    u = np.zeros(3 * nd)
+    v0a = None if v0 is None else _f64(v0).ravel()
+    lib().or_explicit_run(nd, len(etype), _i32(conn), _i32(etype), _f64(xyz).ravel(), E, nu, _f64(mass),
+                          _f64(f_ext).ravel(), None if v0a is None else v0a.ctypes.data, _i32(fixed_mask), dt,
+                          nsteps, u)
+    return u
+
+
+def matfree_apply(conn, etype, xyz, resort, E, nu, x, n, E_elem=None):
+    y = np.zeros(n)
+    Ee = None if E_elem is None else _f64(E_elem)
+    lib().or_matfree_apply(len(etype), _i32(conn), _i32(etype), _f64(xyz).ravel(), _i32(resort), E, nu,
+                           None if Ee is None else Ee.ctypes.data, _f64(x), y, n)
+    return y
