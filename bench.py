@@ -55,11 +55,43 @@ def cpu_baseline_fill(m, resort, reps=3):
     return m.ne / statistics.median(times)
 
 
+def train_learned(workload, args):
+    """Trains the learned block-Jacobi correction on a smaller mesh of the same
+    family (different jitter seed) from that mesh's own block-Jacobi PCG runs;
+    the node-local network then transfers to the benchmark mesh."""
+    import numpy as np
+
+    import openfea_amd as ofa
+    from openfea_amd import learned as L, meshgen as mg
+
+    t0 = time.perf_counter()
+    if workload == "cfg2":
+        m, resort, F = mg.cantilever_c3d4(24, 12, 12, seed=7)
+    else:
+        m, resort, F = mg.cantilever_c3d8(10, 5, 5)
+    ctx = ofa.Context(0)
+    ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
+    ctx.assemble(mg.E_STEEL, mg.NU_STEEL, copy=False)
+    rng = np.random.default_rng(11)
+    rhs = [F] + [ctx.spmv(rng.normal(size=ctx.n)) for _ in range(3)]
+    E, R = L.collect_pairs(ctx, rhs, snapshots=(8, 16, 32, 64, 128))
+    ln = L.Learned(ctx, seed=0)
+    ln.set_samples(E, R)
+    hist = ln.train(steps=args.train_steps, lr=2e-3)
+    params = ln.get_params()
+    ctx.close()
+    info = {"train_mesh": f"{m.ne} elements ({'C3D4 24x12x12, jitter seed 7' if workload == 'cfg2' else 'C3D8 10x5x5'})",
+            "samples": int(len(E)), "steps": args.train_steps, "loss_first": float(hist[0]),
+            "loss_last": float(hist[-1]), "train_s": round(time.perf_counter() - t0, 3), "params": params}
+    log(f"learned: samples {len(E)} loss {hist[0]:.4e} -> {hist[-1]:.4e} in {info['train_s']}s")
+    return info
+
+
 def bench_static(args, workload):
     import numpy as np
 
     import openfea_amd as ofa
-    from openfea_amd import _capi as A, meshgen as mg
+    from openfea_amd import _capi as A, learned as L, meshgen as mg
 
     if workload == "cfg2":
         m, resort, F = mg.cantilever_c3d4()
@@ -94,8 +126,18 @@ def bench_static(args, workload):
     ctx.assemble(E, nu, copy=False)
     ctx.set_rhs(F)
     solves = {}
-    for name, method in (("plain_fp64_jacobi", A.PCG_JACOBI), ("fp64_block_jacobi", A.PCG_BLOCK_JACOBI),
-                         ("mixed_fp32_block_jacobi", A.PCG_MIXED)):
+    all_methods = {"plain_fp64_jacobi": A.PCG_JACOBI, "fp64_block_jacobi": A.PCG_BLOCK_JACOBI,
+                   "mixed_fp32val_block_jacobi": A.PCG_MIXED, "mixed_fp32_all_block_jacobi": A.PCG_MIXED32,
+                   "learned_fp64": A.PCG_LEARNED, "learned_mixed_fp32val": A.PCG_LEARNED_MIXED}
+    chosen = [] if args.no_solve else [m for m in args.solvers.split(",") if m]
+    methods = [(m, all_methods[m]) for m in chosen]
+    learned_info = None
+    if any(m.startswith("learned") for m in chosen):
+        learned_info = train_learned(workload, args)
+        ln = L.Learned(ctx, seed=0)
+        ln.set_params(learned_info.pop("params"))
+        ctx.set_rhs(F)
+    for name, method in methods:
         ctx.solve(None, method=method, rtol=1e-8, maxit=100000, copy=False)  # warm (graph capture)
         ts = []
         for _ in range(args.solve_reps):
@@ -105,9 +147,9 @@ def bench_static(args, workload):
             ts.append(time.perf_counter() - t)
         solves[name] = {"ms": round(statistics.median(ts) * 1e3, 3), "iters": it, "relres": rel, "converged": ok}
         log(f"solve {name}: {solves[name]}")
-    base = solves["plain_fp64_jacobi"]["ms"]
+    base = solves.get("plain_fp64_jacobi", {}).get("ms")
     for k, v in solves.items():
-        v["speedup_vs_plain"] = round(base / v["ms"], 3)
+        v["speedup_vs_plain"] = round(base / v["ms"], 3) if base else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -125,6 +167,7 @@ def bench_static(args, workload):
         "config": {"workload": wl, "elements": m.ne, "nodes": m.nd, "dofs": ctx.n, "nnz": ctx.nnz,
                    "parallelism": "single GPU"},
         "time_to_solve": solves,
+        "learned_training": learned_info,
         "setup_s": round(setup_s, 3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -183,6 +226,10 @@ def main():
     ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3"])
     ap.add_argument("--solve-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-solve", action="store_true", help="assembly only (for counter collection)")
+    ap.add_argument("--solvers", default="plain_fp64_jacobi,fp64_block_jacobi,mixed_fp32val_block_jacobi,"
+                    "mixed_fp32_all_block_jacobi,learned_fp64,learned_mixed_fp32val")
+    ap.add_argument("--train-steps", type=int, default=150)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)

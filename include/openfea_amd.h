@@ -52,8 +52,10 @@ extern "C" {
 /* solver methods for ofea_solve */
 #define OFEA_PCG_JACOBI 0        /* FP64 values, point-Jacobi PCG                       */
 #define OFEA_PCG_BLOCK_JACOBI 1  /* FP64 values, 3x3 node-block Jacobi PCG              */
-#define OFEA_PCG_MIXED 2         /* FP32 inner block-Jacobi PCG + FP64 residual refinement */
-#define OFEA_PCG_LEARNED 3       /* as MIXED with the learned block-Jacobi correction   */
+#define OFEA_PCG_MIXED 2         /* FP32 matrix values, FP64 vectors, block Jacobi, FP64 reliable updates */
+#define OFEA_PCG_LEARNED 3       /* FP64 PCG with the learned block-Jacobi correction            */
+#define OFEA_PCG_MIXED32 4       /* FP32 values and FP32 vectors, block Jacobi, FP64 reliable updates */
+#define OFEA_PCG_LEARNED_MIXED 5 /* as MIXED with the learned block-Jacobi correction            */
 
 /* explicit precision for ofea_explicit_run */
 #define OFEA_FP32 0
@@ -145,10 +147,37 @@ int ofea_explicit_run(ofea_ctx *ctx, double E, double nu, const double *mass, co
  * f_int (n, free DOFs) at the free-DOF displacement u (n). */
 int ofea_nl_assemble(ofea_ctx *ctx, double E, double nu, const double *u, double *nz_val, double *f_int);
 
+/* ---- learned block-Jacobi correction (no counterpart in the reference) ----
+ * A 2-layer node-graph network (hidden 64) maps per-node features of the
+ * resident K to SPD 3x3 preconditioner blocks M_r = R_r^-1 L_r L_r^T R_r^-T
+ * (D_r = R_r^T R_r); L_r = I is block Jacobi.  OFEA_PCG_LEARNED uses them. */
+int ofea_learned_nparam(void);
+/* features of the resident K, deterministic initial parameters (head = 0,
+ * i.e. block Jacobi), forward pass */
+int ofea_learned_init(ofea_ctx *ctx, unsigned long long seed);
+int ofea_learned_set_params(ofea_ctx *ctx, const float *params, int nparam);
+int ofea_learned_get_params(ofea_ctx *ctx, float *params, int nparam);
+/* training pairs (S samples, sample-major, n entries each): errors E and
+ * residuals R = K E of the resident K */
+int ofea_learned_set_samples(ofea_ctx *ctx, const double *E, const double *R, int S);
+/* loss (1/S) sum_k |e_k - M r_k|_K^2 / (e_k^T r_k) and its gradient (nparam) */
+int ofea_learned_loss_grad(ofea_ctx *ctx, double *loss, float *grad);
+/* `steps` Adam steps at learning rate lr; loss_hist (steps, nullable) */
+int ofea_learned_train(ofea_ctx *ctx, int steps, float lr, double *loss_hist);
+/* current blocks (9 per free node, FP64) and features (9 per free node) */
+int ofea_learned_blocks(ofea_ctx *ctx, double *blocks, float *features);
+/* recompute features and blocks for a newly assembled K */
+int ofea_learned_refresh(ofea_ctx *ctx);
+
 /* ---- timing on the context stream (HIP events) ---- */
 int ofea_timer_start(ofea_ctx *ctx);
 int ofea_timer_stop(ofea_ctx *ctx, float *ms);
 int ofea_sync(ofea_ctx *ctx);
+/* HBM calibration: copies `bytes` between two device buffers `reps` times with
+ * a plain vector-load/store kernel on the context stream; *ms is the average
+ * per copy.  Used to confirm the achievable bandwidth and to scale the
+ * profiler's FETCH_SIZE / WRITE_SIZE counters. */
+int ofea_stream_copy(ofea_ctx *ctx, long long bytes, int reps, float *ms);
 
 #pragma GCC visibility pop
 #ifdef __cplusplus

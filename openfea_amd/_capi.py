@@ -29,6 +29,8 @@ PCG_JACOBI = 0
 PCG_BLOCK_JACOBI = 1
 PCG_MIXED = 2
 PCG_LEARNED = 3
+PCG_MIXED32 = 4
+PCG_LEARNED_MIXED = 5
 
 FP32 = 0
 FP64 = 1
@@ -68,9 +70,19 @@ SIGNATURES = {
     "ofea_lumped_mass": (_i, [_vp, _d, _vp]),
     "ofea_explicit_run": (_i, [_vp, _d, _d, _vp, _vp, _vp, _vp, _d, _i, _i, _vp]),
     "ofea_nl_assemble": (_i, [_vp, _d, _d, _vp, _vp, _vp]),
+    "ofea_learned_nparam": (_i, []),
+    "ofea_learned_init": (_i, [_vp, C.c_ulonglong]),
+    "ofea_learned_set_params": (_i, [_vp, _vp, _i]),
+    "ofea_learned_get_params": (_i, [_vp, _vp, _i]),
+    "ofea_learned_set_samples": (_i, [_vp, _vp, _vp, _i]),
+    "ofea_learned_loss_grad": (_i, [_vp, _pd, _vp]),
+    "ofea_learned_train": (_i, [_vp, _i, _f, _vp]),
+    "ofea_learned_blocks": (_i, [_vp, _vp, _vp]),
+    "ofea_learned_refresh": (_i, [_vp]),
     "ofea_timer_start": (_i, [_vp]),
     "ofea_timer_stop": (_i, [_vp, _pf]),
     "ofea_sync": (_i, [_vp]),
+    "ofea_stream_copy": (_i, [_vp, _ll, _i, _pf]),
 }
 
 _LIB = None

@@ -27,6 +27,7 @@ static void free_plan(DevPlan &d)
 
 static void free_mesh(ofea_ctx *c)
 {
+    learned_free(c);
     solver_free(c);
     explicit_free(c);
     dev_free(c->d_xyz);
@@ -179,6 +180,7 @@ int ofea_assemble(ofea_ctx *c, double E, double nu, double *nz_val)
     int rc = launch_assemble(c, E, nu, c->d_val, false);
     if (rc) return rc;
     c->assembled = true;
+    c->kver++;
     c->asm_E = E;
     c->asm_nu = nu;
     solver_invalidate(c);
@@ -274,6 +276,60 @@ int ofea_nl_assemble(ofea_ctx *c, double E, double nu, const double *u, double *
     return nl_assemble(c, E, nu, u, nz_val, f_int);
 }
 
+int ofea_learned_nparam(void) { return learned_nparam(); }
+
+int ofea_learned_init(ofea_ctx *c, unsigned long long seed)
+{
+    CHECK_CTX(c);
+    return learned_init(c, seed);
+}
+
+int ofea_learned_set_params(ofea_ctx *c, const float *params, int nparam)
+{
+    CHECK_CTX(c);
+    if (!params) return fail(OFEA_ERR_ARG, "null params");
+    return learned_set_params(c, params, nparam);
+}
+
+int ofea_learned_get_params(ofea_ctx *c, float *params, int nparam)
+{
+    CHECK_CTX(c);
+    if (!params) return fail(OFEA_ERR_ARG, "null params");
+    return learned_get_params(c, params, nparam);
+}
+
+int ofea_learned_set_samples(ofea_ctx *c, const double *E, const double *R, int S)
+{
+    CHECK_CTX(c);
+    if (!E || !R || S <= 0) return fail(OFEA_ERR_ARG, "bad samples");
+    return learned_set_samples(c, E, R, S);
+}
+
+int ofea_learned_loss_grad(ofea_ctx *c, double *loss, float *grad)
+{
+    CHECK_CTX(c);
+    return learned_loss_grad(c, loss, grad);
+}
+
+int ofea_learned_train(ofea_ctx *c, int steps, float lr, double *loss_hist)
+{
+    CHECK_CTX(c);
+    if (steps < 0 || !(lr > 0.0f)) return fail(OFEA_ERR_ARG, "steps >= 0 and lr > 0 required");
+    return learned_train(c, steps, lr, loss_hist);
+}
+
+int ofea_learned_blocks(ofea_ctx *c, double *blocks, float *features)
+{
+    CHECK_CTX(c);
+    return learned_blocks(c, blocks, features);
+}
+
+int ofea_learned_refresh(ofea_ctx *c)
+{
+    CHECK_CTX(c);
+    return learned_refresh(c);
+}
+
 int ofea_timer_start(ofea_ctx *c)
 {
     CHECK_CTX(c);
@@ -290,6 +346,13 @@ int ofea_timer_stop(ofea_ctx *c, float *ms)
     OFEA_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
     if (ms) *ms = t;
     return OFEA_OK;
+}
+
+int ofea_stream_copy(ofea_ctx *c, long long bytes, int reps, float *ms)
+{
+    CHECK_CTX(c);
+    if (!ms) return fail(OFEA_ERR_ARG, "null ms");
+    return stream_copy(c, bytes, reps, ms);
 }
 
 int ofea_sync(ofea_ctx *c)

@@ -360,6 +360,7 @@ int nl_assemble(ofea_ctx *c, double E, double nu, const double *u, double *nz_va
         OFEA_HIP(hipGetLastError());
     }
     c->assembled = true;
+    c->kver++;
     solver_invalidate(c);
     if (nz_val) OFEA_HIP(hipMemcpyAsync(nz_val, c->d_val, sizeof(double) * c->P.nnz, hipMemcpyDeviceToHost, c->stream));
     if (f_int) OFEA_HIP(hipMemcpyAsync(f_int, c->d_fint, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));

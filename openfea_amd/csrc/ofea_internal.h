@@ -105,6 +105,7 @@ struct ofea_ctx {
     double *d_val = nullptr;    // FP64 values, nnz
     ofea::DevPlan dp[2];
     bool assembled = false;
+    int kver = 0;  // incremented by every assembly of K
     double asm_E = 0.0, asm_nu = 0.0;
     // solver
     ofea_solver_ws *ws = nullptr;
@@ -126,11 +127,23 @@ int solver_spmv(ofea_ctx *ctx, const double *d_x, double *d_y);
 int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, double *relres);
 void solver_free(ofea_ctx *ctx);
 void solver_invalidate(ofea_ctx *ctx);  // K changed: preconditioner and FP32 copy are stale
+int stream_copy(ofea_ctx *ctx, long long bytes, int reps, float *ms);
 // explicit (explicit.hip)
 int explicit_run(ofea_ctx *ctx, double E, double nu, const double *mass, const double *f_ext, const double *v0,
                  const int *fixed_mask, double dt, int nsteps, int precision, double *u_out);
 int lumped_mass(ofea_ctx *ctx, double rho, double *mass);
 void explicit_free(ofea_ctx *ctx);
+// learned preconditioner (learned.hip)
+void learned_free(ofea_ctx *ctx);
+int learned_nparam();
+int learned_init(ofea_ctx *c, unsigned long long seed);
+int learned_set_params(ofea_ctx *c, const float *params, int nparam);
+int learned_get_params(ofea_ctx *c, float *params, int nparam);
+int learned_set_samples(ofea_ctx *c, const double *E, const double *R, int S);
+int learned_loss_grad(ofea_ctx *c, double *loss, float *grad);
+int learned_train(ofea_ctx *c, int steps, float lr, double *loss_hist);
+int learned_blocks(ofea_ctx *c, double *blocks, float *features);
+int learned_refresh(ofea_ctx *c);
 // nonlinear (nl.hip)
 int nl_assemble(ofea_ctx *ctx, double E, double nu, const double *u, double *nz_val, double *f_int);
 

@@ -8,65 +8,31 @@
 // row_idx: 8.44 bytes per stored value in FP64, 4.44 in FP32.  Because K is
 // symmetric this product is K x = K^T x.
 //
-// PCG iteration = 3 kernels (SpMV + <p,q>, update + <r,z>,<r,r>, direction),
-// scalars live on the device and are finalised by the last workgroup of each
-// reduction (fixed-order sum => deterministic).  Batches of iterations are
-// captured in a hipGraph; kernels return at once after convergence, and the
-// host reads the flag between graph launches.
+// PCG iteration = 2 kernels: (p = z + beta p folded into the SpMV, q = K p,
+// <p,q>) and (x += alpha p, r -= alpha q, z = M r, <r,z>, <r,r>).  Both run on
+// a fixed grid; each workgroup reduces the previous kernel's per-workgroup
+// partials itself in a fixed order (deterministic, no atomics).  Batches of
+// iterations are captured in a hipGraph; kernels return at once after
+// convergence and the host reads the scalars between graph launches.
+//
+// Precision: the plain solvers run CG in FP64.  The mixed solver runs one
+// continuous CG in FP32 (FP32 values, FP32 vectors) while accumulating x in
+// FP64, and replaces its recursive residual by the true FP64 residual
+// b - K64 x whenever the residual has dropped by 10x since the last
+// replacement ("reliable updates"); convergence is only accepted on the true
+// FP64 residual.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
+#include "elem_device.h"
 #include "ofea_internal.h"
+#include "solver_ws.h"
 
 namespace ofea {
 
 constexpr int RED_WG = 256;   // threads per workgroup of vector kernels
 constexpr int CG_BATCH = 16;  // iterations per graph launch
-
-enum Scalar { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL2, S_NSCAL };
-
-struct DevScalars {
-    double s[S_NSCAL];
-    int done;  // 0 running, 1 converged, 2 maxit, 3 breakdown
-    int it;
-    int maxit;
-    unsigned int ticket;
-};
-
-template <class T>
-struct VecSet {
-    T *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *x = nullptr;
-};
-
-}  // namespace ofea
-
-struct ofea_solver_ws {
-    int n = 0, nfree = 0;
-    int spmv_group = 16;        // lanes per node row group (autotuned)
-    int spmv_nodes_per_wg = 16; // = 256 / group
-    int nblk_spmv = 0, nblk_vec = 0, nblk_node = 0;
-    double *partials = nullptr;  // 3 * max blocks
-    ofea::DevScalars *sc = nullptr;
-    ofea::DevScalars *sc_host = nullptr;  // pinned
-    ofea::VecSet<double> v64;
-    ofea::VecSet<float> v32;
-    double *r64 = nullptr;    // true residual for refinement
-    double *dinv = nullptr;   // point Jacobi, n
-    double *minv = nullptr;   // 3x3 block inverses, 9 * nfree
-    float *minv32 = nullptr;  // FP32 copy of the preconditioner blocks
-    float *val32 = nullptr;   // FP32 values, nnz
-    bool val32_valid = false, precond_valid = false;
-    hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
-    hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
-};
-
-namespace ofea {
-
-__device__ inline int xcd_remap_s(int i, int n)
-{
-    int q = n >> 3, rem = n & 7, x = i & 7, j = i >> 3;
-    return x < rem ? x * (q + 1) + j : rem * (q + 1) + (x - rem) * q + j;
-}
 
 template <class T>
 __device__ inline T wave_group_sum(T v, int group)
@@ -75,7 +41,7 @@ __device__ inline T wave_group_sum(T v, int group)
     return v;
 }
 
-// Block-wide sum of NV doubles; result valid in thread 0.
+// Block-wide sum of NV doubles in a fixed tree; result valid in thread 0.
 template <int NV>
 __device__ inline void block_sum(double v[NV])
 {
@@ -85,7 +51,8 @@ __device__ inline void block_sum(double v[NV])
         for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
         v[i] = t;
     }
-    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
     if (l == 0)
         for (int i = 0; i < NV; ++i) sm[i][w] = v[i];
     __syncthreads();
@@ -97,38 +64,36 @@ __device__ inline void block_sum(double v[NV])
         }
 }
 
-// Writes the block partials; returns true in thread 0 of the last block to
-// finish, which then sees every partial.
+// Grid-wide sum of NV doubles: every block publishes its partial, the last
+// block to arrive sums all partials (all its threads, strided, then a fixed
+// tree).  Returns true in thread 0 of that last block, with out[] the totals.
 template <int NV>
-__device__ inline bool publish_partials(double v[NV], double *partials, DevScalars *sc)
+__device__ inline bool grid_sum(double v[NV], double *partials, DevScalars *sc, double out[NV])
 {
-    __shared__ bool last;
+    __shared__ int s_last;
     block_sum<NV>(v);
     if (threadIdx.x == 0) {
         for (int i = 0; i < NV; ++i) partials[i * gridDim.x + blockIdx.x] = v[i];
         __threadfence();
         unsigned int t = atomicAdd(&sc->ticket, 1u);
-        last = (t == gridDim.x - 1);
+        s_last = (t == gridDim.x - 1);
     }
     __syncthreads();
-    return last;
-}
-
-template <int NV>
-__device__ inline void final_sum(const double *partials, int nb, double out[NV])
-{
-    // thread 0 of the last block, fixed order; the partials of other CUs are
-    // read past this CU's L1 (agent-scope acquire, then agent-scope loads)
+    if (!s_last) return false;  // uniform over the block
     __threadfence();
+    const int nb = gridDim.x;
     for (int i = 0; i < NV; ++i) {
         double t = 0.0;
-        for (int b = 0; b < nb; ++b)
+        for (int b = threadIdx.x; b < nb; b += RED_WG)
             t += __hip_atomic_load(&partials[i * nb + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         out[i] = t;
     }
+    block_sum<NV>(out);
+    if (threadIdx.x == 0) sc->ticket = 0;
+    return threadIdx.x == 0;
 }
 
-// y = K x (+ optional <x, y> partial for PCG).  One group of G lanes per free node.
+// y = K x (+ optional <x, y> for PCG).  One group of G lanes per free node.
 template <class TV, class T, int G, bool DOT>
 __global__ __launch_bounds__(RED_WG) void k_spmv(int nfree, const int *__restrict__ node_blk,
                                                  const int *__restrict__ bcol, const TV *__restrict__ val,
@@ -137,7 +102,7 @@ __global__ __launch_bounds__(RED_WG) void k_spmv(int nfree, const int *__restric
 {
     if (DOT && sc->done) return;
     constexpr int NPW = RED_WG / G;
-    const int blk = xcd_remap_s(blockIdx.x, gridDim.x);
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int r = blk * NPW + (int)threadIdx.x / G;
     const int lane = threadIdx.x % G;
     double dot = 0.0;
@@ -164,19 +129,157 @@ __global__ __launch_bounds__(RED_WG) void k_spmv(int nfree, const int *__restric
         }
     }
     if (DOT) {
-        double v[1] = {dot};
-        if (publish_partials<1>(v, partials, sc)) {
-            double t[1];
-            final_sum<1>(partials, gridDim.x, t);
+        double v[1] = {dot}, t[1];
+        if (grid_sum<1>(v, partials, sc, t)) {
             sc->s[S_PQ] = t[0];
-            if (!(t[0] > 0.0) || !isfinite(t[0])) sc->done = 3;  // breakdown: K p not positive
+            if (!(t[0] > 0.0) || !isfinite(t[0])) sc->done = 3;  // breakdown: K or M not SPD
             else sc->s[S_ALPHA] = sc->s[S_RZ] / t[0];
-            sc->ticket = 0;
         }
     }
 }
 
-// preconditioner application on node r: z_r = M_r r_r
+// ---------------------------------------------------------------- CG kernels
+// The CG kernels run on a fixed grid of CG_NB workgroups of CG_WG threads
+// (two 16-wave workgroups per CU), each owning a contiguous node range.  A
+// reduction writes one partial per workgroup; every workgroup of the next
+// kernel sums the CG_NB partials itself, in the same fixed order, so all of
+// them derive identical scalars without atomics or a last-block pass.  The
+// batch index bi (kernel argument) and the host-set base values it0/par0 give
+// the iteration number and the parity of the ping-pong buffers.
+constexpr int CG_NB = 512;
+constexpr int CG_WG = 1024;
+
+struct CGParts {
+    double *rz;  // 2 * CG_NB: <r,z> of the current and the previous iteration (by parity)
+    double *rr;  // CG_NB: <r,r>
+    double *pq;  // CG_NB: <p,Kp>
+};
+
+// Sum over the CG_WG threads of a workgroup (fixed tree), result in every thread.
+template <int NV>
+__device__ inline void cg_block_sum(double v[NV])
+{
+    __shared__ double sm[NV][CG_WG / 64];
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+        for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) sm[i][threadIdx.x >> 6] = v[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double t = 0.0;
+        for (int k = 0; k < CG_WG / 64; ++k) t += sm[i][k];
+        v[i] = t;
+    }
+}
+
+// Totals of NV partial arrays (CG_NB entries each), same value everywhere.
+template <int NV>
+__device__ inline void red_sums(const double *const arr[NV], double out[NV])
+{
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+        out[i] = threadIdx.x < CG_NB
+                     ? __hip_atomic_load(&arr[i][threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                     : 0.0;
+    cg_block_sum<NV>(out);
+}
+
+// One partial per workgroup and array (thread 0 writes).
+template <int NV>
+__device__ inline void put_partials(double v[NV], double *const dst[NV])
+{
+    cg_block_sum<NV>(v);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) dst[i][blockIdx.x] = v[i];
+}
+
+// contiguous range [i0, i1) of n items for this workgroup; consecutive ranges
+// go to the same XCD
+__device__ inline void cg_range(int n, int &i0, int &i1)
+{
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
+    i0 = min(b * per, n);
+    i1 = min(i0 + per, n);
+}
+
+// Iteration start: convergence / reliable-update / maxit checks on the
+// residual of the previous update, then p_new = z + beta p_old folded into
+// q = K p_new, and the partial <p_new, q>.
+template <class TV, class T, int G>
+__global__ __launch_bounds__(CG_WG) void k_cg_spmv(int bi, int nfree, const int *__restrict__ node_blk,
+                                                   const int *__restrict__ bcol, const TV *__restrict__ val,
+                                                   const T *__restrict__ z, T *__restrict__ pA, T *__restrict__ pB,
+                                                   T *__restrict__ q, CGParts P, DevScalars *sc)
+{
+    if (sc->done) return;
+    const int par = sc->par0 ^ (bi & 1);
+    double red[3];
+    {
+        const double *const arr[3] = {P.rz + par * CG_NB, P.rz + (par ^ 1) * CG_NB, P.rr};
+        red_sums<3>(arr, red);
+    }
+    const double rz = red[0], rzp = red[1], rr = red[2];
+    const int it = sc->it0 + bi;
+    int dn = 0;
+    if (!isfinite(rz) || !isfinite(rr)) dn = 3;
+    else if (rr <= sc->s[S_TOL2]) dn = 1;
+    else if (rr <= sc->s[S_RU2]) dn = 4;
+    else if (it >= sc->maxit) dn = 2;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->it = it;
+        sc->s[S_RR] = rr;
+        sc->s[S_RZ] = rz;
+        if (dn) {
+            sc->done = dn;
+            sc->bi_done = bi;
+        }
+    }
+    if (dn) return;
+    const double beta = rz / rzp;  // rzp = inf at a start: beta = 0
+    const T *p_old = par ? pB : pA;
+    T *p_new = par ? pA : pB;
+    constexpr int NPW = CG_WG / G;
+    const int lane = threadIdx.x % G;
+    int r0, r1;
+    cg_range(nfree, r0, r1);
+    double dot = 0.0;
+    for (int r = r0 + (int)threadIdx.x / G; r < r1; r += NPW) {
+        const int nb0 = node_blk[r], deg = node_blk[r + 1] - nb0;
+        const int len = 3 * deg;
+        const TV *v0 = val + 9 * (long long)nb0;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int j = lane; j < len; j += G) {
+            const int kk = j / 3, d = j - 3 * kk;
+            const int s = bcol[nb0 + kk];
+            const double pv = (double)(T)((double)z[3 * s + d] + beta * (double)p_old[3 * s + d]);
+            a0 += (double)v0[j] * pv;
+            a1 += (double)v0[len + j] * pv;
+            a2 += (double)v0[2 * len + j] * pv;
+        }
+        a0 = wave_group_sum(a0, G);
+        a1 = wave_group_sum(a1, G);
+        a2 = wave_group_sum(a2, G);
+        if (lane < 3) {
+            const double a = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
+            const int k = 3 * r + lane;
+            const T pn = (T)((double)z[k] + beta * (double)p_old[k]);
+            p_new[k] = pn;
+            q[k] = (T)a;
+            dot += a * (double)pn;
+        }
+    }
+    double part[1] = {dot};
+    double *const dst[1] = {P.pq};
+    put_partials<1>(part, dst);
+}
+
+// preconditioner application on node r: z_r = M_r r_r (point or 3x3 block)
 template <class T, int PC>
 __device__ inline void apply_pc(int r, const T *rr, T *zz, const double *dinv, const T *minv)
 {
@@ -188,96 +291,111 @@ __device__ inline void apply_pc(int r, const T *rr, T *zz, const double *dinv, c
     }
 }
 
-// x += alpha p; r -= alpha q; z = M r; partial <r,z>, <r,r>.  Thread per node.
-template <class T, int PC>
-__global__ __launch_bounds__(RED_WG) void k_update(int nfree, T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
-                                                   const T *__restrict__ p, const T *__restrict__ q,
-                                                   const double *__restrict__ dinv, const T *__restrict__ minv,
-                                                   double *partials, DevScalars *sc)
+// alpha = <r,z>/<p,q>; x += alpha p (x in XT); r -= alpha q; z = M r;
+// partials <r,z> (into the other parity slot) and <r,r>.  Thread per node.
+template <class T, class XT, int PC>
+__global__ __launch_bounds__(CG_WG) void k_cg_update(int bi, int nfree, XT *__restrict__ x, T *__restrict__ r,
+                                                     T *__restrict__ z, const T *__restrict__ pA,
+                                                     const T *__restrict__ pB, const T *__restrict__ q,
+                                                     const double *__restrict__ dinv, const T *__restrict__ minv,
+                                                     CGParts P, DevScalars *sc)
 {
     if (sc->done) return;
-    const double alpha = sc->s[S_ALPHA];
-    const int i = blockIdx.x * RED_WG + threadIdx.x;
-    double rz = 0.0, rr2 = 0.0;
-    if (i < nfree) {
-        T rv[3], zv[3];
+    const int par = sc->par0 ^ (bi & 1);
+    const T *p = par ? pA : pB;  // p_new of this iteration
+    int i0, i1;
+    cg_range(nfree, i0, i1);
+    // the node data do not depend on alpha: issue the loads before the reduction
+    int i = i0 + (int)threadIdx.x;
+    T pv[3] = {T(0), T(0), T(0)}, qv[3] = {T(0), T(0), T(0)}, rv[3] = {T(0), T(0), T(0)};
+    if (i < i1)
         for (int c = 0; c < 3; ++c) {
-            int k = 3 * i + c;
-            x[k] = (T)((double)x[k] + alpha * (double)p[k]);
-            rv[c] = (T)((double)r[k] - alpha * (double)q[k]);
+            pv[c] = p[3 * i + c];
+            qv[c] = q[3 * i + c];
+            rv[c] = r[3 * i + c];
+        }
+    double red[2];
+    {
+        const double *const arr[2] = {P.pq, P.rz + par * CG_NB};
+        red_sums<2>(arr, red);
+    }
+    const double pq = red[0], rz = red[1];
+    if (!(pq > 0.0) || !isfinite(pq)) {  // breakdown: K or M not SPD
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sc->done = 3;
+            sc->bi_done = bi;
+        }
+        return;
+    }
+    const double alpha = rz / pq;
+    double rzn = 0.0, rr2 = 0.0;
+    for (; i < i1; i += CG_WG) {
+        if (i >= i0 + CG_WG)  // later passes (ranges longer than one workgroup) load here
+            for (int c = 0; c < 3; ++c) {
+                pv[c] = p[3 * i + c];
+                qv[c] = q[3 * i + c];
+                rv[c] = r[3 * i + c];
+            }
+        T zv[3];
+        for (int c = 0; c < 3; ++c) {
+            const int k = 3 * i + c;
+            x[k] = (XT)((double)x[k] + alpha * (double)pv[c]);
+            rv[c] = (T)((double)rv[c] - alpha * (double)qv[c]);
             r[k] = rv[c];
         }
         apply_pc<T, PC>(i, rv, zv, dinv, minv);
         for (int c = 0; c < 3; ++c) {
             z[3 * i + c] = zv[c];
-            rz += (double)rv[c] * (double)zv[c];
+            rzn += (double)rv[c] * (double)zv[c];
             rr2 += (double)rv[c] * (double)rv[c];
         }
     }
-    double v[2] = {rz, rr2};
-    if (publish_partials<2>(v, partials, sc)) {
-        double t[2];
-        final_sum<2>(partials, gridDim.x, t);
-        double rz_old = sc->s[S_RZ];
-        sc->s[S_BETA] = t[0] / rz_old;
-        sc->s[S_RZ] = t[0];
-        sc->s[S_RR] = t[1];
-        int it = sc->it + 1;
-        sc->it = it;
-        if (t[1] <= sc->s[S_TOL2]) sc->done = 1;
-        else if (it >= sc->maxit) sc->done = 2;
-        else if (!isfinite(t[0])) sc->done = 3;
-        sc->ticket = 0;
-    }
+    double part[2] = {rzn, rr2};
+    double *const dst[2] = {P.rz + (par ^ 1) * CG_NB, P.rr};
+    put_partials<2>(part, dst);
 }
 
-template <class T>
-__global__ __launch_bounds__(RED_WG) void k_direction(int n, T *__restrict__ p, const T *__restrict__ z, DevScalars *sc)
+// advances the host-visible base iteration after a full batch
+__global__ void k_cg_batch_end(DevScalars *sc, int nbatch)
 {
-    if (sc->done) return;
-    const double beta = sc->s[S_BETA];
-    int i = blockIdx.x * RED_WG + threadIdx.x;
-    if (i < n) p[i] = (T)((double)z[i] + beta * (double)p[i]);
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !sc->done) sc->it0 += nbatch;
 }
 
-// Start of a PCG run on residual r (already set): z = M r, p = z, <r,z>, <r,r>.
+// Residual replacement / start: r = (T) r64, z = M r; partials <r,z> into
+// the slot of parity par0 and <r,r>.  At a start (restart != 0) the other
+// slot is set to +inf, so the next iteration takes beta = 0 (p = z).  At a
+// reliable update the other slot keeps <r,z> of the previous iteration, so
+// beta = <r,z>_new / <r,z>_prev and the search direction is kept.
 template <class T, int PC>
-__global__ __launch_bounds__(RED_WG) void k_cg_start(int nfree, const T *__restrict__ r, T *__restrict__ z,
-                                                     T *__restrict__ p, const double *__restrict__ dinv,
-                                                     const T *__restrict__ minv, double *partials, DevScalars *sc,
-                                                     double tol2_rel, int maxit)
+__global__ __launch_bounds__(CG_WG) void k_replace(int nfree, const double *__restrict__ r64, T *__restrict__ r,
+                                                   T *__restrict__ z, int restart, const double *__restrict__ dinv,
+                                                   const T *__restrict__ minv, CGParts P, DevScalars *sc)
 {
-    const int i = blockIdx.x * RED_WG + threadIdx.x;
+    const int par = sc->par0;
+    int i0, i1;
+    cg_range(nfree, i0, i1);
     double rz = 0.0, rr2 = 0.0;
-    if (i < nfree) {
-        T rv[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]}, zv[3];
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += CG_WG) {
+        T rv[3] = {(T)r64[3 * i], (T)r64[3 * i + 1], (T)r64[3 * i + 2]}, zv[3];
         apply_pc<T, PC>(i, rv, zv, dinv, minv);
         for (int c = 0; c < 3; ++c) {
+            r[3 * i + c] = rv[c];
             z[3 * i + c] = zv[c];
-            p[3 * i + c] = zv[c];
             rz += (double)rv[c] * (double)zv[c];
             rr2 += (double)rv[c] * (double)rv[c];
         }
     }
-    double v[2] = {rz, rr2};
-    if (publish_partials<2>(v, partials, sc)) {
-        double t[2];
-        final_sum<2>(partials, gridDim.x, t);
-        sc->s[S_RZ] = t[0];
-        sc->s[S_RR] = t[1];
-        sc->s[S_TOL2] = tol2_rel * t[1];
-        sc->it = 0;
-        sc->maxit = maxit;
-        sc->done = (t[1] == 0.0) ? 1 : 0;
-        sc->ticket = 0;
-    }
+    double part[2] = {rz, rr2};
+    double *const dst[2] = {P.rz + par * CG_NB, P.rr};
+    put_partials<2>(part, dst);
+    if (restart && threadIdx.x == 0) P.rz[(par ^ 1) * CG_NB + blockIdx.x] = blockIdx.x == 0 ? INFINITY : 0.0;
 }
 
-// r = b - y (FP64) and <r,r>, <b,b> into s[S_RR], s[S_BB]
+// r64 = b - y, <r64,r64> -> S_TRR, <b,b> -> S_BB
 __global__ __launch_bounds__(RED_WG) void k_residual(int n, const double *__restrict__ b, const double *__restrict__ y,
                                                      double *__restrict__ r, double *partials, DevScalars *sc)
 {
-    int i = blockIdx.x * RED_WG + threadIdx.x;
+    const int i = blockIdx.x * RED_WG + threadIdx.x;
     double rr = 0.0, bb = 0.0;
     if (i < n) {
         double v = b[i] - y[i];
@@ -285,27 +403,18 @@ __global__ __launch_bounds__(RED_WG) void k_residual(int n, const double *__rest
         rr = v * v;
         bb = b[i] * b[i];
     }
-    double v[2] = {rr, bb};
-    if (publish_partials<2>(v, partials, sc)) {
-        double t[2];
-        final_sum<2>(partials, gridDim.x, t);
-        sc->s[S_RR] = t[0];
+    double v[2] = {rr, bb}, t[2];
+    if (grid_sum<2>(v, partials, sc, t)) {
+        sc->s[S_TRR] = t[0];
         sc->s[S_BB] = t[1];
-        sc->ticket = 0;
     }
 }
 
 template <class A, class B>
-__global__ void k_convert(int n, const A *__restrict__ a, B *__restrict__ b)
+__global__ void k_convert(long long n, const A *__restrict__ a, B *__restrict__ b)
 {
-    int i = blockIdx.x * RED_WG + threadIdx.x;
+    long long i = (long long)blockIdx.x * RED_WG + threadIdx.x;
     if (i < n) b[i] = (B)a[i];
-}
-
-__global__ void k_add_to(int n, double *__restrict__ x, const float *__restrict__ e)
-{
-    int i = blockIdx.x * RED_WG + threadIdx.x;
-    if (i < n) x[i] += (double)e[i];
 }
 
 template <class T>
@@ -319,7 +428,7 @@ __global__ void k_fill(int n, T *__restrict__ a, T v)
 // each free node's 3x3 diagonal block.  Thread per free node.
 __global__ void k_precond_setup(int nfree, const int *__restrict__ node_blk, const int *__restrict__ bcol,
                                 const double *__restrict__ val, double *__restrict__ dinv, double *__restrict__ minv,
-                                float *__restrict__ minv32)
+                                float *__restrict__ minv32, double *__restrict__ dblk)
 {
     int r = blockIdx.x * RED_WG + threadIdx.x;
     if (r >= nfree) return;
@@ -330,12 +439,10 @@ __global__ void k_precond_setup(int nfree, const int *__restrict__ node_blk, con
     for (int c = 0; c < 3; ++c)
         for (int d = 0; d < 3; ++d) A[3 * c + d] = k < deg ? val[9 * (long long)nb0 + c * 3 * deg + 3 * k + d] : 0.0;
     for (int c = 0; c < 3; ++c) dinv[3 * r + c] = 1.0 / A[4 * c];
-    // block stored as [col c][row d]; K is symmetric so it is its own transpose
-    double c00 = A[4] * A[8] - A[5] * A[7], c01 = A[5] * A[6] - A[3] * A[8], c02 = A[3] * A[7] - A[4] * A[6];
-    double id = 1.0 / (A[0] * c00 + A[1] * c01 + A[2] * c02);
-    double M[9] = {c00 * id, (A[2] * A[7] - A[1] * A[8]) * id, (A[1] * A[5] - A[2] * A[4]) * id,
-                   c01 * id, (A[0] * A[8] - A[2] * A[6]) * id, (A[2] * A[3] - A[0] * A[5]) * id,
-                   c02 * id, (A[1] * A[6] - A[0] * A[7]) * id, (A[0] * A[4] - A[1] * A[3]) * id};
+    for (int i = 0; i < 9; ++i) dblk[9 * r + i] = A[i];
+    // the block is stored [col c][row d]; K is symmetric, so it is its own transpose
+    double M[9];
+    inv3(A, M);
     for (int i = 0; i < 9; ++i) {
         minv[9 * r + i] = M[i];
         minv32[9 * r + i] = (float)M[i];
@@ -346,30 +453,33 @@ __global__ void k_precond_setup(int nfree, const int *__restrict__ node_blk, con
 static int ensure_ws(ofea_ctx *ctx)
 {
     ofea_solver_ws *&w = ctx->ws;
-    if (w && w->n == ctx->P.n) return OFEA_OK;
+    if (w && w->n == ctx->P.n && w->nnz == ctx->P.nnz) return OFEA_OK;
     solver_free(ctx);
     w = new ofea_solver_ws();
     w->n = ctx->P.n;
     w->nfree = ctx->P.nfree;
-    int n = w->n, nf = w->nfree;
+    w->nnz = ctx->P.nnz;
+    const int n = w->n, nf = w->nfree;
     w->nblk_vec = (n + RED_WG - 1) / RED_WG;
     w->nblk_node = (nf + RED_WG - 1) / RED_WG;
-    int maxb = (nf + 1) / 1 + 64;  // spmv with group 1 would be the most blocks; bounded below
-    OFEA_HIP(hipMalloc((void **)&w->partials, sizeof(double) * 3 * (size_t)maxb));
+    const size_t maxb = (size_t)nf + 64;  // SpMV with 4 lanes per node has the largest grid
+    OFEA_HIP(hipMalloc((void **)&w->partials, sizeof(double) * 2 * maxb));
+    OFEA_HIP(hipMalloc((void **)&w->part_rz, sizeof(double) * 2 * CG_NB));
+    OFEA_HIP(hipMalloc((void **)&w->part_rr, sizeof(double) * CG_NB));
+    OFEA_HIP(hipMalloc((void **)&w->part_pq, sizeof(double) * CG_NB));
     OFEA_HIP(hipMalloc((void **)&w->sc, sizeof(DevScalars)));
     OFEA_HIP(hipMemset(w->sc, 0, sizeof(DevScalars)));
     OFEA_HIP(hipHostMalloc((void **)&w->sc_host, sizeof(DevScalars), hipHostMallocDefault));
-    size_t nb = (size_t)std::max(n, 1);
-    for (double **p : {&w->v64.r, &w->v64.z, &w->v64.p, &w->v64.q, &w->r64})
+    const size_t nb = (size_t)std::max(n, 1), nfb = (size_t)std::max(nf, 1);
+    for (double **p : {&w->v64.r, &w->v64.z, &w->v64.p, &w->v64.p2, &w->v64.q, &w->r64, &w->y64})
         OFEA_HIP(hipMalloc((void **)p, sizeof(double) * nb));
-    for (float **p : {&w->v32.r, &w->v32.z, &w->v32.p, &w->v32.q, &w->v32.x})
+    for (float **p : {&w->v32.r, &w->v32.z, &w->v32.p, &w->v32.p2, &w->v32.q})
         OFEA_HIP(hipMalloc((void **)p, sizeof(float) * nb));
     OFEA_HIP(hipMalloc((void **)&w->dinv, sizeof(double) * nb));
-    OFEA_HIP(hipMalloc((void **)&w->minv, sizeof(double) * 9 * (size_t)std::max(nf, 1)));
-    OFEA_HIP(hipMalloc((void **)&w->minv32, sizeof(float) * 9 * (size_t)std::max(nf, 1)));
+    OFEA_HIP(hipMalloc((void **)&w->minv, sizeof(double) * 9 * nfb));
+    OFEA_HIP(hipMalloc((void **)&w->minv32, sizeof(float) * 9 * nfb));
+    OFEA_HIP(hipMalloc((void **)&w->dblk, sizeof(double) * 9 * nfb));
     OFEA_HIP(hipMalloc((void **)&w->val32, sizeof(float) * (size_t)std::max(ctx->P.nnz, 1LL)));
-    w->spmv_nodes_per_wg = RED_WG / w->spmv_group;
-    w->nblk_spmv = (nf + w->spmv_nodes_per_wg - 1) / w->spmv_nodes_per_wg;
     return OFEA_OK;
 }
 
@@ -377,16 +487,18 @@ void solver_free(ofea_ctx *ctx)
 {
     ofea_solver_ws *w = ctx->ws;
     if (!w) return;
-    for (int i = 0; i < 4; ++i) {
-        if (w->exec[i]) hipGraphExecDestroy(w->exec[i]);
-        if (w->graph[i]) hipGraphDestroy(w->graph[i]);
+    for (int i = 0; i < NSLOT; ++i) {
+        if (w->exec[i]) (void)hipGraphExecDestroy(w->exec[i]);
+        if (w->graph[i]) (void)hipGraphDestroy(w->graph[i]);
     }
-    for (double *p : {w->partials, w->v64.r, w->v64.z, w->v64.p, w->v64.q, w->r64, w->dinv, w->minv})
-        if (p) hipFree(p);
-    for (float *p : {w->v32.r, w->v32.z, w->v32.p, w->v32.q, w->v32.x, w->minv32, w->val32})
-        if (p) hipFree(p);
-    if (w->sc) hipFree(w->sc);
-    if (w->sc_host) hipHostFree(w->sc_host);
+    for (double *p : {w->part_rz, w->part_rr, w->part_pq, w->partials, w->v64.r, w->v64.z, w->v64.p, w->v64.p2, w->v64.q, w->r64, w->y64, w->dinv, w->minv,
+                      w->dblk})
+        if (p) (void)hipFree(p);
+    for (float *p : {w->v32.r, w->v32.z, w->v32.p, w->v32.p2, w->v32.q, w->minv32, w->val32, w->learned32})
+        if (p) (void)hipFree(p);
+    if (w->learned64) (void)hipFree(w->learned64);
+    if (w->sc) (void)hipFree(w->sc);
+    if (w->sc_host) (void)hipHostFree(w->sc_host);
     delete w;
     ctx->ws = nullptr;
 }
@@ -428,147 +540,225 @@ int solver_spmv(ofea_ctx *ctx, const double *d_x, double *d_y)
     return OFEA_OK;
 }
 
-static int precond_setup(ofea_ctx *ctx)
+int solver_spmv32(ofea_ctx *ctx, const float *d_x, float *d_y)
 {
+    launch_spmv_t<float, float, false>(ctx, ctx->ws->spmv_group, ctx->ws->val32, d_x, d_y);
+    OFEA_HIP(hipGetLastError());
+    return OFEA_OK;
+}
+
+int solver_prepare(ofea_ctx *ctx)
+{
+    int rc = ensure_ws(ctx);
+    if (rc) return rc;
     ofea_solver_ws *w = ctx->ws;
     if (w->precond_valid) return OFEA_OK;
     if (w->nblk_node)
         hipLaunchKernelGGL(k_precond_setup, dim3(w->nblk_node), dim3(RED_WG), 0, ctx->stream, w->nfree,
-                           ctx->d_node_blk, ctx->d_bcol, ctx->d_val, w->dinv, w->minv, w->minv32);
-    long long nnz = ctx->P.nnz;
+                           ctx->d_node_blk, ctx->d_bcol, ctx->d_val, w->dinv, w->minv, w->minv32, w->dblk);
+    const long long nnz = ctx->P.nnz;
     if (nnz)
         hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((nnz + RED_WG - 1) / RED_WG)), dim3(RED_WG), 0,
-                           ctx->stream, (int)nnz, ctx->d_val, w->val32);
+                           ctx->stream, nnz, ctx->d_val, w->val32);
     OFEA_HIP(hipGetLastError());
     w->precond_valid = true;
     return OFEA_OK;
 }
 
-// One CG iteration (three kernels) on precision T with preconditioner PC.
-template <class TV, class T, int PC>
-static void cg_iteration(ofea_ctx *ctx, const TV *val, VecSet<T> &v, const T *minv)
+// The graph of one batch: CG_BATCH iterations (2 kernels each) + batch end.
+template <class TV, class T, class XT, int PC>
+static void cg_batch(ofea_ctx *ctx, const TV *val, VecSet<T> &v, XT *x, const T *minv)
 {
     ofea_solver_ws *w = ctx->ws;
-    launch_spmv_t<TV, T, true>(ctx, w->spmv_group, val, v.p, v.q);
-    hipLaunchKernelGGL((k_update<T, PC>), dim3(w->nblk_node), dim3(RED_WG), 0, ctx->stream, w->nfree, v.x, v.r, v.z,
-                       v.p, v.q, w->dinv, minv, w->partials, w->sc);
-    hipLaunchKernelGGL((k_direction<T>), dim3(w->nblk_vec), dim3(RED_WG), 0, ctx->stream, w->n, v.p, v.z, w->sc);
+    CGParts P{w->part_rz, w->part_rr, w->part_pq};
+    for (int bi = 0; bi < CG_BATCH; ++bi) {
+#define OFEA_CG_CASE(G)                                                                                          \
+    case G:                                                                                                      \
+        hipLaunchKernelGGL((k_cg_spmv<TV, T, G>), dim3(CG_NB), dim3(CG_WG), 0, ctx->stream, bi, w->nfree,     \
+                           ctx->d_node_blk, ctx->d_bcol, val, v.z, v.p, v.p2, v.q, P, w->sc);                   \
+        break;
+        switch (w->spmv_group) {
+            OFEA_CG_CASE(4)
+            OFEA_CG_CASE(8)
+            OFEA_CG_CASE(16)
+            OFEA_CG_CASE(32)
+            default:
+            OFEA_CG_CASE(64)
+        }
+#undef OFEA_CG_CASE
+        hipLaunchKernelGGL((k_cg_update<T, XT, PC>), dim3(CG_NB), dim3(CG_WG), 0, ctx->stream, bi, w->nfree, x, v.r,
+                           v.z, v.p, v.p2, v.q, w->dinv, minv, P, w->sc);
+    }
+    hipLaunchKernelGGL(k_cg_batch_end, dim3(1), dim3(64), 0, ctx->stream, w->sc, CG_BATCH);
 }
 
-template <class TV, class T, int PC>
-static int run_cg(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T *minv, double tol_rel, int maxit,
-                  int *iters)
+// true FP64 residual r64 = b - K64 x; ||r64||^2 -> S_TRR, ||b||^2 -> S_BB
+static int true_residual(ofea_ctx *ctx, const double *b, const double *x)
 {
     ofea_solver_ws *w = ctx->ws;
-    // start: r already holds the residual, x the initial guess
-    hipLaunchKernelGGL((k_cg_start<T, PC>), dim3(w->nblk_node), dim3(RED_WG), 0, ctx->stream, w->nfree, v.r, v.z, v.p,
-                       w->dinv, minv, w->partials, w->sc, tol_rel * tol_rel, maxit);
-    OFEA_HIP(hipGetLastError());
-    // (re)capture the batch graph for this vector set / precision / preconditioner
-    if (!w->exec[slot]) {
-        OFEA_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < CG_BATCH; ++i) cg_iteration<TV, T, PC>(ctx, val, v, minv);
-        OFEA_HIP(hipStreamEndCapture(ctx->stream, &w->graph[slot]));
-        OFEA_HIP(hipGraphInstantiate(&w->exec[slot], w->graph[slot], nullptr, nullptr, 0));
-    }
-    for (;;) {
-        OFEA_HIP(hipGraphLaunch(w->exec[slot], ctx->stream));
-        OFEA_HIP(hipMemcpyAsync(w->sc_host, w->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
-        OFEA_HIP(hipStreamSynchronize(ctx->stream));
-        if (w->sc_host->done) break;
-    }
-    if (iters) *iters = w->sc_host->it;
-    if (w->sc_host->done == 3) return fail(OFEA_ERR_STATE, "PCG breakdown (matrix or preconditioner not SPD)");
-    return OFEA_OK;
-}
-
-// r64 = b - K x (FP64), relres = ||r|| / ||b||
-static int true_residual(ofea_ctx *ctx, const double *b, const double *x, double *relres)
-{
-    ofea_solver_ws *w = ctx->ws;
-    launch_spmv_t<double, double, false>(ctx, w->spmv_group, ctx->d_val, x, w->v64.q);
-    hipLaunchKernelGGL(k_residual, dim3(w->nblk_vec), dim3(RED_WG), 0, ctx->stream, w->n, b, w->v64.q, w->r64,
+    launch_spmv_t<double, double, false>(ctx, w->spmv_group, ctx->d_val, x, w->y64);
+    hipLaunchKernelGGL(k_residual, dim3(w->nblk_vec), dim3(RED_WG), 0, ctx->stream, w->n, b, w->y64, w->r64,
                        w->partials, w->sc);
     OFEA_HIP(hipGetLastError());
-    OFEA_HIP(hipMemcpyAsync(w->sc_host, w->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
-    OFEA_HIP(hipStreamSynchronize(ctx->stream));
-    double bb = w->sc_host->s[S_BB], rr = w->sc_host->s[S_RR];
-    *relres = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
     return OFEA_OK;
 }
 
-int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, double *relres)
+static int read_scalars(ofea_ctx *ctx)
 {
-    int rc = ensure_ws(ctx);
-    if (rc) return rc;
-    rc = precond_setup(ctx);
-    if (rc) return rc;
     ofea_solver_ws *w = ctx->ws;
-    const int n = w->n;
+    OFEA_HIP(hipMemcpyAsync(w->sc_host, w->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
+    OFEA_HIP(hipStreamSynchronize(ctx->stream));
+    return OFEA_OK;
+}
+
+static int write_scalars(ofea_ctx *ctx)
+{
+    ofea_solver_ws *w = ctx->ws;
+    OFEA_HIP(hipMemcpyAsync(w->sc, w->sc_host, sizeof(DevScalars), hipMemcpyHostToDevice, ctx->stream));
+    return OFEA_OK;
+}
+
+// PCG with reliable updates.  x (FP64, device) starts at 0.  TV/T: matrix and
+// vector precision of the iteration; PC: 0 point Jacobi, 1 block preconditioner.
+// ru_factor > 0 requests a residual replacement each time the recursive
+// residual has dropped by that factor since the last replacement.
+template <class TV, class T, int PC>
+static int pcg_run(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T *minv, double rtol, int maxit,
+                   double ru_factor, int *iters, double *relres)
+{
+    ofea_solver_ws *w = ctx->ws;
+    const double *b = ctx->d_rhs;
+    double *x = ctx->d_x;
     const unsigned gv = w->nblk_vec;
-    double *x = ctx->d_x, *b = ctx->d_rhs;
-    int total_it = 0;
-    double rel = 1.0;
-    if (n == 0) {
+    const int n = w->n;
+    CGParts P{w->part_rz, w->part_rr, w->part_pq};
+    DevScalars &S = *w->sc_host;
+    hipLaunchKernelGGL((k_fill<double>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, x, 0.0);
+    hipLaunchKernelGGL((k_fill<T>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, v.p, T(0));
+    hipLaunchKernelGGL((k_fill<T>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, v.p2, T(0));
+    int rc = true_residual(ctx, b, x);  // r64 = b and ||b||^2
+    if (rc) return rc;
+    if ((rc = read_scalars(ctx))) return rc;
+    const double bb = S.s[S_BB];
+    if (bb == 0.0) {
         if (iters) *iters = 0;
         if (relres) *relres = 0.0;
         return OFEA_OK;
     }
-    hipLaunchKernelGGL((k_fill<double>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, x, 0.0);
-    if (method == OFEA_PCG_JACOBI || method == OFEA_PCG_BLOCK_JACOBI) {
-        // FP64 PCG on r = b; restart from the true residual if the recursive
-        // residual drifted from it (at most 4 restarts)
-        hipLaunchKernelGGL((k_convert<double, double>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, b, w->v64.r);
-        w->v64.x = x;
-        double tol = rtol;
-        for (int restart = 0; restart < 5; ++restart) {
-            int it = 0;
-            int left = maxit - total_it;
-            if (left <= 0) break;
-            if (method == OFEA_PCG_JACOBI)
-                rc = run_cg<double, double, 0>(ctx, 0, ctx->d_val, w->v64, w->minv, tol, left, &it);
-            else
-                rc = run_cg<double, double, 1>(ctx, 1, ctx->d_val, w->v64, w->minv, tol, left, &it);
-            total_it += it;
-            if (rc) return rc;
-            rc = true_residual(ctx, b, x, &rel);
-            if (rc) return rc;
-            if (rel <= rtol) break;
-            // continue from the true residual with a tighter recursive target
-            hipLaunchKernelGGL((k_convert<double, double>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, w->r64,
-                               w->v64.r);
-            // the next run starts from ||r|| = rel ||b||; aim below rtol ||b||
-            tol = std::min(1.0, 0.5 * rtol / rel);
-        }
-    } else if (method == OFEA_PCG_MIXED || method == OFEA_PCG_LEARNED) {
-        if (method == OFEA_PCG_LEARNED && !ctx->learned)
-            return fail(OFEA_ERR_STATE, "learned preconditioner not set (ofea_learned_*)");
-        // FP64 defect correction around an FP32 block-Jacobi PCG
-        const float *M32 = w->minv32;
-        hipLaunchKernelGGL((k_convert<double, double>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, b, w->r64);
-        rc = true_residual(ctx, b, x, &rel);
-        if (rc) return rc;
-        const double inner_tol = 1e-3;
-        for (int outer = 0; outer < 30 && rel > rtol; ++outer) {
-            int left = maxit - total_it;
-            if (left <= 0) break;
-            hipLaunchKernelGGL((k_convert<double, float>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, w->r64, w->v32.r);
-            hipLaunchKernelGGL((k_fill<float>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, w->v32.x, 0.0f);
-            int it = 0;
-            double tol_in = std::max(inner_tol, 0.5 * rtol / rel);
-            rc = run_cg<float, float, 1>(ctx, 2, w->val32, w->v32, M32, tol_in, left, &it);
-            total_it += it;
-            if (rc) return rc;
-            hipLaunchKernelGGL(k_add_to, dim3(gv), dim3(RED_WG), 0, ctx->stream, n, x, w->v32.x);
-            rc = true_residual(ctx, b, x, &rel);
-            if (rc) return rc;
-        }
-    } else {
-        return fail(OFEA_ERR_ARG, "unknown solver method");
+    double tol2 = rtol * rtol * bb;  // absolute target on the recursive ||r||^2
+    S.par0 = 0;
+    S.it0 = 0;
+    S.it = 0;
+    S.maxit = maxit;
+    S.done = 0;
+    S.ticket = 0;
+    S.s[S_TOL2] = tol2;
+    S.s[S_RU2] = ru_factor > 0.0 ? ru_factor * ru_factor * bb : -1.0;
+    if ((rc = write_scalars(ctx))) return rc;
+    hipLaunchKernelGGL((k_replace<T, PC>), dim3(CG_NB), dim3(CG_WG), 0, ctx->stream, w->nfree, w->r64, v.r, v.z, 1,
+                       w->dinv, minv, P, w->sc);
+    OFEA_HIP(hipGetLastError());
+    if (!w->exec[slot]) {
+        OFEA_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        cg_batch<TV, T, double, PC>(ctx, val, v, x, minv);
+        OFEA_HIP(hipStreamEndCapture(ctx->stream, &w->graph[slot]));
+        OFEA_HIP(hipGraphInstantiate(&w->exec[slot], w->graph[slot], nullptr, nullptr, 0));
     }
-    if (iters) *iters = total_it;
+    double rel = 1.0;
+    int tighten = 0;
+    for (;;) {
+        OFEA_HIP(hipGraphLaunch(w->exec[slot], ctx->stream));
+        if ((rc = read_scalars(ctx))) return rc;
+        const int done = S.done;
+        if (done == 0) continue;
+        if (done == 3) return fail(OFEA_ERR_STATE, "PCG breakdown (matrix or preconditioner not SPD, or NaN)");
+        const int it_done = S.it, par_done = S.par0 ^ (S.bi_done & 1);
+        // true FP64 residual at convergence candidates, reliable-update points and maxit
+        if ((rc = true_residual(ctx, b, x))) return rc;
+        if ((rc = read_scalars(ctx))) return rc;
+        rel = std::sqrt(S.s[S_TRR] / bb);
+        if (rel <= rtol || done == 2) break;
+        if (done == 1 && ++tighten > 8) break;  // recursive and true residual disagree persistently
+        if (done == 1) tol2 *= 0.25;            // recursive residual converged but the true one did not
+        // replace the recursive residual, keep the search direction, resume at it_done
+        S.par0 = par_done;
+        S.it0 = it_done;
+        S.done = 0;
+        S.s[S_TOL2] = tol2;
+        S.s[S_RU2] = ru_factor > 0.0 ? ru_factor * ru_factor * S.s[S_TRR] : -1.0;
+        if ((rc = write_scalars(ctx))) return rc;
+        hipLaunchKernelGGL((k_replace<T, PC>), dim3(CG_NB), dim3(CG_WG), 0, ctx->stream, w->nfree, w->r64, v.r, v.z,
+                           0, w->dinv, minv, P, w->sc);
+        OFEA_HIP(hipGetLastError());
+    }
+    if (iters) *iters = S.it;
     if (relres) *relres = rel;
     return rel <= rtol ? OFEA_OK : OFEA_NOT_CONVERGED;
+}
+
+int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, double *relres)
+{
+    int rc = solver_prepare(ctx);
+    if (rc) return rc;
+    ofea_solver_ws *w = ctx->ws;
+    if (w->n == 0) {
+        if (iters) *iters = 0;
+        if (relres) *relres = 0.0;
+        return OFEA_OK;
+    }
+    switch (method) {
+    case OFEA_PCG_JACOBI:
+        return pcg_run<double, double, 0>(ctx, 0, ctx->d_val, w->v64, w->minv, rtol, maxit, -1.0, iters, relres);
+    case OFEA_PCG_BLOCK_JACOBI:
+        return pcg_run<double, double, 1>(ctx, 1, ctx->d_val, w->v64, w->minv, rtol, maxit, -1.0, iters, relres);
+    case OFEA_PCG_MIXED:
+        return pcg_run<float, double, 1>(ctx, 2, w->val32, w->v64, w->minv, rtol, maxit, 0.1, iters, relres);
+    case OFEA_PCG_LEARNED:
+        if (!w->learned64) return fail(OFEA_ERR_STATE, "learned preconditioner not set (ofea_learned_*)");
+        return pcg_run<double, double, 1>(ctx, 3, ctx->d_val, w->v64, w->learned64, rtol, maxit, -1.0, iters, relres);
+    case OFEA_PCG_LEARNED_MIXED:
+        if (!w->learned64) return fail(OFEA_ERR_STATE, "learned preconditioner not set (ofea_learned_*)");
+        return pcg_run<float, double, 1>(ctx, 5, w->val32, w->v64, w->learned64, rtol, maxit, 0.1, iters, relres);
+    case OFEA_PCG_MIXED32:
+        return pcg_run<float, float, 1>(ctx, 4, w->val32, w->v32, w->minv32, rtol, maxit, 0.1, iters, relres);
+    default:
+        return fail(OFEA_ERR_ARG, "unknown solver method");
+    }
+}
+
+// plain streaming copy (16-byte vector loads/stores), grid-stride
+__global__ __launch_bounds__(RED_WG) void k_stream_copy(long long n16, const uint4 *__restrict__ src,
+                                                        uint4 *__restrict__ dst)
+{
+    long long i = (long long)blockIdx.x * RED_WG + threadIdx.x;
+    const long long stride = (long long)gridDim.x * RED_WG;
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+int stream_copy(ofea_ctx *ctx, long long bytes, int reps, float *ms)
+{
+    if (bytes < 16 || reps < 1) return fail(OFEA_ERR_ARG, "bytes >= 16 and reps >= 1 required");
+    const long long n16 = bytes / 16;
+    uint4 *a = nullptr, *b = nullptr;
+    OFEA_HIP(hipMalloc((void **)&a, n16 * 16));
+    if (hipMalloc((void **)&b, n16 * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(OFEA_ERR_HIP, "out of device memory");
+    }
+    (void)hipMemsetAsync(a, 1, n16 * 16, ctx->stream);
+    const unsigned grid = (unsigned)std::min<long long>((n16 + RED_WG - 1) / RED_WG, 256LL * 64);
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(RED_WG), 0, ctx->stream, n16, a, b);
+    (void)hipEventRecord(ctx->ev0, ctx->stream);
+    for (int i = 0; i < reps; ++i) hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(RED_WG), 0, ctx->stream, n16, a, b);
+    (void)hipEventRecord(ctx->ev1, ctx->stream);
+    hipError_t e = hipEventSynchronize(ctx->ev1);
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(OFEA_ERR_HIP, hipGetErrorString(e));
+    *ms = t / reps;
+    return OFEA_OK;
 }
 
 }  // namespace ofea

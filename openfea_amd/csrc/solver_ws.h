@@ -1,0 +1,55 @@
+// solver_ws.h -- PCG work space shared by solver.hip and learned.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ofea {
+
+enum Scalar { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL2, S_RU2, S_TRR, S_NSCAL };
+
+struct DevScalars {
+    double s[S_NSCAL];
+    int done;  // 0 running, 1 recursive residual converged, 2 maxit, 3 breakdown, 4 reliable update due
+    int it;
+    int maxit;
+    unsigned int ticket;
+    int par0;     // parity of the first iteration of the next batch (set by the host)
+    int it0;      // iteration number of the first iteration of the next batch
+    int bi_done;  // batch index at which done was set
+};
+
+// CG vectors; the search direction is double-buffered (p, p2) because the
+// SpMV kernel forms p_new = z + beta p_old on the fly while other workgroups
+// still read p_old.
+template <class T>
+struct VecSet {
+    T *r = nullptr, *z = nullptr, *p = nullptr, *p2 = nullptr, *q = nullptr;
+};
+
+constexpr int NSLOT = 6;  // captured graphs: jacobi, block, mixed, learned, mixed32, learned mixed
+
+}  // namespace ofea
+
+struct ofea_solver_ws {
+    int n = 0, nfree = 0;
+    long long nnz = 0;
+    int spmv_group = 16;  // lanes per node in the SpMV (autotuned)
+    int nblk_vec = 0, nblk_node = 0;
+    double *partials = nullptr;
+    double *part_rz = nullptr, *part_rr = nullptr, *part_pq = nullptr;  // CG partials (fixed grid)
+    ofea::DevScalars *sc = nullptr;
+    ofea::DevScalars *sc_host = nullptr;  // pinned
+    ofea::VecSet<double> v64;
+    ofea::VecSet<float> v32;
+    double *r64 = nullptr, *y64 = nullptr;  // true residual, K x
+    double *dinv = nullptr;                 // point Jacobi, n
+    double *minv = nullptr;                 // 3x3 block-Jacobi inverses, 9 * nfree
+    float *minv32 = nullptr;                // FP32 copy
+    double *dblk = nullptr;                 // 3x3 diagonal blocks of K, 9 * nfree
+    float *val32 = nullptr;                 // FP32 values, nnz
+    double *learned64 = nullptr;            // learned preconditioner blocks (FP64), 9 * nfree
+    float *learned32 = nullptr;             // FP32 copy used by the solver
+    bool precond_valid = false;
+    hipGraph_t graph[ofea::NSLOT] = {};
+    hipGraphExec_t exec[ofea::NSLOT] = {};
+};

@@ -97,20 +97,20 @@ def test_spmv_matches_oracle(ctx):
     assert rel_max(ctx.spmv(x), O.spmv_csc(cp, ri, val, x)) < 1e-13
 
 
-@pytest.mark.parametrize("method", [A.PCG_JACOBI, A.PCG_BLOCK_JACOBI, A.PCG_MIXED])
+@pytest.mark.parametrize("method", [A.PCG_JACOBI, A.PCG_BLOCK_JACOBI, A.PCG_MIXED, A.PCG_MIXED32])
 def test_pcg_cfg1_matches_direct_solve(ctx, method):
     m, resort, F = mg.cantilever_c3d8()
     say(f"pcg method={method} cfg1 20x10x10")
     ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
     cp, ri = ctx.pattern()
     val = ctx.assemble(mg.E_STEEL, mg.NU_STEEL)
-    x, it, rel, ok = ctx.solve(F, method=method, rtol=1e-12, maxit=200000)
-    assert ok and rel <= 1e-12
+    x, it, rel, ok = ctx.solve(F, method=method, rtol=1e-10, maxit=200000)
+    assert ok and rel <= 1e-10, (it, rel)
     xd = spla.spsolve(csc(cp, ri, val), F)
     assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
     # the oracle's own PCG takes a comparable number of iterations (Jacobi only)
     if method == A.PCG_JACOBI:
-        _, it_o, _ = O.pcg_jacobi(cp, ri, val, F, rtol=1e-12)
+        _, it_o, _ = O.pcg_jacobi(cp, ri, val, F, rtol=1e-10)
         assert abs(it - it_o) <= max(10, 0.05 * it_o)
 
 
@@ -122,7 +122,7 @@ def test_pipeline_reference_deck_vs_pardiso_golden(ctx):
     assam = ofa.AssambleStiffness(ctx)
     assam.build_CSR(d)
     assam.fill_CSR_sparse_mat(d, 2.1e5, 0.3)
-    assert ofa.solution_api(assam, d, "GPU_PCG", rtol=1e-12)
+    assert ofa.solution_api(assam, d, "GPU_PCG", rtol=1e-10)
     assert not ofa.solution_api(assam, d, "no_such_solver")
     u = ofa.reset_displacement(d.resort, d.single_dis_vec_).reshape(-1, 3)
     ref = np.load(os.path.join(GOLDEN, "cadcae_o_u.npz"))["u_o"]
