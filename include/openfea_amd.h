@@ -146,6 +146,16 @@ int ofea_explicit_run(ofea_ctx *ctx, double E, double nu, const double *mass, co
 /* Tangent K_T (resident, copied to nz_val when non-NULL) and internal force
  * f_int (n, free DOFs) at the free-DOF displacement u (n). */
 int ofea_nl_assemble(ofea_ctx *ctx, double E, double nu, const double *u, double *nz_val, double *f_int);
+/* Newton loop of CAE_process::implict_analysis (src/cae.cpp:209-253) on the GPU,
+ * extended with load stepping: the load F (n) is ramped in n_incr equal
+ * increments; each increment iterates K_T du = lam F - f_int, u += du until
+ * ||lam F - f_int||_2 <= tol ||lam F||_2 (FP64) or max_newton iterations.
+ * Linear solves: ofea_solve method with inner_rtol; a learned method falls
+ * back to block-Jacobi PCG when it fails.  u_out (n), counters nullable.
+ * Returns OFEA_OK or OFEA_NOT_CONVERGED. */
+int ofea_nl_solve(ofea_ctx *ctx, double E, double nu, const double *F, int n_incr, double tol, int max_newton,
+                  int method, double inner_rtol, double *u_out, int *newton_iters, int *linear_iters,
+                  double *relres);
 
 /* ---- learned block-Jacobi correction (no counterpart in the reference) ----
  * A 2-layer node-graph network (hidden 64) maps per-node features of the
@@ -168,6 +178,32 @@ int ofea_learned_train(ofea_ctx *ctx, int steps, float lr, double *loss_hist);
 int ofea_learned_blocks(ofea_ctx *ctx, double *blocks, float *features);
 /* recompute features and blocks for a newly assembled K */
 int ofea_learned_refresh(ofea_ctx *ctx);
+
+/* ---- batched reanalysis (no counterpart in the reference) ----
+ * Many design variants of one mesh: variant v has per-element moduli
+ * E_var[v*ne + e] and shares nu, pattern and geometry.  Each variant's K_v is
+ * what ofea_set_elem_E + ofea_assemble (fill_CSR_sparse_mat,
+ * src/assemble.cpp:95-151, with a per-element E) would give, all nvar of them
+ * from one kernel launch.  Single-type meshes (all C3D4 or all C3D8),
+ * nvar <= 64.  nz_val (nvar x nnz, nullable) receives the values. */
+int ofea_assemble_batch(ofea_ctx *ctx, int nvar, const double *E_var, double nu, double *nz_val);
+/* nrhs load cases (1, 2, 4 or 8) per variant, solved together by FP64 PCG
+ * with an FP32 3x3 block-Jacobi preconditioner; every read of K_v serves all
+ * nrhs systems.  B and X are [variant][case][n]; iters and relres
+ * (nvar x nrhs, nullable) get per-system iteration counts and true FP64
+ * relative residuals.  Each solve is the one solution_api
+ * (src/linear_solution.cpp:17-64) would do for (K_v, b).  Returns OFEA_OK, or
+ * OFEA_NOT_CONVERGED when a system misses rtol within maxit. */
+int ofea_solve_batch(ofea_ctx *ctx, int nvar, int nrhs, const double *B, double *X, double rtol, int maxit,
+                     int *iters, double *relres);
+/* Times the FP64 SpMV of the resident K with 4, 8, 16, 32 and 64 lanes per
+ * node (times[5] in ms, nullable) and makes the fastest the lane-group size
+ * of every later solve on this context. */
+int ofea_autotune(ofea_ctx *ctx, int *best_group, float *times);
+/* The multi-RHS solver picks its own lane-group size (16, 32 or 64 lanes per
+ * node) per nrhs by timing K_v X on its first call; this reports the choice
+ * and the time of one multi-RHS SpMV (ms).  OFEA_ERR_STATE before that call. */
+int ofea_batch_tiles(ofea_ctx *ctx, int nrhs, int *group, float *ms);
 
 /* ---- timing on the context stream (HIP events) ---- */
 int ofea_timer_start(ofea_ctx *ctx);

@@ -70,6 +70,7 @@ SIGNATURES = {
     "ofea_lumped_mass": (_i, [_vp, _d, _vp]),
     "ofea_explicit_run": (_i, [_vp, _d, _d, _vp, _vp, _vp, _vp, _d, _i, _i, _vp]),
     "ofea_nl_assemble": (_i, [_vp, _d, _d, _vp, _vp, _vp]),
+    "ofea_nl_solve": (_i, [_vp, _d, _d, _vp, _i, _d, _i, _i, _d, _vp, _pi, _pi, _pd]),
     "ofea_learned_nparam": (_i, []),
     "ofea_learned_init": (_i, [_vp, C.c_ulonglong]),
     "ofea_learned_set_params": (_i, [_vp, _vp, _i]),
@@ -79,6 +80,10 @@ SIGNATURES = {
     "ofea_learned_train": (_i, [_vp, _i, _f, _vp]),
     "ofea_learned_blocks": (_i, [_vp, _vp, _vp]),
     "ofea_learned_refresh": (_i, [_vp]),
+    "ofea_assemble_batch": (_i, [_vp, _i, _vp, _d, _vp]),
+    "ofea_solve_batch": (_i, [_vp, _i, _i, _vp, _vp, _d, _i, _vp, _vp]),
+    "ofea_autotune": (_i, [_vp, _pi, _vp]),
+    "ofea_batch_tiles": (_i, [_vp, _i, _pi, _pf]),
     "ofea_timer_start": (_i, [_vp]),
     "ofea_timer_stop": (_i, [_vp, _pf]),
     "ofea_sync": (_i, [_vp]),

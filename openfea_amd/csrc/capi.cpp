@@ -28,6 +28,7 @@ static void free_plan(DevPlan &d)
 static void free_mesh(ofea_ctx *c)
 {
     learned_free(c);
+    batch_free(c);
     solver_free(c);
     explicit_free(c);
     dev_free(c->d_xyz);
@@ -276,6 +277,17 @@ int ofea_nl_assemble(ofea_ctx *c, double E, double nu, const double *u, double *
     return nl_assemble(c, E, nu, u, nz_val, f_int);
 }
 
+int ofea_nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr, double tol, int max_newton,
+                  int method, double inner_rtol, double *u_out, int *newton_iters, int *linear_iters, double *relres)
+{
+    CHECK_CTX(c);
+    if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    if (!F || n_incr < 1 || !(tol > 0.0) || max_newton < 1 || !(inner_rtol > 0.0))
+        return fail(OFEA_ERR_ARG, "bad Newton arguments");
+    return nl_solve(c, E, nu, F, n_incr, tol, max_newton, method, inner_rtol, u_out, newton_iters, linear_iters,
+                    relres);
+}
+
 int ofea_learned_nparam(void) { return learned_nparam(); }
 
 int ofea_learned_init(ofea_ctx *c, unsigned long long seed)
@@ -353,6 +365,34 @@ int ofea_stream_copy(ofea_ctx *c, long long bytes, int reps, float *ms)
     CHECK_CTX(c);
     if (!ms) return fail(OFEA_ERR_ARG, "null ms");
     return stream_copy(c, bytes, reps, ms);
+}
+
+int ofea_assemble_batch(ofea_ctx *c, int nvar, const double *E_var, double nu, double *nz_val)
+{
+    CHECK_CTX(c);
+    if (!c->d_col_ptr) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    if (!E_var) return fail(OFEA_ERR_ARG, "null E_var");
+    return assemble_batch(c, nvar, E_var, nu, nz_val);
+}
+
+int ofea_solve_batch(ofea_ctx *c, int nvar, int nrhs, const double *B, double *X, double rtol, int maxit, int *iters,
+                     double *relres)
+{
+    CHECK_CTX(c);
+    if (!B || nvar < 1 || maxit < 1 || !(rtol > 0.0)) return fail(OFEA_ERR_ARG, "bad B, nvar, maxit or rtol");
+    return solve_batch(c, nvar, nrhs, B, X, rtol, maxit, iters, relres);
+}
+
+int ofea_autotune(ofea_ctx *c, int *best_group, float *times)
+{
+    CHECK_CTX(c);
+    return autotune(c, best_group, times);
+}
+
+int ofea_batch_tiles(ofea_ctx *c, int nrhs, int *group, float *ms)
+{
+    CHECK_CTX(c);
+    return batch_tiles(c, nrhs, group, ms);
 }
 
 int ofea_sync(ofea_ctx *c)

@@ -291,7 +291,7 @@ __global__ void k_ln_head_back(int nfree, const double *__restrict__ G, const do
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
             double acc = 0.0;
-            for (int a = 0; a < 3; ++a) acc += T[3 * i + a] * A[3 * j + a];
+            for (int a = 0; a < 3; ++a) acc += T[3 * i + a] * A[3 * a + j];
             H[3 * i + j] = acc;
         }
     // dL = (H + H^T) L

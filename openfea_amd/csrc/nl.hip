@@ -13,6 +13,8 @@
 //   f_a = w F S g_a
 // The kernels reuse the column-owner gather plan of the linear assembly.
 #include <algorithm>
+#include <cmath>
+#include <vector>
 
 #include "elem_device.h"
 #include "ofea_internal.h"
@@ -336,12 +338,9 @@ __global__ __launch_bounds__(NL_WG) void k_nl_tet(const double *__restrict__ xyz
 
 int nl_plan_cap(int type) { return type == OFEA_C3D8 ? NL_HEX_CAP : NL_TET_CAP; }
 
-int nl_assemble(ofea_ctx *c, double E, double nu, const double *u, double *nz_val, double *f_int)
+static int nl_assemble_dev(ofea_ctx *c, double E, double nu)
 {
     const int n = c->P.n;
-    if (!c->d_u) OFEA_HIP(hipMalloc((void **)&c->d_u, sizeof(double) * std::max(n, 1)));
-    if (!c->d_fint) OFEA_HIP(hipMalloc((void **)&c->d_fint, sizeof(double) * std::max(n, 1)));
-    OFEA_HIP(hipMemcpyAsync(c->d_u, u, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     const bool mixed = c->has_type[0] && c->has_type[1];
     if (mixed) {
         OFEA_HIP(hipMemsetAsync(c->d_val, 0, sizeof(double) * c->P.nnz, c->stream));
@@ -362,10 +361,165 @@ int nl_assemble(ofea_ctx *c, double E, double nu, const double *u, double *nz_va
     c->assembled = true;
     c->kver++;
     solver_invalidate(c);
+    return OFEA_OK;
+}
+
+static int nl_buffers(ofea_ctx *c)
+{
+    const int n = c->P.n;
+    if (!c->d_u) OFEA_HIP(hipMalloc((void **)&c->d_u, sizeof(double) * std::max(n, 1)));
+    if (!c->d_fint) OFEA_HIP(hipMalloc((void **)&c->d_fint, sizeof(double) * std::max(n, 1)));
+    return OFEA_OK;
+}
+
+int nl_assemble(ofea_ctx *c, double E, double nu, const double *u, double *nz_val, double *f_int)
+{
+    const int n = c->P.n;
+    int rc = nl_buffers(c);
+    if (rc) return rc;
+    OFEA_HIP(hipMemcpyAsync(c->d_u, u, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    if ((rc = nl_assemble_dev(c, E, nu))) return rc;
     if (nz_val) OFEA_HIP(hipMemcpyAsync(nz_val, c->d_val, sizeof(double) * c->P.nnz, hipMemcpyDeviceToHost, c->stream));
     if (f_int) OFEA_HIP(hipMemcpyAsync(f_int, c->d_fint, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     OFEA_HIP(hipStreamSynchronize(c->stream));
     return OFEA_OK;
+}
+
+// r = lam F - f_int (into the solver's RHS buffer) with per-block partial
+// sums of r.r and (lam F).(lam F)
+constexpr int NLR_WG = 256;
+__global__ void k_nl_residual(int n, double lam, const double *__restrict__ F, const double *__restrict__ fint,
+                              double *__restrict__ r, double *__restrict__ part)
+{
+    __shared__ double s1[NLR_WG / 64], s2[NLR_WG / 64];
+    const int i = blockIdx.x * NLR_WG + threadIdx.x;
+    double a = 0.0, b = 0.0;
+    if (i < n) {
+        const double f = lam * F[i];
+        const double v = f - fint[i];
+        r[i] = v;
+        a = v * v;
+        b = f * f;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s1[threadIdx.x >> 6] = a;
+        s2[threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ta = 0.0, tb = 0.0;
+        for (int k = 0; k < NLR_WG / 64; ++k) {
+            ta += s1[k];
+            tb += s2[k];
+        }
+        part[2 * blockIdx.x] = ta;
+        part[2 * blockIdx.x + 1] = tb;
+    }
+}
+
+__global__ void k_axpy(int n, double a, const double *__restrict__ x, double *__restrict__ y)
+{
+    const int i = blockIdx.x * NLR_WG + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+
+int learned_refresh(ofea_ctx *c);
+
+// Load-stepped Newton on the total-Lagrangian St.Venant-Kirchhoff problem.
+// Reference loop: src/cae.cpp:209-253 (full load in one step, r = F - f_int,
+// K_T du = r, u += du, stop at ||r||_2/(3n) < 1e-6 or 100 loops).  Here the
+// load is ramped in n_incr increments lam_k = k/n_incr and each increment
+// iterates until ||lam F - f_int|| <= tol ||lam F|| (FP64 residual) or
+// max_newton iterations; the linear solves are GPU PCG (method, inner_rtol),
+// with block-Jacobi PCG as the fallback of the learned method.
+int nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr, double tol, int max_newton, int method,
+             double inner_rtol, double *u_out, int *newton_total, int *linear_total, double *relres)
+{
+    const int n = c->P.n;
+    int rc = nl_buffers(c);
+    if (rc) return rc;
+    double *dF = nullptr, *part = nullptr;
+    const int nb = (n + NLR_WG - 1) / NLR_WG;
+    OFEA_HIP(hipMalloc((void **)&dF, sizeof(double) * std::max(n, 1)));
+    OFEA_HIP(hipMalloc((void **)&part, sizeof(double) * 2 * std::max(nb, 1)));
+    std::vector<double> hp(2 * std::max(nb, 1));
+    auto cleanup = [&]() {
+        (void)hipFree(dF);
+        (void)hipFree(part);
+    };
+    OFEA_HIP(hipMemcpyAsync(dF, F, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    OFEA_HIP(hipMemsetAsync(c->d_u, 0, sizeof(double) * std::max(n, 1), c->stream));
+    int newton = 0, linear = 0, fallbacks = 0;
+    double rel = 0.0;
+    for (int k = 1; k <= n_incr; ++k) {
+        const double lam = (double)k / n_incr;
+        bool conv = false;
+        for (int it = 0; it <= max_newton; ++it) {
+            if ((rc = nl_assemble_dev(c, E, nu))) break;
+            hipLaunchKernelGGL(k_nl_residual, dim3(nb), dim3(NLR_WG), 0, c->stream, n, lam, dF, c->d_fint, c->d_rhs,
+                               part);
+            if (hipMemcpyAsync(hp.data(), part, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess) {
+                rc = fail(OFEA_ERR_HIP, "residual copy failed");
+                break;
+            }
+            double rr = 0.0, ff = 0.0;
+            for (int b = 0; b < nb; ++b) {
+                rr += hp[2 * b];
+                ff += hp[2 * b + 1];
+            }
+            rel = ff > 0.0 ? std::sqrt(rr / ff) : std::sqrt(rr);
+            if (rel <= tol) {
+                conv = true;
+                break;
+            }
+            if (it == max_newton) break;
+            c->rhs_set = true;
+            int lit = 0;
+            double lrel = 0.0;
+            int m = method;
+            if (m == OFEA_PCG_LEARNED || m == OFEA_PCG_LEARNED_MIXED) {
+                if (!c->learned || learned_refresh(c) != OFEA_OK) m = OFEA_PCG_BLOCK_JACOBI;
+            }
+            int src = solver_solve(c, m, inner_rtol, 20000, &lit, &lrel);
+            linear += lit;
+            if (src != OFEA_OK && m != OFEA_PCG_BLOCK_JACOBI) {  // learned path failed: plain PCG judges
+                fallbacks++;
+                src = solver_solve(c, OFEA_PCG_BLOCK_JACOBI, inner_rtol, 20000, &lit, &lrel);
+                linear += lit;
+            }
+            if (src < 0) {
+                rc = src;
+                break;
+            }
+            hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(NLR_WG), 0, c->stream, n, 1.0, c->d_x, c->d_u);
+            newton++;
+        }
+        if (rc) break;
+        if (!conv) {
+            rc = OFEA_NOT_CONVERGED;
+            break;
+        }
+    }
+    if (rc < 0) {
+        cleanup();
+        return rc;
+    }
+    if (u_out) {
+        OFEA_HIP(hipMemcpyAsync(u_out, c->d_u, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        OFEA_HIP(hipStreamSynchronize(c->stream));
+    }
+    cleanup();
+    if (newton_total) *newton_total = newton;
+    if (linear_total) *linear_total = linear;
+    if (relres) *relres = rel;
+    (void)fallbacks;
+    return rc;
 }
 
 }  // namespace ofea

@@ -117,6 +117,8 @@ struct ofea_ctx {
     double *d_u = nullptr, *d_fint = nullptr;  // TL displacement / internal force (n)
     // learned preconditioner parameters (device), set by ofea_learned_*
     void *learned = nullptr;
+    // batched variants / multi-RHS solves (batched.hip)
+    void *batch_ws = nullptr;
 };
 
 namespace ofea {
@@ -146,6 +148,17 @@ int learned_blocks(ofea_ctx *c, double *blocks, float *features);
 int learned_refresh(ofea_ctx *c);
 // nonlinear (nl.hip)
 int nl_assemble(ofea_ctx *ctx, double E, double nu, const double *u, double *nz_val, double *f_int);
+int nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr, double tol, int max_newton, int method,
+             double inner_rtol, double *u_out, int *newton_total, int *linear_total, double *relres);
+// batched variants (batched.hip)
+int assemble_batch(ofea_ctx *c, int nvar, const double *E_var, double nu, double *out);
+int solve_batch(ofea_ctx *c, int nvar, int nrhs, const double *B, double *X, double rtol, int maxit, int *iters,
+                double *relres);
+int autotune(ofea_ctx *c, int *best_group, float *times);
+int batch_tiles(ofea_ctx *c, int nrhs, int *group, float *ms);
+void batch_free(ofea_ctx *c);
+// SpMV of K (FP64) with lane-group size `group`, averaged over reps launches (solver.hip)
+int solver_spmv_group_time(ofea_ctx *ctx, int group, int reps, float *ms);
 
 template <class T>
 int dev_upload(T *&d, const T *h, size_t count)

@@ -540,6 +540,26 @@ int solver_spmv(ofea_ctx *ctx, const double *d_x, double *d_y)
     return OFEA_OK;
 }
 
+int solver_spmv_group_time(ofea_ctx *ctx, int group, int reps, float *ms)
+{
+    int rc = ensure_ws(ctx);
+    if (rc) return rc;
+    ofea_solver_ws *w = ctx->ws;
+    const int gv = (w->n + RED_WG - 1) / RED_WG;
+    if (gv == 0 || reps < 1) return fail(OFEA_ERR_ARG, "empty system or reps < 1");
+    hipLaunchKernelGGL((k_fill<double>), dim3(gv), dim3(RED_WG), 0, ctx->stream, w->n, w->r64, 1.0);
+    launch_spmv_t<double, double, false>(ctx, group, ctx->d_val, w->r64, w->y64);  // warm-up
+    OFEA_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int i = 0; i < reps; ++i) launch_spmv_t<double, double, false>(ctx, group, ctx->d_val, w->r64, w->y64);
+    OFEA_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    OFEA_HIP(hipEventSynchronize(ctx->ev1));
+    OFEA_HIP(hipGetLastError());
+    float t = 0.f;
+    OFEA_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    *ms = t / reps;
+    return OFEA_OK;
+}
+
 int solver_spmv32(ofea_ctx *ctx, const float *d_x, float *d_y)
 {
     launch_spmv_t<float, float, false>(ctx, ctx->ws->spmv_group, ctx->ws->val32, d_x, d_y);

@@ -205,6 +205,44 @@ class Context:
         check(lib().ofea_nl_assemble(self._h, E, nu, ptr(f64(u_free)), ptr(val), ptr(f)))
         return val, f
 
+    def nl_solve(self, E, nu, F, n_incr=10, tol=1e-8, max_newton=25, method=A.PCG_BLOCK_JACOBI, inner_rtol=1e-10):
+        u = np.zeros(self.n)
+        nt, lt, rr = C.c_int(), C.c_int(), C.c_double()
+        rc = check(lib().ofea_nl_solve(self._h, E, nu, ptr(f64(F)), n_incr, tol, max_newton, method, inner_rtol, ptr(u),
+                                       C.byref(nt), C.byref(lt), C.byref(rr)), allow=(A.OFEA_OK, A.OFEA_NOT_CONVERGED))
+        return u, nt.value, lt.value, rr.value, rc == A.OFEA_OK
+
+    # ---- batched variants / load cases
+    def assemble_batch(self, E_var, nu, copy=True):
+        """K_v for every row of E_var (nvar x ne per-element moduli), one launch."""
+        E_var = f64(E_var).reshape(-1, self.ne)
+        out = np.zeros((len(E_var), self.nnz)) if copy else None
+        check(lib().ofea_assemble_batch(self._h, len(E_var), ptr(E_var), nu, ptr(out)))
+        return out
+
+    def solve_batch(self, B, rtol=1e-8, maxit=100000, copy=True):
+        """B: nvar x nrhs x n right-hand sides; returns X, iters, relres, converged."""
+        B = f64(B)
+        nvar, nrhs, n = B.shape
+        X = np.zeros_like(B) if copy else None
+        it = np.zeros((nvar, nrhs), dtype=np.int32)
+        rr = np.zeros((nvar, nrhs))
+        rc = check(lib().ofea_solve_batch(self._h, nvar, nrhs, ptr(B), ptr(X), rtol, maxit, ptr(it), ptr(rr)),
+                   allow=(A.OFEA_OK, A.OFEA_NOT_CONVERGED))
+        return X, it, rr, rc == A.OFEA_OK
+
+    def autotune(self):
+        """Times the SpMV lane-group sizes 4..64 on the resident K; returns (best, times ms)."""
+        best = C.c_int()
+        times = np.zeros(5, dtype=np.float32)
+        check(lib().ofea_autotune(self._h, C.byref(best), ptr(times)))
+        return best.value, times
+
+    def batch_tiles(self, nrhs):
+        g, ms = C.c_int(), C.c_float()
+        check(lib().ofea_batch_tiles(self._h, nrhs, C.byref(g), C.byref(ms)))
+        return g.value, ms.value
+
     # ---- timing on the context stream
     def timer_start(self):
         check(lib().ofea_timer_start(self._h))

@@ -196,3 +196,28 @@ def matfree_apply(conn, etype, xyz, resort, E, nu, x, n, E_elem=None):
     lib().or_matfree_apply(len(etype), _i32(conn), _i32(etype), _f64(xyz).ravel(), _i32(resort), E, nu,
                            None if Ee is None else Ee.ctypes.data, _f64(x), y, n)
     return y
+
+
+def nl_newton(conn, etype, xyz, resort, E, nu, F, col_ptr, row_idx, n_incr=1, tol=1e-10, max_newton=50):
+    """Newton loop of src/cae.cpp:209-253 (r = F - f_int, K_T du = r, u += du)
+    with the load ramped in n_incr increments and the stopping rule
+    ||lam F - f_int|| <= tol ||lam F||; direct sparse solves (scipy SuperLU)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    n = len(col_ptr) - 1
+    u = np.zeros(n)
+    iters = 0
+    for k in range(1, n_incr + 1):
+        lam = k / n_incr
+        for _ in range(max_newton + 1):
+            val, fint = nl_fill(conn, etype, xyz, resort, E, nu, u, col_ptr, row_idx)
+            r = lam * np.asarray(F) - fint
+            if np.linalg.norm(r) <= tol * np.linalg.norm(lam * np.asarray(F)):
+                break
+            K = sp.csc_matrix((val, row_idx, col_ptr), shape=(n, n))
+            u = u + spla.spsolve(K, r)
+            iters += 1
+        else:
+            raise RuntimeError("oracle Newton did not converge")
+    return u, iters

@@ -203,3 +203,22 @@ def test_explicit_incremental_equals_direct_and_conserves_energy():
         v = vn
     energies = np.array(energies)
     assert np.abs(energies - energies[0]).max() < 1e-10 * abs(energies[0])
+
+
+def test_newton_small_load_matches_linear_and_converges():
+    import scipy.sparse.linalg as spla2
+    m, resort, F = mg.cantilever_c3d8(4, 2, 2)
+    n = len(F)
+    cp, ri = O.build_csc(m.conn, m.etype, resort, n)
+    # strains ~1e-5: the Green-Lagrange strain (F^T F - I)/2 of the reference
+    # formulation cancels to ~1e-16/1e-5 relative, so Newton tolerance 1e-9
+    eps = 1e-2
+    u_nl, it = O.nl_newton(m.conn, m.etype, m.xyz, resort, 210000.0, 0.3, eps * F, cp, ri, n_incr=1, tol=1e-9)
+    K = csc_matrix(cp, ri, O.fill_csc(m.conn, m.etype, m.xyz, resort, 210000.0, 0.3, cp, ri))
+    u_lin = spla2.spsolve(K, F)
+    assert np.linalg.norm(u_nl / eps - u_lin) / np.linalg.norm(u_lin) < 1e-3   # geometric effect is O(strain)
+    # a large load: quadratic convergence to a tight residual within a few steps per increment
+    u_big, it_big = O.nl_newton(m.conn, m.etype, m.xyz, resort, 210000.0, 0.3, 50 * F, cp, ri, n_incr=2)
+    _, fint = O.nl_fill(m.conn, m.etype, m.xyz, resort, 210000.0, 0.3, u_big, cp, ri)
+    assert np.linalg.norm(50 * F - fint) <= 1e-10 * np.linalg.norm(50 * F)
+    assert it_big <= 16
