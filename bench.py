@@ -185,7 +185,8 @@ def bench_explicit(args):
     from oracle import oracle as O
 
     m = mg.hex_block(100, 50, 50)
-    wl = "explicit C3D8 100x50x50 (250,000 hexes), lumped mass, central difference, FP32 state"
+    wl = ("explicit C3D8 100x50x50 (250,000 hexes), lumped mass, central difference dt = 0.9 CFL, "
+          "v_z = 1000 sin(pi x/L), x=0 clamped, 1,000 steps, FP32 state")
     ctx = ofa.Context(0)
     ctx.set_mesh(m.xyz, m.conn, m.etype, mg.free_index(m.nd, m.fix_set))
     mass = ctx.lumped_mass(mg.RHO_STEEL)
@@ -198,23 +199,253 @@ def bench_explicit(args):
     me = O.c3d8_mass(m.xyz[m.conn[0] - 1], mg.RHO_STEEL)
     lam = np.linalg.eigvals(Ke / np.repeat(me, 3)[:, None]).real.max()
     dt = 0.9 * 2.0 / np.sqrt(lam)
-    nsteps = args.steps if args.steps > 0 else 1000
-    ctx.explicit_run(mg.E_STEEL, mg.NU_STEEL, mass, fext, v0, fixed, dt, min(args.warmup * 50, 100), copy=False)
+    nsteps = args.explicit_steps
+    ctx.explicit_run(mg.E_STEEL, mg.NU_STEEL, mass, fext, v0, fixed, dt, 100, copy=False)  # warm-up + capture
     ctx.sync()
     ctx.timer_start()
-    ctx.explicit_run(mg.E_STEEL, mg.NU_STEEL, mass, fext, v0, fixed, dt, nsteps, copy=False)
+    u32 = ctx.explicit_run(mg.E_STEEL, mg.NU_STEEL, mass, fext, v0, fixed, dt, nsteps, precision=A.FP32,
+                           copy=False)
     ms = ctx.timer_stop()
     ms_step = ms / nsteps
+    log(f"explicit fp32 {ms_step * 1e3:.2f} us/step")
+    ctx.explicit_run(mg.E_STEEL, mg.NU_STEEL, mass, fext, v0, fixed, dt, 100, precision=A.FP64, copy=False)
+    ctx.sync()
+    ctx.timer_start()
+    ctx.explicit_run(mg.E_STEEL, mg.NU_STEEL, mass, fext, v0, fixed, dt, nsteps, precision=A.FP64, copy=False)
+    ms64 = ctx.timer_stop() / nsteps
+    u32 = ctx.explicit_run(mg.E_STEEL, mg.NU_STEEL, mass, fext, v0, fixed, dt, nsteps, precision=A.FP32)
+    u64 = ctx.explicit_run(mg.E_STEEL, mg.NU_STEEL, mass, fext, v0, fixed, dt, nsteps, precision=A.FP64)
+    err = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
+    log(f"explicit fp64 {ms64 * 1e3:.2f} us/step; fp32 vs fp64 at step {nsteps}: {err:.3e}")
     alg_bytes = 32 * m.ne + 64 * m.nd
+    flops = 5300 * m.ne  # per element-step, see the explicit roofline note
     ctx.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        t = time.perf_counter()
+        k = 3
+        O.explicit_run(m.conn, m.etype, m.xyz, mg.E_STEEL, mg.NU_STEEL, O.lumped_mass(m.conn, m.etype, m.xyz,
+                       mg.RHO_STEEL), fext, v0, fixed, dt, k)
+        el = time.perf_counter() - t
+        cpu = {"value": k / el, "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": f"oracle explicit_run (reference cal_in_force recurrence: Ke rebuilt per element per "
+                         f"step) on the full mesh, {k} steps, FP64 ({el:.1f}s)"}
+    achieved = alg_bytes / (ms_step * 1e-3) / 1e9
     return {
         "metric": "explicit steps/s", "value": 1e3 / ms_step, "unit": "steps/s", "n_gpus": 1, "steps": nsteps,
-        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "warmup": 100, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": wl, "dt": dt, "elements": m.ne},
-        "roofline": {"bound": "hbm", "achieved": alg_bytes / (ms_step * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": alg_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
-        "cpu_baseline": None,
+        "config": {"workload": wl, "dt": dt, "elements": m.ne, "nodes": m.nd},
+        "fp64_ms_per_step": ms64, "fp32_vs_fp64_rel_err_final": err,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("explicit_step_cfg3"),
+                     "kernel": "k_force_hex<float> + k_node_update<float> (one step)",
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "fp32_flops_per_step": flops,
+                     "fp32_tflops": flops / (ms_step * 1e-3) / 1e12, "fp32_peak_tflops": FP32_PEAK_TFLOPS},
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_batched(args):
+    import numpy as np
+
+    import openfea_amd as ofa
+    from openfea_amd import _capi as A, meshgen as mg
+    from oracle import oracle as O
+
+    nvar, nrhs = 64, 8
+    m = mg.hex_block(40, 20, 20)
+    resort = mg.free_index(m.nd, m.fix_set)
+    wl = ("batched reanalysis C3D8 40x20x20 (16,000 hexes) x 64 design variants (E = 210000 f_v(x), f_v in "
+          "[0.5,1.5]) x 8 load cases (1-10 MPa tractions on x=L)")
+    ctx = ofa.Context(0)
+    ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
+    Ev = np.stack([mg.design_field(m, v) for v in range(nvar)])
+    B = mg.batched_loads(m, resort, nvar, nrhs)
+    nu = mg.NU_STEEL
+    log(f"cfg4: ne={m.ne} n={ctx.n} nnz={ctx.nnz} variants {nvar} x {nrhs} load cases")
+    for _ in range(args.warmup):
+        ctx.assemble_batch(Ev, nu, copy=False)
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(args.steps):
+        ctx.assemble_batch(Ev, nu, copy=False)
+    ms_step = ctx.timer_stop() / args.steps
+    value = m.ne * nvar / (ms_step * 1e-3)
+    log(f"batched assembly {ms_step:.3f} ms/launch -> {value:.3e} variant-elems/s")
+    # the same 64 variants one ofea_assemble each
+    ctx.sync()
+    ctx.timer_start()
+    for v in range(nvar):
+        ctx.set_elem_E(Ev[v])
+        ctx.assemble(0.0, nu, copy=False)
+    ms_seq = ctx.timer_stop()
+    ctx.set_elem_E(None)
+    alg_bytes = 24 * m.nd + 4 * 8 * m.ne + 8 * m.ne * nvar + 8 * ctx.nnz * nvar
+    achieved = alg_bytes / (ms_step * 1e-3) / 1e9
+
+    # multi-RHS solves of all 512 systems (rel. residual 1e-8, FP64 vectors, FP32 block-Jacobi)
+    ctx.solve_batch(B[:1], rtol=1e-8, copy=False)  # tile autotune + graph capture
+    t = time.perf_counter()
+    _, it_all, rr_all, ok_all = ctx.solve_batch(B, rtol=1e-8, copy=False)
+    t_all = time.perf_counter() - t
+    g, spmv_ms = ctx.batch_tiles(nrhs)
+    log(f"batched solve 512 systems {t_all:.3f}s iters {it_all.min()}..{it_all.max()} ok={ok_all} lanes={g}")
+    # like for like on the first 8 variants: one ofea_solve per system (plain FP64 Jacobi and block Jacobi)
+    nv8 = 8
+    t = time.perf_counter()
+    _, it8, _, ok8 = ctx.solve_batch(B[:nv8], rtol=1e-8, copy=False)
+    t_b8 = time.perf_counter() - t
+    seq = {}
+    for name, meth in (("plain_fp64_jacobi", A.PCG_JACOBI), ("fp64_block_jacobi", A.PCG_BLOCK_JACOBI)):
+        tot, its = 0.0, []
+        for v in range(nv8):
+            ctx.set_elem_E(Ev[v])
+            ctx.assemble(0.0, nu, copy=False)
+            ctx.solve(B[v, 0], method=meth, rtol=1e-8, copy=False)  # warm
+            for k in range(nrhs):
+                ctx.set_rhs(B[v, k])
+                ctx.sync()
+                t = time.perf_counter()
+                _, it, rel, ok = ctx.solve(None, method=meth, rtol=1e-8, copy=False)
+                tot += time.perf_counter() - t
+                its.append(it)
+        seq[name] = {"s": round(tot, 4), "iters_mean": float(np.mean(its)),
+                     "batched_speedup": round(tot / t_b8, 3)}
+        log(f"sequential {name} on {nv8}x{nrhs}: {tot:.3f}s (batched {t_b8:.3f}s)")
+    ctx.set_elem_E(None)
+    ctx.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        cp, ri = O.build_csc(m.conn, m.etype, resort, 3 * int(resort.max() + 1))
+        k = 16
+        t = time.perf_counter()
+        for v in range(k):
+            O.fill_csc(m.conn, m.etype, m.xyz, resort, 0.0, nu, cp, ri, E_elem=Ev[v])
+        el = time.perf_counter() - t
+        cpu = {"value": k * m.ne / el, "unit": "variant-elems/s", "cores": 1, "kind": "port",
+               "sample": f"oracle fill_csc with per-element E for {k} of the 64 variants ({el:.1f}s)"}
+    return {
+        "metric": "batched assembled variant-elems/s + multi-RHS time-to-solve at rel-resid 1e-8",
+        "value": value, "unit": "variant-elems/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": wl, "elements": m.ne, "variants": nvar, "load_cases": nrhs, "dofs": int(B.shape[2]),
+                   "nnz": ctx.nnz},
+        "sequential_assembly_ms_64_variants": ms_seq,
+        "batched_solve": {"systems": nvar * nrhs, "s": round(t_all, 4), "iters_min": int(it_all.min()),
+                          "iters_max": int(it_all.max()), "relres_max": float(rr_all.max()), "converged": ok_all,
+                          "lanes_per_node": g, "multi_rhs_spmv_ms": spmv_ms,
+                          "first_8_variants_s": round(t_b8, 4), "sequential_first_8_variants": seq},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("assemble_hex_batch_cfg4"),
+                     "kernel": "k_assemble_hex_batch", "algorithmic_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_nl(args):
+    import numpy as np
+
+    import openfea_amd as ofa
+    from openfea_amd import _capi as A, learned as L, meshgen as mg
+    from oracle import oracle as O
+
+    m = mg.hex_block(80, 40, 40)
+    resort = mg.free_index(m.nd, m.fix_set)
+    E, nu = mg.E_STEEL, mg.NU_STEEL
+    wl = ("geometric nonlinear C3D8 80x40x40 (128,000 hexes), TL St.Venant-Kirchhoff, z traction ramped in 10 "
+          "increments to a linear tip deflection of 0.2 L, Newton to |r|/|F| <= 1e-8")
+    ctx = ofa.Context(0)
+    ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
+    log(f"cfg5: ne={m.ne} n={ctx.n} nnz={ctx.nnz}")
+    F1 = mg.free_rhs(mg.face_traction(m, (0.0, 0.0, 1.0)), resort)
+    ctx.assemble(E, nu, copy=False)
+    x1, _, _, _ = ctx.solve(F1, method=A.PCG_BLOCK_JACOBI, rtol=1e-10)
+    uf = ofa.reset_displacement(resort, x1).reshape(-1, 3)
+    L_ = 80.0
+    tip = np.abs(m.xyz[:, 0] - L_) < 1e-9
+    p_star = 0.2 * L_ / uf[tip, 2].mean()
+    F = p_star * F1
+    log(f"p* = {p_star:.3f} MPa")
+    # TL tangent + internal force throughput at the linear solution
+    u0 = 0.5 * p_star * x1
+    for _ in range(args.warmup):
+        ctx.nl_assemble(E, nu, u0, copy=False)
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(args.steps):
+        ctx.nl_assemble(E, nu, u0, copy=False)
+    ms_step = ctx.timer_stop() / args.steps
+    value = m.ne / (ms_step * 1e-3)
+    log(f"TL assembly {ms_step:.3f} ms -> {value:.3e} elems/s")
+    # learned correction trained on pairs from config-4 solves (40x20x20, design variant 0)
+    t0 = time.perf_counter()
+    m4 = mg.hex_block(40, 20, 20)
+    r4 = mg.free_index(m4.nd, m4.fix_set)
+    c4 = ofa.Context(0)
+    c4.set_mesh(m4.xyz, m4.conn, m4.etype, r4)
+    c4.set_elem_E(mg.design_field(m4, 0))
+    c4.assemble(0.0, nu, copy=False)
+    B4 = mg.batched_loads(m4, r4, 1, 4)[0]
+    Es, Rs = L.collect_pairs(c4, list(B4), snapshots=(8, 16, 32, 64, 128))
+    ln4 = L.Learned(c4, seed=0)
+    ln4.set_samples(Es, Rs)
+    hist = ln4.train(steps=args.train_steps, lr=2e-3)
+    params = ln4.get_params()
+    c4.close()
+    train = {"samples": int(len(Es)), "steps": args.train_steps, "loss_first": float(hist[0]),
+             "loss_last": float(hist[-1]), "train_s": round(time.perf_counter() - t0, 3)}
+    log(f"learned (cfg4 pairs): {train}")
+    ln = L.Learned(ctx, seed=0)
+    ln.set_params(params)
+    runs = {}
+    for name, meth in (("plain_fp64_jacobi", A.PCG_JACOBI), ("fp64_block_jacobi", A.PCG_BLOCK_JACOBI),
+                       ("learned_fp64", A.PCG_LEARNED)):
+        ctx.sync()
+        t = time.perf_counter()
+        u, nt, lt, rel, ok = ctx.nl_solve(E, nu, F, n_incr=10, tol=1e-8, max_newton=25, method=meth,
+                                          inner_rtol=args.inner_rtol)
+        el = time.perf_counter() - t
+        uz = ofa.reset_displacement(resort, u).reshape(-1, 3)[tip, 2].mean()
+        runs[name] = {"s": round(el, 3), "newton_iters": nt, "pcg_iters": lt, "relres": rel, "converged": ok,
+                      "tip_uz_over_L": float(uz / L_)}
+        log(f"newton {name}: {runs[name]}")
+    base = runs["plain_fp64_jacobi"]["s"]
+    for v in runs.values():
+        v["speedup_vs_plain"] = round(base / v["s"], 3)
+    ctx.close()
+    alg_bytes = 24 * m.nd + 4 * 8 * m.ne + 8 * ctx.nnz + 3 * 8 * ctx.n
+    achieved = alg_bytes / (ms_step * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        ms_ = mg.hex_block(40, 20, 20)
+        rs_ = mg.free_index(ms_.nd, ms_.fix_set)
+        cp, ri = O.build_csc(ms_.conn, ms_.etype, rs_, 3 * int(rs_.max() + 1))
+        us = np.random.default_rng(0).normal(scale=1e-3, size=len(cp) - 1)
+        t = time.perf_counter()
+        k = 3
+        for _ in range(k):
+            O.nl_fill(ms_.conn, ms_.etype, ms_.xyz, rs_, E, nu, us, cp, ri)
+        el = time.perf_counter() - t
+        cpu = {"value": k * ms_.ne / el, "unit": "elems/s", "cores": 1, "kind": "port",
+               "sample": f"oracle nl_fill (TL tangent + internal force, reference nl fill algorithm) on a "
+                         f"40x20x20 slice of the same family, {k} reps ({el:.1f}s)"}
+    return {
+        "metric": "TL tangent assembled elems/s + Newton time-to-solve (learned vs plain PCG inner solves)",
+        "value": value, "unit": "elems/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": wl, "elements": m.ne, "dofs": ctx.n, "nnz": ctx.nnz, "p_star_MPa": p_star,
+                   "inner_rtol": args.inner_rtol},
+        "newton": runs, "learned_training": train,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("nl_assemble_hex_cfg5"),
+                     "kernel": "k_nl_hex", "algorithmic_bytes_per_launch": alg_bytes,
+                     "fp64_flops_per_launch": 185472 * m.ne,
+                     "fp64_tflops": 185472 * m.ne / (ms_step * 1e-3) / 1e12, "fp64_peak_tflops": FP64_PEAK_TFLOPS},
+        "cpu_baseline": cpu,
     }
 
 
@@ -223,7 +454,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3"])
+    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3", "cfg4", "cfg5"])
+    ap.add_argument("--explicit-steps", type=int, default=1000)
+    ap.add_argument("--inner-rtol", type=float, default=1e-6, help="Newton inner PCG tolerance (cfg5)")
     ap.add_argument("--solve-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true", help="assembly only (for counter collection)")
@@ -236,8 +469,12 @@ def main():
         sys.exit(2)
     if args.workload in ("cfg1", "cfg2"):
         out = bench_static(args, args.workload)
-    else:
+    elif args.workload == "cfg3":
         out = bench_explicit(args)
+    elif args.workload == "cfg4":
+        out = bench_batched(args)
+    else:
+        out = bench_nl(args)
     print(json.dumps(out), flush=True)
 
 

@@ -192,6 +192,18 @@ def load_case(variant: int, case: int):
     return d * rng.uniform(1.0, 10.0)
 
 
+def batched_loads(mesh: Mesh, resort, nvar: int, nrhs: int):
+    """Right-hand sides [variant][case][free DOF] of the load cases
+    load_case(v, k) on the x = L face.  The consistent face load is linear in
+    the traction vector, so three unit-traction loads are combined."""
+    basis = np.stack([free_rhs(face_traction(mesh, np.eye(3)[c]), resort) for c in range(3)])
+    B = np.zeros((nvar, nrhs, basis.shape[1]))
+    for v in range(nvar):
+        for k in range(nrhs):
+            B[v, k] = load_case(v, k) @ basis
+    return B
+
+
 def initial_velocity(mesh: Mesh, amp=1000.0):
     """v_z(x) = amp sin(pi x / L) (nd x 3)."""
     nx, ny, nz, h = mesh.dims
