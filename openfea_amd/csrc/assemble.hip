@@ -14,9 +14,11 @@
 //       C3D8: one lane per (pair, Gauss point) computes the point's gradients
 //             and the 8 blocks K_ab at that point; an 8-lane xor-shuffle
 //             reduce-scatter leaves lane (pair, b) holding the Gauss sum K_ab;
-//   * phase 2 gives one lane per 3x3 block (r, k) of the chunk's columns; it
-//     sums its contributors (precomputed list, ascending element order, the
-//     reference's summation order) from LDS and writes the 9 CSC values once.
+//   * phase 2 sums, for every 3x3 block (r, k) of the chunk's columns, its
+//     contributors (precomputed list, staged in LDS, ascending element order:
+//     the reference's summation order) and writes the 9 CSC values once; C3D8
+//     uses one lane per block, C3D4 one lane per off-diagonal block and three
+//     per diagonal block (see the C3D4 section).
 // For isotropic D = [lambda, mu] the block is
 //   K_ab[c][d] = w (lambda g_a[c] g_b[d] + mu g_a[d] g_b[c] + mu delta_cd g_a.g_b),
 // algebraically equal to B_a^T D B_b, and the CSC slot (row 3s+d, col 3r+c)
@@ -28,14 +30,17 @@
 namespace ofea {
 
 constexpr int ASM_WG = 256;
-constexpr int TET_CAP = 256;  // pairs per tet chunk: one per thread
-constexpr int HEX_CAP = 32;   // pairs per hex chunk: 8 Gauss-point lanes each
+constexpr int TET_CAP = PLAN_TET_CAP;  // pairs per tet chunk: one per thread
+constexpr int HEX_CAP = PLAN_HEX_CAP;  // pairs per hex chunk: 8 Gauss-point lanes each
 
-// Phase 2, shared by both element types: one lane per block (r, k).
+// Phase 2 of the C3D8 kernel: one lane per block (r, k) sums its
+// contributors, read from the chunk's contributor list staged in LDS (s_ct,
+// entries relative to t0).
 template <class BlockSource>
-__device__ inline void gather_blocks(int b0, int b1, const int *__restrict__ blk_rk, const int *__restrict__ contrib_ptr,
-                                     const uint16_t *__restrict__ contrib, const int *__restrict__ col_ptr,
-                                     double *__restrict__ val, int accumulate, const BlockSource &src)
+__device__ inline void gather_blocks(int b0, int b1, int t0, const int *__restrict__ blk_rk,
+                                     const int *__restrict__ contrib_ptr, const uint16_t *s_ct,
+                                     const int *__restrict__ col_ptr, double *__restrict__ val, int accumulate,
+                                     const BlockSource &src)
 {
     for (int bk = b0 + (int)threadIdx.x; bk < b1; bk += ASM_WG) {
         int rk = blk_rk[bk];
@@ -43,8 +48,8 @@ __device__ inline void gather_blocks(int b0, int b1, const int *__restrict__ blk
         double acc[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) acc[i] = 0.0;
-        int t1 = contrib_ptr[bk + 1];
-        for (int t = contrib_ptr[bk]; t < t1; ++t) src.add(contrib[t], acc);
+        int t1 = contrib_ptr[bk + 1] - t0;
+        for (int t = contrib_ptr[bk] - t0; t < t1; ++t) src.add(s_ct[t], acc);
         int base = col_ptr[3 * r];
         int stride = col_ptr[3 * r + 1] - base;  // 3 * deg(r)
         double *dst = val + base + 3 * k;
@@ -60,70 +65,180 @@ __device__ inline void gather_blocks(int b0, int b1, const int *__restrict__ blk
 }
 
 // ------------------------------------------------------------------ C3D4
-struct TetSource {
-    const double (*g)[12];
-    const double (*w)[2];
-    const int *a;
-    __device__ void add(int ci, double acc[9]) const
-    {
-        int lp = ci >> 2, b = ci & 3, aa = a[lp];
-        double ga0 = g[lp][3 * aa], ga1 = g[lp][3 * aa + 1], ga2 = g[lp][3 * aa + 2];
-        double gb0 = g[lp][3 * b], gb1 = g[lp][3 * b + 1], gb2 = g[lp][3 * b + 2];
-        double wl = w[lp][0], wm = w[lp][1];
-        double ga[3] = {ga0, ga1, ga2}, gb[3] = {gb0, gb1, gb2};
-        double dm = wm * (ga0 * gb0 + ga1 * gb1 + ga2 * gb2);
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int d = 0; d < 3; ++d) acc[3 * c + d] += wl * ga[c] * gb[d] + wm * ga[d] * gb[c] + (c == d ? dm : 0.0);
-    }
+// Phase 2 is split by cost.  An off-diagonal block (r, s) has the few
+// elements that contain the edge r-s as contributors; the diagonal block
+// (r, r) has every element of r (about 24 in a Kuhn grid).  Off-diagonal
+// blocks get one lane each; each diagonal block gets three lanes, one per
+// row c, which sum the cheap symmetric form
+//   K_rr[c][d] = sum_e ((lambda + mu) V g_c g_d + delta_cd mu V g.g)
+// over the node's pairs in element order.  The chunk's contributor list is
+// staged in LDS first, so the summation loops issue no global loads.
+constexpr int TET_CT = 4 * TET_CAP;  // contributor entries of a chunk (<= 4 per pair)
+
+// What one phase-2 lane needs from global memory, loaded before phase 1 so
+// that the loads overlap the element computations.
+struct TetTask {
+    int kind;         // 0 none, 1 off-diagonal block, 2 diagonal row
+    int k, c0, c1;    // block slot k in column r; contributor range (kind 1) or pair range (kind 2)
+    int c;            // row of the diagonal block (kind 2)
+    int base, stride;  // CSC offset of column 3r and the column length 3 deg(r)
 };
+
+__device__ inline TetTask tet_task(int task, int nblk, int b0, int n0, int t0, int p0, const int *__restrict__ blk_rk,
+                                   const int *__restrict__ contrib_ptr, const int *__restrict__ node_pair,
+                                   const int *__restrict__ col_ptr, const int *__restrict__ diag_k)
+{
+    TetTask t{0, 0, 0, 0, 0, 0, 0};
+    int r;
+    if (task < nblk) {
+        const int bk = b0 + task, rk = blk_rk[bk];
+        r = rk >> 8;
+        t.k = rk & 255;
+        t.kind = t.k == diag_k[r] ? 0 : 1;
+        t.c0 = contrib_ptr[bk] - t0;
+        t.c1 = contrib_ptr[bk + 1] - t0;
+    } else {
+        const int j = task - nblk;
+        r = n0 + j / 3;
+        t.c = j - 3 * (j / 3);
+        t.k = diag_k[r];
+        t.kind = t.k < 0 ? 0 : 2;
+        t.c0 = node_pair[r] - p0;
+        t.c1 = node_pair[r + 1] - p0;
+    }
+    t.base = col_ptr[3 * r];
+    t.stride = col_ptr[3 * r + 1] - t.base;
+    return t;
+}
 
 __global__ __launch_bounds__(ASM_WG) void k_assemble_tet(const double *__restrict__ xyz, const int *__restrict__ conn,
                                                          const double *__restrict__ Eel, double E, double nu,
                                                          const int *__restrict__ chunk_pair,
-                                                         const int *__restrict__ chunk_blk, const int *__restrict__ pair,
+                                                         const int *__restrict__ chunk_blk,
+                                                         const int *__restrict__ chunk_node,
+                                                         const int *__restrict__ node_pair, const int *__restrict__ pair,
                                                          const int *__restrict__ blk_rk,
                                                          const int *__restrict__ contrib_ptr,
                                                          const uint16_t *__restrict__ contrib,
-                                                         const int *__restrict__ col_ptr, double *__restrict__ val,
+                                                         const int *__restrict__ col_ptr,
+                                                         const int *__restrict__ diag_k, double *__restrict__ val,
                                                          int accumulate, int nchunks)
 {
-    __shared__ double s_g[TET_CAP][12];
-    __shared__ double s_w[TET_CAP][2];
-    __shared__ int s_a[TET_CAP];
+    // structure-of-arrays: phase 1 (lane = pair) writes consecutive addresses.
+    // Slot j of a pair holds the gradient of local node (a + j) & 3, so slot 0
+    // is the owned node a; contributor entries name (pair, slot).
+    __shared__ double s_g[12][TET_CAP];  // s_g[3 j + c][pair] = d N_(a+j) / d x_c
+    __shared__ double s_w[2][TET_CAP];   // lambda V, mu V (signed V)
+    __shared__ uint16_t s_ct[TET_CT];
     const int chunk = xcd_remap(blockIdx.x, nchunks);
     const int p0 = chunk_pair[chunk], p1 = chunk_pair[chunk + 1];
+    const int b0 = chunk_blk[chunk], b1 = chunk_blk[chunk + 1];
+    const int n0 = chunk_node[chunk], nnode = chunk_node[chunk + 1] - n0;
+    const int t0 = contrib_ptr[b0], nct = contrib_ptr[b1] - t0;  // <= 4 (p1 - p0)
+    const int nblk = b1 - b0;
+    // Phase-2 lanes: waves 0-2 take the off-diagonal blocks and wave 3 the
+    // diagonal rows, so that no wave runs both loop bodies.
+    constexpr int OFF_LANES = ASM_WG - 64;
+    const bool diag_wave = (int)threadIdx.x >= OFF_LANES;
+    const int lane2 = diag_wave ? (int)threadIdx.x - OFF_LANES : (int)threadIdx.x;
+    const int step2 = diag_wave ? 64 : OFF_LANES;
+    const int ntask2 = diag_wave ? 3 * nnode : nblk;
+    const int task_base = diag_wave ? nblk : 0;
+    TetTask first{0, 0, 0, 0, 0, 0, 0};
+    if (lane2 < ntask2)
+        first = tet_task(task_base + lane2, nblk, b0, n0, t0, p0, blk_rk, contrib_ptr, node_pair, col_ptr, diag_k);
+    for (int t = threadIdx.x; t < nct; t += ASM_WG) s_ct[t] = contrib[t0 + t];
     double lam0, mu0;
     lame<double>(E, nu, lam0, mu0);
     for (int p = p0 + (int)threadIdx.x; p < p1; p += ASM_WG) {
-        int pk = pair[p];
-        int el = pk >> 3, lp = p - p0;
+        const int pk = pair[p];
+        const int el = pk >> 3, a = pk & 7, lp = p - p0;
         const int *cn = conn + 4 * el;
         double X[12];
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            int node = cn[b];
+            const int node = cn[b];
             X[3 * b] = xyz[3 * node];
             X[3 * b + 1] = xyz[3 * node + 1];
             X[3 * b + 2] = xyz[3 * node + 2];
         }
         double g[3][4];
-        double detJ = tet_grads(X, g);
+        const double detJ = tet_grads(X, g);
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) s_g[lp][3 * b + c] = g[c][b];
+            for (int c = 0; c < 3; ++c) s_g[3 * ((b - a) & 3) + c][lp] = g[c][b];
         double lam = lam0, mu = mu0;
         if (Eel) lame<double>(Eel[el], nu, lam, mu);
-        double V = detJ / 6.0;  // signed (src/Elements/C3D4.cpp:45-54 does not abs())
-        s_w[lp][0] = lam * V;
-        s_w[lp][1] = mu * V;
-        s_a[lp] = pk & 7;
+        const double V = detJ / 6.0;  // signed (src/Elements/C3D4.cpp:45-54 does not abs())
+        s_w[0][lp] = lam * V;
+        s_w[1][lp] = mu * V;
     }
     __syncthreads();
-    TetSource src{s_g, s_w, s_a};
-    gather_blocks(chunk_blk[chunk], chunk_blk[chunk + 1], blk_rk, contrib_ptr, contrib, col_ptr, val, accumulate, src);
+    for (int t2 = lane2; t2 < ntask2; t2 += step2) {
+        const TetTask tk = t2 == lane2 ? first
+                                       : tet_task(task_base + t2, nblk, b0, n0, t0, p0, blk_rk, contrib_ptr,
+                                                  node_pair, col_ptr, diag_k);
+        if (tk.kind == 1) {  // off-diagonal block (r, k)
+            // P = sum lambda V g_a (x) g_b, Q = sum mu V g_a (x) g_b;
+            // K_ab = P + Q^T + I tr(Q)
+            double P[9], Q[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) P[i] = Q[i] = 0.0;
+#pragma unroll 2
+            for (int t = tk.c0; t < tk.c1; ++t) {
+                const int ci = s_ct[t];
+                const int lp = ci >> 2, j = ci & 3;
+                const double wl = s_w[0][lp], wm = s_w[1][lp];
+                const double ga[3] = {s_g[0][lp], s_g[1][lp], s_g[2][lp]};
+                const double gb[3] = {s_g[3 * j][lp], s_g[3 * j + 1][lp], s_g[3 * j + 2][lp]};
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double lc = wl * ga[c], mc = wm * ga[c];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        P[3 * c + d] += lc * gb[d];
+                        Q[3 * c + d] += mc * gb[d];
+                    }
+                }
+            }
+            const double trQ = Q[0] + Q[4] + Q[8];
+            double *dst = val + tk.base + 3 * tk.k;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    double v = P[3 * c + d] + Q[3 * d + c] + (c == d ? trQ : 0.0);
+                    if (accumulate) v += dst[c * tk.stride + d];
+                    dst[c * tk.stride + d] = v;
+                }
+        } else if (tk.kind == 2) {  // row c of the diagonal block: the node's pairs in element order
+            double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, accd = 0.0;
+            const int c = tk.c;
+#pragma unroll 4
+            for (int lp = tk.c0; lp < tk.c1; ++lp) {
+                const double g0 = s_g[0][lp], g1 = s_g[1][lp], g2 = s_g[2][lp], gc = s_g[c][lp];
+                const double wl = s_w[0][lp], wm = s_w[1][lp];
+                const double sc = (wl + wm) * gc;
+                acc0 += sc * g0;
+                acc1 += sc * g1;
+                acc2 += sc * g2;
+                accd += wm * (g0 * g0 + g1 * g1 + g2 * g2);
+            }
+            acc0 += c == 0 ? accd : 0.0;
+            acc1 += c == 1 ? accd : 0.0;
+            acc2 += c == 2 ? accd : 0.0;
+            double *dst = val + tk.base + 3 * tk.k + c * tk.stride;
+            if (accumulate) {
+                acc0 += dst[0];
+                acc1 += dst[1];
+                acc2 += dst[2];
+            }
+            dst[0] = acc0;
+            dst[1] = acc1;
+            dst[2] = acc2;
+        }
+    }
 }
 
 // ------------------------------------------------------------------ C3D8
@@ -147,8 +262,12 @@ __global__ __launch_bounds__(ASM_WG) void k_assemble_hex(const double *__restric
                                                          int accumulate, int nchunks)
 {
     __shared__ double s_blk[HEX_CAP * 8][9];
+    __shared__ uint16_t s_ct[HEX_CAP * 8];
     const int chunk = xcd_remap(blockIdx.x, nchunks);
     const int p0 = chunk_pair[chunk], p1 = chunk_pair[chunk + 1];
+    const int b0 = chunk_blk[chunk], b1 = chunk_blk[chunk + 1];
+    const int t0 = contrib_ptr[b0], nct = contrib_ptr[b1] - t0;  // <= 8 (p1 - p0)
+    for (int t = threadIdx.x; t < nct; t += ASM_WG) s_ct[t] = contrib[t0 + t];
     if (p1 > p0) {  // uniform over the workgroup
         const int lp = threadIdx.x >> 3, gp = threadIdx.x & 7;
         const int p = min(p0 + lp, p1 - 1);  // idle groups recompute the last pair, store nothing
@@ -190,7 +309,7 @@ __global__ __launch_bounds__(ASM_WG) void k_assemble_hex(const double *__restric
     }
     __syncthreads();
     HexSource src{s_blk};
-    gather_blocks(chunk_blk[chunk], chunk_blk[chunk + 1], blk_rk, contrib_ptr, contrib, col_ptr, val, accumulate, src);
+    gather_blocks(b0, b1, t0, blk_rk, contrib_ptr, s_ct, col_ptr, val, accumulate, src);
 }
 
 int launch_assemble(ofea_ctx *ctx, double E, double nu, double *d_out, bool zero_first)
@@ -204,8 +323,8 @@ int launch_assemble(ofea_ctx *ctx, double E, double nu, double *d_out, bool zero
         if (dp.nchunks == 0) continue;
         if (t == OFEA_C3D4)
             hipLaunchKernelGGL(k_assemble_tet, dim3(dp.nchunks), dim3(ASM_WG), 0, ctx->stream, ctx->d_xyz, dp.conn, dp.E,
-                               E, nu, dp.chunk_pair, dp.chunk_blk, dp.pair, dp.blk_rk, dp.contrib_ptr, dp.contrib,
-                               ctx->d_col_ptr, d_out, acc, dp.nchunks);
+                               E, nu, dp.chunk_pair, dp.chunk_blk, dp.chunk_node, dp.node_pair, dp.pair, dp.blk_rk,
+                               dp.contrib_ptr, dp.contrib, ctx->d_col_ptr, ctx->d_diag_k, d_out, acc, dp.nchunks);
         else
             hipLaunchKernelGGL(k_assemble_hex, dim3(dp.nchunks), dim3(ASM_WG), 0, ctx->stream, ctx->d_xyz, dp.conn, dp.E,
                                E, nu, dp.chunk_pair, dp.chunk_blk, dp.pair, dp.blk_rk, dp.contrib_ptr, dp.contrib,

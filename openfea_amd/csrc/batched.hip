@@ -30,8 +30,8 @@
 namespace ofea {
 
 constexpr int BA_WG = 256;
-constexpr int BA_HEX_CAP = 32;
-constexpr int BA_TET_CAP = 256;
+constexpr int BA_HEX_CAP = PLAN_HEX_CAP;
+constexpr int BA_TET_CAP = PLAN_TET_CAP;
 constexpr int BA_VMAX = 64;  // variants per launch (LDS staging of the moduli)
 constexpr int BA_VC = 4;     // variants accumulated per pass over the contributors
 
@@ -141,16 +141,15 @@ __global__ __launch_bounds__(BA_WG) void k_assemble_tet_batch(
 {
     // 256 pairs x 64 moduli would not fit beside the gradients in LDS: the
     // moduli are read from global memory (the element's row of nvar values).
-    __shared__ double s_g[BA_TET_CAP][12];
+    __shared__ double s_g[BA_TET_CAP][12];  // slot j: gradient of local node (a + j) & 3
     __shared__ double s_w[BA_TET_CAP][2];
-    __shared__ int s_a[BA_TET_CAP];
     __shared__ int s_el[BA_TET_CAP];
     const int chunk = xcd_remap(blockIdx.x, nchunks);
     const int p0 = chunk_pair[chunk], p1 = chunk_pair[chunk + 1];
     double lam, mu;
     lame<double>(1.0, nu, lam, mu);
     for (int p = p0 + (int)threadIdx.x; p < p1; p += BA_WG) {
-        const int pk = pair[p], el = pk >> 3, lp = p - p0;
+        const int pk = pair[p], el = pk >> 3, a = pk & 7, lp = p - p0;
         const int *cn = conn + 4 * el;
         double X[12];
 #pragma unroll
@@ -162,10 +161,9 @@ __global__ __launch_bounds__(BA_WG) void k_assemble_tet_batch(
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) s_g[lp][3 * b + c] = g[c][b];
+            for (int c = 0; c < 3; ++c) s_g[lp][3 * ((b - a) & 3) + c] = g[c][b];
         s_w[lp][0] = lam * V;
         s_w[lp][1] = mu * V;
-        s_a[lp] = pk & 7;
         s_el[lp] = el;
     }
     __syncthreads();
@@ -182,9 +180,9 @@ __global__ __launch_bounds__(BA_WG) void k_assemble_tet_batch(
 #pragma unroll
                 for (int i = 0; i < 9; ++i) acc[v][i] = 0.0;
             for (int t = t0; t < t1; ++t) {
-                const int ci = contrib[t], lp = ci >> 2, b = ci & 3, aa = s_a[lp];
-                const double ga[3] = {s_g[lp][3 * aa], s_g[lp][3 * aa + 1], s_g[lp][3 * aa + 2]};
-                const double gb[3] = {s_g[lp][3 * b], s_g[lp][3 * b + 1], s_g[lp][3 * b + 2]};
+                const int ci = contrib[t], lp = ci >> 2, j = ci & 3;
+                const double ga[3] = {s_g[lp][0], s_g[lp][1], s_g[lp][2]};
+                const double gb[3] = {s_g[lp][3 * j], s_g[lp][3 * j + 1], s_g[lp][3 * j + 2]};
                 const double wl = s_w[lp][0], wm = s_w[lp][1];
                 const double dm = wm * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
                 double blk[9];

@@ -35,6 +35,7 @@ static void free_mesh(ofea_ctx *c)
     dev_free(c->d_col_ptr);
     dev_free(c->d_node_blk);
     dev_free(c->d_bcol);
+    dev_free(c->d_diag_k);
     dev_free(c->d_resort);
     dev_free(c->d_val);
     dev_free(c->d_rhs);
@@ -118,6 +119,7 @@ int ofea_set_mesh(ofea_ctx *c, int nd, const double *xyz, int ne, const int *con
     if ((rc = dev_upload(c->d_col_ptr, c->P.col_ptr.data(), c->P.col_ptr.size()))) return rc;
     if ((rc = dev_upload(c->d_node_blk, c->P.node_blk.data(), c->P.node_blk.size()))) return rc;
     if ((rc = dev_upload(c->d_bcol, c->P.bcol.data(), c->P.bcol.size()))) return rc;
+    if ((rc = dev_upload(c->d_diag_k, c->P.diag_k.data(), std::max<size_t>(c->P.diag_k.size(), 1)))) return rc;
     if ((rc = dev_upload(c->d_resort, c->resort.data(), c->resort.size()))) return rc;
     if ((rc = dev_upload<double>(c->d_val, nullptr, (size_t)std::max(c->P.nnz, 1LL)))) return rc;
     if ((rc = dev_upload<double>(c->d_rhs, nullptr, (size_t)std::max(c->P.n, 1)))) return rc;

@@ -314,8 +314,11 @@ int build_pattern(int nd, int ne, const int *conn, const int *etype, const int *
     P.col_ptr.assign(P.n + 1, 0);
     P.row_idx.resize(P.nnz);
     P.bcol.resize(nblk);
+    P.diag_k.assign(nfree, -1);
     for (int r = 0; r < nfree; ++r) {
         int deg = (int)adj[r].size();
+        auto it = std::lower_bound(adj[r].begin(), adj[r].end(), r);
+        if (it != adj[r].end() && *it == r) P.diag_k[r] = (int)(it - adj[r].begin());
         int base = 9 * P.node_blk[r];
         for (int k = 0; k < deg; ++k) P.bcol[P.node_blk[r] + k] = adj[r][k];
         for (int c = 0; c < 3; ++c) {
@@ -391,12 +394,14 @@ int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const i
         const int *nbr = P.bcol.data() + P.node_blk[r];
         lists.assign(deg, std::vector<uint16_t>());
         for (int t = cnt[r]; t < cnt[r + 1]; ++t) {
-            int el = pairs[t] >> 3;
+            int el = pairs[t] >> 3, a = pairs[t] & 7;
             for (int b = 0; b < nn; ++b) {
                 int s = resort[G.conn[(size_t)el * nn + b]];
                 if (s < 0) continue;
                 int k = (int)(std::lower_bound(nbr, nbr + deg, s) - nbr);
-                lists[k].push_back((uint16_t)((t - chunk_first_pair) * nn + b));
+                // C3D4 kernels keep the pair's gradients rotated so that slot 0 is the owned node
+                int slot = nn == 4 ? ((b - a) & 3) : b;
+                lists[k].push_back((uint16_t)((t - chunk_first_pair) * nn + slot));
             }
         }
         for (int k = 0; k < deg; ++k) {

@@ -22,8 +22,8 @@
 namespace ofea {
 
 constexpr int NL_WG = 256;
-constexpr int NL_TET_CAP = 256;
-constexpr int NL_HEX_CAP = 32;
+constexpr int NL_TET_CAP = PLAN_TET_CAP;
+constexpr int NL_HEX_CAP = PLAN_HEX_CAP;
 
 struct NLPoint {
     double F[3][3], S[3][3], FFt[3][3];
@@ -246,18 +246,17 @@ __device__ inline void load_tet_state(const double *st, NLPoint &P)
 }
 
 struct NLTetSource {
-    const double (*g)[12];
+    const double (*g)[12];   // slot j: gradient of local node (a + j) & 3
     const double (*st)[16];  // F (9), S (xx yy zz xy yz xz), w
-    const int *a;
     double lam, mu;
     __device__ void add(int ci, double acc[9]) const
     {
-        int lp = ci >> 2, b = ci & 3, aa = a[lp];
+        int lp = ci >> 2, j = ci & 3;
         NLPoint P;
         load_tet_state(st[lp], P);
         double w = st[lp][15];
-        double ga[3] = {g[lp][3 * aa], g[lp][3 * aa + 1], g[lp][3 * aa + 2]};
-        double gb[3] = {g[lp][3 * b], g[lp][3 * b + 1], g[lp][3 * b + 2]};
+        double ga[3] = {g[lp][0], g[lp][1], g[lp][2]};
+        double gb[3] = {g[lp][3 * j], g[lp][3 * j + 1], g[lp][3 * j + 2]};
         double out[9];
         nl_block(P, ga, gb, w * lam, w * mu, w, out);
 #pragma unroll
@@ -277,7 +276,6 @@ __global__ __launch_bounds__(NL_WG) void k_nl_tet(const double *__restrict__ xyz
 {
     __shared__ double s_g[NL_TET_CAP][12];
     __shared__ double s_st[NL_TET_CAP][16];
-    __shared__ int s_a[NL_TET_CAP];
     const int chunk = xcd_remap(blockIdx.x, nchunks);
     const int p0 = chunk_pair[chunk], p1 = chunk_pair[chunk + 1];
     for (int p = p0 + (int)threadIdx.x; p < p1; p += NL_WG) {
@@ -301,7 +299,7 @@ __global__ __launch_bounds__(NL_WG) void k_nl_tet(const double *__restrict__ xyz
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) s_g[lp][3 * b + c] = g[c][b];
+            for (int c = 0; c < 3; ++c) s_g[lp][3 * ((b - a) & 3) + c] = g[c][b];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -313,17 +311,15 @@ __global__ __launch_bounds__(NL_WG) void k_nl_tet(const double *__restrict__ xyz
         s_st[lp][13] = P.S[1][2];
         s_st[lp][14] = P.S[0][2];
         s_st[lp][15] = w;
-        s_a[lp] = a;
     }
     __syncthreads();
-    NLTetSource src{s_g, s_st, s_a, lam, mu};
+    NLTetSource src{s_g, s_st, lam, mu};
     nl_gather_blocks(chunk_blk[chunk], chunk_blk[chunk + 1], blk_rk, contrib_ptr, contrib, col_ptr, val, accumulate, src);
     // f_a = w F S g_a from the staged point state
     auto pf = [&](int lp, double f[3]) {
         NLPoint P;
         load_tet_state(s_st[lp], P);
-        const int aa = s_a[lp];
-        const double ga[3] = {s_g[lp][3 * aa], s_g[lp][3 * aa + 1], s_g[lp][3 * aa + 2]};
+        const double ga[3] = {s_g[lp][0], s_g[lp][1], s_g[lp][2]};
         const double w = s_st[lp][15];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {

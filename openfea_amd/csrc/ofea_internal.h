@@ -34,6 +34,7 @@ struct Pattern {
     std::vector<int> row_idx;      // nnz
     std::vector<int> node_blk;     // nfree + 1: first 3x3 block of node r (= col_ptr[3r] / 9)
     std::vector<int> bcol;         // nnz / 9: neighbour rank of each block, (r, k) order
+    std::vector<int> diag_k;       // nfree: k of the diagonal block (r, r) of node r, -1 if none
 };
 
 // Column-owner gather plan for one element type.  The free nodes are cut into
@@ -52,7 +53,7 @@ struct GatherPlan {
     std::vector<int> pair;               // (elem_local << 3) | a, pairs sorted by (node, elem)
     std::vector<int> blk_rk;             // (r << 8) | k per block of the chunk list
     std::vector<int> contrib_ptr;        // blocks + 1
-    std::vector<uint16_t> contrib;       // chunk-local pair * nn + b
+    std::vector<uint16_t> contrib;       // chunk-local pair * nn + slot: C3D8 slot = b, C3D4 slot = (b - a) & 3
     std::vector<int> node_pair;          // nfree + 1: pairs of node r = [node_pair[r], node_pair[r+1])
     std::vector<int> chunk_node;         // nchunks + 1: first free node of each chunk
     int nchunks() const { return (int)chunk_pair.size() - 1; }
@@ -61,6 +62,12 @@ struct GatherPlan {
 int build_pattern(int nd, int ne, const int *conn, const int *etype, const int *resort, Pattern &P);
 int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const int *resort, const Pattern &P,
                       int type, int pair_cap, GatherPlan &G);
+
+// Pairs per gather chunk, shared by every kernel that walks a GatherPlan:
+// C3D4 one pair per thread of a 256-thread workgroup, C3D8 one pair per
+// 8-lane Gauss-point group.
+constexpr int PLAN_TET_CAP = 256;
+constexpr int PLAN_HEX_CAP = 32;
 
 // ---------------------------------------------------------------- device plan copy
 struct DevPlan {
@@ -100,7 +107,7 @@ struct ofea_ctx {
     std::vector<double> E_elem;  // empty = global E
     // device
     double *d_xyz = nullptr;
-    int *d_col_ptr = nullptr, *d_node_blk = nullptr, *d_bcol = nullptr;
+    int *d_col_ptr = nullptr, *d_node_blk = nullptr, *d_bcol = nullptr, *d_diag_k = nullptr;
     int *d_resort = nullptr;
     double *d_val = nullptr;    // FP64 values, nnz
     ofea::DevPlan dp[2];
