@@ -285,6 +285,43 @@ def bench_batched(args):
     alg_bytes = 24 * m.nd + 4 * 8 * m.ne + 8 * m.ne * nvar + 8 * ctx.nnz * nvar
     achieved = alg_bytes / (ms_step * 1e-3) / 1e9
 
+    batched_solve = None
+    if args.no_solve:
+        ctx.close()
+    else:
+        batched_solve = bench_batched_solve(args, ctx, m, Ev, B, nu)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cp, ri = O.build_csc(m.conn, m.etype, resort, 3 * int(resort.max() + 1))
+        k = 48
+        t = time.perf_counter()
+        for v in range(k):
+            O.fill_csc(m.conn, m.etype, m.xyz, resort, 0.0, nu, cp, ri, E_elem=Ev[v])
+        el = time.perf_counter() - t
+        cpu = {"value": k * m.ne / el, "unit": "variant-elems/s", "cores": 1, "kind": "port",
+               "sample": f"oracle fill_csc with per-element E for {k} of the 64 variants ({el:.1f}s)"}
+    return {
+        "metric": "batched assembled variant-elems/s + multi-RHS time-to-solve at rel-resid 1e-8",
+        "value": value, "unit": "variant-elems/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": wl, "elements": m.ne, "variants": nvar, "load_cases": nrhs, "dofs": int(B.shape[2]),
+                   "nnz": ctx.nnz},
+        "sequential_assembly_ms_64_variants": ms_seq,
+        "batched_solve": batched_solve,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("assemble_hex_batch_cfg4"),
+                     "kernel": "k_assemble_hex_batch", "algorithmic_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_batched_solve(args, ctx, m, Ev, B, nu):
+    import numpy as np
+
+    from openfea_amd import _capi as A
+
+    nvar, nrhs = B.shape[0], B.shape[1]
     # multi-RHS solves of all 512 systems (rel. residual 1e-8, FP64 vectors, FP32 block-Jacobi)
     ctx.solve_batch(B[:1], rtol=1e-8, copy=False)  # tile autotune + graph capture
     t = time.perf_counter()
@@ -298,7 +335,9 @@ def bench_batched(args):
     _, it8, _, ok8 = ctx.solve_batch(B[:nv8], rtol=1e-8, copy=False)
     t_b8 = time.perf_counter() - t
     seq = {}
-    for name, meth in (("plain_fp64_jacobi", A.PCG_JACOBI), ("fp64_block_jacobi", A.PCG_BLOCK_JACOBI)):
+    seq_all = {"plain_fp64_jacobi": A.PCG_JACOBI, "fp64_block_jacobi": A.PCG_BLOCK_JACOBI}
+    for name in [x for x in args.seq_methods.split(",") if x]:
+        meth = seq_all[name]
         tot, its = 0.0, []
         for v in range(nv8):
             ctx.set_elem_E(Ev[v])
@@ -316,33 +355,10 @@ def bench_batched(args):
         log(f"sequential {name} on {nv8}x{nrhs}: {tot:.3f}s (batched {t_b8:.3f}s)")
     ctx.set_elem_E(None)
     ctx.close()
-    cpu = None
-    if not args.no_cpu_baseline:
-        cp, ri = O.build_csc(m.conn, m.etype, resort, 3 * int(resort.max() + 1))
-        k = 16
-        t = time.perf_counter()
-        for v in range(k):
-            O.fill_csc(m.conn, m.etype, m.xyz, resort, 0.0, nu, cp, ri, E_elem=Ev[v])
-        el = time.perf_counter() - t
-        cpu = {"value": k * m.ne / el, "unit": "variant-elems/s", "cores": 1, "kind": "port",
-               "sample": f"oracle fill_csc with per-element E for {k} of the 64 variants ({el:.1f}s)"}
-    return {
-        "metric": "batched assembled variant-elems/s + multi-RHS time-to-solve at rel-resid 1e-8",
-        "value": value, "unit": "variant-elems/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": wl, "elements": m.ne, "variants": nvar, "load_cases": nrhs, "dofs": int(B.shape[2]),
-                   "nnz": ctx.nnz},
-        "sequential_assembly_ms_64_variants": ms_seq,
-        "batched_solve": {"systems": nvar * nrhs, "s": round(t_all, 4), "iters_min": int(it_all.min()),
-                          "iters_max": int(it_all.max()), "relres_max": float(rr_all.max()), "converged": ok_all,
-                          "lanes_per_node": g, "multi_rhs_spmv_ms": spmv_ms,
-                          "first_8_variants_s": round(t_b8, 4), "sequential_first_8_variants": seq},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("assemble_hex_batch_cfg4"),
-                     "kernel": "k_assemble_hex_batch", "algorithmic_bytes_per_launch": alg_bytes},
-        "cpu_baseline": cpu,
-    }
+    return {"systems": nvar * nrhs, "s": round(t_all, 4), "iters_min": int(it_all.min()),
+            "iters_max": int(it_all.max()), "relres_max": float(rr_all.max()), "converged": ok_all,
+            "lanes_per_node": g, "multi_rhs_spmv_ms": spmv_ms, "first_8_variants_s": round(t_b8, 4),
+            "sequential_first_8_variants": seq}
 
 
 def bench_nl(args):
@@ -362,7 +378,7 @@ def bench_nl(args):
     log(f"cfg5: ne={m.ne} n={ctx.n} nnz={ctx.nnz}")
     F1 = mg.free_rhs(mg.face_traction(m, (0.0, 0.0, 1.0)), resort)
     ctx.assemble(E, nu, copy=False)
-    x1, _, _, _ = ctx.solve(F1, method=A.PCG_BLOCK_JACOBI, rtol=1e-10)
+    x1, _, _, _ = ctx.solve(F1, method=A.PCG_JACOBI, rtol=1e-10)
     uf = ofa.reset_displacement(resort, x1).reshape(-1, 3)
     L_ = 80.0
     tip = np.abs(m.xyz[:, 0] - L_) < 1e-9
@@ -380,6 +396,43 @@ def bench_nl(args):
     ms_step = ctx.timer_stop() / args.steps
     value = m.ne / (ms_step * 1e-3)
     log(f"TL assembly {ms_step:.3f} ms -> {value:.3e} elems/s")
+    runs, train = {}, None
+    if not args.no_solve:
+        runs, train = bench_newton(args, ctx, m, resort, E, nu, F, tip, L_)
+    ctx.close()
+    alg_bytes = 24 * m.nd + 4 * 8 * m.ne + 8 * ctx.nnz + 3 * 8 * ctx.n
+    achieved = alg_bytes / (ms_step * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        cp, ri = O.build_csc(m.conn, m.etype, resort, 3 * int(resort.max() + 1))
+        us = np.random.default_rng(0).normal(scale=1e-3, size=len(cp) - 1)
+        t = time.perf_counter()
+        O.nl_fill(m.conn, m.etype, m.xyz, resort, E, nu, us, cp, ri)
+        el = time.perf_counter() - t
+        cpu = {"value": m.ne / el, "unit": "elems/s", "cores": 1, "kind": "port",
+               "sample": f"oracle nl_fill (TL tangent + internal force, reference nl fill algorithm) over the "
+                         f"full mesh, one pass ({el:.1f}s)"}
+    return {
+        "metric": "TL tangent assembled elems/s + Newton time-to-solve (learned vs plain PCG inner solves)",
+        "value": value, "unit": "elems/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": wl, "elements": m.ne, "dofs": ctx.n, "nnz": ctx.nnz, "p_star_MPa": p_star,
+                   "inner_rtol": args.inner_rtol},
+        "newton": runs, "learned_training": train,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("nl_assemble_hex_cfg5"),
+                     "kernel": "k_nl_hex", "algorithmic_bytes_per_launch": alg_bytes,
+                     "fp64_flops_per_launch": 185472 * m.ne,
+                     "fp64_tflops": 185472 * m.ne / (ms_step * 1e-3) / 1e12, "fp64_peak_tflops": FP64_PEAK_TFLOPS},
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_newton(args, ctx, m, resort, E, nu, F, tip, L_):
+    import openfea_amd as ofa
+    from openfea_amd import _capi as A, learned as L, meshgen as mg
+
     # learned correction trained on pairs from config-4 solves (40x20x20, design variant 0)
     t0 = time.perf_counter()
     m4 = mg.hex_block(40, 20, 20)
@@ -401,8 +454,10 @@ def bench_nl(args):
     ln = L.Learned(ctx, seed=0)
     ln.set_params(params)
     runs = {}
-    for name, meth in (("plain_fp64_jacobi", A.PCG_JACOBI), ("fp64_block_jacobi", A.PCG_BLOCK_JACOBI),
-                       ("learned_fp64", A.PCG_LEARNED)):
+    newton_all = {"plain_fp64_jacobi": A.PCG_JACOBI, "fp64_block_jacobi": A.PCG_BLOCK_JACOBI,
+                  "learned_fp64": A.PCG_LEARNED}
+    for name in [x for x in args.newton_methods.split(",") if x]:
+        meth = newton_all[name]
         ctx.sync()
         t = time.perf_counter()
         u, nt, lt, rel, ok = ctx.nl_solve(E, nu, F, n_incr=10, tol=1e-8, max_newton=25, method=meth,
@@ -412,41 +467,10 @@ def bench_nl(args):
         runs[name] = {"s": round(el, 3), "newton_iters": nt, "pcg_iters": lt, "relres": rel, "converged": ok,
                       "tip_uz_over_L": float(uz / L_)}
         log(f"newton {name}: {runs[name]}")
-    base = runs["plain_fp64_jacobi"]["s"]
+    base = runs.get("plain_fp64_jacobi", {}).get("s")
     for v in runs.values():
-        v["speedup_vs_plain"] = round(base / v["s"], 3)
-    ctx.close()
-    alg_bytes = 24 * m.nd + 4 * 8 * m.ne + 8 * ctx.nnz + 3 * 8 * ctx.n
-    achieved = alg_bytes / (ms_step * 1e-3) / 1e9
-    cpu = None
-    if not args.no_cpu_baseline:
-        ms_ = mg.hex_block(40, 20, 20)
-        rs_ = mg.free_index(ms_.nd, ms_.fix_set)
-        cp, ri = O.build_csc(ms_.conn, ms_.etype, rs_, 3 * int(rs_.max() + 1))
-        us = np.random.default_rng(0).normal(scale=1e-3, size=len(cp) - 1)
-        t = time.perf_counter()
-        k = 3
-        for _ in range(k):
-            O.nl_fill(ms_.conn, ms_.etype, ms_.xyz, rs_, E, nu, us, cp, ri)
-        el = time.perf_counter() - t
-        cpu = {"value": k * ms_.ne / el, "unit": "elems/s", "cores": 1, "kind": "port",
-               "sample": f"oracle nl_fill (TL tangent + internal force, reference nl fill algorithm) on a "
-                         f"40x20x20 slice of the same family, {k} reps ({el:.1f}s)"}
-    return {
-        "metric": "TL tangent assembled elems/s + Newton time-to-solve (learned vs plain PCG inner solves)",
-        "value": value, "unit": "elems/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": wl, "elements": m.ne, "dofs": ctx.n, "nnz": ctx.nnz, "p_star_MPa": p_star,
-                   "inner_rtol": args.inner_rtol},
-        "newton": runs, "learned_training": train,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("nl_assemble_hex_cfg5"),
-                     "kernel": "k_nl_hex", "algorithmic_bytes_per_launch": alg_bytes,
-                     "fp64_flops_per_launch": 185472 * m.ne,
-                     "fp64_tflops": 185472 * m.ne / (ms_step * 1e-3) / 1e12, "fp64_peak_tflops": FP64_PEAK_TFLOPS},
-        "cpu_baseline": cpu,
-    }
+        v["speedup_vs_plain"] = round(base / v["s"], 3) if base else None
+    return runs, train
 
 
 def main():
@@ -457,6 +481,10 @@ def main():
     ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3", "cfg4", "cfg5"])
     ap.add_argument("--explicit-steps", type=int, default=1000)
     ap.add_argument("--inner-rtol", type=float, default=1e-6, help="Newton inner PCG tolerance (cfg5)")
+    ap.add_argument("--seq-methods", default="plain_fp64_jacobi,fp64_block_jacobi",
+                    help="one-system-at-a-time solvers compared with the batched solve (cfg4)")
+    ap.add_argument("--newton-methods", default="plain_fp64_jacobi,fp64_block_jacobi,learned_fp64",
+                    help="inner PCG methods of the Newton runs (cfg5)")
     ap.add_argument("--solve-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true", help="assembly only (for counter collection)")
