@@ -128,7 +128,10 @@ def bench_static(args, workload):
     solves = {}
     all_methods = {"plain_fp64_jacobi": A.PCG_JACOBI, "fp64_block_jacobi": A.PCG_BLOCK_JACOBI,
                    "mixed_fp32val_block_jacobi": A.PCG_MIXED, "mixed_fp32_all_block_jacobi": A.PCG_MIXED32,
-                   "learned_fp64": A.PCG_LEARNED, "learned_mixed_fp32val": A.PCG_LEARNED_MIXED}
+                   "learned_fp64": A.PCG_LEARNED, "learned_mixed_fp32val": A.PCG_LEARNED_MIXED,
+                   "two_level_fp64": A.PCG_TWO_LEVEL, "two_level_mixed_fp32val": A.PCG_TWO_LEVEL_MIXED,
+                   "learned_two_level_fp64": A.PCG_LEARNED_TWO_LEVEL,
+                   "learned_two_level_mixed_fp32val": A.PCG_LEARNED_TWO_LEVEL_MIXED}
     chosen = [] if args.no_solve else [m for m in args.solvers.split(",") if m]
     methods = [(m, all_methods[m]) for m in chosen]
     learned_info = None
@@ -138,14 +141,18 @@ def bench_static(args, workload):
         ln.set_params(learned_info.pop("params"))
         ctx.set_rhs(F)
     for name, method in methods:
-        ctx.solve(None, method=method, rtol=1e-8, maxit=100000, copy=False)  # warm (graph capture)
+        ctx.sync()
+        t = time.perf_counter()
+        ctx.solve(None, method=method, rtol=1e-8, maxit=100000, copy=False)  # warm (setup, graph capture)
+        first_ms = (time.perf_counter() - t) * 1e3
         ts = []
         for _ in range(args.solve_reps):
             ctx.sync()
             t = time.perf_counter()
             _, it, rel, ok = ctx.solve(None, method=method, rtol=1e-8, maxit=100000, copy=False)
             ts.append(time.perf_counter() - t)
-        solves[name] = {"ms": round(statistics.median(ts) * 1e3, 3), "iters": it, "relres": rel, "converged": ok}
+        solves[name] = {"ms": round(statistics.median(ts) * 1e3, 3), "iters": it, "relres": rel, "converged": ok,
+                        "first_call_ms_incl_setup": round(first_ms, 3)}
         log(f"solve {name}: {solves[name]}")
     base = solves.get("plain_fp64_jacobi", {}).get("ms")
     for k, v in solves.items():
@@ -489,7 +496,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true", help="assembly only (for counter collection)")
     ap.add_argument("--solvers", default="plain_fp64_jacobi,fp64_block_jacobi,mixed_fp32val_block_jacobi,"
-                    "mixed_fp32_all_block_jacobi,learned_fp64,learned_mixed_fp32val")
+                    "mixed_fp32_all_block_jacobi,learned_fp64,learned_mixed_fp32val,two_level_fp64,"
+                    "two_level_mixed_fp32val,learned_two_level_fp64,learned_two_level_mixed_fp32val")
     ap.add_argument("--train-steps", type=int, default=150)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -56,6 +56,13 @@ extern "C" {
 #define OFEA_PCG_LEARNED 3       /* FP64 PCG with the learned block-Jacobi correction            */
 #define OFEA_PCG_MIXED32 4       /* FP32 values and FP32 vectors, block Jacobi, FP64 reliable updates */
 #define OFEA_PCG_LEARNED_MIXED 5 /* as MIXED with the learned block-Jacobi correction            */
+/* two-level methods: the local part (3x3 block Jacobi or the learned blocks)
+ * plus an additive coarse correction Z A_c^-1 Z^T r over rigid-body modes of
+ * node aggregates (FP32 coarse inverse, set up from the resident K) */
+#define OFEA_PCG_TWO_LEVEL 6                /* FP64 values, block Jacobi + coarse             */
+#define OFEA_PCG_TWO_LEVEL_MIXED 7          /* FP32 values, FP64 vectors, reliable updates    */
+#define OFEA_PCG_LEARNED_TWO_LEVEL 8        /* FP64 values, learned blocks + coarse           */
+#define OFEA_PCG_LEARNED_TWO_LEVEL_MIXED 9  /* FP32 values, learned blocks + coarse           */
 
 /* explicit precision for ofea_explicit_run */
 #define OFEA_FP32 0
@@ -178,6 +185,10 @@ int ofea_learned_train(ofea_ctx *ctx, int steps, float lr, double *loss_hist);
 int ofea_learned_blocks(ofea_ctx *ctx, double *blocks, float *features);
 /* recompute features and blocks for a newly assembled K */
 int ofea_learned_refresh(ofea_ctx *ctx);
+/* out = Z A_c^-1 Z^T r (n), the coarse correction of the two-level methods for
+ * the resident K (built on first use).  Training pairs of the learned
+ * two-level smoother use e - out as the target. */
+int ofea_coarse_apply(ofea_ctx *ctx, const double *r, double *out);
 
 /* ---- batched reanalysis (no counterpart in the reference) ----
  * Many design variants of one mesh: variant v has per-element moduli

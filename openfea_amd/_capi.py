@@ -31,6 +31,10 @@ PCG_MIXED = 2
 PCG_LEARNED = 3
 PCG_MIXED32 = 4
 PCG_LEARNED_MIXED = 5
+PCG_TWO_LEVEL = 6
+PCG_TWO_LEVEL_MIXED = 7
+PCG_LEARNED_TWO_LEVEL = 8
+PCG_LEARNED_TWO_LEVEL_MIXED = 9
 
 FP32 = 0
 FP64 = 1
@@ -80,6 +84,7 @@ SIGNATURES = {
     "ofea_learned_train": (_i, [_vp, _i, _f, _vp]),
     "ofea_learned_blocks": (_i, [_vp, _vp, _vp]),
     "ofea_learned_refresh": (_i, [_vp]),
+    "ofea_coarse_apply": (_i, [_vp, _vp, _vp]),
     "ofea_assemble_batch": (_i, [_vp, _i, _vp, _d, _vp]),
     "ofea_solve_batch": (_i, [_vp, _i, _i, _vp, _vp, _d, _i, _vp, _vp]),
     "ofea_autotune": (_i, [_vp, _pi, _vp]),

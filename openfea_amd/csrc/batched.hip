@@ -334,7 +334,7 @@ __device__ inline void mb_sum_partials(int nq, Src src, double *s_out)
     for (int q = wave; q < nq; q += MB_NW) {
         const double *p = src(q);
         double v = 0.0;
-        for (int i = lane; i < MB_NB; i += 64) v += __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int i = lane; i < MB_NB; i += 64) v += p[i];  // written by an earlier kernel on the stream
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
         if (lane == 0) s_out[q] = v;
     }

@@ -397,6 +397,14 @@ int ofea_batch_tiles(ofea_ctx *c, int nrhs, int *group, float *ms)
     return batch_tiles(c, nrhs, group, ms);
 }
 
+int ofea_coarse_apply(ofea_ctx *c, const double *r, double *out)
+{
+    CHECK_CTX(c);
+    if (!c->assembled) return fail(OFEA_ERR_STATE, "ofea_assemble first");
+    if (!r || !out) return fail(OFEA_ERR_ARG, "null r or out");
+    return coarse_apply(c, r, out);
+}
+
 int ofea_sync(ofea_ctx *c)
 {
     CHECK_CTX(c);

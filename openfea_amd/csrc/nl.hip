@@ -479,7 +479,8 @@ int nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr, doub
             int lit = 0;
             double lrel = 0.0;
             int m = method;
-            if (m == OFEA_PCG_LEARNED || m == OFEA_PCG_LEARNED_MIXED) {
+            if (m == OFEA_PCG_LEARNED || m == OFEA_PCG_LEARNED_MIXED || m == OFEA_PCG_LEARNED_TWO_LEVEL ||
+                m == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED) {
                 if (!c->learned || learned_refresh(c) != OFEA_OK) m = OFEA_PCG_BLOCK_JACOBI;
             }
             int src = solver_solve(c, m, inner_rtol, 20000, &lit, &lrel);

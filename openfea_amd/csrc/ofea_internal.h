@@ -166,6 +166,8 @@ int batch_tiles(ofea_ctx *c, int nrhs, int *group, float *ms);
 void batch_free(ofea_ctx *c);
 // SpMV of K (FP64) with lane-group size `group`, averaged over reps launches (solver.hip)
 int solver_spmv_group_time(ofea_ctx *ctx, int group, int reps, float *ms);
+// two-level coarse correction Z A_c^-1 Z^T r on the host (solver.hip)
+int coarse_apply(ofea_ctx *ctx, const double *r, double *out);
 
 template <class T>
 int dev_upload(T *&d, const T *h, size_t count)

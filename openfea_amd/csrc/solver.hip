@@ -22,7 +22,9 @@
 // replacement ("reliable updates"); convergence is only accepted on the true
 // FP64 residual.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "elem_device.h"
@@ -155,11 +157,15 @@ struct CGParts {
     double *pq;  // CG_NB: <p,Kp>
 };
 
-// Sum over the CG_WG threads of a workgroup (fixed tree), result in every thread.
-template <int NV>
+// Sum over the WG threads of a workgroup (fixed tree), result in every
+// thread: xor tree within each wave, then every group of WG/64 lanes reads the
+// wave sums and xor-reduces them (one LDS read per value and thread).
+template <int NV, int WG = CG_WG>
 __device__ inline void cg_block_sum(double v[NV])
 {
-    __shared__ double sm[NV][CG_WG / 64];
+    constexpr int NW = WG / 64;
+    static_assert(NW >= 1 && NW <= 64 && (NW & (NW - 1)) == 0, "power-of-two wave count");
+    __shared__ double sm[NV][NW];
 #pragma unroll
     for (int i = 0; i < NV; ++i)
         for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off);
@@ -168,24 +174,25 @@ __device__ inline void cg_block_sum(double v[NV])
 #pragma unroll
         for (int i = 0; i < NV; ++i) sm[i][threadIdx.x >> 6] = v[i];
     __syncthreads();
+    const int l = threadIdx.x & (NW - 1);
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
-        double t = 0.0;
-        for (int k = 0; k < CG_WG / 64; ++k) t += sm[i][k];
+        double t = sm[i][l];
+        for (int off = NW >> 1; off > 0; off >>= 1) t += __shfl_xor(t, off);
         v[i] = t;
     }
 }
 
 // Totals of NV partial arrays (CG_NB entries each), same value everywhere.
-template <int NV>
+// The partials were written by an earlier kernel on the same stream; the
+// kernel boundary makes them visible, so plain (L2-cached) loads suffice.
+template <int NV, int WG = CG_WG>
 __device__ inline void red_sums(const double *const arr[NV], double out[NV])
 {
+    static_assert(WG >= CG_NB, "one partial per thread");
 #pragma unroll
-    for (int i = 0; i < NV; ++i)
-        out[i] = threadIdx.x < CG_NB
-                     ? __hip_atomic_load(&arr[i][threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                     : 0.0;
-    cg_block_sum<NV>(out);
+    for (int i = 0; i < NV; ++i) out[i] = threadIdx.x < CG_NB ? arr[i][threadIdx.x] : 0.0;
+    cg_block_sum<NV, WG>(out);
 }
 
 // One partial per workgroup and array (thread 0 writes).
@@ -293,12 +300,57 @@ __device__ inline void apply_pc(int r, const T *rr, T *zz, const double *dinv, c
 
 // alpha = <r,z>/<p,q>; x += alpha p (x in XT); r -= alpha q; z = M r;
 // partials <r,z> (into the other parity slot) and <r,r>.  Thread per node.
-template <class T, class XT, int PC>
-__global__ __launch_bounds__(CG_WG) void k_cg_update(int bi, int nfree, XT *__restrict__ x, T *__restrict__ r,
+// Per-workgroup partials <r,z>, <r,r> (at blockIdx) and, with the coarse
+// space, Z^T r of the workgroup's node range (at its range index).
+template <bool CO, int WG>
+__device__ inline void put_rz_rr_g(double rzn, double rr2, const double g[6], double *rz_dst, double *rr_dst,
+                                   const CoarseArgs &C)
+{
+    if constexpr (CO) {
+        double v[8] = {rzn, rr2, g[0], g[1], g[2], g[3], g[4], g[5]};
+        cg_block_sum<8, WG>(v);
+        if (threadIdx.x == 0) {
+            rz_dst[blockIdx.x] = v[0];
+            rr_dst[blockIdx.x] = v[1];
+            const int b = xcd_remap(blockIdx.x, gridDim.x);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) C.g[c * CG_NB + b] = v[2 + c];
+        }
+    } else {
+        double v[2] = {rzn, rr2};
+        cg_block_sum<2, WG>(v);
+        if (threadIdx.x == 0) {
+            rz_dst[blockIdx.x] = v[0];
+            rr_dst[blockIdx.x] = v[1];
+        }
+    }
+}
+
+// g += Z_i^T r_i = (r_i, d_i x r_i)
+template <class T>
+__device__ inline void add_zt_r(const CoarseArgs &C, int i, const T rv[3], double g[6])
+{
+    const double dx = C.d[3 * i], dy = C.d[3 * i + 1], dz = C.d[3 * i + 2];
+    const double r0 = rv[0], r1 = rv[1], r2 = rv[2];
+    g[0] += r0;
+    g[1] += r1;
+    g[2] += r2;
+    g[3] += dy * r2 - dz * r1;
+    g[4] += dz * r0 - dx * r2;
+    g[5] += dx * r1 - dy * r0;
+}
+
+// The update kernels run 512-thread workgroups on the CG grid: a node range
+// rarely exceeds 512 nodes, and the smaller workgroup keeps the whole grid
+// resident at up to 80 VGPRs.
+constexpr int UP_WG = 512;
+
+template <class T, class XT, int PC, bool CO>
+__global__ __launch_bounds__(UP_WG) void k_cg_update(int bi, int nfree, XT *__restrict__ x, T *__restrict__ r,
                                                      T *__restrict__ z, const T *__restrict__ pA,
                                                      const T *__restrict__ pB, const T *__restrict__ q,
                                                      const double *__restrict__ dinv, const T *__restrict__ minv,
-                                                     CGParts P, DevScalars *sc)
+                                                     CGParts P, CoarseArgs C, DevScalars *sc)
 {
     if (sc->done) return;
     const int par = sc->par0 ^ (bi & 1);
@@ -317,7 +369,7 @@ __global__ __launch_bounds__(CG_WG) void k_cg_update(int bi, int nfree, XT *__re
     double red[2];
     {
         const double *const arr[2] = {P.pq, P.rz + par * CG_NB};
-        red_sums<2>(arr, red);
+        red_sums<2, UP_WG>(arr, red);
     }
     const double pq = red[0], rz = red[1];
     if (!(pq > 0.0) || !isfinite(pq)) {  // breakdown: K or M not SPD
@@ -328,9 +380,9 @@ __global__ __launch_bounds__(CG_WG) void k_cg_update(int bi, int nfree, XT *__re
         return;
     }
     const double alpha = rz / pq;
-    double rzn = 0.0, rr2 = 0.0;
-    for (; i < i1; i += CG_WG) {
-        if (i >= i0 + CG_WG)  // later passes (ranges longer than one workgroup) load here
+    double rzn = 0.0, rr2 = 0.0, g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (; i < i1; i += UP_WG) {
+        if (i >= i0 + UP_WG)  // later passes (ranges longer than one workgroup) load here
             for (int c = 0; c < 3; ++c) {
                 pv[c] = p[3 * i + c];
                 qv[c] = q[3 * i + c];
@@ -349,10 +401,77 @@ __global__ __launch_bounds__(CG_WG) void k_cg_update(int bi, int nfree, XT *__re
             rzn += (double)rv[c] * (double)zv[c];
             rr2 += (double)rv[c] * (double)rv[c];
         }
+        if constexpr (CO) add_zt_r(C, i, rv, g);
     }
-    double part[2] = {rzn, rr2};
-    double *const dst[2] = {P.rz + (par ^ 1) * CG_NB, P.rr};
-    put_partials<2>(part, dst);
+    put_rz_rr_g<CO, UP_WG>(rzn, rr2, g, P.rz + (par ^ 1) * CG_NB, P.rr, C);
+}
+
+// Coarse correction, one workgroup per aggregate a: sums the per-range
+// partials of Z^T r (all aggregates, same order in every workgroup), forms
+// y_a = rows 6a..6a+5 of A_c^-1 (Z^T r), adds (Z y)_i = t_a + theta_a x d_i
+// to z_i for the nodes of aggregate a (z then holds the full two-level M r)
+// and adds the aggregate's <r, Z y> to one <r, z> partial.  mode 0: after the update of
+// batch index bi (slot of the next iteration); mode 1: after a residual
+// replacement (slot par0).
+template <class T>
+__global__ __launch_bounds__(CG_WG) void k_coarse(int bi, int mode, int nfree, int nagg, const float *__restrict__ ainv,
+                                                  const T *__restrict__ r, T *__restrict__ z, double *rz_part,
+                                                  CoarseArgs C, DevScalars *sc)
+{
+    // every load that does not depend on the scalars is issued before the
+    // done-check, so their latencies overlap
+    const int done = sc->done, par0 = sc->par0;
+    const int nc = 6 * nagg;  // <= 6 CG_NB / CG_AGG < CG_WG: one coarse column per thread
+    const int a = blockIdx.x;
+    const int i0 = min(a * CG_AGG * C.per, nfree), i1 = min(i0 + CG_AGG * C.per, nfree);
+    const int k = threadIdx.x;
+    float am[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    double gk = 0.0;
+    if (k < nc) {
+#pragma unroll
+        for (int i2 = 0; i2 < 6; ++i2) am[i2] = ainv[(size_t)(6 * a + i2) * nc + k];
+        const int aa = k / 6, c = k - 6 * aa;
+        const double *gp = C.g + c * CG_NB + aa * CG_AGG;  // written by the previous kernel
+        const int nb = min(CG_AGG, CG_NB - aa * CG_AGG);
+        for (int b = 0; b < nb; ++b) gk += gp[b];
+    }
+    int i = i0 + (int)threadIdx.x;
+    double dv[3] = {0.0, 0.0, 0.0}, zv[3] = {0.0, 0.0, 0.0}, rv[3] = {0.0, 0.0, 0.0};
+    if (i < i1)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            dv[c] = C.d[3 * i + c];
+            zv[c] = (double)z[3 * i + c];
+            rv[c] = (double)r[3 * i + c];
+        }
+    if (done) return;
+    const int slot = mode ? par0 : (par0 ^ (bi & 1) ^ 1);
+    double y[6];
+#pragma unroll
+    for (int i2 = 0; i2 < 6; ++i2) y[i2] = (double)am[i2] * gk;
+    cg_block_sum<6>(y);
+    double rzc = 0.0;
+    for (; i < i1; i += CG_WG) {
+        if (i >= i0 + CG_WG)  // aggregates longer than the workgroup
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                dv[c] = C.d[3 * i + c];
+                zv[c] = (double)z[3 * i + c];
+                rv[c] = (double)r[3 * i + c];
+            }
+        const double zc[3] = {y[0] + y[4] * dv[2] - y[5] * dv[1], y[1] + y[5] * dv[0] - y[3] * dv[2],
+                              y[2] + y[3] * dv[1] - y[4] * dv[0]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            z[3 * i + c] = (T)(zv[c] + zc[c]);
+            rzc += rv[c] * zc[c];
+        }
+    }
+    double v[1] = {rzc};
+    cg_block_sum<1>(v);
+    // the local part of <r, z> of this slot is complete (previous kernel); entry a
+    // gets the coarse part of aggregate a added, by this workgroup only
+    if (threadIdx.x == 0) rz_part[slot * CG_NB + a] += v[0];
 }
 
 // advances the host-visible base iteration after a full batch
@@ -366,16 +485,17 @@ __global__ void k_cg_batch_end(DevScalars *sc, int nbatch)
 // slot is set to +inf, so the next iteration takes beta = 0 (p = z).  At a
 // reliable update the other slot keeps <r,z> of the previous iteration, so
 // beta = <r,z>_new / <r,z>_prev and the search direction is kept.
-template <class T, int PC>
-__global__ __launch_bounds__(CG_WG) void k_replace(int nfree, const double *__restrict__ r64, T *__restrict__ r,
+template <class T, int PC, bool CO>
+__global__ __launch_bounds__(UP_WG) void k_replace(int nfree, const double *__restrict__ r64, T *__restrict__ r,
                                                    T *__restrict__ z, int restart, const double *__restrict__ dinv,
-                                                   const T *__restrict__ minv, CGParts P, DevScalars *sc)
+                                                   const T *__restrict__ minv, CGParts P, CoarseArgs C,
+                                                   DevScalars *sc)
 {
     const int par = sc->par0;
     int i0, i1;
     cg_range(nfree, i0, i1);
-    double rz = 0.0, rr2 = 0.0;
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += CG_WG) {
+    double rz = 0.0, rr2 = 0.0, g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += UP_WG) {
         T rv[3] = {(T)r64[3 * i], (T)r64[3 * i + 1], (T)r64[3 * i + 2]}, zv[3];
         apply_pc<T, PC>(i, rv, zv, dinv, minv);
         for (int c = 0; c < 3; ++c) {
@@ -384,10 +504,9 @@ __global__ __launch_bounds__(CG_WG) void k_replace(int nfree, const double *__re
             rz += (double)rv[c] * (double)zv[c];
             rr2 += (double)rv[c] * (double)rv[c];
         }
+        if constexpr (CO) add_zt_r(C, i, rv, g);
     }
-    double part[2] = {rz, rr2};
-    double *const dst[2] = {P.rz + par * CG_NB, P.rr};
-    put_partials<2>(part, dst);
+    put_rz_rr_g<CO, UP_WG>(rz, rr2, g, P.rz + par * CG_NB, P.rr, C);
     if (restart && threadIdx.x == 0) P.rz[(par ^ 1) * CG_NB + blockIdx.x] = blockIdx.x == 0 ? INFINITY : 0.0;
 }
 
@@ -497,6 +616,8 @@ void solver_free(ofea_ctx *ctx)
     for (float *p : {w->v32.r, w->v32.z, w->v32.p, w->v32.p2, w->v32.q, w->minv32, w->val32, w->learned32})
         if (p) (void)hipFree(p);
     if (w->learned64) (void)hipFree(w->learned64);
+    for (void *p : {(void *)w->co.g, (void *)w->co.d, (void *)w->co_ainv})
+        if (p) (void)hipFree(p);
     if (w->sc) (void)hipFree(w->sc);
     if (w->sc_host) (void)hipHostFree(w->sc_host);
     delete w;
@@ -505,7 +626,10 @@ void solver_free(ofea_ctx *ctx)
 
 void solver_invalidate(ofea_ctx *ctx)
 {
-    if (ctx->ws) ctx->ws->precond_valid = false;
+    if (ctx->ws) {
+        ctx->ws->precond_valid = false;
+        ctx->ws->co_valid = false;
+    }
 }
 
 template <class TV, class T, bool DOT>
@@ -586,7 +710,7 @@ int solver_prepare(ofea_ctx *ctx)
 }
 
 // The graph of one batch: CG_BATCH iterations (2 kernels each) + batch end.
-template <class TV, class T, class XT, int PC>
+template <class TV, class T, class XT, int PC, bool CO>
 static void cg_batch(ofea_ctx *ctx, const TV *val, VecSet<T> &v, XT *x, const T *minv)
 {
     ofea_solver_ws *w = ctx->ws;
@@ -606,8 +730,11 @@ static void cg_batch(ofea_ctx *ctx, const TV *val, VecSet<T> &v, XT *x, const T 
             OFEA_CG_CASE(64)
         }
 #undef OFEA_CG_CASE
-        hipLaunchKernelGGL((k_cg_update<T, XT, PC>), dim3(CG_NB), dim3(CG_WG), 0, ctx->stream, bi, w->nfree, x, v.r,
-                           v.z, v.p, v.p2, v.q, w->dinv, minv, P, w->sc);
+        hipLaunchKernelGGL((k_cg_update<T, XT, PC, CO>), dim3(CG_NB), dim3(UP_WG), 0, ctx->stream, bi, w->nfree, x,
+                           v.r, v.z, v.p, v.p2, v.q, w->dinv, minv, P, w->co, w->sc);
+        if (CO)
+            hipLaunchKernelGGL((k_coarse<T>), dim3(w->co_nagg), dim3(CG_WG), 0, ctx->stream, bi, 0, w->nfree, w->co_nagg,
+                               w->co_ainv, v.r, v.z, w->part_rz, w->co, w->sc);
     }
     hipLaunchKernelGGL(k_cg_batch_end, dim3(1), dim3(64), 0, ctx->stream, w->sc, CG_BATCH);
 }
@@ -642,7 +769,7 @@ static int write_scalars(ofea_ctx *ctx)
 // vector precision of the iteration; PC: 0 point Jacobi, 1 block preconditioner.
 // ru_factor > 0 requests a residual replacement each time the recursive
 // residual has dropped by that factor since the last replacement.
-template <class TV, class T, int PC>
+template <class TV, class T, int PC, bool CO>
 static int pcg_run(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T *minv, double rtol, int maxit,
                    double ru_factor, int *iters, double *relres)
 {
@@ -675,19 +802,29 @@ static int pcg_run(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T
     S.s[S_TOL2] = tol2;
     S.s[S_RU2] = ru_factor > 0.0 ? ru_factor * ru_factor * bb : -1.0;
     if ((rc = write_scalars(ctx))) return rc;
-    hipLaunchKernelGGL((k_replace<T, PC>), dim3(CG_NB), dim3(CG_WG), 0, ctx->stream, w->nfree, w->r64, v.r, v.z, 1,
-                       w->dinv, minv, P, w->sc);
+    hipLaunchKernelGGL((k_replace<T, PC, CO>), dim3(CG_NB), dim3(UP_WG), 0, ctx->stream, w->nfree, w->r64, v.r, v.z,
+                       1, w->dinv, minv, P, w->co, w->sc);
+    if (CO)
+        hipLaunchKernelGGL((k_coarse<T>), dim3(w->co_nagg), dim3(CG_WG), 0, ctx->stream, 0, 1, w->nfree, w->co_nagg,
+                           w->co_ainv, v.r, v.z, w->part_rz, w->co, w->sc);
     OFEA_HIP(hipGetLastError());
-    if (!w->exec[slot]) {
+    // OFEA_NO_GRAPH=1 launches the batches directly (for tools that cannot follow graph launches)
+    static const bool no_graph = std::getenv("OFEA_NO_GRAPH") && std::getenv("OFEA_NO_GRAPH")[0] == '1';
+    if (!w->exec[slot] && !no_graph) {
         OFEA_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        cg_batch<TV, T, double, PC>(ctx, val, v, x, minv);
+        cg_batch<TV, T, double, PC, CO>(ctx, val, v, x, minv);
         OFEA_HIP(hipStreamEndCapture(ctx->stream, &w->graph[slot]));
         OFEA_HIP(hipGraphInstantiate(&w->exec[slot], w->graph[slot], nullptr, nullptr, 0));
     }
     double rel = 1.0;
     int tighten = 0;
     for (;;) {
-        OFEA_HIP(hipGraphLaunch(w->exec[slot], ctx->stream));
+        if (no_graph) {
+            cg_batch<TV, T, double, PC, CO>(ctx, val, v, x, minv);
+            OFEA_HIP(hipGetLastError());
+        } else {
+            OFEA_HIP(hipGraphLaunch(w->exec[slot], ctx->stream));
+        }
         if ((rc = read_scalars(ctx))) return rc;
         const int done = S.done;
         if (done == 0) continue;
@@ -707,13 +844,194 @@ static int pcg_run(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T
         S.s[S_TOL2] = tol2;
         S.s[S_RU2] = ru_factor > 0.0 ? ru_factor * ru_factor * S.s[S_TRR] : -1.0;
         if ((rc = write_scalars(ctx))) return rc;
-        hipLaunchKernelGGL((k_replace<T, PC>), dim3(CG_NB), dim3(CG_WG), 0, ctx->stream, w->nfree, w->r64, v.r, v.z,
-                           0, w->dinv, minv, P, w->sc);
+        hipLaunchKernelGGL((k_replace<T, PC, CO>), dim3(CG_NB), dim3(UP_WG), 0, ctx->stream, w->nfree, w->r64, v.r,
+                           v.z, 0, w->dinv, minv, P, w->co, w->sc);
+        if (CO)
+            hipLaunchKernelGGL((k_coarse<T>), dim3(w->co_nagg), dim3(CG_WG), 0, ctx->stream, 0, 1, w->nfree, w->co_nagg,
+                               w->co_ainv, v.r, v.z, w->part_rz, w->co, w->sc);
         OFEA_HIP(hipGetLastError());
     }
     if (iters) *iters = S.it;
     if (relres) *relres = rel;
     return rel <= rtol ? OFEA_OK : OFEA_NOT_CONVERGED;
+}
+
+// Coarse space of the two-level methods, built on the host from the resident
+// K: aggregates of CG_AGG consecutive CG node ranges, Z = rigid-body modes
+// about each aggregate's centroid, A_c = Z^T K Z (FP64, 6 nagg square),
+// Cholesky-inverted and stored in FP32.
+static int coarse_prepare(ofea_ctx *ctx)
+{
+    ofea_solver_ws *w = ctx->ws;
+    if (w->co_valid) return OFEA_OK;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int nf = w->nfree;
+    const int per = (nf + CG_NB - 1) / CG_NB;
+    const int nranges = (nf + per - 1) / per;
+    const int nagg = (nranges + CG_AGG - 1) / CG_AGG;
+    const int nc = 6 * nagg;
+    // free-node coordinates and aggregate centroids
+    std::vector<double> X((size_t)3 * nf), cen((size_t)3 * nagg, 0.0);
+    std::vector<int> cnt(nagg, 0);
+    for (int i = 0; i < ctx->nd; ++i) {
+        const int r = ctx->resort[i];
+        if (r < 0) continue;
+        for (int c = 0; c < 3; ++c) X[3 * (size_t)r + c] = ctx->xyz[3 * (size_t)i + c];
+    }
+    auto agg = [&](int r) { return (r / per) / CG_AGG; };
+    for (int r = 0; r < nf; ++r) {
+        const int a = agg(r);
+        cnt[a]++;
+        for (int c = 0; c < 3; ++c) cen[3 * a + c] += X[3 * (size_t)r + c];
+    }
+    for (int a = 0; a < nagg; ++a)
+        for (int c = 0; c < 3; ++c) cen[3 * a + c] /= std::max(cnt[a], 1);
+    std::vector<float> d((size_t)3 * nf);
+    std::vector<double> dd((size_t)3 * nf);
+    for (int r = 0; r < nf; ++r)
+        for (int c = 0; c < 3; ++c) {
+            // the device applies Z with the FP32 offsets: build A_c from the same values
+            d[3 * (size_t)r + c] = (float)(X[3 * (size_t)r + c] - cen[3 * agg(r) + c]);
+            dd[3 * (size_t)r + c] = (double)d[3 * (size_t)r + c];
+        }
+    // A_c = sum over blocks K_sr (rows of s, columns of r) of Z_s^T K_sr Z_r, Z = [I | R], R = -[d]x
+    std::vector<double> val((size_t)ctx->P.nnz);
+    OFEA_HIP(hipMemcpy(val.data(), ctx->d_val, sizeof(double) * val.size(), hipMemcpyDeviceToHost));
+    std::vector<double> A((size_t)nc * nc, 0.0);
+    auto Rmat = [&](int node, double R[3][3]) {
+        const double x = dd[3 * (size_t)node], y = dd[3 * (size_t)node + 1], z = dd[3 * (size_t)node + 2];
+        // R theta = theta x d
+        R[0][0] = 0.0; R[0][1] = z;   R[0][2] = -y;
+        R[1][0] = -z;  R[1][1] = 0.0; R[1][2] = x;
+        R[2][0] = y;   R[2][1] = -x;  R[2][2] = 0.0;
+    };
+    for (int r = 0; r < nf; ++r) {
+        const int nb0 = ctx->P.node_blk[r], deg = ctx->P.node_blk[r + 1] - nb0;
+        const size_t base = 9 * (size_t)nb0;
+        const int ar = agg(r);
+        double Rr[3][3];
+        Rmat(r, Rr);
+        for (int k = 0; k < deg; ++k) {
+            const int s = ctx->P.bcol[nb0 + k];
+            const int as = agg(s);
+            double Ks[3][3];  // Ks[d][c] = K(3s+d, 3r+c)
+            for (int c = 0; c < 3; ++c)
+                for (int dq = 0; dq < 3; ++dq) Ks[dq][c] = val[base + (size_t)c * 3 * deg + 3 * k + dq];
+            double Rs[3][3];
+            Rmat(s, Rs);
+            double Zs[3][6], Zr[3][6];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    Zs[i][j] = i == j ? 1.0 : 0.0;
+                    Zr[i][j] = i == j ? 1.0 : 0.0;
+                    Zs[i][3 + j] = Rs[i][j];
+                    Zr[i][3 + j] = Rr[i][j];
+                }
+            double KZ[3][6];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 6; ++j)
+                    KZ[i][j] = Ks[i][0] * Zr[0][j] + Ks[i][1] * Zr[1][j] + Ks[i][2] * Zr[2][j];
+            for (int i = 0; i < 6; ++i)
+                for (int j = 0; j < 6; ++j)
+                    A[(size_t)(6 * as + i) * nc + 6 * ar + j] +=
+                        Zs[0][i] * KZ[0][j] + Zs[1][i] * KZ[1][j] + Zs[2][i] * KZ[2][j];
+        }
+    }
+    // symmetrise, regularise (a Z column that vanishes, e.g. a rotation of a
+    // collinear aggregate, leaves a zero row; the applied Z ignores it), Cholesky
+    double tr = 0.0;
+    for (int i = 0; i < nc; ++i) tr += A[(size_t)i * nc + i];
+    for (int i = 0; i < nc; ++i)
+        for (int j = 0; j < i; ++j) {
+            const double v = 0.5 * (A[(size_t)i * nc + j] + A[(size_t)j * nc + i]);
+            A[(size_t)i * nc + j] = A[(size_t)j * nc + i] = v;
+        }
+    for (int i = 0; i < nc; ++i) A[(size_t)i * nc + i] += 1e-12 * tr / nc;
+    std::vector<double> L = A;  // lower triangle
+    for (int j = 0; j < nc; ++j) {
+        double s = L[(size_t)j * nc + j];
+        for (int k = 0; k < j; ++k) s -= L[(size_t)j * nc + k] * L[(size_t)j * nc + k];
+        if (!(s > 0.0)) return fail(OFEA_ERR_STATE, "coarse matrix Z^T K Z is not positive definite");
+        const double ljj = std::sqrt(s);
+        L[(size_t)j * nc + j] = ljj;
+        for (int i = j + 1; i < nc; ++i) {
+            double t = L[(size_t)i * nc + j];
+            for (int k = 0; k < j; ++k) t -= L[(size_t)i * nc + k] * L[(size_t)j * nc + k];
+            L[(size_t)i * nc + j] = t / ljj;
+        }
+    }
+    // inverse = L^-T L^-1, column by column
+    std::vector<double> inv((size_t)nc * nc), col(nc);
+    for (int e = 0; e < nc; ++e) {
+        for (int i = 0; i < nc; ++i) {
+            double t = i == e ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) t -= L[(size_t)i * nc + k] * col[k];
+            col[i] = t / L[(size_t)i * nc + i];
+        }
+        for (int i = nc - 1; i >= 0; --i) {
+            double t = col[i];
+            for (int k = i + 1; k < nc; ++k) t -= L[(size_t)k * nc + i] * col[k];
+            col[i] = t / L[(size_t)i * nc + i];
+        }
+        for (int i = 0; i < nc; ++i) inv[(size_t)i * nc + e] = col[i];
+    }
+    std::vector<float> ainv((size_t)nc * nc);
+    for (int i = 0; i < nc; ++i)
+        for (int j = 0; j < nc; ++j)  // symmetric FP32 copy
+            ainv[(size_t)i * nc + j] = (float)(0.5 * (inv[(size_t)i * nc + j] + inv[(size_t)j * nc + i]));
+    // device buffers (sizes depend only on nfree and nagg <= CG_NB / CG_AGG)
+    if (!w->co.g) {
+        OFEA_HIP(hipMalloc((void **)&w->co.g, sizeof(double) * 6 * CG_NB));
+        OFEA_HIP(hipMalloc((void **)&w->co.d, sizeof(float) * 3 * (size_t)std::max(nf, 1)));
+        OFEA_HIP(hipMalloc((void **)&w->co_ainv, sizeof(float) * 36 * (CG_NB / CG_AGG) * (CG_NB / CG_AGG)));
+    }
+    OFEA_HIP(hipMemset(w->co.g, 0, sizeof(double) * 6 * CG_NB));
+    OFEA_HIP(hipMemcpy((void *)w->co.d, d.data(), sizeof(float) * d.size(), hipMemcpyHostToDevice));
+    OFEA_HIP(hipMemcpy(w->co_ainv, ainv.data(), sizeof(float) * ainv.size(), hipMemcpyHostToDevice));
+    w->co.per = per;
+    w->co_nagg = nagg;
+    w->co_d_host = std::move(d);
+    w->co_ainv_host = std::move(ainv);
+    w->co_valid = true;
+    w->co_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return OFEA_OK;
+}
+
+// out = Z A_c^-1 Z^T r on the host, with the FP32 offsets and inverse the
+// device applies (training targets of the learned two-level smoother)
+int coarse_apply(ofea_ctx *ctx, const double *r, double *out)
+{
+    int rc = solver_prepare(ctx);
+    if (rc) return rc;
+    if ((rc = coarse_prepare(ctx))) return rc;
+    ofea_solver_ws *w = ctx->ws;
+    const int nf = w->nfree, per = w->co.per, nagg = w->co_nagg, nc = 6 * nagg;
+    const float *d = w->co_d_host.data();
+    std::vector<double> g(nc, 0.0), y(nc, 0.0);
+    for (int i = 0; i < nf; ++i) {
+        const int a = (i / per) / CG_AGG;
+        const double dx = d[3 * i], dy = d[3 * i + 1], dz = d[3 * i + 2];
+        const double r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
+        g[6 * a] += r0;
+        g[6 * a + 1] += r1;
+        g[6 * a + 2] += r2;
+        g[6 * a + 3] += dy * r2 - dz * r1;
+        g[6 * a + 4] += dz * r0 - dx * r2;
+        g[6 * a + 5] += dx * r1 - dy * r0;
+    }
+    for (int i = 0; i < nc; ++i) {
+        double t = 0.0;
+        for (int k = 0; k < nc; ++k) t += (double)w->co_ainv_host[(size_t)i * nc + k] * g[k];
+        y[i] = t;
+    }
+    for (int i = 0; i < nf; ++i) {
+        const double *ya = y.data() + 6 * ((i / per) / CG_AGG);
+        const double dx = d[3 * i], dy = d[3 * i + 1], dz = d[3 * i + 2];
+        out[3 * i] = ya[0] + ya[4] * dz - ya[5] * dy;
+        out[3 * i + 1] = ya[1] + ya[5] * dx - ya[3] * dz;
+        out[3 * i + 2] = ya[2] + ya[3] * dy - ya[4] * dx;
+    }
+    return OFEA_OK;
 }
 
 int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, double *relres)
@@ -726,21 +1044,39 @@ int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, 
         if (relres) *relres = 0.0;
         return OFEA_OK;
     }
+    const bool learned = method == OFEA_PCG_LEARNED || method == OFEA_PCG_LEARNED_MIXED ||
+                         method == OFEA_PCG_LEARNED_TWO_LEVEL || method == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED;
+    if (learned && !w->learned64) return fail(OFEA_ERR_STATE, "learned preconditioner not set (ofea_learned_*)");
+    if (method >= OFEA_PCG_TWO_LEVEL && method <= OFEA_PCG_LEARNED_TWO_LEVEL_MIXED && (rc = coarse_prepare(ctx)))
+        return rc;
     switch (method) {
     case OFEA_PCG_JACOBI:
-        return pcg_run<double, double, 0>(ctx, 0, ctx->d_val, w->v64, w->minv, rtol, maxit, -1.0, iters, relres);
+        return pcg_run<double, double, 0, false>(ctx, 0, ctx->d_val, w->v64, w->minv, rtol, maxit, -1.0, iters,
+                                                 relres);
     case OFEA_PCG_BLOCK_JACOBI:
-        return pcg_run<double, double, 1>(ctx, 1, ctx->d_val, w->v64, w->minv, rtol, maxit, -1.0, iters, relres);
+        return pcg_run<double, double, 1, false>(ctx, 1, ctx->d_val, w->v64, w->minv, rtol, maxit, -1.0, iters,
+                                                 relres);
     case OFEA_PCG_MIXED:
-        return pcg_run<float, double, 1>(ctx, 2, w->val32, w->v64, w->minv, rtol, maxit, 0.1, iters, relres);
+        return pcg_run<float, double, 1, false>(ctx, 2, w->val32, w->v64, w->minv, rtol, maxit, 0.1, iters, relres);
     case OFEA_PCG_LEARNED:
-        if (!w->learned64) return fail(OFEA_ERR_STATE, "learned preconditioner not set (ofea_learned_*)");
-        return pcg_run<double, double, 1>(ctx, 3, ctx->d_val, w->v64, w->learned64, rtol, maxit, -1.0, iters, relres);
+        return pcg_run<double, double, 1, false>(ctx, 3, ctx->d_val, w->v64, w->learned64, rtol, maxit, -1.0, iters,
+                                                 relres);
     case OFEA_PCG_LEARNED_MIXED:
-        if (!w->learned64) return fail(OFEA_ERR_STATE, "learned preconditioner not set (ofea_learned_*)");
-        return pcg_run<float, double, 1>(ctx, 5, w->val32, w->v64, w->learned64, rtol, maxit, 0.1, iters, relres);
+        return pcg_run<float, double, 1, false>(ctx, 5, w->val32, w->v64, w->learned64, rtol, maxit, 0.1, iters,
+                                                relres);
     case OFEA_PCG_MIXED32:
-        return pcg_run<float, float, 1>(ctx, 4, w->val32, w->v32, w->minv32, rtol, maxit, 0.1, iters, relres);
+        return pcg_run<float, float, 1, false>(ctx, 4, w->val32, w->v32, w->minv32, rtol, maxit, 0.1, iters, relres);
+    case OFEA_PCG_TWO_LEVEL:
+        return pcg_run<double, double, 1, true>(ctx, 6, ctx->d_val, w->v64, w->minv, rtol, maxit, -1.0, iters,
+                                                relres);
+    case OFEA_PCG_TWO_LEVEL_MIXED:
+        return pcg_run<float, double, 1, true>(ctx, 7, w->val32, w->v64, w->minv, rtol, maxit, 0.1, iters, relres);
+    case OFEA_PCG_LEARNED_TWO_LEVEL:
+        return pcg_run<double, double, 1, true>(ctx, 8, ctx->d_val, w->v64, w->learned64, rtol, maxit, -1.0, iters,
+                                                relres);
+    case OFEA_PCG_LEARNED_TWO_LEVEL_MIXED:
+        return pcg_run<float, double, 1, true>(ctx, 9, w->val32, w->v64, w->learned64, rtol, maxit, 0.1, iters,
+                                               relres);
     default:
         return fail(OFEA_ERR_ARG, "unknown solver method");
     }

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace ofea {
 
 enum Scalar { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL2, S_RU2, S_TRR, S_NSCAL };
@@ -26,7 +28,20 @@ struct VecSet {
     T *r = nullptr, *z = nullptr, *p = nullptr, *p2 = nullptr, *q = nullptr;
 };
 
-constexpr int NSLOT = 6;  // captured graphs: jacobi, block, mixed, learned, mixed32, learned mixed
+// captured graphs: jacobi, block, mixed, learned, mixed32, learned mixed, and
+// the four two-level variants (block, mixed, learned, learned mixed)
+constexpr int NSLOT = 10;
+
+// Two-level coarse correction: z = M r + Z A_c^-1 Z^T r, Z = rigid-body modes
+// (3 translations, 3 rotations about the centroid) of node aggregates.  An
+// aggregate is CG_AGG consecutive CG ranges (the node ranges of the fixed CG
+// grid), so Z^T r is a per-range partial.
+constexpr int CG_AGG = 8;
+struct CoarseArgs {
+    const float *d = nullptr;  // 3 * nfree: node position minus its aggregate centroid
+    double *g = nullptr;       // 6 * CG_NB: per-range partials of Z^T r, component-major
+    int per = 1;               // nodes per CG range
+};
 
 }  // namespace ofea
 
@@ -50,6 +65,13 @@ struct ofea_solver_ws {
     double *learned64 = nullptr;            // learned preconditioner blocks (FP64), 9 * nfree
     float *learned32 = nullptr;             // FP32 copy used by the solver
     bool precond_valid = false;
+    // two-level coarse space
+    ofea::CoarseArgs co;
+    int co_nagg = 0;
+    float *co_ainv = nullptr;  // (6 nagg)^2, row-major, FP32
+    bool co_valid = false;
+    double co_setup_ms = 0.0;
+    std::vector<float> co_d_host, co_ainv_host;  // host copies for ofea_coarse_apply
     hipGraph_t graph[ofea::NSLOT] = {};
     hipGraphExec_t exec[ofea::NSLOT] = {};
 };

@@ -24,19 +24,36 @@ def nparam() -> int:
     return lib().ofea_learned_nparam()
 
 
-def collect_pairs(ctx, rhs_list, snapshots=(8, 16, 32, 64, 128, 256), rtol_star=1e-11):
-    """(E, R) arrays (S x n) from block-Jacobi PCG runs on the resident K."""
+def collect_pairs(ctx, rhs_list, snapshots=(8, 16, 32, 64, 128, 256), rtol_star=1e-11, two_level=False):
+    """(E, R) arrays (S x n) from the solver's own PCG iterates on the resident K.
+
+    Block-Jacobi PCG by default.  With two_level the iterates come from the
+    two-level solver and the targets are the errors left after its coarse
+    correction, e - Z A_c^-1 Z^T r: the network then learns the local part
+    of the two-level preconditioner."""
+    method = A.PCG_TWO_LEVEL if two_level else A.PCG_BLOCK_JACOBI
     E, R = [], []
     for b in rhs_list:
-        xs, _, _, _ = ctx.solve(b, method=A.PCG_BLOCK_JACOBI, rtol=rtol_star, maxit=200000)
+        xs, _, _, _ = ctx.solve(b, method=method, rtol=rtol_star, maxit=200000)
         for k in snapshots:
-            xk, it, _, ok = ctx.solve(b, method=A.PCG_BLOCK_JACOBI, rtol=1e-30, maxit=k)
+            xk, it, _, ok = ctx.solve(b, method=method, rtol=1e-30, maxit=k)
             e = xs - xk
             if not np.any(e):
                 continue
+            r = ctx.spmv(e)
+            if two_level:
+                e = e - coarse_apply(ctx, r)
             E.append(e)
-            R.append(ctx.spmv(e))
+            R.append(r)
     return np.ascontiguousarray(np.array(E)), np.ascontiguousarray(np.array(R))
+
+
+def coarse_apply(ctx, r):
+    """Z A_c^-1 Z^T r of the two-level methods for the context's K."""
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.zeros_like(r)
+    check(lib().ofea_coarse_apply(ctx._h, ptr(r), ptr(out)))
+    return out
 
 
 class Learned:

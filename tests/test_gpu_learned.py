@@ -95,3 +95,16 @@ def test_training_and_learned_pcg(setup):
     say(f"iterations learned {it} vs block Jacobi {it_bj}")
     x2, it2, rel2, ok2 = ctx.solve(F, method=A.PCG_LEARNED_MIXED, rtol=1e-10)
     assert ok2 and np.linalg.norm(x2 - xd) / np.linalg.norm(xd) < 1e-8
+    # the two-level variant: trained on the two-level solver's own iterates with
+    # the coarse correction removed from the targets
+    E2, R2 = L.collect_pairs(ctx, rhs, snapshots=(4, 8, 16, 32), two_level=True)
+    ln2 = L.Learned(ctx, seed=1)
+    ln2.set_samples(E2, R2)
+    hist2 = ln2.train(steps=60, lr=3e-3)
+    assert hist2[-1] < hist2[0]
+    _, it_tl, _, _ = ctx.solve(F, method=A.PCG_TWO_LEVEL, rtol=1e-10)
+    for meth in (A.PCG_LEARNED_TWO_LEVEL, A.PCG_LEARNED_TWO_LEVEL_MIXED):
+        x3, it3, rel3, ok3 = ctx.solve(F, method=meth, rtol=1e-10)
+        say(f"learned two-level method {meth}: {it3} iterations (two-level block Jacobi {it_tl}); "
+            f"loss {hist2[0]:.4e} -> {hist2[-1]:.4e}")
+        assert ok3 and np.linalg.norm(x3 - xd) / np.linalg.norm(xd) < 1e-8

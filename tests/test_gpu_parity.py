@@ -97,7 +97,8 @@ def test_spmv_matches_oracle(ctx):
     assert rel_max(ctx.spmv(x), O.spmv_csc(cp, ri, val, x)) < 1e-13
 
 
-@pytest.mark.parametrize("method", [A.PCG_JACOBI, A.PCG_BLOCK_JACOBI, A.PCG_MIXED, A.PCG_MIXED32])
+@pytest.mark.parametrize("method", [A.PCG_JACOBI, A.PCG_BLOCK_JACOBI, A.PCG_MIXED, A.PCG_MIXED32,
+                                    A.PCG_TWO_LEVEL, A.PCG_TWO_LEVEL_MIXED])
 def test_pcg_cfg1_matches_direct_solve(ctx, method):
     m, resort, F = mg.cantilever_c3d8()
     say(f"pcg method={method} cfg1 20x10x10")
@@ -112,6 +113,63 @@ def test_pcg_cfg1_matches_direct_solve(ctx, method):
     if method == A.PCG_JACOBI:
         _, it_o, _ = O.pcg_jacobi(cp, ri, val, F, rtol=1e-10)
         assert abs(it - it_o) <= max(10, 0.05 * it_o)
+    if method == A.PCG_TWO_LEVEL:  # the coarse correction must pay for itself in iterations
+        _, it_bj, _, _ = ctx.solve(F, method=A.PCG_BLOCK_JACOBI, rtol=1e-10, maxit=200000)
+        say(f"two-level {it} vs block Jacobi {it_bj} iterations")
+        assert it < it_bj
+
+
+def test_coarse_correction_matches_numpy(ctx):
+    """Z A_c^-1 Z^T r of the two-level methods against an independent numpy
+    construction: aggregates of 8 consecutive CG node ranges (512 ranges of
+    ceil(nfree/512) nodes), rigid-body modes about each aggregate's centroid,
+    A_c = Z^T K Z.  The library inverts A_c in FP64 and applies it in FP32:
+    bar 1e-5 relative."""
+    m, resort, F = mg.cantilever_c3d8(12, 4, 4)
+    ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
+    cp, ri = ctx.pattern()
+    val = ctx.assemble(mg.E_STEEL, mg.NU_STEEL)
+    K = csc(cp, ri, val)
+    nf = ctx.n // 3
+    X = np.zeros((nf, 3))
+    X[resort[resort >= 0]] = m.xyz[resort >= 0]
+    per = -(-nf // 512)
+    agg = (np.arange(nf) // per) // 8
+    nagg = agg.max() + 1
+    cen = np.stack([np.bincount(agg, weights=X[:, c]) / np.bincount(agg) for c in range(3)], axis=1)
+    d = (X - cen[agg]).astype(np.float32).astype(np.float64)
+    Z = np.zeros((ctx.n, 6 * nagg))
+    for i in range(nf):
+        a = agg[i]
+        dx, dy, dz = d[i]
+        Z[3 * i:3 * i + 3, 6 * a:6 * a + 6] = [[1, 0, 0, 0, dz, -dy], [0, 1, 0, -dz, 0, dx], [0, 0, 1, dy, -dx, 0]]
+    Ac = Z.T @ (K @ Z)
+    r = np.random.default_rng(4).normal(size=ctx.n)
+    ref = Z @ np.linalg.solve(Ac, Z.T @ r)
+    from openfea_amd import learned as L
+    out = L.coarse_apply(ctx, r)
+    assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-5
+
+
+def test_two_level_tet_and_rhs_reuse(ctx):
+    """Two-level PCG on a jittered tet mesh, two right-hand sides in a row
+    (the coarse space is built once per K), and after a reassembly with
+    another modulus field (the coarse space is rebuilt)."""
+    m, resort, F = mg.cantilever_c3d4(12, 6, 6, seed=3)
+    say("two-level tet 12x6x6")
+    ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
+    cp, ri = ctx.pattern()
+    rng = np.random.default_rng(5)
+    for E_elem in (None, rng.uniform(1e5, 3e5, m.ne)):
+        ctx.set_elem_E(E_elem)
+        val = ctx.assemble(mg.E_STEEL, mg.NU_STEEL)
+        lu = spla.splu(csc(cp, ri, val))
+        for b in (F, rng.normal(size=ctx.n)):
+            for method in (A.PCG_TWO_LEVEL, A.PCG_TWO_LEVEL_MIXED):
+                x, it, rel, ok = ctx.solve(b, method=method, rtol=1e-10)
+                xd = lu.solve(b)
+                assert ok and np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8, (method, it, rel)
+    ctx.set_elem_E(None)
 
 
 def test_pipeline_reference_deck_vs_pardiso_golden(ctx):
