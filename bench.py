@@ -55,10 +55,11 @@ def cpu_baseline_fill(m, resort, reps=3):
     return m.ne / statistics.median(times)
 
 
-def train_learned(workload, args):
+def train_learned(workload, args, two_level=False):
     """Trains the learned block-Jacobi correction on a smaller mesh of the same
-    family (different jitter seed) from that mesh's own block-Jacobi PCG runs;
-    the node-local network then transfers to the benchmark mesh."""
+    family (different jitter seed) from that mesh's own PCG runs (block Jacobi,
+    or the two-level solver with coarse-corrected targets); the node-local
+    network then transfers to the benchmark mesh."""
     import numpy as np
 
     import openfea_amd as ofa
@@ -74,13 +75,14 @@ def train_learned(workload, args):
     ctx.assemble(mg.E_STEEL, mg.NU_STEEL, copy=False)
     rng = np.random.default_rng(11)
     rhs = [F] + [ctx.spmv(rng.normal(size=ctx.n)) for _ in range(3)]
-    E, R = L.collect_pairs(ctx, rhs, snapshots=(8, 16, 32, 64, 128))
+    E, R = L.collect_pairs(ctx, rhs, snapshots=(8, 16, 32, 64, 128), two_level=two_level)
     ln = L.Learned(ctx, seed=0)
     ln.set_samples(E, R)
     hist = ln.train(steps=args.train_steps, lr=2e-3)
     params = ln.get_params()
     ctx.close()
     info = {"train_mesh": f"{m.ne} elements ({'C3D4 24x12x12, jitter seed 7' if workload == 'cfg2' else 'C3D8 10x5x5'})",
+            "pairs_from": "two-level PCG, coarse-corrected targets" if two_level else "block-Jacobi PCG",
             "samples": int(len(E)), "steps": args.train_steps, "loss_first": float(hist[0]),
             "loss_last": float(hist[-1]), "train_s": round(time.perf_counter() - t0, 3), "params": params}
     log(f"learned: samples {len(E)} loss {hist[0]:.4e} -> {hist[-1]:.4e} in {info['train_s']}s")
@@ -134,13 +136,17 @@ def bench_static(args, workload):
                    "learned_two_level_mixed_fp32val": A.PCG_LEARNED_TWO_LEVEL_MIXED}
     chosen = [] if args.no_solve else [m for m in args.solvers.split(",") if m]
     methods = [(m, all_methods[m]) for m in chosen]
-    learned_info = None
-    if any(m.startswith("learned") for m in chosen):
-        learned_info = train_learned(workload, args)
-        ln = L.Learned(ctx, seed=0)
-        ln.set_params(learned_info.pop("params"))
-        ctx.set_rhs(F)
+    learned_info, params = {}, {}
+    for tl in (False, True):
+        if any(m.startswith("learned") and (("two_level" in m) == tl) for m in chosen):
+            info = train_learned(workload, args, two_level=tl)
+            params[tl] = info.pop("params")
+            learned_info["two_level" if tl else "block_jacobi"] = info
+    ln = L.Learned(ctx, seed=0) if params else None
+    ctx.set_rhs(F)
     for name, method in methods:
+        if name.startswith("learned"):
+            ln.set_params(params["two_level" in name])
         ctx.sync()
         t = time.perf_counter()
         ctx.solve(None, method=method, rtol=1e-8, maxit=100000, copy=False)  # warm (setup, graph capture)

@@ -357,15 +357,32 @@ __global__ __launch_bounds__(UP_WG) void k_cg_update(int bi, int nfree, XT *__re
     const T *p = par ? pA : pB;  // p_new of this iteration
     int i0, i1;
     cg_range(nfree, i0, i1);
-    // the node data do not depend on alpha: issue the loads before the reduction
+    // the node data do not depend on alpha: issue all loads before the reduction
     int i = i0 + (int)threadIdx.x;
-    T pv[3] = {T(0), T(0), T(0)}, qv[3] = {T(0), T(0), T(0)}, rv[3] = {T(0), T(0), T(0)};
-    if (i < i1)
+    T pv[3], qv[3], rv[3], mv[PC == 0 ? 3 : 9];
+    XT xv[3];
+    float dv[3];
+    auto load_node = [&](int ii) {
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
-            pv[c] = p[3 * i + c];
-            qv[c] = q[3 * i + c];
-            rv[c] = r[3 * i + c];
+            pv[c] = p[3 * ii + c];
+            qv[c] = q[3 * ii + c];
+            rv[c] = r[3 * ii + c];
+            xv[c] = x[3 * ii + c];
         }
+        if constexpr (PC == 0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) mv[c] = (T)dinv[3 * ii + c];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 9; ++e) mv[e] = minv[9 * (size_t)ii + e];
+        }
+        if constexpr (CO) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) dv[c] = C.d[3 * ii + c];
+        }
+    };
+    if (i < i1) load_node(i);
     double red[2];
     {
         const double *const arr[2] = {P.pq, P.rz + par * CG_NB};
@@ -382,26 +399,38 @@ __global__ __launch_bounds__(UP_WG) void k_cg_update(int bi, int nfree, XT *__re
     const double alpha = rz / pq;
     double rzn = 0.0, rr2 = 0.0, g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (; i < i1; i += UP_WG) {
-        if (i >= i0 + UP_WG)  // later passes (ranges longer than one workgroup) load here
-            for (int c = 0; c < 3; ++c) {
-                pv[c] = p[3 * i + c];
-                qv[c] = q[3 * i + c];
-                rv[c] = r[3 * i + c];
-            }
+        if (i >= i0 + UP_WG) load_node(i);  // ranges longer than one workgroup
         T zv[3];
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int k = 3 * i + c;
-            x[k] = (XT)((double)x[k] + alpha * (double)pv[c]);
+            x[k] = (XT)((double)xv[c] + alpha * (double)pv[c]);
             rv[c] = (T)((double)rv[c] - alpha * (double)qv[c]);
             r[k] = rv[c];
         }
-        apply_pc<T, PC>(i, rv, zv, dinv, minv);
+        if constexpr (PC == 0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) zv[c] = rv[c] * mv[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) zv[c] = mv[3 * c] * rv[0] + mv[3 * c + 1] * rv[1] + mv[3 * c + 2] * rv[2];
+        }
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
             z[3 * i + c] = zv[c];
             rzn += (double)rv[c] * (double)zv[c];
             rr2 += (double)rv[c] * (double)rv[c];
         }
-        if constexpr (CO) add_zt_r(C, i, rv, g);
+        if constexpr (CO) {
+            const double d0 = dv[0], d1 = dv[1], d2 = dv[2];
+            const double r0 = rv[0], r1 = rv[1], r2 = rv[2];
+            g[0] += r0;
+            g[1] += r1;
+            g[2] += r2;
+            g[3] += d1 * r2 - d2 * r1;
+            g[4] += d2 * r0 - d0 * r2;
+            g[5] += d0 * r1 - d1 * r0;
+        }
     }
     put_rz_rr_g<CO, UP_WG>(rzn, rr2, g, P.rz + (par ^ 1) * CG_NB, P.rr, C);
 }
@@ -947,30 +976,44 @@ static int coarse_prepare(ofea_ctx *ctx)
             A[(size_t)i * nc + j] = A[(size_t)j * nc + i] = v;
         }
     for (int i = 0; i < nc; ++i) A[(size_t)i * nc + i] += 1e-12 * tr / nc;
-    std::vector<double> L = A;  // lower triangle
+    // A_c couples an aggregate only to aggregates holding neighbour nodes, so
+    // it is banded for consecutive-rank aggregates: banded Cholesky and banded
+    // triangular solves (bandwidth bw) build the explicit inverse in O(nc^2 bw)
+    int bw = 0;
+    for (int i = 0; i < nc; ++i)
+        for (int j = 0; j < i; ++j)
+            if (A[(size_t)i * nc + j] != 0.0) {
+                bw = std::max(bw, i - j);
+                break;
+            }
+    std::vector<double> L = A;  // lower triangle, entries with i - j <= bw
     for (int j = 0; j < nc; ++j) {
+        const int k0 = std::max(0, j - bw);
         double s = L[(size_t)j * nc + j];
-        for (int k = 0; k < j; ++k) s -= L[(size_t)j * nc + k] * L[(size_t)j * nc + k];
+        for (int k = k0; k < j; ++k) s -= L[(size_t)j * nc + k] * L[(size_t)j * nc + k];
         if (!(s > 0.0)) return fail(OFEA_ERR_STATE, "coarse matrix Z^T K Z is not positive definite");
         const double ljj = std::sqrt(s);
         L[(size_t)j * nc + j] = ljj;
-        for (int i = j + 1; i < nc; ++i) {
+        const int i1 = std::min(nc, j + bw + 1);
+        for (int i = j + 1; i < i1; ++i) {
             double t = L[(size_t)i * nc + j];
-            for (int k = 0; k < j; ++k) t -= L[(size_t)i * nc + k] * L[(size_t)j * nc + k];
+            for (int k = std::max(k0, i - bw); k < j; ++k) t -= L[(size_t)i * nc + k] * L[(size_t)j * nc + k];
             L[(size_t)i * nc + j] = t / ljj;
         }
     }
-    // inverse = L^-T L^-1, column by column
+    // inverse = L^-T L^-1, column by column (column e of L^-1 is zero above e)
     std::vector<double> inv((size_t)nc * nc), col(nc);
     for (int e = 0; e < nc; ++e) {
-        for (int i = 0; i < nc; ++i) {
+        std::fill(col.begin(), col.end(), 0.0);
+        for (int i = e; i < nc; ++i) {
             double t = i == e ? 1.0 : 0.0;
-            for (int k = 0; k < i; ++k) t -= L[(size_t)i * nc + k] * col[k];
+            for (int k = std::max(e, i - bw); k < i; ++k) t -= L[(size_t)i * nc + k] * col[k];
             col[i] = t / L[(size_t)i * nc + i];
         }
         for (int i = nc - 1; i >= 0; --i) {
             double t = col[i];
-            for (int k = i + 1; k < nc; ++k) t -= L[(size_t)k * nc + i] * col[k];
+            const int k1 = std::min(nc, i + bw + 1);
+            for (int k = i + 1; k < k1; ++k) t -= L[(size_t)k * nc + i] * col[k];
             col[i] = t / L[(size_t)i * nc + i];
         }
         for (int i = 0; i < nc; ++i) inv[(size_t)i * nc + e] = col[i];

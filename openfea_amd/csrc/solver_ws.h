@@ -36,7 +36,7 @@ constexpr int NSLOT = 10;
 // (3 translations, 3 rotations about the centroid) of node aggregates.  An
 // aggregate is CG_AGG consecutive CG ranges (the node ranges of the fixed CG
 // grid), so Z^T r is a per-range partial.
-constexpr int CG_AGG = 8;
+constexpr int CG_AGG = 4;
 struct CoarseArgs {
     const float *d = nullptr;  // 3 * nfree: node position minus its aggregate centroid
     double *g = nullptr;       // 6 * CG_NB: per-range partials of Z^T r, component-major

@@ -121,7 +121,7 @@ def test_pcg_cfg1_matches_direct_solve(ctx, method):
 
 def test_coarse_correction_matches_numpy(ctx):
     """Z A_c^-1 Z^T r of the two-level methods against an independent numpy
-    construction: aggregates of 8 consecutive CG node ranges (512 ranges of
+    construction: aggregates of 4 consecutive CG node ranges (512 ranges of
     ceil(nfree/512) nodes), rigid-body modes about each aggregate's centroid,
     A_c = Z^T K Z.  The library inverts A_c in FP64 and applies it in FP32:
     bar 1e-5 relative."""
@@ -134,7 +134,7 @@ def test_coarse_correction_matches_numpy(ctx):
     X = np.zeros((nf, 3))
     X[resort[resort >= 0]] = m.xyz[resort >= 0]
     per = -(-nf // 512)
-    agg = (np.arange(nf) // per) // 8
+    agg = (np.arange(nf) // per) // 4
     nagg = agg.max() + 1
     cen = np.stack([np.bincount(agg, weights=X[:, c]) / np.bincount(agg) for c in range(3)], axis=1)
     d = (X - cen[agg]).astype(np.float32).astype(np.float64)
@@ -145,7 +145,9 @@ def test_coarse_correction_matches_numpy(ctx):
         Z[3 * i:3 * i + 3, 6 * a:6 * a + 6] = [[1, 0, 0, 0, dz, -dy], [0, 1, 0, -dz, 0, dx], [0, 0, 1, dy, -dx, 0]]
     Ac = Z.T @ (K @ Z)
     r = np.random.default_rng(4).normal(size=ctx.n)
-    ref = Z @ np.linalg.solve(Ac, Z.T @ r)
+    # a rotation about the line of a collinear aggregate is a zero column of Z:
+    # A_c is then singular and that component of y does not enter Z y
+    ref = Z @ np.linalg.lstsq(Ac, Z.T @ r, rcond=1e-13)[0]
     from openfea_amd import learned as L
     out = L.coarse_apply(ctx, r)
     assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-5
