@@ -224,8 +224,33 @@ __global__ __launch_bounds__(CG_WG) void k_cg_spmv(int bi, int nfree, const int 
                                                    const T *__restrict__ z, T *__restrict__ pA, T *__restrict__ pB,
                                                    T *__restrict__ q, CGParts P, DevScalars *sc)
 {
-    if (sc->done) return;
-    const int par = sc->par0 ^ (bi & 1);
+    const int done0 = sc->done, par = sc->par0 ^ (bi & 1);
+    const T *p_old = par ? pB : pA;
+    T *p_new = par ? pA : pB;
+    constexpr int NPW = CG_WG / G;
+    const int lane = threadIdx.x % G;
+    int r0, r1;
+    cg_range(nfree, r0, r1);
+    // the lane's first matrix column entry and vector values do not depend on
+    // beta: load them before the reduction so that the two latencies overlap
+    int r = r0 + (int)threadIdx.x / G;
+    int nb0 = 0, len = 0;
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0, zf = 0.0, pf = 0.0;
+    if (r < r1) {
+        nb0 = node_blk[r];
+        len = 3 * (node_blk[r + 1] - nb0);
+        if (lane < len) {
+            const int kk = lane / 3, d = lane - 3 * kk;
+            const int s = bcol[nb0 + kk];
+            const TV *v0 = val + 9 * (long long)nb0;
+            m0 = (double)v0[lane];
+            m1 = (double)v0[len + lane];
+            m2 = (double)v0[2 * len + lane];
+            zf = (double)z[3 * s + d];
+            pf = (double)p_old[3 * s + d];
+        }
+    }
+    if (done0) return;
     double red[3];
     {
         const double *const arr[3] = {P.rz + par * CG_NB, P.rz + (par ^ 1) * CG_NB, P.rr};
@@ -249,19 +274,25 @@ __global__ __launch_bounds__(CG_WG) void k_cg_spmv(int bi, int nfree, const int 
     }
     if (dn) return;
     const double beta = rz / rzp;  // rzp = inf at a start: beta = 0
-    const T *p_old = par ? pB : pA;
-    T *p_new = par ? pA : pB;
-    constexpr int NPW = CG_WG / G;
-    const int lane = threadIdx.x % G;
-    int r0, r1;
-    cg_range(nfree, r0, r1);
     double dot = 0.0;
-    for (int r = r0 + (int)threadIdx.x / G; r < r1; r += NPW) {
-        const int nb0 = node_blk[r], deg = node_blk[r + 1] - nb0;
-        const int len = 3 * deg;
+    for (bool first = true; r < r1; r += NPW, first = false) {
+        if (!first) {
+            nb0 = node_blk[r];
+            len = 3 * (node_blk[r + 1] - nb0);
+        }
         const TV *v0 = val + 9 * (long long)nb0;
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-        for (int j = lane; j < len; j += G) {
+        int j = lane;
+        if (first) {
+            if (j < len) {
+                const double pv = (double)(T)(zf + beta * pf);
+                a0 += m0 * pv;
+                a1 += m1 * pv;
+                a2 += m2 * pv;
+            }
+            j += G;
+        }
+        for (; j < len; j += G) {
             const int kk = j / 3, d = j - 3 * kk;
             const int s = bcol[nb0 + kk];
             const double pv = (double)(T)((double)z[3 * s + d] + beta * (double)p_old[3 * s + d]);
@@ -352,8 +383,7 @@ __global__ __launch_bounds__(UP_WG) void k_cg_update(int bi, int nfree, XT *__re
                                                      const double *__restrict__ dinv, const T *__restrict__ minv,
                                                      CGParts P, CoarseArgs C, DevScalars *sc)
 {
-    if (sc->done) return;
-    const int par = sc->par0 ^ (bi & 1);
+    const int done0 = sc->done, par = sc->par0 ^ (bi & 1);
     const T *p = par ? pA : pB;  // p_new of this iteration
     int i0, i1;
     cg_range(nfree, i0, i1);
@@ -383,6 +413,7 @@ __global__ __launch_bounds__(UP_WG) void k_cg_update(int bi, int nfree, XT *__re
         }
     };
     if (i < i1) load_node(i);
+    if (done0) return;
     double red[2];
     {
         const double *const arr[2] = {P.pq, P.rz + par * CG_NB};
@@ -450,19 +481,25 @@ __global__ __launch_bounds__(CG_WG) void k_coarse(int bi, int mode, int nfree, i
     // every load that does not depend on the scalars is issued before the
     // done-check, so their latencies overlap
     const int done = sc->done, par0 = sc->par0;
-    const int nc = 6 * nagg;  // <= 6 CG_NB / CG_AGG < CG_WG: one coarse column per thread
+    const int nc = 6 * nagg;  // <= 6 CG_NB / CG_AGG <= 2 CG_WG: at most two coarse columns per thread
     const int a = blockIdx.x;
-    const int i0 = min(a * CG_AGG * C.per, nfree), i1 = min(i0 + CG_AGG * C.per, nfree);
-    const int k = threadIdx.x;
-    float am[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    double gk = 0.0;
-    if (k < nc) {
+    const int i0 = min(a * C.ra * C.per, nfree), i1 = min(i0 + C.ra * C.per, nfree);
+    double y[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int i2 = 0; i2 < 6; ++i2) am[i2] = ainv[(size_t)(6 * a + i2) * nc + k];
-        const int aa = k / 6, c = k - 6 * aa;
-        const double *gp = C.g + c * CG_NB + aa * CG_AGG;  // written by the previous kernel
-        const int nb = min(CG_AGG, CG_NB - aa * CG_AGG);
-        for (int b = 0; b < nb; ++b) gk += gp[b];
+    for (int kk = 0; kk < 2; ++kk) {
+        const int k = threadIdx.x + kk * CG_WG;
+        if (k < nc) {
+            float am[6];
+#pragma unroll
+            for (int i2 = 0; i2 < 6; ++i2) am[i2] = ainv[(size_t)(6 * a + i2) * nc + k];
+            const int aa = k / 6, c = k - 6 * aa;
+            const double *gp = C.g + c * CG_NB + aa * C.ra;  // written by the previous kernel
+            const int nb = min(C.ra, CG_NB - aa * C.ra);
+            double gk = 0.0;
+            for (int b = 0; b < nb; ++b) gk += gp[b];
+#pragma unroll
+            for (int i2 = 0; i2 < 6; ++i2) y[i2] += (double)am[i2] * gk;
+        }
     }
     int i = i0 + (int)threadIdx.x;
     double dv[3] = {0.0, 0.0, 0.0}, zv[3] = {0.0, 0.0, 0.0}, rv[3] = {0.0, 0.0, 0.0};
@@ -475,9 +512,6 @@ __global__ __launch_bounds__(CG_WG) void k_coarse(int bi, int mode, int nfree, i
         }
     if (done) return;
     const int slot = mode ? par0 : (par0 ^ (bi & 1) ^ 1);
-    double y[6];
-#pragma unroll
-    for (int i2 = 0; i2 < 6; ++i2) y[i2] = (double)am[i2] * gk;
     cg_block_sum<6>(y);
     double rzc = 0.0;
     for (; i < i1; i += CG_WG) {
@@ -617,7 +651,8 @@ static int ensure_ws(ofea_ctx *ctx)
     OFEA_HIP(hipMalloc((void **)&w->part_pq, sizeof(double) * CG_NB));
     OFEA_HIP(hipMalloc((void **)&w->sc, sizeof(DevScalars)));
     OFEA_HIP(hipMemset(w->sc, 0, sizeof(DevScalars)));
-    OFEA_HIP(hipHostMalloc((void **)&w->sc_host, sizeof(DevScalars), hipHostMallocDefault));
+    OFEA_HIP(hipHostMalloc((void **)&w->sc_host, 3 * sizeof(DevScalars), hipHostMallocDefault));  // [0] + 2 in flight
+    for (auto &e : w->pipe_ev) OFEA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const size_t nb = (size_t)std::max(n, 1), nfb = (size_t)std::max(nf, 1);
     for (double **p : {&w->v64.r, &w->v64.z, &w->v64.p, &w->v64.p2, &w->v64.q, &w->r64, &w->y64})
         OFEA_HIP(hipMalloc((void **)p, sizeof(double) * nb));
@@ -649,6 +684,8 @@ void solver_free(ofea_ctx *ctx)
         if (p) (void)hipFree(p);
     if (w->sc) (void)hipFree(w->sc);
     if (w->sc_host) (void)hipHostFree(w->sc_host);
+    for (auto &e : w->pipe_ev)
+        if (e) (void)hipEventDestroy(e);
     delete w;
     ctx->ws = nullptr;
 }
@@ -847,16 +884,37 @@ static int pcg_run(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T
     }
     double rel = 1.0;
     int tighten = 0;
-    for (;;) {
+    // Up to two batches are kept in flight: the host checks the scalars of
+    // batch k (pinned copy + event) while batch k+1 runs.  After convergence
+    // the batch in flight returns at once from every kernel.
+    DevScalars *Hb = w->sc_host + 1;
+    auto launch_batch = [&](int hb) -> int {
         if (no_graph) {
             cg_batch<TV, T, double, PC, CO>(ctx, val, v, x, minv);
             OFEA_HIP(hipGetLastError());
         } else {
             OFEA_HIP(hipGraphLaunch(w->exec[slot], ctx->stream));
         }
+        OFEA_HIP(hipMemcpyAsync(&Hb[hb], w->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
+        OFEA_HIP(hipEventRecord(w->pipe_ev[hb], ctx->stream));
+        return OFEA_OK;
+    };
+    // with reliable updates (frequent stops) the batch in flight is mostly
+    // wasted work, so those methods keep one batch at a time
+    const int depth = ru_factor > 0.0 ? 1 : 2;
+    for (;;) {
+        for (int d = 0; d < depth; ++d)
+            if ((rc = launch_batch(d))) return rc;
+        int cur = 0;
+        for (;;) {
+            OFEA_HIP(hipEventSynchronize(w->pipe_ev[cur]));
+            if (Hb[cur].done) break;
+            if ((rc = launch_batch(cur))) return rc;
+            cur = (cur + 1) % depth;
+        }
+        OFEA_HIP(hipStreamSynchronize(ctx->stream));  // drain the batch in flight (it returned early)
         if ((rc = read_scalars(ctx))) return rc;
         const int done = S.done;
-        if (done == 0) continue;
         if (done == 3) return fail(OFEA_ERR_STATE, "PCG breakdown (matrix or preconditioner not SPD, or NaN)");
         const int it_done = S.it, par_done = S.par0 ^ (S.bi_done & 1);
         // true FP64 residual at convergence candidates, reliable-update points and maxit
@@ -886,7 +944,8 @@ static int pcg_run(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T
 }
 
 // Coarse space of the two-level methods, built on the host from the resident
-// K: aggregates of CG_AGG consecutive CG node ranges, Z = rigid-body modes
+// K: aggregates of ra >= CG_AGG consecutive CG node ranges (at least 64
+// nodes each, so that the rotations stay well conditioned), Z = rigid-body modes
 // about each aggregate's centroid, A_c = Z^T K Z (FP64, 6 nagg square),
 // Cholesky-inverted and stored in FP32.
 static int coarse_prepare(ofea_ctx *ctx)
@@ -897,7 +956,8 @@ static int coarse_prepare(ofea_ctx *ctx)
     const int nf = w->nfree;
     const int per = (nf + CG_NB - 1) / CG_NB;
     const int nranges = (nf + per - 1) / per;
-    const int nagg = (nranges + CG_AGG - 1) / CG_AGG;
+    const int ra = std::max(CG_AGG, (64 + per - 1) / per);
+    const int nagg = (nranges + ra - 1) / ra;
     const int nc = 6 * nagg;
     // free-node coordinates and aggregate centroids
     std::vector<double> X((size_t)3 * nf), cen((size_t)3 * nagg, 0.0);
@@ -907,7 +967,7 @@ static int coarse_prepare(ofea_ctx *ctx)
         if (r < 0) continue;
         for (int c = 0; c < 3; ++c) X[3 * (size_t)r + c] = ctx->xyz[3 * (size_t)i + c];
     }
-    auto agg = [&](int r) { return (r / per) / CG_AGG; };
+    auto agg = [&](int r) { return (r / per) / ra; };
     for (int r = 0; r < nf; ++r) {
         const int a = agg(r);
         cnt[a]++;
@@ -1032,6 +1092,7 @@ static int coarse_prepare(ofea_ctx *ctx)
     OFEA_HIP(hipMemcpy((void *)w->co.d, d.data(), sizeof(float) * d.size(), hipMemcpyHostToDevice));
     OFEA_HIP(hipMemcpy(w->co_ainv, ainv.data(), sizeof(float) * ainv.size(), hipMemcpyHostToDevice));
     w->co.per = per;
+    w->co.ra = ra;
     w->co_nagg = nagg;
     w->co_d_host = std::move(d);
     w->co_ainv_host = std::move(ainv);
@@ -1052,7 +1113,7 @@ int coarse_apply(ofea_ctx *ctx, const double *r, double *out)
     const float *d = w->co_d_host.data();
     std::vector<double> g(nc, 0.0), y(nc, 0.0);
     for (int i = 0; i < nf; ++i) {
-        const int a = (i / per) / CG_AGG;
+        const int a = (i / per) / w->co.ra;
         const double dx = d[3 * i], dy = d[3 * i + 1], dz = d[3 * i + 2];
         const double r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
         g[6 * a] += r0;
@@ -1068,7 +1129,7 @@ int coarse_apply(ofea_ctx *ctx, const double *r, double *out)
         y[i] = t;
     }
     for (int i = 0; i < nf; ++i) {
-        const double *ya = y.data() + 6 * ((i / per) / CG_AGG);
+        const double *ya = y.data() + 6 * ((i / per) / w->co.ra);
         const double dx = d[3 * i], dy = d[3 * i + 1], dz = d[3 * i + 2];
         out[3 * i] = ya[0] + ya[4] * dz - ya[5] * dy;
         out[3 * i + 1] = ya[1] + ya[5] * dx - ya[3] * dz;

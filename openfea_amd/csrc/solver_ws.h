@@ -34,13 +34,14 @@ constexpr int NSLOT = 10;
 
 // Two-level coarse correction: z = M r + Z A_c^-1 Z^T r, Z = rigid-body modes
 // (3 translations, 3 rotations about the centroid) of node aggregates.  An
-// aggregate is CG_AGG consecutive CG ranges (the node ranges of the fixed CG
-// grid), so Z^T r is a per-range partial.
-constexpr int CG_AGG = 4;
+// aggregate is ra >= CG_AGG consecutive CG ranges (the node ranges of the
+// fixed CG grid), so Z^T r is a per-range partial.
+constexpr int CG_AGG = 2;
 struct CoarseArgs {
     const float *d = nullptr;  // 3 * nfree: node position minus its aggregate centroid
     double *g = nullptr;       // 6 * CG_NB: per-range partials of Z^T r, component-major
     int per = 1;               // nodes per CG range
+    int ra = CG_AGG;           // CG ranges per aggregate
 };
 
 }  // namespace ofea
@@ -53,7 +54,8 @@ struct ofea_solver_ws {
     double *partials = nullptr;
     double *part_rz = nullptr, *part_rr = nullptr, *part_pq = nullptr;  // CG partials (fixed grid)
     ofea::DevScalars *sc = nullptr;
-    ofea::DevScalars *sc_host = nullptr;  // pinned
+    ofea::DevScalars *sc_host = nullptr;  // pinned, 3 entries: [0] and two batches in flight
+    hipEvent_t pipe_ev[2] = {nullptr, nullptr};
     ofea::VecSet<double> v64;
     ofea::VecSet<float> v32;
     double *r64 = nullptr, *y64 = nullptr;  // true residual, K x

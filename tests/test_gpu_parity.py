@@ -121,8 +121,8 @@ def test_pcg_cfg1_matches_direct_solve(ctx, method):
 
 def test_coarse_correction_matches_numpy(ctx):
     """Z A_c^-1 Z^T r of the two-level methods against an independent numpy
-    construction: aggregates of 4 consecutive CG node ranges (512 ranges of
-    ceil(nfree/512) nodes), rigid-body modes about each aggregate's centroid,
+    construction: aggregates of max(2, ceil(64/per)) consecutive CG node
+    ranges (512 ranges of per = ceil(nfree/512) nodes), rigid-body modes about each aggregate's centroid,
     A_c = Z^T K Z.  The library inverts A_c in FP64 and applies it in FP32:
     bar 1e-5 relative."""
     m, resort, F = mg.cantilever_c3d8(12, 4, 4)
@@ -134,7 +134,8 @@ def test_coarse_correction_matches_numpy(ctx):
     X = np.zeros((nf, 3))
     X[resort[resort >= 0]] = m.xyz[resort >= 0]
     per = -(-nf // 512)
-    agg = (np.arange(nf) // per) // 4
+    ra = max(2, -(-64 // per))  # ranges per aggregate: at least 2 and 64 nodes
+    agg = (np.arange(nf) // per) // ra
     nagg = agg.max() + 1
     cen = np.stack([np.bincount(agg, weights=X[:, c]) / np.bincount(agg) for c in range(3)], axis=1)
     d = (X - cen[agg]).astype(np.float32).astype(np.float64)
