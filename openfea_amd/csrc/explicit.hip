@@ -11,7 +11,9 @@
 //   element phase  lane per (hex, Gauss point) [or thread per tet]: grad u,
 //                  sigma = lambda tr(eps) I + 2 mu eps, nodal forces
 //                  w detJ sigma g_a; an 8-lane reduce-scatter leaves each lane
-//                  the force of one element node -> fe[e][a] (no atomics);
+//                  the force of one element node -> fe[a][e] (local-node-major,
+//                  so that the node phase's gathers of consecutive nodes read
+//                  consecutive elements; no atomics);
 //   node phase     thread per node: f = -sum fe over incident elements in
 //                  ascending order, a = (f + f_ext)/m, v += a dt,
 //                  u_next = u + v dt (0 at clamped nodes).
@@ -33,7 +35,7 @@ struct ExplicitWS {
     int *conn[2] = {nullptr, nullptr};       // type-local connectivity (0-based), copied from the plans
     int ne[2] = {0, 0};
     void *xyz_t = nullptr;                   // coordinates in the run precision
-    void *fe[2] = {nullptr, nullptr};        // element nodal forces, [ne][nn][3]
+    void *fe[2] = {nullptr, nullptr};        // element nodal forces, [nn][ne][3]
     void *u[2] = {nullptr, nullptr}, *v = nullptr, *minv = nullptr, *fext = nullptr;
     int *fixed = nullptr;
     int prec = -1;
@@ -220,7 +222,7 @@ __global__ __launch_bounds__(EX_WG) void k_force_hex(int ne, const int *__restri
     T f[3];
     reduce_scatter8<3, T>(gp, blk, f);
     if (e_raw < ne) {
-        T *dst = fe + 24 * (size_t)e + 3 * gp;
+        T *dst = fe + 3 * ((size_t)gp * ne + e);
         dst[0] = f[0];
         dst[1] = f[1];
         dst[2] = f[2];
@@ -257,13 +259,13 @@ __global__ __launch_bounds__(EX_WG) void k_force_tet(int ne, const int *__restri
             T acc = T(0);
 #pragma unroll
             for (int j = 0; j < 3; ++j) acc += (mu * (H[c][j] + H[j][c]) + (c == j ? lam * tr : T(0))) * g[j][a];
-            fe[12 * (size_t)e + 3 * a + c] = V * acc;
+            fe[3 * ((size_t)a * ne + e) + c] = V * acc;
         }
 }
 
 template <class T>
 __global__ __launch_bounds__(EX_WG) void k_node_update(int nd, const int *__restrict__ inc_ptr, const int *__restrict__ inc,
-                                                       const T *__restrict__ fe4, const T *__restrict__ fe8,
+                                                       int ne4, int ne8, const T *__restrict__ fe4, const T *__restrict__ fe8,
                                                        const T *__restrict__ ucur, T *__restrict__ unext,
                                                        T *__restrict__ v, const T *__restrict__ minv,
                                                        const T *__restrict__ fext, const int *__restrict__ fixed, T dt)
@@ -273,7 +275,7 @@ __global__ __launch_bounds__(EX_WG) void k_node_update(int nd, const int *__rest
     T f0 = T(0), f1 = T(0), f2 = T(0);
     for (int t = inc_ptr[i]; t < inc_ptr[i + 1]; ++t) {
         int code = inc[t], el = code >> 4, type = (code >> 3) & 1, a = code & 7;
-        const T *src = type ? fe8 + 24 * (size_t)el + 3 * a : fe4 + 12 * (size_t)el + 3 * a;
+        const T *src = type ? fe8 + 3 * ((size_t)a * ne8 + el) : fe4 + 3 * ((size_t)a * ne4 + el);
         f0 -= src[0];
         f1 -= src[1];
         f2 -= src[2];
@@ -315,7 +317,8 @@ static void launch_step(ofea_ctx *c, ExplicitWS *w, T lam, T mu, T dt, const T *
         hipLaunchKernelGGL(k_force_tet<T>, dim3((w->ne[0] + EX_WG - 1) / EX_WG), dim3(EX_WG), 0, c->stream, w->ne[0],
                            w->conn[0], (const T *)w->xyz_t, ucur, lam, mu, (T *)w->fe[0]);
     hipLaunchKernelGGL(k_node_update<T>, dim3((w->nd + EX_WG - 1) / EX_WG), dim3(EX_WG), 0, c->stream, w->nd,
-                       w->inc_ptr, w->inc, (const T *)w->fe[0], (const T *)w->fe[1], ucur, unext, (T *)w->v,
+                       w->inc_ptr, w->inc, w->ne[0], w->ne[1], (const T *)w->fe[0], (const T *)w->fe[1], ucur, unext,
+                       (T *)w->v,
                        (const T *)w->minv, (const T *)w->fext, w->fixed, dt);
 }
 
