@@ -196,7 +196,8 @@ int ofea_coarse_apply(ofea_ctx *ctx, const double *r, double *out);
  * what ofea_set_elem_E + ofea_assemble (fill_CSR_sparse_mat,
  * src/assemble.cpp:95-151, with a per-element E) would give, all nvar of them
  * from one kernel launch.  Single-type meshes (all C3D4 or all C3D8),
- * nvar <= 64.  nz_val (nvar x nnz, nullable) receives the values. */
+ * nvar <= 64.  E_var NULL reuses the resident moduli of the previous call
+ * (same nvar).  nz_val (nvar x nnz, nullable) receives the values. */
 int ofea_assemble_batch(ofea_ctx *ctx, int nvar, const double *E_var, double nu, double *nz_val);
 /* nrhs load cases (1, 2, 4 or 8) per variant, solved together by FP64 PCG
  * with an FP32 3x3 block-Jacobi preconditioner; every read of K_v serves all

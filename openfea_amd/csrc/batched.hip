@@ -212,11 +212,22 @@ __global__ __launch_bounds__(BA_WG) void k_assemble_tet_batch(
     }
 }
 
+__global__ void k_transpose_moduli(int ne, int nvar, const double *__restrict__ Ev, double *__restrict__ Et)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;  // output index e * nvar + v
+    if (i >= (long long)ne * nvar) return;
+    const int e = (int)(i / nvar), v = (int)(i - (long long)e * nvar);
+    Et[i] = Ev[(size_t)v * ne + e];
+}
+
 struct BatchWS {
     int nvar = 0;
     long long nnz = 0;
     double *valb = nullptr;  // nvar * nnz
     double *Evar[2] = {nullptr, nullptr};
+    double *Eraw = nullptr;  // E_var as given, [variant][element]
+    size_t Eraw_n = 0;
+    int E_nvar = 0;
     // multi-RHS CG
     int nrhs = 0;
     double *B = nullptr, *X = nullptr, *R = nullptr, *Z = nullptr, *PA = nullptr, *PB = nullptr, *Q = nullptr,
@@ -241,7 +252,7 @@ void batch_free(ofea_ctx *c)
     if (!w) return;
     if (w->exec) (void)hipGraphExecDestroy(w->exec);
     if (w->graph) (void)hipGraphDestroy(w->graph);
-    for (void *p : {(void *)w->valb, (void *)w->Evar[0], (void *)w->Evar[1], (void *)w->B, (void *)w->X, (void *)w->R,
+    for (void *p : {(void *)w->valb, (void *)w->Evar[0], (void *)w->Evar[1], (void *)w->Eraw, (void *)w->B, (void *)w->X, (void *)w->R,
                     (void *)w->Z, (void *)w->PA, (void *)w->PB, (void *)w->Q, (void *)w->Y, (void *)w->minv,
                     (void *)w->part, w->ms, (void *)w->valp})
         if (p) (void)hipFree(p);
@@ -271,17 +282,30 @@ int assemble_batch(ofea_ctx *c, int nvar, const double *E_var, double nu, double
         w->nvar = nvar;
         w->nnz = nnz;
     }
-    // moduli in element-major order per type, variants contiguous
-    for (int t = 0; t < 2; ++t) {
-        if (!c->has_type[t]) continue;
-        const GatherPlan &G = c->G[t];
-        std::vector<double> Et((size_t)G.ne * nvar);
-        for (int i = 0; i < G.ne; ++i)
-            for (int v = 0; v < nvar; ++v) Et[(size_t)i * nvar + v] = E_var[(size_t)v * c->ne + G.elem_global[i]];
-        if (w->Evar[t]) (void)hipFree(w->Evar[t]);
-        w->Evar[t] = nullptr;
-        int rc = dev_upload(w->Evar[t], Et.data(), Et.size());
-        if (rc) return rc;
+    // moduli: E_var [variant][element] is copied as given and transposed on the
+    // device to [element][variant]; NULL keeps the moduli of the previous call
+    // (single-type mesh: type-local element index == global index)
+    const int t_used = c->has_type[OFEA_C3D8] ? OFEA_C3D8 : OFEA_C3D4;
+    const size_t ne = (size_t)c->ne;
+    if (E_var) {
+        if (w->Eraw_n != ne * nvar) {
+            for (double **pp : {&w->Eraw, &w->Evar[0], &w->Evar[1]})
+                if (*pp) {
+                    (void)hipFree(*pp);
+                    *pp = nullptr;
+                }
+            OFEA_HIP(hipMalloc((void **)&w->Eraw, sizeof(double) * std::max<size_t>(ne * nvar, 1)));
+            OFEA_HIP(hipMalloc((void **)&w->Evar[t_used], sizeof(double) * std::max<size_t>(ne * nvar, 1)));
+            w->Eraw_n = ne * nvar;
+        }
+        OFEA_HIP(hipMemcpyAsync(w->Eraw, E_var, sizeof(double) * ne * nvar, hipMemcpyHostToDevice, c->stream));
+        const long long tot = (long long)ne * nvar;
+        hipLaunchKernelGGL(k_transpose_moduli, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, (int)ne,
+                           nvar, (const double *)w->Eraw, w->Evar[t_used]);
+        OFEA_HIP(hipGetLastError());
+        w->E_nvar = nvar;
+    } else if (w->E_nvar != nvar || !w->Evar[t_used]) {
+        return fail(OFEA_ERR_STATE, "E_var == NULL needs an earlier call with the same nvar");
     }
     for (int t = 0; t < 2; ++t) {
         if (!c->has_type[t]) continue;

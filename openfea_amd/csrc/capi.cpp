@@ -373,7 +373,6 @@ int ofea_assemble_batch(ofea_ctx *c, int nvar, const double *E_var, double nu, d
 {
     CHECK_CTX(c);
     if (!c->d_col_ptr) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
-    if (!E_var) return fail(OFEA_ERR_ARG, "null E_var");
     return assemble_batch(c, nvar, E_var, nu, nz_val);
 }
 

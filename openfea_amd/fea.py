@@ -213,11 +213,16 @@ class Context:
         return u, nt.value, lt.value, rr.value, rc == A.OFEA_OK
 
     # ---- batched variants / load cases
-    def assemble_batch(self, E_var, nu, copy=True):
-        """K_v for every row of E_var (nvar x ne per-element moduli), one launch."""
-        E_var = f64(E_var).reshape(-1, self.ne)
-        out = np.zeros((len(E_var), self.nnz)) if copy else None
-        check(lib().ofea_assemble_batch(self._h, len(E_var), ptr(E_var), nu, ptr(out)))
+    def assemble_batch(self, E_var, nu, copy=True, nvar=None):
+        """K_v for every row of E_var (nvar x ne per-element moduli), one launch.
+        E_var None reuses the moduli resident from the previous call (pass nvar)."""
+        if E_var is None:
+            nv = int(nvar)
+        else:
+            E_var = f64(E_var).reshape(-1, self.ne)
+            nv = len(E_var)
+        out = np.zeros((nv, self.nnz)) if copy else None
+        check(lib().ofea_assemble_batch(self._h, nv, ptr(E_var), nu, ptr(out)))
         return out
 
     def solve_batch(self, B, rtol=1e-8, maxit=100000, copy=True):

@@ -63,8 +63,9 @@ def test_assemble_batch_matches_oracle(ctx, kind, nvar):
     for v in sorted({0, nvar // 2, nvar - 1}):
         ref = O.fill_csc(m.conn, m.etype, m.xyz, resort, 0.0, mg.NU_STEEL, cp, ri, E_elem=Ev[v])
         assert rel_max(vals[v], ref) < 1e-12, v
-    # deterministic: a second launch gives the same bits
+    # deterministic: a second launch gives the same bits, also from the resident moduli
     assert np.array_equal(ctx.assemble_batch(Ev, mg.NU_STEEL), vals)
+    assert np.array_equal(ctx.assemble_batch(None, mg.NU_STEEL, nvar=nvar), vals)
 
 
 def test_assemble_batch_limits(ctx):
