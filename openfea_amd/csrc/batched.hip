@@ -20,6 +20,7 @@
 //    keeps the fastest for the solvers.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -728,24 +729,34 @@ static int mb_solve_variant(ofea_ctx *c, BatchWS *w, double rtol, int maxit, con
     hipLaunchKernelGGL((k_mb_start<NR>), dim3(MB_NB), dim3(MB_WG), 0, c->stream, nf, w->B, w->X, w->R, w->Z, w->PA,
                        w->PB, w->minv, w->part, (const MScalars *)w->ms);
     OFEA_HIP(hipGetLastError());
-    if (!w->exec || w->graph_nrhs != NR || w->graph_group != group) {
-        if (w->exec) (void)hipGraphExecDestroy(w->exec);
-        if (w->graph) (void)hipGraphDestroy(w->graph);
-        w->exec = nullptr;
-        w->graph = nullptr;
-        OFEA_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    auto direct_batch = [&]() {
         switch (group) {
         case 32: mb_capture_batch<NR, 32>(c, w); break;
         case 64: mb_capture_batch<NR, 64>(c, w); break;
         default: mb_capture_batch<NR, 16>(c, w); break;
         }
+    };
+    // OFEA_NO_GRAPH=1 launches the batches directly (for tools that cannot follow graph launches)
+    static const bool no_graph = std::getenv("OFEA_NO_GRAPH") && std::getenv("OFEA_NO_GRAPH")[0] == '1';
+    if (!no_graph && (!w->exec || w->graph_nrhs != NR || w->graph_group != group)) {
+        if (w->exec) (void)hipGraphExecDestroy(w->exec);
+        if (w->graph) (void)hipGraphDestroy(w->graph);
+        w->exec = nullptr;
+        w->graph = nullptr;
+        OFEA_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        direct_batch();
         OFEA_HIP(hipStreamEndCapture(c->stream, &w->graph));
         OFEA_HIP(hipGraphInstantiate(&w->exec, w->graph, nullptr, nullptr, 0));
         w->graph_nrhs = NR;
         w->graph_group = group;
     }
     for (;;) {
-        OFEA_HIP(hipGraphLaunch(w->exec, c->stream));
+        if (no_graph) {
+            direct_batch();
+            OFEA_HIP(hipGetLastError());
+        } else {
+            OFEA_HIP(hipGraphLaunch(w->exec, c->stream));
+        }
         OFEA_HIP(hipMemcpyAsync(&S, w->ms, sizeof(S), hipMemcpyDeviceToHost, c->stream));
         OFEA_HIP(hipStreamSynchronize(c->stream));
         if (S.all_done) break;
