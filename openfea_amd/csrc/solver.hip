@@ -36,10 +36,58 @@ namespace ofea {
 constexpr int RED_WG = 256;   // threads per workgroup of vector kernels
 constexpr int CG_BATCH = 16;  // iterations per graph launch
 
-template <class T>
-__device__ inline T wave_group_sum(T v, int group)
+// Cross-lane moves for the reductions, on the VALU (DPP) and the scalar
+// read-lane path: a shuffle through ds_bpermute costs LDS issue slots in every
+// wave, which at the end of a memory-bound kernel is exposed time.
+template <int CTRL>
+__device__ inline double dpp_mov(double x)
 {
-    for (int off = group >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__device__ inline double read_lane(double x, int l)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Sum over each row of 16 lanes, in every lane of the row: pairs (quad_perm
+// [1,0,3,2]), quads ([2,3,0,1]), half-row mirror, row mirror.  Each step adds
+// two values that the partner lane adds in the other order, so all lanes of
+// the row hold the same bits.
+__device__ inline double row16_sum(double x)
+{
+    x += dpp_mov<0xB1>(x);
+    x += dpp_mov<0x4E>(x);
+    x += dpp_mov<0x141>(x);
+    x += dpp_mov<0x140>(x);
+    return x;
+}
+
+// Sum over the 64 lanes of a wave, the same value in every lane (fixed order).
+__device__ inline double wave_sum(double x)
+{
+    x = row16_sum(x);
+    return (read_lane(x, 0) + read_lane(x, 16)) + (read_lane(x, 32) + read_lane(x, 48));
+}
+
+// Sum over aligned groups of G lanes, in every lane of the group: DPP steps
+// within a 16-lane row, shuffles only across rows (G = 32, 64).
+template <int G>
+__device__ inline double group_sum(double v)
+{
+    static_assert(G >= 4 && G <= 64 && (G & (G - 1)) == 0, "group of 4..64 lanes");
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    if constexpr (G >= 8) v += dpp_mov<0x141>(v);
+    if constexpr (G >= 16) v += dpp_mov<0x140>(v);
+    if constexpr (G >= 32) v += __shfl_xor(v, 16);
+    if constexpr (G >= 64) v += __shfl_xor(v, 32);
     return v;
 }
 
@@ -121,9 +169,9 @@ __global__ __launch_bounds__(RED_WG) void k_spmv(int nfree, const int *__restric
             a1 += (double)v0[len + j] * xv;
             a2 += (double)v0[2 * len + j] * xv;
         }
-        a0 = wave_group_sum(a0, G);
-        a1 = wave_group_sum(a1, G);
-        a2 = wave_group_sum(a2, G);
+        a0 = group_sum<G>(a0);
+        a1 = group_sum<G>(a1);
+        a2 = group_sum<G>(a2);
         if (lane < 3) {
             double a = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
             y[3 * r + lane] = (T)a;
@@ -157,29 +205,27 @@ struct CGParts {
     double *pq;  // CG_NB: <p,Kp>
 };
 
-// Sum over the WG threads of a workgroup (fixed tree), result in every
-// thread: xor tree within each wave, then every group of WG/64 lanes reads the
-// wave sums and xor-reduces them (one LDS read per value and thread).
+// Sum over the WG threads of a workgroup (fixed order), result in every
+// thread: wave sums, one LDS slot per wave, then lanes 0..NW-1 of every wave
+// read the NW wave sums and reduce them as a 16-lane row.
 template <int NV, int WG = CG_WG>
 __device__ inline void cg_block_sum(double v[NV])
 {
     constexpr int NW = WG / 64;
-    static_assert(NW >= 1 && NW <= 64 && (NW & (NW - 1)) == 0, "power-of-two wave count");
+    static_assert(NW >= 1 && NW <= 16, "at most 16 waves");
     __shared__ double sm[NV][NW];
 #pragma unroll
-    for (int i = 0; i < NV; ++i)
-        for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off);
-    __syncthreads();
+    for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+    __syncthreads();  // an earlier use of sm by this workgroup is complete
     if ((threadIdx.x & 63) == 0)
 #pragma unroll
         for (int i = 0; i < NV; ++i) sm[i][threadIdx.x >> 6] = v[i];
     __syncthreads();
-    const int l = threadIdx.x & (NW - 1);
+    const int l = threadIdx.x & 63;
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
-        double t = sm[i][l];
-        for (int off = NW >> 1; off > 0; off >>= 1) t += __shfl_xor(t, off);
-        v[i] = t;
+        double t = l < NW ? sm[i][l] : 0.0;
+        v[i] = read_lane(row16_sum(t), 0);
     }
 }
 
@@ -217,12 +263,14 @@ __device__ inline void cg_range(int n, int &i0, int &i1)
 
 // Iteration start: convergence / reliable-update / maxit checks on the
 // residual of the previous update, then p_new = z + beta p_old folded into
-// q = K p_new, and the partial <p_new, q>.
-template <class TV, class T, int G>
+// q = K p_new, and the partial <p_new, q>.  With the coarse space (CO) the
+// workgroup also writes its range's partial of Z^T q, from which the update
+// kernel advances Z^T r without another pass over r.
+template <class TV, class T, int G, bool CO>
 __global__ __launch_bounds__(CG_WG) void k_cg_spmv(int bi, int nfree, const int *__restrict__ node_blk,
                                                    const int *__restrict__ bcol, const TV *__restrict__ val,
                                                    const T *__restrict__ z, T *__restrict__ pA, T *__restrict__ pB,
-                                                   T *__restrict__ q, CGParts P, DevScalars *sc)
+                                                   T *__restrict__ q, CGParts P, CoarseArgs C, DevScalars *sc)
 {
     const int done0 = sc->done, par = sc->par0 ^ (bi & 1);
     const T *p_old = par ? pB : pA;
@@ -236,7 +284,11 @@ __global__ __launch_bounds__(CG_WG) void k_cg_spmv(int bi, int nfree, const int 
     int r = r0 + (int)threadIdx.x / G;
     int nb0 = 0, len = 0;
     double m0 = 0.0, m1 = 0.0, m2 = 0.0, zf = 0.0, pf = 0.0;
+    float dr[3] = {0.f, 0.f, 0.f};  // coarse offsets of node r (CO), loaded a pass ahead of their use
     if (r < r1) {
+        if constexpr (CO)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) dr[c] = C.d[3 * r + c];
         nb0 = node_blk[r];
         len = 3 * (node_blk[r + 1] - nb0);
         if (lane < len) {
@@ -274,11 +326,14 @@ __global__ __launch_bounds__(CG_WG) void k_cg_spmv(int bi, int nfree, const int 
     }
     if (dn) return;
     const double beta = rz / rzp;  // rzp = inf at a start: beta = 0
-    double dot = 0.0;
+    double dot = 0.0, zq = 0.0;  // zq: component (lane < 6) of Z^T q
     for (bool first = true; r < r1; r += NPW, first = false) {
         if (!first) {
             nb0 = node_blk[r];
             len = 3 * (node_blk[r + 1] - nb0);
+            if constexpr (CO)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) dr[c] = C.d[3 * r + c];
         }
         const TV *v0 = val + 9 * (long long)nb0;
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
@@ -300,9 +355,9 @@ __global__ __launch_bounds__(CG_WG) void k_cg_spmv(int bi, int nfree, const int 
             a1 += (double)v0[len + j] * pv;
             a2 += (double)v0[2 * len + j] * pv;
         }
-        a0 = wave_group_sum(a0, G);
-        a1 = wave_group_sum(a1, G);
-        a2 = wave_group_sum(a2, G);
+        a0 = group_sum<G>(a0);
+        a1 = group_sum<G>(a1);
+        a2 = group_sum<G>(a2);
         if (lane < 3) {
             const double a = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
             const int k = 3 * r + lane;
@@ -311,10 +366,36 @@ __global__ __launch_bounds__(CG_WG) void k_cg_spmv(int bi, int nfree, const int 
             q[k] = (T)a;
             dot += a * (double)pn;
         }
+        if constexpr (CO) {
+            // Z_r^T q_r = (q_r, d_r x q_r) from the stored (T) values; lane c < 6 of the
+            // node's group accumulates component c (one register instead of six)
+            if (lane < 6) {
+                const double q0 = (double)(T)a0, q1 = (double)(T)a1, q2 = (double)(T)a2;
+                if (lane < 3) {
+                    zq += lane == 0 ? q0 : (lane == 1 ? q1 : q2);
+                } else {
+                    const double d0 = dr[0], d1 = dr[1], d2 = dr[2];
+                    zq += lane == 3 ? d1 * q2 - d2 * q1 : (lane == 4 ? d2 * q0 - d0 * q2 : d0 * q1 - d1 * q0);
+                }
+            }
+        }
     }
-    double part[1] = {dot};
-    double *const dst[1] = {P.pq};
-    put_partials<1>(part, dst);
+    if constexpr (CO) {
+        double v[7] = {dot, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v[1 + c] = lane == c ? zq : 0.0;
+        cg_block_sum<7>(v);
+        if (threadIdx.x == 0) {
+            P.pq[blockIdx.x] = v[0];
+            const int b = xcd_remap(blockIdx.x, gridDim.x);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) C.zq[c * CG_NB + b] = v[1 + c];
+        }
+    } else {
+        double part[1] = {dot};
+        double *const dst[1] = {P.pq};
+        put_partials<1>(part, dst);
+    }
 }
 
 // preconditioner application on node r: z_r = M_r r_r (point or 3x3 block)
@@ -413,13 +494,69 @@ __global__ __launch_bounds__(UP_WG) void k_cg_update(int bi, int nfree, XT *__re
         }
     };
     if (i < i1) load_node(i);
+    // Coarse part (CO): y = A_c^-1 Z^T r of this workgroup's aggregate ag follows
+    // the recursion y_new = y_old - alpha A_c^-1 Z^T q (r_new = r - alpha q).
+    // Thread k (< 3 UP_WG) holds coarse columns k, k + UP_WG, k + 2 UP_WG: the
+    // summed Z^T q partials of the column's aggregate and the six A_c^-1
+    // entries of rows 6 ag .. 6 ag + 5.  The six products are reduced together
+    // with <p,q> and <r,z>.  Both workgroups of an aggregate sum in the same
+    // order and hold the same y; the first one stores it for the next iteration.
+    constexpr int KC = CO ? 3 : 1;
+    static_assert(!CO || CG_NC_MAX <= KC * UP_WG, "coarse columns per thread");
+    double zqs[KC], y[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    float am[KC][6];
+    int ag = 0;
+    bool own_agg = false;
+    if constexpr (CO) {
+        const int b = xcd_remap(blockIdx.x, gridDim.x);
+        ag = b / C.ra;
+        own_agg = ag < C.nagg;  // trailing workgroups without nodes may lie past the last aggregate
+        const int nc = 6 * C.nagg;
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk) {
+            const int k = threadIdx.x + kk * UP_WG;
+            zqs[kk] = 0.0;
+#pragma unroll
+            for (int m = 0; m < 6; ++m) am[kk][m] = 0.f;
+            if (own_agg && k < nc) {
+#pragma unroll
+                for (int m = 0; m < 6; ++m) am[kk][m] = C.ainv[(size_t)(6 * ag + m) * nc + k];
+                const int aa = k / 6, c = k - 6 * aa;
+                const double *zp = C.zq + c * CG_NB + aa * C.ra;  // written by the SpMV kernel
+                const int nb = min(C.ra, CG_NB - aa * C.ra);
+                for (int b2 = 0; b2 < nb; ++b2) zqs[kk] += zp[b2];
+            }
+        }
+        if (own_agg)
+#pragma unroll
+            for (int m = 0; m < 6; ++m) y[m] = C.ya[par * CG_NC_MAX + 6 * ag + m];
+    }
     if (done0) return;
-    double red[2];
-    {
+    double pq, rz, yq[6];
+    if constexpr (CO) {
+        static_assert(UP_WG == CG_NB, "one partial per thread");
+        double v[8];
+        v[0] = P.pq[threadIdx.x];
+        v[1] = P.rz[par * CG_NB + threadIdx.x];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            double t = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk) t += (double)am[kk][m] * zqs[kk];
+            v[2 + m] = t;
+        }
+        cg_block_sum<8, UP_WG>(v);
+        pq = v[0];
+        rz = v[1];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) yq[m] = v[2 + m];
+    } else {
+        double red[2];
         const double *const arr[2] = {P.pq, P.rz + par * CG_NB};
         red_sums<2, UP_WG>(arr, red);
+        pq = red[0];
+        rz = red[1];
     }
-    const double pq = red[0], rz = red[1];
     if (!(pq > 0.0) || !isfinite(pq)) {  // breakdown: K or M not SPD
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             sc->done = 3;
@@ -428,7 +565,17 @@ __global__ __launch_bounds__(UP_WG) void k_cg_update(int bi, int nfree, XT *__re
         return;
     }
     const double alpha = rz / pq;
-    double rzn = 0.0, rr2 = 0.0, g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if constexpr (CO) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) y[m] -= alpha * yq[m];
+        if (own_agg && xcd_remap(blockIdx.x, gridDim.x) % C.ra == 0 && threadIdx.x < 6) {
+            double t = y[0];
+#pragma unroll
+            for (int m = 1; m < 6; ++m) t = (int)threadIdx.x == m ? y[m] : t;
+            C.ya[(par ^ 1) * CG_NC_MAX + 6 * ag + threadIdx.x] = t;
+        }
+    }
+    double rzn = 0.0, rr2 = 0.0;
     for (; i < i1; i += UP_WG) {
         if (i >= i0 + UP_WG) load_node(i);  // ranges longer than one workgroup
         T zv[3];
@@ -446,41 +593,43 @@ __global__ __launch_bounds__(UP_WG) void k_cg_update(int bi, int nfree, XT *__re
 #pragma unroll
             for (int c = 0; c < 3; ++c) zv[c] = mv[3 * c] * rv[0] + mv[3 * c + 1] * rv[1] + mv[3 * c + 2] * rv[2];
         }
+        if constexpr (CO) {  // z_i += (Z y)_i = t + theta x d_i of this workgroup's aggregate
+            const double d0 = dv[0], d1 = dv[1], d2 = dv[2];
+            const double zc[3] = {y[0] + y[4] * d2 - y[5] * d1, y[1] + y[5] * d0 - y[3] * d2,
+                                  y[2] + y[3] * d1 - y[4] * d0};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) zv[c] = (T)((double)zv[c] + zc[c]);
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             z[3 * i + c] = zv[c];
             rzn += (double)rv[c] * (double)zv[c];
             rr2 += (double)rv[c] * (double)rv[c];
         }
-        if constexpr (CO) {
-            const double d0 = dv[0], d1 = dv[1], d2 = dv[2];
-            const double r0 = rv[0], r1 = rv[1], r2 = rv[2];
-            g[0] += r0;
-            g[1] += r1;
-            g[2] += r2;
-            g[3] += d1 * r2 - d2 * r1;
-            g[4] += d2 * r0 - d0 * r2;
-            g[5] += d0 * r1 - d1 * r0;
-        }
     }
-    put_rz_rr_g<CO, UP_WG>(rzn, rr2, g, P.rz + (par ^ 1) * CG_NB, P.rr, C);
+    double v[2] = {rzn, rr2};
+    cg_block_sum<2, UP_WG>(v);
+    if (threadIdx.x == 0) {
+        P.rz[(par ^ 1) * CG_NB + blockIdx.x] = v[0];
+        P.rr[blockIdx.x] = v[1];
+    }
 }
 
-// Coarse correction, one workgroup per aggregate a: sums the per-range
-// partials of Z^T r (all aggregates, same order in every workgroup), forms
-// y_a = rows 6a..6a+5 of A_c^-1 (Z^T r), adds (Z y)_i = t_a + theta_a x d_i
-// to z_i for the nodes of aggregate a (z then holds the full two-level M r)
-// and adds the aggregate's <r, Z y> to one <r, z> partial.  mode 0: after the update of
-// batch index bi (slot of the next iteration); mode 1: after a residual
-// replacement (slot par0).
+// Coarse correction after a start or residual replacement (k_replace), one
+// workgroup per aggregate a: sums the per-range partials of Z^T r (all
+// aggregates, same order in every workgroup), forms y_a = rows 6a..6a+5 of
+// A_c^-1 (Z^T r), adds (Z y)_i = t_a + theta_a x d_i to z_i for the nodes of
+// aggregate a (z then holds the full two-level M r) and adds the aggregate's
+// <r, Z y> to one <r, z> partial of slot par0.  y is stored (slot par0) as
+// the start of the update kernels' recursion.
 template <class T>
-__global__ __launch_bounds__(CG_WG) void k_coarse(int bi, int mode, int nfree, int nagg, const float *__restrict__ ainv,
+__global__ __launch_bounds__(CG_WG) void k_coarse(int nfree, int nagg, const float *__restrict__ ainv,
                                                   const T *__restrict__ r, T *__restrict__ z, double *rz_part,
                                                   CoarseArgs C, DevScalars *sc)
 {
     // every load that does not depend on the scalars is issued before the
     // done-check, so their latencies overlap
-    const int done = sc->done, par0 = sc->par0;
+    const int done = sc->done, slot = sc->par0;
     const int nc = 6 * nagg;  // <= 6 CG_NB / CG_AGG <= 2 CG_WG: at most two coarse columns per thread
     const int a = blockIdx.x;
     const int i0 = min(a * C.ra * C.per, nfree), i1 = min(i0 + C.ra * C.per, nfree);
@@ -511,8 +660,13 @@ __global__ __launch_bounds__(CG_WG) void k_coarse(int bi, int mode, int nfree, i
             rv[c] = (double)r[3 * i + c];
         }
     if (done) return;
-    const int slot = mode ? par0 : (par0 ^ (bi & 1) ^ 1);
     cg_block_sum<6>(y);
+    if (threadIdx.x < 6) {  // y of aggregate a for the update kernels' recursion
+        double t = y[0];
+#pragma unroll
+        for (int m = 1; m < 6; ++m) t = (int)threadIdx.x == m ? y[m] : t;
+        C.ya[slot * CG_NC_MAX + 6 * a + threadIdx.x] = t;
+    }
     double rzc = 0.0;
     for (; i < i1; i += CG_WG) {
         if (i >= i0 + CG_WG)  // aggregates longer than the workgroup
@@ -680,7 +834,7 @@ void solver_free(ofea_ctx *ctx)
     for (float *p : {w->v32.r, w->v32.z, w->v32.p, w->v32.p2, w->v32.q, w->minv32, w->val32, w->learned32})
         if (p) (void)hipFree(p);
     if (w->learned64) (void)hipFree(w->learned64);
-    for (void *p : {(void *)w->co.g, (void *)w->co.d, (void *)w->co_ainv})
+    for (void *p : {(void *)w->co.g, (void *)w->co.d, (void *)w->co_ainv, (void *)w->co.zq, (void *)w->co.ya})
         if (p) (void)hipFree(p);
     if (w->sc) (void)hipFree(w->sc);
     if (w->sc_host) (void)hipHostFree(w->sc_host);
@@ -784,8 +938,8 @@ static void cg_batch(ofea_ctx *ctx, const TV *val, VecSet<T> &v, XT *x, const T 
     for (int bi = 0; bi < CG_BATCH; ++bi) {
 #define OFEA_CG_CASE(G)                                                                                          \
     case G:                                                                                                      \
-        hipLaunchKernelGGL((k_cg_spmv<TV, T, G>), dim3(CG_NB), dim3(CG_WG), 0, ctx->stream, bi, w->nfree,     \
-                           ctx->d_node_blk, ctx->d_bcol, val, v.z, v.p, v.p2, v.q, P, w->sc);                   \
+        hipLaunchKernelGGL((k_cg_spmv<TV, T, G, CO>), dim3(CG_NB), dim3(CG_WG), 0, ctx->stream, bi, w->nfree, \
+                           ctx->d_node_blk, ctx->d_bcol, val, v.z, v.p, v.p2, v.q, P, w->co, w->sc);            \
         break;
         switch (w->spmv_group) {
             OFEA_CG_CASE(4)
@@ -798,9 +952,6 @@ static void cg_batch(ofea_ctx *ctx, const TV *val, VecSet<T> &v, XT *x, const T 
 #undef OFEA_CG_CASE
         hipLaunchKernelGGL((k_cg_update<T, XT, PC, CO>), dim3(CG_NB), dim3(UP_WG), 0, ctx->stream, bi, w->nfree, x,
                            v.r, v.z, v.p, v.p2, v.q, w->dinv, minv, P, w->co, w->sc);
-        if (CO)
-            hipLaunchKernelGGL((k_coarse<T>), dim3(w->co_nagg), dim3(CG_WG), 0, ctx->stream, bi, 0, w->nfree, w->co_nagg,
-                               w->co_ainv, v.r, v.z, w->part_rz, w->co, w->sc);
     }
     hipLaunchKernelGGL(k_cg_batch_end, dim3(1), dim3(64), 0, ctx->stream, w->sc, CG_BATCH);
 }
@@ -871,7 +1022,7 @@ static int pcg_run(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T
     hipLaunchKernelGGL((k_replace<T, PC, CO>), dim3(CG_NB), dim3(UP_WG), 0, ctx->stream, w->nfree, w->r64, v.r, v.z,
                        1, w->dinv, minv, P, w->co, w->sc);
     if (CO)
-        hipLaunchKernelGGL((k_coarse<T>), dim3(w->co_nagg), dim3(CG_WG), 0, ctx->stream, 0, 1, w->nfree, w->co_nagg,
+        hipLaunchKernelGGL((k_coarse<T>), dim3(w->co_nagg), dim3(CG_WG), 0, ctx->stream, w->nfree, w->co_nagg,
                            w->co_ainv, v.r, v.z, w->part_rz, w->co, w->sc);
     OFEA_HIP(hipGetLastError());
     // OFEA_NO_GRAPH=1 launches the batches directly (for tools that cannot follow graph launches)
@@ -934,7 +1085,7 @@ static int pcg_run(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T
         hipLaunchKernelGGL((k_replace<T, PC, CO>), dim3(CG_NB), dim3(UP_WG), 0, ctx->stream, w->nfree, w->r64, v.r,
                            v.z, 0, w->dinv, minv, P, w->co, w->sc);
         if (CO)
-            hipLaunchKernelGGL((k_coarse<T>), dim3(w->co_nagg), dim3(CG_WG), 0, ctx->stream, 0, 1, w->nfree, w->co_nagg,
+            hipLaunchKernelGGL((k_coarse<T>), dim3(w->co_nagg), dim3(CG_WG), 0, ctx->stream, w->nfree, w->co_nagg,
                                w->co_ainv, v.r, v.z, w->part_rz, w->co, w->sc);
         OFEA_HIP(hipGetLastError());
     }
@@ -1087,13 +1238,20 @@ static int coarse_prepare(ofea_ctx *ctx)
         OFEA_HIP(hipMalloc((void **)&w->co.g, sizeof(double) * 6 * CG_NB));
         OFEA_HIP(hipMalloc((void **)&w->co.d, sizeof(float) * 3 * (size_t)std::max(nf, 1)));
         OFEA_HIP(hipMalloc((void **)&w->co_ainv, sizeof(float) * 36 * (CG_NB / CG_AGG) * (CG_NB / CG_AGG)));
+        OFEA_HIP(hipMalloc((void **)&w->co.zq, sizeof(double) * 6 * CG_NB));
+        OFEA_HIP(hipMalloc((void **)&w->co.ya, sizeof(double) * 2 * CG_NC_MAX));
+        w->co.ainv = w->co_ainv;
     }
+    static_assert(CG_NC_MAX == 6 * (CG_NB / CG_AGG), "coarse dimension bound");
     OFEA_HIP(hipMemset(w->co.g, 0, sizeof(double) * 6 * CG_NB));
+    OFEA_HIP(hipMemset(w->co.zq, 0, sizeof(double) * 6 * CG_NB));
+    OFEA_HIP(hipMemset(w->co.ya, 0, sizeof(double) * 2 * CG_NC_MAX));
     OFEA_HIP(hipMemcpy((void *)w->co.d, d.data(), sizeof(float) * d.size(), hipMemcpyHostToDevice));
     OFEA_HIP(hipMemcpy(w->co_ainv, ainv.data(), sizeof(float) * ainv.size(), hipMemcpyHostToDevice));
     w->co.per = per;
     w->co.ra = ra;
     w->co_nagg = nagg;
+    w->co.nagg = nagg;
     w->co_d_host = std::move(d);
     w->co_ainv_host = std::move(ainv);
     w->co_valid = true;

@@ -37,11 +37,16 @@ constexpr int NSLOT = 10;
 // aggregate is ra >= CG_AGG consecutive CG ranges (the node ranges of the
 // fixed CG grid), so Z^T r is a per-range partial.
 constexpr int CG_AGG = 2;
+constexpr int CG_NC_MAX = 6 * (512 / CG_AGG);  // coarse dimension bound (512 CG ranges)
 struct CoarseArgs {
-    const float *d = nullptr;  // 3 * nfree: node position minus its aggregate centroid
-    double *g = nullptr;       // 6 * CG_NB: per-range partials of Z^T r, component-major
-    int per = 1;               // nodes per CG range
-    int ra = CG_AGG;           // CG ranges per aggregate
+    const float *d = nullptr;     // 3 * nfree: node position minus its aggregate centroid
+    double *g = nullptr;          // 6 * CG_NB: per-range partials of Z^T r, component-major
+    double *zq = nullptr;         // 6 * CG_NB: per-range partials of Z^T q (q = K p), component-major
+    double *ya = nullptr;         // 2 * CG_NC_MAX: y = A_c^-1 Z^T r per aggregate (6 each), by iteration parity
+    const float *ainv = nullptr;  // A_c^-1, (6 nagg)^2 row-major, FP32
+    int per = 1;                  // nodes per CG range
+    int ra = CG_AGG;              // CG ranges per aggregate
+    int nagg = 0;                 // aggregates
 };
 
 }  // namespace ofea
