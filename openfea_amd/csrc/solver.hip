@@ -229,16 +229,43 @@ __device__ inline void cg_block_sum(double v[NV])
     }
 }
 
-// Totals of NV partial arrays (CG_NB entries each), same value everywhere.
-// The partials were written by an earlier kernel on the same stream; the
-// kernel boundary makes them visible, so plain (L2-cached) loads suffice.
+// Total of N doubles at col (LDS or global), in every lane of the calling
+// wave: lane l adds entries l, l + 64, ... in that order, then a wave sum.
+template <int N>
+__device__ inline double col_sum(const double *col)
+{
+    const int l = threadIdx.x & 63;
+    double t = 0.0;
+    if constexpr (N >= 64) {
+        static_assert(N % 64 == 0, "whole waves");
+#pragma unroll
+        for (int k = 0; k < N / 64; ++k) t += col[l + 64 * k];
+    } else {
+        t = l < N ? col[l] : 0.0;
+    }
+    return wave_sum(t);
+}
+
+// Totals of NV partial arrays (CG_NB entries each), same value in every
+// thread: wave w < NV sums array w, the totals pass through LDS (one call
+// per kernel).  The partials were written by an earlier kernel on the same
+// stream; the kernel boundary makes them visible, so plain (L2-cached) loads
+// suffice.
 template <int NV, int WG = CG_WG>
 __device__ inline void red_sums(const double *const arr[NV], double out[NV])
 {
-    static_assert(WG >= CG_NB, "one partial per thread");
+    static_assert(NV <= WG / 64, "one wave per array");
+    __shared__ double s_tot[NV];
+    const int w = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i < NV; ++i) out[i] = threadIdx.x < CG_NB ? arr[i][threadIdx.x] : 0.0;
-    cg_block_sum<NV, WG>(out);
+    for (int i = 0; i < NV; ++i)
+        if (w == i) {
+            const double t = col_sum<CG_NB>(arr[i]);
+            if ((threadIdx.x & 63) == 0) s_tot[i] = t;
+        }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) out[i] = s_tot[i];
 }
 
 // One partial per workgroup and array (thread 0 writes).
@@ -326,7 +353,12 @@ __global__ __launch_bounds__(CG_WG) void k_cg_spmv(int bi, int nfree, const int 
     }
     if (dn) return;
     const double beta = rz / rzp;  // rzp = inf at a start: beta = 0
-    double dot = 0.0, zq = 0.0;  // zq: component (lane < 6) of Z^T q
+    // dot: lane c < 3 of a node group accumulates q_c p_c; zq[u]: lane c
+    // accumulates component c + u G (< 6) of Z^T q
+    constexpr int ZC = G >= 6 ? 1 : 2;
+    double dot = 0.0, zq[ZC];
+#pragma unroll
+    for (int u = 0; u < ZC; ++u) zq[u] = 0.0;
     for (bool first = true; r < r1; r += NPW, first = false) {
         if (!first) {
             nb0 = node_blk[r];
@@ -367,34 +399,42 @@ __global__ __launch_bounds__(CG_WG) void k_cg_spmv(int bi, int nfree, const int 
             dot += a * (double)pn;
         }
         if constexpr (CO) {
-            // Z_r^T q_r = (q_r, d_r x q_r) from the stored (T) values; lane c < 6 of the
-            // node's group accumulates component c (one register instead of six)
-            if (lane < 6) {
-                const double q0 = (double)(T)a0, q1 = (double)(T)a1, q2 = (double)(T)a2;
-                if (lane < 3) {
-                    zq += lane == 0 ? q0 : (lane == 1 ? q1 : q2);
-                } else {
-                    const double d0 = dr[0], d1 = dr[1], d2 = dr[2];
-                    zq += lane == 3 ? d1 * q2 - d2 * q1 : (lane == 4 ? d2 * q0 - d0 * q2 : d0 * q1 - d1 * q0);
-                }
+            // Z_r^T q_r = (q_r, d_r x q_r) from the stored (T) values
+            const double q0 = (double)(T)a0, q1 = (double)(T)a1, q2 = (double)(T)a2;
+            const double d0 = dr[0], d1 = dr[1], d2 = dr[2];
+#pragma unroll
+            for (int u = 0; u < ZC; ++u) {
+                const int c = lane + u * G;
+                if (c < 6)
+                    zq[u] += c == 0 ? q0
+                           : c == 1 ? q1
+                           : c == 2 ? q2
+                           : c == 3 ? d1 * q2 - d2 * q1
+                           : c == 4 ? d2 * q0 - d0 * q2
+                                    : d0 * q1 - d1 * q0;
             }
         }
     }
-    if constexpr (CO) {
-        double v[7] = {dot, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    // Partials: every node group stores its lane values in LDS (column c, row
+    // = group); wave 0 sums the three <p,q> columns, wave 1 + c column c of
+    // Z^T q, and lane 0 of each writes the workgroup's partial.
+    {
+        constexpr int NQ = CO ? 9 : 3;
+        __shared__ double s_q[NQ][NPW];
+        const int gi = (int)threadIdx.x / G, w = (int)threadIdx.x >> 6;
+        if (lane < 3) s_q[lane][gi] = dot;
+        if constexpr (CO)
 #pragma unroll
-        for (int c = 0; c < 6; ++c) v[1 + c] = lane == c ? zq : 0.0;
-        cg_block_sum<7>(v);
-        if (threadIdx.x == 0) {
-            P.pq[blockIdx.x] = v[0];
-            const int b = xcd_remap(blockIdx.x, gridDim.x);
-#pragma unroll
-            for (int c = 0; c < 6; ++c) C.zq[c * CG_NB + b] = v[1 + c];
+            for (int u = 0; u < ZC; ++u)
+                if (lane + u * G < 6) s_q[3 + lane + u * G][gi] = zq[u];
+        __syncthreads();
+        if (w == 0) {
+            const double t = (col_sum<NPW>(s_q[0]) + col_sum<NPW>(s_q[1])) + col_sum<NPW>(s_q[2]);
+            if (threadIdx.x == 0) P.pq[blockIdx.x] = t;
+        } else if (CO && w <= 6) {
+            const double t = col_sum<NPW>(s_q[2 + w]);
+            if ((threadIdx.x & 63) == 0) C.zq[(w - 1) * CG_NB + xcd_remap(blockIdx.x, gridDim.x)] = t;
         }
-    } else {
-        double part[1] = {dot};
-        double *const dst[1] = {P.pq};
-        put_partials<1>(part, dst);
     }
 }
 
@@ -532,31 +572,40 @@ __global__ __launch_bounds__(UP_WG) void k_cg_update(int bi, int nfree, XT *__re
             for (int m = 0; m < 6; ++m) y[m] = C.ya[par * CG_NC_MAX + 6 * ag + m];
     }
     if (done0) return;
-    double pq, rz, yq[6];
+    // wave 0: <p,q>, wave 1: <r,z> from the partial arrays; CO: every thread
+    // stores its six A_c^-1 Z^T q products (column m, row = thread) and wave
+    // 2 + m sums column m.  Totals pass through LDS to every thread.
+    constexpr int NT = CO ? 8 : 2;
+    static_assert(NT <= UP_WG / 64, "one wave per total");
+    __shared__ double s_tot[NT];
+    __shared__ double s_col[CO ? 6 : 2][UP_WG];  // CO products, then the <r,z>, <r,r> partials
+    const int w = (int)threadIdx.x >> 6;
     if constexpr (CO) {
-        static_assert(UP_WG == CG_NB, "one partial per thread");
-        double v[8];
-        v[0] = P.pq[threadIdx.x];
-        v[1] = P.rz[par * CG_NB + threadIdx.x];
 #pragma unroll
         for (int m = 0; m < 6; ++m) {
             double t = 0.0;
 #pragma unroll
             for (int kk = 0; kk < KC; ++kk) t += (double)am[kk][m] * zqs[kk];
-            v[2 + m] = t;
+            s_col[m][threadIdx.x] = t;
         }
-        cg_block_sum<8, UP_WG>(v);
-        pq = v[0];
-        rz = v[1];
-#pragma unroll
-        for (int m = 0; m < 6; ++m) yq[m] = v[2 + m];
-    } else {
-        double red[2];
-        const double *const arr[2] = {P.pq, P.rz + par * CG_NB};
-        red_sums<2, UP_WG>(arr, red);
-        pq = red[0];
-        rz = red[1];
     }
+    if (w < 2) {
+        const double t = col_sum<CG_NB>(w == 0 ? P.pq : P.rz + par * CG_NB);
+        if ((threadIdx.x & 63) == 0) s_tot[w] = t;
+    }
+    __syncthreads();
+    if constexpr (CO) {
+        if (w >= 2) {
+            const double t = col_sum<UP_WG>(s_col[w - 2]);
+            if ((threadIdx.x & 63) == 0) s_tot[w] = t;
+        }
+        __syncthreads();
+    }
+    const double pq = s_tot[0], rz = s_tot[1];
+    double yq[6];
+    if constexpr (CO)
+#pragma unroll
+        for (int m = 0; m < 6; ++m) yq[m] = s_tot[2 + m];
     if (!(pq > 0.0) || !isfinite(pq)) {  // breakdown: K or M not SPD
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             sc->done = 3;
@@ -607,11 +656,15 @@ __global__ __launch_bounds__(UP_WG) void k_cg_update(int bi, int nfree, XT *__re
             rr2 += (double)rv[c] * (double)rv[c];
         }
     }
-    double v[2] = {rzn, rr2};
-    cg_block_sum<2, UP_WG>(v);
-    if (threadIdx.x == 0) {
-        P.rz[(par ^ 1) * CG_NB + blockIdx.x] = v[0];
-        P.rr[blockIdx.x] = v[1];
+    // partials <r,z> (wave 0) and <r,r> (wave 1) through LDS; the CO columns
+    // of s_col were last read before the second barrier above
+    s_col[0][threadIdx.x] = rzn;
+    s_col[1][threadIdx.x] = rr2;
+    __syncthreads();
+    if (w < 2) {
+        const double t = col_sum<UP_WG>(s_col[w]);
+        if (threadIdx.x == 0) P.rz[(par ^ 1) * CG_NB + blockIdx.x] = t;
+        if (threadIdx.x == 64) P.rr[blockIdx.x] = t;
     }
 }
 
