@@ -68,47 +68,75 @@ __device__ inline void gather_blocks(int b0, int b1, int t0, const int *__restri
 // Phase 2 is split by cost.  An off-diagonal block (r, s) has the few
 // elements that contain the edge r-s as contributors; the diagonal block
 // (r, r) has every element of r (about 24 in a Kuhn grid).  Off-diagonal
-// blocks get one lane each; each diagonal block gets three lanes, one per
-// row c, which sum the cheap symmetric form
+// blocks get one lane each; each diagonal block gets six lanes, two per row
+// c (each half of the node's pairs), which sum the cheap symmetric form
 //   K_rr[c][d] = sum_e ((lambda + mu) V g_c g_d + delta_cd mu V g.g)
-// over the node's pairs in element order.  The chunk's contributor list is
-// staged in LDS first, so the summation loops issue no global loads.
-constexpr int TET_CT = 4 * TET_CAP;  // contributor entries of a chunk (<= 4 per pair)
+// in element order and add the two halves.  Poisson's ratio is one value per
+// launch, so lambda_e = rho mu_e with rho = 2 nu / (1 - 2 nu) for every
+// element and only mu V is kept per pair:
+//   K_ab = rho Q + Q^T + I tr Q,  Q = sum_e mu_e V_e g_a (x) g_b.
+// The chunk's contributor list is staged in LDS first, so the summation
+// loops issue no global loads.
+constexpr int TET_CT = 3 * TET_CAP;  // off-diagonal contributor entries of a chunk (<= 3 per pair)
+static_assert(TET_CT % 256 == 0, "whole rounds of the workgroup");
 
-// What one phase-2 lane needs from global memory, loaded before phase 1 so
-// that the loads overlap the element computations.
+// value of the neighbouring lane (lane ^ 1), through DPP quad_perm [1,0,3,2]
+__device__ inline double dpp_xor1(double x)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0xB1, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// A phase-2 task: an off-diagonal block (r, k) or half of row c of the
+// diagonal block of r.  The global loads (the block's chunk-local node and
+// contributor range) are issued before phase 1 so that they overlap the
+// element computations; the rest comes from the chunk's node tables in LDS.
 struct TetTask {
-    int kind;         // 0 none, 1 off-diagonal block, 2 diagonal row
-    int k, c0, c1;    // block slot k in column r; contributor range (kind 1) or pair range (kind 2)
-    int c;            // row of the diagonal block (kind 2)
+    int kind;          // 0 none, 1 off-diagonal block, 2 diagonal half-row
+    int rl, k, c0, c1;  // chunk-local node, block slot k in column r; contributor range (1) or pair range (2)
+    int c;             // row of the diagonal block (kind 2)
     int base, stride;  // CSC offset of column 3r and the column length 3 deg(r)
 };
 
-__device__ inline TetTask tet_task(int task, int nblk, int b0, int n0, int t0, int p0, const int *__restrict__ blk_rk,
-                                   const int *__restrict__ contrib_ptr, const int *__restrict__ node_pair,
-                                   const int *__restrict__ col_ptr, const int *__restrict__ diag_k)
+// global part of a task (off-diagonal candidates): chunk-local node and
+// contributor range
+__device__ inline void tet_task_load(int task, int nblk, int b0, int t0, const uint8_t *__restrict__ blk_rl,
+                                     const int *__restrict__ off_ptr, TetTask &t)
 {
-    TetTask t{0, 0, 0, 0, 0, 0, 0};
-    int r;
+    t.kind = 0;
     if (task < nblk) {
-        const int bk = b0 + task, rk = blk_rk[bk];
-        r = rk >> 8;
-        t.k = rk & 255;
-        t.kind = t.k == diag_k[r] ? 0 : 1;
-        t.c0 = contrib_ptr[bk] - t0;
-        t.c1 = contrib_ptr[bk + 1] - t0;
+        const int bk = b0 + task;
+        t.rl = blk_rl[bk];
+        t.k = bk;  // global block index until resolved
+        t.c0 = off_ptr[bk] - t0;
+        t.c1 = off_ptr[bk + 1] - t0;
+        t.kind = 1;
     } else {
-        const int j = task - nblk;
-        r = n0 + j / 3;
-        t.c = j - 3 * (j / 3);
-        t.k = diag_k[r];
-        t.kind = t.k < 0 ? 0 : 2;
-        t.c0 = node_pair[r] - p0;
-        t.c1 = node_pair[r + 1] - p0;
+        const int j = (task - nblk) >> 1;  // half-row task 2 (3 node + c) + h
+        t.rl = j / 3;
+        t.c = j - 3 * t.rl;
+        t.kind = 2;
     }
-    t.base = col_ptr[3 * r];
-    t.stride = col_ptr[3 * r + 1] - t.base;
-    return t;
+}
+
+// LDS part: node tables s_nb (node_blk of the chunk's nodes, nnode + 1), s_dk
+// (diagonal slot), s_np (first pair, chunk-relative, nnode + 1)
+__device__ inline void tet_task_resolve(TetTask &t, const int *s_nb, const int *s_dk, const int *s_np)
+{
+    const int nb = s_nb[t.rl], deg = s_nb[t.rl + 1] - nb;
+    t.base = 9 * nb;
+    t.stride = 3 * deg;
+    if (t.kind == 1) {
+        t.k -= nb;
+        if (t.k == s_dk[t.rl]) t.kind = 0;  // the diagonal block: its rows are kind-2 tasks
+    } else {
+        t.k = s_dk[t.rl];
+        if (t.k < 0) t.kind = 0;
+        t.c0 = s_np[t.rl];
+        t.c1 = s_np[t.rl + 1];
+    }
 }
 
 __global__ __launch_bounds__(ASM_WG) void k_assemble_tet(const double *__restrict__ xyz, const int *__restrict__ conn,
@@ -117,10 +145,10 @@ __global__ __launch_bounds__(ASM_WG) void k_assemble_tet(const double *__restric
                                                          const int *__restrict__ chunk_blk,
                                                          const int *__restrict__ chunk_node,
                                                          const int *__restrict__ node_pair, const int *__restrict__ pair,
-                                                         const int *__restrict__ blk_rk,
-                                                         const int *__restrict__ contrib_ptr,
-                                                         const uint16_t *__restrict__ contrib,
-                                                         const int *__restrict__ col_ptr,
+                                                         const uint8_t *__restrict__ blk_rl,
+                                                         const int *__restrict__ off_ptr,
+                                                         const uint16_t *__restrict__ off_contrib,
+                                                         const int *__restrict__ node_blk,
                                                          const int *__restrict__ diag_k, double *__restrict__ val,
                                                          int accumulate, int nchunks)
 {
@@ -128,28 +156,46 @@ __global__ __launch_bounds__(ASM_WG) void k_assemble_tet(const double *__restric
     // Slot j of a pair holds the gradient of local node (a + j) & 3, so slot 0
     // is the owned node a; contributor entries name (pair, slot).
     __shared__ double s_g[12][TET_CAP];  // s_g[3 j + c][pair] = d N_(a+j) / d x_c
-    __shared__ double s_w[2][TET_CAP];   // lambda V, mu V (signed V)
+    __shared__ double s_w[TET_CAP];      // mu V (signed V)
     __shared__ uint16_t s_ct[TET_CT];
+    __shared__ int s_nb[PLAN_CHUNK_NODES + 1], s_dk[PLAN_CHUNK_NODES], s_np[PLAN_CHUNK_NODES + 1];
     const int chunk = xcd_remap(blockIdx.x, nchunks);
     const int p0 = chunk_pair[chunk], p1 = chunk_pair[chunk + 1];
     const int b0 = chunk_blk[chunk], b1 = chunk_blk[chunk + 1];
-    const int n0 = chunk_node[chunk], nnode = chunk_node[chunk + 1] - n0;
-    const int t0 = contrib_ptr[b0], nct = contrib_ptr[b1] - t0;  // <= 4 (p1 - p0)
+    const int n0 = chunk_node[chunk], nnode = chunk_node[chunk + 1] - n0;  // <= PLAN_CHUNK_NODES
+    const int t0 = off_ptr[b0], nct = off_ptr[b1] - t0;                    // <= 3 (p1 - p0)
     const int nblk = b1 - b0;
+    // Staged tables are loaded into registers here and stored to LDS after
+    // phase 1: a store right after its load would hold the wave until the load
+    // returns, before it has issued the element loads of phase 1.
+    const bool node_lane = (int)threadIdx.x <= nnode;
+    int nb_v = 0, np_v = 0, dk_v = 0;
+    if (node_lane) {
+        nb_v = node_blk[n0 + threadIdx.x];
+        np_v = node_pair[n0 + threadIdx.x] - p0;
+        if ((int)threadIdx.x < nnode) dk_v = diag_k[n0 + threadIdx.x];
+    }
+    constexpr int CT_PER = TET_CT / ASM_WG;
+    uint16_t ct_v[CT_PER];
+#pragma unroll
+    for (int u = 0; u < CT_PER; ++u) {
+        const int t = (int)threadIdx.x + u * ASM_WG;
+        ct_v[u] = t < nct ? off_contrib[t0 + t] : (uint16_t)0;
+    }
     // Phase-2 lanes: waves 0-2 take the off-diagonal blocks and wave 3 the
-    // diagonal rows, so that no wave runs both loop bodies.
+    // diagonal half-rows (task 2 (3 node + c) + h), so that no wave runs both
+    // loop bodies.
     constexpr int OFF_LANES = ASM_WG - 64;
     const bool diag_wave = (int)threadIdx.x >= OFF_LANES;
     const int lane2 = diag_wave ? (int)threadIdx.x - OFF_LANES : (int)threadIdx.x;
     const int step2 = diag_wave ? 64 : OFF_LANES;
-    const int ntask2 = diag_wave ? 3 * nnode : nblk;
+    const int ntask2 = diag_wave ? 6 * nnode : nblk;
     const int task_base = diag_wave ? nblk : 0;
-    TetTask first{0, 0, 0, 0, 0, 0, 0};
-    if (lane2 < ntask2)
-        first = tet_task(task_base + lane2, nblk, b0, n0, t0, p0, blk_rk, contrib_ptr, node_pair, col_ptr, diag_k);
-    for (int t = threadIdx.x; t < nct; t += ASM_WG) s_ct[t] = contrib[t0 + t];
+    TetTask first{0, 0, 0, 0, 0, 0, 0, 0};
+    if (lane2 < ntask2) tet_task_load(task_base + lane2, nblk, b0, t0, blk_rl, off_ptr, first);
     double lam0, mu0;
     lame<double>(E, nu, lam0, mu0);
+    const double rho = 2.0 * nu / (1.0 - 2.0 * nu);  // lambda / mu
     for (int p = p0 + (int)threadIdx.x; p < p1; p += ASM_WG) {
         const int pk = pair[p];
         const int el = pk >> 3, a = pk & 7, lp = p - p0;
@@ -171,35 +217,39 @@ __global__ __launch_bounds__(ASM_WG) void k_assemble_tet(const double *__restric
         double lam = lam0, mu = mu0;
         if (Eel) lame<double>(Eel[el], nu, lam, mu);
         const double V = detJ / 6.0;  // signed (src/Elements/C3D4.cpp:45-54 does not abs())
-        s_w[0][lp] = lam * V;
-        s_w[1][lp] = mu * V;
+        s_w[lp] = mu * V;
+    }
+#pragma unroll
+    for (int u = 0; u < CT_PER; ++u) {
+        const int t = (int)threadIdx.x + u * ASM_WG;
+        if (t < nct) s_ct[t] = ct_v[u];
+    }
+    if (node_lane) {
+        s_nb[threadIdx.x] = nb_v;
+        s_np[threadIdx.x] = np_v;
+        if ((int)threadIdx.x < nnode) s_dk[threadIdx.x] = dk_v;
     }
     __syncthreads();
     for (int t2 = lane2; t2 < ntask2; t2 += step2) {
-        const TetTask tk = t2 == lane2 ? first
-                                       : tet_task(task_base + t2, nblk, b0, n0, t0, p0, blk_rk, contrib_ptr,
-                                                  node_pair, col_ptr, diag_k);
-        if (tk.kind == 1) {  // off-diagonal block (r, k)
-            // P = sum lambda V g_a (x) g_b, Q = sum mu V g_a (x) g_b;
-            // K_ab = P + Q^T + I tr(Q)
-            double P[9], Q[9];
+        TetTask tk = first;
+        if (t2 != lane2) tet_task_load(task_base + t2, nblk, b0, t0, blk_rl, off_ptr, tk);
+        tet_task_resolve(tk, s_nb, s_dk, s_np);
+        if (tk.kind == 1) {  // off-diagonal block (r, k): K_ab = rho Q + Q^T + I tr(Q)
+            double Q[9];
 #pragma unroll
-            for (int i = 0; i < 9; ++i) P[i] = Q[i] = 0.0;
+            for (int i = 0; i < 9; ++i) Q[i] = 0.0;
 #pragma unroll 2
             for (int t = tk.c0; t < tk.c1; ++t) {
                 const int ci = s_ct[t];
                 const int lp = ci >> 2, j = ci & 3;
-                const double wl = s_w[0][lp], wm = s_w[1][lp];
+                const double wm = s_w[lp];
                 const double ga[3] = {s_g[0][lp], s_g[1][lp], s_g[2][lp]};
                 const double gb[3] = {s_g[3 * j][lp], s_g[3 * j + 1][lp], s_g[3 * j + 2][lp]};
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const double lc = wl * ga[c], mc = wm * ga[c];
+                    const double mc = wm * ga[c];
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        P[3 * c + d] += lc * gb[d];
-                        Q[3 * c + d] += mc * gb[d];
-                    }
+                    for (int d = 0; d < 3; ++d) Q[3 * c + d] += mc * gb[d];
                 }
             }
             const double trQ = Q[0] + Q[4] + Q[8];
@@ -208,35 +258,49 @@ __global__ __launch_bounds__(ASM_WG) void k_assemble_tet(const double *__restric
             for (int c = 0; c < 3; ++c)
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    double v = P[3 * c + d] + Q[3 * d + c] + (c == d ? trQ : 0.0);
+                    double v = rho * Q[3 * c + d] + Q[3 * d + c] + (c == d ? trQ : 0.0);
                     if (accumulate) v += dst[c * tk.stride + d];
                     dst[c * tk.stride + d] = v;
                 }
-        } else if (tk.kind == 2) {  // row c of the diagonal block: the node's pairs in element order
+        }
+        if (diag_wave) {  // half h of row c of the diagonal block, pairs in element order
+            // lanes 2m and 2m + 1 hold the two halves of one row; both run this
+            // branch in the same iteration (the task count is even)
             double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, accd = 0.0;
-            const int c = tk.c;
+            const int c = tk.c, h = t2 & 1;
+            const int mid = (tk.c0 + tk.c1 + 1) >> 1;
+            const int l0 = h ? mid : tk.c0, l1 = h ? tk.c1 : mid;
+            if (tk.kind == 2) {
 #pragma unroll 4
-            for (int lp = tk.c0; lp < tk.c1; ++lp) {
-                const double g0 = s_g[0][lp], g1 = s_g[1][lp], g2 = s_g[2][lp], gc = s_g[c][lp];
-                const double wl = s_w[0][lp], wm = s_w[1][lp];
-                const double sc = (wl + wm) * gc;
-                acc0 += sc * g0;
-                acc1 += sc * g1;
-                acc2 += sc * g2;
-                accd += wm * (g0 * g0 + g1 * g1 + g2 * g2);
+                for (int lp = l0; lp < l1; ++lp) {
+                    const double g0 = s_g[0][lp], g1 = s_g[1][lp], g2 = s_g[2][lp];
+                    const double wm = s_w[lp];
+                    const double sc = wm * (c == 0 ? g0 : (c == 1 ? g1 : g2));
+                    acc0 += sc * g0;
+                    acc1 += sc * g1;
+                    acc2 += sc * g2;
+                    accd += wm * (g0 * g0 + g1 * g1 + g2 * g2);
+                }
             }
-            acc0 += c == 0 ? accd : 0.0;
-            acc1 += c == 1 ? accd : 0.0;
-            acc2 += c == 2 ? accd : 0.0;
-            double *dst = val + tk.base + 3 * tk.k + c * tk.stride;
-            if (accumulate) {
-                acc0 += dst[0];
-                acc1 += dst[1];
-                acc2 += dst[2];
+            acc0 += dpp_xor1(acc0);
+            acc1 += dpp_xor1(acc1);
+            acc2 += dpp_xor1(acc2);
+            accd += dpp_xor1(accd);
+            if (tk.kind == 2 && h == 0) {
+                const double rho1 = rho + 1.0;
+                acc0 = rho1 * acc0 + (c == 0 ? accd : 0.0);
+                acc1 = rho1 * acc1 + (c == 1 ? accd : 0.0);
+                acc2 = rho1 * acc2 + (c == 2 ? accd : 0.0);
+                double *dst = val + tk.base + 3 * tk.k + c * tk.stride;
+                if (accumulate) {
+                    acc0 += dst[0];
+                    acc1 += dst[1];
+                    acc2 += dst[2];
+                }
+                dst[0] = acc0;
+                dst[1] = acc1;
+                dst[2] = acc2;
             }
-            dst[0] = acc0;
-            dst[1] = acc1;
-            dst[2] = acc2;
         }
     }
 }
@@ -323,8 +387,8 @@ int launch_assemble(ofea_ctx *ctx, double E, double nu, double *d_out, bool zero
         if (dp.nchunks == 0) continue;
         if (t == OFEA_C3D4)
             hipLaunchKernelGGL(k_assemble_tet, dim3(dp.nchunks), dim3(ASM_WG), 0, ctx->stream, ctx->d_xyz, dp.conn, dp.E,
-                               E, nu, dp.chunk_pair, dp.chunk_blk, dp.chunk_node, dp.node_pair, dp.pair, dp.blk_rk,
-                               dp.contrib_ptr, dp.contrib, ctx->d_col_ptr, ctx->d_diag_k, d_out, acc, dp.nchunks);
+                               E, nu, dp.chunk_pair, dp.chunk_blk, dp.chunk_node, dp.node_pair, dp.pair, dp.blk_rl,
+                               dp.off_ptr, dp.off_contrib, ctx->d_node_blk, ctx->d_diag_k, d_out, acc, dp.nchunks);
         else
             hipLaunchKernelGGL(k_assemble_hex, dim3(dp.nchunks), dim3(ASM_WG), 0, ctx->stream, ctx->d_xyz, dp.conn, dp.E,
                                E, nu, dp.chunk_pair, dp.chunk_blk, dp.pair, dp.blk_rk, dp.contrib_ptr, dp.contrib,

@@ -19,6 +19,9 @@ static void free_plan(DevPlan &d)
     dev_free(d.pair);
     dev_free(d.blk_rk);
     dev_free(d.contrib_ptr);
+    dev_free(d.off_ptr);
+    dev_free(d.off_contrib);
+    dev_free(d.blk_rl);
     dev_free(d.contrib);
     dev_free(d.node_pair);
     dev_free(d.chunk_node);
@@ -141,6 +144,9 @@ int ofea_set_mesh(ofea_ctx *c, int nd, const double *xyz, int ne, const int *con
         if ((rc = dev_upload(d.contrib, G.contrib.data(), std::max<size_t>(G.contrib.size(), 1)))) return rc;
         if ((rc = dev_upload(d.node_pair, G.node_pair.data(), G.node_pair.size()))) return rc;
         if ((rc = dev_upload(d.chunk_node, G.chunk_node.data(), G.chunk_node.size()))) return rc;
+        if ((rc = dev_upload(d.off_ptr, G.off_ptr.data(), G.off_ptr.size()))) return rc;
+        if ((rc = dev_upload(d.off_contrib, G.off_contrib.data(), std::max<size_t>(G.off_contrib.size(), 1)))) return rc;
+        if ((rc = dev_upload(d.blk_rl, G.blk_rl.data(), std::max<size_t>(G.blk_rl.size(), 1)))) return rc;
     }
     return OFEA_OK;
 }

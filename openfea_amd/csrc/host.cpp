@@ -375,7 +375,8 @@ int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const i
     G.chunk_pair.push_back(0);
     G.chunk_blk.push_back(0);
     G.contrib_ptr.push_back(0);
-    int chunk_first_pair = 0;
+    G.off_ptr.push_back(0);
+    int chunk_first_pair = 0, chunk_first_node = 0;
     int in_chunk = 0;
     std::vector<std::vector<uint16_t>> lists;
     for (int r = 0; r < nfree; ++r) {
@@ -383,11 +384,12 @@ int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const i
         if (np > pair_cap) return fail(OFEA_ERR_LIMIT, "a node has more incident elements than the gather chunk holds");
         int deg = P.node_blk[r + 1] - P.node_blk[r];
         if (deg > 255) return fail(OFEA_ERR_LIMIT, "a node has more than 255 neighbours");
-        if (in_chunk + np > pair_cap) {
+        if (in_chunk + np > pair_cap || r - chunk_first_node >= PLAN_CHUNK_NODES) {
             G.chunk_pair.push_back(cnt[r]);
             G.chunk_blk.push_back((int)G.blk_rk.size());
             G.chunk_node.push_back(r);
             chunk_first_pair = cnt[r];
+            chunk_first_node = r;
             in_chunk = 0;
         }
         in_chunk += np;
@@ -406,8 +408,11 @@ int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const i
         }
         for (int k = 0; k < deg; ++k) {
             G.blk_rk.push_back((r << 8) | k);
+            G.blk_rl.push_back((uint8_t)(r - chunk_first_node));
             G.contrib.insert(G.contrib.end(), lists[k].begin(), lists[k].end());
             G.contrib_ptr.push_back((int)G.contrib.size());
+            if (nbr[k] != r) G.off_contrib.insert(G.off_contrib.end(), lists[k].begin(), lists[k].end());
+            G.off_ptr.push_back((int)G.off_contrib.size());
         }
     }
     G.chunk_pair.push_back(cnt[nfree]);

@@ -55,7 +55,13 @@ struct GatherPlan {
     std::vector<int> contrib_ptr;        // blocks + 1
     std::vector<uint16_t> contrib;       // chunk-local pair * nn + slot: C3D8 slot = b, C3D4 slot = (b - a) & 3
     std::vector<int> node_pair;          // nfree + 1: pairs of node r = [node_pair[r], node_pair[r+1])
-    std::vector<int> chunk_node;         // nchunks + 1: first free node of each chunk
+    std::vector<int> chunk_node;         // nchunks + 1: first free node of each chunk (<= PLAN_CHUNK_NODES nodes)
+    // C3D4 linear kernel: contributor lists of the off-diagonal blocks only (its
+    // diagonal rows walk the node's pairs), and per block the node's index in
+    // its chunk
+    std::vector<int> off_ptr;            // blocks + 1
+    std::vector<uint16_t> off_contrib;   // as contrib, diagonal blocks empty
+    std::vector<uint8_t> blk_rl;         // r - chunk_node[chunk] per block
     int nchunks() const { return (int)chunk_pair.size() - 1; }
 };
 
@@ -68,6 +74,7 @@ int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const i
 // 8-lane Gauss-point group.
 constexpr int PLAN_TET_CAP = 256;
 constexpr int PLAN_HEX_CAP = 32;
+constexpr int PLAN_CHUNK_NODES = 64;  // free nodes per chunk (node tables staged in LDS)
 
 // ---------------------------------------------------------------- device plan copy
 struct DevPlan {
@@ -77,6 +84,9 @@ struct DevPlan {
     int *chunk_pair = nullptr, *chunk_blk = nullptr, *pair = nullptr, *blk_rk = nullptr, *contrib_ptr = nullptr;
     uint16_t *contrib = nullptr;
     int *node_pair = nullptr, *chunk_node = nullptr;
+    int *off_ptr = nullptr;
+    uint16_t *off_contrib = nullptr;
+    uint8_t *blk_rl = nullptr;
 };
 
 }  // namespace ofea
