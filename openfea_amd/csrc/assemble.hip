@@ -330,8 +330,11 @@ __global__ __launch_bounds__(ASM_WG) void k_assemble_hex(const double *__restric
     const int chunk = xcd_remap(blockIdx.x, nchunks);
     const int p0 = chunk_pair[chunk], p1 = chunk_pair[chunk + 1];
     const int b0 = chunk_blk[chunk], b1 = chunk_blk[chunk + 1];
-    const int t0 = contrib_ptr[b0], nct = contrib_ptr[b1] - t0;  // <= 8 (p1 - p0)
-    for (int t = threadIdx.x; t < nct; t += ASM_WG) s_ct[t] = contrib[t0 + t];
+    const int t0 = contrib_ptr[b0], nct = contrib_ptr[b1] - t0;  // <= 8 (p1 - p0) <= ASM_WG
+    static_assert(HEX_CAP * 8 <= ASM_WG, "one contributor entry per thread");
+    // loaded here, stored to LDS after phase 1 (a store right after its load
+    // would hold the wave before it issues the element loads)
+    const uint16_t ct_v = (int)threadIdx.x < nct ? contrib[t0 + threadIdx.x] : (uint16_t)0;
     if (p1 > p0) {  // uniform over the workgroup
         const int lp = threadIdx.x >> 3, gp = threadIdx.x & 7;
         const int p = min(p0 + lp, p1 - 1);  // idle groups recompute the last pair, store nothing
@@ -371,6 +374,7 @@ __global__ __launch_bounds__(ASM_WG) void k_assemble_hex(const double *__restric
             for (int i = 0; i < 9; ++i) s_blk[lp * 8 + gp][i] = mine[i];
         }
     }
+    if ((int)threadIdx.x < nct) s_ct[threadIdx.x] = ct_v;
     __syncthreads();
     HexSource src{s_blk};
     gather_blocks(b0, b1, t0, blk_rk, contrib_ptr, s_ct, col_ptr, val, accumulate, src);
