@@ -28,6 +28,14 @@
  *   ofea_lumped_mass      ele_base::build_ele_mass over all elements (src/cae.cpp:439-448)
  *   ofea_explicit_run     CAE_process::explicit_analysis time loop (src/cae.cpp:420-574)
  *   ofea_reset_displacement simulation_post::reset_displacement (src/post_process.cpp:18-40)
+ *   ofea_ca_info_*        ReadInfo::CA_read_change (src/data2cae.cpp:519-665)
+ *   ofea_ca_delta         ca_get_delt_stiffness / ca_fill_CSR_sparse_mat (src/ca_analysis.cpp:18-143)
+ *   ofea_ca_solve         ca_build_rom + ca_SVD + ca_solve (src/ca_analysis.cpp:146-309),
+ *                         driven as in CAE_process::CA_ReAnalysis (src/cae.cpp:317-376)
+ *   ofea_reset_ca_displacement simulation_post::reset_ca_displacement (src/post_process.cpp:42-76)
+ *   ofea_post_nodal       simulation_post::get_cauchy_stress_3d (src/post_process.cpp:79-168)
+ *   ofea_write_vtk        data_process::export_dis_2_vtk / CA_export_dis_2_vtk
+ *                         (src/data2vtk.cpp:47-264, :266-427)
  */
 #ifndef OPENFEA_AMD_H
 #define OPENFEA_AMD_H
@@ -64,12 +72,17 @@ extern "C" {
 #define OFEA_PCG_LEARNED_TWO_LEVEL 8        /* FP64 values, learned blocks + coarse           */
 #define OFEA_PCG_LEARNED_TWO_LEVEL_MIXED 9  /* FP32 values, learned blocks + coarse           */
 
+/* quantities of ofea_post_nodal */
+#define OFEA_POST_STRAIN 0  /* B u_e and its "Mises" value: what the reference writes as S_* */
+#define OFEA_POST_STRESS 1  /* Cauchy stress D B u_e and its von Mises value                */
+
 /* explicit precision for ofea_explicit_run */
 #define OFEA_FP32 0
 #define OFEA_FP64 1
 
 typedef struct ofea_ctx ofea_ctx;
 typedef struct ofea_deck ofea_deck;
+typedef struct ofea_ca_info ofea_ca_info;
 
 const char *ofea_last_error(void);
 /* library version string, e.g. "openfea_amd 0.1 gfx950" */
@@ -105,6 +118,33 @@ int ofea_build_csr(int nd, int ne, const int *conn, const int *etype, const int 
                    int *col_ptr, int *row_idx);
 /* reset_displacement: u_full (3*nd) from the free-DOF u (3*free). */
 int ofea_reset_displacement(int nd, const int *resort, const double *u_free, double *u_full);
+
+/* reset_ca_displacement: u_new (3*nd_new, zeroed first) receives, for every
+ * listed node i < nmap with idref2new[i] != -1, the displacement of node i of
+ * the original model (free-DOF u of that model, 0 at clamped nodes). */
+int ofea_reset_ca_displacement(int nd_all, const int *resort, const double *u_free, int nmap, const int *idref2new,
+                               int nd_new, double *u_new);
+
+/* ---- host drop-in: CA change file (no GPU needed) ---- */
+/* "*nodes_all, N" + "x, y, z" lines; "*eles_change, M, type=T" + "kind, n1..nk"
+ * lines (kind 0 delete, 1 move, 2 add; node ids 0-based); "*idref2new, N" +
+ * one index per line (-1 = node removed) up to "*End File". */
+int ofea_ca_info_open(const char *path, ofea_ca_info **info);
+int ofea_ca_info_sizes(const ofea_ca_info *info, int *nd_all, int *nmod, int *nmap);
+/* xyz_all nd_all*3, kind nmod, conn_mod nmod*8 (0-based, unused slots 0),
+ * etype_mod nmod, idref2new nmap; any may be NULL */
+int ofea_ca_info_get(const ofea_ca_info *info, double *xyz_all, int *kind, int *conn_mod, int *etype_mod,
+                     int *idref2new);
+void ofea_ca_info_close(ofea_ca_info *info);
+
+/* ---- host drop-in: VTK output (no GPU needed) ---- */
+/* Legacy ASCII VTK in the reference's dialect: points xyz + scale*u, tets in
+ * node order (0,2,1,3), hexes in voxel order, cell types 10/11, fields u_x,
+ * u_y, u_z, u_magnitude and, if node_fields (7*nd, component-major, as
+ * ofea_post_nodal writes) is not NULL, S_xx S_yy S_zz S_xy S_yz S_xz S_Mises.
+ * conn is ne*8, 1-based. */
+int ofea_write_vtk(const char *path, int nd, const double *xyz, int ne, const int *conn, const int *etype,
+                   const double *u_full, double scale, const double *node_fields);
 
 /* ---- context ---- */
 int ofea_create(ofea_ctx **ctx, int device);
@@ -189,6 +229,44 @@ int ofea_learned_refresh(ofea_ctx *ctx);
  * the resident K (built on first use).  Training pairs of the learned
  * two-level smoother use e - out as the target. */
 int ofea_coarse_apply(ofea_ctx *ctx, const double *r, double *out);
+
+/* ---- combined-approximation (CA) reanalysis on the resident K0 ----
+ * ofea_ca_delta: Delta K on the pattern of the resident K (nnz values, copied
+ * to dK_val when not NULL) from nmod modified elements: kind 0 (deleted) adds
+ * -K_e, kind 2 (added) +K_e, kind 1 (moved) K_e(new) - K_e(old); conn_mod is
+ * nmod*8, 0-based node ids of the context's mesh; the element stiffness uses
+ * xyz_m (nd*3, the modified coordinates; NULL = the mesh's), the old side of a
+ * moved element the mesh's coordinates.  Summation per CSC slot is in list
+ * order.  Every modified element must couple only node pairs already in the
+ * pattern (OFEA_ERR_ARG otherwise; the reference leaves new couplings as a
+ * TODO, src/cae.cpp:324-329). */
+int ofea_ca_delta(ofea_ctx *ctx, double E, double nu, int nmod, const int *kind, const int *conn_mod,
+                  const int *etype_mod, const double *xyz_m, double *dK_val);
+/* ofea_ca_solve: u_out (n) = U y with U an orthonormal basis of
+ * span{r_0 = u0, r_i = -K0^-1 Delta K r_{i-1}, i < n_basis} and
+ * (U^T (K0 + Delta K) U) y = U^T F; the K0 solves run the device PCG `method`
+ * on the resident K0 to rtol.  Series vectors numerically inside the span of
+ * the earlier ones are dropped; *n_used (nullable) is the basis size kept,
+ * *pcg_iters (nullable) the PCG iterations over all K0 solves, reduced
+ * (nullable, >= n_basis^2 + 2 n_basis doubles) receives K_r (row-major),
+ * F_r and y.  n_basis in [1, 32].  Overwrites the context's right-hand side.
+ * Returns OFEA_NOT_CONVERGED if a K0 solve missed rtol. */
+int ofea_ca_solve(ofea_ctx *ctx, int n_basis, const double *u0, const double *F, int method, double rtol, int maxit,
+                  double *u_out, int *n_used, int *pcg_iters, double *reduced);
+
+/* ---- nodal strain / stress recovery ----
+ * From the full displacement u_full (3*nd, as ofea_reset_displacement gives):
+ * per element 7 values [xx, yy, zz, xy, yz, xz, Mises] (elem_out 7*ne,
+ * component-major, nullable), and their averages over each node's element
+ * incidences (node_out 7*nd, component-major, nullable; 0 for a node in no
+ * element).  quantity OFEA_POST_STRAIN gives what the reference writes
+ * (engineering strain B u_e with its formula sqrt(0.5((e0-e1)^2 + (e1-e2)^2 +
+ * (e2-e0)^2 + 6(e3^2+e4^2+e5^2)))); OFEA_POST_STRESS the Cauchy stress
+ * D B u_e (E, or the moduli of ofea_set_elem_E) and its von Mises value.
+ * C3D8 (no routine in the reference) uses the detJ-weighted mean of its 8
+ * Gauss-point values. */
+int ofea_post_nodal(ofea_ctx *ctx, double E, double nu, const double *u_full, int quantity, double *elem_out,
+                    double *node_out);
 
 /* ---- batched reanalysis (no counterpart in the reference) ----
  * Many design variants of one mesh: variant v has per-element moduli

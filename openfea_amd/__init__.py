@@ -5,8 +5,10 @@ kernels, include/openfea_amd.h); this package is its host-side mirror of the
 reference's call surface (fea.py) plus synthetic workload generators (meshgen.py).
 """
 from . import _capi, meshgen  # noqa: F401
-from .fea import (AssambleStiffness, Context, Deck, build_csr, build_free_index, build_single_load,  # noqa: F401
-                  read_inp, reset_displacement, solution_api)
+from .fea import (AssambleStiffness, CA_ReAnalysis, CAInfo, Context, Deck, build_csr, build_free_index,  # noqa: F401
+                  build_single_load, export_dis_2_vtk, get_cauchy_stress_3d, read_ca_info, read_inp,
+                  reset_ca_displacement, reset_displacement, solution_api)
 
-__all__ = ["AssambleStiffness", "Context", "Deck", "build_csr", "build_free_index", "build_single_load", "read_inp",
-           "reset_displacement", "solution_api", "meshgen"]
+__all__ = ["AssambleStiffness", "CA_ReAnalysis", "CAInfo", "Context", "Deck", "build_csr", "build_free_index",
+           "build_single_load", "export_dis_2_vtk", "get_cauchy_stress_3d", "read_ca_info", "read_inp",
+           "reset_ca_displacement", "reset_displacement", "solution_api", "meshgen"]

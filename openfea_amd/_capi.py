@@ -39,6 +39,9 @@ PCG_LEARNED_TWO_LEVEL_MIXED = 9
 FP32 = 0
 FP64 = 1
 
+POST_STRAIN = 0
+POST_STRESS = 1
+
 _d = C.c_double
 _i = C.c_int
 _f = C.c_float
@@ -93,6 +96,15 @@ SIGNATURES = {
     "ofea_timer_stop": (_i, [_vp, _pf]),
     "ofea_sync": (_i, [_vp]),
     "ofea_stream_copy": (_i, [_vp, _ll, _i, _pf]),
+    "ofea_reset_ca_displacement": (_i, [_i, _vp, _vp, _i, _vp, _i, _vp]),
+    "ofea_ca_info_open": (_i, [C.c_char_p, C.POINTER(_vp)]),
+    "ofea_ca_info_sizes": (_i, [_vp, _pi, _pi, _pi]),
+    "ofea_ca_info_get": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "ofea_ca_info_close": (None, [_vp]),
+    "ofea_write_vtk": (_i, [C.c_char_p, _i, _vp, _i, _vp, _vp, _vp, _d, _vp]),
+    "ofea_ca_delta": (_i, [_vp, _d, _d, _i, _vp, _vp, _vp, _vp, _vp]),
+    "ofea_ca_solve": (_i, [_vp, _i, _vp, _vp, _i, _d, _i, _vp, _pi, _pi, _vp]),
+    "ofea_post_nodal": (_i, [_vp, _d, _d, _vp, _i, _vp, _vp]),
 }
 
 _LIB = None

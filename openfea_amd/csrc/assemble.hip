@@ -102,14 +102,17 @@ struct TetTask {
 
 // global part of a task (off-diagonal candidates): chunk-local node and
 // contributor range
+template <bool SPARSE>
 __device__ inline void tet_task_load(int task, int nblk, int b0, int t0, const uint8_t *__restrict__ blk_rl,
-                                     const int *__restrict__ off_ptr, TetTask &t)
+                                     const int *__restrict__ off_ptr, const int *__restrict__ blk_gid, TetTask &t)
 {
     t.kind = 0;
     if (task < nblk) {
         const int bk = b0 + task;
         t.rl = blk_rl[bk];
-        t.k = bk;  // global block index until resolved
+        // global block index until resolved: a full plan lists every block in
+        // order, a sparse plan (a few modified elements) names each one
+        t.k = SPARSE ? blk_gid[bk] : bk;
         t.c0 = off_ptr[bk] - t0;
         t.c1 = off_ptr[bk + 1] - t0;
         t.kind = 1;
@@ -139,6 +142,7 @@ __device__ inline void tet_task_resolve(TetTask &t, const int *s_nb, const int *
     }
 }
 
+template <bool SPARSE>
 __global__ __launch_bounds__(ASM_WG) void k_assemble_tet(const double *__restrict__ xyz, const int *__restrict__ conn,
                                                          const double *__restrict__ Eel, double E, double nu,
                                                          const int *__restrict__ chunk_pair,
@@ -148,6 +152,7 @@ __global__ __launch_bounds__(ASM_WG) void k_assemble_tet(const double *__restric
                                                          const uint8_t *__restrict__ blk_rl,
                                                          const int *__restrict__ off_ptr,
                                                          const uint16_t *__restrict__ off_contrib,
+                                                         const int *__restrict__ blk_gid,
                                                          const int *__restrict__ node_blk,
                                                          const int *__restrict__ diag_k, double *__restrict__ val,
                                                          int accumulate, int nchunks)
@@ -192,7 +197,7 @@ __global__ __launch_bounds__(ASM_WG) void k_assemble_tet(const double *__restric
     const int ntask2 = diag_wave ? 6 * nnode : nblk;
     const int task_base = diag_wave ? nblk : 0;
     TetTask first{0, 0, 0, 0, 0, 0, 0, 0};
-    if (lane2 < ntask2) tet_task_load(task_base + lane2, nblk, b0, t0, blk_rl, off_ptr, first);
+    if (lane2 < ntask2) tet_task_load<SPARSE>(task_base + lane2, nblk, b0, t0, blk_rl, off_ptr, blk_gid, first);
     double lam0, mu0;
     lame<double>(E, nu, lam0, mu0);
     const double rho = 2.0 * nu / (1.0 - 2.0 * nu);  // lambda / mu
@@ -232,7 +237,7 @@ __global__ __launch_bounds__(ASM_WG) void k_assemble_tet(const double *__restric
     __syncthreads();
     for (int t2 = lane2; t2 < ntask2; t2 += step2) {
         TetTask tk = first;
-        if (t2 != lane2) tet_task_load(task_base + t2, nblk, b0, t0, blk_rl, off_ptr, tk);
+        if (t2 != lane2) tet_task_load<SPARSE>(task_base + t2, nblk, b0, t0, blk_rl, off_ptr, blk_gid, tk);
         tet_task_resolve(tk, s_nb, s_dk, s_np);
         if (tk.kind == 1) {  // off-diagonal block (r, k): K_ab = rho Q + Q^T + I tr(Q)
             double Q[9];
@@ -380,6 +385,28 @@ __global__ __launch_bounds__(ASM_WG) void k_assemble_hex(const double *__restric
     gather_blocks(b0, b1, t0, blk_rk, contrib_ptr, s_ct, col_ptr, val, accumulate, src);
 }
 
+int launch_assemble_plan(ofea_ctx *ctx, int type, const DevPlan &dp, const double *d_xyz, double E, double nu,
+                         double *d_out, int accumulate)
+{
+    if (dp.nchunks == 0) return OFEA_OK;
+    if (type == OFEA_C3D4 && dp.blk_gid)
+        hipLaunchKernelGGL(k_assemble_tet<true>, dim3(dp.nchunks), dim3(ASM_WG), 0, ctx->stream, d_xyz, dp.conn, dp.E,
+                           E, nu, dp.chunk_pair, dp.chunk_blk, dp.chunk_node, dp.node_pair, dp.pair, dp.blk_rl,
+                           dp.off_ptr, dp.off_contrib, dp.blk_gid, ctx->d_node_blk, ctx->d_diag_k, d_out, accumulate,
+                           dp.nchunks);
+    else if (type == OFEA_C3D4)
+        hipLaunchKernelGGL(k_assemble_tet<false>, dim3(dp.nchunks), dim3(ASM_WG), 0, ctx->stream, d_xyz, dp.conn, dp.E,
+                           E, nu, dp.chunk_pair, dp.chunk_blk, dp.chunk_node, dp.node_pair, dp.pair, dp.blk_rl,
+                           dp.off_ptr, dp.off_contrib, nullptr, ctx->d_node_blk, ctx->d_diag_k, d_out, accumulate,
+                           dp.nchunks);
+    else
+        hipLaunchKernelGGL(k_assemble_hex, dim3(dp.nchunks), dim3(ASM_WG), 0, ctx->stream, d_xyz, dp.conn, dp.E, E, nu,
+                           dp.chunk_pair, dp.chunk_blk, dp.pair, dp.blk_rk, dp.contrib_ptr, dp.contrib,
+                           ctx->d_col_ptr, d_out, accumulate, dp.nchunks);
+    OFEA_HIP(hipGetLastError());
+    return OFEA_OK;
+}
+
 int launch_assemble(ofea_ctx *ctx, double E, double nu, double *d_out, bool zero_first)
 {
     bool mixed = ctx->has_type[0] && ctx->has_type[1];
@@ -387,17 +414,8 @@ int launch_assemble(ofea_ctx *ctx, double E, double nu, double *d_out, bool zero
     int acc = (zero_first || mixed) ? 1 : 0;
     for (int t = 0; t < 2; ++t) {
         if (!ctx->has_type[t]) continue;
-        const DevPlan &dp = ctx->dp[t];
-        if (dp.nchunks == 0) continue;
-        if (t == OFEA_C3D4)
-            hipLaunchKernelGGL(k_assemble_tet, dim3(dp.nchunks), dim3(ASM_WG), 0, ctx->stream, ctx->d_xyz, dp.conn, dp.E,
-                               E, nu, dp.chunk_pair, dp.chunk_blk, dp.chunk_node, dp.node_pair, dp.pair, dp.blk_rl,
-                               dp.off_ptr, dp.off_contrib, ctx->d_node_blk, ctx->d_diag_k, d_out, acc, dp.nchunks);
-        else
-            hipLaunchKernelGGL(k_assemble_hex, dim3(dp.nchunks), dim3(ASM_WG), 0, ctx->stream, ctx->d_xyz, dp.conn, dp.E,
-                               E, nu, dp.chunk_pair, dp.chunk_blk, dp.pair, dp.blk_rk, dp.contrib_ptr, dp.contrib,
-                               ctx->d_col_ptr, d_out, acc, dp.nchunks);
-        OFEA_HIP(hipGetLastError());
+        int rc = launch_assemble_plan(ctx, t, ctx->dp[t], ctx->d_xyz, E, nu, d_out, acc);
+        if (rc) return rc;
     }
     return OFEA_OK;
 }

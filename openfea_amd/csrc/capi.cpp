@@ -10,7 +10,8 @@ namespace ofea {
 int plan_caps(int type);
 }
 
-static void free_plan(DevPlan &d)
+namespace ofea {
+void free_plan(DevPlan &d)
 {
     dev_free(d.conn);
     dev_free(d.E);
@@ -25,13 +26,40 @@ static void free_plan(DevPlan &d)
     dev_free(d.contrib);
     dev_free(d.node_pair);
     dev_free(d.chunk_node);
+    dev_free(d.blk_gid);
     d = DevPlan();
 }
+
+int upload_plan(const GatherPlan &G, DevPlan &d)
+{
+    int rc;
+    d.nn = G.nn;
+    d.ne = G.ne;
+    d.nchunks = G.nchunks();
+    d.nblk = (int)G.blk_rk.size();
+    if ((rc = dev_upload(d.conn, G.conn.data(), std::max<size_t>(G.conn.size(), 1)))) return rc;
+    if ((rc = dev_upload(d.chunk_pair, G.chunk_pair.data(), G.chunk_pair.size()))) return rc;
+    if ((rc = dev_upload(d.chunk_blk, G.chunk_blk.data(), G.chunk_blk.size()))) return rc;
+    if ((rc = dev_upload(d.pair, G.pair.data(), std::max<size_t>(G.pair.size(), 1)))) return rc;
+    if ((rc = dev_upload(d.blk_rk, G.blk_rk.data(), std::max<size_t>(G.blk_rk.size(), 1)))) return rc;
+    if ((rc = dev_upload(d.contrib_ptr, G.contrib_ptr.data(), G.contrib_ptr.size()))) return rc;
+    if ((rc = dev_upload(d.contrib, G.contrib.data(), std::max<size_t>(G.contrib.size(), 1)))) return rc;
+    if ((rc = dev_upload(d.node_pair, G.node_pair.data(), G.node_pair.size()))) return rc;
+    if ((rc = dev_upload(d.chunk_node, G.chunk_node.data(), G.chunk_node.size()))) return rc;
+    if ((rc = dev_upload(d.off_ptr, G.off_ptr.data(), G.off_ptr.size()))) return rc;
+    if ((rc = dev_upload(d.off_contrib, G.off_contrib.data(), std::max<size_t>(G.off_contrib.size(), 1)))) return rc;
+    if ((rc = dev_upload(d.blk_rl, G.blk_rl.data(), std::max<size_t>(G.blk_rl.size(), 1)))) return rc;
+    if (!G.blk_gid.empty() && (rc = dev_upload(d.blk_gid, G.blk_gid.data(), G.blk_gid.size()))) return rc;
+    return OFEA_OK;
+}
+}  // namespace ofea
 
 static void free_mesh(ofea_ctx *c)
 {
     learned_free(c);
     batch_free(c);
+    ca_free(c);
+    post_free(c);
     solver_free(c);
     explicit_free(c);
     dev_free(c->d_xyz);
@@ -129,24 +157,7 @@ int ofea_set_mesh(ofea_ctx *c, int nd, const double *xyz, int ne, const int *con
     if ((rc = dev_upload<double>(c->d_x, nullptr, (size_t)std::max(c->P.n, 1)))) return rc;
     for (int t = 0; t < 2; ++t) {
         if (!c->has_type[t]) continue;
-        const GatherPlan &G = c->G[t];
-        DevPlan &d = c->dp[t];
-        d.nn = G.nn;
-        d.ne = G.ne;
-        d.nchunks = G.nchunks();
-        d.nblk = (int)G.blk_rk.size();
-        if ((rc = dev_upload(d.conn, G.conn.data(), G.conn.size()))) return rc;
-        if ((rc = dev_upload(d.chunk_pair, G.chunk_pair.data(), G.chunk_pair.size()))) return rc;
-        if ((rc = dev_upload(d.chunk_blk, G.chunk_blk.data(), G.chunk_blk.size()))) return rc;
-        if ((rc = dev_upload(d.pair, G.pair.data(), std::max<size_t>(G.pair.size(), 1)))) return rc;
-        if ((rc = dev_upload(d.blk_rk, G.blk_rk.data(), std::max<size_t>(G.blk_rk.size(), 1)))) return rc;
-        if ((rc = dev_upload(d.contrib_ptr, G.contrib_ptr.data(), G.contrib_ptr.size()))) return rc;
-        if ((rc = dev_upload(d.contrib, G.contrib.data(), std::max<size_t>(G.contrib.size(), 1)))) return rc;
-        if ((rc = dev_upload(d.node_pair, G.node_pair.data(), G.node_pair.size()))) return rc;
-        if ((rc = dev_upload(d.chunk_node, G.chunk_node.data(), G.chunk_node.size()))) return rc;
-        if ((rc = dev_upload(d.off_ptr, G.off_ptr.data(), G.off_ptr.size()))) return rc;
-        if ((rc = dev_upload(d.off_contrib, G.off_contrib.data(), std::max<size_t>(G.off_contrib.size(), 1)))) return rc;
-        if ((rc = dev_upload(d.blk_rl, G.blk_rl.data(), std::max<size_t>(G.blk_rl.size(), 1)))) return rc;
+        if ((rc = upload_plan(c->G[t], c->dp[t]))) return rc;
     }
     return OFEA_OK;
 }
@@ -294,6 +305,28 @@ int ofea_nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr,
         return fail(OFEA_ERR_ARG, "bad Newton arguments");
     return nl_solve(c, E, nu, F, n_incr, tol, max_newton, method, inner_rtol, u_out, newton_iters, linear_iters,
                     relres);
+}
+
+int ofea_ca_delta(ofea_ctx *c, double E, double nu, int nmod, const int *kind, const int *conn_mod,
+                  const int *etype_mod, const double *xyz_m, double *dK_val)
+{
+    CHECK_CTX(c);
+    return ca_delta(c, E, nu, nmod, kind, conn_mod, etype_mod, xyz_m, dK_val);
+}
+
+int ofea_ca_solve(ofea_ctx *c, int n_basis, const double *u0, const double *F, int method, double rtol, int maxit,
+                  double *u_out, int *n_used, int *pcg_iters, double *reduced)
+{
+    CHECK_CTX(c);
+    if (!(rtol > 0.0) || maxit <= 0) return fail(OFEA_ERR_ARG, "rtol must be > 0 and maxit > 0");
+    return ca_solve(c, n_basis, u0, F, method, rtol, maxit, u_out, n_used, pcg_iters, reduced);
+}
+
+int ofea_post_nodal(ofea_ctx *c, double E, double nu, const double *u_full, int quantity, double *elem_out,
+                    double *node_out)
+{
+    CHECK_CTX(c);
+    return post_nodal(c, E, nu, u_full, quantity, elem_out, node_out);
 }
 
 int ofea_learned_nparam(void) { return learned_nparam(); }

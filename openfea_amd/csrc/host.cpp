@@ -333,7 +333,7 @@ int build_pattern(int nd, int ne, const int *conn, const int *etype, const int *
 }
 
 int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const int *resort, const Pattern &P,
-                      int type, int pair_cap, GatherPlan &G)
+                      int type, int pair_cap, GatherPlan &G, bool sparse)
 {
     G = GatherPlan();
     G.nn = type == OFEA_C3D8 ? 8 : 4;
@@ -401,14 +401,17 @@ int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const i
                 int s = resort[G.conn[(size_t)el * nn + b]];
                 if (s < 0) continue;
                 int k = (int)(std::lower_bound(nbr, nbr + deg, s) - nbr);
+                if (k == deg || nbr[k] != s) return fail(OFEA_ERR_ARG, "an element couples nodes outside the pattern");
                 // C3D4 kernels keep the pair's gradients rotated so that slot 0 is the owned node
                 int slot = nn == 4 ? ((b - a) & 3) : b;
                 lists[k].push_back((uint16_t)((t - chunk_first_pair) * nn + slot));
             }
         }
         for (int k = 0; k < deg; ++k) {
+            if (sparse && lists[k].empty()) continue;  // a block no element of this list touches
             G.blk_rk.push_back((r << 8) | k);
             G.blk_rl.push_back((uint8_t)(r - chunk_first_node));
+            if (sparse) G.blk_gid.push_back(P.node_blk[r] + k);
             G.contrib.insert(G.contrib.end(), lists[k].begin(), lists[k].end());
             G.contrib_ptr.push_back((int)G.contrib.size());
             if (nbr[k] != r) G.off_contrib.insert(G.off_contrib.end(), lists[k].begin(), lists[k].end());
@@ -419,6 +422,80 @@ int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const i
     G.chunk_blk.push_back((int)G.blk_rk.size());
     G.chunk_node.push_back(nfree);
     (void)nd;
+    return OFEA_OK;
+}
+
+// ---------------------------------------------------------------- CA change file
+// ReadInfo::CA_read_change (src/data2cae.cpp:519-665): "*nodes_all, N" then
+// "x, y, z" lines; "*eles_change, M, type=T" then "kind, n1, ..., nk" lines
+// (kind 0 delete, 1 move, 2 add; node ids 0-based into the node list); both
+// end at the next '*' line, the scan at "*End Assembly"; then "*idref2new, N"
+// and one new-model node index per line (-1 = removed) up to "*End File".
+static int read_ca_info(const std::string &path, ofea_ca_info &I)
+{
+    std::ifstream in(path);
+    if (!in) return fail(OFEA_ERR_IO, "cannot open " + path);
+    std::map<std::string, int> types = {{"C3D4", OFEA_C3D4}, {"C3D8", OFEA_C3D8}};
+    std::string line;
+    bool node_rec = false, elem_rec = false, map_rec = false;
+    int cur_type = OFEA_C3D4;
+    while (get_line(in, line)) {
+        if (map_rec) {
+            if (line.find("*End File") != std::string::npos) break;
+            I.idref2new.push_back(std::atoi(line.c_str()));
+            continue;
+        }
+        if (line.find("*idref2new") != std::string::npos) {
+            map_rec = true;
+            node_rec = elem_rec = false;
+            continue;
+        }
+        if (line.find("*End Assembly") != std::string::npos) {
+            node_rec = elem_rec = false;
+            continue;
+        }
+        if (line.find("*nodes_all") != std::string::npos) {
+            node_rec = true;
+            elem_rec = false;
+            continue;
+        }
+        if (line.find("*eles_change") != std::string::npos) {
+            size_t eq = line.find('=');
+            cur_type = eq == std::string::npos ? OFEA_C3D4 : types[strip_blanks(line.substr(eq + 1))];
+            elem_rec = true;
+            node_rec = false;
+            continue;
+        }
+        if (line.find('*') != std::string::npos) {
+            node_rec = elem_rec = false;
+            continue;
+        }
+        if (node_rec) {
+            std::istringstream iss(line);
+            std::string x, y, z;
+            iss >> x >> y >> z;
+            drop_last(x);
+            drop_last(y);
+            try {
+                I.xyz.push_back(std::stod(x));
+                I.xyz.push_back(std::stod(y));
+                I.xyz.push_back(std::stod(z));
+            } catch (...) {
+                return fail(OFEA_ERR_IO, "bad node line in " + path + ": " + line);
+            }
+        } else if (elem_rec) {
+            std::istringstream iss(line);
+            std::string tok;
+            std::vector<int> v;
+            while (std::getline(iss, tok, ',')) v.push_back(std::atoi(strip_blanks(tok).c_str()));
+            const int nn = cur_type == OFEA_C3D8 ? 8 : 4;
+            if ((int)v.size() < nn + 1) return fail(OFEA_ERR_IO, "short element line in " + path + ": " + line);
+            I.kind.push_back(v[0]);
+            I.etype.push_back(cur_type);
+            for (int a = 0; a < 8; ++a) I.conn.push_back(a < nn ? v[1 + a] : 0);
+        }
+    }
+    if (I.xyz.empty()) return fail(OFEA_ERR_IO, "no *nodes_all section in " + path);
     return OFEA_OK;
 }
 
@@ -533,6 +610,130 @@ int ofea_reset_displacement(int nd, const int *resort, const double *u_free, dou
         int r = resort[i];
         for (int c = 0; c < 3; ++c) u_full[3 * i + c] = r >= 0 ? u_free[3 * r + c] : 0.0;
     }
+    return OFEA_OK;
+}
+
+int ofea_ca_info_open(const char *path, ofea_ca_info **info)
+{
+    if (!path || !info) return fail(OFEA_ERR_ARG, "null argument");
+    *info = nullptr;
+    auto *I = new ofea_ca_info();
+    int rc = read_ca_info(path, *I);
+    if (rc != OFEA_OK) {
+        delete I;
+        return rc;
+    }
+    *info = I;
+    return OFEA_OK;
+}
+
+int ofea_ca_info_sizes(const ofea_ca_info *I, int *nd_all, int *nmod, int *nmap)
+{
+    if (!I) return fail(OFEA_ERR_ARG, "null info");
+    if (nd_all) *nd_all = (int)(I->xyz.size() / 3);
+    if (nmod) *nmod = (int)I->kind.size();
+    if (nmap) *nmap = (int)I->idref2new.size();
+    return OFEA_OK;
+}
+
+int ofea_ca_info_get(const ofea_ca_info *I, double *xyz_all, int *kind, int *conn_mod, int *etype_mod,
+                     int *idref2new)
+{
+    if (!I) return fail(OFEA_ERR_ARG, "null info");
+    if (xyz_all) std::memcpy(xyz_all, I->xyz.data(), I->xyz.size() * sizeof(double));
+    if (kind && !I->kind.empty()) std::memcpy(kind, I->kind.data(), I->kind.size() * sizeof(int));
+    if (conn_mod && !I->conn.empty()) std::memcpy(conn_mod, I->conn.data(), I->conn.size() * sizeof(int));
+    if (etype_mod && !I->etype.empty()) std::memcpy(etype_mod, I->etype.data(), I->etype.size() * sizeof(int));
+    if (idref2new && !I->idref2new.empty())
+        std::memcpy(idref2new, I->idref2new.data(), I->idref2new.size() * sizeof(int));
+    return OFEA_OK;
+}
+
+void ofea_ca_info_close(ofea_ca_info *info) { delete info; }
+
+int ofea_reset_ca_displacement(int nd_all, const int *resort, const double *u_free, int nmap, const int *idref2new,
+                               int nd_new, double *u_new)
+{
+    if (nd_all < 0 || nmap < 0 || nd_new < 0 || !resort || !u_free || !u_new || (nmap > 0 && !idref2new))
+        return fail(OFEA_ERR_ARG, "bad argument");
+    if (nmap > nd_all) return fail(OFEA_ERR_ARG, "more map entries than nodes");
+    std::fill(u_new, u_new + 3 * (size_t)nd_new, 0.0);
+    for (int i = 0; i < nmap; ++i) {
+        const int j = idref2new[i];
+        if (j == -1) continue;
+        if (j < 0 || j >= nd_new) return fail(OFEA_ERR_ARG, "idref2new entry out of range");
+        const int r = resort[i];
+        for (int c = 0; c < 3; ++c) u_new[3 * (size_t)j + c] = r >= 0 ? u_free[3 * (size_t)r + c] : 0.0;
+    }
+    return OFEA_OK;
+}
+
+// data_process::export_dis_2_vtk (src/data2vtk.cpp:47-264) and
+// CA_export_dis_2_vtk (:266-427): legacy ASCII unstructured grid, default
+// stream formatting (6 significant digits), tets written in node order
+// (0, 2, 1, 3), hexes in VTK_VOXEL order (0, 1, 3, 2, 4, 5, 7, 6), cell types
+// 10 and 11, u_x / u_y / u_z / u_magnitude and, when given, the seven nodal
+// fields S_xx ... S_Mises.
+int ofea_write_vtk(const char *path, int nd, const double *xyz, int ne, const int *conn, const int *etype,
+                   const double *u_full, double scale, const double *node_fields)
+{
+    if (!path || nd < 0 || ne < 0 || (nd > 0 && (!xyz || !u_full)) || (ne > 0 && (!conn || !etype)))
+        return fail(OFEA_ERR_ARG, "bad argument");
+    std::ofstream f(path);
+    if (!f) return fail(OFEA_ERR_IO, std::string("cannot open ") + path);
+    f << "# vtk DataFile Version 3.0\n";
+    f << "The density field of the optimized results\n";
+    f << "ASCII\n";
+    f << "DATASET UNSTRUCTURED_GRID\n\n";
+    f << "POINTS\t" << nd << "\tdouble\n";
+    for (int i = 0; i < nd; ++i)
+        f << xyz[3 * i] + scale * u_full[3 * i] << "\t\t" << xyz[3 * i + 1] + scale * u_full[3 * i + 1] << "\t\t"
+          << xyz[3 * i + 2] + scale * u_full[3 * i + 2] << "\n";
+    int n4 = 0, n8 = 0;
+    for (int e = 0; e < ne; ++e) {
+        if (etype[e] == OFEA_C3D4) ++n4;
+        else if (etype[e] == OFEA_C3D8) ++n8;
+        else return fail(OFEA_ERR_ARG, "unknown element type id");
+    }
+    static const int tet_order[4] = {0, 2, 1, 3};
+    static const int hex_order[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+    f << "CELLS\t" << ne << "\t" << n8 * 9 + n4 * 5 << "\n";
+    for (int e = 0; e < ne; ++e) {
+        const int *cn = conn + 8 * (size_t)e;
+        if (etype[e] == OFEA_C3D4) {
+            f << 4;
+            for (int a : tet_order) f << "\t" << cn[a] - 1;
+        } else {
+            f << 8;
+            for (int a : hex_order) f << "\t" << cn[a] - 1;
+        }
+        f << "\n";
+    }
+    f << "CELL_TYPES\t\t" << ne << "\n";
+    for (int e = 0; e < ne; ++e) f << (etype[e] == OFEA_C3D4 ? 10 : 11) << "\n";
+    static const char *ucomp[3] = {"u_x", "u_y", "u_z"};
+    f << "\nPOINT_DATA\t" << nd;
+    for (int c = 0; c < 3; ++c) {
+        f << "\nSCALARS " << ucomp[c] << " double 1\n"
+          << "LOOKUP_TABLE  table" << c + 1 << "\n";
+        for (int i = 0; i < nd; ++i) f << u_full[3 * i + c] << "\n";
+    }
+    f << "\nSCALARS u_magnitude double 1\n"
+      << "LOOKUP_TABLE  table4\n";
+    for (int i = 0; i < nd; ++i) {
+        const double *u = u_full + 3 * (size_t)i;
+        f << std::sqrt(u[2] * u[2] + u[1] * u[1] + u[0] * u[0]) << "\n";
+    }
+    if (node_fields) {
+        static const char *names[7] = {"S_xx", "S_yy", "S_zz", "S_xy", "S_yz", "S_xz", "S_Mises"};
+        for (int k = 0; k < 7; ++k) {
+            f << "\nSCALARS " << names[k] << " double 1\n"
+              << "LOOKUP_TABLE  table" << k + 5 << "\n";
+            for (int i = 0; i < nd; ++i) f << node_fields[(size_t)k * nd + i] << "\n";
+        }
+    }
+    f.close();
+    if (!f) return fail(OFEA_ERR_IO, std::string("write failed: ") + path);
     return OFEA_OK;
 }
 

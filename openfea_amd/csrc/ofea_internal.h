@@ -62,12 +62,15 @@ struct GatherPlan {
     std::vector<int> off_ptr;            // blocks + 1
     std::vector<uint16_t> off_contrib;   // as contrib, diagonal blocks empty
     std::vector<uint8_t> blk_rl;         // r - chunk_node[chunk] per block
+    std::vector<int> blk_gid;            // sparse plans only: node_blk[r] + k of each listed block
     int nchunks() const { return (int)chunk_pair.size() - 1; }
 };
 
 int build_pattern(int nd, int ne, const int *conn, const int *etype, const int *resort, Pattern &P);
+// sparse = true leaves out the blocks that no element of the list touches
+// (plans over a few modified elements, whose kernels accumulate into K).
 int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const int *resort, const Pattern &P,
-                      int type, int pair_cap, GatherPlan &G);
+                      int type, int pair_cap, GatherPlan &G, bool sparse = false);
 
 // Pairs per gather chunk, shared by every kernel that walks a GatherPlan:
 // C3D4 one pair per thread of a 256-thread workgroup, C3D8 one pair per
@@ -87,6 +90,7 @@ struct DevPlan {
     int *off_ptr = nullptr;
     uint16_t *off_contrib = nullptr;
     uint8_t *blk_rl = nullptr;
+    int *blk_gid = nullptr;  // null for full plans
 };
 
 }  // namespace ofea
@@ -99,6 +103,15 @@ struct ofea_deck {
     int load_dof = 0;
     double load_value = 0.0;
     double E, nu, rho, dt = 0.0, t_total = 0.0;
+};
+
+// CA change file (ReadInfo::CA_read_change, src/data2cae.cpp:519-665)
+struct ofea_ca_info {
+    std::vector<double> xyz;      // nodes_all x 3
+    std::vector<int> kind;        // per modified element: 0 delete, 1 move, 2 add
+    std::vector<int> etype;       // per modified element
+    std::vector<int> conn;        // 8 per modified element, 0-based node ids
+    std::vector<int> idref2new;   // new-model node of each listed node, -1 = removed
 };
 
 struct ofea_solver_ws;  // solver work space (solver.hip)
@@ -136,13 +149,25 @@ struct ofea_ctx {
     void *learned = nullptr;
     // batched variants / multi-RHS solves (batched.hip)
     void *batch_ws = nullptr;
+    // combined-approximation reanalysis (ca.hip)
+    void *ca_ws = nullptr;
+    // nodal strain/stress recovery (post.hip)
+    void *post_ws = nullptr;
 };
 
 namespace ofea {
 // kernels (assemble.hip)
 int launch_assemble(ofea_ctx *ctx, double E, double nu, double *d_out, bool zero_first);
+// one plan of element type `type` over coordinates d_xyz (any plan built
+// against ctx->P); accumulate = 1 adds to d_out instead of overwriting
+int launch_assemble_plan(ofea_ctx *ctx, int type, const DevPlan &dp, const double *d_xyz, double E, double nu,
+                         double *d_out, int accumulate);
+int upload_plan(const GatherPlan &G, DevPlan &d);
+void free_plan(DevPlan &d);
 // solver (solver.hip)
 int solver_spmv(ofea_ctx *ctx, const double *d_x, double *d_y);
+// y = A x for any FP64 value array on the pattern of K
+int solver_spmv_vals(ofea_ctx *ctx, const double *d_vals, const double *d_x, double *d_y);
 int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, double *relres);
 void solver_free(ofea_ctx *ctx);
 void solver_invalidate(ofea_ctx *ctx);  // K changed: preconditioner and FP32 copy are stale
@@ -174,6 +199,16 @@ int solve_batch(ofea_ctx *c, int nvar, int nrhs, const double *B, double *X, dou
 int autotune(ofea_ctx *c, int *best_group, float *times);
 int batch_tiles(ofea_ctx *c, int nrhs, int *group, float *ms);
 void batch_free(ofea_ctx *c);
+// combined-approximation reanalysis (ca.hip)
+int ca_delta(ofea_ctx *c, double E, double nu, int nmod, const int *kind, const int *conn_mod, const int *etype_mod,
+             const double *xyz_m, double *dK_val);
+int ca_solve(ofea_ctx *c, int n_basis, const double *u0, const double *F, int method, double rtol, int maxit,
+             double *u_out, int *n_used, int *pcg_iters, double *reduced);
+void ca_free(ofea_ctx *c);
+// nodal strain / stress recovery (post.hip)
+int post_nodal(ofea_ctx *c, double E, double nu, const double *u_full, int quantity, double *elem_out,
+               double *node_out);
+void post_free(ofea_ctx *c);
 // SpMV of K (FP64) with lane-group size `group`, averaged over reps launches (solver.hip)
 int solver_spmv_group_time(ofea_ctx *ctx, int group, int reps, float *ms);
 // two-level coarse correction Z A_c^-1 Z^T r on the host (solver.hip)

@@ -1,0 +1,233 @@
+// post.hip -- element strain / stress and their nodal averages on the device.
+//
+// Replaces simulation_post::get_cauchy_stress_3d (src/post_process.cpp:79-168)
+// with the element routine get_stress_node_hammer
+// (src/Elements/C3D4_hammer.cpp:185-226).  Two kernels:
+//   k_post_elem: one thread per element gathers its node coordinates and
+//     displacements and writes 7 values, component-major ([c][e]):
+//       OFEA_POST_STRAIN  the reference's output, B u_e: engineering strain
+//                         [xx, yy, zz, xy, yz, xz] (the reference leaves
+//                         D * B u commented out), and its "Mises" formula
+//                         sqrt(0.5 ((e0-e1)^2 + (e1-e2)^2 + (e2-e0)^2
+//                         + 6 (e3^2 + e4^2 + e5^2))) on those values;
+//       OFEA_POST_STRESS  Cauchy stress D B u_e (small strain) and its von
+//                         Mises value, same formula;
+//     C3D4 strain is constant; the reference has no C3D8 routine (ele_base's
+//     no-op), here C3D8 takes the volume average of the 2x2x2 Gauss-point
+//     strains, sum_g detJ_g B_g u_e / sum_g detJ_g;
+//   k_post_node: one thread per (node, component) sums the values of the
+//     node's elements in ascending element order (the reference's loop
+//     order) and divides by the number of element incidences; a node in no
+//     element keeps 0.
+#include <algorithm>
+#include <vector>
+
+#include "elem_device.h"
+#include "ofea_internal.h"
+
+namespace ofea {
+namespace {
+
+constexpr int POST_WG = 256;
+
+struct PostWs {
+    int nd = 0, ne = 0;
+    int *conn = nullptr;     // ne x 8, 0-based
+    int *etype = nullptr;    // ne
+    int *inc_ptr = nullptr;  // nd + 1
+    int *inc = nullptr;      // element of each incidence, ascending per node
+    double *E = nullptr;     // ne per-element moduli or null
+    double *u = nullptr;     // 3 nd
+    double *ev = nullptr;    // 7 x ne
+    double *nv = nullptr;    // 7 x nd
+};
+
+__device__ inline double mises_form(const double s[6])
+{
+    const double a = s[0] - s[1], b = s[1] - s[2], c = s[2] - s[0];
+    return sqrt(0.5 * (a * a + b * b + c * c + 6.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5])));
+}
+
+// strain rows of B (src/Elements/C3D4_hammer.cpp:205-219) applied to u
+template <int NN>
+__device__ inline void strain_of(const double g[3][NN], const double *U, double w, double eps[6])
+{
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+        const double ux = U[3 * a], uy = U[3 * a + 1], uz = U[3 * a + 2];
+        eps[0] += w * g[0][a] * ux;
+        eps[1] += w * g[1][a] * uy;
+        eps[2] += w * g[2][a] * uz;
+        eps[3] += w * (g[1][a] * ux + g[0][a] * uy);
+        eps[4] += w * (g[2][a] * uy + g[1][a] * uz);
+        eps[5] += w * (g[2][a] * ux + g[0][a] * uz);
+    }
+}
+
+__global__ __launch_bounds__(POST_WG) void k_post_elem(int ne, const int *__restrict__ conn,
+                                                       const int *__restrict__ etype, const double *__restrict__ xyz,
+                                                       const double *__restrict__ u, const double *__restrict__ Eel,
+                                                       double E, double nu, int quantity, double *__restrict__ ev)
+{
+    const int e = blockIdx.x * POST_WG + threadIdx.x;
+    if (e >= ne) return;
+    const int *cn = conn + 8 * (size_t)e;
+    double eps[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (etype[e] == OFEA_C3D4) {
+        double X[12], U[12];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int nd = cn[a];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                X[3 * a + c] = xyz[3 * nd + c];
+                U[3 * a + c] = u[3 * nd + c];
+            }
+        }
+        double g[3][4];
+        tet_grads(X, g);
+        strain_of<4>(g, U, 1.0, eps);
+    } else {
+        double X[24], U[24];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const int nd = cn[a];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                X[3 * a + c] = xyz[3 * nd + c];
+                U[3 * a + c] = u[3 * nd + c];
+            }
+        }
+        double vol = 0.0;
+        for (int gp = 0; gp < 8; ++gp) {
+            double g[3][8];
+            const double detJ = hex_point(X, gp, g);
+            strain_of<8>(g, U, detJ, eps);
+            vol += detJ;
+        }
+        const double iv = 1.0 / vol;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) eps[i] *= iv;
+    }
+    double out[6];
+    if (quantity == OFEA_POST_STRESS) {
+        double lam, mu;
+        lame<double>(Eel ? Eel[e] : E, nu, lam, mu);
+        const double tr = eps[0] + eps[1] + eps[2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) out[i] = lam * tr + 2.0 * mu * eps[i];
+#pragma unroll
+        for (int i = 3; i < 6; ++i) out[i] = mu * eps[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) out[i] = eps[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) ev[(size_t)i * ne + e] = out[i];
+    ev[(size_t)6 * ne + e] = mises_form(out);
+}
+
+__global__ __launch_bounds__(POST_WG) void k_post_node(int nd, int ne, const int *__restrict__ inc_ptr,
+                                                       const int *__restrict__ inc, const double *__restrict__ ev,
+                                                       double *__restrict__ nv)
+{
+    const int t = blockIdx.x * POST_WG + threadIdx.x;
+    if (t >= 7 * nd) return;
+    const int i = t / 7, c = t - 7 * i;
+    const int k0 = inc_ptr[i], k1 = inc_ptr[i + 1];
+    const double *col = ev + (size_t)c * ne;
+    double acc = 0.0;
+    for (int k = k0; k < k1; ++k) acc += col[inc[k]];
+    nv[(size_t)c * nd + i] = k1 > k0 ? acc / (double)(k1 - k0) : 0.0;
+}
+
+void free_ws(PostWs *w)
+{
+    if (!w) return;
+    dev_free(w->conn);
+    dev_free(w->etype);
+    dev_free(w->inc_ptr);
+    dev_free(w->inc);
+    dev_free(w->E);
+    dev_free(w->u);
+    dev_free(w->ev);
+    dev_free(w->nv);
+    delete w;
+}
+
+int ensure(ofea_ctx *c, PostWs *&w)
+{
+    w = (PostWs *)c->post_ws;
+    if (w) return OFEA_OK;
+    w = new PostWs();
+    c->post_ws = w;
+    w->nd = c->nd;
+    w->ne = c->ne;
+    const int nd = c->nd, ne = c->ne;
+    std::vector<int> conn0(8 * (size_t)ne, 0), ptr(nd + 1, 0);
+    for (int e = 0; e < ne; ++e) {
+        const int nn = c->etype[e] == OFEA_C3D8 ? 8 : 4;
+        for (int a = 0; a < nn; ++a) {
+            const int node = c->conn[8 * (size_t)e + a] - 1;
+            if (node < 0 || node >= nd) return fail(OFEA_ERR_ARG, "element node out of range");
+            conn0[8 * (size_t)e + a] = node;
+            ptr[node + 1]++;
+        }
+    }
+    for (int i = 0; i < nd; ++i) ptr[i + 1] += ptr[i];
+    std::vector<int> inc(ptr[nd]), fill(ptr.begin(), ptr.end() - 1);
+    for (int e = 0; e < ne; ++e) {  // ascending e: the reference's accumulation order per node
+        const int nn = c->etype[e] == OFEA_C3D8 ? 8 : 4;
+        for (int a = 0; a < nn; ++a) inc[fill[conn0[8 * (size_t)e + a]]++] = e;
+    }
+    int rc;
+    if ((rc = dev_upload(w->conn, conn0.data(), conn0.size()))) return rc;
+    if ((rc = dev_upload(w->etype, c->etype.data(), c->etype.size()))) return rc;
+    if ((rc = dev_upload(w->inc_ptr, ptr.data(), ptr.size()))) return rc;
+    if ((rc = dev_upload(w->inc, inc.data(), std::max<size_t>(inc.size(), 1)))) return rc;
+    if ((rc = dev_upload<double>(w->u, nullptr, 3 * (size_t)nd))) return rc;
+    if ((rc = dev_upload<double>(w->ev, nullptr, 7 * (size_t)ne))) return rc;
+    if ((rc = dev_upload<double>(w->nv, nullptr, 7 * (size_t)nd))) return rc;
+    return OFEA_OK;
+}
+
+}  // namespace
+
+void post_free(ofea_ctx *c)
+{
+    free_ws((PostWs *)c->post_ws);
+    c->post_ws = nullptr;
+}
+
+int post_nodal(ofea_ctx *c, double E, double nu, const double *u_full, int quantity, double *elem_out,
+               double *node_out)
+{
+    if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    if (!u_full) return fail(OFEA_ERR_ARG, "null displacement");
+    if (quantity != OFEA_POST_STRAIN && quantity != OFEA_POST_STRESS)
+        return fail(OFEA_ERR_ARG, "quantity must be OFEA_POST_STRAIN or OFEA_POST_STRESS");
+    if (quantity == OFEA_POST_STRESS && !(nu > -1.0 && nu < 0.5))
+        return fail(OFEA_ERR_ARG, "Poisson ratio must lie in (-1, 0.5)");
+    PostWs *w = nullptr;
+    int rc = ensure(c, w);
+    if (rc) return rc;
+    // per-element moduli follow ofea_set_elem_E
+    dev_free(w->E);
+    if (!c->E_elem.empty() && (rc = dev_upload(w->E, c->E_elem.data(), c->E_elem.size()))) return rc;
+    const int nd = c->nd, ne = c->ne;
+    OFEA_HIP(hipMemcpyAsync(w->u, u_full, sizeof(double) * 3 * nd, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_post_elem, dim3((ne + POST_WG - 1) / POST_WG), dim3(POST_WG), 0, c->stream, ne, w->conn,
+                       w->etype, c->d_xyz, w->u, w->E, E, nu, quantity, w->ev);
+    OFEA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_post_node, dim3((7 * nd + POST_WG - 1) / POST_WG), dim3(POST_WG), 0, c->stream, nd, ne,
+                       w->inc_ptr, w->inc, w->ev, w->nv);
+    OFEA_HIP(hipGetLastError());
+    if (elem_out)
+        OFEA_HIP(hipMemcpyAsync(elem_out, w->ev, sizeof(double) * 7 * ne, hipMemcpyDeviceToHost, c->stream));
+    if (node_out)
+        OFEA_HIP(hipMemcpyAsync(node_out, w->nv, sizeof(double) * 7 * nd, hipMemcpyDeviceToHost, c->stream));
+    OFEA_HIP(hipStreamSynchronize(c->stream));
+    return OFEA_OK;
+}
+
+}  // namespace ofea

@@ -937,6 +937,15 @@ int solver_spmv(ofea_ctx *ctx, const double *d_x, double *d_y)
     return OFEA_OK;
 }
 
+int solver_spmv_vals(ofea_ctx *ctx, const double *d_vals, const double *d_x, double *d_y)
+{
+    int rc = ensure_ws(ctx);
+    if (rc) return rc;
+    launch_spmv_t<double, double, false>(ctx, ctx->ws->spmv_group, d_vals, d_x, d_y);
+    OFEA_HIP(hipGetLastError());
+    return OFEA_OK;
+}
+
 int solver_spmv_group_time(ofea_ctx *ctx, int group, int reps, float *ms)
 {
     int rc = ensure_ws(ctx);

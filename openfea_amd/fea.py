@@ -7,6 +7,11 @@ Names follow the reference so that a driver written against it reads the same:
   assamble_stiffness               -> AssambleStiffness  (src/include/assemble.h:35-72)
   solution_api(..., solver_type)   -> solution_api()     (src/linear_solution.cpp:17-64)
   simulation_post::reset_displacement -> reset_displacement() (src/post_process.cpp:18-40)
+  simulation_post::get_cauchy_stress_3d -> get_cauchy_stress_3d() (src/post_process.cpp:79-168)
+  data_process::export_dis_2_vtk   -> export_dis_2_vtk() (src/data2vtk.cpp:47-264)
+  ReadInfo::CA_read_change          -> read_ca_info()     (src/data2cae.cpp:519-665)
+  CAE_process::CA_ReAnalysis        -> CA_ReAnalysis()    (src/cae.cpp:317-376)
+  simulation_post::reset_ca_displacement -> reset_ca_displacement() (src/post_process.cpp:42-76)
 All numerical work runs in the native library; this module only marshals arrays.
 """
 from __future__ import annotations
@@ -106,6 +111,52 @@ def reset_displacement(resort, u_free) -> np.ndarray:
     out = np.zeros(3 * len(resort))
     check(lib().ofea_reset_displacement(len(resort), ptr(resort), ptr(f64(u_free)), ptr(out)))
     return out
+
+
+def reset_ca_displacement(resort, u_free, idref2new, nd_new) -> np.ndarray:
+    """Displacement of the modified model's nodes (3*nd_new) from the free-DOF
+    solution on the original model's numbering."""
+    resort, idref = i32(resort), i32(idref2new)
+    out = np.zeros(3 * nd_new)
+    check(lib().ofea_reset_ca_displacement(len(resort), ptr(resort), ptr(f64(u_free)), len(idref), ptr(idref), nd_new,
+                                           ptr(out)))
+    return out
+
+
+@dataclass
+class CAInfo:
+    """Fields CA_read_change fills (coords_m_, node_topos_m_, ele_list_idx_m_, node_idx_m_)."""
+    xyz: np.ndarray        # nodes_all x 3
+    kind: np.ndarray       # nmod: 0 delete, 1 move, 2 add
+    conn: np.ndarray       # nmod x 8, 0-based
+    etype: np.ndarray      # nmod
+    idref2new: np.ndarray  # nodes_all: new-model node, -1 removed
+
+
+def read_ca_info(path: str) -> CAInfo:
+    L = lib()
+    h = C.c_void_p()
+    check(L.ofea_ca_info_open(path.encode(), C.byref(h)))
+    try:
+        nd, nm, nmap = C.c_int(), C.c_int(), C.c_int()
+        check(L.ofea_ca_info_sizes(h, C.byref(nd), C.byref(nm), C.byref(nmap)))
+        xyz = np.zeros((nd.value, 3))
+        kind = np.zeros(nm.value, dtype=np.int32)
+        conn = np.zeros((nm.value, 8), dtype=np.int32)
+        et = np.zeros(nm.value, dtype=np.int32)
+        idref = np.zeros(nmap.value, dtype=np.int32)
+        check(L.ofea_ca_info_get(h, ptr(xyz), ptr(kind), ptr(conn), ptr(et), ptr(idref)))
+    finally:
+        L.ofea_ca_info_close(h)
+    return CAInfo(xyz, kind, conn, et, idref)
+
+
+def export_dis_2_vtk(path: str, deck: Deck, u_full, scale: float = 1.0, node_fields=None):
+    """Legacy ASCII VTK in the reference's dialect (u fields, optional S_* fields)."""
+    nf = None if node_fields is None else f64(node_fields).reshape(7, -1)
+    xyz, conn, et = f64(deck.xyz), i32(deck.conn), i32(deck.etype)
+    check(lib().ofea_write_vtk(path.encode(), len(xyz), ptr(xyz), len(et), ptr(conn), ptr(et), ptr(f64(u_full)),
+                               scale, ptr(nf)))
 
 
 class Context:
@@ -248,6 +299,36 @@ class Context:
         check(lib().ofea_batch_tiles(self._h, nrhs, C.byref(g), C.byref(ms)))
         return g.value, ms.value
 
+    # ---- combined-approximation reanalysis
+    def ca_delta(self, E, nu, kind, conn_mod, etype_mod, xyz_m=None, copy=True):
+        """Delta K of the modified elements on the resident pattern (nnz values)."""
+        kind, etype_mod = i32(kind), i32(etype_mod)
+        conn_mod = i32(conn_mod).reshape(-1, 8)
+        xm = None if xyz_m is None else f64(xyz_m).reshape(-1, 3)
+        out = np.zeros(self.nnz) if copy else None
+        check(lib().ofea_ca_delta(self._h, E, nu, len(kind), ptr(kind), ptr(conn_mod), ptr(etype_mod), ptr(xm),
+                                  ptr(out)))
+        return out
+
+    def ca_solve(self, n_basis, u0, F, method=A.PCG_TWO_LEVEL, rtol=1e-12, maxit=200000):
+        """CA solution (n), basis size kept, PCG iterations, (K_r, F_r, y)."""
+        u = np.zeros(self.n)
+        m, it = C.c_int(), C.c_int()
+        red = np.zeros(n_basis * n_basis + 2 * n_basis)
+        rc = check(lib().ofea_ca_solve(self._h, n_basis, ptr(f64(u0)), ptr(f64(F)), method, rtol, maxit, ptr(u),
+                                       C.byref(m), C.byref(it), ptr(red)), allow=(A.OFEA_OK, A.OFEA_NOT_CONVERGED))
+        k = m.value
+        reduced = (red[:k * k].reshape(k, k), red[k * k:k * k + k], red[k * k + k:k * k + 2 * k])
+        return u, k, it.value, reduced, rc == A.OFEA_OK
+
+    # ---- post-processing
+    def post_nodal(self, E, nu, u_full, quantity=A.POST_STRAIN, elem=False):
+        """(node fields 7 x nd, element fields 7 x ne or None) from the full displacement."""
+        nv = np.zeros((7, self.nd))
+        ev = np.zeros((7, self.ne)) if elem else None
+        check(lib().ofea_post_nodal(self._h, E, nu, ptr(f64(u_full)), quantity, ptr(ev), ptr(nv)))
+        return nv, ev
+
     # ---- timing on the context stream
     def timer_start(self):
         check(lib().ofea_timer_start(self._h))
@@ -299,3 +380,33 @@ def solution_api(assam: AssambleStiffness, deck: Deck, solver_type: str = "GPU_P
     deck.single_dis_vec_ = x
     deck.solve_info = {"iters": it, "relres": rel, "converged": ok}
     return True
+
+
+def get_cauchy_stress_3d(ctx: Context, deck: Deck, u_full, E: float | None = None, nu: float | None = None,
+                         quantity=A.POST_STRAIN):
+    """Nodal averages of the element values (7 x nd: xx, yy, zz, xy, yz, xz,
+    Mises).  POST_STRAIN is what the reference computes and writes as S_*."""
+    E = deck.E if E is None else E
+    nu = deck.nu if nu is None else nu
+    nv, _ = ctx.post_nodal(E, nu, u_full, quantity)
+    deck.stress_node_mat_ = nv
+    return nv
+
+
+def CA_ReAnalysis(assam: AssambleStiffness, deck: Deck, info: CAInfo, deck_new: Deck, n_basis: int = 4,
+                  E: float | None = None, nu: float | None = None, solver_type: str = "GPU_PCG_TWO_LEVEL",
+                  rtol: float = 1e-12):
+    """CAE_process::CA_ReAnalysis (src/cae.cpp:317-376) on the resident K0:
+    Delta K of the changed elements, the n_basis-term CA basis, the reduced
+    solve, and the displacement mapped onto the modified model's nodes.
+    deck.single_dis_vec_ (the K0 solution) and deck.F are the original
+    model's; returns (u on the modified model, 3*nd_new) and stores the
+    free-DOF CA solution in deck.ca_dis_vec_."""
+    E = deck.E if E is None else E
+    nu = deck.nu if nu is None else nu
+    ctx = assam.ctx
+    ctx.ca_delta(E, nu, info.kind, info.conn, info.etype, info.xyz, copy=False)
+    u, m, it, _, ok = ctx.ca_solve(n_basis, deck.single_dis_vec_, deck.F, SOLVERS[solver_type], rtol)
+    deck.ca_dis_vec_ = u
+    deck.ca_info_ = {"basis": m, "pcg_iters": it, "converged": ok}
+    return reset_ca_displacement(deck.resort, u, info.idref2new, deck_new.nd)

@@ -708,3 +708,137 @@ OR_API void or_matfree_apply(int ne, const int *conn, const int *etype, const do
         }
     }
 }
+
+/* CA Delta K (ca_fill_CSR_sparse_mat, src/ca_analysis.cpp:55-143): zeroed
+ * values on the pattern of K0, then for each modified element in list order
+ * its signed stiffness scattered like src/assemble.cpp:128-146.  kind 0
+ * (deleted): -Ke(xyz_m); kind 2 (added): +Ke(xyz_m); kind 1 (moved):
+ * Ke(xyz_m) - Ke(xyz_o), the difference the reference's comments describe
+ * (its code fills the old coordinates at row i of the element counter,
+ * src/ca_analysis.cpp:95-101).  conn_mod holds 8 0-based node ids per
+ * element.  Returns -1 if an element couples a pair outside the pattern. */
+OR_API int or_ca_fill(int nmod, const int *kind, const int *conn_mod, const int *etype, const double *xyz_m,
+                      const double *xyz_o, const int *resort, double E, double nu, const int *col_ptr,
+                      const int *row_idx, long long nnz, double *val)
+{
+    for (long long i = 0; i < nnz; ++i) val[i] = 0.0;
+    double D[36], X[24], Ke[576], Ko[576];
+    int dofs[24], conn1[8];
+    or_dmat(E, nu, D);
+    for (int i = 0; i < nmod; ++i) {
+        int nn = elem_nn(etype[i]), nc = 3 * nn;
+        for (int a = 0; a < 8; ++a) conn1[a] = conn_mod[8 * i + a] + 1;
+        gather_X(0, conn1, nn, xyz_m, X);
+        if (nn == 8)
+            or_c3d8_ke(X, D, Ke);
+        else
+            or_c3d4_ke(X, D, Ke);
+        if (kind[i] == 0) {
+            for (int k = 0; k < nc * nc; ++k) Ke[k] = -Ke[k];
+        } else if (kind[i] == 1) {
+            gather_X(0, conn1, nn, xyz_o, X);
+            if (nn == 8)
+                or_c3d8_ke(X, D, Ko);
+            else
+                or_c3d4_ke(X, D, Ko);
+            for (int k = 0; k < nc * nc; ++k) Ke[k] -= Ko[k];
+        } else if (kind[i] != 2) {
+            return -1;
+        }
+        elem_dofs(0, conn1, nn, resort, dofs);
+        for (int mm = 0; mm < nc; ++mm) {
+            int jj = dofs[mm];
+            if (jj < 0) continue;
+            for (int q = 0; q < nc; ++q) {
+                int ii = dofs[q];
+                if (ii < 0) continue;
+                int t = col_ptr[jj];
+                while (t < col_ptr[jj + 1] && row_idx[t] < ii) t++;
+                if (t == col_ptr[jj + 1] || row_idx[t] != ii) return -1;
+                val[t] += Ke[mm * nc + q];
+            }
+        }
+    }
+    return 0;
+}
+
+/* Element strain / stress (get_stress_node_hammer, src/Elements/C3D4_hammer.cpp:185-226,
+ * called from src/post_process.cpp:83-119): ev[c * ne + e], c = xx, yy, zz,
+ * xy, yz, xz (engineering shear) and the Mises form of src/post_process.cpp:104-110.
+ * quantity 0: strain B u_e (what the reference outputs); 1: stress D B u_e.
+ * C3D8: detJ-weighted mean over the 2x2x2 Gauss points (no reference routine). */
+OR_API void or_post_elem(int ne, const int *conn, const int *etype, const double *xyz, const double *u_full,
+                         double E, double nu, const double *E_elem, int quantity, double *ev)
+{
+    double X[24], U[24], dN[24], g[24], B[6 * 24], D[36];
+    for (int e = 0; e < ne; ++e) {
+        int nn = elem_nn(etype[e]), nc = 3 * nn;
+        gather_X(e, conn, nn, xyz, X);
+        for (int a = 0; a < nn; ++a) {
+            int node = conn[8 * e + a] - 1;
+            for (int c = 0; c < 3; ++c) U[3 * a + c] = u_full[3 * node + c];
+        }
+        double eps[6] = {0, 0, 0, 0, 0, 0};
+        if (nn == 4) {
+            static const double dNt[12] = {-1, 1, 0, 0, -1, 0, 1, 0, -1, 0, 0, 1};
+            grads(4, dNt, X, g);
+            bmat(4, g, B);
+            for (int i = 0; i < 6; ++i)
+                for (int k = 0; k < nc; ++k) eps[i] += B[i * nc + k] * U[k];
+        } else {
+            double vol = 0.0;
+            for (int gp = 0; gp < 8; ++gp) {
+                double r, s, t;
+                hex_gp(gp, &r, &s, &t);
+                double dNh[3][8];
+                hex_dN(r, s, t, dNh);
+                for (int i = 0; i < 3; ++i)
+                    for (int a = 0; a < 8; ++a) dN[i * 8 + a] = dNh[i][a];
+                double detJ = grads(8, dN, X, g);
+                bmat(8, g, B);
+                for (int i = 0; i < 6; ++i) {
+                    double acc = 0.0;
+                    for (int k = 0; k < nc; ++k) acc += B[i * nc + k] * U[k];
+                    eps[i] += detJ * acc;
+                }
+                vol += detJ;
+            }
+            for (int i = 0; i < 6; ++i) eps[i] /= vol;
+        }
+        double out[6];
+        if (quantity == 1) {
+            or_dmat(E_elem ? E_elem[e] : E, nu, D);
+            for (int i = 0; i < 6; ++i) {
+                out[i] = 0.0;
+                for (int k = 0; k < 6; ++k) out[i] += D[i * 6 + k] * eps[k];
+            }
+        } else {
+            for (int i = 0; i < 6; ++i) out[i] = eps[i];
+        }
+        for (int i = 0; i < 6; ++i) ev[(long long)i * ne + e] = out[i];
+        double a = out[0] - out[1], b = out[1] - out[2], c = out[2] - out[0];
+        ev[6LL * ne + e] = sqrt(0.5 * (a * a + b * b + c * c + 6.0 * (out[3] * out[3] + out[4] * out[4] + out[5] * out[5])));
+    }
+}
+
+/* Nodal averages (src/post_process.cpp:122-166): every element adds its 7
+ * values to each of its nodes in ascending element order, then each node
+ * divides by its incidence count; nodes of no element stay 0. */
+OR_API void or_post_node(int nd, int ne, const int *conn, const int *etype, const double *ev, double *nv)
+{
+    double *cnt = (double *)calloc((size_t)nd, sizeof(double));
+    for (long long i = 0; i < 7LL * nd; ++i) nv[i] = 0.0;
+    for (int e = 0; e < ne; ++e) {
+        int nn = elem_nn(etype[e]);
+        for (int a = 0; a < nn; ++a) {
+            int node = conn[8 * e + a] - 1;
+            for (int c = 0; c < 7; ++c) nv[(long long)c * nd + node] += ev[(long long)c * ne + e];
+            cnt[node] += 1.0;
+        }
+    }
+    for (int i = 0; i < nd; ++i) {
+        if (cnt[i] == 0.0) continue;
+        for (int c = 0; c < 7; ++c) nv[(long long)c * nd + i] /= cnt[i];
+    }
+    free(cnt);
+}

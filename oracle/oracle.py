@@ -51,6 +51,10 @@ def lib():
         L.or_lumped_mass.argtypes = [_i, _i, _pi, _pi, _pd, _d, _pd]
         L.or_explicit_run.argtypes = [_i, _i, _pi, _pi, _pd, _d, _d, _pd, _pd, _vp, _pi, _d, _i, _pd]
         L.or_matfree_apply.argtypes = [_i, _pi, _pi, _pd, _pi, _d, _d, _vp, _pd, _pd, _i]
+        L.or_ca_fill.argtypes = [_i, _pi, _pi, _pi, _pd, _pd, _pi, _d, _d, _pi, _pi, _ll, _pd]
+        L.or_ca_fill.restype = _i
+        L.or_post_elem.argtypes = [_i, _pi, _pi, _pd, _pd, _d, _d, _vp, _i, _pd]
+        L.or_post_node.argtypes = [_i, _i, _pi, _pi, _pd, _pd]
         _LIB = L
     return _LIB
 
@@ -221,3 +225,55 @@ def nl_newton(conn, etype, xyz, resort, E, nu, F, col_ptr, row_idx, n_incr=1, to
         else:
             raise RuntimeError("oracle Newton did not converge")
     return u, iters
+
+
+def ca_fill(kind, conn_mod, etype_mod, xyz_m, xyz_o, resort, E, nu, col_ptr, row_idx):
+    """Delta K of the modified elements on the pattern of K0 (src/ca_analysis.cpp:55-143)."""
+    val = np.zeros(len(row_idx))
+    kind = _i32(kind)
+    rc = lib().or_ca_fill(len(kind), kind, _i32(conn_mod).reshape(-1, 8), _i32(etype_mod), _f64(xyz_m).ravel(),
+                          _f64(xyz_o).ravel(), _i32(resort), E, nu, _i32(col_ptr), _i32(row_idx), len(row_idx), val)
+    if rc != 0:
+        raise ValueError("modified element outside the pattern or bad kind")
+    return val
+
+
+def ca_solve(col_ptr, row_idx, K0, dK, u0, F, n_basis, rel_tol=1e-10):
+    """Combined approximations (ca_build_rom / ca_SVD / ca_solve,
+    src/ca_analysis.cpp:146-309): r_0 = u0, r_i = -K0^-1 (dK r_{i-1}) by a
+    direct factorisation of K0 (scipy SuperLU); thin SVD of the basis in
+    float64, keeping the left singular vectors with sigma > rel_tol *
+    sigma_max (the reference keeps all n, in float32); reduced solve
+    (U^T (K0 + dK) U) y = U^T F; returns U y and the number of vectors kept."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    n = len(col_ptr) - 1
+    A0 = sp.csc_matrix((K0, row_idx, col_ptr), shape=(n, n))
+    dA = sp.csc_matrix((dK, row_idx, col_ptr), shape=(n, n))
+    lu = spla.splu(A0)
+    B = [np.asarray(u0, dtype=np.float64)]
+    for _ in range(1, n_basis):
+        B.append(-lu.solve(dA @ B[-1]))
+    U, sv, _ = np.linalg.svd(np.array(B).T, full_matrices=False)
+    U = U[:, sv > rel_tol * sv.max()]
+    Kr = U.T @ ((A0 + dA) @ U)
+    y = np.linalg.solve(Kr, U.T @ F)
+    return U @ y, U.shape[1]
+
+
+def post_elem(conn, etype, xyz, u_full, E, nu, quantity=0, E_elem=None):
+    """7 x ne element values (strain or stress, then the Mises form)."""
+    ne = len(etype)
+    ev = np.zeros((7, ne))
+    Ee = None if E_elem is None else _f64(E_elem)
+    lib().or_post_elem(ne, _i32(conn).reshape(-1, 8), _i32(etype), _f64(xyz).ravel(), _f64(u_full).ravel(), E, nu,
+                       None if Ee is None else Ee.ctypes.data, quantity, ev)
+    return ev
+
+
+def post_node(conn, etype, nd, ev):
+    """7 x nd nodal averages of the element values (src/post_process.cpp:122-166)."""
+    nv = np.zeros((7, nd))
+    lib().or_post_node(nd, len(etype), _i32(conn).reshape(-1, 8), _i32(etype), _f64(ev), nv)
+    return nv

@@ -1,0 +1,175 @@
+"""GPU parity of the CA reanalysis and the nodal strain/stress recovery,
+through the C ABI, against the CPU oracle and the reference's sample output.
+
+Bars: Delta K within 1e-12 of max|Delta K| entrywise; CA displacements within
+1e-8 (relative 2-norm) of the oracle's CA with the same basis size (the K0
+solves run PCG to 1e-12); the sample reanalysis against the reference's
+printed output (6 significant digits) as in the oracle test; element and
+nodal strain/stress within 1e-12 of the field maximum; patch test exact to
+1e-10.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import GOLDEN, rel_max
+from oracle import oracle as O
+import openfea_amd as ofa
+from openfea_amd import _capi as A, meshgen as mg
+from test_oracle_ca_post import E0, NU0, load_sample, to_new_model
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = ofa.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def sample():
+    return load_sample()
+
+
+def test_ca_sample_pipeline(ctx, sample):
+    s = sample
+    d = ofa.read_inp(os.path.join(GOLDEN, "mesh_o.inp"))
+    d.resort, nfree = ofa.build_free_index(d.nd, d.fix_set)
+    d.F = ofa.build_single_load(3 * nfree, d.resort, d.load_set, d.load_dof, d.load_value)
+    say("CA sample: mesh_o.inp K0 + 384 deleted tets")
+    assam = ofa.AssambleStiffness(ctx)
+    assam.build_CSR(d)
+    assam.fill_CSR_sparse_mat(d, E0, NU0)
+    assert ofa.solution_api(assam, d, "GPU_PCG_TWO_LEVEL", rtol=1e-12)
+    info = ofa.read_ca_info(os.path.join(GOLDEN, "CA_info.txt"))
+    dK = ctx.ca_delta(E0, NU0, info.kind, info.conn, info.etype, info.xyz)
+    dKo = O.ca_fill(s["kind"], s["cm"], np.zeros(len(s["kind"]), np.int32), s["nodes"], s["xyz"], s["resort"], E0, NU0,
+                    s["cp"], s["ri"])
+    assert rel_max(dK, dKo) < 1e-12
+    new = ofa.read_inp(os.path.join(GOLDEN, "mesh_m.inp"))
+    ref = np.load(os.path.join(GOLDEN, "cadcae_ca_u.npz"))["u_ca"]
+    for nb in (1, 4):
+        say(f"CA sample n_basis={nb}")
+        u_new = ofa.CA_ReAnalysis(assam, d, info, new, n_basis=nb, E=E0, nu=NU0).reshape(-1, 3)
+        uo, mo = O.ca_solve(s["cp"], s["ri"], s["K0"], dKo, s["u0"], s["F"], nb)
+        assert d.ca_info_["basis"] == mo and d.ca_info_["converged"]
+        assert np.linalg.norm(d.ca_dis_vec_ - uo) / np.linalg.norm(uo) < 1e-8
+        assert np.allclose(u_new, to_new_model(s["resort"], uo, s["idref"], new.nd), rtol=1e-6, atol=1e-9)
+        if nb == 1:
+            assert rel_max(u_new, ref) < 2e-5
+
+
+def test_ca_moves_and_additions_hex(ctx):
+    m = mg.hex_block(6, 4, 3, jitter=0.15, seed=2)
+    resort = mg.free_index(m.nd, m.fix_set)
+    say("CA hex: delete / move / add on a jittered 6x4x3 block")
+    ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
+    cp, ri = ctx.pattern()
+    K0 = ctx.assemble(E0, NU0)
+    rng = np.random.default_rng(4)
+    sel = rng.choice(m.ne, 12, replace=False)
+    kind = np.array([0] * 5 + [1] * 4 + [2] * 3, dtype=np.int32)
+    cm = (m.conn[sel] - 1).astype(np.int32)
+    xm = m.xyz.copy()
+    xm[np.unique(cm[5:9])] += rng.uniform(-0.05, 0.05, (len(np.unique(cm[5:9])), 3))
+    dK = ctx.ca_delta(E0, NU0, kind, cm, np.ones(12, np.int32), xm)
+    dKo = O.ca_fill(kind, cm, np.ones(12, np.int32), xm, m.xyz, resort, E0, NU0, cp, ri)
+    assert rel_max(dK, dKo) < 1e-12
+    n = ctx.n
+    F = mg.free_rhs(mg.face_traction(m, (0.0, 0.0, -5.0)), resort)
+    u0 = spla.spsolve(sp.csc_matrix((K0, ri, cp), shape=(n, n)), F)
+    for nb in (2, 6):
+        u, k, it, red, ok = ctx.ca_solve(nb, u0, F, method=A.PCG_BLOCK_JACOBI, rtol=1e-12)
+        uo, ko = O.ca_solve(cp, ri, K0, dKo, u0, F, nb)
+        assert ok and k == ko
+        assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8, nb
+        Kr, Fr, y = red
+        assert np.allclose(Kr, Kr.T, rtol=1e-8, atol=1e-8 * np.abs(Kr).max())
+    # a modified element coupling nodes outside the pattern is rejected
+    bad = (m.conn[-1:] - 1).astype(np.int32)  # last hex (x = L) with one node moved to x = h: both free, no shared element
+    first_free = int(np.argmax(resort >= 0))
+    assert abs(m.xyz[first_free, 0] - m.xyz[bad[0, 0], 0]) > 2.0
+    bad[0, 0] = first_free
+    with pytest.raises(A.OfeaError):
+        ctx.ca_delta(E0, NU0, [2], bad, [1])
+
+
+def test_ca_single_deletion_exact_gpu(ctx):
+    m, resort, F = mg.cantilever_c3d4(6, 3, 3, seed=5)
+    say("CA tet: one deleted interior tet, n_basis=7")
+    ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
+    cp, ri = ctx.pattern()
+    K0 = ctx.assemble(E0, NU0)
+    n = ctx.n
+    u0 = spla.spsolve(sp.csc_matrix((K0, ri, cp), shape=(n, n)), F)
+    e = len(m.etype) // 2
+    cm = np.zeros((1, 8), dtype=np.int32)
+    cm[0, :4] = m.conn[e, :4] - 1
+    dK = ctx.ca_delta(E0, NU0, [0], cm, [0])
+    ue = spla.spsolve(sp.csc_matrix((K0 + dK, ri, cp), shape=(n, n)), F)
+    u, k, it, _, ok = ctx.ca_solve(7, u0, F, method=A.PCG_TWO_LEVEL, rtol=1e-12)
+    assert ok and np.linalg.norm(u - ue) / np.linalg.norm(ue) < 1e-8
+
+
+POST_MESHES = {
+    "tet_mesh_o": None,
+    "hex_jitter_5x3x3": lambda: mg.hex_block(5, 3, 3, jitter=0.2, seed=8),
+    "tet_kuhn_6x3x3": lambda: mg.kuhn_tets(6, 3, 3, jitter=0.2, seed=3),
+}
+
+
+@pytest.mark.parametrize("name", list(POST_MESHES))
+def test_post_nodal_matches_oracle(ctx, name, sample):
+    rng = np.random.default_rng(6)
+    if name == "tet_mesh_o":
+        s = sample
+        xyz, conn, etype, resort = s["xyz"], s["conn"], s["etype"], s["resort"]
+        u_full = O.reset_displacement(resort, s["u0"])
+    else:
+        m = POST_MESHES[name]()
+        xyz, conn, etype = m.xyz, m.conn, m.etype
+        resort = mg.free_index(m.nd, m.fix_set)
+        u_full = rng.normal(size=3 * m.nd) * 1e-3
+    say(f"post {name}")
+    ctx.set_mesh(xyz, conn, etype, resort)
+    E_elem = rng.uniform(1e5, 3e5, len(etype))
+    for q, Ee in ((A.POST_STRAIN, None), (A.POST_STRESS, None), (A.POST_STRESS, E_elem)):
+        ctx.set_elem_E(Ee)
+        nv, ev = ctx.post_nodal(E0, NU0, u_full, q, elem=True)
+        evo = O.post_elem(conn, etype, xyz, u_full, E0, NU0, q, Ee)
+        nvo = O.post_node(conn, etype, len(xyz), evo)
+        for c in range(7):
+            assert rel_max(ev[c], evo[c]) < 1e-12, (q, c)
+            assert rel_max(nv[c], nvo[c]) < 1e-12, (q, c)
+    ctx.set_elem_E(None)
+    if name == "tet_mesh_o":
+        d = ofa.Deck(xyz, conn, etype, np.zeros(0, np.int32), np.zeros(0, np.int32), E=E0, nu=NU0)
+        nv = ofa.get_cauchy_stress_3d(ctx, d, u_full)
+        ref = np.load(os.path.join(GOLDEN, "cadcae_o_fields.npz"))["S"]
+        for c in range(7):
+            assert np.all(np.abs(nv[c] - ref[c]) <= 1e-5 * np.abs(ref[c]) + 1e-5 * np.abs(ref[c]).max()), c
+
+
+def test_post_patch_test_gpu(ctx):
+    m = mg.hex_block(4, 3, 3, jitter=0.2, seed=11)
+    resort = mg.free_index(m.nd, m.fix_set)
+    say("post patch test (distorted hexes)")
+    ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
+    A3 = np.random.default_rng(2).normal(size=(3, 3)) * 1e-3
+    u = (m.xyz @ A3.T).ravel()
+    nv, ev = ctx.post_nodal(E0, NU0, u, A.POST_STRAIN, elem=True)
+    eps = [A3[0, 0], A3[1, 1], A3[2, 2], A3[0, 1] + A3[1, 0], A3[1, 2] + A3[2, 1], A3[0, 2] + A3[2, 0]]
+    for c in range(6):
+        assert np.allclose(ev[c], eps[c], rtol=1e-10, atol=1e-16)
+        assert np.allclose(nv[c], eps[c], rtol=1e-10, atol=1e-16)
