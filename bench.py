@@ -11,7 +11,7 @@ line reports time-to-solve to a true relative residual of 1e-8 for plain
 FP64 Jacobi PCG and the mixed-precision/learned solvers, the roofline of the
 assembly kernel and the oracle's single-core CPU rate.
 
-Other workloads (--workload cfg1|cfg3|cfg4|cfg5) print their own line.
+Other workloads (--workload cfg1|cfg3|cfg4|cfg5|ca|post) print their own line.
 """
 import argparse
 import json
@@ -487,12 +487,202 @@ def bench_newton(args, ctx, m, resort, E, nu, F, tip, L_):
     return runs, train
 
 
+def hole_mesh(m, x0, z0, radius):
+    """Elements of m whose centroid lies in the cylinder (x - x0)^2 + (z - z0)^2 < r^2
+    (a hole through the y extent), and the mesh without them (nodes renumbered,
+    orphans dropped)."""
+    import numpy as np
+
+    from openfea_amd import meshgen as mg
+
+    c = mg.element_centroids(m)
+    dele = np.nonzero((c[:, 0] - x0) ** 2 + (c[:, 2] - z0) ** 2 < radius ** 2)[0]
+    keep = np.setdiff1d(np.arange(m.ne), dele)
+    used = np.unique(m.conn[keep, :4] - 1)
+    new_id = -np.ones(m.nd, dtype=np.int32)
+    new_id[used] = np.arange(len(used), dtype=np.int32)
+    conn = np.zeros((len(keep), 8), dtype=np.int32)
+    conn[:, :4] = new_id[m.conn[keep, :4] - 1] + 1
+    fix = new_id[m.fix_set - 1]
+    sub = mg.Mesh(m.xyz[used], conn, m.etype[keep], np.sort(fix[fix >= 0] + 1), m.dims)
+    return dele, sub, new_id
+
+
+def bench_ca(args):
+    import numpy as np
+
+    import openfea_amd as ofa
+    from openfea_amd import _capi as A, meshgen as mg
+    from oracle import oracle as O
+
+    def case(nx, ny, nz):
+        m, resort, F = mg.cantilever_c3d4(nx, ny, nz)
+        dele, sub, new_id = hole_mesh(m, nx / 2.0, nz / 2.0, nz / 7.5)
+        return m, resort, F, dele, sub, new_id
+
+    m, resort, F, dele, sub, new_id = case(60, 30, 30)
+    E, nu = mg.E_STEEL, mg.NU_STEEL
+    nb = args.ca_basis
+    wl = (f"CA reanalysis: C3D4 60x30x30 Kuhn (324,000 tets, jitter seed 20261203), {len(dele)} tets of a "
+          f"radius-4h hole through y at x = 30, z = 15 deleted, {nb}-vector CA basis, K0 solves two-level PCG "
+          f"to 1e-10, vs a full two-level PCG solve of the modified mesh")
+    ctx = ofa.Context(0)
+    ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
+    ctx.assemble(E, nu, copy=False)
+    u0, it0, _, ok0 = ctx.solve(F, method=A.PCG_TWO_LEVEL, rtol=1e-10)
+    log(f"ca: ne={m.ne} deleted {len(dele)}; K0 solve {it0} it ok={ok0}")
+    kind = np.zeros(len(dele), dtype=np.int32)
+    cm = (m.conn[dele] - 1).astype(np.int32)
+    cm[:, 4:] = 0
+    et = np.zeros(len(dele), dtype=np.int32)
+    for _ in range(2):  # warm-up (workspace, graphs)
+        ctx.ca_delta(E, nu, kind, cm, et, copy=False)
+        ctx.ca_solve(nb, u0, F, method=A.PCG_TWO_LEVEL, rtol=1e-10)
+    t_d, t_s = [], []
+    for _ in range(args.solve_reps):
+        ctx.sync()
+        t = time.perf_counter()
+        ctx.ca_delta(E, nu, kind, cm, et, copy=False)
+        t_d.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        u_ca, k, its, _, ok = ctx.ca_solve(nb, u0, F, method=A.PCG_TWO_LEVEL, rtol=1e-10)
+        t_s.append(time.perf_counter() - t)
+    ms_delta = statistics.median(t_d) * 1e3
+    ms_ca = statistics.median(t_s) * 1e3
+    log(f"ca: delta K {ms_delta:.2f} ms, CA solve {ms_ca:.2f} ms ({its} PCG it, basis {k}, ok={ok})")
+    n = ctx.n
+    nnz = ctx.nnz
+    ctx.close()
+    # full re-solve of the modified mesh (new pattern, new K, new preconditioner)
+    r_sub = mg.free_index(sub.nd, sub.fix_set)
+    F_full = ofa.reset_displacement(resort, F).reshape(-1, 3)
+    used = new_id >= 0
+    F_sub = np.zeros((sub.nd, 3))
+    F_sub[new_id[used]] = F_full[used]
+    F_sub_free = np.zeros(3 * int(r_sub.max() + 1))
+    fr = r_sub >= 0
+    F_sub_free.reshape(-1, 3)[r_sub[fr]] = F_sub[fr]
+    c2 = ofa.Context(0)
+    c2.sync()
+    t = time.perf_counter()
+    c2.set_mesh(sub.xyz, sub.conn, sub.etype, r_sub)
+    c2.assemble(E, nu, copy=False)
+    u_full_new, it_full, _, ok_full = c2.solve(F_sub_free, method=A.PCG_TWO_LEVEL, rtol=1e-10)
+    ms_full_first = (time.perf_counter() - t) * 1e3
+    c2.sync()
+    t = time.perf_counter()
+    _, it_full, _, _ = c2.solve(F_sub_free, method=A.PCG_TWO_LEVEL, rtol=1e-10, copy=False)
+    ms_full_solve = (time.perf_counter() - t) * 1e3
+    c2.close()
+    # accuracy of CA against the full solution on the kept nodes
+    u_ca_nodes = ofa.reset_displacement(resort, u_ca).reshape(-1, 3)[used]
+    u_new_nodes = ofa.reset_displacement(r_sub, u_full_new).reshape(-1, 3)[new_id[used]]
+    ca_err = float(np.linalg.norm(u_ca_nodes - u_new_nodes) / np.linalg.norm(u_new_nodes))
+    u0_err = float(np.linalg.norm(ofa.reset_displacement(resort, u0).reshape(-1, 3)[used] - u_new_nodes) /
+                   np.linalg.norm(u_new_nodes))
+    log(f"full re-solve {ms_full_first:.1f} ms incl. pattern/assembly/setup, solve {ms_full_solve:.2f} ms; "
+        f"CA rel err {ca_err:.3e} (unmodified u0: {u0_err:.3e})")
+    # bandwidth of the K0 solves (the dominant kernels): SpMV + vector passes per iteration
+    per_it = 12 * nnz + 4 * (n + 1) + 88 * n
+    achieved = its * per_it / (ms_ca * 1e-3) / 1e9 if its else 0.0
+    cpu = None
+    if not args.no_cpu_baseline:
+        ms_, rs_, Fs_, ds_, _, _ = case(30, 15, 15)
+        t = time.perf_counter()
+        ns = 3 * int(rs_.max() + 1)
+        cps, ris = O.build_csc(ms_.conn, ms_.etype, rs_, ns)
+        K0s = O.fill_csc(ms_.conn, ms_.etype, ms_.xyz, rs_, E, nu, cps, ris)
+        import scipy.sparse as sp
+        import scipy.sparse.linalg as spla
+        u0s = spla.splu(sp.csc_matrix((K0s, ris, cps), shape=(ns, ns))).solve(Fs_)
+        cms = (ms_.conn[ds_] - 1).astype(np.int32)
+        cms[:, 4:] = 0
+        t1 = time.perf_counter()
+        dKs = O.ca_fill(np.zeros(len(ds_), np.int32), cms, np.zeros(len(ds_), np.int32), ms_.xyz, ms_.xyz, rs_, E,
+                        nu, cps, ris)
+        O.ca_solve(cps, ris, K0s, dKs, u0s, Fs_, nb)
+        el = time.perf_counter() - t1
+        cpu = {"value": el * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+               "sample": f"oracle ca_fill + ca_solve (SuperLU factorisation of K0 inside, reference CA algorithm) "
+                         f"on the 30x15x15 Kuhn mesh ({ms_.ne} tets, {len(ds_)} deleted), {nb} vectors; K0 and u0 "
+                         f"prepared beforehand ({t1 - t:.1f}s)"}
+    return {
+        "metric": "CA reanalysis time (delta K + reduced solve) vs full re-solve", "value": ms_delta + ms_ca,
+        "unit": "ms", "n_gpus": 1, "steps": args.solve_reps, "warmup": 2, "ms_per_step": ms_delta + ms_ca,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": wl, "elements": m.ne, "deleted": int(len(dele)), "dofs": n, "nnz": nnz},
+        "delta_K_ms": ms_delta, "ca_solve_ms": ms_ca, "ca_pcg_iters": its, "ca_basis_kept": k,
+        "full_resolve_ms_incl_setup": ms_full_first, "full_resolve_solve_only_ms": ms_full_solve,
+        "full_resolve_pcg_iters": it_full, "ca_rel_err_vs_full": ca_err, "unmodified_u0_rel_err_vs_full": u0_err,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_cg_spmv + k_cg_update of the K0 solves (per iteration 12 nnz + 4 (n+1) + 88 n "
+                               "bytes; coarse-space terms not counted)",
+                     "algorithmic_bytes_per_launch": per_it},
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_post(args):
+    import numpy as np
+
+    import openfea_amd as ofa
+    from openfea_amd import _capi as A, meshgen as mg
+    from oracle import oracle as O
+
+    m, resort, F = mg.cantilever_c3d4()
+    E, nu = mg.E_STEEL, mg.NU_STEEL
+    wl = ("nodal strain recovery (get_cauchy_stress_3d): C3D4 60x30x30 Kuhn (324,000 tets), displacement = the "
+          "resident two-level PCG solution, per-element B u_e + Mises form, nodal averages")
+    ctx = ofa.Context(0)
+    ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
+    ctx.assemble(E, nu, copy=False)
+    x, it, _, ok = ctx.solve(F, method=A.PCG_TWO_LEVEL, rtol=1e-10)
+    for _ in range(args.warmup):
+        ctx.post_nodal(E, nu, None, A.POST_STRAIN, copy=False)
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(args.steps):
+        ctx.post_nodal(E, nu, None, A.POST_STRAIN, copy=False)
+    ms_step = ctx.timer_stop() / args.steps
+    value = m.ne / (ms_step * 1e-3)
+    nv, _ = ctx.post_nodal(E, nu, None, A.POST_STRAIN)
+    u_full = ofa.reset_displacement(resort, x)
+    ref = O.post_node(m.conn, m.etype, m.nd, O.post_elem(m.conn, m.etype, m.xyz, u_full, E, nu, 0))
+    err = max(float(np.abs(nv[c] - ref[c]).max() / np.abs(ref[c]).max()) for c in range(7))
+    log(f"post: {ms_step * 1e3:.1f} us/call -> {value:.3e} elems/s; max rel err vs oracle {err:.2e}")
+    n = ctx.n
+    ctx.close()
+    alg_bytes = 24 * m.nd + 8 * n + 4 * m.nd + 16 * m.ne + 56 * m.nd
+    achieved = alg_bytes / (ms_step * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        t = time.perf_counter()
+        O.post_node(m.conn, m.etype, m.nd, O.post_elem(m.conn, m.etype, m.xyz, u_full, E, nu, 0))
+        el = time.perf_counter() - t
+        cpu = {"value": m.ne / el, "unit": "elems/s", "cores": 1, "kind": "port",
+               "sample": f"oracle post_elem + post_node (reference get_cauchy_stress_3d algorithm) over the full "
+                         f"mesh, one pass ({el:.2f}s)"}
+    return {
+        "metric": "nodal strain recovery elems/s", "value": value, "unit": "elems/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": wl, "elements": m.ne, "nodes": m.nd},
+        "max_rel_err_vs_oracle": err,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("post_strain_cfg2"),
+                     "kernel": "k_expand + k_post_elem + k_post_node (one call)",
+                     "algorithmic_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3", "cfg4", "cfg5"])
+    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3", "cfg4", "cfg5", "ca", "post"])
     ap.add_argument("--explicit-steps", type=int, default=1000)
     ap.add_argument("--inner-rtol", type=float, default=1e-6, help="Newton inner PCG tolerance (cfg5)")
     ap.add_argument("--seq-methods", default="plain_fp64_jacobi,fp64_block_jacobi",
@@ -506,6 +696,7 @@ def main():
                     "mixed_fp32_all_block_jacobi,learned_fp64,learned_mixed_fp32val,two_level_fp64,"
                     "two_level_mixed_fp32val,learned_two_level_fp64,learned_two_level_mixed_fp32val")
     ap.add_argument("--train-steps", type=int, default=150)
+    ap.add_argument("--ca-basis", type=int, default=4, help="CA basis size (ca workload)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
@@ -516,6 +707,10 @@ def main():
         out = bench_explicit(args)
     elif args.workload == "cfg4":
         out = bench_batched(args)
+    elif args.workload == "ca":
+        out = bench_ca(args)
+    elif args.workload == "post":
+        out = bench_post(args)
     else:
         out = bench_nl(args)
     print(json.dumps(out), flush=True)

@@ -264,7 +264,10 @@ int ofea_ca_solve(ofea_ctx *ctx, int n_basis, const double *u0, const double *F,
  * (e2-e0)^2 + 6(e3^2+e4^2+e5^2)))); OFEA_POST_STRESS the Cauchy stress
  * D B u_e (E, or the moduli of ofea_set_elem_E) and its von Mises value.
  * C3D8 (no routine in the reference) uses the detJ-weighted mean of its 8
- * Gauss-point values. */
+ * Gauss-point values.  u_full NULL uses the context's resident solution of
+ * the last ofea_solve (expanded on the device, 0 at clamped nodes); with
+ * elem_out and node_out NULL the results stay on the device and the call
+ * returns without synchronising (time it with ofea_timer_*). */
 int ofea_post_nodal(ofea_ctx *ctx, double E, double nu, const double *u_full, int quantity, double *elem_out,
                     double *node_out);
 

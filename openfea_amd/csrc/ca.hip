@@ -67,13 +67,16 @@ __global__ __launch_bounds__(CA_WG) void k_mdot(int n, int nv, const double *__r
     }
 }
 
-__global__ void k_mdot_final(int nv, int nb, const double *__restrict__ part, double *__restrict__ out)
+// one wave per vector: lane l sums partials l, l + 64, ... in order, then a
+// fixed xor-shuffle tree; every total has one summation order
+__global__ __launch_bounds__(64) void k_mdot_final(int nb, const double *__restrict__ part, double *__restrict__ out)
 {
-    const int v = threadIdx.x;
-    if (v >= nv) return;
+    const int v = blockIdx.x, l = threadIdx.x;
     double acc = 0.0;
-    for (int b = 0; b < nb; ++b) acc += part[(size_t)v * nb + b];
-    out[v] = acc;
+    for (int b = l; b < nb; b += 64) acc += part[(size_t)v * nb + b];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (l == 0) out[v] = acc;
 }
 
 // y = beta * y + sum_v coef[v] V[v]
@@ -104,15 +107,8 @@ void free_ws(CaWs *w)
     delete w;
 }
 
-int ensure(ofea_ctx *c, CaWs *&w)
+int alloc_ws(CaWs *w)
 {
-    w = (CaWs *)c->ca_ws;
-    if (w && w->n == c->P.n && w->nnz == c->P.nnz) return OFEA_OK;
-    free_ws(w);
-    w = new CaWs();
-    c->ca_ws = w;
-    w->n = c->P.n;
-    w->nnz = c->P.nnz;
     const size_t n = std::max(w->n, 1);
     OFEA_HIP(hipMalloc((void **)&w->dK, sizeof(double) * std::max(w->nnz, 1LL)));
     OFEA_HIP(hipMalloc((void **)&w->basis, sizeof(double) * n * CA_MAXB));
@@ -125,12 +121,31 @@ int ensure(ofea_ctx *c, CaWs *&w)
     return OFEA_OK;
 }
 
+int ensure(ofea_ctx *c, CaWs *&w)
+{
+    w = (CaWs *)c->ca_ws;
+    if (w && w->n == c->P.n && w->nnz == c->P.nnz) return OFEA_OK;
+    free_ws(w);
+    c->ca_ws = nullptr;
+    w = new CaWs();
+    w->n = c->P.n;
+    w->nnz = c->P.nnz;
+    int rc = alloc_ws(w);
+    if (rc) {
+        free_ws(w);
+        w = nullptr;
+        return rc;
+    }
+    c->ca_ws = w;
+    return OFEA_OK;
+}
+
 // dots[v] = V[v] . x for v < nv, copied to the host
 int mdot(ofea_ctx *c, CaWs *w, const double *V, int nv, const double *x, double *host)
 {
     if (nv == 0) return OFEA_OK;
     hipLaunchKernelGGL(k_mdot, dim3(CA_NB), dim3(CA_WG), 0, c->stream, w->n, nv, V, x, w->part);
-    hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(64), 0, c->stream, nv, CA_NB, w->part, w->dots);
+    hipLaunchKernelGGL(k_mdot_final, dim3(nv), dim3(64), 0, c->stream, CA_NB, w->part, w->dots);
     OFEA_HIP(hipGetLastError());
     OFEA_HIP(hipMemcpyAsync(host, w->dots, sizeof(double) * nv, hipMemcpyDeviceToHost, c->stream));
     OFEA_HIP(hipStreamSynchronize(c->stream));
@@ -158,7 +173,6 @@ int lin2(ofea_ctx *c, CaWs *w, double a, const double *x, double b, const double
 // LU with partial pivoting of the m x m row-major A; solves A y = b in place
 bool dense_solve(int m, std::vector<double> A, std::vector<double> &b)
 {
-    std::vector<int> piv(m);
     for (int k = 0; k < m; ++k) {
         int p = k;
         for (int i = k + 1; i < m; ++i)

@@ -393,6 +393,7 @@ int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const i
             in_chunk = 0;
         }
         in_chunk += np;
+        if (sparse && np == 0) continue;  // a node no listed element touches: no blocks
         const int *nbr = P.bcol.data() + P.node_blk[r];
         lists.assign(deg, std::vector<uint16_t>());
         for (int t = cnt[r]; t < cnt[r + 1]; ++t) {

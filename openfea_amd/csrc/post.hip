@@ -4,7 +4,10 @@
 // with the element routine get_stress_node_hammer
 // (src/Elements/C3D4_hammer.cpp:185-226).  Two kernels:
 //   k_post_elem: one thread per element gathers its node coordinates and
-//     displacements and writes 7 values, component-major ([c][e]):
+//     displacements (a host vector, or the context's resident free-DOF
+//     solution read through the nodes' free ranks, 0 at clamped nodes) and
+//     writes its 7
+//     values as one 64-byte row:
 //       OFEA_POST_STRAIN  the reference's output, B u_e: engineering strain
 //                         [xx, yy, zz, xy, yz, xz] (the reference leaves
 //                         D * B u commented out), and its "Mises" formula
@@ -15,9 +18,9 @@
 //     C3D4 strain is constant; the reference has no C3D8 routine (ele_base's
 //     no-op), here C3D8 takes the volume average of the 2x2x2 Gauss-point
 //     strains, sum_g detJ_g B_g u_e / sum_g detJ_g;
-//   k_post_node: one thread per (node, component) sums the values of the
+//   k_post_node: eight lanes per node (lane = component) sum the rows of the
 //     node's elements in ascending element order (the reference's loop
-//     order) and divides by the number of element incidences; a node in no
+//     order) and divide by the number of element incidences; a node in no
 //     element keeps 0.
 #include <algorithm>
 #include <vector>
@@ -32,13 +35,15 @@ constexpr int POST_WG = 256;
 
 struct PostWs {
     int nd = 0, ne = 0;
-    int *conn = nullptr;     // ne x 8, 0-based
+    int *conn = nullptr;     // ne x 8, 0-based; C3D4 rows keep the nodes' free ranks in slots 4-7
+    int *rank8 = nullptr;    // ne x 8 free ranks of C3D8 nodes (meshes with C3D8 only)
     int *etype = nullptr;    // ne
     int *inc_ptr = nullptr;  // nd + 1
     int *inc = nullptr;      // element of each incidence, ascending per node
     double *E = nullptr;     // ne per-element moduli or null
     double *u = nullptr;     // 3 nd
-    double *ev = nullptr;    // 7 x ne
+    double *ev = nullptr;    // ne x 8: the 7 element values of element e in row e
+    double *ev7 = nullptr;   // 7 x ne, component-major copy for the caller (allocated on first request)
     double *nv = nullptr;    // 7 x nd
 };
 
@@ -64,25 +69,47 @@ __device__ inline void strain_of(const double g[3][NN], const double *U, double 
     }
 }
 
+// One thread per element.  RESIDENT: the displacement is the free-DOF solution
+// x read through the nodes' free ranks (0 at clamped nodes); otherwise the
+// full 3 nd vector u.
+// Output row e of ev holds the 7 values in doubles 0..6 (64-byte rows).
+// Consecutive workgroups go to one XCD (xcd_remap), so elements that share
+// nodes find them in the same L2.
+template <bool RESIDENT>
 __global__ __launch_bounds__(POST_WG) void k_post_elem(int ne, const int *__restrict__ conn,
-                                                       const int *__restrict__ etype, const double *__restrict__ xyz,
-                                                       const double *__restrict__ u, const double *__restrict__ Eel,
-                                                       double E, double nu, int quantity, double *__restrict__ ev)
+                                                       const int *__restrict__ rank8, const int *__restrict__ etype,
+                                                       const double *__restrict__ xyz, const double *__restrict__ u,
+                                                       const double *__restrict__ Eel, double E, double nu,
+                                                       int quantity, double *__restrict__ ev)
 {
-    const int e = blockIdx.x * POST_WG + threadIdx.x;
+    const int e = xcd_remap(blockIdx.x, gridDim.x) * POST_WG + threadIdx.x;
     if (e >= ne) return;
     const int *cn = conn + 8 * (size_t)e;
+    // free rank of each element node (-1 clamped): C3D4 keeps them in slots 4-7
+    // of its connectivity row, C3D8 in rank8
+    const int *rk = !RESIDENT ? nullptr : (etype[e] == OFEA_C3D4 ? cn + 4 : rank8 + 8 * (size_t)e);
+    auto load_u = [&](int a, int node, double *U) {
+        if (RESIDENT) {
+            const int r = rk[a];
+            const double *x = u + 3 * (size_t)max(r, 0);
+            U[0] = r >= 0 ? x[0] : 0.0;
+            U[1] = r >= 0 ? x[1] : 0.0;
+            U[2] = r >= 0 ? x[2] : 0.0;
+        } else {
+            U[0] = u[3 * (size_t)node];
+            U[1] = u[3 * (size_t)node + 1];
+            U[2] = u[3 * (size_t)node + 2];
+        }
+    };
     double eps[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (etype[e] == OFEA_C3D4) {
         double X[12], U[12];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-            const int nd = cn[a];
+            const int node = cn[a];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                X[3 * a + c] = xyz[3 * nd + c];
-                U[3 * a + c] = u[3 * nd + c];
-            }
+            for (int c = 0; c < 3; ++c) X[3 * a + c] = xyz[3 * node + c];
+            load_u(a, node, U + 3 * a);
         }
         double g[3][4];
         tet_grads(X, g);
@@ -91,12 +118,10 @@ __global__ __launch_bounds__(POST_WG) void k_post_elem(int ne, const int *__rest
         double X[24], U[24];
 #pragma unroll
         for (int a = 0; a < 8; ++a) {
-            const int nd = cn[a];
+            const int node = cn[a];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                X[3 * a + c] = xyz[3 * nd + c];
-                U[3 * a + c] = u[3 * nd + c];
-            }
+            for (int c = 0; c < 3; ++c) X[3 * a + c] = xyz[3 * node + c];
+            load_u(a, node, U + 3 * a);
         }
         double vol = 0.0;
         for (int gp = 0; gp < 8; ++gp) {
@@ -109,7 +134,7 @@ __global__ __launch_bounds__(POST_WG) void k_post_elem(int ne, const int *__rest
 #pragma unroll
         for (int i = 0; i < 6; ++i) eps[i] *= iv;
     }
-    double out[6];
+    double out[8];
     if (quantity == OFEA_POST_STRESS) {
         double lam, mu;
         lame<double>(Eel ? Eel[e] : E, nu, lam, mu);
@@ -122,45 +147,57 @@ __global__ __launch_bounds__(POST_WG) void k_post_elem(int ne, const int *__rest
 #pragma unroll
         for (int i = 0; i < 6; ++i) out[i] = eps[i];
     }
+    out[6] = mises_form(out);
+    out[7] = 0.0;
+    double2 *row = reinterpret_cast<double2 *>(ev + 8 * (size_t)e);
 #pragma unroll
-    for (int i = 0; i < 6; ++i) ev[(size_t)i * ne + e] = out[i];
-    ev[(size_t)6 * ne + e] = mises_form(out);
+    for (int i = 0; i < 4; ++i) row[i] = make_double2(out[2 * i], out[2 * i + 1]);
 }
 
-__global__ __launch_bounds__(POST_WG) void k_post_node(int nd, int ne, const int *__restrict__ inc_ptr,
+// Eight lanes per node, lane c < 7 owns component c: the lanes of a node read
+// one 64-byte element row per incidence, in ascending element order.  Output
+// component-major [c][node] (the layout the caller receives).
+__global__ __launch_bounds__(POST_WG) void k_post_node(int nd, const int *__restrict__ inc_ptr,
                                                        const int *__restrict__ inc, const double *__restrict__ ev,
                                                        double *__restrict__ nv)
 {
-    const int t = blockIdx.x * POST_WG + threadIdx.x;
-    if (t >= 7 * nd) return;
-    const int i = t / 7, c = t - 7 * i;
+    const int t = xcd_remap(blockIdx.x, gridDim.x) * POST_WG + threadIdx.x;
+    const int i = t >> 3, c = t & 7;
+    if (i >= nd) return;
     const int k0 = inc_ptr[i], k1 = inc_ptr[i + 1];
-    const double *col = ev + (size_t)c * ne;
     double acc = 0.0;
-    for (int k = k0; k < k1; ++k) acc += col[inc[k]];
-    nv[(size_t)c * nd + i] = k1 > k0 ? acc / (double)(k1 - k0) : 0.0;
+#pragma unroll 4
+    for (int k = k0; k < k1; ++k) acc += ev[8 * (size_t)inc[k] + c];
+    if (c < 7) nv[(size_t)c * nd + i] = k1 > k0 ? acc / (double)(k1 - k0) : 0.0;
+}
+
+// element rows [e][8] -> component-major [c][e] for the caller
+__global__ __launch_bounds__(POST_WG) void k_post_unpack(int ne, const double *__restrict__ ev, double *__restrict__ out)
+{
+    const int t = blockIdx.x * POST_WG + threadIdx.x;
+    if (t >= 7 * ne) return;
+    const int c = t / ne, e = t - c * ne;
+    out[t] = ev[8 * (size_t)e + c];
 }
 
 void free_ws(PostWs *w)
 {
     if (!w) return;
     dev_free(w->conn);
+    dev_free(w->rank8);
     dev_free(w->etype);
     dev_free(w->inc_ptr);
     dev_free(w->inc);
     dev_free(w->E);
     dev_free(w->u);
     dev_free(w->ev);
+    dev_free(w->ev7);
     dev_free(w->nv);
     delete w;
 }
 
-int ensure(ofea_ctx *c, PostWs *&w)
+int build(ofea_ctx *c, PostWs *w)
 {
-    w = (PostWs *)c->post_ws;
-    if (w) return OFEA_OK;
-    w = new PostWs();
-    c->post_ws = w;
     w->nd = c->nd;
     w->ne = c->ne;
     const int nd = c->nd, ne = c->ne;
@@ -174,6 +211,16 @@ int ensure(ofea_ctx *c, PostWs *&w)
             ptr[node + 1]++;
         }
     }
+    std::vector<int> rank8;
+    for (int e = 0; e < ne; ++e) {
+        int *row = conn0.data() + 8 * (size_t)e;
+        if (c->etype[e] == OFEA_C3D4) {
+            for (int a = 0; a < 4; ++a) row[4 + a] = c->resort[row[a]];
+        } else {
+            if (rank8.empty()) rank8.assign(8 * (size_t)ne, -1);
+            for (int a = 0; a < 8; ++a) rank8[8 * (size_t)e + a] = c->resort[row[a]];
+        }
+    }
     for (int i = 0; i < nd; ++i) ptr[i + 1] += ptr[i];
     std::vector<int> inc(ptr[nd]), fill(ptr.begin(), ptr.end() - 1);
     for (int e = 0; e < ne; ++e) {  // ascending e: the reference's accumulation order per node
@@ -182,12 +229,28 @@ int ensure(ofea_ctx *c, PostWs *&w)
     }
     int rc;
     if ((rc = dev_upload(w->conn, conn0.data(), conn0.size()))) return rc;
+    if (!rank8.empty() && (rc = dev_upload(w->rank8, rank8.data(), rank8.size()))) return rc;
     if ((rc = dev_upload(w->etype, c->etype.data(), c->etype.size()))) return rc;
     if ((rc = dev_upload(w->inc_ptr, ptr.data(), ptr.size()))) return rc;
     if ((rc = dev_upload(w->inc, inc.data(), std::max<size_t>(inc.size(), 1)))) return rc;
     if ((rc = dev_upload<double>(w->u, nullptr, 3 * (size_t)nd))) return rc;
-    if ((rc = dev_upload<double>(w->ev, nullptr, 7 * (size_t)ne))) return rc;
+    if ((rc = dev_upload<double>(w->ev, nullptr, 8 * (size_t)ne))) return rc;
     if ((rc = dev_upload<double>(w->nv, nullptr, 7 * (size_t)nd))) return rc;
+    return OFEA_OK;
+}
+
+int ensure(ofea_ctx *c, PostWs *&w)
+{
+    w = (PostWs *)c->post_ws;
+    if (w) return OFEA_OK;
+    w = new PostWs();
+    int rc = build(c, w);
+    if (rc) {
+        free_ws(w);
+        w = nullptr;
+        return rc;
+    }
+    c->post_ws = w;
     return OFEA_OK;
 }
 
@@ -203,7 +266,7 @@ int post_nodal(ofea_ctx *c, double E, double nu, const double *u_full, int quant
                double *node_out)
 {
     if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
-    if (!u_full) return fail(OFEA_ERR_ARG, "null displacement");
+    if (!u_full && !c->d_x) return fail(OFEA_ERR_STATE, "no displacement given and no resident solution");
     if (quantity != OFEA_POST_STRAIN && quantity != OFEA_POST_STRESS)
         return fail(OFEA_ERR_ARG, "quantity must be OFEA_POST_STRAIN or OFEA_POST_STRESS");
     if (quantity == OFEA_POST_STRESS && !(nu > -1.0 && nu < 0.5))
@@ -215,18 +278,28 @@ int post_nodal(ofea_ctx *c, double E, double nu, const double *u_full, int quant
     dev_free(w->E);
     if (!c->E_elem.empty() && (rc = dev_upload(w->E, c->E_elem.data(), c->E_elem.size()))) return rc;
     const int nd = c->nd, ne = c->ne;
-    OFEA_HIP(hipMemcpyAsync(w->u, u_full, sizeof(double) * 3 * nd, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_post_elem, dim3((ne + POST_WG - 1) / POST_WG), dim3(POST_WG), 0, c->stream, ne, w->conn,
-                       w->etype, c->d_xyz, w->u, w->E, E, nu, quantity, w->ev);
+    const dim3 ge((ne + POST_WG - 1) / POST_WG), gn((8 * nd + POST_WG - 1) / POST_WG), blk(POST_WG);
+    if (u_full) {
+        OFEA_HIP(hipMemcpyAsync(w->u, u_full, sizeof(double) * 3 * nd, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_post_elem<false>, ge, blk, 0, c->stream, ne, w->conn, w->rank8, w->etype, c->d_xyz, w->u,
+                           w->E, E, nu, quantity, w->ev);
+    } else {  // the context's resident free-DOF solution, read through the nodes' free ranks
+        hipLaunchKernelGGL(k_post_elem<true>, ge, blk, 0, c->stream, ne, w->conn, w->rank8, w->etype, c->d_xyz, c->d_x,
+                           w->E, E, nu, quantity, w->ev);
+    }
     OFEA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_post_node, dim3((7 * nd + POST_WG - 1) / POST_WG), dim3(POST_WG), 0, c->stream, nd, ne,
-                       w->inc_ptr, w->inc, w->ev, w->nv);
+    hipLaunchKernelGGL(k_post_node, gn, blk, 0, c->stream, nd, w->inc_ptr, w->inc, w->ev, w->nv);
     OFEA_HIP(hipGetLastError());
-    if (elem_out)
-        OFEA_HIP(hipMemcpyAsync(elem_out, w->ev, sizeof(double) * 7 * ne, hipMemcpyDeviceToHost, c->stream));
+    if (elem_out) {
+        if (!w->ev7 && (rc = dev_upload<double>(w->ev7, nullptr, 7 * (size_t)ne))) return rc;
+        hipLaunchKernelGGL(k_post_unpack, dim3((7 * ne + POST_WG - 1) / POST_WG), blk, 0, c->stream, ne, w->ev,
+                           w->ev7);
+        OFEA_HIP(hipGetLastError());
+        OFEA_HIP(hipMemcpyAsync(elem_out, w->ev7, sizeof(double) * 7 * ne, hipMemcpyDeviceToHost, c->stream));
+    }
     if (node_out)
         OFEA_HIP(hipMemcpyAsync(node_out, w->nv, sizeof(double) * 7 * nd, hipMemcpyDeviceToHost, c->stream));
-    OFEA_HIP(hipStreamSynchronize(c->stream));
+    if (elem_out || node_out) OFEA_HIP(hipStreamSynchronize(c->stream));
     return OFEA_OK;
 }
 

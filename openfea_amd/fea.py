@@ -322,11 +322,14 @@ class Context:
         return u, k, it.value, reduced, rc == A.OFEA_OK
 
     # ---- post-processing
-    def post_nodal(self, E, nu, u_full, quantity=A.POST_STRAIN, elem=False):
-        """(node fields 7 x nd, element fields 7 x ne or None) from the full displacement."""
-        nv = np.zeros((7, self.nd))
-        ev = np.zeros((7, self.ne)) if elem else None
-        check(lib().ofea_post_nodal(self._h, E, nu, ptr(f64(u_full)), quantity, ptr(ev), ptr(nv)))
+    def post_nodal(self, E, nu, u_full, quantity=A.POST_STRAIN, elem=False, copy=True):
+        """(node fields 7 x nd, element fields 7 x ne or None) from the full
+        displacement; u_full None uses the resident solution of the last
+        solve, copy=False leaves the results on the device."""
+        nv = np.zeros((7, self.nd)) if copy else None
+        ev = np.zeros((7, self.ne)) if (elem and copy) else None
+        uu = None if u_full is None else f64(u_full)
+        check(lib().ofea_post_nodal(self._h, E, nu, ptr(uu), quantity, ptr(ev), ptr(nv)))
         return nv, ev
 
     # ---- timing on the context stream
