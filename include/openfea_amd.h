@@ -103,6 +103,31 @@ int ofea_inp_get(const ofea_deck *deck, double *xyz, int *conn, int *etype, int 
                  double *t_total);
 void ofea_inp_close(ofea_deck *deck);
 
+/* ---- .inp dialect extensions (no reference counterpart; no GPU needed) ----
+ * ofea_inp_open also reads, at keyword level, what the reference's reader
+ * leaves out: *Nset and *Elset (ids, set names, generate), *Surface type=ELEMENT
+ * ("elset_or_element, S#", Abaqus face numbering), every *Cload line (loads
+ * add), *Dsload (P: pressure, traction -p n_out; TRVEC: magnitude and
+ * direction) as consistent nodal forces (tri faces exact, quad faces 2x2
+ * Gauss), *Boundary (ENCASTRE, PINNED, XSYMM/YSYMM/ZSYMM or "first[, last[,
+ * value]]"), *Initial Conditions type=VELOCITY and *Material, *Elastic,
+ * *Density assigned by *Solid Section.  Ids map to line-order indices id-1
+ * as in the reference.  A deck whose extension blocks cannot be read still
+ * opens; these getters then return OFEA_ERR_IO with the reason. */
+/* f: nd*3 nodal forces of all *Cload and *Dsload lines */
+int ofea_inp_nodal_loads(const ofea_deck *deck, double *f);
+/* nodes with all three DOFs constrained (1-based ids into fixed, capacity nd;
+ * ascending; NULL to count): the clamp list of ofea_build_free_index.
+ * npartial counts constrained DOFs of nodes not fully fixed and nnonzero the
+ * entries with a prescribed nonzero value; neither fits the reference's
+ * node elimination (callers reject or treat them otherwise). */
+int ofea_inp_boundary(const ofea_deck *deck, int *nfixed, int *fixed, int *npartial, int *nnonzero);
+/* v0: nd*3 initial velocity (0 where none is given) */
+int ofea_inp_initial_velocity(const ofea_deck *deck, double *v0);
+/* per-element E, nu, rho from the section assignments (ne each, nullable;
+ * NaN for an element without a section) */
+int ofea_inp_materials(const ofea_deck *deck, double *E_elem, double *nu_elem, double *rho_elem);
+
 /* ---- host drop-in: boundary conditions (no GPU needed) ---- */
 /* set_BCs::build_free_index: sorts fix_set in place, fills resort (nd) with
  * the free-node rank or -1; returns the number of free nodes (>= 0) or < 0. */

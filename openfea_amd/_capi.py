@@ -105,6 +105,10 @@ SIGNATURES = {
     "ofea_ca_delta": (_i, [_vp, _d, _d, _i, _vp, _vp, _vp, _vp, _vp]),
     "ofea_ca_solve": (_i, [_vp, _i, _vp, _vp, _i, _d, _i, _vp, _pi, _pi, _vp]),
     "ofea_post_nodal": (_i, [_vp, _d, _d, _vp, _i, _vp, _vp]),
+    "ofea_inp_nodal_loads": (_i, [_vp, _vp]),
+    "ofea_inp_boundary": (_i, [_vp, _pi, _vp, _pi, _pi]),
+    "ofea_inp_initial_velocity": (_i, [_vp, _vp]),
+    "ofea_inp_materials": (_i, [_vp, _vp, _vp, _vp]),
 }
 
 _LIB = None

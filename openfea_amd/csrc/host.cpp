@@ -523,6 +523,14 @@ int ofea_inp_open(const char *path, ofea_deck **deck)
         delete D;
         return rc;
     }
+    // the extensions never make a deck the reference reads unreadable: their
+    // error is reported by the ofea_inp_* getters that need them
+    const std::string keep = ofea_last_error();
+    if (parse_inp_ext(path, *D) != OFEA_OK) {
+        D->ext_ok = false;
+        D->ext_error = ofea_last_error();
+    }
+    set_error(keep);
     *deck = D;
     return OFEA_OK;
 }

@@ -67,6 +67,8 @@ struct GatherPlan {
 };
 
 int build_pattern(int nd, int ne, const int *conn, const int *etype, const int *resort, Pattern &P);
+// keyword-level deck extensions into D (inp_ext.cpp); D's mesh must be read
+int parse_inp_ext(const std::string &path, ofea_deck &D);
 // sparse = true leaves out the blocks that no element of the list touches
 // (plans over a few modified elements, whose kernels accumulate into K).
 int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const int *resort, const Pattern &P,
@@ -103,6 +105,15 @@ struct ofea_deck {
     int load_dof = 0;
     double load_value = 0.0;
     double E, nu, rho, dt = 0.0, t_total = 0.0;
+    // keyword-level extensions (inp_ext.cpp); ext_ok false with ext_error set
+    // when a supported keyword block could not be read
+    bool ext_ok = false;
+    std::string ext_error;
+    std::vector<double> f_nodal, v0;               // nd x 3: summed *Cload + *Dsload forces; initial velocity
+    std::vector<unsigned char> bc_mask;            // nd: bit d set = DOF d+1 constrained by *Boundary
+    int bc_nonzero = 0;                            // constrained (node, range) entries with a nonzero value
+    std::vector<double> E_elem, nu_elem, rho_elem;  // ne, NaN without a section
+    std::vector<std::pair<std::string, std::string>> sections;  // (elset, material), lower case
 };
 
 // CA change file (ReadInfo::CA_read_change, src/data2cae.cpp:519-665)

@@ -12,6 +12,7 @@ Names follow the reference so that a driver written against it reads the same:
   ReadInfo::CA_read_change          -> read_ca_info()     (src/data2cae.cpp:519-665)
   CAE_process::CA_ReAnalysis        -> CA_ReAnalysis()    (src/cae.cpp:317-376)
   simulation_post::reset_ca_displacement -> reset_ca_displacement() (src/post_process.cpp:42-76)
+  CAE_process::implict_analysis     -> implict_analysis() (src/cae.cpp:198-301)
 All numerical work runs in the native library; this module only marshals arrays.
 """
 from __future__ import annotations
@@ -42,6 +43,16 @@ class Deck:
     t_total: float = 0.0
     resort: np.ndarray = field(default=None)
     F: np.ndarray = field(default=None)
+    # keyword-level extensions (None when the deck's extension blocks are unreadable: see ext_error)
+    f_nodal: np.ndarray = field(default=None)    # nd x 3, all *Cload and *Dsload
+    v0: np.ndarray = field(default=None)         # nd x 3, *Initial Conditions type=VELOCITY
+    bc_fixed: np.ndarray = field(default=None)   # 1-based ids of nodes with all DOFs constrained
+    bc_partial: int = 0
+    bc_nonzero: int = 0
+    E_elem: np.ndarray = field(default=None)     # per-element section material (NaN without one)
+    nu_elem: np.ndarray = field(default=None)
+    rho_elem: np.ndarray = field(default=None)
+    ext_error: str = ""
 
     @property
     def nd(self):
@@ -69,10 +80,27 @@ def read_inp(path: str) -> Deck:
         vals = [C.c_double() for _ in range(6)]
         check(L.ofea_inp_get(h, ptr(xyz), ptr(conn), ptr(etype), ptr(fix), ptr(load), C.byref(dof),
                              *[C.byref(v) for v in vals]))
+        ext = _read_ext(L, h, nd.value, ne.value)
     finally:
         L.ofea_inp_close(h)
     lv, E, nu, rho, dt, T = [v.value for v in vals]
-    return Deck(xyz, conn, etype, fix, load, dof.value, lv, E, nu, rho, dt, T)
+    return Deck(xyz, conn, etype, fix, load, dof.value, lv, E, nu, rho, dt, T, **ext)
+
+
+def _read_ext(L, h, nd, ne):
+    f = np.zeros((nd, 3))
+    rc = L.ofea_inp_nodal_loads(h, ptr(f))
+    if rc != A.OFEA_OK:
+        return {"ext_error": L.ofea_last_error().decode(errors="replace")}
+    v0 = np.zeros((nd, 3))
+    check(L.ofea_inp_initial_velocity(h, ptr(v0)))
+    nf, npart, nnz = C.c_int(), C.c_int(), C.c_int()
+    fixed = np.zeros(nd, dtype=np.int32)
+    check(L.ofea_inp_boundary(h, C.byref(nf), ptr(fixed), C.byref(npart), C.byref(nnz)))
+    Ee, nue, rhoe = np.zeros(ne), np.zeros(ne), np.zeros(ne)
+    check(L.ofea_inp_materials(h, ptr(Ee), ptr(nue), ptr(rhoe)))
+    return {"f_nodal": f, "v0": v0, "bc_fixed": fixed[:nf.value].copy(), "bc_partial": npart.value,
+            "bc_nonzero": nnz.value, "E_elem": Ee, "nu_elem": nue, "rho_elem": rhoe}
 
 
 def build_free_index(nd: int, fix_set) -> tuple[np.ndarray, int]:
@@ -413,3 +441,55 @@ def CA_ReAnalysis(assam: AssambleStiffness, deck: Deck, info: CAInfo, deck_new: 
     deck.ca_dis_vec_ = u
     deck.ca_info_ = {"basis": m, "pcg_iters": it, "converged": ok}
     return reset_ca_displacement(deck.resort, u, info.idref2new, deck_new.nd)
+
+
+def implict_analysis(deck: Deck, result_path: str | None = None, solver_type: str = "GPU_PCG_TWO_LEVEL",
+                     ctx: Context | None = None, E: float | None = None, nu: float | None = None, rtol: float = 1e-10):
+    """CAE_process::implict_analysis (src/cae.cpp:198-301) for a linear static
+    deck: clamp (Set-fix, plus nodes *Boundary fixes in all three DOFs),
+    loads (every *Cload and *Dsload; on the reference's decks this equals
+    build_single_load), material (E, nu of the deck, or per-element E of
+    *Solid Section regions with one Poisson ratio), pattern + assembly,
+    the GPU solve, reset_displacement, the nodal strain fields of
+    get_cauchy_stress_3d and, with result_path, the VTK file of
+    export_dis_2_vtk.  Returns (u_full nd x 3, nodal fields 7 x nd)."""
+    if deck.ext_error:
+        raise ValueError(f"deck extensions unreadable: {deck.ext_error}")
+    if deck.bc_partial or deck.bc_nonzero:
+        raise ValueError("*Boundary constrains single DOFs or prescribes values: the node-elimination "
+                         "numbering clamps whole nodes only")
+    fix = np.union1d(i32(deck.fix_set), i32(deck.bc_fixed)).astype(np.int32)
+    deck.resort, nfree = build_free_index(deck.nd, fix)
+    F = np.zeros(3 * nfree)
+    free = deck.resort >= 0
+    F.reshape(-1, 3)[deck.resort[free]] = deck.f_nodal[free]
+    deck.F = F
+    E = deck.E if E is None else E
+    nu = deck.nu if nu is None else nu
+    own = ctx is None
+    ctx = ctx or Context()
+    try:
+        assam = AssambleStiffness(ctx)
+        assam.build_CSR(deck)
+        sec = deck.E_elem is not None and np.all(np.isfinite(deck.E_elem))
+        if sec:
+            nus = np.unique(deck.nu_elem)
+            if len(nus) != 1:
+                raise ValueError("section materials with different Poisson ratios: one nu per assembly")
+            nu = float(nus[0])
+            ctx.set_elem_E(deck.E_elem)
+        else:
+            ctx.set_elem_E(None)
+        assam.fill_CSR_sparse_mat(deck, E, nu)
+        if not solution_api(assam, deck, solver_type, rtol=rtol):
+            raise ValueError(f"unknown solver {solver_type}")
+        if not deck.solve_info["converged"]:
+            raise RuntimeError(f"solver did not converge: {deck.solve_info}")
+        u_full = reset_displacement(deck.resort, deck.single_dis_vec_)
+        fields = get_cauchy_stress_3d(ctx, deck, u_full, E, nu)
+        if result_path:
+            export_dis_2_vtk(result_path, deck, u_full, 1.0, fields)
+        return u_full.reshape(-1, 3), fields
+    finally:
+        if own:
+            ctx.close()

@@ -228,3 +228,105 @@ def cantilever_c3d4(nx=60, ny=30, nz=30, h=1.0, jitter=0.2, seed=20261203):
     resort = free_index(m.nd, m.fix_set)
     F = free_rhs(face_traction(m, (0.0, 0.0, -5.0)), resort)
     return m, resort, F
+
+
+# Abaqus face numbering (0-based local nodes): C3D8 S1..S6, C3D4 S1..S4
+HEX_FACES = [(0, 1, 2, 3), (4, 7, 6, 5), (0, 4, 5, 1), (1, 5, 6, 2), (2, 6, 7, 3), (3, 7, 4, 0)]
+TET_FACES = [(0, 1, 2), (0, 3, 1), (1, 3, 2), (2, 3, 0)]
+
+
+def faces_on_plane(mesh: Mesh, axis=0, value=None):
+    """(element index, Abaqus face number 1-based) of the element faces lying on
+    the plane x_axis = value (default: x = L)."""
+    nx, ny, nz, h = mesh.dims
+    value = nx * h if value is None else value
+    on = np.abs(mesh.xyz[:, axis] - value) < 1e-9 * max(h, 1.0)
+    out = []
+    for e in range(mesh.ne):
+        table = HEX_FACES if mesh.etype[e] == 1 else TET_FACES
+        cn = mesh.conn[e] - 1
+        for f, fl in enumerate(table):
+            if all(on[cn[a]] for a in fl):
+                out.append((e, f + 1))
+    return out
+
+
+def write_inp(path, mesh: Mesh, traction=None, pressure=None, cloads=(), velocity=None, materials=None,
+              boundary="ENCASTRE", dynamic=None):
+    """Writes mesh as a deck in the reference's dialect (count lines after the
+    first *Nset/*Elset, Set-fix/Set-load node sets, *Cload, *Elastic) plus the
+    keyword extensions: a *Surface of the x = L faces with a TRVEC traction
+    (traction = vector) or pressure (*Dsload, P), extra point loads (cloads:
+    (node id, dof, value)), *Initial Conditions type=VELOCITY (velocity: nd x 3),
+    per-region materials (materials: list of (element mask, E, nu, rho)) and
+    *Boundary on Set-fix.  dynamic = (dt, T) adds the *Dynamic, Explicit line."""
+    nd, ne = mesh.nd, mesh.ne
+    L = []
+    L.append("*Node")
+    for i, (x, y, z) in enumerate(mesh.xyz):
+        L.append(f"{i + 1}, {float(x)!r}, {float(y)!r}, {float(z)!r}")
+    for et, name, nn in ((1, "C3D8", 8), (0, "C3D4", 4)):
+        ids = np.nonzero(mesh.etype == et)[0]
+        if len(ids) == 0:
+            continue
+        L.append(f"*Element, type={name}")
+        for e in ids:
+            L.append(", ".join([str(e + 1)] + [str(v) for v in mesh.conn[e, :nn]]))
+    L += ["*Nset", f"1, {nd}, 1", "*Elset", f"1, {ne}, 1"]
+    L.append("*Nset, nset=Set-fix")
+    fix = list(mesh.fix_set)
+    for k in range(0, len(fix), 16):
+        L.append(", ".join(str(v) for v in fix[k:k + 16]))
+    nx, ny, nz, h = mesh.dims
+    tip = np.nonzero(np.abs(mesh.xyz[:, 0] - nx * h) < 1e-9 * max(h, 1.0))[0] + 1
+    L.append("*Nset, nset=Set-load")
+    for k in range(0, len(tip), 16):
+        L.append(", ".join(str(v) for v in tip[k:k + 16]))
+    mats = materials or [(np.ones(ne, bool), E_STEEL, NU_STEEL, RHO_STEEL)]
+    for m, (mask, E, nu, rho) in enumerate(mats):
+        ids = np.nonzero(mask)[0] + 1
+        L.append(f"*Elset, elset=Region-{m + 1}")
+        for k in range(0, len(ids), 16):
+            L.append(", ".join(str(v) for v in ids[k:k + 16]))
+        L.append(f"*Solid Section, elset=Region-{m + 1}, material=Mat-{m + 1}")
+        L.append(",")
+    faces = faces_on_plane(mesh)
+    by_face = {}
+    for e, f in faces:
+        by_face.setdefault(f, []).append(e + 1)
+    for f, ids in sorted(by_face.items()):
+        L.append(f"*Elset, elset=_Surf-L_S{f}, internal")
+        for k in range(0, len(ids), 16):
+            L.append(", ".join(str(v) for v in ids[k:k + 16]))
+    L.append("*Surface, type=ELEMENT, name=Surf-L")
+    for f in sorted(by_face):
+        L.append(f"_Surf-L_S{f}, S{f}")
+    L.append("*End Assembly")
+    for m, (mask, E, nu, rho) in enumerate(mats):
+        L += [f"*Material, name=Mat-{m + 1}", "*Density", f"{float(rho)!r},", "*Elastic", f"{float(E)!r}, {float(nu)!r}"]
+    if velocity is not None:
+        L.append("*Initial Conditions, type=VELOCITY")
+        for i in range(nd):
+            for c in range(3):
+                if velocity[i, c] != 0.0:
+                    L.append(f"{i + 1}, {c + 1}, {float(velocity[i, c])!r}")
+    L.append("** STEP")
+    if dynamic is not None:
+        L += ["*Dynamic, Explicit", f"{float(dynamic[0])!r}, {float(dynamic[1])!r}"]
+    else:
+        L += ["*Static", "1., 1., 1e-05, 1."]
+    if boundary:
+        L += ["*Boundary", f"Set-fix, {boundary}"]
+    L += ["*Cload", "Set-load, 3, 0."]
+    for node, dof, val in cloads:
+        L.append(f"{node}, {dof}, {float(val)!r}")
+    if traction is not None:
+        t = np.asarray(traction, dtype=np.float64)
+        mag = float(np.linalg.norm(t))
+        d = t / mag if mag > 0 else np.array([0.0, 0.0, 1.0])
+        L += ["*Dsload", f"Surf-L, TRVEC, {mag!r}, {float(d[0])!r}, {float(d[1])!r}, {float(d[2])!r}"]
+    if pressure is not None:
+        L += ["*Dsload", f"Surf-L, P, {float(pressure)!r}"]
+    L.append("*End Step")
+    with open(path, "w") as fh:
+        fh.write("\n".join(L) + "\n")
