@@ -51,7 +51,8 @@ def test_ca_sample_pipeline(ctx, sample):
     assam = ofa.AssambleStiffness(ctx)
     assam.build_CSR(d)
     assam.fill_CSR_sparse_mat(d, E0, NU0)
-    assert ofa.solution_api(assam, d, "GPU_PCG_TWO_LEVEL", rtol=1e-12)
+    assert ofa.solution_api(assam, d, "GPU_PCG_TWO_LEVEL", rtol=1e-11)
+    assert d.solve_info["converged"], d.solve_info
     info = ofa.read_ca_info(os.path.join(GOLDEN, "CA_info.txt"))
     dK = ctx.ca_delta(E0, NU0, info.kind, info.conn, info.etype, info.xyz)
     dKo = O.ca_fill(s["kind"], s["cm"], np.zeros(len(s["kind"]), np.int32), s["nodes"], s["xyz"], s["resort"], E0, NU0,
