@@ -11,7 +11,7 @@ line reports time-to-solve to a true relative residual of 1e-8 for plain
 FP64 Jacobi PCG and the mixed-precision/learned solvers, the roofline of the
 assembly kernel and the oracle's single-core CPU rate.
 
-Other workloads (--workload cfg1|cfg3|cfg4|cfg5|ca|post) print their own line.
+Other workloads (--workload cfg1|cfg3|cfg4|cfg5|ca|post|sfem) print their own line.
 """
 import argparse
 import json
@@ -677,12 +677,69 @@ def bench_post(args):
     }
 
 
+def bench_sfem(args):
+    import numpy as np
+
+    import openfea_amd as ofa
+    from openfea_amd import _capi as A, meshgen as mg
+    from oracle import oracle as O
+
+    m, resort, F = mg.cantilever_c3d4()
+    E, nu = mg.E_STEEL, mg.NU_STEEL
+    wl = ("node-based smoothed FEM (SFEM_C3D4): C3D4 60x30x30 Kuhn (324,000 tets, 58,621 smoothing domains), "
+          "smoothed gradients + stabilisation, FP64 assembly into the smoothed CSC pattern + two-level PCG")
+    ctx = ofa.Context(0)
+    ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
+    t = time.perf_counter()
+    ctx.sfem_setup()
+    setup_s = time.perf_counter() - t
+    log(f"sfem: setup {setup_s:.2f}s n={ctx.n} nnz={ctx.nnz}")
+    for _ in range(args.warmup):
+        ctx.sfem_assemble(E, nu, copy=False)
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(args.steps):
+        ctx.sfem_assemble(E, nu, copy=False)
+    ms_step = ctx.timer_stop() / args.steps
+    value = m.ne / (ms_step * 1e-3)
+    x, it, rel, ok = ctx.solve(F, method=A.PCG_TWO_LEVEL, rtol=1e-8)
+    t = time.perf_counter()
+    _, it, rel, ok = ctx.solve(F, method=A.PCG_TWO_LEVEL, rtol=1e-8, copy=False)
+    solve_ms = (time.perf_counter() - t) * 1e3
+    log(f"sfem assembly {ms_step * 1e3:.1f} us -> {value:.3e} elems/s; solve {solve_ms:.1f} ms ({it} it, ok={ok})")
+    nnz, n = ctx.nnz, ctx.n
+    ctx.close()
+    alg_bytes = 24 * m.nd + 16 * m.ne + 8 * nnz
+    achieved = alg_bytes / (ms_step * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        ms_, rs_, _ = mg.cantilever_c3d4(30, 15, 15)
+        cp, ri = O.sfem_build_csc(ms_.conn, ms_.xyz, rs_)
+        t = time.perf_counter()
+        O.sfem_fill(ms_.conn, ms_.xyz, rs_, E, nu, cp, ri)
+        el = time.perf_counter() - t
+        cpu = {"value": ms_.ne / el, "unit": "elems/s", "cores": 1, "kind": "port",
+               "sample": f"oracle sfem_fill (reference six-term domain stiffness + linear-search scatter) on the "
+                         f"30x15x15 Kuhn mesh ({ms_.ne} tets), one pass ({el:.1f}s)"}
+    return {
+        "metric": "smoothed-FEM assembled elems/s", "value": value, "unit": "elems/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": wl, "elements": m.ne, "nodes": m.nd, "dofs": n, "nnz": nnz},
+        "setup_s": round(setup_s, 3), "two_level_solve_ms": solve_ms, "solve_iters": it, "solve_converged": ok,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("sfem_assemble_cfg2"),
+                     "kernel": "k_sfem_grad + k_sfem_gather (one assembly)", "algorithmic_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3", "cfg4", "cfg5", "ca", "post"])
+    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3", "cfg4", "cfg5", "ca", "post", "sfem"])
     ap.add_argument("--explicit-steps", type=int, default=1000)
     ap.add_argument("--inner-rtol", type=float, default=1e-6, help="Newton inner PCG tolerance (cfg5)")
     ap.add_argument("--seq-methods", default="plain_fp64_jacobi,fp64_block_jacobi",
@@ -711,6 +768,8 @@ def main():
         out = bench_ca(args)
     elif args.workload == "post":
         out = bench_post(args)
+    elif args.workload == "sfem":
+        out = bench_sfem(args)
     else:
         out = bench_nl(args)
     print(json.dumps(out), flush=True)

@@ -36,6 +36,10 @@
  *   ofea_post_nodal       simulation_post::get_cauchy_stress_3d (src/post_process.cpp:79-168)
  *   ofea_write_vtk        data_process::export_dis_2_vtk / CA_export_dis_2_vtk
  *                         (src/data2vtk.cpp:47-264, :266-427)
+ *   ofea_sfem_setup       SFEM3D::build + assamble_stiffness::SFEM_build_CSR
+ *                         (src/SFEM3D.cpp:8-134, src/assemble.cpp:291-350)
+ *   ofea_sfem_assemble    SFEM_fill_CSR_sparse_mat + SFEM_C3D4::build_ele_stiff_mat
+ *                         (src/assemble.cpp:369-414, src/Elements/SFEM_C3D4.cpp:53-427)
  */
 #ifndef OPENFEA_AMD_H
 #define OPENFEA_AMD_H
@@ -295,6 +299,24 @@ int ofea_ca_solve(ofea_ctx *ctx, int n_basis, const double *u0, const double *F,
  * returns without synchronising (time it with ofea_timer_*). */
 int ofea_post_nodal(ofea_ctx *ctx, double E, double nu, const double *u_full, int quantity, double *elem_out,
                     double *node_out);
+
+/* ---- node-based smoothed FEM, C3D4 only (CAE_process::implict_SFEManalysis,
+ * src/cae.cpp:379-416) ----
+ * ofea_sfem_setup builds the smoothing domains of the context's mesh (one per
+ * node: its cell in the surrounding tets, closed by the surface faces through
+ * it) and replaces the context's pattern by the smoothed one: column j holds
+ * row i when some node's ring contains both (SFEM_build_CSR), so ofea_pattern,
+ * ofea_solve, ofea_spmv and ofea_post_nodal work on the smoothed system.
+ * ofea_assemble, ofea_nl_*, ofea_assemble_batch and ofea_ca_delta return
+ * OFEA_ERR_STATE on such a context until ofea_set_mesh is called again.
+ * ofea_sfem_assemble fills K = sum over domains k of V_k B(G)^T D B(G) +
+ * (V_k/3) sum_in B(a_in)^T D B(a_in), G the smoothed gradients and a_in the
+ * columns of the gradient-smoothing term scaled by (3 V_k / 4 pi)^(1/3) (the
+ * reference's six-term form, expanded), into the CSC values (resident, and
+ * copied to nz_val when not NULL).  Blocks sum their domains in ascending
+ * order; no atomics. */
+int ofea_sfem_setup(ofea_ctx *ctx);
+int ofea_sfem_assemble(ofea_ctx *ctx, double E, double nu, double *nz_val);
 
 /* ---- batched reanalysis (no counterpart in the reference) ----
  * Many design variants of one mesh: variant v has per-element moduli

@@ -109,6 +109,8 @@ SIGNATURES = {
     "ofea_inp_boundary": (_i, [_vp, _pi, _vp, _pi, _pi]),
     "ofea_inp_initial_velocity": (_i, [_vp, _vp]),
     "ofea_inp_materials": (_i, [_vp, _vp, _vp, _vp]),
+    "ofea_sfem_setup": (_i, [_vp]),
+    "ofea_sfem_assemble": (_i, [_vp, _d, _d, _vp]),
 }
 
 _LIB = None

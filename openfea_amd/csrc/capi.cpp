@@ -60,6 +60,8 @@ static void free_mesh(ofea_ctx *c)
     batch_free(c);
     ca_free(c);
     post_free(c);
+    sfem_free(c);
+    c->sfem = false;
     solver_free(c);
     explicit_free(c);
     dev_free(c->d_xyz);
@@ -192,10 +194,16 @@ int ofea_set_elem_E(ofea_ctx *c, const double *E_elem)
     return OFEA_OK;
 }
 
+#define NOT_SFEM(c)                                                                                       \
+    do {                                                                                                  \
+        if ((c)->sfem) return fail(OFEA_ERR_STATE, "the context holds the smoothed (SFEM) pattern: ofea_set_mesh again"); \
+    } while (0)
+
 int ofea_assemble(ofea_ctx *c, double E, double nu, double *nz_val)
 {
     CHECK_CTX(c);
     if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    NOT_SFEM(c);
     if (!(nu > -1.0 && nu < 0.5)) return fail(OFEA_ERR_ARG, "Poisson ratio must lie in (-1, 0.5)");
     int rc = launch_assemble(c, E, nu, c->d_val, false);
     if (rc) return rc;
@@ -292,6 +300,7 @@ int ofea_nl_assemble(ofea_ctx *c, double E, double nu, const double *u, double *
 {
     CHECK_CTX(c);
     if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    NOT_SFEM(c);
     if (!u) return fail(OFEA_ERR_ARG, "null displacement");
     return nl_assemble(c, E, nu, u, nz_val, f_int);
 }
@@ -301,6 +310,7 @@ int ofea_nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr,
 {
     CHECK_CTX(c);
     if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    NOT_SFEM(c);
     if (!F || n_incr < 1 || !(tol > 0.0) || max_newton < 1 || !(inner_rtol > 0.0))
         return fail(OFEA_ERR_ARG, "bad Newton arguments");
     return nl_solve(c, E, nu, F, n_incr, tol, max_newton, method, inner_rtol, u_out, newton_iters, linear_iters,
@@ -311,6 +321,7 @@ int ofea_ca_delta(ofea_ctx *c, double E, double nu, int nmod, const int *kind, c
                   const int *etype_mod, const double *xyz_m, double *dK_val)
 {
     CHECK_CTX(c);
+    NOT_SFEM(c);
     return ca_delta(c, E, nu, nmod, kind, conn_mod, etype_mod, xyz_m, dK_val);
 }
 
@@ -320,6 +331,18 @@ int ofea_ca_solve(ofea_ctx *c, int n_basis, const double *u0, const double *F, i
     CHECK_CTX(c);
     if (!(rtol > 0.0) || maxit <= 0) return fail(OFEA_ERR_ARG, "rtol must be > 0 and maxit > 0");
     return ca_solve(c, n_basis, u0, F, method, rtol, maxit, u_out, n_used, pcg_iters, reduced);
+}
+
+int ofea_sfem_setup(ofea_ctx *c)
+{
+    CHECK_CTX(c);
+    return sfem_setup(c);
+}
+
+int ofea_sfem_assemble(ofea_ctx *c, double E, double nu, double *nz_val)
+{
+    CHECK_CTX(c);
+    return sfem_assemble(c, E, nu, nz_val);
 }
 
 int ofea_post_nodal(ofea_ctx *c, double E, double nu, const double *u_full, int quantity, double *elem_out,
@@ -412,6 +435,7 @@ int ofea_assemble_batch(ofea_ctx *c, int nvar, const double *E_var, double nu, d
 {
     CHECK_CTX(c);
     if (!c->d_col_ptr) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    NOT_SFEM(c);
     return assemble_batch(c, nvar, E_var, nu, nz_val);
 }
 

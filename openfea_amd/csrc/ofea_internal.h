@@ -164,6 +164,9 @@ struct ofea_ctx {
     void *ca_ws = nullptr;
     // nodal strain/stress recovery (post.hip)
     void *post_ws = nullptr;
+    // node-based smoothed FEM (sfem.hip): once set up, P is the smoothing-domain pattern
+    void *sfem_ws = nullptr;
+    bool sfem = false;
 };
 
 namespace ofea {
@@ -216,6 +219,10 @@ int ca_delta(ofea_ctx *c, double E, double nu, int nmod, const int *kind, const 
 int ca_solve(ofea_ctx *c, int n_basis, const double *u0, const double *F, int method, double rtol, int maxit,
              double *u_out, int *n_used, int *pcg_iters, double *reduced);
 void ca_free(ofea_ctx *c);
+// node-based smoothed FEM (sfem.hip)
+int sfem_setup(ofea_ctx *c);
+int sfem_assemble(ofea_ctx *c, double E, double nu, double *nz_val);
+void sfem_free(ofea_ctx *c);
 // nodal strain / stress recovery (post.hip)
 int post_nodal(ofea_ctx *c, double E, double nu, const double *u_full, int quantity, double *elem_out,
                double *node_out);

@@ -1,0 +1,543 @@
+// sfem.hip -- node-based smoothed FEM (C3D4) on the device.
+//
+// Replaces SFEM3D::build / SFEM_C3D4::build_ele_stiff_mat / shapeGradient
+// (src/SFEM3D.cpp:8-163, src/Elements/SFEM_C3D4.cpp:53-427) and
+// assamble_stiffness::SFEM_build_CSR / SFEM_fill_CSR_sparse_mat
+// (src/assemble.cpp:291-414), driven as CAE_process::implict_SFEManalysis
+// (src/cae.cpp:379-416).
+//
+// Smoothing domain k = the cell of node k: in each tet around k, the part
+// bounded by three interior facets (edge midpoint, two face centroids, the
+// tet centroid) plus, where the tet's face through k lies on the surface,
+// the corner quad of that face.  Per member i of ring(k) (the nodes of the
+// tets around k):
+//   G_i  = (1 / V_k) sum over the cell boundary of  int N_i n dA
+//   GG_i = l (1 / V_k) sum of  int grad N_i (x) n dA,  l = (3 V_k / (4 pi))^(1/3)
+// with V_k = sum |V_e| / 4 and n pointing out of the cell.  The reference's
+// six-term loop, sum over m of (V_k/6) B(G + s GG[:, in])^T D B(G + s GG[:, in])
+// (s = -1 for m < 3, +1 otherwise, in = m mod 3), expands to
+//   K_k = V_k K(G, G) + (V_k / 3) sum_in K(a_in, a_in),  a_in = GG[:, in],
+//   K(x, y)_ij[c][d] = lambda x_i[c] y_j[d] + mu x_i[d] y_j[c] + mu delta_cd x_i . y_j,
+// the form used here.  Facet integrals: each facet is a planar convex quad,
+// bilinearly mapped, integrated by 2x2 Gauss (exact for N_i |x_xi x x_eta|).
+//
+// Kernels:
+//   k_sfem_grad   one workgroup per domain: lane (element, rotated corner j)
+//                 integrates the element's facets of the cell for corner j;
+//                 lane = member then sums them in ascending element order
+//                 and writes a 16-double row {G, a_0, a_1, a_2, V_k, 0..}
+//   k_sfem_gather one lane per 3x3 block (r, s) of the pattern sums its
+//                 contributors (domains containing both r and s, ascending
+//                 domain: the reference's summation order) and writes the 9
+//                 CSC values once; no atomics.
+#include <algorithm>
+#include <cmath>
+#include <unordered_map>
+#include <vector>
+
+#include "elem_device.h"
+#include "ofea_internal.h"
+
+namespace ofea {
+namespace {
+
+constexpr int SF_WG = 128;
+constexpr int SF_MAXE = 96;   // tets around one node
+constexpr int SF_MAXM = 128;  // nodes in one node's ring (<= SF_WG)
+
+struct SfemWs {
+    int nd = 0, ndom = 0;
+    long long nrows = 0;            // total (domain, member) rows
+    int *conn = nullptr;            // ne x 4, 0-based
+    unsigned char *fflag = nullptr; // ne: bit f = face f on the surface (faces 123, 124, 234, 134)
+    int *dom_mptr = nullptr;        // ndom + 1 into members
+    int *dom_member = nullptr;      // member node ids (ascending per domain)
+    int *dom_eptr = nullptr;        // ndom + 1 into elements
+    int *dom_elem = nullptr;        // incident elements (ascending)
+    double *vol = nullptr;          // ndom: V_k
+    double *rows = nullptr;         // nrows x 16
+    int *cptr = nullptr;            // blocks + 1
+    int2 *contrib = nullptr;        // (row of r, row of s) per contributor
+    long long ncontrib = 0;
+    int *blk_r = nullptr;           // column node of each block
+};
+
+__device__ inline void d_cross(const double a[3], const double b[3], double o[3])
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Adds int_facet N_j n dA (gsum) and int_facet n dA (nsum) of one planar quad
+// facet with corners P[4]; N_j at the corners is phi[c][j]; n is oriented away
+// from the point ref.
+__device__ inline void facet(const double P[4][3], const double phi[4][4], const double ref[3], double gsum[4][3],
+                             double nsum[3])
+{
+    double v12[3], v13[3], nrm[3];
+    for (int c = 0; c < 3; ++c) {
+        v12[c] = P[1][c] - P[0][c];
+        v13[c] = P[2][c] - P[0][c];
+    }
+    d_cross(v12, v13, nrm);
+    const double nl = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    double cen[3];
+    for (int c = 0; c < 3; ++c) cen[c] = 0.25 * (P[0][c] + P[1][c] + P[2][c] + P[3][c]) - ref[c];
+    const double sg = (cen[0] * nrm[0] + cen[1] * nrm[1] + cen[2] * nrm[2]) < 0.0 ? -1.0 : 1.0;
+    for (int c = 0; c < 3; ++c) nrm[c] *= sg / nl;
+    const double q = 0.57735026918962576451;
+    const double xs[4] = {-1.0, 1.0, 1.0, -1.0}, es[4] = {-1.0, -1.0, 1.0, 1.0};
+    for (int g = 0; g < 4; ++g) {
+        const double xi = xs[g] * q, eta = es[g] * q;
+        double H[4], dx[3] = {0, 0, 0}, de[3] = {0, 0, 0};
+        for (int a = 0; a < 4; ++a) {
+            H[a] = 0.25 * (1.0 + xs[a] * xi) * (1.0 + es[a] * eta);
+            const double hx = 0.25 * xs[a] * (1.0 + es[a] * eta), he = 0.25 * es[a] * (1.0 + xs[a] * xi);
+            for (int c = 0; c < 3; ++c) {
+                dx[c] += hx * P[a][c];
+                de[c] += he * P[a][c];
+            }
+        }
+        double cr[3];
+        d_cross(dx, de, cr);
+        const double dA = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+        for (int c = 0; c < 3; ++c) nsum[c] += nrm[c] * dA;
+        for (int j = 0; j < 4; ++j) {
+            const double N = H[0] * phi[0][j] + H[1] * phi[1][j] + H[2] * phi[2][j] + H[3] * phi[3][j];
+            for (int c = 0; c < 3; ++c) gsum[j][c] += N * nrm[c] * dA;
+        }
+    }
+}
+
+// Contribution of one tet (corners x[4], rotated so that corner 0 is the
+// domain node) to the 4 corners: gsum[j] = int N_j n over the cell's facets
+// in this tet, nsum = int n (for GG = grad N_j (x) nsum).  bflag bit b: the
+// face opposite rotated corner b+1 (corners {0, .., } without b+1) is a
+// surface face.
+__device__ inline void tet_cell(const double x[4][3], int bflag, double gsum[4][3], double nsum[3])
+{
+    double m1[3], m2[3], m3[3], c1[3], c2[3], c3[3], cc[3];
+    for (int c = 0; c < 3; ++c) {
+        m1[c] = 0.5 * (x[0][c] + x[1][c]);
+        m2[c] = 0.5 * (x[0][c] + x[2][c]);
+        m3[c] = 0.5 * (x[0][c] + x[3][c]);
+        c1[c] = (x[0][c] + x[1][c] + x[2][c]) / 3.0;  // face 0-1-2
+        c2[c] = (x[0][c] + x[2][c] + x[3][c]) / 3.0;  // face 0-2-3
+        c3[c] = (x[0][c] + x[1][c] + x[3][c]) / 3.0;  // face 0-1-3
+        cc[c] = 0.25 * (x[0][c] + x[1][c] + x[2][c] + x[3][c]);
+    }
+    const double t = 1.0 / 3.0, h = 0.5, f = 0.25;
+    // interior facets: (c3, m1, c1, cc), (c1, m2, c2, cc), (c2, m3, c3, cc), outward from corner 0
+    {
+        double P[4][3];
+        for (int c = 0; c < 3; ++c) { P[0][c] = c3[c]; P[1][c] = m1[c]; P[2][c] = c1[c]; P[3][c] = cc[c]; }
+        const double phi[4][4] = {{t, t, 0, t}, {h, h, 0, 0}, {t, t, t, 0}, {f, f, f, f}};
+        facet(P, phi, x[0], gsum, nsum);
+    }
+    {
+        double P[4][3];
+        for (int c = 0; c < 3; ++c) { P[0][c] = c1[c]; P[1][c] = m2[c]; P[2][c] = c2[c]; P[3][c] = cc[c]; }
+        const double phi[4][4] = {{t, t, t, 0}, {h, 0, h, 0}, {t, 0, t, t}, {f, f, f, f}};
+        facet(P, phi, x[0], gsum, nsum);
+    }
+    {
+        double P[4][3];
+        for (int c = 0; c < 3; ++c) { P[0][c] = c2[c]; P[1][c] = m3[c]; P[2][c] = c3[c]; P[3][c] = cc[c]; }
+        const double phi[4][4] = {{t, 0, t, t}, {h, 0, 0, h}, {t, t, 0, t}, {f, f, f, f}};
+        facet(P, phi, x[0], gsum, nsum);
+    }
+    // surface corner quads of corner 0, outward from the tet centroid
+    if (bflag & 1) {  // face 0-2-3 (opposite corner 1): corner 0, mid 0-2, centroid, mid 0-3
+        double P[4][3];
+        for (int c = 0; c < 3; ++c) { P[0][c] = m2[c]; P[1][c] = c2[c]; P[2][c] = m3[c]; P[3][c] = x[0][c]; }
+        const double phi[4][4] = {{h, 0, h, 0}, {t, 0, t, t}, {h, 0, 0, h}, {1, 0, 0, 0}};
+        facet(P, phi, cc, gsum, nsum);
+    }
+    if (bflag & 4) {  // face 0-1-2 (opposite corner 3)
+        double P[4][3];
+        for (int c = 0; c < 3; ++c) { P[0][c] = m1[c]; P[1][c] = c1[c]; P[2][c] = m2[c]; P[3][c] = x[0][c]; }
+        const double phi[4][4] = {{h, h, 0, 0}, {t, t, t, 0}, {h, 0, h, 0}, {1, 0, 0, 0}};
+        facet(P, phi, cc, gsum, nsum);
+    }
+    if (bflag & 2) {  // face 0-1-3 (opposite corner 2)
+        double P[4][3];
+        for (int c = 0; c < 3; ++c) { P[0][c] = m3[c]; P[1][c] = c3[c]; P[2][c] = m1[c]; P[3][c] = x[0][c]; }
+        const double phi[4][4] = {{h, 0, 0, h}, {t, t, 0, t}, {h, h, 0, 0}, {1, 0, 0, 0}};
+        facet(P, phi, cc, gsum, nsum);
+    }
+}
+
+// face index (faces 123, 124, 234, 134 of the element's own node order) of
+// the face opposite local node p
+__device__ inline int face_opposite(int p) { return p == 0 ? 2 : (p == 1 ? 3 : (p == 2 ? 1 : 0)); }
+
+__global__ __launch_bounds__(SF_WG) void k_sfem_grad(int ndom, const double *__restrict__ xyz, const int *__restrict__ conn,
+                                                     const unsigned char *__restrict__ fflag,
+                                                     const int *__restrict__ mptr, const int *__restrict__ member,
+                                                     const int *__restrict__ eptr, const int *__restrict__ elem,
+                                                     const double *__restrict__ vol, double *__restrict__ rows)
+{
+    __shared__ double s_g[SF_MAXE * 4][3];
+    __shared__ double s_n[SF_MAXE][3];
+    __shared__ double s_t[SF_MAXE * 4][3];  // grad N of the element's rotated corners
+    __shared__ int s_node[SF_MAXE * 4];
+    const int k = xcd_remap(blockIdx.x, gridDim.x);
+    if (k >= ndom) return;
+    const int e0 = eptr[k], ne = eptr[k + 1] - e0;
+    const int m0 = mptr[k], nm = mptr[k + 1] - m0;
+    for (int q = threadIdx.x; q < ne; q += SF_WG) {
+        const int e = elem[e0 + q];
+        const int *cn = conn + 4 * (size_t)e;
+        int p = 0;
+        for (int a = 1; a < 4; ++a)
+            if (cn[a] == k) p = a;
+        double x[4][3], X[12];
+        int nodes[4];
+        for (int j = 0; j < 4; ++j) {
+            nodes[j] = cn[(p + j) & 3];
+            for (int c = 0; c < 3; ++c) x[j][c] = X[3 * j + c] = xyz[3 * nodes[j] + c];
+        }
+        // surface flags of the faces opposite rotated corners 1, 3, 2
+        const int ff = fflag[e];
+        int b = 0;
+        if (ff >> face_opposite((p + 1) & 3) & 1) b |= 1;
+        if (ff >> face_opposite((p + 2) & 3) & 1) b |= 2;
+        if (ff >> face_opposite((p + 3) & 3) & 1) b |= 4;
+        double gsum[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, nsum[3] = {0, 0, 0};
+        tet_cell(x, b, gsum, nsum);
+        double g[3][4];
+        tet_grads(X, g);
+        for (int j = 0; j < 4; ++j) {
+            s_node[4 * q + j] = nodes[j];
+            for (int c = 0; c < 3; ++c) {
+                s_g[4 * q + j][c] = gsum[j][c];
+                s_t[4 * q + j][c] = g[c][j];
+            }
+        }
+        for (int c = 0; c < 3; ++c) s_n[q][c] = nsum[c];
+    }
+    __syncthreads();
+    const double V = vol[k];
+    const double l = cbrt(3.0 * V / (4.0 * 3.14159265358979323846));
+    for (int i = threadIdx.x; i < nm; i += SF_WG) {
+        const int node = member[m0 + i];
+        double G[3] = {0, 0, 0}, GG[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int q = 0; q < ne; ++q)
+            for (int j = 0; j < 4; ++j)
+                if (s_node[4 * q + j] == node) {
+                    for (int c = 0; c < 3; ++c) G[c] += s_g[4 * q + j][c];
+                    for (int a = 0; a < 3; ++a)
+                        for (int c = 0; c < 3; ++c) GG[a][c] += s_t[4 * q + j][a] * s_n[q][c];
+                }
+        double *row = rows + 16 * (size_t)(m0 + i);
+        const double iv = 1.0 / V;
+        for (int c = 0; c < 3; ++c) row[c] = G[c] * iv;
+        for (int in = 0; in < 3; ++in)
+            for (int a = 0; a < 3; ++a) row[3 + 3 * in + a] = GG[a][in] * iv * l;
+        row[12] = V;
+        row[13] = row[14] = row[15] = 0.0;
+    }
+}
+
+__device__ inline void kform(const double *x, const double *y, double lam, double mu, double w, double acc[9])
+{
+    const double dot = x[0] * y[0] + x[1] * y[1] + x[2] * y[2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) acc[3 * c + d] += w * (lam * x[c] * y[d] + mu * x[d] * y[c] + (c == d ? mu * dot : 0.0));
+}
+
+__global__ __launch_bounds__(256) void k_sfem_gather(int nblk, const int *__restrict__ blk_r, const int *__restrict__ node_blk,
+                                                     const int *__restrict__ col_ptr, const int *__restrict__ cptr,
+                                                     const int2 *__restrict__ contrib, const double *__restrict__ rows,
+                                                     double lam, double mu, double *__restrict__ val)
+{
+    const int b = xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (b >= nblk) return;
+    const int r = blk_r[b];
+    const int k = b - node_blk[r];
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int t = cptr[b]; t < cptr[b + 1]; ++t) {
+        const int2 pr = contrib[t];
+        const double *x = rows + 16 * (size_t)pr.x, *y = rows + 16 * (size_t)pr.y;
+        const double V = x[12];
+        kform(x, y, lam, mu, V, acc);
+        kform(x + 3, y + 3, lam, mu, V / 3.0, acc);
+        kform(x + 6, y + 6, lam, mu, V / 3.0, acc);
+        kform(x + 9, y + 9, lam, mu, V / 3.0, acc);
+    }
+    const int base = col_ptr[3 * r], stride = col_ptr[3 * r + 1] - base;
+    double *dst = val + base + 3 * k;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) dst[c * stride + d] = acc[3 * c + d];
+}
+
+void free_ws(SfemWs *w)
+{
+    if (!w) return;
+    dev_free(w->conn);
+    dev_free(w->fflag);
+    dev_free(w->dom_mptr);
+    dev_free(w->dom_member);
+    dev_free(w->dom_eptr);
+    dev_free(w->dom_elem);
+    dev_free(w->vol);
+    dev_free(w->rows);
+    dev_free(w->cptr);
+    dev_free(w->contrib);
+    dev_free(w->blk_r);
+    delete w;
+}
+
+}  // namespace
+
+struct SfemHost {
+    std::vector<int> conn;                // ne x 4, 0-based
+    std::vector<unsigned char> fflag;     // ne
+    std::vector<int> mptr, member, eptr, elem;
+    std::vector<double> vol;
+};
+
+// Smoothing domains of a C3D4 mesh (SFEM3D::eleData / nodeData,
+// src/SFEM3D.cpp:31-134): surface faces are the faces met once; per node its
+// tets (ascending), its ring (sorted node ids of those tets) and V_k = sum |V_e| / 4.
+int sfem_domains(int nd, int ne, const int *conn8, const double *xyz, SfemHost &H)
+{
+    H.conn.resize(4 * (size_t)ne);
+    for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < 4; ++a) {
+            const int v = conn8[8 * (size_t)e + a] - 1;
+            if (v < 0 || v >= nd) return fail(OFEA_ERR_ARG, "element node out of range");
+            H.conn[4 * (size_t)e + a] = v;
+        }
+    static const int FACES[4][3] = {{0, 1, 2}, {0, 1, 3}, {1, 2, 3}, {0, 2, 3}};
+    std::unordered_map<unsigned long long, int> count;
+    count.reserve(4 * (size_t)ne);
+    auto key = [&](int e, int f) {
+        int v[3] = {H.conn[4 * (size_t)e + FACES[f][0]], H.conn[4 * (size_t)e + FACES[f][1]],
+                    H.conn[4 * (size_t)e + FACES[f][2]]};
+        std::sort(v, v + 3);
+        return ((unsigned long long)v[0] << 42) | ((unsigned long long)v[1] << 21) | (unsigned long long)v[2];
+    };
+    if (nd >= (1 << 21)) return fail(OFEA_ERR_LIMIT, "too many nodes for the face key");
+    for (int e = 0; e < ne; ++e)
+        for (int f = 0; f < 4; ++f) count[key(e, f)] ^= 1;  // a face met twice is interior
+    H.fflag.assign(ne, 0);
+    for (int e = 0; e < ne; ++e)
+        for (int f = 0; f < 4; ++f)
+            if (count[key(e, f)]) H.fflag[e] |= (unsigned char)(1 << f);
+    H.eptr.assign(nd + 1, 0);
+    for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < 4; ++a) H.eptr[H.conn[4 * (size_t)e + a] + 1]++;
+    for (int i = 0; i < nd; ++i) H.eptr[i + 1] += H.eptr[i];
+    H.elem.resize(H.eptr[nd]);
+    std::vector<int> fill(H.eptr.begin(), H.eptr.end() - 1);
+    for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < 4; ++a) H.elem[fill[H.conn[4 * (size_t)e + a]]++] = e;
+    H.vol.assign(nd, 0.0);
+    std::vector<double> ve(ne);
+    for (int e = 0; e < ne; ++e) {
+        const double *p[4];
+        for (int a = 0; a < 4; ++a) p[a] = xyz + 3 * (size_t)H.conn[4 * (size_t)e + a];
+        double u[3], v[3], w[3];
+        for (int c = 0; c < 3; ++c) {
+            u[c] = p[0][c] - p[3][c];
+            v[c] = p[1][c] - p[3][c];
+            w[c] = p[2][c] - p[3][c];
+        }
+        ve[e] = std::fabs(u[0] * (v[1] * w[2] - v[2] * w[1]) - u[1] * (v[0] * w[2] - v[2] * w[0]) +
+                          u[2] * (v[0] * w[1] - v[1] * w[0])) / 6.0;
+    }
+    H.mptr.assign(nd + 1, 0);
+    H.member.clear();
+    std::vector<int> ring;
+    for (int k = 0; k < nd; ++k) {
+        ring.clear();
+        for (int t = H.eptr[k]; t < H.eptr[k + 1]; ++t) {
+            const int e = H.elem[t];
+            H.vol[k] += ve[e] / 4.0;
+            for (int a = 0; a < 4; ++a) ring.push_back(H.conn[4 * (size_t)e + a]);
+        }
+        std::sort(ring.begin(), ring.end());
+        ring.erase(std::unique(ring.begin(), ring.end()), ring.end());
+        if (H.eptr[k + 1] - H.eptr[k] > SF_MAXE) return fail(OFEA_ERR_LIMIT, "a node has more than 96 tets");
+        if ((int)ring.size() > SF_MAXM) return fail(OFEA_ERR_LIMIT, "a node ring has more than 128 nodes");
+        H.member.insert(H.member.end(), ring.begin(), ring.end());
+        H.mptr[k + 1] = (int)H.member.size();
+    }
+    return OFEA_OK;
+}
+
+// SFEM_build_CSR (src/assemble.cpp:291-350): column j holds row i when some
+// domain's ring contains both (rows ascending, clamped DOFs dropped), built
+// node-block-wise like build_pattern.
+int sfem_pattern(int nd, const SfemHost &H, const int *resort, Pattern &P)
+{
+    P = Pattern();
+    int nfree = 0;
+    for (int i = 0; i < nd; ++i)
+        if (resort[i] >= 0) nfree = std::max(nfree, resort[i] + 1);
+    P.nfree = nfree;
+    P.n = 3 * nfree;
+    std::vector<std::vector<int>> nb(nfree);
+    for (int k = 0; k < nd; ++k)
+        for (int a = H.mptr[k]; a < H.mptr[k + 1]; ++a) {
+            const int r = resort[H.member[a]];
+            if (r < 0) continue;
+            for (int b = H.mptr[k]; b < H.mptr[k + 1]; ++b) {
+                const int s = resort[H.member[b]];
+                if (s >= 0) nb[r].push_back(s);
+            }
+        }
+    P.node_blk.assign(nfree + 1, 0);
+    for (int r = 0; r < nfree; ++r) {
+        auto &v = nb[r];
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        if (v.size() > 1024) return fail(OFEA_ERR_LIMIT, "an SFEM node couples more than 1024 nodes");
+        P.node_blk[r + 1] = P.node_blk[r] + (int)v.size();
+    }
+    const long long nblk = P.node_blk[nfree];
+    if (9 * nblk >= (1LL << 31)) return fail(OFEA_ERR_LIMIT, "pattern exceeds int32 indices");
+    P.nnz = 9 * nblk;
+    P.bcol.resize(nblk);
+    P.diag_k.assign(nfree, -1);
+    P.col_ptr.assign(P.n + 1, 0);
+    P.row_idx.resize(P.nnz);
+    for (int r = 0; r < nfree; ++r) {
+        const auto &v = nb[r];
+        const int deg = (int)v.size();
+        std::copy(v.begin(), v.end(), P.bcol.begin() + P.node_blk[r]);
+        for (int k = 0; k < deg; ++k)
+            if (v[k] == r) P.diag_k[r] = k;
+        for (int c = 0; c < 3; ++c) {
+            const long long base = 9LL * P.node_blk[r] + (long long)c * 3 * deg;
+            P.col_ptr[3 * r + c] = (int)base;
+            for (int k = 0; k < deg; ++k)
+                for (int d = 0; d < 3; ++d) P.row_idx[base + 3 * k + d] = 3 * v[k] + d;
+        }
+    }
+    P.col_ptr[P.n] = (int)P.nnz;
+    return OFEA_OK;
+}
+
+void sfem_free(ofea_ctx *c)
+{
+    free_ws((SfemWs *)c->sfem_ws);
+    c->sfem_ws = nullptr;
+}
+
+int sfem_setup(ofea_ctx *c)
+{
+    if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    if (c->has_type[OFEA_C3D8]) return fail(OFEA_ERR_ARG, "the smoothed method takes C3D4 meshes only");
+    SfemHost H;
+    int rc = sfem_domains(c->nd, c->ne, c->conn.data(), c->xyz.data(), H);
+    if (rc) return rc;
+    Pattern P;
+    if ((rc = sfem_pattern(c->nd, H, c->resort.data(), P))) return rc;
+    // contributors of every block (r, k), ascending domain
+    const int nblk = P.node_blk[P.nfree];
+    std::vector<int> cnt(nblk + 1, 0);
+    auto block_of = [&](int r, int s) {
+        const int *b = P.bcol.data() + P.node_blk[r], *e = P.bcol.data() + P.node_blk[r + 1];
+        return P.node_blk[r] + (int)(std::lower_bound(b, e, s) - b);
+    };
+    for (int k = 0; k < c->nd; ++k)
+        for (int a = H.mptr[k]; a < H.mptr[k + 1]; ++a) {
+            const int r = c->resort[H.member[a]];
+            if (r < 0) continue;
+            for (int b = H.mptr[k]; b < H.mptr[k + 1]; ++b) {
+                const int s = c->resort[H.member[b]];
+                if (s >= 0) cnt[block_of(r, s) + 1]++;
+            }
+        }
+    for (int b = 0; b < nblk; ++b) cnt[b + 1] += cnt[b];
+    if (cnt[nblk] < 0 || (long long)cnt[nblk] >= (1LL << 31)) return fail(OFEA_ERR_LIMIT, "too many contributors");
+    std::vector<int2> contrib(cnt[nblk]);
+    std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+    for (int k = 0; k < c->nd; ++k)  // ascending domain: the reference's summation order
+        for (int a = H.mptr[k]; a < H.mptr[k + 1]; ++a) {
+            const int r = c->resort[H.member[a]];
+            if (r < 0) continue;
+            for (int b = H.mptr[k]; b < H.mptr[k + 1]; ++b) {
+                const int s = c->resort[H.member[b]];
+                if (s >= 0) contrib[fillp[block_of(r, s)]++] = make_int2(a, b);
+            }
+        }
+    std::vector<int> blk_r(nblk);
+    for (int r = 0; r < P.nfree; ++r)
+        for (int b = P.node_blk[r]; b < P.node_blk[r + 1]; ++b) blk_r[b] = r;
+    // device: replace the context's pattern (the FEM plans stay but no longer match)
+    hipStreamSynchronize(c->stream);
+    solver_free(c);
+    batch_free(c);
+    learned_free(c);
+    ca_free(c);
+    sfem_free(c);
+    c->P = std::move(P);
+    c->sfem = true;
+    c->assembled = false;
+    if ((rc = dev_upload(c->d_col_ptr, c->P.col_ptr.data(), c->P.col_ptr.size()))) return rc;
+    if ((rc = dev_upload(c->d_node_blk, c->P.node_blk.data(), c->P.node_blk.size()))) return rc;
+    if ((rc = dev_upload(c->d_bcol, c->P.bcol.data(), std::max<size_t>(c->P.bcol.size(), 1)))) return rc;
+    if ((rc = dev_upload(c->d_diag_k, c->P.diag_k.data(), std::max<size_t>(c->P.diag_k.size(), 1)))) return rc;
+    if ((rc = dev_upload<double>(c->d_val, nullptr, (size_t)std::max(c->P.nnz, 1LL)))) return rc;
+    if ((rc = dev_upload<double>(c->d_rhs, nullptr, (size_t)std::max(c->P.n, 1)))) return rc;
+    if ((rc = dev_upload<double>(c->d_x, nullptr, (size_t)std::max(c->P.n, 1)))) return rc;
+    c->rhs_set = false;
+    auto *w = new SfemWs();
+    c->sfem_ws = w;
+    w->nd = c->nd;
+    w->ndom = c->nd;
+    w->nrows = (long long)H.member.size();
+    w->ncontrib = (long long)contrib.size();
+    if ((rc = dev_upload(w->conn, H.conn.data(), H.conn.size()))) return rc;
+    if ((rc = dev_upload(w->fflag, H.fflag.data(), H.fflag.size()))) return rc;
+    if ((rc = dev_upload(w->dom_mptr, H.mptr.data(), H.mptr.size()))) return rc;
+    if ((rc = dev_upload(w->dom_member, H.member.data(), std::max<size_t>(H.member.size(), 1)))) return rc;
+    if ((rc = dev_upload(w->dom_eptr, H.eptr.data(), H.eptr.size()))) return rc;
+    if ((rc = dev_upload(w->dom_elem, H.elem.data(), std::max<size_t>(H.elem.size(), 1)))) return rc;
+    if ((rc = dev_upload(w->vol, H.vol.data(), H.vol.size()))) return rc;
+    if ((rc = dev_upload<double>(w->rows, nullptr, 16 * (size_t)std::max(w->nrows, 1LL)))) return rc;
+    if ((rc = dev_upload(w->cptr, cnt.data(), cnt.size()))) return rc;
+    if ((rc = dev_upload(w->contrib, contrib.data(), std::max<size_t>(contrib.size(), 1)))) return rc;
+    int *d_blk_r = nullptr;
+    if ((rc = dev_upload(d_blk_r, blk_r.data(), std::max<size_t>(blk_r.size(), 1)))) return rc;
+    w->blk_r = d_blk_r;
+    return OFEA_OK;
+}
+
+int sfem_assemble(ofea_ctx *c, double E, double nu, double *nz_val)
+{
+    SfemWs *w = (SfemWs *)c->sfem_ws;
+    if (!c->sfem || !w) return fail(OFEA_ERR_STATE, "ofea_sfem_setup first");
+    if (!(nu > -1.0 && nu < 0.5)) return fail(OFEA_ERR_ARG, "Poisson ratio must lie in (-1, 0.5)");
+    double lam, mu;
+    lame<double>(E, nu, lam, mu);
+    if (w->ndom > 0)
+        hipLaunchKernelGGL(k_sfem_grad, dim3(w->ndom), dim3(SF_WG), 0, c->stream, w->ndom, c->d_xyz, w->conn, w->fflag,
+                           w->dom_mptr, w->dom_member, w->dom_eptr, w->dom_elem, w->vol, w->rows);
+    OFEA_HIP(hipGetLastError());
+    const int nblk = c->P.node_blk[c->P.nfree];
+    if (nblk > 0)
+        hipLaunchKernelGGL(k_sfem_gather, dim3((nblk + 255) / 256), dim3(256), 0, c->stream, nblk, w->blk_r,
+                           c->d_node_blk, c->d_col_ptr, w->cptr, w->contrib, w->rows, lam, mu, c->d_val);
+    OFEA_HIP(hipGetLastError());
+    c->assembled = true;
+    c->kver++;
+    c->asm_E = E;
+    c->asm_nu = nu;
+    solver_invalidate(c);
+    if (nz_val) {
+        OFEA_HIP(hipMemcpyAsync(nz_val, c->d_val, sizeof(double) * c->P.nnz, hipMemcpyDeviceToHost, c->stream));
+        OFEA_HIP(hipStreamSynchronize(c->stream));
+    }
+    return OFEA_OK;
+}
+
+}  // namespace ofea

@@ -13,6 +13,7 @@ Names follow the reference so that a driver written against it reads the same:
   CAE_process::CA_ReAnalysis        -> CA_ReAnalysis()    (src/cae.cpp:317-376)
   simulation_post::reset_ca_displacement -> reset_ca_displacement() (src/post_process.cpp:42-76)
   CAE_process::implict_analysis     -> implict_analysis() (src/cae.cpp:198-301)
+  CAE_process::implict_SFEManalysis -> implict_SFEManalysis() (src/cae.cpp:379-416)
 All numerical work runs in the native library; this module only marshals arrays.
 """
 from __future__ import annotations
@@ -349,6 +350,20 @@ class Context:
         reduced = (red[:k * k].reshape(k, k), red[k * k:k * k + k], red[k * k + k:k * k + 2 * k])
         return u, k, it.value, reduced, rc == A.OFEA_OK
 
+    # ---- node-based smoothed FEM
+    def sfem_setup(self):
+        """Smoothing domains and the smoothed pattern (replaces the context's pattern)."""
+        check(lib().ofea_sfem_setup(self._h))
+        n = C.c_int()
+        nnz = C.c_longlong()
+        check(lib().ofea_pattern(self._h, C.byref(n), C.byref(nnz), None, None))
+        self.n, self.nnz = n.value, nnz.value
+
+    def sfem_assemble(self, E: float, nu: float, copy: bool = True):
+        out = np.zeros(self.nnz) if copy else None
+        check(lib().ofea_sfem_assemble(self._h, E, nu, ptr(out)))
+        return out
+
     # ---- post-processing
     def post_nodal(self, E, nu, u_full, quantity=A.POST_STRAIN, elem=False, copy=True):
         """(node fields 7 x nd, element fields 7 x ne or None) from the full
@@ -490,6 +505,44 @@ def implict_analysis(deck: Deck, result_path: str | None = None, solver_type: st
         if result_path:
             export_dis_2_vtk(result_path, deck, u_full, 1.0, fields)
         return u_full.reshape(-1, 3), fields
+    finally:
+        if own:
+            ctx.close()
+
+
+def implict_SFEManalysis(deck: Deck, result_path: str | None = None, solver_type: str = "GPU_PCG_TWO_LEVEL",
+                         ctx: Context | None = None, E: float | None = None, nu: float | None = None,
+                         rtol: float = 1e-10):
+    """CAE_process::implict_SFEManalysis (src/cae.cpp:379-416): the
+    node-based smoothed C3D4 model of the deck (SFEM3D::build,
+    SFEM_build_CSR, SFEM_fill_CSR_sparse_mat), the GPU solve (the reference
+    uses "SuperLU_func"), reset_displacement and the VTK file (u fields only,
+    as the reference writes for this analysis).  Clamp and loads as in
+    implict_analysis.  Returns u_full (nd x 3)."""
+    if deck.ext_error:
+        raise ValueError(f"deck extensions unreadable: {deck.ext_error}")
+    fix = np.union1d(i32(deck.fix_set), i32(deck.bc_fixed)).astype(np.int32)
+    deck.resort, nfree = build_free_index(deck.nd, fix)
+    F = np.zeros(3 * nfree)
+    free = deck.resort >= 0
+    F.reshape(-1, 3)[deck.resort[free]] = deck.f_nodal[free]
+    deck.F = F
+    E = deck.E if E is None else E
+    nu = deck.nu if nu is None else nu
+    own = ctx is None
+    ctx = ctx or Context()
+    try:
+        ctx.set_mesh(deck.xyz, deck.conn, deck.etype, deck.resort)
+        ctx.sfem_setup()
+        ctx.sfem_assemble(E, nu, copy=False)
+        x, it, rel, ok = ctx.solve(F, SOLVERS[solver_type], rtol, 200000)
+        if not ok:
+            raise RuntimeError(f"solver did not converge: {it} iterations, relres {rel}")
+        deck.single_dis_vec_ = x
+        u_full = reset_displacement(deck.resort, x)
+        if result_path:
+            export_dis_2_vtk(result_path, deck, u_full, 1.0, None)
+        return u_full.reshape(-1, 3)
     finally:
         if own:
             ctx.close()

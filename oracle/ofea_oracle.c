@@ -842,3 +842,382 @@ OR_API void or_post_node(int nd, int ne, const int *conn, const int *etype, cons
     }
     free(cnt);
 }
+
+/* ---------------------------------------------------------------- SFEM (node-based smoothed C3D4)
+ * Restates SFEM3D::eleData/nodeData (src/SFEM3D.cpp:31-134),
+ * SFEM_C3D4::shapeGradient/shapeAtSurf/T4G/build_ele_stiff_mat
+ * (src/Elements/SFEM_C3D4.cpp:53-412) and SFEM_build_CSR /
+ * SFEM_fill_CSR_sparse_mat (src/assemble.cpp:291-414).  Facet integrals use
+ * the reference's planar local coordinates (edge lengths and angles) and the
+ * 2x2 Gauss rule of the bilinear quad; the stiffness uses its six-term loop. */
+typedef struct {
+    int nd, ne;
+    int *conn;            /* ne x 4, 0-based */
+    unsigned char *flag;  /* ne: bit f = face f (123, 124, 234, 134) on the surface */
+    int *eptr, *elem;     /* node -> tets (ascending) */
+    int *mptr, *member;   /* node -> ring (ascending node ids) */
+    double *vol;          /* node cell volume */
+} sf_mesh;
+
+typedef struct { int v[3]; int e, f; } sf_face;
+
+static int sf_face_cmp(const void *a, const void *b)
+{
+    const sf_face *x = (const sf_face *)a, *y = (const sf_face *)b;
+    for (int i = 0; i < 3; ++i)
+        if (x->v[i] != y->v[i]) return x->v[i] < y->v[i] ? -1 : 1;
+    return 0;
+}
+
+static void sf_build(int nd, int ne, const int *conn8, const double *xyz, sf_mesh *M)
+{
+    static const int FACES[4][3] = {{0, 1, 2}, {0, 1, 3}, {1, 2, 3}, {0, 2, 3}};
+    M->nd = nd;
+    M->ne = ne;
+    M->conn = (int *)malloc(sizeof(int) * 4 * (size_t)ne);
+    for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < 4; ++a) M->conn[4 * e + a] = conn8[8 * e + a] - 1;
+    sf_face *F = (sf_face *)malloc(sizeof(sf_face) * 4 * (size_t)ne);
+    for (int e = 0; e < ne; ++e)
+        for (int f = 0; f < 4; ++f) {
+            sf_face *q = F + 4 * e + f;
+            for (int i = 0; i < 3; ++i) q->v[i] = M->conn[4 * e + FACES[f][i]];
+            qsort(q->v, 3, sizeof(int), cmp_int);
+            q->e = e;
+            q->f = f;
+        }
+    qsort(F, 4 * (size_t)ne, sizeof(sf_face), sf_face_cmp);
+    M->flag = (unsigned char *)calloc((size_t)ne, 1);
+    for (long long i = 0; i < 4LL * ne;) {
+        long long j = i;
+        while (j < 4LL * ne && sf_face_cmp(F + i, F + j) == 0) ++j;
+        if ((j - i) % 2 == 1) /* met an odd number of times: stays in the surface set */
+            for (long long t = i; t < j; ++t) M->flag[F[t].e] |= (unsigned char)(1 << F[t].f);
+        i = j;
+    }
+    free(F);
+    M->eptr = (int *)calloc((size_t)nd + 1, sizeof(int));
+    for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < 4; ++a) M->eptr[M->conn[4 * e + a] + 1]++;
+    for (int i = 0; i < nd; ++i) M->eptr[i + 1] += M->eptr[i];
+    M->elem = (int *)malloc(sizeof(int) * (size_t)(M->eptr[nd] + 1));
+    int *fill = (int *)malloc(sizeof(int) * (size_t)nd);
+    for (int i = 0; i < nd; ++i) fill[i] = M->eptr[i];
+    for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < 4; ++a) M->elem[fill[M->conn[4 * e + a]]++] = e;
+    free(fill);
+    M->vol = (double *)calloc((size_t)nd, sizeof(double));
+    M->mptr = (int *)calloc((size_t)nd + 1, sizeof(int));
+    int cap = 16 * nd + 16, used = 0;
+    M->member = (int *)malloc(sizeof(int) * (size_t)cap);
+    int *tmp = (int *)malloc(sizeof(int) * 4096);
+    for (int k = 0; k < nd; ++k) {
+        int nt = 0;
+        for (int t = M->eptr[k]; t < M->eptr[k + 1]; ++t) {
+            int e = M->elem[t];
+            const double *p[4];
+            for (int a = 0; a < 4; ++a) p[a] = xyz + 3 * M->conn[4 * e + a];
+            double u[3], v[3], w[3];
+            for (int c = 0; c < 3; ++c) {
+                u[c] = p[0][c] - p[3][c];
+                v[c] = p[1][c] - p[3][c];
+                w[c] = p[2][c] - p[3][c];
+            }
+            double val = u[0] * v[1] * w[2] + u[1] * v[2] * w[0] + u[2] * v[0] * w[1] - u[0] * v[2] * w[1] -
+                         u[1] * v[0] * w[2] - u[2] * v[1] * w[0];
+            M->vol[k] += fabs(val / 6.0) / 4.0;
+            for (int a = 0; a < 4 && nt < 4096; ++a) tmp[nt++] = M->conn[4 * e + a];
+        }
+        qsort(tmp, nt, sizeof(int), cmp_int);
+        int nu = 0;
+        for (int i = 0; i < nt; ++i)
+            if (nu == 0 || tmp[i] != tmp[nu - 1]) tmp[nu++] = tmp[i];
+        if (used + nu > cap) {
+            cap = 2 * (used + nu);
+            M->member = (int *)realloc(M->member, sizeof(int) * (size_t)cap);
+        }
+        memcpy(M->member + used, tmp, sizeof(int) * (size_t)nu);
+        used += nu;
+        M->mptr[k + 1] = used;
+    }
+    free(tmp);
+}
+
+static void sf_free(sf_mesh *M)
+{
+    free(M->conn);
+    free(M->flag);
+    free(M->eptr);
+    free(M->elem);
+    free(M->mptr);
+    free(M->member);
+    free(M->vol);
+}
+
+static int sf_index(const sf_mesh *M, int k, int node)
+{
+    for (int a = M->mptr[k]; a < M->mptr[k + 1]; ++a)
+        if (M->member[a] == node) return a - M->mptr[k];
+    return -1;
+}
+
+/* one quad facet (corners L[4]) of domain k's cell inside a tet with rotated
+ * nodes nd4 and gradients t4g (4 x 3): adds into sg (members x 3) and gg
+ * (members x 9); n away from ref. */
+static void sf_facet(const sf_mesh *M, int k, const double L[4][3], const double *ref, const double phi_c[4][4],
+                     const int nd4[4], const double t4g[4][3], double *sg, double *gg)
+{
+    double v12[3], v13[3], v14[3], nrm[3];
+    for (int c = 0; c < 3; ++c) {
+        v12[c] = L[1][c] - L[0][c];
+        v13[c] = L[2][c] - L[0][c];
+        v14[c] = L[3][c] - L[0][c];
+    }
+    nrm[0] = v12[1] * v13[2] - v12[2] * v13[1];
+    nrm[1] = v12[2] * v13[0] - v12[0] * v13[2];
+    nrm[2] = v12[0] * v13[1] - v12[1] * v13[0];
+    double nl = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    double vec = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        nrm[c] /= nl;
+        vec += ((L[0][c] + L[1][c] + L[2][c] + L[3][c]) / 4.0 - ref[c]) * nrm[c];
+    }
+    if (vec < 0.0)
+        for (int c = 0; c < 3; ++c) nrm[c] = -nrm[c];
+    /* planar local coordinates of the corners (src/Elements/SFEM_C3D4.cpp:352-369) */
+    double d12 = sqrt(v12[0] * v12[0] + v12[1] * v12[1] + v12[2] * v12[2]);
+    double d13 = sqrt(v13[0] * v13[0] + v13[1] * v13[1] + v13[2] * v13[2]);
+    double d14 = sqrt(v14[0] * v14[0] + v14[1] * v14[1] + v14[2] * v14[2]);
+    double a13 = acos((v12[0] * v13[0] + v12[1] * v13[1] + v12[2] * v13[2]) / d12 / d13);
+    double a14 = acos((v12[0] * v14[0] + v12[1] * v14[1] + v12[2] * v14[2]) / d12 / d14);
+    double ll[4][2] = {{0, 0}, {d12, 0}, {d13 * cos(a13), d13 * sin(a13)}, {d14 * cos(a14), d14 * sin(a14)}};
+    const double q = 1.0 / sqrt(3.0), xs[4] = {-1, 1, 1, -1}, es[4] = {-1, -1, 1, 1};
+    const double vol = M->vol[k];
+    for (int g = 0; g < 4; ++g) {
+        double xi = xs[g] * q, eta = es[g] * q, J[2][2] = {{0, 0}, {0, 0}}, phi[4] = {0, 0, 0, 0};
+        for (int a = 0; a < 4; ++a) {
+            double H = 0.25 * (1 + xs[a] * xi) * (1 + es[a] * eta);
+            double Hr = 0.25 * xs[a] * (1 + es[a] * eta), Hs = 0.25 * es[a] * (1 + xs[a] * xi);
+            J[0][0] += Hr * ll[a][0];
+            J[0][1] += Hr * ll[a][1];
+            J[1][0] += Hs * ll[a][0];
+            J[1][1] += Hs * ll[a][1];
+            for (int j = 0; j < 4; ++j) phi[j] += H * phi_c[a][j];
+        }
+        double detJ = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        for (int j = 0; j < 4; ++j) {
+            int idx = sf_index(M, k, nd4[j]);
+            for (int c = 0; c < 3; ++c) sg[3 * idx + c] += phi[j] * nrm[c] * detJ / vol;
+            for (int a = 0; a < 3; ++a)
+                for (int c = 0; c < 3; ++c) gg[9 * idx + 3 * a + c] += t4g[j][a] * nrm[c] * detJ / vol;
+        }
+    }
+}
+
+/* G (members x 3) and GG (members x 9, row-major [a][c], scaled by the
+ * equivalent radius) of domain k */
+static void sf_grads(const sf_mesh *M, const double *xyz, int k, double *sg, double *gg)
+{
+    int nm = M->mptr[k + 1] - M->mptr[k];
+    memset(sg, 0, sizeof(double) * 3 * (size_t)nm);
+    memset(gg, 0, sizeof(double) * 9 * (size_t)nm);
+    const double t = 1.0 / 3.0, h = 0.5, f = 0.25;
+    for (int te = M->eptr[k]; te < M->eptr[k + 1]; ++te) {
+        int e = M->elem[te], p = 0;
+        const int *cn = M->conn + 4 * e;
+        for (int a = 0; a < 4; ++a)
+            if (cn[a] == k) p = a;
+        int nd4[4];
+        double x[4][3], X[12], g[12];
+        for (int j = 0; j < 4; ++j) {
+            nd4[j] = cn[(p + j) % 4];
+            for (int c = 0; c < 3; ++c) x[j][c] = X[3 * j + c] = xyz[3 * nd4[j] + c];
+        }
+        static const double dNt[12] = {-1, 1, 0, 0, -1, 0, 1, 0, -1, 0, 0, 1};
+        grads(4, dNt, X, g);
+        double t4g[4][3];
+        for (int j = 0; j < 4; ++j)
+            for (int c = 0; c < 3; ++c) t4g[j][c] = g[c * 4 + j];
+        double m1[3], m2[3], m3[3], c1[3], c2[3], c3[3], cc[3];
+        for (int c = 0; c < 3; ++c) {
+            m1[c] = (x[0][c] + x[1][c]) / 2;
+            m2[c] = (x[0][c] + x[2][c]) / 2;
+            m3[c] = (x[0][c] + x[3][c]) / 2;
+            c1[c] = (x[0][c] + x[1][c] + x[2][c]) / 3;
+            c2[c] = (x[0][c] + x[2][c] + x[3][c]) / 3;
+            c3[c] = (x[0][c] + x[1][c] + x[3][c]) / 3;
+            cc[c] = (x[0][c] + x[1][c] + x[2][c] + x[3][c]) / 4;
+        }
+        double L[4][3];
+        const double *pts1[4] = {c3, m1, c1, cc}, *pts2[4] = {c1, m2, c2, cc}, *pts3[4] = {c2, m3, c3, cc};
+        const double ph1[4][4] = {{t, t, 0, t}, {h, h, 0, 0}, {t, t, t, 0}, {f, f, f, f}};
+        const double ph2[4][4] = {{t, t, t, 0}, {h, 0, h, 0}, {t, 0, t, t}, {f, f, f, f}};
+        const double ph3[4][4] = {{t, 0, t, t}, {h, 0, 0, h}, {t, t, 0, t}, {f, f, f, f}};
+        const double *const *pts[3] = {pts1, pts2, pts3};
+        const double (*phs[3])[4] = {ph1, ph2, ph3};
+        for (int s = 0; s < 3; ++s) {
+            for (int a = 0; a < 4; ++a)
+                for (int c = 0; c < 3; ++c) L[a][c] = pts[s][a][c];
+            sf_facet(M, k, L, x[0], phs[s], nd4, t4g, sg, gg);
+        }
+        /* surface corner quads: faces opposite rotated corners 1, 3, 2 */
+        static const int OPP_FACE[4] = {2, 3, 1, 0}; /* face index opposite local node p */
+        const double *bq1[4] = {m2, c2, m3, x[0]}, *bq2[4] = {m1, c1, m2, x[0]}, *bq3[4] = {m3, c3, m1, x[0]};
+        const double bp1[4][4] = {{h, 0, h, 0}, {t, 0, t, t}, {h, 0, 0, h}, {1, 0, 0, 0}};
+        const double bp2[4][4] = {{h, h, 0, 0}, {t, t, t, 0}, {h, 0, h, 0}, {1, 0, 0, 0}};
+        const double bp3[4][4] = {{h, 0, 0, h}, {t, t, 0, t}, {h, h, 0, 0}, {1, 0, 0, 0}};
+        const double *const *bqs[3] = {bq1, bq2, bq3};
+        const double (*bps[3])[4] = {bp1, bp2, bp3};
+        const int opp[3] = {(p + 1) % 4, (p + 3) % 4, (p + 2) % 4};
+        for (int s = 0; s < 3; ++s) {
+            if (!(M->flag[e] >> OPP_FACE[opp[s]] & 1)) continue;
+            for (int a = 0; a < 4; ++a)
+                for (int c = 0; c < 3; ++c) L[a][c] = bqs[s][a][c];
+            sf_facet(M, k, L, cc, bps[s], nd4, t4g, sg, gg);
+        }
+    }
+    double l = cbrt(3.0 * M->vol[k] / 4.0 / 3.14159265358979323846);
+    for (int i = 0; i < 9 * nm; ++i) gg[i] *= l;
+}
+
+/* SFEM pattern: returns nnz; fills col_ptr (n+1) and row_idx when not NULL */
+OR_API long long or_sfem_build_csc(int nd, int ne, const int *conn, const double *xyz, const int *resort, int n,
+                                   int *col_ptr, int *row_idx)
+{
+    sf_mesh M;
+    sf_build(nd, ne, conn, xyz, &M);
+    /* pseudo-elements = rings, through the generic dense-set builder */
+    char *mark = (char *)calloc((size_t)n, 1);
+    int *list = (int *)malloc(sizeof(int) * (size_t)n);
+    long long nnz = 0;
+    int *nbr_ptr = (int *)calloc((size_t)nd + 1, sizeof(int));
+    /* column j (node r): union over domains k containing r of ring(k) */
+    for (int node = 0; node < nd; ++node) {
+        int r = resort[node];
+        if (r < 0) continue;
+        for (int c = 0; c < 3; ++c) {
+            int cnt = 0;
+            for (int a = M.mptr[node]; a < M.mptr[node + 1]; ++a) {
+                int k = M.member[a]; /* r is in ring(k) iff k is in ring(r) */
+                for (int b = M.mptr[k]; b < M.mptr[k + 1]; ++b) {
+                    int s = resort[M.member[b]];
+                    if (s < 0) continue;
+                    for (int d = 0; d < 3; ++d)
+                        if (!mark[3 * s + d]) {
+                            mark[3 * s + d] = 1;
+                            list[cnt++] = 3 * s + d;
+                        }
+                }
+            }
+            qsort(list, cnt, sizeof(int), cmp_int);
+            int j = 3 * r + c;
+            if (col_ptr) col_ptr[j] = (int)nnz;
+            for (int i = 0; i < cnt; ++i) {
+                if (row_idx) row_idx[nnz + i] = list[i];
+                mark[list[i]] = 0;
+            }
+            nnz += cnt;
+        }
+    }
+    if (col_ptr) col_ptr[n] = (int)nnz;
+    free(nbr_ptr);
+    free(mark);
+    free(list);
+    sf_free(&M);
+    return nnz;
+}
+
+/* SFEM values: per domain the reference's six-term stiffness, scattered with
+ * the linear row search of src/assemble.cpp:391-411, domains ascending */
+OR_API void or_sfem_fill(int nd, int ne, const int *conn, const double *xyz, const int *resort, double E, double nu,
+                         const int *col_ptr, const int *row_idx, long long nnz, double *val)
+{
+    sf_mesh M;
+    sf_build(nd, ne, conn, xyz, &M);
+    for (long long i = 0; i < nnz; ++i) val[i] = 0.0;
+    double D[36];
+    or_dmat(E, nu, D);
+    for (int k = 0; k < nd; ++k) {
+        int nm = M.mptr[k + 1] - M.mptr[k];
+        if (nm == 0) continue;
+        double *sg = (double *)malloc(sizeof(double) * 3 * nm), *gg = (double *)malloc(sizeof(double) * 9 * nm);
+        double *ke = (double *)calloc((size_t)9 * nm * nm, sizeof(double));
+        sf_grads(&M, xyz, k, sg, gg);
+        double dv = M.vol[k] / 6.0;
+        int n3 = 3 * nm;
+        double *B = (double *)malloc(sizeof(double) * 6 * n3);
+        for (int m = 0; m < 6; ++m) {
+            double sgn = m <= 2 ? -1.0 : 1.0;
+            int in = m <= 2 ? m : m - 3;
+            double gi[3];
+            double *g = (double *)malloc(sizeof(double) * 3 * nm);
+            for (int i = 0; i < nm; ++i) {
+                for (int a = 0; a < 3; ++a) gi[a] = sg[3 * i + a] + sgn * gg[9 * i + 3 * a + in];
+                for (int a = 0; a < 3; ++a) g[a * nm + i] = gi[a];
+            }
+            bmat(nm, g, B);
+            /* ke += dv B^T D B */
+            for (int r = 0; r < n3; ++r)
+                for (int s = 0; s < n3; ++s) {
+                    double acc = 0.0;
+                    for (int x = 0; x < 6; ++x) {
+                        double db = 0.0;
+                        for (int y = 0; y < 6; ++y) db += D[6 * x + y] * B[y * n3 + s];
+                        acc += B[x * n3 + r] * db;
+                    }
+                    ke[(size_t)r * n3 + s] += dv * acc;
+                }
+            free(g);
+        }
+        int *dofs = (int *)malloc(sizeof(int) * n3);
+        for (int i = 0; i < nm; ++i) {
+            int rr = resort[M.member[M.mptr[k] + i]];
+            for (int c = 0; c < 3; ++c) dofs[3 * i + c] = 3 * rr + c;
+        }
+        for (int mm = 0; mm < n3; ++mm) {
+            int jj = dofs[mm];
+            if (jj < 0) continue;
+            for (int q = 0; q < n3; ++q) {
+                int ii = dofs[q];
+                if (ii < 0) continue;
+                int t = col_ptr[jj];
+                while (row_idx[t] < ii) t++;
+                val[t] += ke[(size_t)mm * n3 + q];
+            }
+        }
+        free(dofs);
+        free(B);
+        free(ke);
+        free(sg);
+        free(gg);
+    }
+    sf_free(&M);
+}
+
+/* G and GG of every domain, rows of 12 (G, then GG row-major) per (domain, member)
+ * in domain order; returns the number of rows (call with NULL outputs to size) */
+OR_API long long or_sfem_grads(int nd, int ne, const int *conn, const double *xyz, int *mptr, int *member, double *rows)
+{
+    sf_mesh M;
+    sf_build(nd, ne, conn, xyz, &M);
+    long long total = M.mptr[nd];
+    if (mptr) memcpy(mptr, M.mptr, sizeof(int) * (size_t)(nd + 1));
+    if (member) memcpy(member, M.member, sizeof(int) * (size_t)total);
+    if (rows) {
+        for (int k = 0; k < nd; ++k) {
+            int nm = M.mptr[k + 1] - M.mptr[k];
+            if (nm == 0) continue;
+            double *sg = (double *)malloc(sizeof(double) * 3 * nm), *gg = (double *)malloc(sizeof(double) * 9 * nm);
+            sf_grads(&M, xyz, k, sg, gg);
+            for (int i = 0; i < nm; ++i) {
+                double *row = rows + 12 * (size_t)(M.mptr[k] + i);
+                memcpy(row, sg + 3 * i, 3 * sizeof(double));
+                memcpy(row + 3, gg + 9 * i, 9 * sizeof(double));
+            }
+            free(sg);
+            free(gg);
+        }
+    }
+    sf_free(&M);
+    return total;
+}

@@ -55,6 +55,11 @@ def lib():
         L.or_ca_fill.restype = _i
         L.or_post_elem.argtypes = [_i, _pi, _pi, _pd, _pd, _d, _d, _vp, _i, _pd]
         L.or_post_node.argtypes = [_i, _i, _pi, _pi, _pd, _pd]
+        L.or_sfem_build_csc.argtypes = [_i, _i, _pi, _pd, _pi, _i, _vp, _vp]
+        L.or_sfem_build_csc.restype = _ll
+        L.or_sfem_fill.argtypes = [_i, _i, _pi, _pd, _pi, _d, _d, _pi, _pi, _ll, _pd]
+        L.or_sfem_grads.argtypes = [_i, _i, _pi, _pd, _vp, _vp, _vp]
+        L.or_sfem_grads.restype = _ll
         _LIB = L
     return _LIB
 
@@ -277,3 +282,36 @@ def post_node(conn, etype, nd, ev):
     nv = np.zeros((7, nd))
     lib().or_post_node(nd, len(etype), _i32(conn).reshape(-1, 8), _i32(etype), _f64(ev), nv)
     return nv
+
+
+def sfem_build_csc(conn, xyz, resort):
+    """Smoothed-FEM CSC pattern (SFEM_build_CSR, src/assemble.cpp:291-350)."""
+    conn, xyz, resort = _i32(conn).reshape(-1, 8), _f64(xyz).ravel(), _i32(resort)
+    nd, ne = len(resort), len(conn)
+    n = 3 * int(resort.max() + 1)
+    nnz = lib().or_sfem_build_csc(nd, ne, conn, xyz, resort, n, None, None)
+    cp = np.zeros(n + 1, dtype=np.int32)
+    ri = np.zeros(nnz, dtype=np.int32)
+    lib().or_sfem_build_csc(nd, ne, conn, xyz, resort, n, cp.ctypes.data, ri.ctypes.data)
+    return cp, ri
+
+
+def sfem_fill(conn, xyz, resort, E, nu, col_ptr, row_idx):
+    """Smoothed-FEM values (SFEM_fill_CSR_sparse_mat, six-term domain stiffness)."""
+    val = np.zeros(len(row_idx))
+    conn = _i32(conn).reshape(-1, 8)
+    lib().or_sfem_fill(len(resort), len(conn), conn, _f64(xyz).ravel(), _i32(resort), E, nu, _i32(col_ptr),
+                       _i32(row_idx), len(row_idx), val)
+    return val
+
+
+def sfem_grads(conn, xyz, nd):
+    """(mptr, members, rows): per (domain, member) G (3) and GG (3 x 3, scaled)."""
+    conn = _i32(conn).reshape(-1, 8)
+    xyzf = _f64(xyz).ravel()
+    tot = lib().or_sfem_grads(nd, len(conn), conn, xyzf, None, None, None)
+    mptr = np.zeros(nd + 1, dtype=np.int32)
+    mem = np.zeros(tot, dtype=np.int32)
+    rows = np.zeros((tot, 12))
+    lib().or_sfem_grads(nd, len(conn), conn, xyzf, mptr.ctypes.data, mem.ctypes.data, rows.ctypes.data)
+    return mptr, mem, rows
