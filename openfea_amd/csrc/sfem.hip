@@ -18,14 +18,16 @@
 // (s = -1 for m < 3, +1 otherwise, in = m mod 3), expands to
 //   K_k = V_k K(G, G) + (V_k / 3) sum_in K(a_in, a_in),  a_in = GG[:, in],
 //   K(x, y)_ij[c][d] = lambda x_i[c] y_j[d] + mu x_i[d] y_j[c] + mu delta_cd x_i . y_j,
-// the form used here.  Facet integrals: each facet is a planar convex quad,
-// bilinearly mapped, integrated by 2x2 Gauss (exact for N_i |x_xi x x_eta|).
+// the form used here (with M the sum of the outer products of the scaled
+// rows, K = lambda M + mu M^T + mu tr(M) I).  Facet integrals: each facet is
+// a planar convex quad; N_i is linear on it, so two triangles' vector areas
+// times mean corner values integrate it exactly.
 //
 // Kernels:
-//   k_sfem_grad   one workgroup per domain: lane (element, rotated corner j)
-//                 integrates the element's facets of the cell for corner j;
-//                 lane = member then sums them in ascending element order
-//                 and writes a 16-double row {G, a_0, a_1, a_2, V_k, 0..}
+//   k_sfem_grad   one workgroup per chunk of domains: lane = (domain,
+//                 element) integrates the element's facets of the cell; lane
+//                 = (domain, member) then sums them in ascending element
+//                 order and writes a 12-double row {G sqrt(V), a_in sqrt(V/3)}
 //   k_sfem_gather one lane per 3x3 block (r, s) of the pattern sums its
 //                 contributors (domains containing both r and s, ascending
 //                 domain: the reference's summation order) and writes the 9
@@ -41,9 +43,9 @@
 namespace ofea {
 namespace {
 
-constexpr int SF_WG = 128;
-constexpr int SF_MAXE = 96;   // tets around one node
-constexpr int SF_MAXM = 128;  // nodes in one node's ring (<= SF_WG)
+constexpr int SF_PAIRS = 256;  // (domain, element) pairs and members per workgroup
+constexpr int SF_MAXE = 96;    // tets around one node
+constexpr int SF_MAXM = 128;   // nodes in one node's ring
 
 struct SfemWs {
     int nd = 0, ndom = 0;
@@ -55,7 +57,11 @@ struct SfemWs {
     int *dom_eptr = nullptr;        // ndom + 1 into elements
     int *dom_elem = nullptr;        // incident elements (ascending)
     double *vol = nullptr;          // ndom: V_k
-    double *rows = nullptr;         // nrows x 16
+    double *rows = nullptr;         // nrows x 12 (scaled G and a_0..a_2)
+    int *chunk_dom = nullptr;       // nchunks + 1
+    int nchunks = 0;
+    int *mdom = nullptr;            // domain of each member row
+    int *pdom = nullptr;            // domain of each (domain, element) pair
     int *cptr = nullptr;            // blocks + 1
     int2 *contrib = nullptr;        // (row of r, row of s) per contributor
     long long ncontrib = 0;
@@ -70,43 +76,34 @@ __device__ inline void d_cross(const double a[3], const double b[3], double o[3]
 }
 
 // Adds int_facet N_j n dA (gsum) and int_facet n dA (nsum) of one planar quad
-// facet with corners P[4]; N_j at the corners is phi[c][j]; n is oriented away
-// from the point ref.
+// facet with corners P[4] (cyclic order); N_j at the corners is phi[c][j]; n
+// is oriented away from the point ref.  N_j is linear, so its integral over
+// each triangle of the split (0, 1, 2) + (0, 2, 3) is the triangle's vector
+// area times the mean corner value: exact, as the reference's 2x2 Gauss rule
+// on the bilinear map is for a planar quad.
 __device__ inline void facet(const double P[4][3], const double phi[4][4], const double ref[3], double gsum[4][3],
                              double nsum[3])
 {
-    double v12[3], v13[3], nrm[3];
+    double e1[3], e2[3], e3[3], S1[3], S2[3];
     for (int c = 0; c < 3; ++c) {
-        v12[c] = P[1][c] - P[0][c];
-        v13[c] = P[2][c] - P[0][c];
+        e1[c] = P[1][c] - P[0][c];
+        e2[c] = P[2][c] - P[0][c];
+        e3[c] = P[3][c] - P[0][c];
     }
-    d_cross(v12, v13, nrm);
-    const double nl = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
-    double cen[3];
-    for (int c = 0; c < 3; ++c) cen[c] = 0.25 * (P[0][c] + P[1][c] + P[2][c] + P[3][c]) - ref[c];
-    const double sg = (cen[0] * nrm[0] + cen[1] * nrm[1] + cen[2] * nrm[2]) < 0.0 ? -1.0 : 1.0;
-    for (int c = 0; c < 3; ++c) nrm[c] *= sg / nl;
-    const double q = 0.57735026918962576451;
-    const double xs[4] = {-1.0, 1.0, 1.0, -1.0}, es[4] = {-1.0, -1.0, 1.0, 1.0};
-    for (int g = 0; g < 4; ++g) {
-        const double xi = xs[g] * q, eta = es[g] * q;
-        double H[4], dx[3] = {0, 0, 0}, de[3] = {0, 0, 0};
-        for (int a = 0; a < 4; ++a) {
-            H[a] = 0.25 * (1.0 + xs[a] * xi) * (1.0 + es[a] * eta);
-            const double hx = 0.25 * xs[a] * (1.0 + es[a] * eta), he = 0.25 * es[a] * (1.0 + xs[a] * xi);
-            for (int c = 0; c < 3; ++c) {
-                dx[c] += hx * P[a][c];
-                de[c] += he * P[a][c];
-            }
-        }
-        double cr[3];
-        d_cross(dx, de, cr);
-        const double dA = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
-        for (int c = 0; c < 3; ++c) nsum[c] += nrm[c] * dA;
-        for (int j = 0; j < 4; ++j) {
-            const double N = H[0] * phi[0][j] + H[1] * phi[1][j] + H[2] * phi[2][j] + H[3] * phi[3][j];
-            for (int c = 0; c < 3; ++c) gsum[j][c] += N * nrm[c] * dA;
-        }
+    d_cross(e1, e2, S1);  // twice the vector areas
+    d_cross(e2, e3, S2);
+    double dir = 0.0;
+    for (int c = 0; c < 3; ++c) dir += (S1[c] + S2[c]) * (0.25 * (P[0][c] + P[1][c] + P[2][c] + P[3][c]) - ref[c]);
+    const double h = dir < 0.0 ? -0.5 : 0.5;
+    for (int c = 0; c < 3; ++c) {
+        S1[c] *= h;
+        S2[c] *= h;
+        nsum[c] += S1[c] + S2[c];
+    }
+    const double t = 1.0 / 3.0;
+    for (int j = 0; j < 4; ++j) {
+        const double w1 = t * (phi[0][j] + phi[1][j] + phi[2][j]), w2 = t * (phi[0][j] + phi[2][j] + phi[3][j]);
+        for (int c = 0; c < 3; ++c) gsum[j][c] += w1 * S1[c] + w2 * S2[c];
     }
 }
 
@@ -172,83 +169,83 @@ __device__ inline void tet_cell(const double x[4][3], int bflag, double gsum[4][
 // the face opposite local node p
 __device__ inline int face_opposite(int p) { return p == 0 ? 2 : (p == 1 ? 3 : (p == 2 ? 1 : 0)); }
 
-__global__ __launch_bounds__(SF_WG) void k_sfem_grad(int ndom, const double *__restrict__ xyz, const int *__restrict__ conn,
-                                                     const unsigned char *__restrict__ fflag,
-                                                     const int *__restrict__ mptr, const int *__restrict__ member,
-                                                     const int *__restrict__ eptr, const int *__restrict__ elem,
-                                                     const double *__restrict__ vol, double *__restrict__ rows)
+// One workgroup per chunk of consecutive domains (<= SF_PAIRS (domain,
+// element) pairs and <= SF_PAIRS members).  Phase 1, lane = pair: the
+// element's facets of the domain's cell, per rotated corner j the integral
+// of N_j n, the integral of n and grad N_j, into LDS.  Phase 2, lane =
+// (domain, member): the member's entries summed in ascending element order,
+// scaled, written as a 12-double row {G sqrt(V), a_0 sqrt(V/3), a_1 .., a_2 ..}
+// so that the block gather needs no volume.
+__global__ __launch_bounds__(SF_PAIRS) void k_sfem_grad(int nchunks, const int *__restrict__ chunk_dom,
+                                                        const double *__restrict__ xyz, const int *__restrict__ conn,
+                                                        const unsigned char *__restrict__ fflag,
+                                                        const int *__restrict__ mptr, const int *__restrict__ member,
+                                                        const int *__restrict__ mdom, const int *__restrict__ eptr,
+                                                        const int *__restrict__ elem, const int *__restrict__ pdom,
+                                                        const double *__restrict__ vol, double *__restrict__ rows)
 {
-    __shared__ double s_g[SF_MAXE * 4][3];
-    __shared__ double s_n[SF_MAXE][3];
-    __shared__ double s_t[SF_MAXE * 4][3];  // grad N of the element's rotated corners
-    __shared__ int s_node[SF_MAXE * 4];
-    const int k = xcd_remap(blockIdx.x, gridDim.x);
-    if (k >= ndom) return;
-    const int e0 = eptr[k], ne = eptr[k + 1] - e0;
-    const int m0 = mptr[k], nm = mptr[k + 1] - m0;
-    for (int q = threadIdx.x; q < ne; q += SF_WG) {
-        const int e = elem[e0 + q];
+    __shared__ double s_g[SF_PAIRS][4][3];
+    __shared__ double s_t[SF_PAIRS][4][3];  // grad N of the element's rotated corners
+    __shared__ double s_n[SF_PAIRS][3];
+    __shared__ int s_node[SF_PAIRS][4];
+    const int chunk = xcd_remap(blockIdx.x, nchunks);
+    const int d0 = chunk_dom[chunk], d1 = chunk_dom[chunk + 1];
+    const int p0 = eptr[d0], p1 = eptr[d1];
+    const int p = p0 + (int)threadIdx.x;
+    if (p < p1) {
+        const int k = pdom[p], e = elem[p], q = p - p0;
         const int *cn = conn + 4 * (size_t)e;
-        int p = 0;
+        int pos = 0;
         for (int a = 1; a < 4; ++a)
-            if (cn[a] == k) p = a;
+            if (cn[a] == k) pos = a;
         double x[4][3], X[12];
         int nodes[4];
         for (int j = 0; j < 4; ++j) {
-            nodes[j] = cn[(p + j) & 3];
+            nodes[j] = cn[(pos + j) & 3];
             for (int c = 0; c < 3; ++c) x[j][c] = X[3 * j + c] = xyz[3 * nodes[j] + c];
         }
         // surface flags of the faces opposite rotated corners 1, 3, 2
         const int ff = fflag[e];
         int b = 0;
-        if (ff >> face_opposite((p + 1) & 3) & 1) b |= 1;
-        if (ff >> face_opposite((p + 2) & 3) & 1) b |= 2;
-        if (ff >> face_opposite((p + 3) & 3) & 1) b |= 4;
+        if (ff >> face_opposite((pos + 1) & 3) & 1) b |= 1;
+        if (ff >> face_opposite((pos + 2) & 3) & 1) b |= 2;
+        if (ff >> face_opposite((pos + 3) & 3) & 1) b |= 4;
         double gsum[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, nsum[3] = {0, 0, 0};
         tet_cell(x, b, gsum, nsum);
         double g[3][4];
         tet_grads(X, g);
         for (int j = 0; j < 4; ++j) {
-            s_node[4 * q + j] = nodes[j];
+            s_node[q][j] = nodes[j];
             for (int c = 0; c < 3; ++c) {
-                s_g[4 * q + j][c] = gsum[j][c];
-                s_t[4 * q + j][c] = g[c][j];
+                s_g[q][j][c] = gsum[j][c];
+                s_t[q][j][c] = g[c][j];
             }
         }
         for (int c = 0; c < 3; ++c) s_n[q][c] = nsum[c];
     }
     __syncthreads();
+    const int m = mptr[d0] + (int)threadIdx.x;
+    if (m >= mptr[d1]) return;
+    const int k = mdom[m], node = member[m];
     const double V = vol[k];
+    double G[3] = {0, 0, 0}, GG[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int q = eptr[k] - p0; q < eptr[k + 1] - p0; ++q)
+        for (int j = 0; j < 4; ++j)
+            if (s_node[q][j] == node) {
+                for (int c = 0; c < 3; ++c) G[c] += s_g[q][j][c];
+                for (int a = 0; a < 3; ++a)
+                    for (int c = 0; c < 3; ++c) GG[a][c] += s_t[q][j][a] * s_n[q][c];
+            }
     const double l = cbrt(3.0 * V / (4.0 * 3.14159265358979323846));
-    for (int i = threadIdx.x; i < nm; i += SF_WG) {
-        const int node = member[m0 + i];
-        double G[3] = {0, 0, 0}, GG[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        for (int q = 0; q < ne; ++q)
-            for (int j = 0; j < 4; ++j)
-                if (s_node[4 * q + j] == node) {
-                    for (int c = 0; c < 3; ++c) G[c] += s_g[4 * q + j][c];
-                    for (int a = 0; a < 3; ++a)
-                        for (int c = 0; c < 3; ++c) GG[a][c] += s_t[4 * q + j][a] * s_n[q][c];
-                }
-        double *row = rows + 16 * (size_t)(m0 + i);
-        const double iv = 1.0 / V;
-        for (int c = 0; c < 3; ++c) row[c] = G[c] * iv;
-        for (int in = 0; in < 3; ++in)
-            for (int a = 0; a < 3; ++a) row[3 + 3 * in + a] = GG[a][in] * iv * l;
-        row[12] = V;
-        row[13] = row[14] = row[15] = 0.0;
-    }
+    const double iv = 1.0 / V, sg = sqrt(V), sa = sqrt(V / 3.0);
+    double *row = rows + 12 * (size_t)m;
+    for (int c = 0; c < 3; ++c) row[c] = G[c] * iv * sg;
+    for (int in = 0; in < 3; ++in)
+        for (int a = 0; a < 3; ++a) row[3 + 3 * in + a] = GG[a][in] * iv * l * sa;
 }
 
-__device__ inline void kform(const double *x, const double *y, double lam, double mu, double w, double acc[9])
-{
-    const double dot = x[0] * y[0] + x[1] * y[1] + x[2] * y[2];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) acc[3 * c + d] += w * (lam * x[c] * y[d] + mu * x[d] * y[c] + (c == d ? mu * dot : 0.0));
-}
-
+// K_rs = lambda M + mu M^T + mu tr(M) I with M = sum over the block's
+// domains of the outer products of the scaled rows (G and the three a_in)
 __global__ __launch_bounds__(256) void k_sfem_gather(int nblk, const int *__restrict__ blk_r, const int *__restrict__ node_blk,
                                                      const int *__restrict__ col_ptr, const int *__restrict__ cptr,
                                                      const int2 *__restrict__ contrib, const double *__restrict__ rows,
@@ -258,22 +255,34 @@ __global__ __launch_bounds__(256) void k_sfem_gather(int nblk, const int *__rest
     if (b >= nblk) return;
     const int r = blk_r[b];
     const int k = b - node_blk[r];
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int t = cptr[b]; t < cptr[b + 1]; ++t) {
         const int2 pr = contrib[t];
-        const double *x = rows + 16 * (size_t)pr.x, *y = rows + 16 * (size_t)pr.y;
-        const double V = x[12];
-        kform(x, y, lam, mu, V, acc);
-        kform(x + 3, y + 3, lam, mu, V / 3.0, acc);
-        kform(x + 6, y + 6, lam, mu, V / 3.0, acc);
-        kform(x + 9, y + 9, lam, mu, V / 3.0, acc);
+        const double2 *x2 = reinterpret_cast<const double2 *>(rows + 12 * (size_t)pr.x);
+        const double2 *y2 = reinterpret_cast<const double2 *>(rows + 12 * (size_t)pr.y);
+        double x[12], y[12];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            const double2 u = x2[i], v = y2[i];
+            x[2 * i] = u.x;
+            x[2 * i + 1] = u.y;
+            y[2 * i] = v.x;
+            y[2 * i + 1] = v.y;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) M[3 * c + d] += x[3 * v + c] * y[3 * v + d];
     }
+    const double tr = M[0] + M[4] + M[8];
     const int base = col_ptr[3 * r], stride = col_ptr[3 * r + 1] - base;
     double *dst = val + base + 3 * k;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) dst[c * stride + d] = acc[3 * c + d];
+        for (int d = 0; d < 3; ++d) dst[c * stride + d] = lam * M[3 * c + d] + mu * M[3 * d + c] + (c == d ? mu * tr : 0.0);
 }
 
 void free_ws(SfemWs *w)
@@ -290,6 +299,9 @@ void free_ws(SfemWs *w)
     dev_free(w->cptr);
     dev_free(w->contrib);
     dev_free(w->blk_r);
+    dev_free(w->chunk_dom);
+    dev_free(w->mdom);
+    dev_free(w->pdom);
     delete w;
 }
 
@@ -503,7 +515,27 @@ int sfem_setup(ofea_ctx *c)
     if ((rc = dev_upload(w->dom_eptr, H.eptr.data(), H.eptr.size()))) return rc;
     if ((rc = dev_upload(w->dom_elem, H.elem.data(), std::max<size_t>(H.elem.size(), 1)))) return rc;
     if ((rc = dev_upload(w->vol, H.vol.data(), H.vol.size()))) return rc;
-    if ((rc = dev_upload<double>(w->rows, nullptr, 16 * (size_t)std::max(w->nrows, 1LL)))) return rc;
+    if ((rc = dev_upload<double>(w->rows, nullptr, 12 * (size_t)std::max(w->nrows, 1LL)))) return rc;
+    {  // chunks of consecutive domains: <= SF_PAIRS pairs and <= SF_PAIRS members each
+        std::vector<int> chunk{0}, mdom(H.member.size()), pdom(std::max<size_t>(H.elem.size(), 1));
+        int np = 0, nm = 0;
+        for (int k = 0; k < c->nd; ++k) {
+            const int dp = H.eptr[k + 1] - H.eptr[k], dm = H.mptr[k + 1] - H.mptr[k];
+            if (np + dp > SF_PAIRS || nm + dm > SF_PAIRS) {
+                chunk.push_back(k);
+                np = nm = 0;
+            }
+            np += dp;
+            nm += dm;
+            for (int a = H.mptr[k]; a < H.mptr[k + 1]; ++a) mdom[a] = k;
+            for (int a = H.eptr[k]; a < H.eptr[k + 1]; ++a) pdom[a] = k;
+        }
+        chunk.push_back(c->nd);
+        w->nchunks = (int)chunk.size() - 1;
+        if ((rc = dev_upload(w->chunk_dom, chunk.data(), chunk.size()))) return rc;
+        if ((rc = dev_upload(w->mdom, mdom.data(), std::max<size_t>(mdom.size(), 1)))) return rc;
+        if ((rc = dev_upload(w->pdom, pdom.data(), pdom.size()))) return rc;
+    }
     if ((rc = dev_upload(w->cptr, cnt.data(), cnt.size()))) return rc;
     if ((rc = dev_upload(w->contrib, contrib.data(), std::max<size_t>(contrib.size(), 1)))) return rc;
     int *d_blk_r = nullptr;
@@ -519,9 +551,10 @@ int sfem_assemble(ofea_ctx *c, double E, double nu, double *nz_val)
     if (!(nu > -1.0 && nu < 0.5)) return fail(OFEA_ERR_ARG, "Poisson ratio must lie in (-1, 0.5)");
     double lam, mu;
     lame<double>(E, nu, lam, mu);
-    if (w->ndom > 0)
-        hipLaunchKernelGGL(k_sfem_grad, dim3(w->ndom), dim3(SF_WG), 0, c->stream, w->ndom, c->d_xyz, w->conn, w->fflag,
-                           w->dom_mptr, w->dom_member, w->dom_eptr, w->dom_elem, w->vol, w->rows);
+    if (w->nchunks > 0)
+        hipLaunchKernelGGL(k_sfem_grad, dim3(w->nchunks), dim3(SF_PAIRS), 0, c->stream, w->nchunks, w->chunk_dom,
+                           c->d_xyz, w->conn, w->fflag, w->dom_mptr, w->dom_member, w->mdom, w->dom_eptr, w->dom_elem,
+                           w->pdom, w->vol, w->rows);
     OFEA_HIP(hipGetLastError());
     const int nblk = c->P.node_blk[c->P.nfree];
     if (nblk > 0)
