@@ -133,7 +133,8 @@ def bench_static(args, workload):
                    "learned_fp64": A.PCG_LEARNED, "learned_mixed_fp32val": A.PCG_LEARNED_MIXED,
                    "two_level_fp64": A.PCG_TWO_LEVEL, "two_level_mixed_fp32val": A.PCG_TWO_LEVEL_MIXED,
                    "learned_two_level_fp64": A.PCG_LEARNED_TWO_LEVEL,
-                   "learned_two_level_mixed_fp32val": A.PCG_LEARNED_TWO_LEVEL_MIXED}
+                   "learned_two_level_mixed_fp32val": A.PCG_LEARNED_TWO_LEVEL_MIXED,
+                   "learned_two_level_mixed_fp32val_bf16net": A.PCG_LEARNED_TWO_LEVEL_MIXED}
     chosen = [] if args.no_solve else [m for m in args.solvers.split(",") if m]
     methods = [(m, all_methods[m]) for m in chosen]
     learned_info, params = {}, {}
@@ -143,10 +144,25 @@ def bench_static(args, workload):
             params[tl] = info.pop("params")
             learned_info["two_level" if tl else "block_jacobi"] = info
     ln = L.Learned(ctx, seed=0) if params else None
+    infer = {}
+    if ln is not None:  # network inference for the resident K: FP32 vs BF16 on the matrix cores
+        ln.set_params(next(iter(params.values())))
+        for prec, tag in ((A.FP32, "fp32"), (A.BF16, "bf16_mfma")):
+            ln.precision(prec)
+            ts = []
+            for _ in range(5):
+                ctx.sync()
+                t = time.perf_counter()
+                ln.refresh()
+                ctx.sync()
+                ts.append(time.perf_counter() - t)
+            infer[tag] = round(statistics.median(ts) * 1e3, 3)
+        log(f"learned inference ms: {infer}")
     ctx.set_rhs(F)
     for name, method in methods:
         if name.startswith("learned"):
             ln.set_params(params["two_level" in name])
+            ln.precision(A.BF16 if name.endswith("bf16net") else A.FP32)
         ctx.sync()
         t = time.perf_counter()
         ctx.solve(None, method=method, rtol=1e-8, maxit=100000, copy=False)  # warm (setup, graph capture)
@@ -181,6 +197,7 @@ def bench_static(args, workload):
                    "parallelism": "single GPU"},
         "time_to_solve": solves,
         "learned_training": learned_info,
+        "learned_inference_ms": infer,
         "setup_s": round(setup_s, 3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -469,9 +486,10 @@ def bench_newton(args, ctx, m, resort, E, nu, F, tip, L_):
     ln.set_params(params)
     runs = {}
     newton_all = {"plain_fp64_jacobi": A.PCG_JACOBI, "fp64_block_jacobi": A.PCG_BLOCK_JACOBI,
-                  "learned_fp64": A.PCG_LEARNED}
+                  "learned_fp64": A.PCG_LEARNED, "learned_fp64_bf16net": A.PCG_LEARNED}
     for name in [x for x in args.newton_methods.split(",") if x]:
         meth = newton_all[name]
+        ln.precision(A.BF16 if name.endswith("bf16net") else A.FP32)
         ctx.sync()
         t = time.perf_counter()
         u, nt, lt, rel, ok = ctx.nl_solve(E, nu, F, n_incr=10, tol=1e-8, max_newton=25, method=meth,
@@ -744,14 +762,15 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=1e-6, help="Newton inner PCG tolerance (cfg5)")
     ap.add_argument("--seq-methods", default="plain_fp64_jacobi,fp64_block_jacobi",
                     help="one-system-at-a-time solvers compared with the batched solve (cfg4)")
-    ap.add_argument("--newton-methods", default="plain_fp64_jacobi,fp64_block_jacobi,learned_fp64",
+    ap.add_argument("--newton-methods", default="plain_fp64_jacobi,fp64_block_jacobi,learned_fp64,learned_fp64_bf16net",
                     help="inner PCG methods of the Newton runs (cfg5)")
     ap.add_argument("--solve-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true", help="assembly only (for counter collection)")
     ap.add_argument("--solvers", default="plain_fp64_jacobi,fp64_block_jacobi,mixed_fp32val_block_jacobi,"
                     "mixed_fp32_all_block_jacobi,learned_fp64,learned_mixed_fp32val,two_level_fp64,"
-                    "two_level_mixed_fp32val,learned_two_level_fp64,learned_two_level_mixed_fp32val")
+                    "two_level_mixed_fp32val,learned_two_level_fp64,learned_two_level_mixed_fp32val,"
+                    "learned_two_level_mixed_fp32val_bf16net")
     ap.add_argument("--train-steps", type=int, default=150)
     ap.add_argument("--ca-basis", type=int, default=4, help="CA basis size (ca workload)")
     args = ap.parse_args()

@@ -83,6 +83,7 @@ extern "C" {
 /* explicit precision for ofea_explicit_run */
 #define OFEA_FP32 0
 #define OFEA_FP64 1
+#define OFEA_BF16 2            /* learned-network inference only (ofea_learned_precision) */
 
 typedef struct ofea_ctx ofea_ctx;
 typedef struct ofea_deck ofea_deck;
@@ -254,6 +255,11 @@ int ofea_learned_train(ofea_ctx *ctx, int steps, float lr, double *loss_hist);
 int ofea_learned_blocks(ofea_ctx *ctx, double *blocks, float *features);
 /* recompute features and blocks for a newly assembled K */
 int ofea_learned_refresh(ofea_ctx *ctx);
+/* inference precision of the network's hidden layer when it produces the
+ * solver's blocks: OFEA_FP32 (default) or OFEA_BF16 (BF16 operands, FP32
+ * accumulation on the matrix cores, v_mfma_f32_16x16x16_bf16); training stays
+ * FP32.  Re-runs the network for the resident K. */
+int ofea_learned_precision(ofea_ctx *ctx, int precision);
 /* out = Z A_c^-1 Z^T r (n), the coarse correction of the two-level methods for
  * the resident K (built on first use).  Training pairs of the learned
  * two-level smoother use e - out as the target. */

@@ -38,6 +38,7 @@ PCG_LEARNED_TWO_LEVEL_MIXED = 9
 
 FP32 = 0
 FP64 = 1
+BF16 = 2
 
 POST_STRAIN = 0
 POST_STRESS = 1
@@ -87,6 +88,7 @@ SIGNATURES = {
     "ofea_learned_train": (_i, [_vp, _i, _f, _vp]),
     "ofea_learned_blocks": (_i, [_vp, _vp, _vp]),
     "ofea_learned_refresh": (_i, [_vp]),
+    "ofea_learned_precision": (_i, [_vp, _i]),
     "ofea_coarse_apply": (_i, [_vp, _vp, _vp]),
     "ofea_assemble_batch": (_i, [_vp, _i, _vp, _d, _vp]),
     "ofea_solve_batch": (_i, [_vp, _i, _i, _vp, _vp, _d, _i, _vp, _vp]),

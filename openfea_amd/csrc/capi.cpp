@@ -400,6 +400,12 @@ int ofea_learned_blocks(ofea_ctx *c, double *blocks, float *features)
     return learned_blocks(c, blocks, features);
 }
 
+int ofea_learned_precision(ofea_ctx *c, int precision)
+{
+    CHECK_CTX(c);
+    return learned_precision(c, precision);
+}
+
 int ofea_learned_refresh(ofea_ctx *c)
 {
     CHECK_CTX(c);

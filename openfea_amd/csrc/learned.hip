@@ -64,6 +64,7 @@ struct LearnedWS {
     double *wsmp = nullptr;                         // S: 1/(S e_k^T r_k)
     double *loss_part = nullptr;
     int feat_kver = -1;  // K version the features were computed from
+    bool bf16 = false;   // inference of layer 2 on the matrix cores in BF16 (training stays FP32)
 };
 
 static LearnedWS *lws(ofea_ctx *c) { return (LearnedWS *)c->learned; }
@@ -186,6 +187,73 @@ __global__ void k_ln_dense(int nfree, int Cin, int Cout, const float *__restrict
     for (int i = 0; i < Cin; ++i) acc += ws[i] * hs[i] + wn[i] * ha[i];
     Z[idx] = acc;
     H[idx] = acc > 0.0f ? acc : 0.0f;
+}
+
+// BF16 with round-to-nearest-even (finite inputs)
+__device__ inline short to_bf16(float f)
+{
+    unsigned u = __builtin_bit_cast(unsigned, f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+typedef short ln_short4 __attribute__((ext_vector_type(4)));
+typedef float ln_float4 __attribute__((ext_vector_type(4)));
+
+// Inference form of k_ln_dense for CIN = COUT = 64 on the matrix cores:
+// Z = [Hs | Ha] [Ws | Wn]^T + b (K = 128), BF16 operands, FP32 accumulation,
+// v_mfma_f32_16x16x16_bf16.  Four waves per workgroup, each a 16-node x
+// 64-channel tile (four 16x16 accumulators, eight K steps).  Operand layout
+// of the instruction: lane l supplies A[l % 16][4 (l / 16) + i] and
+// B[4 (l / 16) + i][l % 16] (i = 0..3) and receives D[4 (l / 16) + i][l % 16].
+// The weights are converted to BF16 once per workgroup into LDS.
+constexpr int LN_MF_WG = 256;
+__global__ __launch_bounds__(LN_MF_WG) void k_ln_dense_mfma(int nfree, const float *__restrict__ Hs,
+                                                            const float *__restrict__ Ha, const float *__restrict__ Ws,
+                                                            const float *__restrict__ Wn, const float *__restrict__ b,
+                                                            float *__restrict__ Z, float *__restrict__ H)
+{
+    constexpr int CIN = LN_H, COUT = LN_H, K = 2 * CIN;
+    __shared__ short s_w[COUT][K + 4];  // +4: rows start on different banks
+    for (int i = threadIdx.x; i < COUT * K; i += LN_MF_WG) {
+        const int o = i / K, k = i - o * K;
+        s_w[o][k] = to_bf16(k < CIN ? Ws[o * CIN + k] : Wn[o * CIN + k - CIN]);
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row0 = (blockIdx.x * (LN_MF_WG / 64) + wave) * 16;
+    const int m = lane & 15, kq = lane >> 4;
+    const int node = row0 + m;
+    ln_float4 acc[COUT / 16];
+#pragma unroll
+    for (int t = 0; t < COUT / 16; ++t) acc[t] = ln_float4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < K; kk += 16) {
+        const int k0 = kk + 4 * kq;  // never straddles the Hs | Ha boundary (CIN % 4 == 0)
+        const float *src = k0 < CIN ? Hs + (size_t)node * CIN + k0 : Ha + (size_t)node * CIN + (k0 - CIN);
+        ln_short4 a = {0, 0, 0, 0};
+        if (node < nfree) {
+            const float4 v = *reinterpret_cast<const float4 *>(src);
+            a = ln_short4{to_bf16(v.x), to_bf16(v.y), to_bf16(v.z), to_bf16(v.w)};
+        }
+#pragma unroll
+        for (int t = 0; t < COUT / 16; ++t) {
+            const short *wr = &s_w[16 * t + m][k0];
+            const ln_short4 bv = {wr[0], wr[1], wr[2], wr[3]};
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < COUT / 16; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = row0 + 4 * kq + i, o = 16 * t + m;
+            if (r < nfree) {
+                const float z = acc[t][i] + b[o];
+                Z[(size_t)r * COUT + o] = z;
+                H[(size_t)r * COUT + o] = z > 0.0f ? z : 0.0f;
+            }
+        }
 }
 
 // head: o = Wo h2 + bo; M = A L L^T A^T (FP64).  Thread per node.
@@ -421,8 +489,10 @@ static int ln_features(ofea_ctx *c)
     return OFEA_OK;
 }
 
-// forward pass with the device parameters; M blocks (FP64) into learned64
-static int ln_forward(ofea_ctx *c)
+// forward pass with the device parameters; M blocks (FP64) into learned64.
+// infer: the network produces the solver's blocks (BF16 layer 2 when set);
+// training always runs FP32.
+static int ln_forward(ofea_ctx *c, bool infer)
 {
     LearnedWS *w = lws(c);
     ofea_solver_ws *s = c->ws;
@@ -432,8 +502,12 @@ static int ln_forward(ofea_ctx *c)
                        w->A0, P + OFF_W1S, P + OFF_W1N, P + OFF_B1, w->Z1, w->H1);
     hipLaunchKernelGGL(k_ln_agg, dim3(grid_for((long long)N * LN_H)), dim3(LN_WG), 0, c->stream, N, LN_H,
                        c->d_node_blk, c->d_bcol, w->H1, w->A1, 0);
-    hipLaunchKernelGGL(k_ln_dense, dim3(grid_for((long long)N * LN_H)), dim3(LN_WG), 0, c->stream, N, LN_H, LN_H,
-                       w->H1, w->A1, P + OFF_W2S, P + OFF_W2N, P + OFF_B2, w->Z2, w->H2);
+    if (infer && w->bf16)
+        hipLaunchKernelGGL(k_ln_dense_mfma, dim3((N + 63) / 64), dim3(LN_MF_WG), 0, c->stream, N, w->H1, w->A1,
+                           P + OFF_W2S, P + OFF_W2N, P + OFF_B2, w->Z2, w->H2);
+    else
+        hipLaunchKernelGGL(k_ln_dense, dim3(grid_for((long long)N * LN_H)), dim3(LN_WG), 0, c->stream, N, LN_H, LN_H,
+                           w->H1, w->A1, P + OFF_W2S, P + OFF_W2N, P + OFF_B2, w->Z2, w->H2);
     if (!s->learned64) OFEA_HIP(hipMalloc((void **)&s->learned64, sizeof(double) * 9 * (size_t)std::max(N, 1)));
     hipLaunchKernelGGL(k_ln_head, dim3(grid_for(N)), dim3(LN_WG), 0, c->stream, N, w->H2, P + OFF_WO, P + OFF_BO,
                        w->Rinv, w->O, s->learned64);
@@ -469,7 +543,7 @@ static int ln_loss_grad(ofea_ctx *c, double *loss)
 {
     LearnedWS *w = lws(c);
     const int N = w->nfree, S = w->S, n = 3 * N;
-    int rc = ln_forward(c);
+    int rc = ln_forward(c, false);
     if (rc) return rc;
     ofea_solver_ws *s = c->ws;
     hipLaunchKernelGGL(k_ln_residual, dim3(grid_for((long long)N * S)), dim3(LN_WG), 0, c->stream, N, S, s->learned64,
@@ -540,7 +614,7 @@ int learned_init(ofea_ctx *c, unsigned long long seed)
     if (rc) return rc;
     init_params(lws(c)->params, seed);
     if ((rc = upload_params(c))) return rc;
-    return ln_forward(c);
+    return ln_forward(c, true);
 }
 
 int learned_set_params(ofea_ctx *c, const float *params, int nparam)
@@ -550,7 +624,7 @@ int learned_set_params(ofea_ctx *c, const float *params, int nparam)
     if (rc) return rc;
     std::memcpy(lws(c)->params.data(), params, sizeof(float) * LN_NPARAM);
     if ((rc = upload_params(c))) return rc;
-    return ln_forward(c);
+    return ln_forward(c, true);
 }
 
 int learned_get_params(ofea_ctx *c, float *params, int nparam)
@@ -629,7 +703,7 @@ int learned_train(ofea_ctx *c, int steps, float lr, double *loss_hist)
         }
         if ((rc = upload_params(c))) return rc;
     }
-    int rc = ln_forward(c);
+    int rc = ln_forward(c, true);
     if (rc) return rc;
     OFEA_HIP(hipStreamSynchronize(c->stream));
     return OFEA_OK;
@@ -648,6 +722,19 @@ int learned_blocks(ofea_ctx *c, double *blocks, float *features)
     return OFEA_OK;
 }
 
+int learned_precision(ofea_ctx *c, int precision)
+{
+    if (precision != OFEA_FP32 && precision != OFEA_BF16)
+        return fail(OFEA_ERR_ARG, "inference precision must be OFEA_FP32 or OFEA_BF16");
+    LearnedWS *w = lws(c);
+    if (!w) return fail(OFEA_ERR_STATE, "learned preconditioner not initialised");
+    w->bf16 = precision == OFEA_BF16;
+    int rc = ln_prepare(c);
+    if (rc) return rc;
+    if ((rc = upload_params(c))) return rc;
+    return ln_forward(c, true);
+}
+
 // re-run the network on the resident K (after a new assembly)
 int learned_refresh(ofea_ctx *c)
 {
@@ -655,7 +742,7 @@ int learned_refresh(ofea_ctx *c)
     int rc = ln_prepare(c);
     if (rc) return rc;
     if ((rc = upload_params(c))) return rc;
-    return ln_forward(c);
+    return ln_forward(c, true);
 }
 
 }  // namespace ofea

@@ -202,6 +202,7 @@ int learned_loss_grad(ofea_ctx *c, double *loss, float *grad);
 int learned_train(ofea_ctx *c, int steps, float lr, double *loss_hist);
 int learned_blocks(ofea_ctx *c, double *blocks, float *features);
 int learned_refresh(ofea_ctx *c);
+int learned_precision(ofea_ctx *c, int precision);
 // nonlinear (nl.hip)
 int nl_assemble(ofea_ctx *ctx, double E, double nu, const double *u, double *nz_val, double *f_int);
 int nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr, double tol, int max_newton, int method,

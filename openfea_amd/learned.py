@@ -96,5 +96,9 @@ class Learned:
         check(lib().ofea_learned_blocks(self.ctx._h, ptr(b), ptr(f)))
         return b.reshape(n, 3, 3), f.reshape(n, 9)
 
+    def precision(self, prec):
+        """Inference precision of the hidden layer (A.FP32 or A.BF16 on the matrix cores)."""
+        check(lib().ofea_learned_precision(self.ctx._h, prec))
+
     def refresh(self):
         check(lib().ofea_learned_refresh(self.ctx._h))

@@ -102,14 +102,27 @@ def lmat(o):
     return L
 
 
-def forward(g: Graph, p):
+def bf16(x):
+    """Round to the nearest BF16 value (ties to even), returned as float64."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward(g: Graph, p, bf16_hidden=False):
+    """Blocks M (n x 3 x 3) and the activations.  bf16_hidden rounds the
+    hidden layer's inputs and weights to BF16 (products accumulated exactly),
+    the arithmetic of the library's BF16 inference."""
     P = unpack(p)
     X = g.features()
     A0 = g.agg(X)
     Z1 = X @ P["W1s"].T + A0 @ P["W1n"].T + P["b1"]
     H1 = np.maximum(Z1, 0)
     A1 = g.agg(H1)
-    Z2 = H1 @ P["W2s"].T + A1 @ P["W2n"].T + P["b2"]
+    if bf16_hidden:
+        Z2 = bf16(H1) @ bf16(P["W2s"]).T + bf16(A1) @ bf16(P["W2n"]).T + P["b2"]
+    else:
+        Z2 = H1 @ P["W2s"].T + A1 @ P["W2n"].T + P["b2"]
     H2 = np.maximum(Z2, 0)
     Oo = H2 @ P["Wo"].T + P["bo"]
     A = g.rinv()

@@ -108,3 +108,29 @@ def test_training_and_learned_pcg(setup):
         say(f"learned two-level method {meth}: {it3} iterations (two-level block Jacobi {it_tl}); "
             f"loss {hist2[0]:.4e} -> {hist2[-1]:.4e}")
         assert ok3 and np.linalg.norm(x3 - xd) / np.linalg.norm(xd) < 1e-8
+
+
+def test_bf16_inference_on_matrix_cores(setup):
+    """BF16 inference (v_mfma_f32_16x16x16_bf16) against the oracle with the
+    hidden layer's inputs and weights rounded to BF16: 2e-4 and at least ten
+    times closer than to the FP32 forward (which pins the operand layout);
+    PCG with the BF16 blocks still reaches the direct solution."""
+    ctx, g, F, cp, ri, val = setup
+    say("learned BF16 inference")
+    ln = L.Learned(ctx, seed=3)
+    p = random_params(7, head=0.5)
+    ln.set_params(p)
+    b32, _ = ln.blocks()
+    ln.precision(A.BF16)
+    b16, _ = ln.blocks()
+    Me, _ = LO.forward(g, p.astype(np.float64), bf16_hidden=True)
+    M32, _ = LO.forward(g, p.astype(np.float64))
+    e16, e32 = rel_max(b16, Me), rel_max(b16, M32)
+    say(f"bf16 blocks: {e16:.2e} from the BF16 oracle, {e32:.2e} from FP32")
+    assert e16 < 2e-4 and 10 * e16 < e32
+    assert rel_max(b32, M32) < 1e-5
+    x, it, rel, ok = ctx.solve(F, method=A.PCG_LEARNED, rtol=1e-10, maxit=100000)
+    xd = spla.spsolve(sp.csc_matrix((val, ri, cp), shape=(len(cp) - 1,) * 2), F)
+    assert ok and np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
+    ln.precision(A.FP32)
+    assert rel_max(ln.blocks()[0], b32) == 0.0

@@ -63,3 +63,11 @@ def test_gradient_finite_differences(graph):
             pm[i] -= h
             fd = (LO.loss_grad(graph, pp, E, R)[0] - LO.loss_grad(graph, pm, E, R)[0]) / (2 * h)
             assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-5 * np.abs(g).max(), (LO.SIZES[gi][0], i, fd, g[i])
+
+
+def test_bf16_rounding():
+    # BF16 keeps 7 mantissa bits: 1 + 2^-7 is exact; 1 + 2^-8 and 1 + 1.5 * 2^-7 are ties that go to the even mantissa
+    x = np.array([1.0, 1.0 + 2 ** -7, 1.0 + 2 ** -8, 1.0 + 1.5 * 2 ** -7, 1.0 + 2 ** -9, -3.0, 0.0], dtype=np.float32)
+    assert np.array_equal(LO.bf16(x), [1.0, 1.0 + 2 ** -7, 1.0, 1.0 + 2 ** -6, 1.0, -3.0, 0.0])
+    r = np.random.default_rng(0).normal(size=1000).astype(np.float32)
+    assert np.all(np.abs(LO.bf16(r) - r) <= 2 ** -8 * np.abs(r))
