@@ -106,22 +106,27 @@ template <bool SPARSE>
 __device__ inline void tet_task_load(int task, int nblk, int b0, int t0, const uint8_t *__restrict__ blk_rl,
                                      const int *__restrict__ off_ptr, const int *__restrict__ blk_gid, TetTask &t)
 {
-    t.kind = 0;
-    if (task < nblk) {
-        const int bk = b0 + task;
-        t.rl = blk_rl[bk];
+    // every field is written on both paths (selects, no partial writes), so
+    // the task stays in registers instead of a dynamically indexed stack slot
+    const bool off = task < nblk;
+    const int bk = b0 + (off ? task : 0);
+    int rl1 = 0, k1 = 0, c0 = 0, c1 = 0;
+    if (off) {
+        rl1 = blk_rl[bk];
         // global block index until resolved: a full plan lists every block in
         // order, a sparse plan (a few modified elements) names each one
-        t.k = SPARSE ? blk_gid[bk] : bk;
-        t.c0 = off_ptr[bk] - t0;
-        t.c1 = off_ptr[bk + 1] - t0;
-        t.kind = 1;
-    } else {
-        const int j = (task - nblk) >> 1;  // half-row task 2 (3 node + c) + h
-        t.rl = j / 3;
-        t.c = j - 3 * t.rl;
-        t.kind = 2;
+        k1 = SPARSE ? blk_gid[bk] : bk;
+        c0 = off_ptr[bk] - t0;
+        c1 = off_ptr[bk + 1] - t0;
     }
+    const int j = (task - nblk) >> 1;  // half-row task 2 (3 node + c) + h
+    const int rl2 = off ? 0 : j / 3;
+    t.kind = off ? 1 : 2;
+    t.rl = off ? rl1 : rl2;
+    t.k = k1;
+    t.c0 = c0;
+    t.c1 = c1;
+    t.c = off ? 0 : j - 3 * rl2;
 }
 
 // LDS part: node tables s_nb (node_blk of the chunk's nodes, nnode + 1), s_dk
