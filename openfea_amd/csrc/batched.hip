@@ -43,13 +43,21 @@ __device__ inline void gather_blocks_batch(int b0, int b1, const int *__restrict
                                            const int *__restrict__ col_ptr, double *__restrict__ valb, long long nnz,
                                            int nvar, int nn, const double (*sE)[BA_VMAX], const UnitBlock &unit)
 {
-    for (int bk = b0 + (int)threadIdx.x; bk < b1; bk += BA_WG) {
+    // one task per (block, group of BA_VC variants), blocks fastest: every
+    // lane of the workgroup has work although a chunk has fewer blocks than
+    // lanes, and neighbouring lanes still write neighbouring blocks
+    const int nblk = b1 - b0;
+    const int ntask = nblk * ((nvar + BA_VC - 1) / BA_VC);
+    for (int task = (int)threadIdx.x; task < ntask; task += BA_WG) {
+        const int grp = task / nblk;
+        const int bk = b0 + (task - grp * nblk);
+        const int v0 = grp * BA_VC;
         const int rk = blk_rk[bk];
         const int r = rk >> 8, k = rk & 255;
         const int t0 = contrib_ptr[bk], t1 = contrib_ptr[bk + 1];
         const int base = col_ptr[3 * r];
         const int stride = col_ptr[3 * r + 1] - base;
-        for (int v0 = 0; v0 < nvar; v0 += BA_VC) {
+        {
             double acc[BA_VC][9];
 #pragma unroll
             for (int v = 0; v < BA_VC; ++v)
@@ -168,13 +176,19 @@ __global__ __launch_bounds__(BA_WG) void k_assemble_tet_batch(
         s_el[lp] = el;
     }
     __syncthreads();
-    for (int bk = chunk_blk[chunk] + (int)threadIdx.x; bk < chunk_blk[chunk + 1]; bk += BA_WG) {
+    // tasks (block, group of BA_VC variants), blocks fastest (as gather_blocks_batch)
+    const int b0 = chunk_blk[chunk], nblk = chunk_blk[chunk + 1] - b0;
+    const int ntask = nblk * ((nvar + BA_VC - 1) / BA_VC);
+    for (int task = (int)threadIdx.x; task < ntask; task += BA_WG) {
+        const int grp = task / nblk;
+        const int bk = b0 + (task - grp * nblk);
+        const int v0 = grp * BA_VC;
         const int rk = blk_rk[bk];
         const int r = rk >> 8, k = rk & 255;
         const int t0 = contrib_ptr[bk], t1 = contrib_ptr[bk + 1];
         const int base = col_ptr[3 * r];
         const int stride = col_ptr[3 * r + 1] - base;
-        for (int v0 = 0; v0 < nvar; v0 += BA_VC) {
+        {
             double acc[BA_VC][9];
 #pragma unroll
             for (int v = 0; v < BA_VC; ++v)
