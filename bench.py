@@ -42,17 +42,26 @@ def load_traffic(kernel_tag):
 
 
 def cpu_baseline_fill(m, resort, reps=3):
-    import numpy as np
+    """Oracle fill rate (elems/s, one core) and the K it filled."""
     from oracle import oracle as O
 
     cp, ri = O.build_csc(m.conn, m.etype, resort, 3 * int(resort.max() + 1))
     times = []
     for _ in range(reps):
         t = time.perf_counter()
-        O.fill_csc(m.conn, m.etype, m.xyz, resort, 210000.0, 0.3, cp, ri)
+        val = O.fill_csc(m.conn, m.etype, m.xyz, resort, 210000.0, 0.3, cp, ri)
         times.append(time.perf_counter() - t)
-    del np
-    return m.ne / statistics.median(times)
+    return m.ne / statistics.median(times), (cp, ri, val)
+
+
+def cpu_baseline_solve(K, F):
+    """Oracle Jacobi PCG (the plain FP64 method, one core) to rel. residual 1e-8."""
+    from oracle import oracle as O
+
+    cp, ri, val = K
+    t = time.perf_counter()
+    _, it, rel = O.pcg_jacobi(cp, ri, val, F, rtol=1e-8, maxit=100000)
+    return (time.perf_counter() - t) * 1e3, it, rel
 
 
 def train_learned(workload, args, two_level=False):
@@ -183,10 +192,16 @@ def bench_static(args, workload):
     cpu = None
     if not args.no_cpu_baseline:
         t = time.perf_counter()
-        rate = cpu_baseline_fill(m, resort)
+        rate, K = cpu_baseline_fill(m, resort)
         cpu = {"value": rate, "unit": "elems/s", "cores": 1, "kind": "port",
                "sample": f"oracle fill_csc (reference fill_CSR_sparse_mat algorithm) over the full {m.ne}-element "
                          f"mesh, pattern prebuilt, median of 3 ({time.perf_counter() - t:.1f}s)"}
+        if workload == "cfg2" and not args.no_solve:
+            ms, it, rel = cpu_baseline_solve(K, F)
+            cpu["time_to_solve"] = {"ms": round(ms, 1), "iters": it, "relres": rel,
+                                    "method": "oracle Jacobi PCG, FP64, one core, same system and stopping rule "
+                                              "as plain_fp64_jacobi"}
+            log(f"cpu baseline: fill {rate:.3e} elems/s, Jacobi PCG {ms:.0f} ms ({it} it)")
     ctx.close()
     return {
         "metric": "assembled elems/s + time-to-solve at rel-resid 1e-8 (learned vs plain PCG)",
