@@ -264,6 +264,12 @@ int ofea_learned_precision(ofea_ctx *ctx, int precision);
  * the resident K (built on first use).  Training pairs of the learned
  * two-level smoother use e - out as the target. */
 int ofea_coarse_apply(ofea_ctx *ctx, const double *r, double *out);
+/* The aggregates of that coarse space (no counterpart in the reference):
+ * agg (nfree = n/3) receives the aggregate of every free node; returns the
+ * number of aggregates (>= 0) or an error.  Aggregates are compact parts of
+ * the free nodes (slabs across the longest axis, cut by recursive coordinate
+ * bisection), all of the same node count except the last. */
+int ofea_coarse_aggregates(ofea_ctx *ctx, int *agg);
 
 /* ---- combined-approximation (CA) reanalysis on the resident K0 ----
  * ofea_ca_delta: Delta K on the pattern of the resident K (nnz values, copied

@@ -473,6 +473,14 @@ int ofea_coarse_apply(ofea_ctx *c, const double *r, double *out)
     return coarse_apply(c, r, out);
 }
 
+int ofea_coarse_aggregates(ofea_ctx *c, int *agg)
+{
+    CHECK_CTX(c);
+    if (!c->assembled) return fail(OFEA_ERR_STATE, "ofea_assemble first");
+    if (!agg) return fail(OFEA_ERR_ARG, "null agg");
+    return coarse_aggregates(c, agg);
+}
+
 int ofea_sync(ofea_ctx *c)
 {
     CHECK_CTX(c);

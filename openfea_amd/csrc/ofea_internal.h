@@ -232,6 +232,7 @@ void post_free(ofea_ctx *c);
 int solver_spmv_group_time(ofea_ctx *ctx, int group, int reps, float *ms);
 // two-level coarse correction Z A_c^-1 Z^T r on the host (solver.hip)
 int coarse_apply(ofea_ctx *ctx, const double *r, double *out);
+int coarse_aggregates(ofea_ctx *ctx, int *agg);
 
 template <class T>
 int dev_upload(T *&d, const T *h, size_t count)

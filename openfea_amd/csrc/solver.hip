@@ -25,6 +25,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <thread>
 #include <vector>
 
 #include "elem_device.h"
@@ -838,6 +839,33 @@ __global__ void k_precond_setup(int nfree, const int *__restrict__ node_blk, con
     }
 }
 
+// Reordered system of the two-level methods: the node rows of K copied into
+// the new node order (64 lanes per node; a row keeps its block order, its
+// neighbour indices are renumbered on the host).
+__global__ __launch_bounds__(RED_WG) void k_perm_vals(int nfree, const int *__restrict__ perm,
+                                                      const int *__restrict__ node_blk, const int *__restrict__ nb_p,
+                                                      const double *__restrict__ val, double *__restrict__ val_p)
+{
+    const int node = blockIdx.x * (RED_WG / 64) + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (node >= nfree) return;
+    const int old = perm[node];
+    const long long s0 = 9LL * node_blk[old], d0 = 9LL * nb_p[node];
+    const int len = 9 * (node_blk[old + 1] - node_blk[old]);
+    for (int i = lane; i < len; i += 64) val_p[d0 + i] = val[s0 + i];
+}
+
+// dst[NV i + c] = src[NV map[i] + c] for the nfree nodes (vectors NV = 3,
+// preconditioner blocks NV = 9)
+template <int NV>
+__global__ __launch_bounds__(RED_WG) void k_gather_nodes(int nfree, const int *__restrict__ map,
+                                                         const double *__restrict__ src, double *__restrict__ dst)
+{
+    const long long i = (long long)blockIdx.x * RED_WG + threadIdx.x;
+    if (i >= (long long)NV * nfree) return;
+    const int node = (int)(i / NV), c = (int)(i - (long long)node * NV);
+    dst[i] = src[(long long)NV * map[node] + c];
+}
+
 // ------------------------------------------------------------------ host side
 static int ensure_ws(ofea_ctx *ctx)
 {
@@ -889,6 +917,10 @@ void solver_free(ofea_ctx *ctx)
     if (w->learned64) (void)hipFree(w->learned64);
     for (void *p : {(void *)w->co.g, (void *)w->co.d, (void *)w->co_ainv, (void *)w->co.zq, (void *)w->co.ya})
         if (p) (void)hipFree(p);
+    for (void *p : {(void *)w->perm, (void *)w->iperm, (void *)w->nb_p, (void *)w->bcol_p, (void *)w->val_p,
+                    (void *)w->val32_p, (void *)w->dinv_p, (void *)w->minv_p, (void *)w->dblk_p,
+                    (void *)w->learned64_p, (void *)w->minv32_p, (void *)w->b_p, (void *)w->x_p})
+        if (p) (void)hipFree(p);
     if (w->sc) (void)hipFree(w->sc);
     if (w->sc_host) (void)hipHostFree(w->sc_host);
     for (auto &e : w->pipe_ev)
@@ -906,7 +938,8 @@ void solver_invalidate(ofea_ctx *ctx)
 }
 
 template <class TV, class T, bool DOT>
-static void launch_spmv_t(ofea_ctx *ctx, int group, const TV *val, const T *x, T *y)
+static void launch_spmv_p(ofea_ctx *ctx, int group, const int *node_blk, const int *bcol, const TV *val, const T *x,
+                          T *y)
 {
     ofea_solver_ws *w = ctx->ws;
     int npw = RED_WG / group;
@@ -915,7 +948,7 @@ static void launch_spmv_t(ofea_ctx *ctx, int group, const TV *val, const T *x, T
 #define OFEA_SPMV_CASE(G)                                                                                        \
     case G:                                                                                                      \
         hipLaunchKernelGGL((k_spmv<TV, T, G, DOT>), dim3(nb), dim3(RED_WG), 0, ctx->stream, w->nfree,          \
-                           ctx->d_node_blk, ctx->d_bcol, val, x, y, w->partials, w->sc);                        \
+                           node_blk, bcol, val, x, y, w->partials, w->sc);                                      \
         break;
     switch (group) {
         OFEA_SPMV_CASE(4)
@@ -926,6 +959,33 @@ static void launch_spmv_t(ofea_ctx *ctx, int group, const TV *val, const T *x, T
         OFEA_SPMV_CASE(64)
     }
 #undef OFEA_SPMV_CASE
+}
+
+template <class TV, class T, bool DOT>
+static void launch_spmv_t(ofea_ctx *ctx, int group, const TV *val, const T *x, T *y)
+{
+    launch_spmv_p<TV, T, DOT>(ctx, group, ctx->d_node_blk, ctx->d_bcol, val, x, y);
+}
+
+// The system a PCG run works on: the context's K and vectors, or the
+// reordered copy of the two-level methods.
+struct SysView {
+    const int *node_blk, *bcol;
+    const double *val64;  // FP64 values (true residual)
+    const double *dinv;   // point-Jacobi inverse diagonal
+    const double *b;      // right-hand side
+    double *x;            // solution
+};
+
+static SysView ctx_view(ofea_ctx *ctx)
+{
+    return {ctx->d_node_blk, ctx->d_bcol, ctx->d_val, ctx->ws->dinv, ctx->d_rhs, ctx->d_x};
+}
+
+static SysView perm_view(ofea_ctx *ctx)
+{
+    ofea_solver_ws *w = ctx->ws;
+    return {w->nb_p, w->bcol_p, w->val_p, w->dinv_p, w->b_p, w->x_p};
 }
 
 int solver_spmv(ofea_ctx *ctx, const double *d_x, double *d_y)
@@ -993,7 +1053,7 @@ int solver_prepare(ofea_ctx *ctx)
 
 // The graph of one batch: CG_BATCH iterations (2 kernels each) + batch end.
 template <class TV, class T, class XT, int PC, bool CO>
-static void cg_batch(ofea_ctx *ctx, const TV *val, VecSet<T> &v, XT *x, const T *minv)
+static void cg_batch(ofea_ctx *ctx, const SysView &S, const TV *val, VecSet<T> &v, XT *x, const T *minv)
 {
     ofea_solver_ws *w = ctx->ws;
     CGParts P{w->part_rz, w->part_rr, w->part_pq};
@@ -1001,7 +1061,7 @@ static void cg_batch(ofea_ctx *ctx, const TV *val, VecSet<T> &v, XT *x, const T 
 #define OFEA_CG_CASE(G)                                                                                          \
     case G:                                                                                                      \
         hipLaunchKernelGGL((k_cg_spmv<TV, T, G, CO>), dim3(CG_NB), dim3(CG_WG), 0, ctx->stream, bi, w->nfree, \
-                           ctx->d_node_blk, ctx->d_bcol, val, v.z, v.p, v.p2, v.q, P, w->co, w->sc);            \
+                           S.node_blk, S.bcol, val, v.z, v.p, v.p2, v.q, P, w->co, w->sc);                      \
         break;
         switch (w->spmv_group) {
             OFEA_CG_CASE(4)
@@ -1013,16 +1073,16 @@ static void cg_batch(ofea_ctx *ctx, const TV *val, VecSet<T> &v, XT *x, const T 
         }
 #undef OFEA_CG_CASE
         hipLaunchKernelGGL((k_cg_update<T, XT, PC, CO>), dim3(CG_NB), dim3(UP_WG), 0, ctx->stream, bi, w->nfree, x,
-                           v.r, v.z, v.p, v.p2, v.q, w->dinv, minv, P, w->co, w->sc);
+                           v.r, v.z, v.p, v.p2, v.q, S.dinv, minv, P, w->co, w->sc);
     }
     hipLaunchKernelGGL(k_cg_batch_end, dim3(1), dim3(64), 0, ctx->stream, w->sc, CG_BATCH);
 }
 
 // true FP64 residual r64 = b - K64 x; ||r64||^2 -> S_TRR, ||b||^2 -> S_BB
-static int true_residual(ofea_ctx *ctx, const double *b, const double *x)
+static int true_residual(ofea_ctx *ctx, const SysView &S, const double *b, const double *x)
 {
     ofea_solver_ws *w = ctx->ws;
-    launch_spmv_t<double, double, false>(ctx, w->spmv_group, ctx->d_val, x, w->y64);
+    launch_spmv_p<double, double, false>(ctx, w->spmv_group, S.node_blk, S.bcol, S.val64, x, w->y64);
     hipLaunchKernelGGL(k_residual, dim3(w->nblk_vec), dim3(RED_WG), 0, ctx->stream, w->n, b, w->y64, w->r64,
                        w->partials, w->sc);
     OFEA_HIP(hipGetLastError());
@@ -1049,12 +1109,12 @@ static int write_scalars(ofea_ctx *ctx)
 // ru_factor > 0 requests a residual replacement each time the recursive
 // residual has dropped by that factor since the last replacement.
 template <class TV, class T, int PC, bool CO>
-static int pcg_run(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T *minv, double rtol, int maxit,
-                   double ru_factor, int *iters, double *relres)
+static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, VecSet<T> &v, const T *minv,
+                   double rtol, int maxit, double ru_factor, int *iters, double *relres)
 {
     ofea_solver_ws *w = ctx->ws;
-    const double *b = ctx->d_rhs;
-    double *x = ctx->d_x;
+    const double *b = Sv.b;
+    double *x = Sv.x;
     const unsigned gv = w->nblk_vec;
     const int n = w->n;
     CGParts P{w->part_rz, w->part_rr, w->part_pq};
@@ -1062,7 +1122,7 @@ static int pcg_run(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T
     hipLaunchKernelGGL((k_fill<double>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, x, 0.0);
     hipLaunchKernelGGL((k_fill<T>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, v.p, T(0));
     hipLaunchKernelGGL((k_fill<T>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, v.p2, T(0));
-    int rc = true_residual(ctx, b, x);  // r64 = b and ||b||^2
+    int rc = true_residual(ctx, Sv, b, x);  // r64 = b and ||b||^2
     if (rc) return rc;
     if ((rc = read_scalars(ctx))) return rc;
     const double bb = S.s[S_BB];
@@ -1082,7 +1142,7 @@ static int pcg_run(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T
     S.s[S_RU2] = ru_factor > 0.0 ? ru_factor * ru_factor * bb : -1.0;
     if ((rc = write_scalars(ctx))) return rc;
     hipLaunchKernelGGL((k_replace<T, PC, CO>), dim3(CG_NB), dim3(UP_WG), 0, ctx->stream, w->nfree, w->r64, v.r, v.z,
-                       1, w->dinv, minv, P, w->co, w->sc);
+                       1, Sv.dinv, minv, P, w->co, w->sc);
     if (CO)
         hipLaunchKernelGGL((k_coarse<T>), dim3(w->co_nagg), dim3(CG_WG), 0, ctx->stream, w->nfree, w->co_nagg,
                            w->co_ainv, v.r, v.z, w->part_rz, w->co, w->sc);
@@ -1091,7 +1151,7 @@ static int pcg_run(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T
     static const bool no_graph = std::getenv("OFEA_NO_GRAPH") && std::getenv("OFEA_NO_GRAPH")[0] == '1';
     if (!w->exec[slot] && !no_graph) {
         OFEA_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        cg_batch<TV, T, double, PC, CO>(ctx, val, v, x, minv);
+        cg_batch<TV, T, double, PC, CO>(ctx, Sv, val, v, x, minv);
         OFEA_HIP(hipStreamEndCapture(ctx->stream, &w->graph[slot]));
         OFEA_HIP(hipGraphInstantiate(&w->exec[slot], w->graph[slot], nullptr, nullptr, 0));
     }
@@ -1103,7 +1163,7 @@ static int pcg_run(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T
     DevScalars *Hb = w->sc_host + 1;
     auto launch_batch = [&](int hb) -> int {
         if (no_graph) {
-            cg_batch<TV, T, double, PC, CO>(ctx, val, v, x, minv);
+            cg_batch<TV, T, double, PC, CO>(ctx, Sv, val, v, x, minv);
             OFEA_HIP(hipGetLastError());
         } else {
             OFEA_HIP(hipGraphLaunch(w->exec[slot], ctx->stream));
@@ -1131,7 +1191,7 @@ static int pcg_run(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T
         if (done == 3) return fail(OFEA_ERR_STATE, "PCG breakdown (matrix or preconditioner not SPD, or NaN)");
         const int it_done = S.it, par_done = S.par0 ^ (S.bi_done & 1);
         // true FP64 residual at convergence candidates, reliable-update points and maxit
-        if ((rc = true_residual(ctx, b, x))) return rc;
+        if ((rc = true_residual(ctx, Sv, b, x))) return rc;
         if ((rc = read_scalars(ctx))) return rc;
         rel = std::sqrt(S.s[S_TRR] / bb);
         if (rel <= rtol || done == 2) break;
@@ -1145,7 +1205,7 @@ static int pcg_run(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T
         S.s[S_RU2] = ru_factor > 0.0 ? ru_factor * ru_factor * S.s[S_TRR] : -1.0;
         if ((rc = write_scalars(ctx))) return rc;
         hipLaunchKernelGGL((k_replace<T, PC, CO>), dim3(CG_NB), dim3(UP_WG), 0, ctx->stream, w->nfree, w->r64, v.r,
-                           v.z, 0, w->dinv, minv, P, w->co, w->sc);
+                           v.z, 0, Sv.dinv, minv, P, w->co, w->sc);
         if (CO)
             hipLaunchKernelGGL((k_coarse<T>), dim3(w->co_nagg), dim3(CG_WG), 0, ctx->stream, w->nfree, w->co_nagg,
                                w->co_ainv, v.r, v.z, w->part_rz, w->co, w->sc);
@@ -1156,11 +1216,79 @@ static int pcg_run(ofea_ctx *ctx, int slot, const TV *val, VecSet<T> &v, const T
     return rel <= rtol ? OFEA_OK : OFEA_NOT_CONVERGED;
 }
 
+// Recursive coordinate bisection of b[0, m) into parts of exactly L nodes
+// (the last part may be shorter): split across the longest axis of the set's
+// bounding box after the first half of its parts; a part keeps its nodes in
+// ascending index order.
+static void rcb_parts(int *b, int m, int L, const double *X)
+{
+    const int parts = (m + L - 1) / L;
+    if (parts <= 1) {
+        std::sort(b, b + m);
+        return;
+    }
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < m; ++i)
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = std::min(lo[c], X[3 * (size_t)b[i] + c]);
+            hi[c] = std::max(hi[c], X[3 * (size_t)b[i] + c]);
+        }
+    int ax = 0;
+    for (int c = 1; c < 3; ++c)
+        if (hi[c] - lo[c] > hi[ax] - lo[ax]) ax = c;
+    const int lm = (parts / 2) * L;
+    std::nth_element(b, b + lm, b + m, [X, ax](int p, int q) {
+        const double xp = X[3 * (size_t)p + ax], xq = X[3 * (size_t)q + ax];
+        return xp < xq || (xp == xq && p < q);
+    });
+    rcb_parts(b, lm, L, X);
+    rcb_parts(b + lm, m - lm, L, X);
+}
+
+// Order of the free nodes for the two-level methods: compact parts of
+// exactly L nodes (the last part may be shorter), as slabs of whole parts
+// across the longest axis of the bounding box -- about as thick as a cubic
+// part, so A_c stays banded (a part couples only to parts of its own and the
+// neighbouring slabs) -- each slab cut into its parts by recursive coordinate
+// bisection.  perm[new] = old.
+static void coarse_order(int nf, const double *X, int L, std::vector<int> &perm)
+{
+    perm.resize(nf);
+    for (int i = 0; i < nf; ++i) perm[i] = i;
+    const int nparts = (nf + L - 1) / L;
+    if (nparts <= 1) return;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < nf; ++i)
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = std::min(lo[c], X[3 * (size_t)i + c]);
+            hi[c] = std::max(hi[c], X[3 * (size_t)i + c]);
+        }
+    int ax = 0;
+    for (int c = 1; c < 3; ++c)
+        if (hi[c] - lo[c] > hi[ax] - lo[ax]) ax = c;
+    const double emax = hi[ax] - lo[ax];
+    double vol = 1.0;
+    for (int c = 0; c < 3; ++c) vol *= std::max(hi[c] - lo[c], 1e-3 * emax);
+    const double edge = std::cbrt(vol / nparts);
+    const int slabs = std::max(1, std::min(nparts, (int)std::lround(emax / std::max(edge, 1e-300))));
+    const int per_slab = (nparts + slabs - 1) / slabs;  // parts per slab
+    std::sort(perm.begin(), perm.end(), [X, ax](int p, int q) {
+        const double xp = X[3 * (size_t)p + ax], xq = X[3 * (size_t)q + ax];
+        return xp < xq || (xp == xq && p < q);
+    });
+    const long long slab_nodes = (long long)per_slab * L;
+    for (long long s0 = 0; s0 < nf; s0 += slab_nodes)
+        rcb_parts(perm.data() + s0, (int)std::min<long long>(slab_nodes, nf - s0), L, X);
+}
+
 // Coarse space of the two-level methods, built on the host from the resident
-// K: aggregates of ra >= CG_AGG consecutive CG node ranges (at least 64
-// nodes each, so that the rotations stay well conditioned), Z = rigid-body modes
-// about each aggregate's centroid, A_c = Z^T K Z (FP64, 6 nagg square),
-// Cholesky-inverted and stored in FP32.
+// K.  The free nodes are reordered into compact parts of ra >= CG_AGG CG
+// node ranges (at least 64 nodes each, so that the rotations stay well
+// conditioned); in the new order an aggregate is ra consecutive CG ranges, so
+// the CG kernels form Z^T of a vector as per-range partials.  Z = rigid-body
+// modes about each aggregate's centroid, A_c = Z^T K Z (FP64, 6 nagg square),
+// Cholesky-inverted and stored in FP32.  The reordered copy of K, its
+// block-Jacobi inverses and FP32 values are built here as well.
 static int coarse_prepare(ofea_ctx *ctx)
 {
     ofea_solver_ws *w = ctx->ws;
@@ -1172,7 +1300,7 @@ static int coarse_prepare(ofea_ctx *ctx)
     const int ra = std::max(CG_AGG, (64 + per - 1) / per);
     const int nagg = (nranges + ra - 1) / ra;
     const int nc = 6 * nagg;
-    // free-node coordinates and aggregate centroids
+    // free-node coordinates, the new order and the aggregate centroids
     std::vector<double> X((size_t)3 * nf), cen((size_t)3 * nagg, 0.0);
     std::vector<int> cnt(nagg, 0);
     for (int i = 0; i < ctx->nd; ++i) {
@@ -1180,7 +1308,10 @@ static int coarse_prepare(ofea_ctx *ctx)
         if (r < 0) continue;
         for (int c = 0; c < 3; ++c) X[3 * (size_t)r + c] = ctx->xyz[3 * (size_t)i + c];
     }
-    auto agg = [&](int r) { return (r / per) / ra; };
+    std::vector<int> perm, iperm(nf);
+    coarse_order(nf, X.data(), per * ra, perm);
+    for (int i = 0; i < nf; ++i) iperm[perm[i]] = i;
+    auto agg = [&](int r) { return (iperm[r] / per) / ra; };  // aggregate of free node r (old numbering)
     for (int r = 0; r < nf; ++r) {
         const int a = agg(r);
         cnt[a]++;
@@ -1188,16 +1319,18 @@ static int coarse_prepare(ofea_ctx *ctx)
     }
     for (int a = 0; a < nagg; ++a)
         for (int c = 0; c < 3; ++c) cen[3 * a + c] /= std::max(cnt[a], 1);
-    std::vector<float> d((size_t)3 * nf);
-    std::vector<double> dd((size_t)3 * nf);
+    std::vector<float> d((size_t)3 * nf);     // new order, as the device reads it
+    std::vector<double> dd((size_t)3 * nf);   // old order
     for (int r = 0; r < nf; ++r)
         for (int c = 0; c < 3; ++c) {
             // the device applies Z with the FP32 offsets: build A_c from the same values
-            d[3 * (size_t)r + c] = (float)(X[3 * (size_t)r + c] - cen[3 * agg(r) + c]);
-            dd[3 * (size_t)r + c] = (double)d[3 * (size_t)r + c];
+            const float v = (float)(X[3 * (size_t)r + c] - cen[3 * agg(r) + c]);
+            d[3 * (size_t)iperm[r] + c] = v;
+            dd[3 * (size_t)r + c] = (double)v;
         }
     // A_c = sum over blocks K_sr (rows of s, columns of r) of Z_s^T K_sr Z_r, Z = [I | R], R = -[d]x
     std::vector<double> val((size_t)ctx->P.nnz);
+    OFEA_HIP(hipStreamSynchronize(ctx->stream));  // an assembly may still run on the context stream
     OFEA_HIP(hipMemcpy(val.data(), ctx->d_val, sizeof(double) * val.size(), hipMemcpyDeviceToHost));
     std::vector<double> A((size_t)nc * nc, 0.0);
     auto Rmat = [&](int node, double R[3][3]) {
@@ -1274,22 +1407,33 @@ static int coarse_prepare(ofea_ctx *ctx)
             L[(size_t)i * nc + j] = t / ljj;
         }
     }
-    // inverse = L^-T L^-1, column by column (column e of L^-1 is zero above e)
-    std::vector<double> inv((size_t)nc * nc), col(nc);
-    for (int e = 0; e < nc; ++e) {
-        std::fill(col.begin(), col.end(), 0.0);
-        for (int i = e; i < nc; ++i) {
-            double t = i == e ? 1.0 : 0.0;
-            for (int k = std::max(e, i - bw); k < i; ++k) t -= L[(size_t)i * nc + k] * col[k];
-            col[i] = t / L[(size_t)i * nc + i];
+    // inverse = L^-T L^-1, column by column (column e of L^-1 is zero above
+    // e); the columns are independent and split over host threads
+    std::vector<double> inv((size_t)nc * nc);
+    auto inv_columns = [&](int e0, int e1) {
+        std::vector<double> col(nc);
+        for (int e = e0; e < e1; ++e) {
+            std::fill(col.begin(), col.end(), 0.0);
+            for (int i = e; i < nc; ++i) {
+                double t = i == e ? 1.0 : 0.0;
+                for (int k = std::max(e, i - bw); k < i; ++k) t -= L[(size_t)i * nc + k] * col[k];
+                col[i] = t / L[(size_t)i * nc + i];
+            }
+            for (int i = nc - 1; i >= 0; --i) {
+                double t = col[i];
+                const int k1 = std::min(nc, i + bw + 1);
+                for (int k = i + 1; k < k1; ++k) t -= L[(size_t)k * nc + i] * col[k];
+                col[i] = t / L[(size_t)i * nc + i];
+            }
+            for (int i = 0; i < nc; ++i) inv[(size_t)i * nc + e] = col[i];
         }
-        for (int i = nc - 1; i >= 0; --i) {
-            double t = col[i];
-            const int k1 = std::min(nc, i + bw + 1);
-            for (int k = i + 1; k < k1; ++k) t -= L[(size_t)k * nc + i] * col[k];
-            col[i] = t / L[(size_t)i * nc + i];
-        }
-        for (int i = 0; i < nc; ++i) inv[(size_t)i * nc + e] = col[i];
+    };
+    {
+        const int nth = std::max(1, std::min<int>({16, (int)std::thread::hardware_concurrency(), nc / 64 + 1}));
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nth; ++t) pool.emplace_back(inv_columns, (int)((long long)nc * t / nth),
+                                                        (int)((long long)nc * (t + 1) / nth));
+        for (auto &th : pool) th.join();
     }
     std::vector<float> ainv((size_t)nc * nc);
     for (int i = 0; i < nc; ++i)
@@ -1310,6 +1454,47 @@ static int coarse_prepare(ofea_ctx *ctx)
     OFEA_HIP(hipMemset(w->co.ya, 0, sizeof(double) * 2 * CG_NC_MAX));
     OFEA_HIP(hipMemcpy((void *)w->co.d, d.data(), sizeof(float) * d.size(), hipMemcpyHostToDevice));
     OFEA_HIP(hipMemcpy(w->co_ainv, ainv.data(), sizeof(float) * ainv.size(), hipMemcpyHostToDevice));
+    // the reordered system: pattern renumbered on the host, values copied on the device
+    const long long nnz = ctx->P.nnz;
+    const int nblk = ctx->P.node_blk[nf];
+    const size_t nfb = (size_t)std::max(nf, 1);
+    if (!w->perm) {
+        OFEA_HIP(hipMalloc((void **)&w->perm, sizeof(int) * nfb));
+        OFEA_HIP(hipMalloc((void **)&w->iperm, sizeof(int) * nfb));
+        OFEA_HIP(hipMalloc((void **)&w->nb_p, sizeof(int) * (nfb + 1)));
+        OFEA_HIP(hipMalloc((void **)&w->bcol_p, sizeof(int) * (size_t)std::max(nblk, 1)));
+        OFEA_HIP(hipMalloc((void **)&w->val_p, sizeof(double) * (size_t)std::max(nnz, 1LL)));
+        OFEA_HIP(hipMalloc((void **)&w->val32_p, sizeof(float) * (size_t)std::max(nnz, 1LL)));
+        OFEA_HIP(hipMalloc((void **)&w->dinv_p, sizeof(double) * 3 * nfb));
+        OFEA_HIP(hipMalloc((void **)&w->b_p, sizeof(double) * 3 * nfb));
+        OFEA_HIP(hipMalloc((void **)&w->x_p, sizeof(double) * 3 * nfb));
+        for (double **p : {&w->minv_p, &w->dblk_p, &w->learned64_p})
+            OFEA_HIP(hipMalloc((void **)p, sizeof(double) * 9 * nfb));
+        OFEA_HIP(hipMalloc((void **)&w->minv32_p, sizeof(float) * 9 * nfb));
+    }
+    std::vector<int> nb_p((size_t)nf + 1, 0), bcol_p((size_t)std::max(nblk, 1));
+    for (int i = 0; i < nf; ++i) {
+        const int o = perm[i], o0 = ctx->P.node_blk[o], deg = ctx->P.node_blk[o + 1] - o0;
+        nb_p[i + 1] = nb_p[i] + deg;
+        for (int k = 0; k < deg; ++k) bcol_p[(size_t)nb_p[i] + k] = iperm[ctx->P.bcol[(size_t)o0 + k]];
+    }
+    if (nf) {
+        OFEA_HIP(hipMemcpyAsync(w->perm, perm.data(), sizeof(int) * nf, hipMemcpyHostToDevice, ctx->stream));
+        OFEA_HIP(hipMemcpyAsync(w->iperm, iperm.data(), sizeof(int) * nf, hipMemcpyHostToDevice, ctx->stream));
+        OFEA_HIP(hipMemcpyAsync(w->nb_p, nb_p.data(), sizeof(int) * (nf + 1), hipMemcpyHostToDevice, ctx->stream));
+        OFEA_HIP(hipMemcpyAsync(w->bcol_p, bcol_p.data(), sizeof(int) * (size_t)std::max(nblk, 1),
+                                hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_perm_vals, dim3((nf + RED_WG / 64 - 1) / (RED_WG / 64)), dim3(RED_WG), 0, ctx->stream, nf,
+                           w->perm, ctx->d_node_blk, w->nb_p, ctx->d_val, w->val_p);
+        hipLaunchKernelGGL(k_precond_setup, dim3(w->nblk_node), dim3(RED_WG), 0, ctx->stream, nf, w->nb_p, w->bcol_p,
+                           w->val_p, w->dinv_p, w->minv_p, w->minv32_p, w->dblk_p);
+    }
+    if (nnz)
+        hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((nnz + RED_WG - 1) / RED_WG)), dim3(RED_WG), 0,
+                           ctx->stream, nnz, w->val_p, w->val32_p);
+    OFEA_HIP(hipGetLastError());
+    OFEA_HIP(hipStreamSynchronize(ctx->stream));
+    w->iperm_host = std::move(iperm);
     w->co.per = per;
     w->co.ra = ra;
     w->co_nagg = nagg;
@@ -1330,11 +1515,12 @@ int coarse_apply(ofea_ctx *ctx, const double *r, double *out)
     if ((rc = coarse_prepare(ctx))) return rc;
     ofea_solver_ws *w = ctx->ws;
     const int nf = w->nfree, per = w->co.per, nagg = w->co_nagg, nc = 6 * nagg;
-    const float *d = w->co_d_host.data();
+    const float *d = w->co_d_host.data();  // offsets in the two-level node order
+    const int *ip = w->iperm_host.data();
     std::vector<double> g(nc, 0.0), y(nc, 0.0);
-    for (int i = 0; i < nf; ++i) {
-        const int a = (i / per) / w->co.ra;
-        const double dx = d[3 * i], dy = d[3 * i + 1], dz = d[3 * i + 2];
+    for (int i = 0; i < nf; ++i) {  // i: free node in the caller's numbering
+        const int j = ip[i], a = (j / per) / w->co.ra;
+        const double dx = d[3 * j], dy = d[3 * j + 1], dz = d[3 * j + 2];
         const double r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
         g[6 * a] += r0;
         g[6 * a + 1] += r1;
@@ -1349,13 +1535,26 @@ int coarse_apply(ofea_ctx *ctx, const double *r, double *out)
         y[i] = t;
     }
     for (int i = 0; i < nf; ++i) {
-        const double *ya = y.data() + 6 * ((i / per) / w->co.ra);
-        const double dx = d[3 * i], dy = d[3 * i + 1], dz = d[3 * i + 2];
+        const int j = ip[i];
+        const double *ya = y.data() + 6 * ((j / per) / w->co.ra);
+        const double dx = d[3 * j], dy = d[3 * j + 1], dz = d[3 * j + 2];
         out[3 * i] = ya[0] + ya[4] * dz - ya[5] * dy;
         out[3 * i + 1] = ya[1] + ya[5] * dx - ya[3] * dz;
         out[3 * i + 2] = ya[2] + ya[3] * dy - ya[4] * dx;
     }
     return OFEA_OK;
+}
+
+// aggregate of every free node (caller's numbering) of the two-level coarse
+// space for the resident K; returns the number of aggregates
+int coarse_aggregates(ofea_ctx *ctx, int *agg)
+{
+    int rc = solver_prepare(ctx);
+    if (rc) return rc;
+    if ((rc = coarse_prepare(ctx))) return rc;
+    ofea_solver_ws *w = ctx->ws;
+    for (int i = 0; i < w->nfree; ++i) agg[i] = (w->iperm_host[i] / w->co.per) / w->co.ra;
+    return w->co_nagg;
 }
 
 int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, double *relres)
@@ -1371,35 +1570,48 @@ int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, 
     const bool learned = method == OFEA_PCG_LEARNED || method == OFEA_PCG_LEARNED_MIXED ||
                          method == OFEA_PCG_LEARNED_TWO_LEVEL || method == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED;
     if (learned && !w->learned64) return fail(OFEA_ERR_STATE, "learned preconditioner not set (ofea_learned_*)");
-    if (method >= OFEA_PCG_TWO_LEVEL && method <= OFEA_PCG_LEARNED_TWO_LEVEL_MIXED && (rc = coarse_prepare(ctx)))
+    if (method >= OFEA_PCG_TWO_LEVEL && method <= OFEA_PCG_LEARNED_TWO_LEVEL_MIXED) {
+        // two-level: the reordered system (b in, x out through the permutation)
+        if ((rc = coarse_prepare(ctx))) return rc;
+        const int nf = w->nfree;
+        const unsigned g3 = (unsigned)((3LL * nf + RED_WG - 1) / RED_WG), g9 = (unsigned)((9LL * nf + RED_WG - 1) / RED_WG);
+        hipLaunchKernelGGL(k_gather_nodes<3>, dim3(g3), dim3(RED_WG), 0, ctx->stream, nf, w->perm, ctx->d_rhs, w->b_p);
+        if (learned)  // the learned blocks may have been refreshed since the last solve
+            hipLaunchKernelGGL(k_gather_nodes<9>, dim3(g9), dim3(RED_WG), 0, ctx->stream, nf, w->perm, w->learned64,
+                               w->learned64_p);
+        OFEA_HIP(hipGetLastError());
+        const SysView S = perm_view(ctx);
+        const double *mb = learned ? w->learned64_p : w->minv_p;
+        if (method == OFEA_PCG_TWO_LEVEL || method == OFEA_PCG_LEARNED_TWO_LEVEL)
+            rc = pcg_run<double, double, 1, true>(ctx, S, method == OFEA_PCG_TWO_LEVEL ? 6 : 8, w->val_p, w->v64, mb,
+                                                  rtol, maxit, -1.0, iters, relres);
+        else
+            rc = pcg_run<float, double, 1, true>(ctx, S, method == OFEA_PCG_TWO_LEVEL_MIXED ? 7 : 9, w->val32_p,
+                                                 w->v64, mb, rtol, maxit, 0.1, iters, relres);
+        if (rc != OFEA_OK && rc != OFEA_NOT_CONVERGED) return rc;
+        hipLaunchKernelGGL(k_gather_nodes<3>, dim3(g3), dim3(RED_WG), 0, ctx->stream, nf, w->iperm, w->x_p, ctx->d_x);
+        OFEA_HIP(hipGetLastError());
         return rc;
+    }
+    const SysView S = ctx_view(ctx);
     switch (method) {
     case OFEA_PCG_JACOBI:
-        return pcg_run<double, double, 0, false>(ctx, 0, ctx->d_val, w->v64, w->minv, rtol, maxit, -1.0, iters,
+        return pcg_run<double, double, 0, false>(ctx, S, 0, ctx->d_val, w->v64, w->minv, rtol, maxit, -1.0, iters,
                                                  relres);
     case OFEA_PCG_BLOCK_JACOBI:
-        return pcg_run<double, double, 1, false>(ctx, 1, ctx->d_val, w->v64, w->minv, rtol, maxit, -1.0, iters,
+        return pcg_run<double, double, 1, false>(ctx, S, 1, ctx->d_val, w->v64, w->minv, rtol, maxit, -1.0, iters,
                                                  relres);
     case OFEA_PCG_MIXED:
-        return pcg_run<float, double, 1, false>(ctx, 2, w->val32, w->v64, w->minv, rtol, maxit, 0.1, iters, relres);
+        return pcg_run<float, double, 1, false>(ctx, S, 2, w->val32, w->v64, w->minv, rtol, maxit, 0.1, iters,
+                                                relres);
     case OFEA_PCG_LEARNED:
-        return pcg_run<double, double, 1, false>(ctx, 3, ctx->d_val, w->v64, w->learned64, rtol, maxit, -1.0, iters,
-                                                 relres);
+        return pcg_run<double, double, 1, false>(ctx, S, 3, ctx->d_val, w->v64, w->learned64, rtol, maxit, -1.0,
+                                                 iters, relres);
     case OFEA_PCG_LEARNED_MIXED:
-        return pcg_run<float, double, 1, false>(ctx, 5, w->val32, w->v64, w->learned64, rtol, maxit, 0.1, iters,
+        return pcg_run<float, double, 1, false>(ctx, S, 5, w->val32, w->v64, w->learned64, rtol, maxit, 0.1, iters,
                                                 relres);
     case OFEA_PCG_MIXED32:
-        return pcg_run<float, float, 1, false>(ctx, 4, w->val32, w->v32, w->minv32, rtol, maxit, 0.1, iters, relres);
-    case OFEA_PCG_TWO_LEVEL:
-        return pcg_run<double, double, 1, true>(ctx, 6, ctx->d_val, w->v64, w->minv, rtol, maxit, -1.0, iters,
-                                                relres);
-    case OFEA_PCG_TWO_LEVEL_MIXED:
-        return pcg_run<float, double, 1, true>(ctx, 7, w->val32, w->v64, w->minv, rtol, maxit, 0.1, iters, relres);
-    case OFEA_PCG_LEARNED_TWO_LEVEL:
-        return pcg_run<double, double, 1, true>(ctx, 8, ctx->d_val, w->v64, w->learned64, rtol, maxit, -1.0, iters,
-                                                relres);
-    case OFEA_PCG_LEARNED_TWO_LEVEL_MIXED:
-        return pcg_run<float, double, 1, true>(ctx, 9, w->val32, w->v64, w->learned64, rtol, maxit, 0.1, iters,
+        return pcg_run<float, float, 1, false>(ctx, S, 4, w->val32, w->v32, w->minv32, rtol, maxit, 0.1, iters,
                                                relres);
     default:
         return fail(OFEA_ERR_ARG, "unknown solver method");

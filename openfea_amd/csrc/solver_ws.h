@@ -79,6 +79,17 @@ struct ofea_solver_ws {
     bool co_valid = false;
     double co_setup_ms = 0.0;
     std::vector<float> co_d_host, co_ainv_host;  // host copies for ofea_coarse_apply
+    // The two-level methods run on the free nodes reordered into compact parts
+    // (one part = one aggregate = ra consecutive CG ranges of the new order),
+    // on a copy of K whose node rows follow that order.
+    int *perm = nullptr, *iperm = nullptr;  // nfree: new -> old, old -> new
+    int *nb_p = nullptr, *bcol_p = nullptr;  // node_blk and bcol in the new order
+    double *val_p = nullptr;                 // K values, node rows in the new order
+    float *val32_p = nullptr;
+    double *dinv_p = nullptr, *minv_p = nullptr, *dblk_p = nullptr, *learned64_p = nullptr;
+    float *minv32_p = nullptr;
+    double *b_p = nullptr, *x_p = nullptr;  // right-hand side and solution in the new order
+    std::vector<int> iperm_host;
     hipGraph_t graph[ofea::NSLOT] = {};
     hipGraphExec_t exec[ofea::NSLOT] = {};
 };

@@ -56,6 +56,15 @@ def coarse_apply(ctx, r):
     return out
 
 
+def coarse_aggregates(ctx):
+    """Aggregate of every free node of the two-level coarse space (and their count)."""
+    agg = np.zeros(ctx.n // 3, dtype=np.int32)
+    nagg = lib().ofea_coarse_aggregates(ctx._h, ptr(agg))
+    if nagg < 0:
+        check(nagg)
+    return agg, nagg
+
+
 class Learned:
     """Handle on the learned preconditioner of one context."""
 

@@ -121,10 +121,13 @@ def test_pcg_cfg1_matches_direct_solve(ctx, method):
 
 def test_coarse_correction_matches_numpy(ctx):
     """Z A_c^-1 Z^T r of the two-level methods against an independent numpy
-    construction: aggregates of max(2, ceil(64/per)) consecutive CG node
-    ranges (512 ranges of per = ceil(nfree/512) nodes), rigid-body modes about each aggregate's centroid,
-    A_c = Z^T K Z.  The library inverts A_c in FP64 and applies it in FP32:
-    bar 1e-5 relative."""
+    construction from the library's aggregates: rigid-body modes about each
+    aggregate's centroid, A_c = Z^T K Z.  The aggregates hold exactly
+    per * ra nodes (per = ceil(nfree/512) nodes per CG range, ra =
+    max(2, ceil(64/per)) ranges) except the last.  The library inverts A_c in
+    FP64 and applies it in FP32: bar 1e-5 relative."""
+    from openfea_amd import learned as L
+
     m, resort, F = mg.cantilever_c3d8(12, 4, 4)
     ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
     cp, ri = ctx.pattern()
@@ -135,8 +138,9 @@ def test_coarse_correction_matches_numpy(ctx):
     X[resort[resort >= 0]] = m.xyz[resort >= 0]
     per = -(-nf // 512)
     ra = max(2, -(-64 // per))  # ranges per aggregate: at least 2 and 64 nodes
-    agg = (np.arange(nf) // per) // ra
-    nagg = agg.max() + 1
+    agg, nagg = L.coarse_aggregates(ctx)
+    counts = np.bincount(agg, minlength=nagg)
+    assert nagg == -(-nf // (per * ra)) and np.all(counts[:-1] == per * ra) and 0 < counts[-1] <= per * ra
     cen = np.stack([np.bincount(agg, weights=X[:, c]) / np.bincount(agg) for c in range(3)], axis=1)
     d = (X - cen[agg]).astype(np.float32).astype(np.float64)
     Z = np.zeros((ctx.n, 6 * nagg))
@@ -149,9 +153,33 @@ def test_coarse_correction_matches_numpy(ctx):
     # a rotation about the line of a collinear aggregate is a zero column of Z:
     # A_c is then singular and that component of y does not enter Z y
     ref = Z @ np.linalg.lstsq(Ac, Z.T @ r, rcond=1e-13)[0]
-    from openfea_amd import learned as L
     out = L.coarse_apply(ctx, r)
     assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-5
+
+
+def test_coarse_aggregates_compact_and_effective(ctx):
+    """The two-level aggregates are compact parts of the node cloud (their
+    bounding boxes are not much longer than wide, where consecutive node
+    ranges would give strips through the whole block) and the coarse space
+    cuts the block-Jacobi iteration count several times over."""
+    from openfea_amd import learned as L
+
+    m, resort, F = mg.cantilever_c3d4(24, 12, 12)
+    say("two-level aggregates, C3D4 24x12x12")
+    ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
+    ctx.assemble(mg.E_STEEL, mg.NU_STEEL)
+    agg, nagg = L.coarse_aggregates(ctx)
+    X = np.zeros((ctx.n // 3, 3))
+    X[resort[resort >= 0]] = m.xyz[resort >= 0]
+    aspect = []
+    for a in range(nagg - 1):
+        ext = np.ptp(X[agg == a], axis=0) + 1.0  # +h: a one-layer part has thickness h
+        aspect.append(ext.max() / ext.min())
+    assert np.median(aspect) <= 2.5, np.median(aspect)
+    _, it_bj, _, ok1 = ctx.solve(F, method=A.PCG_BLOCK_JACOBI, rtol=1e-8)
+    _, it_tl, _, ok2 = ctx.solve(F, method=A.PCG_TWO_LEVEL, rtol=1e-8)
+    say(f"block Jacobi {it_bj} it, two-level {it_tl} it, {nagg} aggregates")
+    assert ok1 and ok2 and it_tl * 4 <= it_bj, (it_bj, it_tl)
 
 
 def test_two_level_tet_and_rhs_reuse(ctx):
