@@ -227,8 +227,13 @@ int ofea_nl_assemble(ofea_ctx *ctx, double E, double nu, const double *u, double
  * extended with load stepping: the load F (n) is ramped in n_incr equal
  * increments; each increment iterates K_T du = lam F - f_int, u += du until
  * ||lam F - f_int||_2 <= tol ||lam F||_2 (FP64) or max_newton iterations.
- * Linear solves: ofea_solve method with inner_rtol; a learned method falls
- * back to block-Jacobi PCG when it fails.  u_out (n), counters nullable.
+ * Linear solves: ofea_solve method with inner_rtol.  A learned method is
+ * judged by the same solver with plain block Jacobi: on the first linear
+ * solve the plain solver runs, then the learned one limited to the plain
+ * iteration count, and the learned method serves the rest of the run only if
+ * it converged in fewer iterations; the plain solver also takes over whenever
+ * the learned one fails (linear_total counts every PCG iteration).  u_out
+ * (n), counters nullable.
  * Returns OFEA_OK or OFEA_NOT_CONVERGED. */
 int ofea_nl_solve(ofea_ctx *ctx, double E, double nu, const double *F, int n_incr, double tol, int max_newton,
                   int method, double inner_rtol, double *u_out, int *newton_iters, int *linear_iters,

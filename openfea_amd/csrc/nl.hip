@@ -430,26 +430,44 @@ int learned_refresh(ofea_ctx *c);
 // K_T du = r, u += du, stop at ||r||_2/(3n) < 1e-6 or 100 loops).  Here the
 // load is ramped in n_incr increments lam_k = k/n_incr and each increment
 // iterates until ||lam F - f_int|| <= tol ||lam F|| (FP64 residual) or
-// max_newton iterations; the linear solves are GPU PCG (method, inner_rtol),
-// with block-Jacobi PCG as the fallback of the learned method.
+// max_newton iterations; the linear solves are GPU PCG (method, inner_rtol).
+// A learned method is judged by the same solver with plain block Jacobi: on
+// the first linear solve the plain solver runs, then the learned one limited
+// to the plain iteration count; the learned method serves the rest of the run
+// only if it converged in fewer iterations, and the plain solver takes over
+// whenever it fails later.
 int nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr, double tol, int max_newton, int method,
              double inner_rtol, double *u_out, int *newton_total, int *linear_total, double *relres)
 {
     const int n = c->P.n;
     int rc = nl_buffers(c);
     if (rc) return rc;
-    double *dF = nullptr, *part = nullptr;
+    double *dF = nullptr, *part = nullptr, *xs = nullptr;
     const int nb = (n + NLR_WG - 1) / NLR_WG;
     OFEA_HIP(hipMalloc((void **)&dF, sizeof(double) * std::max(n, 1)));
     OFEA_HIP(hipMalloc((void **)&part, sizeof(double) * 2 * std::max(nb, 1)));
+    OFEA_HIP(hipMalloc((void **)&xs, sizeof(double) * std::max(n, 1)));  // plain solution while the learned one is judged
     std::vector<double> hp(2 * std::max(nb, 1));
     auto cleanup = [&]() {
         (void)hipFree(dF);
         (void)hipFree(part);
+        (void)hipFree(xs);
     };
     OFEA_HIP(hipMemcpyAsync(dF, F, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     OFEA_HIP(hipMemsetAsync(c->d_u, 0, sizeof(double) * std::max(n, 1), c->stream));
+    auto is_learned = [](int m) {
+        return m == OFEA_PCG_LEARNED || m == OFEA_PCG_LEARNED_MIXED || m == OFEA_PCG_LEARNED_TWO_LEVEL ||
+               m == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED;
+    };
+    auto plain_of = [](int m) {  // the same solver with block Jacobi in place of the learned blocks
+        return m == OFEA_PCG_LEARNED_MIXED        ? OFEA_PCG_MIXED
+               : m == OFEA_PCG_LEARNED_TWO_LEVEL  ? OFEA_PCG_TWO_LEVEL
+               : m == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED ? OFEA_PCG_TWO_LEVEL_MIXED
+                                                       : OFEA_PCG_BLOCK_JACOBI;
+    };
     int newton = 0, linear = 0, fallbacks = 0;
+    int use = method;
+    bool judged = !is_learned(method);
     double rel = 0.0;
     for (int k = 1; k <= n_incr; ++k) {
         const double lam = (double)k / n_incr;
@@ -478,17 +496,47 @@ int nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr, doub
             c->rhs_set = true;
             int lit = 0;
             double lrel = 0.0;
-            int m = method;
-            if (m == OFEA_PCG_LEARNED || m == OFEA_PCG_LEARNED_MIXED || m == OFEA_PCG_LEARNED_TWO_LEVEL ||
-                m == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED) {
-                if (!c->learned || learned_refresh(c) != OFEA_OK) m = OFEA_PCG_BLOCK_JACOBI;
-            }
-            int src = solver_solve(c, m, inner_rtol, 20000, &lit, &lrel);
-            linear += lit;
-            if (src != OFEA_OK && m != OFEA_PCG_BLOCK_JACOBI) {  // learned path failed: plain PCG judges
-                fallbacks++;
-                src = solver_solve(c, OFEA_PCG_BLOCK_JACOBI, inner_rtol, 20000, &lit, &lrel);
+            int m = use;
+            if (is_learned(m) && (!c->learned || learned_refresh(c) != OFEA_OK)) m = use = plain_of(m);
+            int src;
+            if (is_learned(m) && !judged) {
+                // first solve: the plain solver, then the learned one with the
+                // plain iteration count as its limit; the learned method serves
+                // the rest of the run only if it converged in fewer iterations
+                judged = true;
+                int pit = 0;
+                src = solver_solve(c, plain_of(m), inner_rtol, 20000, &pit, &lrel);
+                linear += pit;
+                if (src == OFEA_OK) {
+                    if (hipMemcpyAsync(xs, c->d_x, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream) !=
+                        hipSuccess) {
+                        rc = fail(OFEA_ERR_HIP, "solution copy failed");
+                        break;
+                    }
+                    const int lsrc = solver_solve(c, m, inner_rtol, pit, &lit, &lrel);
+                    if (lsrc < 0 && lsrc != OFEA_ERR_STATE) {  // OFEA_ERR_STATE: breakdown of the learned PCG
+                        src = lsrc;
+                    } else {
+                        linear += lit;
+                        if (lsrc != OFEA_OK || lit >= pit) {
+                            use = plain_of(m);
+                            if (hipMemcpyAsync(c->d_x, xs, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                               c->stream) != hipSuccess) {
+                                rc = fail(OFEA_ERR_HIP, "solution copy failed");
+                                break;
+                            }
+                        }
+                    }
+                }
+            } else {
+                src = solver_solve(c, m, inner_rtol, 20000, &lit, &lrel);
                 linear += lit;
+                if (src != OFEA_OK && is_learned(m)) {  // learned path failed: the plain solver takes over
+                    fallbacks++;
+                    use = plain_of(m);
+                    src = solver_solve(c, use, inner_rtol, 20000, &lit, &lrel);
+                    linear += lit;
+                }
             }
             if (src < 0) {
                 rc = src;

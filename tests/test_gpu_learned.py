@@ -14,6 +14,7 @@ import scipy.sparse.linalg as spla
 
 from conftest import rel_max
 from oracle import learned_oracle as LO
+from oracle import oracle as O
 import openfea_amd as ofa
 from openfea_amd import _capi as A, learned as L, meshgen as mg
 
@@ -134,3 +135,26 @@ def test_bf16_inference_on_matrix_cores(setup):
     assert ok and np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
     ln.precision(A.FP32)
     assert rel_max(ln.blocks()[0], b32) == 0.0
+
+
+def test_newton_learned_judged_by_block_jacobi(setup):
+    """Newton with a learned method whose network is far from block Jacobi
+    (large random head): plain block-Jacobi PCG judges it on the first linear
+    solve and serves the rest of the run, so the total PCG count stays near
+    the plain run's and the solution equals the oracle's Newton solution."""
+    ctx, g, F, cp, ri, val = setup
+    m, resort, _ = mg.cantilever_c3d8(6, 3, 3)
+    say("newton: learned method judged by block Jacobi")
+    load = 40 * F
+    u_bj, nt_bj, lt_bj, rel_bj, ok_bj = ctx.nl_solve(mg.E_STEEL, mg.NU_STEEL, load, n_incr=3, tol=1e-10,
+                                                     max_newton=20, method=A.PCG_BLOCK_JACOBI, inner_rtol=1e-11)
+    ln = L.Learned(ctx, seed=3)
+    ln.set_params(random_params(5, head=3.0))
+    u_l, nt_l, lt_l, rel_l, ok_l = ctx.nl_solve(mg.E_STEEL, mg.NU_STEEL, load, n_incr=3, tol=1e-10,
+                                                max_newton=20, method=A.PCG_LEARNED, inner_rtol=1e-11)
+    uo, _ = O.nl_newton(m.conn, m.etype, m.xyz, resort, mg.E_STEEL, mg.NU_STEEL, load, cp, ri, n_incr=3, tol=1e-10)
+    say(f"PCG iterations: block Jacobi {lt_bj}, learned (judged) {lt_l}; Newton {nt_bj} / {nt_l}")
+    assert ok_bj and ok_l and rel_l <= 1e-10
+    assert np.linalg.norm(u_l - uo) / np.linalg.norm(uo) < 1e-8
+    assert lt_l <= 2 * lt_bj, (lt_l, lt_bj)
+    ctx.assemble(mg.E_STEEL, mg.NU_STEEL)  # leave the linear K resident for later tests
