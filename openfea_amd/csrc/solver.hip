@@ -919,7 +919,9 @@ void solver_free(ofea_ctx *ctx)
         if (p) (void)hipFree(p);
     for (void *p : {(void *)w->perm, (void *)w->iperm, (void *)w->nb_p, (void *)w->bcol_p, (void *)w->val_p,
                     (void *)w->val32_p, (void *)w->dinv_p, (void *)w->minv_p, (void *)w->dblk_p,
-                    (void *)w->learned64_p, (void *)w->minv32_p, (void *)w->b_p, (void *)w->x_p})
+                    (void *)w->learned64_p, (void *)w->minv32_p, (void *)w->b_p, (void *)w->x_p, (void *)w->co_A,
+                    (void *)w->co_inv, (void *)w->co_flag, (void *)w->sl_ptr, (void *)w->sl_part, (void *)w->sl_bptr,
+                    (void *)w->sl_blk})
         if (p) (void)hipFree(p);
     if (w->sc) (void)hipFree(w->sc);
     if (w->sc_host) (void)hipHostFree(w->sc_host);
@@ -1281,25 +1283,183 @@ static void coarse_order(int nf, const double *X, int L, std::vector<int> &perm)
         rcb_parts(perm.data() + s0, (int)std::min<long long>(slab_nodes, nf - s0), L, X);
 }
 
-// Coarse space of the two-level methods, built on the host from the resident
-// K.  The free nodes are reordered into compact parts of ra >= CG_AGG CG
-// node ranges (at least 64 nodes each, so that the rotations stay well
-// conditioned); in the new order an aggregate is ra consecutive CG ranges, so
-// the CG kernels form Z^T of a vector as per-range partials.  Z = rigid-body
-// modes about each aggregate's centroid, A_c = Z^T K Z (FP64, 6 nagg square),
-// Cholesky-inverted and stored in FP32.  The reordered copy of K, its
-// block-Jacobi inverses and FP32 values are built here as well.
-static int coarse_prepare(ofea_ctx *ctx)
+// ---- coarse-space kernels (device set-up of A_c^-1 for each K) ----
+// column q of Z at a node with offset dv from its aggregate centroid:
+// translations e_q (q < 3), rotations R e_(q-3) with R theta = theta x d
+__device__ inline void zcol(const float *dv, int q, double z[3])
+{
+    const double x = dv[0], y = dv[1], w = dv[2];
+    z[0] = q == 0 ? 1.0 : q == 4 ? w : q == 5 ? -y : 0.0;
+    z[1] = q == 1 ? 1.0 : q == 3 ? -w : q == 5 ? x : 0.0;
+    z[2] = q == 2 ? 1.0 : q == 3 ? y : q == 4 ? -x : 0.0;
+}
+
+// A_c = Z^T K Z on the reordered K: workgroup b owns the column blocks of
+// aggregate b.  The blocks (r, k) of b's nodes are listed per coupled
+// aggregate a ("slot"); one thread per (slot, p, q) sums its 6x6 entry over
+// the slot's blocks in list order and writes A[6a+p][6b+q] (no atomics).
+__global__ __launch_bounds__(1024) void k_coarse_ac(int nc, const int *__restrict__ nb_p,
+                                                    const int *__restrict__ bcol_p, const double *__restrict__ val_p,
+                                                    const float *__restrict__ d, const int *__restrict__ sl_ptr,
+                                                    const int *__restrict__ sl_part, const int *__restrict__ sl_bptr,
+                                                    const int *__restrict__ sl_blk, double *__restrict__ A)
+{
+    const int b = blockIdx.x;
+    const int s0 = sl_ptr[b], ns = sl_ptr[b + 1] - s0;
+    for (int t = threadIdx.x; t < 36 * ns; t += blockDim.x) {
+        const int s = s0 + t / 36, pq = t % 36, p = pq / 6, q = pq % 6;
+        double acc = 0.0;
+        for (int e = sl_bptr[s]; e < sl_bptr[s + 1]; ++e) {
+            const int rk = sl_blk[e], r = rk >> 8, k = rk & 255;
+            const int nb0 = nb_p[r], deg = nb_p[r + 1] - nb0;
+            double zs[3], zr[3];
+            zcol(d + 3 * (size_t)bcol_p[nb0 + k], p, zs);
+            zcol(d + 3 * (size_t)r, q, zr);
+            const double *blk = val_p + 9LL * nb0 + 3 * k;  // blk[c * 3 deg + m] = K(3s + m, 3r + c)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int m = 0; m < 3; ++m) acc += zs[m] * blk[c * 3 * deg + m] * zr[c];
+        }
+        A[(size_t)(6 * sl_part[s] + p) * nc + 6 * b + q] = acc;
+    }
+}
+
+// In-place banded Cholesky of A (nc x nc row-major, half-bandwidth bw) in one
+// workgroup: the band is symmetrised (lower := mean of both triangles) and
+// the diagonal lifted by 1e-12 tr(A)/nc (a Z column that vanishes, e.g. a
+// rotation of a collinear aggregate, leaves a zero row), then column j is
+// factored and the trailing band updated, j = 0 .. nc-1.  Afterwards the lower
+// triangle holds L and the upper triangle L^T.  *flag = 0, or j + 1 when the
+// pivot of column j is not positive.
+constexpr int CH_WG = 1024;
+__global__ __launch_bounds__(CH_WG) void k_band_cholesky(int nc, int bw, double *__restrict__ A, int *__restrict__ flag)
+{
+    __shared__ double s_l[CG_NC_MAX];
+    __shared__ double s_red[CH_WG / 64];
+    __shared__ double s_piv;
+    __shared__ int s_bad;
+    const int tid = threadIdx.x;
+    double tr = 0.0;
+    for (int i = tid; i < nc; i += CH_WG) tr += A[(size_t)i * nc + i];
+    for (int o = 32; o > 0; o >>= 1) tr += __shfl_xor(tr, o);
+    if ((tid & 63) == 0) s_red[tid >> 6] = tr;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < CH_WG / 64; ++w) t += s_red[w];
+        s_piv = 1e-12 * t / nc;
+    }
+    __syncthreads();
+    const double lift = s_piv;
+    const long long band = (long long)nc * (bw + 1);
+    for (long long t = tid; t < band; t += CH_WG) {
+        const int j = (int)(t / (bw + 1)), u = (int)(t - (long long)j * (bw + 1)), i = j + u;
+        if (i >= nc) continue;
+        if (u == 0)
+            A[(size_t)j * nc + j] += lift;
+        else
+            A[(size_t)i * nc + j] = 0.5 * (A[(size_t)i * nc + j] + A[(size_t)j * nc + i]);
+    }
+    __syncthreads();
+    for (int j = 0; j < nc; ++j) {
+        const int m = min(nc - 1, j + bw) - j;  // band rows below the diagonal
+        if (tid == 0) {
+            const double dj = A[(size_t)j * nc + j];
+            s_bad = !(dj > 0.0);
+            s_piv = s_bad ? 1.0 : sqrt(dj);
+            A[(size_t)j * nc + j] = s_piv;
+        }
+        __syncthreads();
+        if (s_bad) {
+            if (tid == 0) *flag = j + 1;
+            return;
+        }
+        const double piv = s_piv;
+        for (int u = tid; u < m; u += CH_WG) {
+            const double l = A[(size_t)(j + 1 + u) * nc + j] / piv;
+            A[(size_t)(j + 1 + u) * nc + j] = l;
+            s_l[u] = l;
+        }
+        __syncthreads();
+        for (int t = tid; t < m * m; t += CH_WG) {
+            const int u = t / m, v = t - u * m;
+            if (v <= u) A[(size_t)(j + 1 + u) * nc + j + 1 + v] -= s_l[u] * s_l[v];
+        }
+        __syncthreads();
+    }
+    for (long long t = tid; t < band; t += CH_WG) {
+        const int j = (int)(t / (bw + 1)), u = (int)(t - (long long)j * (bw + 1)), i = j + u;
+        if (u > 0 && i < nc) A[(size_t)j * nc + i] = A[(size_t)i * nc + j];
+    }
+    if (tid == 0) *flag = 0;
+}
+
+// Column e of A^-1 = L^-T L^-1 e_e by one wave: forward then backward
+// substitution over the band (lane-split dot products, shuffle sums), written
+// as row e of inv (A^-1 is symmetric).
+constexpr int INV_WAVES = 4;
+__global__ __launch_bounds__(64 * INV_WAVES) void k_band_inverse(int nc, int bw, const double *__restrict__ L,
+                                                                 double *__restrict__ inv)
+{
+    __shared__ double s_y[INV_WAVES][CG_NC_MAX];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int e = blockIdx.x * INV_WAVES + w;
+    if (e >= nc) return;  // whole waves; no workgroup barrier below
+    double *y = s_y[w];
+    for (int i = lane; i < e; i += 64) y[i] = 0.0;
+    __builtin_amdgcn_wave_barrier();
+    for (int i = e; i < nc; ++i) {  // L y = e_e
+        const int lo = max(e, i - bw);
+        double part = 0.0;
+        for (int k = lo + lane; k < i; k += 64) part += L[(size_t)i * nc + k] * y[k];
+        for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+        const double yi = ((i == e ? 1.0 : 0.0) - part) / L[(size_t)i * nc + i];
+        if (lane == 0) y[i] = yi;
+        __builtin_amdgcn_wave_barrier();
+    }
+    for (int i = nc - 1; i >= 0; --i) {  // L^T x = y (L^T is the upper triangle)
+        const int hi = min(nc - 1, i + bw);
+        double part = 0.0;
+        for (int k = i + 1 + lane; k <= hi; k += 64) part += L[(size_t)i * nc + k] * y[k];
+        for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+        const double xi = (y[i] - part) / L[(size_t)i * nc + i];
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) y[i] = xi;
+        __builtin_amdgcn_wave_barrier();
+    }
+    for (int i = lane; i < nc; i += 64) inv[(size_t)e * nc + i] = y[i];
+}
+
+// symmetric FP32 copy of the inverse
+__global__ __launch_bounds__(RED_WG) void k_sym32(int nc, const double *__restrict__ inv, float *__restrict__ out)
+{
+    const long long t = (long long)blockIdx.x * RED_WG + threadIdx.x;
+    if (t >= (long long)nc * nc) return;
+    const int i = (int)(t / nc), j = (int)(t - (long long)i * nc);
+    out[t] = (float)(0.5 * (inv[(size_t)i * nc + j] + inv[(size_t)j * nc + i]));
+}
+
+// Coarse space of the two-level methods.  The free nodes are reordered into
+// compact parts of ra >= CG_AGG CG node ranges (at least 64 nodes each, so
+// that the rotations stay well conditioned); in the new order an aggregate is
+// ra consecutive CG ranges, so the CG kernels form Z^T of a vector as
+// per-range partials.  Z = rigid-body modes about each aggregate's centroid,
+// A_c = Z^T K Z (FP64, 6 nagg square), Cholesky-inverted and stored in FP32.
+// coarse_structure (once per mesh and pattern, host): the order, the
+// renumbered pattern, the offsets d, the per-aggregate block lists of A_c and
+// its bandwidth.  coarse_values (once per K, device): the reordered values,
+// their block-Jacobi inverses and FP32 copy, A_c, its Cholesky factor and the
+// inverse.
+static int coarse_structure(ofea_ctx *ctx)
 {
     ofea_solver_ws *w = ctx->ws;
-    if (w->co_valid) return OFEA_OK;
-    const auto t0 = std::chrono::steady_clock::now();
     const int nf = w->nfree;
     const int per = (nf + CG_NB - 1) / CG_NB;
     const int nranges = (nf + per - 1) / per;
     const int ra = std::max(CG_AGG, (64 + per - 1) / per);
     const int nagg = (nranges + ra - 1) / ra;
-    const int nc = 6 * nagg;
+    const int L = per * ra;
     // free-node coordinates, the new order and the aggregate centroids
     std::vector<double> X((size_t)3 * nf), cen((size_t)3 * nagg, 0.0);
     std::vector<int> cnt(nagg, 0);
@@ -1309,156 +1469,80 @@ static int coarse_prepare(ofea_ctx *ctx)
         for (int c = 0; c < 3; ++c) X[3 * (size_t)r + c] = ctx->xyz[3 * (size_t)i + c];
     }
     std::vector<int> perm, iperm(nf);
-    coarse_order(nf, X.data(), per * ra, perm);
+    coarse_order(nf, X.data(), L, perm);
     for (int i = 0; i < nf; ++i) iperm[perm[i]] = i;
-    auto agg = [&](int r) { return (iperm[r] / per) / ra; };  // aggregate of free node r (old numbering)
-    for (int r = 0; r < nf; ++r) {
-        const int a = agg(r);
+    for (int i = 0; i < nf; ++i) {
+        const int a = i / L;
         cnt[a]++;
-        for (int c = 0; c < 3; ++c) cen[3 * a + c] += X[3 * (size_t)r + c];
+        for (int c = 0; c < 3; ++c) cen[3 * a + c] += X[3 * (size_t)perm[i] + c];
     }
     for (int a = 0; a < nagg; ++a)
         for (int c = 0; c < 3; ++c) cen[3 * a + c] /= std::max(cnt[a], 1);
-    std::vector<float> d((size_t)3 * nf);     // new order, as the device reads it
-    std::vector<double> dd((size_t)3 * nf);   // old order
+    std::vector<float> d((size_t)3 * nf);  // new order, FP32 as the device applies Z
+    for (int i = 0; i < nf; ++i)
+        for (int c = 0; c < 3; ++c) d[3 * (size_t)i + c] = (float)(X[3 * (size_t)perm[i] + c] - cen[3 * (i / L) + c]);
+    // renumbered pattern: node rows in the new order, each keeping its block order
+    const int nblk = ctx->P.node_blk[nf];
+    if (nf >= (1 << 23)) return fail(OFEA_ERR_LIMIT, "too many free nodes for the coarse block lists");
     for (int r = 0; r < nf; ++r)
-        for (int c = 0; c < 3; ++c) {
-            // the device applies Z with the FP32 offsets: build A_c from the same values
-            const float v = (float)(X[3 * (size_t)r + c] - cen[3 * agg(r) + c]);
-            d[3 * (size_t)iperm[r] + c] = v;
-            dd[3 * (size_t)r + c] = (double)v;
-        }
-    // A_c = sum over blocks K_sr (rows of s, columns of r) of Z_s^T K_sr Z_r, Z = [I | R], R = -[d]x
-    std::vector<double> val((size_t)ctx->P.nnz);
-    OFEA_HIP(hipStreamSynchronize(ctx->stream));  // an assembly may still run on the context stream
-    OFEA_HIP(hipMemcpy(val.data(), ctx->d_val, sizeof(double) * val.size(), hipMemcpyDeviceToHost));
-    std::vector<double> A((size_t)nc * nc, 0.0);
-    auto Rmat = [&](int node, double R[3][3]) {
-        const double x = dd[3 * (size_t)node], y = dd[3 * (size_t)node + 1], z = dd[3 * (size_t)node + 2];
-        // R theta = theta x d
-        R[0][0] = 0.0; R[0][1] = z;   R[0][2] = -y;
-        R[1][0] = -z;  R[1][1] = 0.0; R[1][2] = x;
-        R[2][0] = y;   R[2][1] = -x;  R[2][2] = 0.0;
-    };
-    for (int r = 0; r < nf; ++r) {
-        const int nb0 = ctx->P.node_blk[r], deg = ctx->P.node_blk[r + 1] - nb0;
-        const size_t base = 9 * (size_t)nb0;
-        const int ar = agg(r);
-        double Rr[3][3];
-        Rmat(r, Rr);
-        for (int k = 0; k < deg; ++k) {
-            const int s = ctx->P.bcol[nb0 + k];
-            const int as = agg(s);
-            double Ks[3][3];  // Ks[d][c] = K(3s+d, 3r+c)
-            for (int c = 0; c < 3; ++c)
-                for (int dq = 0; dq < 3; ++dq) Ks[dq][c] = val[base + (size_t)c * 3 * deg + 3 * k + dq];
-            double Rs[3][3];
-            Rmat(s, Rs);
-            double Zs[3][6], Zr[3][6];
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) {
-                    Zs[i][j] = i == j ? 1.0 : 0.0;
-                    Zr[i][j] = i == j ? 1.0 : 0.0;
-                    Zs[i][3 + j] = Rs[i][j];
-                    Zr[i][3 + j] = Rr[i][j];
+        if (ctx->P.node_blk[r + 1] - ctx->P.node_blk[r] > 256)
+            return fail(OFEA_ERR_LIMIT, "a node has more than 256 neighbours (coarse block lists)");
+    std::vector<int> nb_p((size_t)nf + 1, 0), bcol_p((size_t)std::max(nblk, 1));
+    for (int i = 0; i < nf; ++i) {
+        const int o = perm[i], o0 = ctx->P.node_blk[o], deg = ctx->P.node_blk[o + 1] - o0;
+        nb_p[i + 1] = nb_p[i] + deg;
+        for (int k = 0; k < deg; ++k) bcol_p[(size_t)nb_p[i] + k] = iperm[ctx->P.bcol[(size_t)o0 + k]];
+    }
+    // per aggregate b: coupled aggregates a (ascending) and the blocks (r, k)
+    // of b's nodes with bcol in a, in node and block order
+    std::vector<int> sl_ptr(1, 0), sl_part, sl_bptr(1, 0), sl_blk((size_t)std::max(nblk, 1));
+    std::vector<int> slot_of(nagg, -1), scount;
+    int bw = 5;
+    for (int b = 0; b < nagg; ++b) {
+        const int i0 = b * L, i1 = std::min(nf, i0 + L);
+        const int first = (int)sl_part.size();
+        for (int i = i0; i < i1; ++i)
+            for (int q = nb_p[i]; q < nb_p[i + 1]; ++q) {
+                const int a = bcol_p[q] / L;
+                if (slot_of[a] < 0) {
+                    slot_of[a] = 0;
+                    sl_part.push_back(a);
                 }
-            double KZ[3][6];
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 6; ++j)
-                    KZ[i][j] = Ks[i][0] * Zr[0][j] + Ks[i][1] * Zr[1][j] + Ks[i][2] * Zr[2][j];
-            for (int i = 0; i < 6; ++i)
-                for (int j = 0; j < 6; ++j)
-                    A[(size_t)(6 * as + i) * nc + 6 * ar + j] +=
-                        Zs[0][i] * KZ[0][j] + Zs[1][i] * KZ[1][j] + Zs[2][i] * KZ[2][j];
-        }
-    }
-    // symmetrise, regularise (a Z column that vanishes, e.g. a rotation of a
-    // collinear aggregate, leaves a zero row; the applied Z ignores it), Cholesky
-    double tr = 0.0;
-    for (int i = 0; i < nc; ++i) tr += A[(size_t)i * nc + i];
-    for (int i = 0; i < nc; ++i)
-        for (int j = 0; j < i; ++j) {
-            const double v = 0.5 * (A[(size_t)i * nc + j] + A[(size_t)j * nc + i]);
-            A[(size_t)i * nc + j] = A[(size_t)j * nc + i] = v;
-        }
-    for (int i = 0; i < nc; ++i) A[(size_t)i * nc + i] += 1e-12 * tr / nc;
-    // A_c couples an aggregate only to aggregates holding neighbour nodes, so
-    // it is banded for consecutive-rank aggregates: banded Cholesky and banded
-    // triangular solves (bandwidth bw) build the explicit inverse in O(nc^2 bw)
-    int bw = 0;
-    for (int i = 0; i < nc; ++i)
-        for (int j = 0; j < i; ++j)
-            if (A[(size_t)i * nc + j] != 0.0) {
-                bw = std::max(bw, i - j);
-                break;
             }
-    std::vector<double> L = A;  // lower triangle, entries with i - j <= bw
-    for (int j = 0; j < nc; ++j) {
-        const int k0 = std::max(0, j - bw);
-        double s = L[(size_t)j * nc + j];
-        for (int k = k0; k < j; ++k) s -= L[(size_t)j * nc + k] * L[(size_t)j * nc + k];
-        if (!(s > 0.0)) return fail(OFEA_ERR_STATE, "coarse matrix Z^T K Z is not positive definite");
-        const double ljj = std::sqrt(s);
-        L[(size_t)j * nc + j] = ljj;
-        const int i1 = std::min(nc, j + bw + 1);
-        for (int i = j + 1; i < i1; ++i) {
-            double t = L[(size_t)i * nc + j];
-            for (int k = std::max(k0, i - bw); k < j; ++k) t -= L[(size_t)i * nc + k] * L[(size_t)j * nc + k];
-            L[(size_t)i * nc + j] = t / ljj;
+        std::sort(sl_part.begin() + first, sl_part.end());
+        const int ns = (int)sl_part.size() - first;
+        for (int t = 0; t < ns; ++t) slot_of[sl_part[first + t]] = t;
+        scount.assign(ns + 1, 0);
+        for (int i = i0; i < i1; ++i)
+            for (int q = nb_p[i]; q < nb_p[i + 1]; ++q) scount[slot_of[bcol_p[q] / L] + 1]++;
+        for (int t = 0; t < ns; ++t) scount[t + 1] += scount[t];
+        const int base = sl_bptr.back();
+        for (int i = i0; i < i1; ++i)
+            for (int q = nb_p[i]; q < nb_p[i + 1]; ++q)
+                sl_blk[(size_t)base + scount[slot_of[bcol_p[q] / L]]++] = (i << 8) | (q - nb_p[i]);
+        for (int t = 0; t < ns; ++t) {
+            const int a = sl_part[first + t];
+            bw = std::max(bw, 6 * std::abs(a - b) + 5);
+            sl_bptr.push_back(base + scount[t]);  // scount[t] now ends slot t
+            slot_of[a] = -1;
         }
+        sl_ptr.push_back((int)sl_part.size());
     }
-    // inverse = L^-T L^-1, column by column (column e of L^-1 is zero above
-    // e); the columns are independent and split over host threads
-    std::vector<double> inv((size_t)nc * nc);
-    auto inv_columns = [&](int e0, int e1) {
-        std::vector<double> col(nc);
-        for (int e = e0; e < e1; ++e) {
-            std::fill(col.begin(), col.end(), 0.0);
-            for (int i = e; i < nc; ++i) {
-                double t = i == e ? 1.0 : 0.0;
-                for (int k = std::max(e, i - bw); k < i; ++k) t -= L[(size_t)i * nc + k] * col[k];
-                col[i] = t / L[(size_t)i * nc + i];
-            }
-            for (int i = nc - 1; i >= 0; --i) {
-                double t = col[i];
-                const int k1 = std::min(nc, i + bw + 1);
-                for (int k = i + 1; k < k1; ++k) t -= L[(size_t)k * nc + i] * col[k];
-                col[i] = t / L[(size_t)i * nc + i];
-            }
-            for (int i = 0; i < nc; ++i) inv[(size_t)i * nc + e] = col[i];
-        }
-    };
-    {
-        const int nth = std::max(1, std::min<int>({16, (int)std::thread::hardware_concurrency(), nc / 64 + 1}));
-        std::vector<std::thread> pool;
-        for (int t = 0; t < nth; ++t) pool.emplace_back(inv_columns, (int)((long long)nc * t / nth),
-                                                        (int)((long long)nc * (t + 1) / nth));
-        for (auto &th : pool) th.join();
-    }
-    std::vector<float> ainv((size_t)nc * nc);
-    for (int i = 0; i < nc; ++i)
-        for (int j = 0; j < nc; ++j)  // symmetric FP32 copy
-            ainv[(size_t)i * nc + j] = (float)(0.5 * (inv[(size_t)i * nc + j] + inv[(size_t)j * nc + i]));
-    // device buffers (sizes depend only on nfree and nagg <= CG_NB / CG_AGG)
+    const int nc = 6 * nagg;
+    bw = std::min(bw, nc - 1);
+    // device buffers (sizes depend only on nfree, nnz and nagg <= CG_NB / CG_AGG)
+    const size_t nfb = (size_t)std::max(nf, 1);
+    const long long nnz = ctx->P.nnz;
+    static_assert(CG_NC_MAX == 6 * (CG_NB / CG_AGG), "coarse dimension bound");
     if (!w->co.g) {
         OFEA_HIP(hipMalloc((void **)&w->co.g, sizeof(double) * 6 * CG_NB));
-        OFEA_HIP(hipMalloc((void **)&w->co.d, sizeof(float) * 3 * (size_t)std::max(nf, 1)));
-        OFEA_HIP(hipMalloc((void **)&w->co_ainv, sizeof(float) * 36 * (CG_NB / CG_AGG) * (CG_NB / CG_AGG)));
+        OFEA_HIP(hipMalloc((void **)&w->co.d, sizeof(float) * 3 * nfb));
+        OFEA_HIP(hipMalloc((void **)&w->co_ainv, sizeof(float) * CG_NC_MAX * CG_NC_MAX));
         OFEA_HIP(hipMalloc((void **)&w->co.zq, sizeof(double) * 6 * CG_NB));
         OFEA_HIP(hipMalloc((void **)&w->co.ya, sizeof(double) * 2 * CG_NC_MAX));
-        w->co.ainv = w->co_ainv;
-    }
-    static_assert(CG_NC_MAX == 6 * (CG_NB / CG_AGG), "coarse dimension bound");
-    OFEA_HIP(hipMemset(w->co.g, 0, sizeof(double) * 6 * CG_NB));
-    OFEA_HIP(hipMemset(w->co.zq, 0, sizeof(double) * 6 * CG_NB));
-    OFEA_HIP(hipMemset(w->co.ya, 0, sizeof(double) * 2 * CG_NC_MAX));
-    OFEA_HIP(hipMemcpy((void *)w->co.d, d.data(), sizeof(float) * d.size(), hipMemcpyHostToDevice));
-    OFEA_HIP(hipMemcpy(w->co_ainv, ainv.data(), sizeof(float) * ainv.size(), hipMemcpyHostToDevice));
-    // the reordered system: pattern renumbered on the host, values copied on the device
-    const long long nnz = ctx->P.nnz;
-    const int nblk = ctx->P.node_blk[nf];
-    const size_t nfb = (size_t)std::max(nf, 1);
-    if (!w->perm) {
+        OFEA_HIP(hipMalloc((void **)&w->co_A, sizeof(double) * CG_NC_MAX * CG_NC_MAX));
+        OFEA_HIP(hipMalloc((void **)&w->co_inv, sizeof(double) * CG_NC_MAX * CG_NC_MAX));
+        OFEA_HIP(hipMalloc((void **)&w->co_flag, sizeof(int)));
         OFEA_HIP(hipMalloc((void **)&w->perm, sizeof(int) * nfb));
         OFEA_HIP(hipMalloc((void **)&w->iperm, sizeof(int) * nfb));
         OFEA_HIP(hipMalloc((void **)&w->nb_p, sizeof(int) * (nfb + 1)));
@@ -1471,36 +1555,88 @@ static int coarse_prepare(ofea_ctx *ctx)
         for (double **p : {&w->minv_p, &w->dblk_p, &w->learned64_p})
             OFEA_HIP(hipMalloc((void **)p, sizeof(double) * 9 * nfb));
         OFEA_HIP(hipMalloc((void **)&w->minv32_p, sizeof(float) * 9 * nfb));
+        OFEA_HIP(hipMalloc((void **)&w->sl_ptr, sizeof(int) * (CG_NB / CG_AGG + 1)));
+        OFEA_HIP(hipMalloc((void **)&w->sl_blk, sizeof(int) * (size_t)std::max(nblk, 1)));
     }
-    std::vector<int> nb_p((size_t)nf + 1, 0), bcol_p((size_t)std::max(nblk, 1));
-    for (int i = 0; i < nf; ++i) {
-        const int o = perm[i], o0 = ctx->P.node_blk[o], deg = ctx->P.node_blk[o + 1] - o0;
-        nb_p[i + 1] = nb_p[i] + deg;
-        for (int k = 0; k < deg; ++k) bcol_p[(size_t)nb_p[i] + k] = iperm[ctx->P.bcol[(size_t)o0 + k]];
-    }
+    if (w->sl_part) (void)hipFree(w->sl_part);
+    if (w->sl_bptr) (void)hipFree(w->sl_bptr);
+    w->sl_part = w->sl_bptr = nullptr;
+    OFEA_HIP(hipMalloc((void **)&w->sl_part, sizeof(int) * std::max<size_t>(sl_part.size(), 1)));
+    OFEA_HIP(hipMalloc((void **)&w->sl_bptr, sizeof(int) * sl_bptr.size()));
+    hipStream_t st = ctx->stream;
+    OFEA_HIP(hipMemsetAsync(w->co.g, 0, sizeof(double) * 6 * CG_NB, st));
+    OFEA_HIP(hipMemsetAsync(w->co.zq, 0, sizeof(double) * 6 * CG_NB, st));
+    OFEA_HIP(hipMemsetAsync(w->co.ya, 0, sizeof(double) * 2 * CG_NC_MAX, st));
+    OFEA_HIP(hipMemcpyAsync((void *)w->co.d, d.data(), sizeof(float) * d.size(), hipMemcpyHostToDevice, st));
     if (nf) {
-        OFEA_HIP(hipMemcpyAsync(w->perm, perm.data(), sizeof(int) * nf, hipMemcpyHostToDevice, ctx->stream));
-        OFEA_HIP(hipMemcpyAsync(w->iperm, iperm.data(), sizeof(int) * nf, hipMemcpyHostToDevice, ctx->stream));
-        OFEA_HIP(hipMemcpyAsync(w->nb_p, nb_p.data(), sizeof(int) * (nf + 1), hipMemcpyHostToDevice, ctx->stream));
+        OFEA_HIP(hipMemcpyAsync(w->perm, perm.data(), sizeof(int) * nf, hipMemcpyHostToDevice, st));
+        OFEA_HIP(hipMemcpyAsync(w->iperm, iperm.data(), sizeof(int) * nf, hipMemcpyHostToDevice, st));
+        OFEA_HIP(hipMemcpyAsync(w->nb_p, nb_p.data(), sizeof(int) * (nf + 1), hipMemcpyHostToDevice, st));
         OFEA_HIP(hipMemcpyAsync(w->bcol_p, bcol_p.data(), sizeof(int) * (size_t)std::max(nblk, 1),
-                                hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL(k_perm_vals, dim3((nf + RED_WG / 64 - 1) / (RED_WG / 64)), dim3(RED_WG), 0, ctx->stream, nf,
-                           w->perm, ctx->d_node_blk, w->nb_p, ctx->d_val, w->val_p);
-        hipLaunchKernelGGL(k_precond_setup, dim3(w->nblk_node), dim3(RED_WG), 0, ctx->stream, nf, w->nb_p, w->bcol_p,
-                           w->val_p, w->dinv_p, w->minv_p, w->minv32_p, w->dblk_p);
+                                hipMemcpyHostToDevice, st));
+        OFEA_HIP(hipMemcpyAsync(w->sl_blk, sl_blk.data(), sizeof(int) * (size_t)std::max(nblk, 1),
+                                hipMemcpyHostToDevice, st));
     }
-    if (nnz)
-        hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((nnz + RED_WG - 1) / RED_WG)), dim3(RED_WG), 0,
-                           ctx->stream, nnz, w->val_p, w->val32_p);
-    OFEA_HIP(hipGetLastError());
-    OFEA_HIP(hipStreamSynchronize(ctx->stream));
-    w->iperm_host = std::move(iperm);
+    OFEA_HIP(hipMemcpyAsync(w->sl_ptr, sl_ptr.data(), sizeof(int) * sl_ptr.size(), hipMemcpyHostToDevice, st));
+    if (!sl_part.empty())
+        OFEA_HIP(hipMemcpyAsync(w->sl_part, sl_part.data(), sizeof(int) * sl_part.size(), hipMemcpyHostToDevice, st));
+    OFEA_HIP(hipMemcpyAsync(w->sl_bptr, sl_bptr.data(), sizeof(int) * sl_bptr.size(), hipMemcpyHostToDevice, st));
+    OFEA_HIP(hipStreamSynchronize(st));  // the host vectors end with this function
     w->co.per = per;
     w->co.ra = ra;
     w->co_nagg = nagg;
     w->co.nagg = nagg;
+    w->co.ainv = w->co_ainv;
+    w->co_bw = bw;
     w->co_d_host = std::move(d);
-    w->co_ainv_host = std::move(ainv);
+    w->iperm_host = std::move(iperm);
+    w->co_struct_valid = true;
+    return OFEA_OK;
+}
+
+static int coarse_values(ofea_ctx *ctx)
+{
+    ofea_solver_ws *w = ctx->ws;
+    const int nf = w->nfree, nc = 6 * w->co_nagg;
+    const long long nnz = ctx->P.nnz;
+    hipStream_t st = ctx->stream;
+    if (nf) {
+        hipLaunchKernelGGL(k_perm_vals, dim3((nf + RED_WG / 64 - 1) / (RED_WG / 64)), dim3(RED_WG), 0, st, nf,
+                           w->perm, ctx->d_node_blk, w->nb_p, ctx->d_val, w->val_p);
+        hipLaunchKernelGGL(k_precond_setup, dim3(w->nblk_node), dim3(RED_WG), 0, st, nf, w->nb_p, w->bcol_p, w->val_p,
+                           w->dinv_p, w->minv_p, w->minv32_p, w->dblk_p);
+    }
+    if (nnz)
+        hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((nnz + RED_WG - 1) / RED_WG)), dim3(RED_WG), 0,
+                           st, nnz, w->val_p, w->val32_p);
+    if (nc == 0) return OFEA_OK;
+    const long long nc2 = (long long)nc * nc;
+    const unsigned g2 = (unsigned)((nc2 + RED_WG - 1) / RED_WG);
+    hipLaunchKernelGGL((k_fill<double>), dim3(g2), dim3(RED_WG), 0, st, (int)nc2, w->co_A, 0.0);
+    hipLaunchKernelGGL(k_coarse_ac, dim3(w->co_nagg), dim3(1024), 0, st, nc, w->nb_p, w->bcol_p, w->val_p, w->co.d,
+                       w->sl_ptr, w->sl_part, w->sl_bptr, w->sl_blk, w->co_A);
+    hipLaunchKernelGGL(k_band_cholesky, dim3(1), dim3(CH_WG), 0, st, nc, w->co_bw, w->co_A, w->co_flag);
+    OFEA_HIP(hipGetLastError());
+    int flag = 0;
+    OFEA_HIP(hipMemcpyAsync(&flag, w->co_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    OFEA_HIP(hipStreamSynchronize(st));
+    if (flag) return fail(OFEA_ERR_STATE, "coarse matrix Z^T K Z is not positive definite");
+    hipLaunchKernelGGL(k_band_inverse, dim3((nc + INV_WAVES - 1) / INV_WAVES), dim3(64 * INV_WAVES), 0, st, nc,
+                       w->co_bw, w->co_A, w->co_inv);
+    hipLaunchKernelGGL(k_sym32, dim3(g2), dim3(RED_WG), 0, st, nc, w->co_inv, w->co_ainv);
+    OFEA_HIP(hipGetLastError());
+    w->co_ainv_host.clear();  // downloaded again by coarse_apply when needed
+    return OFEA_OK;
+}
+
+static int coarse_prepare(ofea_ctx *ctx)
+{
+    ofea_solver_ws *w = ctx->ws;
+    if (w->co_valid) return OFEA_OK;
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    if (!w->co_struct_valid && (rc = coarse_structure(ctx))) return rc;
+    if ((rc = coarse_values(ctx))) return rc;
     w->co_valid = true;
     w->co_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return OFEA_OK;
@@ -1515,6 +1651,12 @@ int coarse_apply(ofea_ctx *ctx, const double *r, double *out)
     if ((rc = coarse_prepare(ctx))) return rc;
     ofea_solver_ws *w = ctx->ws;
     const int nf = w->nfree, per = w->co.per, nagg = w->co_nagg, nc = 6 * nagg;
+    if (w->co_ainv_host.size() != (size_t)nc * nc) {
+        w->co_ainv_host.resize((size_t)nc * nc);
+        OFEA_HIP(hipMemcpyAsync(w->co_ainv_host.data(), w->co_ainv, sizeof(float) * nc * nc, hipMemcpyDeviceToHost,
+                                ctx->stream));
+        OFEA_HIP(hipStreamSynchronize(ctx->stream));
+    }
     const float *d = w->co_d_host.data();  // offsets in the two-level node order
     const int *ip = w->iperm_host.data();
     std::vector<double> g(nc, 0.0), y(nc, 0.0);

@@ -90,6 +90,11 @@ struct ofea_solver_ws {
     float *minv32_p = nullptr;
     double *b_p = nullptr, *x_p = nullptr;  // right-hand side and solution in the new order
     std::vector<int> iperm_host;
+    bool co_struct_valid = false;  // order, pattern and block lists built for this mesh
+    int co_bw = 0;                 // half-bandwidth of A_c
+    int *sl_ptr = nullptr, *sl_part = nullptr, *sl_bptr = nullptr, *sl_blk = nullptr;  // A_c block lists
+    double *co_A = nullptr, *co_inv = nullptr;  // A_c (then L and L^T) and its inverse, FP64
+    int *co_flag = nullptr;
     hipGraph_t graph[ofea::NSLOT] = {};
     hipGraphExec_t exec[ofea::NSLOT] = {};
 };
