@@ -920,8 +920,8 @@ void solver_free(ofea_ctx *ctx)
     for (void *p : {(void *)w->perm, (void *)w->iperm, (void *)w->nb_p, (void *)w->bcol_p, (void *)w->val_p,
                     (void *)w->val32_p, (void *)w->dinv_p, (void *)w->minv_p, (void *)w->dblk_p,
                     (void *)w->learned64_p, (void *)w->minv32_p, (void *)w->b_p, (void *)w->x_p, (void *)w->co_A,
-                    (void *)w->co_inv, (void *)w->co_flag, (void *)w->sl_ptr, (void *)w->sl_part, (void *)w->sl_bptr,
-                    (void *)w->sl_blk})
+                    (void *)w->co_inv, (void *)w->co_flag, (void *)w->sl_ptr, (void *)w->sl_part, (void *)w->sl_cptr,
+                    (void *)w->ch_ptr, (void *)w->ch_beg, (void *)w->sl_blk})
         if (p) (void)hipFree(p);
     if (w->sc) (void)hipFree(w->sc);
     if (w->sc_host) (void)hipHostFree(w->sc_host);
@@ -1296,20 +1296,25 @@ __device__ inline void zcol(const float *dv, int q, double z[3])
 
 // A_c = Z^T K Z on the reordered K: workgroup b owns the column blocks of
 // aggregate b.  The blocks (r, k) of b's nodes are listed per coupled
-// aggregate a ("slot"); one thread per (slot, p, q) sums its 6x6 entry over
-// the slot's blocks in list order and writes A[6a+p][6b+q] (no atomics).
+// aggregate a ("slot") and the lists are cut into chunks; one thread per
+// (chunk, p, q) sums its entry over the chunk's blocks into LDS, then one per
+// (slot, p, q) adds the slot's chunk sums in order and writes A[6a+p][6b+q]
+// (fixed order, no atomics).
+constexpr int CA_MAXCH = 160;  // chunks per aggregate (host-enforced)
 __global__ __launch_bounds__(1024) void k_coarse_ac(int nc, const int *__restrict__ nb_p,
                                                     const int *__restrict__ bcol_p, const double *__restrict__ val_p,
                                                     const float *__restrict__ d, const int *__restrict__ sl_ptr,
-                                                    const int *__restrict__ sl_part, const int *__restrict__ sl_bptr,
+                                                    const int *__restrict__ sl_part, const int *__restrict__ sl_cptr,
+                                                    const int *__restrict__ ch_ptr, const int *__restrict__ ch_beg,
                                                     const int *__restrict__ sl_blk, double *__restrict__ A)
 {
+    __shared__ double s_part[CA_MAXCH * 36];
     const int b = blockIdx.x;
-    const int s0 = sl_ptr[b], ns = sl_ptr[b + 1] - s0;
-    for (int t = threadIdx.x; t < 36 * ns; t += blockDim.x) {
-        const int s = s0 + t / 36, pq = t % 36, p = pq / 6, q = pq % 6;
+    const int c0 = ch_ptr[b], nch = ch_ptr[b + 1] - c0;
+    for (int t = threadIdx.x; t < 36 * nch; t += blockDim.x) {
+        const int ch = c0 + t / 36, pq = t % 36, p = pq / 6, q = pq % 6;
         double acc = 0.0;
-        for (int e = sl_bptr[s]; e < sl_bptr[s + 1]; ++e) {
+        for (int e = ch_beg[ch]; e < ch_beg[ch + 1]; ++e) {
             const int rk = sl_blk[e], r = rk >> 8, k = rk & 255;
             const int nb0 = nb_p[r], deg = nb_p[r + 1] - nb0;
             double zs[3], zr[3];
@@ -1321,21 +1326,44 @@ __global__ __launch_bounds__(1024) void k_coarse_ac(int nc, const int *__restric
 #pragma unroll
                 for (int m = 0; m < 3; ++m) acc += zs[m] * blk[c * 3 * deg + m] * zr[c];
         }
-        A[(size_t)(6 * sl_part[s] + p) * nc + 6 * b + q] = acc;
+        s_part[t] = acc;
+    }
+    __syncthreads();
+    const int s0 = sl_ptr[b], ns = sl_ptr[b + 1] - s0;
+    for (int t = threadIdx.x; t < 36 * ns; t += blockDim.x) {
+        const int sl = s0 + t / 36, pq = t % 36;
+        double acc = 0.0;
+        for (int ch = sl_cptr[sl]; ch < sl_cptr[sl + 1]; ++ch) acc += s_part[(ch - c0) * 36 + pq];
+        A[(size_t)(6 * sl_part[sl] + pq / 6) * nc + 6 * b + pq % 6] = acc;
     }
 }
 
-// In-place banded Cholesky of A (nc x nc row-major, half-bandwidth bw) in one
-// workgroup: the band is symmetrised (lower := mean of both triangles) and
-// the diagonal lifted by 1e-12 tr(A)/nc (a Z column that vanishes, e.g. a
-// rotation of a collinear aggregate, leaves a zero row), then column j is
-// factored and the trailing band updated, j = 0 .. nc-1.  Afterwards the lower
-// triangle holds L and the upper triangle L^T.  *flag = 0, or j + 1 when the
-// pivot of column j is not positive.
+// In-place banded Cholesky of the lower band of A (nc x nc row-major,
+// half-bandwidth bw; symmetrised beforehand by k_band_sym) in one workgroup.
+// The diagonal is first lifted by 1e-12 tr(A)/nc (a Z column that vanishes,
+// e.g. a rotation of a collinear aggregate, leaves a zero row).
+// Panels of CH_NB columns are factored in LDS (rows of the panel and of the
+// band below it), written back, and applied to the trailing band as a rank-
+// CH_NB update in 4x4 tiles; a band wider than the LDS panel falls back to
+// column-by-column updates in global memory.  Afterwards the lower triangle
+// holds L (k_band_upper then copies L^T into the upper triangle).  *flag = 0,
+// or j + 1 when the pivot of column j is not positive.
+// orders the LDS accesses of one wave's lanes around it (LDS requests of a
+// wave complete in issue order)
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 constexpr int CH_WG = 1024;
+constexpr int CH_NB = 16;                       // panel width
+constexpr int CH_ROWS = 460;                    // panel rows held in LDS (bw + CH_NB <= CH_ROWS)
+constexpr int CH_LD = CH_NB + 1;                // LDS row stride: odd, so a column spreads over the banks
 __global__ __launch_bounds__(CH_WG) void k_band_cholesky(int nc, int bw, double *__restrict__ A, int *__restrict__ flag)
 {
-    __shared__ double s_l[CG_NC_MAX];
+    __shared__ double s_p[CH_ROWS * CH_LD];  // panel, row-major [row][column], stride CH_LD
     __shared__ double s_red[CH_WG / 64];
     __shared__ double s_piv;
     __shared__ int s_bad;
@@ -1349,56 +1377,166 @@ __global__ __launch_bounds__(CH_WG) void k_band_cholesky(int nc, int bw, double 
         double t = 0.0;
         for (int w = 0; w < CH_WG / 64; ++w) t += s_red[w];
         s_piv = 1e-12 * t / nc;
+        s_bad = 0;
     }
     __syncthreads();
     const double lift = s_piv;
-    const long long band = (long long)nc * (bw + 1);
-    for (long long t = tid; t < band; t += CH_WG) {
-        const int j = (int)(t / (bw + 1)), u = (int)(t - (long long)j * (bw + 1)), i = j + u;
-        if (i >= nc) continue;
-        if (u == 0)
-            A[(size_t)j * nc + j] += lift;
-        else
-            A[(size_t)i * nc + j] = 0.5 * (A[(size_t)i * nc + j] + A[(size_t)j * nc + i]);
-    }
+    for (int i = tid; i < nc; i += CH_WG) A[(size_t)i * nc + i] += lift;
     __syncthreads();
-    for (int j = 0; j < nc; ++j) {
-        const int m = min(nc - 1, j + bw) - j;  // band rows below the diagonal
-        if (tid == 0) {
-            const double dj = A[(size_t)j * nc + j];
-            s_bad = !(dj > 0.0);
-            s_piv = s_bad ? 1.0 : sqrt(dj);
-            A[(size_t)j * nc + j] = s_piv;
+    if (bw + CH_NB <= CH_ROWS) {
+        for (int j0 = 0; j0 < nc; j0 += CH_NB) {
+            const int nb = min(CH_NB, nc - j0);
+            const int H = min(nc, j0 + nb + bw) - j0;  // panel rows (<= CH_ROWS)
+            for (int t = tid; t < H * CH_NB; t += CH_WG) {
+                const int r = t / CH_NB, c = t % CH_NB;
+                s_p[r * CH_LD + c] = (r >= c && c < nb) ? A[(size_t)(j0 + r) * nc + j0 + c] : 0.0;
+            }
+            __syncthreads();
+            if (tid < 64) {
+                // the panel's nb x nb diagonal block by wave 0, lane l owning row l in
+                // registers (pivots and L[c2][c] broadcast by read_lane)
+                double v[CH_NB];
+#pragma unroll
+                for (int c = 0; c < CH_NB; ++c) v[c] = (tid < nb && c <= tid) ? s_p[tid * CH_LD + c] : 0.0;
+#pragma unroll
+                for (int c = 0; c < CH_NB; ++c) {
+                    if (c < nb) {
+                        const double dc = read_lane(v[c], c);
+                        const double piv = dc > 0.0 ? sqrt(dc) : 1.0;
+                        if (tid == 0 && !(dc > 0.0) && !s_bad) s_bad = j0 + c + 1;
+                        if (tid == c) v[c] = piv;
+                        if (tid > c) v[c] /= piv;
+#pragma unroll
+                        for (int c2 = c + 1; c2 < CH_NB; ++c2) {
+                            const double lc2 = read_lane(v[c], c2);
+                            if (tid >= c2 && c2 < nb) v[c2] -= v[c] * lc2;
+                        }
+                    }
+                }
+                if (tid < nb)
+#pragma unroll
+                    for (int c = 0; c < CH_NB; ++c)
+                        if (c <= tid) s_p[tid * CH_LD + c] = v[c];
+            }
+            __syncthreads();
+            if (s_bad) {
+                if (tid == 0) *flag = s_bad;
+                return;
+            }
+            // rows below the diagonal block: x L_diag^T = a, one row per thread
+            for (int r = nb + tid; r < H; r += CH_WG) {
+                double x[CH_NB];
+#pragma unroll
+                for (int c = 0; c < CH_NB; ++c) x[c] = s_p[r * CH_LD + c];
+#pragma unroll
+                for (int c = 0; c < CH_NB; ++c) {
+                    if (c < nb) {
+                        double t = x[c];
+#pragma unroll
+                        for (int c2 = 0; c2 < c; ++c2) t -= x[c2] * s_p[c * CH_LD + c2];
+                        x[c] = t / s_p[c * CH_LD + c];
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < CH_NB; ++c) s_p[r * CH_LD + c] = x[c];
+            }
+            __syncthreads();
+            for (int t = tid; t < H * CH_NB; t += CH_WG) {
+                const int r = t / CH_NB, c = t % CH_NB;
+                if (r >= c && c < nb) A[(size_t)(j0 + r) * nc + j0 + c] = s_p[r * CH_LD + c];
+            }
+            // trailing band: rows/columns nb .. H-1 of the panel window, 4x4 tiles of the lower triangle
+            const int m = H - nb, T = (m + 3) / 4;
+            for (int t = tid; t < T * T; t += CH_WG) {
+                const int ti = t / T, tk = t - ti * T;
+                if (tk > ti) continue;
+                double acc[4][4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int bq = 0; bq < 4; ++bq) acc[a][bq] = 0.0;
+#pragma unroll 4
+                for (int c = 0; c < CH_NB; ++c) {  // columns past nb are zero in the panel
+                    double li[4], lk[4];
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        const int ri = nb + 4 * ti + a, rk = nb + 4 * tk + a;
+                        li[a] = ri < H ? s_p[ri * CH_LD + c] : 0.0;
+                        lk[a] = rk < H ? s_p[rk * CH_LD + c] : 0.0;
+                    }
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+#pragma unroll
+                        for (int bq = 0; bq < 4; ++bq) acc[a][bq] += li[a] * lk[bq];
+                }
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int bq = 0; bq < 4; ++bq) {
+                        const int i = j0 + nb + 4 * ti + a, k = j0 + nb + 4 * tk + bq;
+                        if (i < j0 + H && k <= i) A[(size_t)i * nc + k] -= acc[a][bq];
+                    }
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        if (s_bad) {
-            if (tid == 0) *flag = j + 1;
-            return;
+    } else {
+        double *s_l = s_p;  // CH_ROWS * CH_NB >= CG_NC_MAX
+        static_assert(CH_ROWS * CH_LD >= CG_NC_MAX, "column buffer");
+        for (int j = 0; j < nc; ++j) {
+            const int m = min(nc - 1, j + bw) - j;  // band rows below the diagonal
+            if (tid == 0) {
+                const double dj = A[(size_t)j * nc + j];
+                if (!(dj > 0.0)) s_bad = j + 1;
+                s_piv = dj > 0.0 ? sqrt(dj) : 1.0;
+                A[(size_t)j * nc + j] = s_piv;
+            }
+            __syncthreads();
+            if (s_bad) {
+                if (tid == 0) *flag = s_bad;
+                return;
+            }
+            const double piv = s_piv;
+            for (int u = tid; u < m; u += CH_WG) {
+                const double l = A[(size_t)(j + 1 + u) * nc + j] / piv;
+                A[(size_t)(j + 1 + u) * nc + j] = l;
+                s_l[u] = l;
+            }
+            __syncthreads();
+            for (int t = tid; t < m * m; t += CH_WG) {
+                const int u = t / m, v = t - u * m;
+                if (v <= u) A[(size_t)(j + 1 + u) * nc + j + 1 + v] -= s_l[u] * s_l[v];
+            }
+            __syncthreads();
         }
-        const double piv = s_piv;
-        for (int u = tid; u < m; u += CH_WG) {
-            const double l = A[(size_t)(j + 1 + u) * nc + j] / piv;
-            A[(size_t)(j + 1 + u) * nc + j] = l;
-            s_l[u] = l;
-        }
-        __syncthreads();
-        for (int t = tid; t < m * m; t += CH_WG) {
-            const int u = t / m, v = t - u * m;
-            if (v <= u) A[(size_t)(j + 1 + u) * nc + j + 1 + v] -= s_l[u] * s_l[v];
-        }
-        __syncthreads();
-    }
-    for (long long t = tid; t < band; t += CH_WG) {
-        const int j = (int)(t / (bw + 1)), u = (int)(t - (long long)j * (bw + 1)), i = j + u;
-        if (u > 0 && i < nc) A[(size_t)j * nc + i] = A[(size_t)i * nc + j];
     }
     if (tid == 0) *flag = 0;
 }
 
+// the band below the diagonal := mean of both triangles (A_c is symmetric up
+// to rounding); one thread per band entry
+__global__ __launch_bounds__(RED_WG) void k_band_sym(int nc, int bw, double *__restrict__ A)
+{
+    const long long t = (long long)blockIdx.x * RED_WG + threadIdx.x;
+    const int j = (int)(t / bw), i = j + 1 + (int)(t - (long long)j * bw);
+    if (j >= nc || i >= nc) return;
+    A[(size_t)i * nc + j] = 0.5 * (A[(size_t)i * nc + j] + A[(size_t)j * nc + i]);
+}
+
+// L^T into the upper triangle of the band
+__global__ __launch_bounds__(RED_WG) void k_band_upper(int nc, int bw, double *__restrict__ A)
+{
+    const long long t = (long long)blockIdx.x * RED_WG + threadIdx.x;
+    const int j = (int)(t / bw), i = j + 1 + (int)(t - (long long)j * bw);
+    if (j >= nc || i >= nc) return;
+    A[(size_t)j * nc + i] = A[(size_t)i * nc + j];
+}
+
 // Column e of A^-1 = L^-T L^-1 e_e by one wave: forward then backward
-// substitution over the band (lane-split dot products, shuffle sums), written
-// as row e of inv (A^-1 is symmetric).
+// substitution over the band (lane-split dot products, shuffle sums),
+// written as row e of inv (A^-1 is symmetric).  For bw < 64 INV_U the band
+// row of the next step is loaded while the current one is summed.
 constexpr int INV_WAVES = 4;
+constexpr int INV_U = 8;
 __global__ __launch_bounds__(64 * INV_WAVES) void k_band_inverse(int nc, int bw, const double *__restrict__ L,
                                                                  double *__restrict__ inv)
 {
@@ -1408,25 +1546,88 @@ __global__ __launch_bounds__(64 * INV_WAVES) void k_band_inverse(int nc, int bw,
     if (e >= nc) return;  // whole waves; no workgroup barrier below
     double *y = s_y[w];
     for (int i = lane; i < e; i += 64) y[i] = 0.0;
-    __builtin_amdgcn_wave_barrier();
-    for (int i = e; i < nc; ++i) {  // L y = e_e
-        const int lo = max(e, i - bw);
-        double part = 0.0;
-        for (int k = lo + lane; k < i; k += 64) part += L[(size_t)i * nc + k] * y[k];
-        for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
-        const double yi = ((i == e ? 1.0 : 0.0) - part) / L[(size_t)i * nc + i];
-        if (lane == 0) y[i] = yi;
-        __builtin_amdgcn_wave_barrier();
-    }
-    for (int i = nc - 1; i >= 0; --i) {  // L^T x = y (L^T is the upper triangle)
-        const int hi = min(nc - 1, i + bw);
-        double part = 0.0;
-        for (int k = i + 1 + lane; k <= hi; k += 64) part += L[(size_t)i * nc + k] * y[k];
-        for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
-        const double xi = (y[i] - part) / L[(size_t)i * nc + i];
-        __builtin_amdgcn_wave_barrier();
-        if (lane == 0) y[i] = xi;
-        __builtin_amdgcn_wave_barrier();
+    wave_sync();
+    if (bw < 64 * INV_U) {
+        // forward, L y = e_e: row i holds L[i][k], k in [max(e, i - bw), i)
+        auto fload = [&](int i, double v[INV_U]) {
+            const int lo = max(e, i - bw);
+#pragma unroll
+            for (int u = 0; u < INV_U; ++u) {
+                const int k = lo + lane + 64 * u;
+                v[u] = (i < nc && k < i) ? L[(size_t)i * nc + k] : 0.0;
+            }
+        };
+        double cur[INV_U], nxt[INV_U];
+        fload(e, cur);
+        double dcur = L[(size_t)e * nc + e];
+        for (int i = e; i < nc; ++i) {
+            fload(i + 1, nxt);
+            const double dnxt = i + 1 < nc ? L[(size_t)(i + 1) * nc + i + 1] : 1.0;
+            const int lo = max(e, i - bw);
+            double part = 0.0;
+#pragma unroll
+            for (int u = 0; u < INV_U; ++u) {
+                const int k = lo + lane + 64 * u;
+                part += k < i ? cur[u] * y[min(k, nc - 1)] : 0.0;
+            }
+            part = wave_sum(part);
+            const double yi = ((i == e ? 1.0 : 0.0) - part) / dcur;
+            if (lane == 0) y[i] = yi;
+            wave_sync();
+#pragma unroll
+            for (int u = 0; u < INV_U; ++u) cur[u] = nxt[u];
+            dcur = dnxt;
+        }
+        // backward, L^T x = y: row i of the upper triangle, k in (i, min(nc - 1, i + bw)]
+        auto bload = [&](int i, double v[INV_U]) {
+            const int hi = min(nc - 1, i + bw);
+#pragma unroll
+            for (int u = 0; u < INV_U; ++u) {
+                const int k = i + 1 + lane + 64 * u;
+                v[u] = (i >= 0 && k <= hi) ? L[(size_t)i * nc + k] : 0.0;
+            }
+        };
+        bload(nc - 1, cur);
+        dcur = L[(size_t)(nc - 1) * nc + nc - 1];
+        for (int i = nc - 1; i >= 0; --i) {
+            bload(i - 1, nxt);
+            const double dnxt = i > 0 ? L[(size_t)(i - 1) * nc + i - 1] : 1.0;
+            const int hi = min(nc - 1, i + bw);
+            double part = 0.0;
+#pragma unroll
+            for (int u = 0; u < INV_U; ++u) {
+                const int k = i + 1 + lane + 64 * u;
+                part += k <= hi ? cur[u] * y[min(k, nc - 1)] : 0.0;
+            }
+            part = wave_sum(part);
+            const double xi = (y[i] - part) / dcur;
+            wave_sync();
+            if (lane == 0) y[i] = xi;
+            wave_sync();
+#pragma unroll
+            for (int u = 0; u < INV_U; ++u) cur[u] = nxt[u];
+            dcur = dnxt;
+        }
+    } else {
+        for (int i = e; i < nc; ++i) {  // L y = e_e
+            const int lo = max(e, i - bw);
+            double part = 0.0;
+            for (int k = lo + lane; k < i; k += 64) part += L[(size_t)i * nc + k] * y[k];
+            for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+            const double yi = ((i == e ? 1.0 : 0.0) - part) / L[(size_t)i * nc + i];
+            if (lane == 0) y[i] = yi;
+            wave_sync();
+        }
+        for (int i = nc - 1; i >= 0; --i) {  // L^T x = y (L^T is the upper triangle)
+            const int hi = min(nc - 1, i + bw);
+            double part = 0.0;
+            for (int k = i + 1 + lane; k <= hi; k += 64) part += L[(size_t)i * nc + k] * y[k];
+            for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+            const double xi = (y[i] - part) / L[(size_t)i * nc + i];
+            wave_sync();
+            if (lane == 0) y[i] = xi;
+            wave_sync();
+        }
     }
     for (int i = lane; i < nc; i += 64) inv[(size_t)e * nc + i] = y[i];
 }
@@ -1496,6 +1697,7 @@ static int coarse_structure(ofea_ctx *ctx)
     // per aggregate b: coupled aggregates a (ascending) and the blocks (r, k)
     // of b's nodes with bcol in a, in node and block order
     std::vector<int> sl_ptr(1, 0), sl_part, sl_bptr(1, 0), sl_blk((size_t)std::max(nblk, 1));
+    std::vector<int> sl_cptr, ch_ptr(1, 0), ch_beg;  // chunk index of each slot, chunks per aggregate, chunk starts
     std::vector<int> slot_of(nagg, -1), scount;
     int bw = 5;
     for (int b = 0; b < nagg; ++b) {
@@ -1527,7 +1729,23 @@ static int coarse_structure(ofea_ctx *ctx)
             slot_of[a] = -1;
         }
         sl_ptr.push_back((int)sl_part.size());
+        // chunks of the slot lists: at most CA_MAXCH per aggregate
+        if (ns > CA_MAXCH) return fail(OFEA_ERR_LIMIT, "an aggregate couples to too many aggregates");
+        const int nbl = sl_bptr.back() - base;
+        for (int cs = std::max(64, (nbl + CA_MAXCH - 1) / CA_MAXCH);; cs *= 2) {
+            int nch = 0;
+            for (int t = 0; t < ns; ++t) nch += (sl_bptr[first + t + 1] - sl_bptr[first + t] + cs - 1) / cs;
+            if (nch > CA_MAXCH) continue;
+            for (int t = 0; t < ns; ++t) {
+                sl_cptr.push_back((int)ch_beg.size());
+                for (int e = sl_bptr[first + t]; e < sl_bptr[first + t + 1]; e += cs) ch_beg.push_back(e);
+            }
+            ch_ptr.push_back((int)ch_beg.size());
+            break;
+        }
     }
+    sl_cptr.push_back((int)ch_beg.size());
+    ch_beg.push_back(sl_bptr.back());
     const int nc = 6 * nagg;
     bw = std::min(bw, nc - 1);
     // device buffers (sizes depend only on nfree, nnz and nagg <= CG_NB / CG_AGG)
@@ -1556,13 +1774,16 @@ static int coarse_structure(ofea_ctx *ctx)
             OFEA_HIP(hipMalloc((void **)p, sizeof(double) * 9 * nfb));
         OFEA_HIP(hipMalloc((void **)&w->minv32_p, sizeof(float) * 9 * nfb));
         OFEA_HIP(hipMalloc((void **)&w->sl_ptr, sizeof(int) * (CG_NB / CG_AGG + 1)));
+        OFEA_HIP(hipMalloc((void **)&w->ch_ptr, sizeof(int) * (CG_NB / CG_AGG + 1)));
         OFEA_HIP(hipMalloc((void **)&w->sl_blk, sizeof(int) * (size_t)std::max(nblk, 1)));
     }
-    if (w->sl_part) (void)hipFree(w->sl_part);
-    if (w->sl_bptr) (void)hipFree(w->sl_bptr);
-    w->sl_part = w->sl_bptr = nullptr;
+    for (int **p : {&w->sl_part, &w->sl_cptr, &w->ch_beg}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
     OFEA_HIP(hipMalloc((void **)&w->sl_part, sizeof(int) * std::max<size_t>(sl_part.size(), 1)));
-    OFEA_HIP(hipMalloc((void **)&w->sl_bptr, sizeof(int) * sl_bptr.size()));
+    OFEA_HIP(hipMalloc((void **)&w->sl_cptr, sizeof(int) * sl_cptr.size()));
+    OFEA_HIP(hipMalloc((void **)&w->ch_beg, sizeof(int) * ch_beg.size()));
     hipStream_t st = ctx->stream;
     OFEA_HIP(hipMemsetAsync(w->co.g, 0, sizeof(double) * 6 * CG_NB, st));
     OFEA_HIP(hipMemsetAsync(w->co.zq, 0, sizeof(double) * 6 * CG_NB, st));
@@ -1580,7 +1801,9 @@ static int coarse_structure(ofea_ctx *ctx)
     OFEA_HIP(hipMemcpyAsync(w->sl_ptr, sl_ptr.data(), sizeof(int) * sl_ptr.size(), hipMemcpyHostToDevice, st));
     if (!sl_part.empty())
         OFEA_HIP(hipMemcpyAsync(w->sl_part, sl_part.data(), sizeof(int) * sl_part.size(), hipMemcpyHostToDevice, st));
-    OFEA_HIP(hipMemcpyAsync(w->sl_bptr, sl_bptr.data(), sizeof(int) * sl_bptr.size(), hipMemcpyHostToDevice, st));
+    OFEA_HIP(hipMemcpyAsync(w->sl_cptr, sl_cptr.data(), sizeof(int) * sl_cptr.size(), hipMemcpyHostToDevice, st));
+    OFEA_HIP(hipMemcpyAsync(w->ch_ptr, ch_ptr.data(), sizeof(int) * ch_ptr.size(), hipMemcpyHostToDevice, st));
+    OFEA_HIP(hipMemcpyAsync(w->ch_beg, ch_beg.data(), sizeof(int) * ch_beg.size(), hipMemcpyHostToDevice, st));
     OFEA_HIP(hipStreamSynchronize(st));  // the host vectors end with this function
     w->co.per = per;
     w->co.ra = ra;
@@ -1614,8 +1837,11 @@ static int coarse_values(ofea_ctx *ctx)
     const unsigned g2 = (unsigned)((nc2 + RED_WG - 1) / RED_WG);
     hipLaunchKernelGGL((k_fill<double>), dim3(g2), dim3(RED_WG), 0, st, (int)nc2, w->co_A, 0.0);
     hipLaunchKernelGGL(k_coarse_ac, dim3(w->co_nagg), dim3(1024), 0, st, nc, w->nb_p, w->bcol_p, w->val_p, w->co.d,
-                       w->sl_ptr, w->sl_part, w->sl_bptr, w->sl_blk, w->co_A);
+                       w->sl_ptr, w->sl_part, w->sl_cptr, w->ch_ptr, w->ch_beg, w->sl_blk, w->co_A);
+    const unsigned gb = (unsigned)(((long long)nc * w->co_bw + RED_WG - 1) / RED_WG);
+    hipLaunchKernelGGL(k_band_sym, dim3(gb), dim3(RED_WG), 0, st, nc, w->co_bw, w->co_A);
     hipLaunchKernelGGL(k_band_cholesky, dim3(1), dim3(CH_WG), 0, st, nc, w->co_bw, w->co_A, w->co_flag);
+    hipLaunchKernelGGL(k_band_upper, dim3(gb), dim3(RED_WG), 0, st, nc, w->co_bw, w->co_A);
     OFEA_HIP(hipGetLastError());
     int flag = 0;
     OFEA_HIP(hipMemcpyAsync(&flag, w->co_flag, sizeof(int), hipMemcpyDeviceToHost, st));
