@@ -501,7 +501,8 @@ def bench_newton(args, ctx, m, resort, E, nu, F, tip, L_):
     ln.set_params(params)
     runs = {}
     newton_all = {"plain_fp64_jacobi": A.PCG_JACOBI, "fp64_block_jacobi": A.PCG_BLOCK_JACOBI,
-                  "learned_fp64": A.PCG_LEARNED, "learned_fp64_bf16net": A.PCG_LEARNED}
+                  "learned_fp64": A.PCG_LEARNED, "learned_fp64_bf16net": A.PCG_LEARNED,
+                  "two_level_fp64": A.PCG_TWO_LEVEL}
     for name in [x for x in args.newton_methods.split(",") if x]:
         meth = newton_all[name]
         ln.precision(A.BF16 if name.endswith("bf16net") else A.FP32)
@@ -777,7 +778,8 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=1e-6, help="Newton inner PCG tolerance (cfg5)")
     ap.add_argument("--seq-methods", default="plain_fp64_jacobi,fp64_block_jacobi",
                     help="one-system-at-a-time solvers compared with the batched solve (cfg4)")
-    ap.add_argument("--newton-methods", default="plain_fp64_jacobi,fp64_block_jacobi,learned_fp64,learned_fp64_bf16net",
+    ap.add_argument("--newton-methods", default="plain_fp64_jacobi,fp64_block_jacobi,learned_fp64,learned_fp64_bf16net,"
+                    "two_level_fp64",
                     help="inner PCG methods of the Newton runs (cfg5)")
     ap.add_argument("--solve-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
