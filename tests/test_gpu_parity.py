@@ -119,6 +119,40 @@ def test_pcg_cfg1_matches_direct_solve(ctx, method):
         assert it < it_bj
 
 
+TWO_LEVEL_MESHES = {
+    # one aggregate (nc = 6, a single Cholesky panel)
+    "hex_2x1x1": lambda: mg.hex_block(2, 1, 1, jitter=0.1, seed=3),
+    # a few aggregates, jittered tets
+    "tet_5x3x3": lambda: mg.kuhn_tets(5, 3, 3, jitter=0.2, seed=5),
+    # long thin block: many slabs of one part
+    "hex_40x2x2": lambda: mg.hex_block(40, 2, 2, jitter=0.1, seed=6),
+}
+
+
+@pytest.mark.parametrize("name", list(TWO_LEVEL_MESHES))
+def test_two_level_small_and_thin_meshes(ctx, name):
+    """Two-level PCG (coarse set-up on the device) reaches the direct solution
+    on meshes with one aggregate, a few, and a long row of slabs; the
+    aggregates cover the free nodes with all but the last of equal size."""
+    from openfea_amd import learned as L
+
+    m = TWO_LEVEL_MESHES[name]()
+    resort = mg.free_index(m.nd, m.fix_set)
+    say(f"two-level on {name} ne={m.ne}")
+    ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
+    cp, ri = ctx.pattern()
+    val = ctx.assemble(mg.E_STEEL, mg.NU_STEEL)
+    F = ctx.spmv(np.random.default_rng(8).normal(size=ctx.n))
+    xd = spla.spsolve(csc(cp, ri, val), F)
+    for meth in (A.PCG_TWO_LEVEL, A.PCG_TWO_LEVEL_MIXED):
+        # rtol 1e-12: the thin block's condition number turns a 1e-10 residual into ~4e-8 of error
+        x, it, rel, ok = ctx.solve(F, method=meth, rtol=1e-12, maxit=100000)
+        assert ok and np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8, (meth, it, rel)
+    agg, nagg = L.coarse_aggregates(ctx)
+    counts = np.bincount(agg, minlength=nagg)
+    assert counts.sum() == ctx.n // 3 and len(set(counts[:-1].tolist())) <= 1
+
+
 def test_coarse_correction_matches_numpy(ctx):
     """Z A_c^-1 Z^T r of the two-level methods against an independent numpy
     construction from the library's aggregates: rigid-body modes about each
