@@ -1306,7 +1306,7 @@ __global__ __launch_bounds__(1024) void k_coarse_ac(int nc, const int *__restric
                                                     const float *__restrict__ d, const int *__restrict__ sl_ptr,
                                                     const int *__restrict__ sl_part, const int *__restrict__ sl_cptr,
                                                     const int *__restrict__ ch_ptr, const int *__restrict__ ch_beg,
-                                                    const int *__restrict__ sl_blk, double *__restrict__ A)
+                                                    const int2 *__restrict__ sl_blk, double *__restrict__ A)
 {
     __shared__ double s_part[CA_MAXCH * 36];
     const int b = blockIdx.x;
@@ -1315,7 +1315,8 @@ __global__ __launch_bounds__(1024) void k_coarse_ac(int nc, const int *__restric
         const int ch = c0 + t / 36, pq = t % 36, p = pq / 6, q = pq % 6;
         double acc = 0.0;
         for (int e = ch_beg[ch]; e < ch_beg[ch + 1]; ++e) {
-            const int rk = sl_blk[e], r = rk >> 8, k = rk & 255;
+            const int2 rk = sl_blk[e];
+            const int r = rk.x, k = rk.y;
             const int nb0 = nb_p[r], deg = nb_p[r + 1] - nb0;
             double zs[3], zr[3];
             zcol(d + 3 * (size_t)bcol_p[nb0 + k], p, zs);
@@ -1684,10 +1685,6 @@ static int coarse_structure(ofea_ctx *ctx)
         for (int c = 0; c < 3; ++c) d[3 * (size_t)i + c] = (float)(X[3 * (size_t)perm[i] + c] - cen[3 * (i / L) + c]);
     // renumbered pattern: node rows in the new order, each keeping its block order
     const int nblk = ctx->P.node_blk[nf];
-    if (nf >= (1 << 23)) return fail(OFEA_ERR_LIMIT, "too many free nodes for the coarse block lists");
-    for (int r = 0; r < nf; ++r)
-        if (ctx->P.node_blk[r + 1] - ctx->P.node_blk[r] > 256)
-            return fail(OFEA_ERR_LIMIT, "a node has more than 256 neighbours (coarse block lists)");
     std::vector<int> nb_p((size_t)nf + 1, 0), bcol_p((size_t)std::max(nblk, 1));
     for (int i = 0; i < nf; ++i) {
         const int o = perm[i], o0 = ctx->P.node_blk[o], deg = ctx->P.node_blk[o + 1] - o0;
@@ -1696,7 +1693,8 @@ static int coarse_structure(ofea_ctx *ctx)
     }
     // per aggregate b: coupled aggregates a (ascending) and the blocks (r, k)
     // of b's nodes with bcol in a, in node and block order
-    std::vector<int> sl_ptr(1, 0), sl_part, sl_bptr(1, 0), sl_blk((size_t)std::max(nblk, 1));
+    std::vector<int> sl_ptr(1, 0), sl_part, sl_bptr(1, 0);
+    std::vector<int2> sl_blk((size_t)std::max(nblk, 1));
     std::vector<int> sl_cptr, ch_ptr(1, 0), ch_beg;  // chunk index of each slot, chunks per aggregate, chunk starts
     std::vector<int> slot_of(nagg, -1), scount;
     int bw = 5;
@@ -1721,7 +1719,7 @@ static int coarse_structure(ofea_ctx *ctx)
         const int base = sl_bptr.back();
         for (int i = i0; i < i1; ++i)
             for (int q = nb_p[i]; q < nb_p[i + 1]; ++q)
-                sl_blk[(size_t)base + scount[slot_of[bcol_p[q] / L]]++] = (i << 8) | (q - nb_p[i]);
+                sl_blk[(size_t)base + scount[slot_of[bcol_p[q] / L]]++] = make_int2(i, q - nb_p[i]);
         for (int t = 0; t < ns; ++t) {
             const int a = sl_part[first + t];
             bw = std::max(bw, 6 * std::abs(a - b) + 5);
@@ -1775,7 +1773,7 @@ static int coarse_structure(ofea_ctx *ctx)
         OFEA_HIP(hipMalloc((void **)&w->minv32_p, sizeof(float) * 9 * nfb));
         OFEA_HIP(hipMalloc((void **)&w->sl_ptr, sizeof(int) * (CG_NB / CG_AGG + 1)));
         OFEA_HIP(hipMalloc((void **)&w->ch_ptr, sizeof(int) * (CG_NB / CG_AGG + 1)));
-        OFEA_HIP(hipMalloc((void **)&w->sl_blk, sizeof(int) * (size_t)std::max(nblk, 1)));
+        OFEA_HIP(hipMalloc((void **)&w->sl_blk, sizeof(int2) * (size_t)std::max(nblk, 1)));
     }
     for (int **p : {&w->sl_part, &w->sl_cptr, &w->ch_beg}) {
         if (*p) (void)hipFree(*p);
@@ -1795,7 +1793,7 @@ static int coarse_structure(ofea_ctx *ctx)
         OFEA_HIP(hipMemcpyAsync(w->nb_p, nb_p.data(), sizeof(int) * (nf + 1), hipMemcpyHostToDevice, st));
         OFEA_HIP(hipMemcpyAsync(w->bcol_p, bcol_p.data(), sizeof(int) * (size_t)std::max(nblk, 1),
                                 hipMemcpyHostToDevice, st));
-        OFEA_HIP(hipMemcpyAsync(w->sl_blk, sl_blk.data(), sizeof(int) * (size_t)std::max(nblk, 1),
+        OFEA_HIP(hipMemcpyAsync(w->sl_blk, sl_blk.data(), sizeof(int2) * (size_t)std::max(nblk, 1),
                                 hipMemcpyHostToDevice, st));
     }
     OFEA_HIP(hipMemcpyAsync(w->sl_ptr, sl_ptr.data(), sizeof(int) * sl_ptr.size(), hipMemcpyHostToDevice, st));
