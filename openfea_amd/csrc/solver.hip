@@ -1696,30 +1696,30 @@ static int coarse_structure(ofea_ctx *ctx)
     std::vector<int> sl_ptr(1, 0), sl_part, sl_bptr(1, 0);
     std::vector<int2> sl_blk((size_t)std::max(nblk, 1));
     std::vector<int> sl_cptr, ch_ptr(1, 0), ch_beg;  // chunk index of each slot, chunks per aggregate, chunk starts
-    std::vector<int> slot_of(nagg, -1), scount;
+    std::vector<int> slot_of(nagg, -1), scount, bpart((size_t)std::max(nblk, 1));
+    for (int q = 0; q < nblk; ++q) bpart[q] = bcol_p[q] / L;  // aggregate of each block's row node
     int bw = 5;
     for (int b = 0; b < nagg; ++b) {
         const int i0 = b * L, i1 = std::min(nf, i0 + L);
+        const int q0 = nb_p[i0], q1 = nb_p[i1];
         const int first = (int)sl_part.size();
-        for (int i = i0; i < i1; ++i)
-            for (int q = nb_p[i]; q < nb_p[i + 1]; ++q) {
-                const int a = bcol_p[q] / L;
-                if (slot_of[a] < 0) {
-                    slot_of[a] = 0;
-                    sl_part.push_back(a);
-                }
+        for (int q = q0; q < q1; ++q) {
+            const int a = bpart[q];
+            if (slot_of[a] < 0) {
+                slot_of[a] = 0;
+                sl_part.push_back(a);
             }
+        }
         std::sort(sl_part.begin() + first, sl_part.end());
         const int ns = (int)sl_part.size() - first;
         for (int t = 0; t < ns; ++t) slot_of[sl_part[first + t]] = t;
         scount.assign(ns + 1, 0);
-        for (int i = i0; i < i1; ++i)
-            for (int q = nb_p[i]; q < nb_p[i + 1]; ++q) scount[slot_of[bcol_p[q] / L] + 1]++;
+        for (int q = q0; q < q1; ++q) scount[slot_of[bpart[q]] + 1]++;
         for (int t = 0; t < ns; ++t) scount[t + 1] += scount[t];
         const int base = sl_bptr.back();
         for (int i = i0; i < i1; ++i)
             for (int q = nb_p[i]; q < nb_p[i + 1]; ++q)
-                sl_blk[(size_t)base + scount[slot_of[bcol_p[q] / L]]++] = make_int2(i, q - nb_p[i]);
+                sl_blk[(size_t)base + scount[slot_of[bpart[q]]]++] = make_int2(i, q - nb_p[i]);
         for (int t = 0; t < ns; ++t) {
             const int a = sl_part[first + t];
             bw = std::max(bw, 6 * std::abs(a - b) + 5);
