@@ -1222,7 +1222,20 @@ static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, Ve
 // (the last part may be shorter): split across the longest axis of the set's
 // bounding box after the first half of its parts; a part keeps its nodes in
 // ascending index order.
-static void rcb_parts(int *b, int m, int L, const double *X)
+// Nodes as (coordinate on the split axis, index) keys: one total order
+// (coordinate, then index), kept in contiguous arrays so that the sorts and
+// selections below do not chase node indices into X.
+struct AxisKey {
+    double x;
+    int i;
+    bool operator<(const AxisKey &o) const { return x < o.x || (x == o.x && i < o.i); }
+};
+
+// Cuts b[0..m) into parts of L nodes (the last may be shorter) by recursive
+// coordinate bisection across the longest side of each box; each part ends
+// sorted by index.  The parts depend only on the total order, not on the
+// arrangement nth_element leaves inside each side.
+static void rcb_parts(int *b, int m, int L, const double *X, std::vector<AxisKey> &key)
 {
     const int parts = (m + L - 1) / L;
     if (parts <= 1) {
@@ -1239,12 +1252,12 @@ static void rcb_parts(int *b, int m, int L, const double *X)
     for (int c = 1; c < 3; ++c)
         if (hi[c] - lo[c] > hi[ax] - lo[ax]) ax = c;
     const int lm = (parts / 2) * L;
-    std::nth_element(b, b + lm, b + m, [X, ax](int p, int q) {
-        const double xp = X[3 * (size_t)p + ax], xq = X[3 * (size_t)q + ax];
-        return xp < xq || (xp == xq && p < q);
-    });
-    rcb_parts(b, lm, L, X);
-    rcb_parts(b + lm, m - lm, L, X);
+    key.resize(m);
+    for (int i = 0; i < m; ++i) key[i] = {X[3 * (size_t)b[i] + ax], b[i]};
+    std::nth_element(key.begin(), key.begin() + lm, key.begin() + m);
+    for (int i = 0; i < m; ++i) b[i] = key[i].i;
+    rcb_parts(b, lm, L, X, key);
+    rcb_parts(b + lm, m - lm, L, X, key);
 }
 
 // Order of the free nodes for the two-level methods: compact parts of
@@ -1274,13 +1287,25 @@ static void coarse_order(int nf, const double *X, int L, std::vector<int> &perm)
     const double edge = std::cbrt(vol / nparts);
     const int slabs = std::max(1, std::min(nparts, (int)std::lround(emax / std::max(edge, 1e-300))));
     const int per_slab = (nparts + slabs - 1) / slabs;  // parts per slab
-    std::sort(perm.begin(), perm.end(), [X, ax](int p, int q) {
-        const double xp = X[3 * (size_t)p + ax], xq = X[3 * (size_t)q + ax];
-        return xp < xq || (xp == xq && p < q);
-    });
+    std::vector<AxisKey> key(nf);
+    for (int i = 0; i < nf; ++i) key[i] = {X[3 * (size_t)i + ax], i};
+    std::sort(key.begin(), key.end());
+    for (int i = 0; i < nf; ++i) perm[i] = key[i].i;
+    // the slabs are cut independently, on up to 8 host threads
     const long long slab_nodes = (long long)per_slab * L;
-    for (long long s0 = 0; s0 < nf; s0 += slab_nodes)
-        rcb_parts(perm.data() + s0, (int)std::min<long long>(slab_nodes, nf - s0), L, X);
+    const int nslab = (int)((nf + slab_nodes - 1) / slab_nodes);
+    const int nthr = std::max(1, std::min({8, nslab, (int)std::max(1u, std::thread::hardware_concurrency())}));
+    auto cut = [&](int th, std::vector<AxisKey> &k) {
+        for (int sl = th; sl < nslab; sl += nthr) {
+            const long long s0 = sl * slab_nodes;
+            rcb_parts(perm.data() + s0, (int)std::min<long long>(slab_nodes, nf - s0), L, X, k);
+        }
+    };
+    std::vector<std::thread> pool;
+    std::vector<std::vector<AxisKey>> keys(nthr - 1);
+    for (int th = 1; th < nthr; ++th) pool.emplace_back(cut, th, std::ref(keys[th - 1]));
+    cut(0, key);
+    for (auto &t : pool) t.join();
 }
 
 // ---- coarse-space kernels (device set-up of A_c^-1 for each K) ----
@@ -1691,59 +1716,93 @@ static int coarse_structure(ofea_ctx *ctx)
         nb_p[i + 1] = nb_p[i] + deg;
         for (int k = 0; k < deg; ++k) bcol_p[(size_t)nb_p[i] + k] = iperm[ctx->P.bcol[(size_t)o0 + k]];
     }
-    // per aggregate b: coupled aggregates a (ascending) and the blocks (r, k)
-    // of b's nodes with bcol in a, in node and block order
-    std::vector<int> sl_ptr(1, 0), sl_part, sl_bptr(1, 0);
-    std::vector<int2> sl_blk((size_t)std::max(nblk, 1));
-    std::vector<int> sl_cptr, ch_ptr(1, 0), ch_beg;  // chunk index of each slot, chunks per aggregate, chunk starts
-    std::vector<int> slot_of(nagg, -1), scount, bpart((size_t)std::max(nblk, 1));
-    for (int q = 0; q < nblk; ++q) bpart[q] = bcol_p[q] / L;  // aggregate of each block's row node
-    int bw = 5;
-    for (int b = 0; b < nagg; ++b) {
-        const int i0 = b * L, i1 = std::min(nf, i0 + L);
-        const int q0 = nb_p[i0], q1 = nb_p[i1];
-        const int first = (int)sl_part.size();
-        for (int q = q0; q < q1; ++q) {
-            const int a = bpart[q];
-            if (slot_of[a] < 0) {
-                slot_of[a] = 0;
-                sl_part.push_back(a);
-            }
-        }
-        std::sort(sl_part.begin() + first, sl_part.end());
-        const int ns = (int)sl_part.size() - first;
-        for (int t = 0; t < ns; ++t) slot_of[sl_part[first + t]] = t;
-        scount.assign(ns + 1, 0);
-        for (int q = q0; q < q1; ++q) scount[slot_of[bpart[q]] + 1]++;
-        for (int t = 0; t < ns; ++t) scount[t + 1] += scount[t];
-        const int base = sl_bptr.back();
-        for (int i = i0; i < i1; ++i)
-            for (int q = nb_p[i]; q < nb_p[i + 1]; ++q)
-                sl_blk[(size_t)base + scount[slot_of[bpart[q]]]++] = make_int2(i, q - nb_p[i]);
-        for (int t = 0; t < ns; ++t) {
-            const int a = sl_part[first + t];
-            bw = std::max(bw, 6 * std::abs(a - b) + 5);
-            sl_bptr.push_back(base + scount[t]);  // scount[t] now ends slot t
-            slot_of[a] = -1;
-        }
-        sl_ptr.push_back((int)sl_part.size());
-        // chunks of the slot lists: at most CA_MAXCH per aggregate
-        if (ns > CA_MAXCH) return fail(OFEA_ERR_LIMIT, "an aggregate couples to too many aggregates");
-        const int nbl = sl_bptr.back() - base;
-        for (int cs = std::max(64, (nbl + CA_MAXCH - 1) / CA_MAXCH);; cs *= 2) {
-            int nch = 0;
-            for (int t = 0; t < ns; ++t) nch += (sl_bptr[first + t + 1] - sl_bptr[first + t] + cs - 1) / cs;
-            if (nch > CA_MAXCH) continue;
+    // per aggregate b: coupled aggregates a (ascending, "slots") and the
+    // blocks (r, k) of b's nodes with bcol in a, in node and block order,
+    // stored as (node, slot in its row) pairs.  b's blocks fill
+    // [nb_p[b L], nb_p[(b + 1) L]) of the list, so host threads over ranges of
+    // aggregates write disjoint parts; their slot and chunk lists are joined
+    // in aggregate order afterwards.
+    std::vector<int> sl_blk((size_t)2 * std::max(nblk, 1));  // uploaded as int2
+    struct AggLists {
+        std::vector<int> part, cptr, ch_beg, nslot, nchunk;  // cptr: into this range's ch_beg
+        int bw = 5;
+        bool too_many = false;
+    };
+    const int nthr = std::max(1, std::min({8, nagg, (int)std::max(1u, std::thread::hardware_concurrency())}));
+    std::vector<AggLists> lists(nthr);
+    auto build_range = [&](int th) {
+        AggLists &o = lists[th];
+        const int b_lo = (int)((long long)nagg * th / nthr), b_hi = (int)((long long)nagg * (th + 1) / nthr);
+        std::vector<int> slot_of(nagg, -1), scount, bstart, bp;
+        for (int b = b_lo; b < b_hi; ++b) {
+            const int i0 = b * L, i1 = std::min(nf, i0 + L);
+            const int q0 = nb_p[i0], q1 = nb_p[i1];
+            bp.resize(q1 - q0);
+            for (int q = q0; q < q1; ++q) bp[q - q0] = bcol_p[q] / L;  // aggregate of the block's row node
+            const int first = (int)o.part.size();
+            for (int q = 0; q < q1 - q0; ++q)
+                if (slot_of[bp[q]] < 0) {
+                    slot_of[bp[q]] = 0;
+                    o.part.push_back(bp[q]);
+                }
+            std::sort(o.part.begin() + first, o.part.end());
+            const int ns = (int)o.part.size() - first;
+            for (int t = 0; t < ns; ++t) slot_of[o.part[first + t]] = t;
+            scount.assign(ns + 1, 0);
+            for (int q = 0; q < q1 - q0; ++q) scount[slot_of[bp[q]] + 1]++;
+            for (int t = 0; t < ns; ++t) scount[t + 1] += scount[t];
+            bstart = scount;  // slot t: [q0 + bstart[t], q0 + bstart[t + 1])
+            for (int i = i0; i < i1; ++i)
+                for (int q = nb_p[i]; q < nb_p[i + 1]; ++q) {
+                    const size_t pos = (size_t)q0 + scount[slot_of[bp[q - q0]]]++;
+                    sl_blk[2 * pos] = i;
+                    sl_blk[2 * pos + 1] = q - nb_p[i];
+                }
             for (int t = 0; t < ns; ++t) {
-                sl_cptr.push_back((int)ch_beg.size());
-                for (int e = sl_bptr[first + t]; e < sl_bptr[first + t + 1]; e += cs) ch_beg.push_back(e);
+                const int a = o.part[first + t];
+                o.bw = std::max(o.bw, 6 * std::abs(a - b) + 5);
+                slot_of[a] = -1;
             }
-            ch_ptr.push_back((int)ch_beg.size());
-            break;
+            if (ns > CA_MAXCH) {
+                o.too_many = true;
+                return;
+            }
+            // chunks of the slot lists: at most CA_MAXCH per aggregate
+            const int nbl = q1 - q0, ch0 = (int)o.ch_beg.size();
+            for (int cs = std::max(64, (nbl + CA_MAXCH - 1) / CA_MAXCH);; cs *= 2) {
+                int nch = 0;
+                for (int t = 0; t < ns; ++t) nch += (bstart[t + 1] - bstart[t] + cs - 1) / cs;
+                if (nch > CA_MAXCH) continue;
+                for (int t = 0; t < ns; ++t) {
+                    o.cptr.push_back((int)o.ch_beg.size());
+                    for (int e = q0 + bstart[t]; e < q0 + bstart[t + 1]; e += cs) o.ch_beg.push_back(e);
+                }
+                break;
+            }
+            o.nslot.push_back(ns);
+            o.nchunk.push_back((int)o.ch_beg.size() - ch0);
         }
+    };
+    {
+        std::vector<std::thread> pool;
+        for (int th = 1; th < nthr; ++th) pool.emplace_back(build_range, th);
+        build_range(0);
+        for (auto &t : pool) t.join();
+    }
+    std::vector<int> sl_ptr(1, 0), sl_part, sl_cptr, ch_ptr(1, 0), ch_beg;
+    int bw = 5;
+    for (const AggLists &o : lists) {
+        if (o.too_many) return fail(OFEA_ERR_LIMIT, "an aggregate couples to too many aggregates");
+        bw = std::max(bw, o.bw);
+        const int coff = (int)ch_beg.size();
+        for (int ns : o.nslot) sl_ptr.push_back(sl_ptr.back() + ns);
+        for (int nch : o.nchunk) ch_ptr.push_back(ch_ptr.back() + nch);
+        sl_part.insert(sl_part.end(), o.part.begin(), o.part.end());
+        for (int c : o.cptr) sl_cptr.push_back(coff + c);
+        ch_beg.insert(ch_beg.end(), o.ch_beg.begin(), o.ch_beg.end());
     }
     sl_cptr.push_back((int)ch_beg.size());
-    ch_beg.push_back(sl_bptr.back());
+    ch_beg.push_back(nblk);
     const int nc = 6 * nagg;
     bw = std::min(bw, nc - 1);
     // device buffers (sizes depend only on nfree, nnz and nagg <= CG_NB / CG_AGG)
