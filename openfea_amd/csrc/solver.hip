@@ -1560,9 +1560,11 @@ __global__ __launch_bounds__(RED_WG) void k_band_upper(int nc, int bw, double *_
 // Column e of A^-1 = L^-T L^-1 e_e by one wave: forward then backward
 // substitution over the band (lane-split dot products, shuffle sums),
 // written as row e of inv (A^-1 is symmetric).  For bw < 64 INV_U the band
-// row of the next step is loaded while the current one is summed.
+// rows of the next INV_D - 1 steps are in flight while the current one is
+// summed (each step waits on the previous one, so a row load must not).
 constexpr int INV_WAVES = 4;
 constexpr int INV_U = 8;
+constexpr int INV_D = 4;  // rows of L in flight per wave
 __global__ __launch_bounds__(64 * INV_WAVES) void k_band_inverse(int nc, int bw, const double *__restrict__ L,
                                                                  double *__restrict__ inv)
 {
@@ -1579,30 +1581,42 @@ __global__ __launch_bounds__(64 * INV_WAVES) void k_band_inverse(int nc, int bw,
             const int lo = max(e, i - bw);
 #pragma unroll
             for (int u = 0; u < INV_U; ++u) {
+                // unconditional load of a clamped (in-range) index, then a
+                // select: loads under branches would make the compiler wait
+                // for every load in flight
                 const int k = lo + lane + 64 * u;
-                v[u] = (i < nc && k < i) ? L[(size_t)i * nc + k] : 0.0;
+                const bool ok = i < nc && k < i;
+                const double t = L[ok ? (size_t)i * nc + k : 0];
+                v[u] = ok ? t : 0.0;
             }
         };
-        double cur[INV_U], nxt[INV_U];
-        fload(e, cur);
-        double dcur = L[(size_t)e * nc + e];
-        for (int i = e; i < nc; ++i) {
-            fload(i + 1, nxt);
-            const double dnxt = i + 1 < nc ? L[(size_t)(i + 1) * nc + i + 1] : 1.0;
-            const int lo = max(e, i - bw);
-            double part = 0.0;
+        // rows i + 1 .. i + INV_D - 1 are in flight while row i is reduced
+        double buf[INV_D][INV_U], dg[INV_D];
 #pragma unroll
-            for (int u = 0; u < INV_U; ++u) {
-                const int k = lo + lane + 64 * u;
-                part += k < i ? cur[u] * y[min(k, nc - 1)] : 0.0;
+        for (int s = 0; s < INV_D; ++s) {
+            fload(e + s, buf[s]);
+            dg[s] = e + s < nc ? L[(size_t)(e + s) * nc + e + s] : 1.0;
+        }
+        for (int i0 = e; i0 < nc; i0 += INV_D) {
+#pragma unroll
+            for (int s = 0; s < INV_D; ++s) {
+                const int i = i0 + s;
+                if (i < nc) {  // the same for the whole wave
+                    const int lo = max(e, i - bw);
+                    double part = 0.0;
+#pragma unroll
+                    for (int u = 0; u < INV_U; ++u) {
+                        const int k = lo + lane + 64 * u;
+                        part += k < i ? buf[s][u] * y[min(k, nc - 1)] : 0.0;
+                    }
+                    part = wave_sum(part);
+                    const double yi = ((i == e ? 1.0 : 0.0) - part) / dg[s];
+                    if (lane == 0) y[i] = yi;
+                    wave_sync();
+                }
+                fload(i + INV_D, buf[s]);
+                dg[s] = i + INV_D < nc ? L[(size_t)(i + INV_D) * nc + i + INV_D] : 1.0;
             }
-            part = wave_sum(part);
-            const double yi = ((i == e ? 1.0 : 0.0) - part) / dcur;
-            if (lane == 0) y[i] = yi;
-            wave_sync();
-#pragma unroll
-            for (int u = 0; u < INV_U; ++u) cur[u] = nxt[u];
-            dcur = dnxt;
         }
         // backward, L^T x = y: row i of the upper triangle, k in (i, min(nc - 1, i + bw)]
         auto bload = [&](int i, double v[INV_U]) {
@@ -1610,29 +1624,37 @@ __global__ __launch_bounds__(64 * INV_WAVES) void k_band_inverse(int nc, int bw,
 #pragma unroll
             for (int u = 0; u < INV_U; ++u) {
                 const int k = i + 1 + lane + 64 * u;
-                v[u] = (i >= 0 && k <= hi) ? L[(size_t)i * nc + k] : 0.0;
+                const bool ok = i >= 0 && k <= hi;
+                const double t = L[ok ? (size_t)i * nc + k : 0];
+                v[u] = ok ? t : 0.0;
             }
         };
-        bload(nc - 1, cur);
-        dcur = L[(size_t)(nc - 1) * nc + nc - 1];
-        for (int i = nc - 1; i >= 0; --i) {
-            bload(i - 1, nxt);
-            const double dnxt = i > 0 ? L[(size_t)(i - 1) * nc + i - 1] : 1.0;
-            const int hi = min(nc - 1, i + bw);
-            double part = 0.0;
 #pragma unroll
-            for (int u = 0; u < INV_U; ++u) {
-                const int k = i + 1 + lane + 64 * u;
-                part += k <= hi ? cur[u] * y[min(k, nc - 1)] : 0.0;
+        for (int s = 0; s < INV_D; ++s) {
+            bload(nc - 1 - s, buf[s]);
+            dg[s] = nc - 1 - s >= 0 ? L[(size_t)(nc - 1 - s) * nc + nc - 1 - s] : 1.0;
+        }
+        for (int i0 = nc - 1; i0 >= 0; i0 -= INV_D) {
+#pragma unroll
+            for (int s = 0; s < INV_D; ++s) {
+                const int i = i0 - s;
+                if (i >= 0) {
+                    const int hi = min(nc - 1, i + bw);
+                    double part = 0.0;
+#pragma unroll
+                    for (int u = 0; u < INV_U; ++u) {
+                        const int k = i + 1 + lane + 64 * u;
+                        part += k <= hi ? buf[s][u] * y[min(k, nc - 1)] : 0.0;
+                    }
+                    part = wave_sum(part);
+                    const double xi = (y[i] - part) / dg[s];
+                    wave_sync();
+                    if (lane == 0) y[i] = xi;
+                    wave_sync();
+                }
+                bload(i - INV_D, buf[s]);
+                dg[s] = i - INV_D >= 0 ? L[(size_t)(i - INV_D) * nc + i - INV_D] : 1.0;
             }
-            part = wave_sum(part);
-            const double xi = (y[i] - part) / dcur;
-            wave_sync();
-            if (lane == 0) y[i] = xi;
-            wave_sync();
-#pragma unroll
-            for (int u = 0; u < INV_U; ++u) cur[u] = nxt[u];
-            dcur = dnxt;
         }
     } else {
         for (int i = e; i < nc; ++i) {  // L y = e_e
