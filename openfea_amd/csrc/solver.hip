@@ -842,6 +842,19 @@ __global__ void k_precond_setup(int nfree, const int *__restrict__ node_blk, con
 // Reordered system of the two-level methods: the node rows of K copied into
 // the new node order (64 lanes per node; a row keeps its block order, its
 // neighbour indices are renumbered on the host).
+// bcol of the reordered pattern: node row i of the new order is row perm[i]
+// of K with its block order kept and each neighbour renumbered by iperm
+__global__ __launch_bounds__(RED_WG) void k_perm_bcol(int nfree, const int *__restrict__ perm,
+                                                      const int *__restrict__ iperm, const int *__restrict__ node_blk,
+                                                      const int *__restrict__ bcol, const int *__restrict__ nb_p,
+                                                      int *__restrict__ bcol_p)
+{
+    const int i = blockIdx.x * RED_WG + (int)threadIdx.x;
+    if (i >= nfree) return;
+    const int o = perm[i], o0 = node_blk[o], deg = node_blk[o + 1] - o0, q0 = nb_p[i];
+    for (int k = 0; k < deg; ++k) bcol_p[q0 + k] = iperm[bcol[o0 + k]];
+}
+
 __global__ __launch_bounds__(RED_WG) void k_perm_vals(int nfree, const int *__restrict__ perm,
                                                       const int *__restrict__ node_blk, const int *__restrict__ nb_p,
                                                       const double *__restrict__ val, double *__restrict__ val_p)
@@ -1331,7 +1344,7 @@ __global__ __launch_bounds__(1024) void k_coarse_ac(int nc, const int *__restric
                                                     const float *__restrict__ d, const int *__restrict__ sl_ptr,
                                                     const int *__restrict__ sl_part, const int *__restrict__ sl_cptr,
                                                     const int *__restrict__ ch_ptr, const int *__restrict__ ch_beg,
-                                                    const int2 *__restrict__ sl_blk, double *__restrict__ A)
+                                                    const int *__restrict__ sl_blk, double *__restrict__ A)
 {
     __shared__ double s_part[CA_MAXCH * 36];
     const int b = blockIdx.x;
@@ -1340,8 +1353,8 @@ __global__ __launch_bounds__(1024) void k_coarse_ac(int nc, const int *__restric
         const int ch = c0 + t / 36, pq = t % 36, p = pq / 6, q = pq % 6;
         double acc = 0.0;
         for (int e = ch_beg[ch]; e < ch_beg[ch + 1]; ++e) {
-            const int2 rk = sl_blk[e];
-            const int r = rk.x, k = rk.y;
+            const int rk = sl_blk[e];  // (node << 8) | slot in the node's row
+            const int r = rk >> 8, k = rk & 255;
             const int nb0 = nb_p[r], deg = nb_p[r + 1] - nb0;
             double zs[3], zr[3];
             zcol(d + 3 * (size_t)bcol_p[nb0 + k], p, zs);
@@ -1731,20 +1744,18 @@ static int coarse_structure(ofea_ctx *ctx)
     for (int i = 0; i < nf; ++i)
         for (int c = 0; c < 3; ++c) d[3 * (size_t)i + c] = (float)(X[3 * (size_t)perm[i] + c] - cen[3 * (i / L) + c]);
     // renumbered pattern: node rows in the new order, each keeping its block order
+    // (bcol_p itself is built on the device by k_perm_bcol)
     const int nblk = ctx->P.node_blk[nf];
-    std::vector<int> nb_p((size_t)nf + 1, 0), bcol_p((size_t)std::max(nblk, 1));
-    for (int i = 0; i < nf; ++i) {
-        const int o = perm[i], o0 = ctx->P.node_blk[o], deg = ctx->P.node_blk[o + 1] - o0;
-        nb_p[i + 1] = nb_p[i] + deg;
-        for (int k = 0; k < deg; ++k) bcol_p[(size_t)nb_p[i] + k] = iperm[ctx->P.bcol[(size_t)o0 + k]];
-    }
+    if (nf >= (1 << 23)) return fail(OFEA_ERR_LIMIT, "too many free nodes for the packed coarse block lists");
+    std::vector<int> nb_p((size_t)nf + 1, 0);
+    for (int i = 0; i < nf; ++i) nb_p[i + 1] = nb_p[i] + ctx->P.node_blk[perm[i] + 1] - ctx->P.node_blk[perm[i]];
     // per aggregate b: coupled aggregates a (ascending, "slots") and the
     // blocks (r, k) of b's nodes with bcol in a, in node and block order,
     // stored as (node, slot in its row) pairs.  b's blocks fill
     // [nb_p[b L], nb_p[(b + 1) L]) of the list, so host threads over ranges of
     // aggregates write disjoint parts; their slot and chunk lists are joined
     // in aggregate order afterwards.
-    std::vector<int> sl_blk((size_t)2 * std::max(nblk, 1));  // uploaded as int2
+    std::vector<int> sl_blk((size_t)std::max(nblk, 1));  // (node << 8) | slot in the node's row
     struct AggLists {
         std::vector<int> part, cptr, ch_beg, nslot, nchunk;  // cptr: into this range's ch_beg
         int bw = 5;
@@ -1759,8 +1770,11 @@ static int coarse_structure(ofea_ctx *ctx)
         for (int b = b_lo; b < b_hi; ++b) {
             const int i0 = b * L, i1 = std::min(nf, i0 + L);
             const int q0 = nb_p[i0], q1 = nb_p[i1];
-            bp.resize(q1 - q0);
-            for (int q = q0; q < q1; ++q) bp[q - q0] = bcol_p[q] / L;  // aggregate of the block's row node
+            bp.resize(q1 - q0);  // aggregate of each block's row node
+            for (int i = i0; i < i1; ++i) {
+                const int o0 = ctx->P.node_blk[perm[i]];
+                for (int q = nb_p[i]; q < nb_p[i + 1]; ++q) bp[q - q0] = iperm[ctx->P.bcol[(size_t)o0 + q - nb_p[i]]] / L;
+            }
             const int first = (int)o.part.size();
             for (int q = 0; q < q1 - q0; ++q)
                 if (slot_of[bp[q]] < 0) {
@@ -1777,8 +1791,7 @@ static int coarse_structure(ofea_ctx *ctx)
             for (int i = i0; i < i1; ++i)
                 for (int q = nb_p[i]; q < nb_p[i + 1]; ++q) {
                     const size_t pos = (size_t)q0 + scount[slot_of[bp[q - q0]]]++;
-                    sl_blk[2 * pos] = i;
-                    sl_blk[2 * pos + 1] = q - nb_p[i];
+                    sl_blk[pos] = (i << 8) | (q - nb_p[i]);
                 }
             for (int t = 0; t < ns; ++t) {
                 const int a = o.part[first + t];
@@ -1854,7 +1867,7 @@ static int coarse_structure(ofea_ctx *ctx)
         OFEA_HIP(hipMalloc((void **)&w->minv32_p, sizeof(float) * 9 * nfb));
         OFEA_HIP(hipMalloc((void **)&w->sl_ptr, sizeof(int) * (CG_NB / CG_AGG + 1)));
         OFEA_HIP(hipMalloc((void **)&w->ch_ptr, sizeof(int) * (CG_NB / CG_AGG + 1)));
-        OFEA_HIP(hipMalloc((void **)&w->sl_blk, sizeof(int2) * (size_t)std::max(nblk, 1)));
+        OFEA_HIP(hipMalloc((void **)&w->sl_blk, sizeof(int) * (size_t)std::max(nblk, 1)));
     }
     for (int **p : {&w->sl_part, &w->sl_cptr, &w->ch_beg}) {
         if (*p) (void)hipFree(*p);
@@ -1872,10 +1885,11 @@ static int coarse_structure(ofea_ctx *ctx)
         OFEA_HIP(hipMemcpyAsync(w->perm, perm.data(), sizeof(int) * nf, hipMemcpyHostToDevice, st));
         OFEA_HIP(hipMemcpyAsync(w->iperm, iperm.data(), sizeof(int) * nf, hipMemcpyHostToDevice, st));
         OFEA_HIP(hipMemcpyAsync(w->nb_p, nb_p.data(), sizeof(int) * (nf + 1), hipMemcpyHostToDevice, st));
-        OFEA_HIP(hipMemcpyAsync(w->bcol_p, bcol_p.data(), sizeof(int) * (size_t)std::max(nblk, 1),
+        OFEA_HIP(hipMemcpyAsync(w->sl_blk, sl_blk.data(), sizeof(int) * (size_t)std::max(nblk, 1),
                                 hipMemcpyHostToDevice, st));
-        OFEA_HIP(hipMemcpyAsync(w->sl_blk, sl_blk.data(), sizeof(int2) * (size_t)std::max(nblk, 1),
-                                hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_perm_bcol, dim3((nf + RED_WG - 1) / RED_WG), dim3(RED_WG), 0, st, nf, w->perm, w->iperm,
+                           ctx->d_node_blk, ctx->d_bcol, w->nb_p, w->bcol_p);
+        OFEA_HIP(hipGetLastError());
     }
     OFEA_HIP(hipMemcpyAsync(w->sl_ptr, sl_ptr.data(), sizeof(int) * sl_ptr.size(), hipMemcpyHostToDevice, st));
     if (!sl_part.empty())

@@ -93,9 +93,9 @@ struct ofea_solver_ws {
     bool co_struct_valid = false;  // order, pattern and block lists built for this mesh
     int co_bw = 0;                 // half-bandwidth of A_c
     // A_c block lists: slots (coupled aggregates) per aggregate, chunks per
-    // slot and per aggregate, chunk starts, blocks (node, slot in its row)
+    // slot and per aggregate, chunk starts, blocks ((node << 8) | slot in its row)
     int *sl_ptr = nullptr, *sl_part = nullptr, *sl_cptr = nullptr, *ch_ptr = nullptr, *ch_beg = nullptr;
-    int2 *sl_blk = nullptr;
+    int *sl_blk = nullptr;
     double *co_A = nullptr, *co_inv = nullptr;  // A_c (then L and L^T) and its inverse, FP64
     int *co_flag = nullptr;
     hipGraph_t graph[ofea::NSLOT] = {};
