@@ -1302,7 +1302,20 @@ static void coarse_order(int nf, const double *X, int L, std::vector<int> &perm)
     const int per_slab = (nparts + slabs - 1) / slabs;  // parts per slab
     std::vector<AxisKey> key(nf);
     for (int i = 0; i < nf; ++i) key[i] = {X[3 * (size_t)i + ax], i};
-    std::sort(key.begin(), key.end());
+    {
+        // sorted in up to 8 ranges on host threads, then merged pairwise
+        const int ns = nf < 16384 ? 1 : std::max(1, std::min(8, (int)std::max(1u, std::thread::hardware_concurrency())));
+        std::vector<size_t> cut(ns + 1);
+        for (int t = 0; t <= ns; ++t) cut[t] = (size_t)nf * t / ns;
+        std::vector<std::thread> pool;
+        for (int t = 1; t < ns; ++t)
+            pool.emplace_back([&key, &cut, t] { std::sort(key.begin() + cut[t], key.begin() + cut[t + 1]); });
+        std::sort(key.begin() + cut[0], key.begin() + cut[1]);
+        for (auto &th : pool) th.join();
+        for (int w = 1; w < ns; w *= 2)
+            for (int t = 0; t + w < ns; t += 2 * w)
+                std::inplace_merge(key.begin() + cut[t], key.begin() + cut[t + w], key.begin() + cut[std::min(ns, t + 2 * w)]);
+    }
     for (int i = 0; i < nf; ++i) perm[i] = key[i].i;
     // the slabs are cut independently, on up to 8 host threads
     const long long slab_nodes = (long long)per_slab * L;
