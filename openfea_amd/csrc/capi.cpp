@@ -80,6 +80,42 @@ static void free_mesh(ofea_ctx *c)
     c->rhs_set = false;
 }
 
+// forget the host side of the mesh (device buffers are freed by free_mesh)
+static void reset_mesh(ofea_ctx *c)
+{
+    c->nd = c->ne = 0;
+    c->xyz.clear();
+    c->conn.clear();
+    c->etype.clear();
+    c->resort.clear();
+    c->E_elem.clear();
+    c->P = Pattern();
+    for (int t = 0; t < 2; ++t) {
+        c->G[t] = GatherPlan();
+        c->has_type[t] = false;
+    }
+    c->kver++;
+}
+
+static int upload_mesh(ofea_ctx *c)
+{
+    int rc;
+    if ((rc = dev_upload(c->d_xyz, c->xyz.data(), c->xyz.size()))) return rc;
+    if ((rc = dev_upload(c->d_col_ptr, c->P.col_ptr.data(), c->P.col_ptr.size()))) return rc;
+    if ((rc = dev_upload(c->d_node_blk, c->P.node_blk.data(), c->P.node_blk.size()))) return rc;
+    if ((rc = dev_upload(c->d_bcol, c->P.bcol.data(), c->P.bcol.size()))) return rc;
+    if ((rc = dev_upload(c->d_diag_k, c->P.diag_k.data(), std::max<size_t>(c->P.diag_k.size(), 1)))) return rc;
+    if ((rc = dev_upload(c->d_resort, c->resort.data(), c->resort.size()))) return rc;
+    if ((rc = dev_upload<double>(c->d_val, nullptr, (size_t)std::max(c->P.nnz, 1LL)))) return rc;
+    if ((rc = dev_upload<double>(c->d_rhs, nullptr, (size_t)std::max(c->P.n, 1)))) return rc;
+    if ((rc = dev_upload<double>(c->d_x, nullptr, (size_t)std::max(c->P.n, 1)))) return rc;
+    for (int t = 0; t < 2; ++t) {
+        if (!c->has_type[t]) continue;
+        if ((rc = upload_plan(c->G[t], c->dp[t]))) return rc;
+    }
+    return OFEA_OK;
+}
+
 #define CHECK_CTX(c)                                                  \
     do {                                                              \
         if (!(c)) return fail(OFEA_ERR_ARG, "null context");         \
@@ -132,35 +168,38 @@ int ofea_set_mesh(ofea_ctx *c, int nd, const double *xyz, int ne, const int *con
         if (resort[i] >= nd) return fail(OFEA_ERR_ARG, "resort entry out of range");
     hipStreamSynchronize(c->stream);
     free_mesh(c);
-    c->nd = nd;
-    c->ne = ne;
+    reset_mesh(c);
+    // The host structures are built into locals and the context takes them
+    // only when all of them (and the uploads) succeeded: after a failure the
+    // context holds no mesh and every later call reports OFEA_ERR_STATE.
+    Pattern P;
+    GatherPlan G[2];
+    bool has_type[2] = {false, false};
+    int rc = build_pattern(nd, ne, conn, etype, resort, P);
+    if (rc) return rc;
+    for (int t = 0; t < 2; ++t) {
+        has_type[t] = std::find(etype, etype + ne, t) != etype + ne;
+        if (!has_type[t]) continue;
+        rc = build_gather_plan(nd, ne, conn, etype, resort, P, t, plan_caps(t), G[t]);
+        if (rc) return rc;
+    }
     c->xyz.assign(xyz, xyz + 3 * (size_t)nd);
     c->conn.assign(conn, conn + 8 * (size_t)ne);
     c->etype.assign(etype, etype + ne);
     c->resort.assign(resort, resort + nd);
-    c->E_elem.clear();
-    int rc = build_pattern(nd, ne, conn, etype, resort, c->P);
-    if (rc) return rc;
+    c->P = std::move(P);
     for (int t = 0; t < 2; ++t) {
-        c->has_type[t] = std::find(etype, etype + ne, t) != etype + ne;
-        if (!c->has_type[t]) continue;
-        rc = build_gather_plan(nd, ne, conn, etype, resort, c->P, t, plan_caps(t), c->G[t]);
-        if (rc) return rc;
+        c->G[t] = std::move(G[t]);
+        c->has_type[t] = has_type[t];
     }
-    // upload
-    if ((rc = dev_upload(c->d_xyz, c->xyz.data(), c->xyz.size()))) return rc;
-    if ((rc = dev_upload(c->d_col_ptr, c->P.col_ptr.data(), c->P.col_ptr.size()))) return rc;
-    if ((rc = dev_upload(c->d_node_blk, c->P.node_blk.data(), c->P.node_blk.size()))) return rc;
-    if ((rc = dev_upload(c->d_bcol, c->P.bcol.data(), c->P.bcol.size()))) return rc;
-    if ((rc = dev_upload(c->d_diag_k, c->P.diag_k.data(), std::max<size_t>(c->P.diag_k.size(), 1)))) return rc;
-    if ((rc = dev_upload(c->d_resort, c->resort.data(), c->resort.size()))) return rc;
-    if ((rc = dev_upload<double>(c->d_val, nullptr, (size_t)std::max(c->P.nnz, 1LL)))) return rc;
-    if ((rc = dev_upload<double>(c->d_rhs, nullptr, (size_t)std::max(c->P.n, 1)))) return rc;
-    if ((rc = dev_upload<double>(c->d_x, nullptr, (size_t)std::max(c->P.n, 1)))) return rc;
-    for (int t = 0; t < 2; ++t) {
-        if (!c->has_type[t]) continue;
-        if ((rc = upload_plan(c->G[t], c->dp[t]))) return rc;
+    rc = upload_mesh(c);
+    if (rc) {
+        free_mesh(c);
+        reset_mesh(c);
+        return rc;
     }
+    c->nd = nd;
+    c->ne = ne;
     return OFEA_OK;
 }
 
@@ -224,19 +263,17 @@ int ofea_spmv(ofea_ctx *c, const double *x, double *y)
     CHECK_CTX(c);
     if (!c->assembled) return fail(OFEA_ERR_STATE, "ofea_assemble first");
     if (!x || !y) return fail(OFEA_ERR_ARG, "null vector");
-    size_t n = c->P.n;
+    // staged through two n-vectors of the solver work space (allocated once
+    // per mesh, reused by every call)
     double *dx = nullptr, *dy = nullptr;
-    OFEA_HIP(hipMalloc((void **)&dx, sizeof(double) * std::max<size_t>(n, 1)));
-    OFEA_HIP(hipMalloc((void **)&dy, sizeof(double) * std::max<size_t>(n, 1)));
+    int rc = solver_staging(c, &dx, &dy);
+    if (rc) return rc;
+    const size_t n = c->P.n;
     OFEA_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    int rc = solver_spmv(c, dx, dy);
-    if (rc == OFEA_OK) {
-        hipMemcpyAsync(y, dy, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
-        hipStreamSynchronize(c->stream);
-    }
-    hipFree(dx);
-    hipFree(dy);
-    return rc;
+    if ((rc = solver_spmv(c, dx, dy))) return rc;
+    OFEA_HIP(hipMemcpyAsync(y, dy, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    OFEA_HIP(hipStreamSynchronize(c->stream));
+    return OFEA_OK;
 }
 
 int ofea_set_rhs(ofea_ctx *c, const double *b)

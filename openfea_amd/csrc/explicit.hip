@@ -329,6 +329,12 @@ static int run_t(ofea_ctx *c, double E, double nu, const double *mass, const dou
     ExplicitWS *w = ews(c);
     const int nd = c->nd;
     const size_t n3 = 3 * (size_t)nd;
+    for (int i = 0; i < nd; ++i)
+        if (!(mass[i] > 0.0)) return fail(OFEA_ERR_ARG, "nodal mass must be positive");
+    // The blocking copies below run on the null stream, which the context's
+    // non-blocking stream does not order against: a previous run whose last
+    // steps still read v, fext or minv must have finished first.
+    OFEA_HIP(hipStreamSynchronize(c->stream));
     if (w->prec != (int)sizeof(T)) {
         if (w->exec) hipGraphExecDestroy(w->exec);
         if (w->graph) hipGraphDestroy(w->graph);
@@ -359,10 +365,7 @@ static int run_t(ofea_ctx *c, double E, double nu, const double *mass, const dou
     for (size_t i = 0; i < n3; ++i) h[i] = v0 ? (T)v0[i] : T(0);
     OFEA_HIP(hipMemcpy(w->v, h.data(), sizeof(T) * n3, hipMemcpyHostToDevice));
     std::vector<T> mi(nd);
-    for (int i = 0; i < nd; ++i) {
-        if (!(mass[i] > 0.0)) return fail(OFEA_ERR_ARG, "nodal mass must be positive");
-        mi[i] = (T)(1.0 / mass[i]);
-    }
+    for (int i = 0; i < nd; ++i) mi[i] = (T)(1.0 / mass[i]);
     OFEA_HIP(hipMemcpy(w->minv, mi.data(), sizeof(T) * nd, hipMemcpyHostToDevice));
     OFEA_HIP(hipMemcpy(w->fixed, fixed_mask, sizeof(int) * nd, hipMemcpyHostToDevice));
     double lam_d, mu_d;

@@ -113,18 +113,65 @@ Cross cross(Cross a, Cross b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x *
 
 }  // namespace
 
+// Abaqus part/instance scoping.  Sets defined between *Part and *End Part
+// belong to that part and are stored under "part.set" (and, for decks that
+// name them unqualified, under "set" as well); *Instance, name=I, part=P lets
+// the assembly and the steps name them "I.set" (the reference's own decks,
+// e.g. sample_model/NLFEA.inp:82956-82978, write "Part-1-1.Set-fix").  Ids
+// written inside a part, or qualified by an instance, are labels of that
+// part: with the reference's line-order numbering, node L of a part is
+// global index (nodes of the earlier parts) + L - 1, and elements likewise.
+struct Scope {
+    std::string part;  // lower case; empty outside *Part ... *End Part
+    int node_off = 0, elem_off = 0;
+};
+struct PartInfo {
+    int node_off = 0, elem_off = 0;
+};
+
 // Resolves a "node id or node set" target to 0-based node indices.
 static int resolve(const std::map<std::string, std::vector<int>> &sets, const std::string &tok, int count,
-                   std::vector<int> &out, const char *what)
+                   std::vector<int> &out, const char *what, const Scope &sc = Scope(),
+                   const std::map<std::string, std::string> *inst = nullptr,
+                   const std::map<std::string, PartInfo> *parts = nullptr, bool nodes = true)
 {
     out.clear();
     long id;
-    if (to_int(tok, id)) {
-        if (id < 1 || id > count) return fail(OFEA_ERR_IO, std::string(what) + " id out of range: " + tok);
-        out.push_back((int)id - 1);
+    auto by_id = [&](long lab, int off) -> int {
+        const long g = lab + off;
+        if (lab < 1 || g > count) return fail(OFEA_ERR_IO, std::string(what) + " id out of range: " + tok);
+        out.push_back((int)g - 1);
         return OFEA_OK;
+    };
+    if (to_int(tok, id)) return by_id(id, nodes ? sc.node_off : sc.elem_off);
+    const std::string t = lower(tok);
+    // instance-qualified: "instance.set" or "instance.label"
+    const size_t dot = t.find('.');
+    if (dot != std::string::npos && inst) {
+        auto ii = inst->find(t.substr(0, dot));
+        if (ii != inst->end()) {
+            const std::string rest = t.substr(dot + 1);
+            int off = 0;
+            if (parts) {
+                auto pi = parts->find(ii->second);
+                if (pi != parts->end()) off = nodes ? pi->second.node_off : pi->second.elem_off;
+            }
+            if (to_int(rest, id)) return by_id(id, off);
+            auto it = sets.find(ii->second + "." + rest);
+            if (it != sets.end()) {
+                out = it->second;
+                return OFEA_OK;
+            }
+        }
     }
-    auto it = sets.find(lower(tok));
+    if (!sc.part.empty()) {  // inside a part its own sets come first
+        auto it = sets.find(sc.part + "." + t);
+        if (it != sets.end()) {
+            out = it->second;
+            return OFEA_OK;
+        }
+    }
+    auto it = sets.find(t);
     if (it == sets.end()) return fail(OFEA_ERR_IO, std::string("unknown ") + what + " set: " + tok);
     out = it->second;
     return OFEA_OK;
@@ -153,8 +200,13 @@ int parse_inp_ext(const std::string &path, ofea_deck &D)
     D.nu_elem.assign((size_t)ne, NAN);
     D.rho_elem.assign((size_t)ne, NAN);
     std::vector<int> tgt;
-    auto set_members = [&](const Keyword &k, bool nodes, std::vector<int> &members) -> int {
+    Scope scope;                                // current part (empty at model/assembly level)
+    std::map<std::string, PartInfo> parts;      // part -> offsets of its nodes and elements
+    std::map<std::string, std::string> inst;    // instance -> part
+    int node_rows = 0, elem_rows = 0;           // *Node / *Element data lines read so far
+    auto set_members = [&](const Keyword &k, bool nodes, const Scope &sc, std::vector<int> &members) -> int {
         const int count = nodes ? nd : ne;
+        const int off = nodes ? sc.node_off : sc.elem_off;
         auto &pool = nodes ? nsets : elsets;
         const bool gen = k.params.count("generate") > 0;
         for (const auto &row : k.data) {
@@ -164,29 +216,65 @@ int parse_inp_ext(const std::string &path, ofea_deck &D)
                     st <= 0)
                     return fail(OFEA_ERR_IO, "bad generate line in *" + k.name);
                 for (long i = a; i <= b; i += st) {
-                    if (i < 1 || i > count) return fail(OFEA_ERR_IO, "generated id out of range in *" + k.name);
-                    members.push_back((int)i - 1);
+                    if (i < 1 || i + off > count) return fail(OFEA_ERR_IO, "generated id out of range in *" + k.name);
+                    members.push_back((int)(i + off) - 1);
                 }
                 continue;
             }
             for (const auto &tok : row) {
                 if (tok.empty()) continue;
                 std::vector<int> r;
-                int rc2 = resolve(pool, tok, count, r, nodes ? "node" : "element");
+                int rc2 = resolve(pool, tok, count, r, nodes ? "node" : "element", sc, &inst, &parts, nodes);
                 if (rc2) return rc2;
                 members.insert(members.end(), r.begin(), r.end());
             }
         }
         return OFEA_OK;
     };
+    // node set or node id of a *Boundary / *Cload / *Initial Conditions line
+    auto resolve_nodes = [&](const std::string &tok, std::vector<int> &out) {
+        return resolve(nsets, tok, nd, out, "node", scope, &inst, &parts, true);
+    };
     for (const auto &k : kws) {
-        if (k.name == "nset" || k.name == "elset") {
+        if (k.name == "part") {
+            auto nm = k.params.find("name");
+            scope.part = nm == k.params.end() ? std::string("part") : lower(nm->second);
+            scope.node_off = node_rows;
+            scope.elem_off = elem_rows;
+            parts[scope.part] = PartInfo{node_rows, elem_rows};
+        } else if (k.name == "end part") {
+            scope = Scope();
+        } else if (k.name == "instance") {
+            auto nm = k.params.find("name");
+            auto pt = k.params.find("part");
+            if (nm != k.params.end() && pt != k.params.end()) inst[lower(nm->second)] = lower(pt->second);
+        } else if (k.name == "node") {
+            node_rows += (int)k.data.size();
+        } else if (k.name == "element") {
+            elem_rows += (int)k.data.size();
+        } else if (k.name == "nset" || k.name == "elset") {
             const bool nodes = k.name == "nset";
             auto it = k.params.find(nodes ? "nset" : "elset");
             if (it == k.params.end()) continue;
+            // an assembly-level set of one instance ("instance=I") takes labels of that part
+            Scope sc = scope;
+            auto in = k.params.find("instance");
+            if (scope.part.empty() && in != k.params.end()) {
+                auto ii = inst.find(lower(in->second));
+                if (ii == inst.end()) return fail(OFEA_ERR_IO, "*" + k.name + " names an unknown instance " + in->second);
+                const PartInfo &pi = parts[ii->second];
+                sc.node_off = pi.node_off;
+                sc.elem_off = pi.elem_off;
+            }
             std::vector<int> m;
-            if ((rc = set_members(k, nodes, m))) return rc;
-            auto &dst = (nodes ? nsets : elsets)[lower(it->second)];
+            if ((rc = set_members(k, nodes, sc, m))) return rc;
+            auto &pool = nodes ? nsets : elsets;
+            const std::string name = lower(it->second);
+            if (!scope.part.empty()) {
+                auto &q = pool[scope.part + "." + name];
+                q.insert(q.end(), m.begin(), m.end());
+            }
+            auto &dst = pool[name];
             dst.insert(dst.end(), m.begin(), m.end());
         } else if (k.name == "surface") {
             auto ty = k.params.find("type");
@@ -200,7 +288,7 @@ int parse_inp_ext(const std::string &path, ofea_deck &D)
                 if (f.size() < 2 || f[0] != 's') return fail(OFEA_ERR_IO, "bad face label " + row[1]);
                 const int face = std::atoi(f.c_str() + 1) - 1;
                 std::vector<int> els;
-                if ((rc = resolve(elsets, row[0], ne, els, "element"))) return rc;
+                if ((rc = resolve(elsets, row[0], ne, els, "element", scope, &inst, &parts, false))) return rc;
                 for (int e : els) {
                     const int nf = D.etype[e] == OFEA_C3D8 ? 6 : 4;
                     if (face < 0 || face >= nf) return fail(OFEA_ERR_IO, "face label out of range: " + row[1]);
@@ -228,7 +316,10 @@ int parse_inp_ext(const std::string &path, ofea_deck &D)
             auto es = k.params.find("elset");
             auto mt = k.params.find("material");
             if (es == k.params.end() || mt == k.params.end()) continue;
-            D.sections.emplace_back(lower(es->second), lower(mt->second));
+            // a section inside a part names that part's element set
+            const std::string set = lower(es->second);
+            const bool own = !scope.part.empty() && elsets.count(scope.part + "." + set);
+            D.sections.emplace_back(own ? scope.part + "." + set : set, lower(mt->second));
         } else if (k.name == "cload" || (k.name == "initial conditions" && k.params.count("type") &&
                                           lower(k.params.at("type")) == "velocity")) {
             const bool vel = k.name != "cload";
@@ -237,7 +328,7 @@ int parse_inp_ext(const std::string &path, ofea_deck &D)
                 double val;
                 if (row.size() < 3 || !to_int(row[1], dof) || !to_double(row[2], val))
                     return fail(OFEA_ERR_IO, "bad *" + k.name + " line");
-                if ((rc = resolve(nsets, row[0], nd, tgt, "node"))) return rc;
+                if ((rc = resolve_nodes(row[0], tgt))) return rc;
                 if (dof < 1 || dof > 3) continue;  // rotational DOFs do not exist on solid nodes
                 for (int i : tgt) {
                     if (vel)
@@ -249,7 +340,7 @@ int parse_inp_ext(const std::string &path, ofea_deck &D)
         } else if (k.name == "boundary") {
             for (const auto &row : k.data) {
                 if (row.size() < 2) return fail(OFEA_ERR_IO, "bad *Boundary line");
-                if ((rc = resolve(nsets, row[0], nd, tgt, "node"))) return rc;
+                if ((rc = resolve_nodes(row[0], tgt))) return rc;
                 int mask = 0;
                 long a, b;
                 const std::string t = lower(row[1]);

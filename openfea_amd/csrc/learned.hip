@@ -644,6 +644,9 @@ int learned_set_samples(ofea_ctx *c, const double *E, const double *R, int S)
     if (rc) return rc;
     LearnedWS *w = lws(c);
     const size_t n = 3 * (size_t)w->nfree;
+    // the buffers below are freed and refilled by blocking copies on the null
+    // stream: kernels of the context's stream must not still read them
+    OFEA_HIP(hipStreamSynchronize(c->stream));
     for (double **p : {&w->E, &w->R, &w->Dv, &w->Q, &w->wsmp})
         if (*p) {
             (void)hipFree(*p);

@@ -180,6 +180,9 @@ int upload_plan(const GatherPlan &G, DevPlan &d);
 void free_plan(DevPlan &d);
 // solver (solver.hip)
 int solver_spmv(ofea_ctx *ctx, const double *d_x, double *d_y);
+// two n-vectors of the solver work space for host<->device staging between
+// solves (their contents do not survive a solve)
+int solver_staging(ofea_ctx *ctx, double **d_x, double **d_y);
 // y = A x for any FP64 value array on the pattern of K
 int solver_spmv_vals(ofea_ctx *ctx, const double *d_vals, const double *d_x, double *d_y);
 int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, double *relres);

@@ -11,7 +11,11 @@
 // PCG iteration = 2 kernels: (p = z + beta p folded into the SpMV, q = K p,
 // <p,q>) and (x += alpha p, r -= alpha q, z = M r, <r,z>, <r,r>).  Both run on
 // a fixed grid; each workgroup reduces the previous kernel's per-workgroup
-// partials itself in a fixed order (deterministic, no atomics).  Batches of
+// partials itself in a fixed order (deterministic; the two iteration kernels
+// use no atomics).  The stand-alone kernels outside the iteration (the SpMV
+// with a dot product and the true residual) find their last workgroup by one
+// atomicAdd ticket per workgroup; that workgroup then sums the partials in a
+// fixed order, so their results are deterministic as well.  Batches of
 // iterations are captured in a hipGraph; kernels return at once after
 // convergence and the host reads the scalars between graph launches.
 //
@@ -1009,6 +1013,15 @@ int solver_spmv(ofea_ctx *ctx, const double *d_x, double *d_y)
     if (rc) return rc;
     launch_spmv_t<double, double, false>(ctx, ctx->ws->spmv_group, ctx->d_val, d_x, d_y);
     OFEA_HIP(hipGetLastError());
+    return OFEA_OK;
+}
+
+int solver_staging(ofea_ctx *ctx, double **d_x, double **d_y)
+{
+    int rc = ensure_ws(ctx);
+    if (rc) return rc;
+    *d_x = ctx->ws->r64;  // both are recomputed at the start of every solve
+    *d_y = ctx->ws->y64;
     return OFEA_OK;
 }
 

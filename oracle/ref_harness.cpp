@@ -3,8 +3,12 @@
 // TEST INFRASTRUCTURE ONLY.  Compiled together with the reference's element
 // translation units, taken where they lie under the reference checkout
 // (src/Elements/C3D8.cpp, C3D8_Gauss.cpp, C3D4.cpp, C3D4_hammer.cpp), into
-// oracle/_ref/libofea_ref.so by oracle/Makefile.  Only the golden-vector
-// script under tests/golden/ loads it; nothing of it ships or travels.
+// oracle/_ref/libofea_ref.so by oracle/Makefile.  The library is git-ignored
+// (no reference code enters the history) but, like every library built in
+// the tree, it is copied with the working tree to the machines that run the
+// tests and the benchmark.  It is loaded only as a checker or a CPU baseline:
+// by the golden-vector script under tests/golden/, by tests that compare the
+// restatement with it where it exists, and by bench.py's CPU-baseline leg.
 //
 // Each entry point instantiates the registered element class
 // (src/Elements/C3D8.cpp:19, src/Elements/C3D4.cpp:19), sets the material

@@ -72,6 +72,18 @@ def test_api_errors_and_recovery():
         fr = i32(np.arange(len(fx)))
         assert L.ofea_set_mesh(h, len(fx), p(fx), len(fe), p(fc), p(fe), p(fr)) == A.OFEA_ERR_LIMIT
         assert b"neighbour" in L.ofea_last_error()
+        say("errors: after a failed set_mesh the context holds no mesh")
+        nn = C.c_int(-1)
+        assert L.ofea_pattern(h, C.byref(nn), None, None, None) == A.OFEA_ERR_STATE
+        assert L.ofea_assemble(h, mg.E_STEEL, mg.NU_STEEL, None) == A.OFEA_ERR_STATE
+        y = np.zeros(3)
+        assert L.ofea_spmv(h, p(b), p(y)) == A.OFEA_ERR_STATE
+        assert L.ofea_solve(h, A.PCG_JACOBI, p(b), p(x), 1e-8, 100, None, None) == A.OFEA_ERR_STATE
+        uu = np.zeros(3)
+        assert L.ofea_nl_solve(h, mg.E_STEEL, mg.NU_STEEL, p(b), 1, 1e-8, 5, A.PCG_JACOBI, 1e-10, p(uu),
+                               None, None, None) == A.OFEA_ERR_STATE
+        assert L.ofea_nl_assemble(h, mg.E_STEEL, mg.NU_STEEL, p(uu), None, None) == A.OFEA_ERR_STATE
+        assert L.ofea_assemble_batch(h, 1, p(b), mg.NU_STEEL, None) == A.OFEA_ERR_STATE
         say("errors: state and arguments on a valid mesh")
         # through the wrapper, so that its sizes (n, nnz) follow this mesh
         ctx.set_mesh(xyz, conn, et, resort)

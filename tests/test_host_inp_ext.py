@@ -118,3 +118,64 @@ def test_reference_deck_cload_matches_single_load():
     F = ofa.build_single_load(3 * nfree, r, d.load_set, d.load_dof, d.load_value)
     assert np.array_equal(mg.free_rhs(d.f_nodal, r), F)
     assert np.all(np.isnan(d.E_elem))  # no *Solid Section in the deck
+
+
+def _part_deck(m):
+    """The mesh m as one Abaqus part with one instance; sets are part-local
+    and the step names them instance-qualified (the layout Abaqus/CAE writes,
+    as in the reference's sample_model/NLFEA.inp)."""
+    lines = ["*Heading", "*Part, name=Beam", "*Node"]
+    lines += [f"{i + 1}, {x:.17g}, {y:.17g}, {z:.17g}" for i, (x, y, z) in enumerate(m.xyz)]
+    lines.append("*Element, type=C3D8")
+    lines += [f"{e + 1}, " + ", ".join(str(int(v)) for v in m.conn[e]) for e in range(m.ne)]
+    # the reference dialect takes the counts from the first *Nset / *Elset (generate) lines
+    lines += ["*Nset, nset=All-nodes, internal, generate", f"1, {m.nd}, 1", "*Elset, elset=All, generate",
+              f"1, {m.ne}, 1"]
+    lines += ["*Nset, nset=Set-fix"] + [", ".join(str(i + 1) for i in m.fix_set)]
+    tip = np.flatnonzero(np.abs(m.xyz[:, 0] - m.xyz[:, 0].max()) < 1e-12)
+    lines += ["*Nset, nset=Set-load"] + [", ".join(str(i + 1) for i in tip)]
+    lines += ["*Solid Section, elset=All, material=Steel", ",",
+              "*End Part", "*Assembly, name=Assembly", "*Instance, name=Beam-1, part=Beam", "*End Instance",
+              "*Nset, nset=Corner, instance=Beam-1", "1", "*End Assembly",
+              "*Material, name=Steel", "*Elastic", "210000., 0.3", "*Step, name=Step-1", "*Static",
+              "*Boundary", "Beam-1.Set-fix, ENCASTRE", "*Cload", "Beam-1.Set-load, 2, -4.", "Beam-1.3, 3, 1.5",
+              "Corner, 1, 2.", "*End Step"]
+    return "\n".join(lines) + "\n", tip
+
+
+def test_instance_qualified_sets(tmp_path):
+    m = mg.hex_block(3, 2, 2)
+    text, tip = _part_deck(m)
+    path = str(tmp_path / "part.inp")
+    open(path, "w").write(text)
+    d = ofa.read_inp(path)
+    assert d.ext_error == ""
+    assert np.array_equal(np.sort(d.bc_fixed), np.sort(m.fix_set + 1))
+    f = np.zeros((m.nd, 3))
+    f[tip, 1] = -4.0
+    f[2, 2] += 1.5  # instance-qualified node label
+    f[0, 0] += 2.0  # assembly-level set of the instance
+    assert np.array_equal(d.f_nodal, f)
+    assert np.all(d.E_elem == 210000.0) and np.all(d.nu_elem == 0.3)
+    # an instance-qualified name of a set the part does not hold stays an error
+    open(path, "w").write(text.replace("Beam-1.Set-fix, ENCASTRE", "Beam-1.Set-none, ENCASTRE"))
+    assert "Set-none" in ofa.read_inp(path).ext_error
+
+
+REF_SAMPLES = "/root/reference/sample_model"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SAMPLES), reason="reference checkout not present")
+@pytest.mark.parametrize("deck,load", [("NLFEA.inp", -1000000.0), ("L_NL/nlcae.inp", -10.0)])
+def test_reference_part_decks(deck, load):
+    """The reference's own Abaqus/CAE decks (one part, one instance) load their
+    *Boundary (Part-1-1.Set-fix, ENCASTRE) and *Cload (Part-1-1.Set-load, 2, F)
+    lines through the instance-qualified names."""
+    d = ofa.read_inp(os.path.join(REF_SAMPLES, deck))
+    assert d.ext_error == ""
+    assert np.array_equal(np.sort(d.bc_fixed), np.sort(d.fix_set))  # the part's Set-fix, 1-based ids
+    f = np.zeros((d.nd, 3))
+    f[d.load_set - 1, 1] = load
+    assert np.array_equal(d.f_nodal, f)
+    assert d.load_dof == 2 and d.load_value == load
+    assert np.all(d.E_elem == d.E) and np.all(d.nu_elem == d.nu)
