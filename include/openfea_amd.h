@@ -209,6 +209,25 @@ int ofea_solve(ofea_ctx *ctx, int method, const double *b, double *x, double rto
                double *relres);
 int ofea_get_solution(ofea_ctx *ctx, double *x);
 
+/* Largest system (free DOFs) that ofea_solve_noalloc takes. */
+#define OFEA_NOALLOC_MAX_DOFS 262144
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ofea_solve's OFEA_PCG_JACOBI / OFEA_PCG_BLOCK_JACOBI on
+ * caller-owned DEVICE vectors d_b (read) and d_x (written), n = the free DOFs
+ * of the resident K (ofea_set_mesh + ofea_assemble beforehand).  The call
+ * allocates nothing: the run's vectors, preconditioner, partial sums and
+ * scalars live in one static device array of 23.1 MB that every GPU the
+ * library runs on holds from ofea_create until the process ends.  It returns
+ * the same x (bit for bit) and iteration count as ofea_solve with the same
+ * method; systems above OFEA_NOALLOC_MAX_DOFS give OFEA_ERR_LIMIT.  All calls
+ * on one GPU share that array: calls that overlap in time on one GPU (other
+ * threads, other contexts of the same device) race and must not be issued.
+ * The call returns after the solve has finished (it synchronises the context
+ * stream); the caller must have finished writing d_b beforehand. */
+int ofea_solve_noalloc(ofea_ctx *ctx, int method, const double *d_b, double *d_x, double rtol, int maxit,
+                       int *iters, double *relres);
+
 /* ---- explicit dynamics ---- */
 /* Lumped nodal mass (nd) of all elements at density rho. */
 int ofea_lumped_mass(ofea_ctx *ctx, double rho, double *mass);

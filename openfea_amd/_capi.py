@@ -74,6 +74,7 @@ SIGNATURES = {
     "ofea_spmv": (_i, [_vp, _vp, _vp]),
     "ofea_set_rhs": (_i, [_vp, _vp]),
     "ofea_solve": (_i, [_vp, _i, _vp, _vp, _d, _i, _pi, _pd]),
+    "ofea_solve_noalloc": (_i, [_vp, _i, _vp, _vp, _d, _i, _pi, _pd]),
     "ofea_get_solution": (_i, [_vp, _vp]),
     "ofea_lumped_mass": (_i, [_vp, _d, _vp]),
     "ofea_explicit_run": (_i, [_vp, _d, _d, _vp, _vp, _vp, _vp, _d, _i, _i, _vp]),

@@ -134,6 +134,8 @@ int ofea_create(ofea_ctx **ctx, int device)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(OFEA_ERR_HIP, "no HIP device available");
     if (device < 0 || device >= ndev) return fail(OFEA_ERR_ARG, "device index out of range");
     OFEA_HIP(hipSetDevice(device));
+    int rc = noalloc_init(device);  // the device's static scratch (ofea_solve_noalloc)
+    if (rc) return rc;
     auto *c = new ofea_ctx();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -304,6 +306,15 @@ int ofea_solve(ofea_ctx *c, int method, const double *b, double *x, double rtol,
         OFEA_HIP(hipStreamSynchronize(c->stream));
     }
     return rc;
+}
+
+int ofea_solve_noalloc(ofea_ctx *c, int method, const double *d_b, double *d_x, double rtol, int maxit, int *iters,
+                       double *relres)
+{
+    CHECK_CTX(c);
+    if (!d_b || !d_x) return fail(OFEA_ERR_ARG, "null device vector");
+    if (!(rtol > 0.0) || maxit <= 0) return fail(OFEA_ERR_ARG, "rtol must be > 0 and maxit > 0");
+    return solve_noalloc(c, method, d_b, d_x, rtol, maxit, iters, relres);
 }
 
 int ofea_get_solution(ofea_ctx *c, double *x)

@@ -187,6 +187,12 @@ int solver_staging(ofea_ctx *ctx, double **d_x, double **d_y);
 int solver_spmv_vals(ofea_ctx *ctx, const double *d_vals, const double *d_x, double *d_y);
 int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, double *relres);
 void solver_free(ofea_ctx *ctx);
+// solve without allocation: the static scratch of each device (taken once at
+// context creation) and the FP64 (block) Jacobi PCG over it
+int noalloc_init(int device);
+size_t noalloc_bytes();
+int solve_noalloc(ofea_ctx *ctx, int method, const double *d_b, double *d_x, double rtol, int maxit, int *iters,
+                  double *relres);
 void solver_invalidate(ofea_ctx *ctx);  // K changed: preconditioner and FP32 copy are stale
 int stream_copy(ofea_ctx *ctx, long long bytes, int reps, float *ms);
 // explicit (explicit.hip)

@@ -29,6 +29,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -833,13 +835,14 @@ __global__ void k_precond_setup(int nfree, const int *__restrict__ node_blk, con
     for (int c = 0; c < 3; ++c)
         for (int d = 0; d < 3; ++d) A[3 * c + d] = k < deg ? val[9 * (long long)nb0 + c * 3 * deg + 3 * k + d] : 0.0;
     for (int c = 0; c < 3; ++c) dinv[3 * r + c] = 1.0 / A[4 * c];
-    for (int i = 0; i < 9; ++i) dblk[9 * r + i] = A[i];
+    if (dblk)
+        for (int i = 0; i < 9; ++i) dblk[9 * r + i] = A[i];
     // the block is stored [col c][row d]; K is symmetric, so it is its own transpose
     double M[9];
     inv3(A, M);
     for (int i = 0; i < 9; ++i) {
         minv[9 * r + i] = M[i];
-        minv32[9 * r + i] = (float)M[i];
+        if (minv32) minv32[9 * r + i] = (float)M[i];
     }
 }
 
@@ -956,18 +959,48 @@ void solver_invalidate(ofea_ctx *ctx)
     }
 }
 
-template <class TV, class T, bool DOT>
-static void launch_spmv_p(ofea_ctx *ctx, int group, const int *node_blk, const int *bcol, const TV *val, const T *x,
-                          T *y)
+// The buffers one PCG run works in: the context's solver work space, or the
+// static scratch of ofea_solve_noalloc.
+struct PcgBufs {
+    int nfree = 0, n = 0, nblk_vec = 0, spmv_group = 16;
+    double *r64 = nullptr, *y64 = nullptr;  // true residual, K x
+    double *partials = nullptr;             // 2 * nblk_vec (true residual)
+    CGParts P{};
+    CoarseArgs co{};
+    int co_nagg = 0;
+    const float *co_ainv = nullptr;
+    DevScalars *sc = nullptr;
+};
+
+static PcgBufs ws_bufs(ofea_solver_ws *w)
 {
-    ofea_solver_ws *w = ctx->ws;
+    PcgBufs B;
+    B.nfree = w->nfree;
+    B.n = w->n;
+    B.nblk_vec = w->nblk_vec;
+    B.spmv_group = w->spmv_group;
+    B.r64 = w->r64;
+    B.y64 = w->y64;
+    B.partials = w->partials;
+    B.P = CGParts{w->part_rz, w->part_rr, w->part_pq};
+    B.co = w->co;
+    B.co_nagg = w->co_nagg;
+    B.co_ainv = w->co_ainv;
+    B.sc = w->sc;
+    return B;
+}
+
+template <class TV, class T, bool DOT>
+static void launch_spmv_b(hipStream_t st, int nfree, double *partials, DevScalars *sc, int group,
+                          const int *node_blk, const int *bcol, const TV *val, const T *x, T *y)
+{
     int npw = RED_WG / group;
-    int nb = (w->nfree + npw - 1) / npw;
+    int nb = (nfree + npw - 1) / npw;
     if (nb == 0) return;
 #define OFEA_SPMV_CASE(G)                                                                                        \
     case G:                                                                                                      \
-        hipLaunchKernelGGL((k_spmv<TV, T, G, DOT>), dim3(nb), dim3(RED_WG), 0, ctx->stream, w->nfree,          \
-                           node_blk, bcol, val, x, y, w->partials, w->sc);                                      \
+        hipLaunchKernelGGL((k_spmv<TV, T, G, DOT>), dim3(nb), dim3(RED_WG), 0, st, nfree, node_blk, bcol, val, x, \
+                           y, partials, sc);                                                                     \
         break;
     switch (group) {
         OFEA_SPMV_CASE(4)
@@ -978,6 +1011,14 @@ static void launch_spmv_p(ofea_ctx *ctx, int group, const int *node_blk, const i
         OFEA_SPMV_CASE(64)
     }
 #undef OFEA_SPMV_CASE
+}
+
+template <class TV, class T, bool DOT>
+static void launch_spmv_p(ofea_ctx *ctx, int group, const int *node_blk, const int *bcol, const TV *val, const T *x,
+                          T *y)
+{
+    ofea_solver_ws *w = ctx->ws;
+    launch_spmv_b<TV, T, DOT>(ctx->stream, w->nfree, w->partials, w->sc, group, node_blk, bcol, val, x, y);
 }
 
 template <class TV, class T, bool DOT>
@@ -1081,17 +1122,17 @@ int solver_prepare(ofea_ctx *ctx)
 
 // The graph of one batch: CG_BATCH iterations (2 kernels each) + batch end.
 template <class TV, class T, class XT, int PC, bool CO>
-static void cg_batch(ofea_ctx *ctx, const SysView &S, const TV *val, VecSet<T> &v, XT *x, const T *minv)
+static void cg_batch(ofea_ctx *ctx, const PcgBufs &w, const SysView &S, const TV *val, VecSet<T> &v, XT *x,
+                     const T *minv)
 {
-    ofea_solver_ws *w = ctx->ws;
-    CGParts P{w->part_rz, w->part_rr, w->part_pq};
+    const CGParts P = w.P;
     for (int bi = 0; bi < CG_BATCH; ++bi) {
 #define OFEA_CG_CASE(G)                                                                                          \
     case G:                                                                                                      \
-        hipLaunchKernelGGL((k_cg_spmv<TV, T, G, CO>), dim3(CG_NB), dim3(CG_WG), 0, ctx->stream, bi, w->nfree, \
-                           S.node_blk, S.bcol, val, v.z, v.p, v.p2, v.q, P, w->co, w->sc);                      \
+        hipLaunchKernelGGL((k_cg_spmv<TV, T, G, CO>), dim3(CG_NB), dim3(CG_WG), 0, ctx->stream, bi, w.nfree,  \
+                           S.node_blk, S.bcol, val, v.z, v.p, v.p2, v.q, P, w.co, w.sc);                        \
         break;
-        switch (w->spmv_group) {
+        switch (w.spmv_group) {
             OFEA_CG_CASE(4)
             OFEA_CG_CASE(8)
             OFEA_CG_CASE(16)
@@ -1100,19 +1141,19 @@ static void cg_batch(ofea_ctx *ctx, const SysView &S, const TV *val, VecSet<T> &
             OFEA_CG_CASE(64)
         }
 #undef OFEA_CG_CASE
-        hipLaunchKernelGGL((k_cg_update<T, XT, PC, CO>), dim3(CG_NB), dim3(UP_WG), 0, ctx->stream, bi, w->nfree, x,
-                           v.r, v.z, v.p, v.p2, v.q, S.dinv, minv, P, w->co, w->sc);
+        hipLaunchKernelGGL((k_cg_update<T, XT, PC, CO>), dim3(CG_NB), dim3(UP_WG), 0, ctx->stream, bi, w.nfree, x,
+                           v.r, v.z, v.p, v.p2, v.q, S.dinv, minv, P, w.co, w.sc);
     }
-    hipLaunchKernelGGL(k_cg_batch_end, dim3(1), dim3(64), 0, ctx->stream, w->sc, CG_BATCH);
+    hipLaunchKernelGGL(k_cg_batch_end, dim3(1), dim3(64), 0, ctx->stream, w.sc, CG_BATCH);
 }
 
 // true FP64 residual r64 = b - K64 x; ||r64||^2 -> S_TRR, ||b||^2 -> S_BB
-static int true_residual(ofea_ctx *ctx, const SysView &S, const double *b, const double *x)
+static int true_residual(ofea_ctx *ctx, const PcgBufs &w, const SysView &S, const double *b, const double *x)
 {
-    ofea_solver_ws *w = ctx->ws;
-    launch_spmv_p<double, double, false>(ctx, w->spmv_group, S.node_blk, S.bcol, S.val64, x, w->y64);
-    hipLaunchKernelGGL(k_residual, dim3(w->nblk_vec), dim3(RED_WG), 0, ctx->stream, w->n, b, w->y64, w->r64,
-                       w->partials, w->sc);
+    launch_spmv_b<double, double, false>(ctx->stream, w.nfree, w.partials, w.sc, w.spmv_group, S.node_blk, S.bcol,
+                                         S.val64, x, w.y64);
+    hipLaunchKernelGGL(k_residual, dim3(w.nblk_vec), dim3(RED_WG), 0, ctx->stream, w.n, b, w.y64, w.r64, w.partials,
+                       w.sc);
     OFEA_HIP(hipGetLastError());
     return OFEA_OK;
 }
@@ -1141,6 +1182,7 @@ static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, Ve
                    double rtol, int maxit, double ru_factor, int *iters, double *relres)
 {
     ofea_solver_ws *w = ctx->ws;
+    const PcgBufs B = ws_bufs(w);
     const double *b = Sv.b;
     double *x = Sv.x;
     const unsigned gv = w->nblk_vec;
@@ -1150,7 +1192,7 @@ static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, Ve
     hipLaunchKernelGGL((k_fill<double>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, x, 0.0);
     hipLaunchKernelGGL((k_fill<T>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, v.p, T(0));
     hipLaunchKernelGGL((k_fill<T>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, v.p2, T(0));
-    int rc = true_residual(ctx, Sv, b, x);  // r64 = b and ||b||^2
+    int rc = true_residual(ctx, B, Sv, b, x);  // r64 = b and ||b||^2
     if (rc) return rc;
     if ((rc = read_scalars(ctx))) return rc;
     const double bb = S.s[S_BB];
@@ -1179,7 +1221,7 @@ static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, Ve
     static const bool no_graph = std::getenv("OFEA_NO_GRAPH") && std::getenv("OFEA_NO_GRAPH")[0] == '1';
     if (!w->exec[slot] && !no_graph) {
         OFEA_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        cg_batch<TV, T, double, PC, CO>(ctx, Sv, val, v, x, minv);
+        cg_batch<TV, T, double, PC, CO>(ctx, B, Sv, val, v, x, minv);
         OFEA_HIP(hipStreamEndCapture(ctx->stream, &w->graph[slot]));
         OFEA_HIP(hipGraphInstantiate(&w->exec[slot], w->graph[slot], nullptr, nullptr, 0));
     }
@@ -1191,7 +1233,7 @@ static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, Ve
     DevScalars *Hb = w->sc_host + 1;
     auto launch_batch = [&](int hb) -> int {
         if (no_graph) {
-            cg_batch<TV, T, double, PC, CO>(ctx, Sv, val, v, x, minv);
+            cg_batch<TV, T, double, PC, CO>(ctx, B, Sv, val, v, x, minv);
             OFEA_HIP(hipGetLastError());
         } else {
             OFEA_HIP(hipGraphLaunch(w->exec[slot], ctx->stream));
@@ -1219,7 +1261,7 @@ static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, Ve
         if (done == 3) return fail(OFEA_ERR_STATE, "PCG breakdown (matrix or preconditioner not SPD, or NaN)");
         const int it_done = S.it, par_done = S.par0 ^ (S.bi_done & 1);
         // true FP64 residual at convergence candidates, reliable-update points and maxit
-        if ((rc = true_residual(ctx, Sv, b, x))) return rc;
+        if ((rc = true_residual(ctx, B, Sv, b, x))) return rc;
         if ((rc = read_scalars(ctx))) return rc;
         rel = std::sqrt(S.s[S_TRR] / bb);
         if (rel <= rtol || done == 2) break;
@@ -2103,6 +2145,173 @@ int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, 
     default:
         return fail(OFEA_ERR_ARG, "unknown solver method");
     }
+}
+
+// ------------------------------------------------------------------ solve without allocation
+// ofea_solve_noalloc runs the FP64 point- or block-Jacobi PCG of ofea_solve
+// on caller-owned device vectors with every buffer of the run (CG vectors,
+// true residual, preconditioner, partial sums, scalars) in one file-scope
+// device array, so that the call allocates nothing.  It launches the same
+// kernels with the same grids as ofea_solve (batches launched directly, one
+// batch in flight), so both give the same iterates bit for bit.
+//
+// Layout of the array, in doubles: r z p p2 q r64 y64 dinv (NA_N each),
+// minv (3 NA_N: 9 per free node), residual partials (2 per 256 DOFs), CG
+// partials (4 CG_NB), then the scalar block.
+constexpr size_t NA_N = OFEA_NOALLOC_MAX_DOFS;
+constexpr size_t NA_VBLK = (NA_N + RED_WG - 1) / RED_WG;
+constexpr size_t NA_DOUBLES = 11 * NA_N + 2 * NA_VBLK + 4 * CG_NB;
+constexpr size_t NA_SC = (sizeof(DevScalars) + sizeof(double) - 1) / sizeof(double);
+static __device__ __align__(256) double scratch[NA_DOUBLES + NA_SC];
+
+// Device address of each GPU's copy of scratch, taken once per device at
+// context creation.  In host code the array's name is only a stand-in: every
+// use of the array's memory goes through these pointers.
+constexpr int NA_MAX_DEV = 64;
+static double *g_na_ptr[NA_MAX_DEV] = {};
+
+int noalloc_init(int device)
+{
+    if (device < 0 || device >= NA_MAX_DEV) return fail(OFEA_ERR_LIMIT, "device index beyond the scratch table");
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    if (g_na_ptr[device]) return OFEA_OK;
+    double *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));  // the caller made `device` current
+    if (e != hipSuccess || !ptr)
+        return fail(OFEA_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_na_ptr[device] = ptr;
+    return OFEA_OK;
+}
+
+size_t noalloc_bytes() { return sizeof(double) * (NA_DOUBLES + NA_SC); }
+
+int solve_noalloc(ofea_ctx *ctx, int method, const double *d_b, double *d_x, double rtol, int maxit, int *iters,
+                  double *relres)
+{
+    if (method != OFEA_PCG_JACOBI && method != OFEA_PCG_BLOCK_JACOBI)
+        return fail(OFEA_ERR_ARG, "ofea_solve_noalloc takes OFEA_PCG_JACOBI or OFEA_PCG_BLOCK_JACOBI");
+    if (!ctx->assembled) return fail(OFEA_ERR_STATE, "ofea_assemble first");
+    const int n = ctx->P.n, nf = ctx->P.nfree;
+    if ((size_t)n > NA_N)
+        return fail(OFEA_ERR_LIMIT, "ofea_solve_noalloc holds at most " + std::to_string(NA_N) + " DOFs, the system has " +
+                                        std::to_string(n));
+    double *base = ctx->device >= 0 && ctx->device < NA_MAX_DEV ? g_na_ptr[ctx->device] : nullptr;
+    if (!base) return fail(OFEA_ERR_STATE, "scratch of this device not set up (ofea_create)");
+    if (n == 0) {
+        if (iters) *iters = 0;
+        if (relres) *relres = 0.0;
+        return OFEA_OK;
+    }
+    // carve the array
+    VecSet<double> v;
+    double *c = base;
+    v.r = c, c += NA_N;
+    v.z = c, c += NA_N;
+    v.p = c, c += NA_N;
+    v.p2 = c, c += NA_N;
+    v.q = c, c += NA_N;
+    PcgBufs B;
+    B.r64 = c, c += NA_N;
+    B.y64 = c, c += NA_N;
+    double *dinv = c;
+    c += NA_N;
+    double *minv = c;
+    c += 3 * NA_N;
+    B.partials = c, c += 2 * NA_VBLK;
+    B.P = CGParts{c, c + 2 * CG_NB, c + 3 * CG_NB};
+    c += 4 * CG_NB;
+    B.sc = reinterpret_cast<DevScalars *>(c);
+    B.nfree = nf;
+    B.n = n;
+    B.nblk_vec = (n + RED_WG - 1) / RED_WG;
+    B.spmv_group = ctx->ws ? ctx->ws->spmv_group : 16;  // the group ofea_solve would use
+    const SysView S{ctx->d_node_blk, ctx->d_bcol, ctx->d_val, dinv, d_b, d_x};
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(k_precond_setup, dim3((nf + RED_WG - 1) / RED_WG), dim3(RED_WG), 0, st, nf, ctx->d_node_blk,
+                       ctx->d_bcol, ctx->d_val, dinv, minv, (float *)nullptr, (double *)nullptr);
+    const unsigned gv = B.nblk_vec;
+    hipLaunchKernelGGL((k_fill<double>), dim3(gv), dim3(RED_WG), 0, st, n, d_x, 0.0);
+    hipLaunchKernelGGL((k_fill<double>), dim3(gv), dim3(RED_WG), 0, st, n, v.p, 0.0);
+    hipLaunchKernelGGL((k_fill<double>), dim3(gv), dim3(RED_WG), 0, st, n, v.p2, 0.0);
+    // the scalar block keeps the previous call's values: the host rewrites it
+    // (ticket included) before any kernel reads it
+    DevScalars Sh{};
+    Sh.ticket = 0;
+    OFEA_HIP(hipMemcpyAsync(B.sc, &Sh, sizeof(DevScalars), hipMemcpyHostToDevice, st));
+    auto read_sc = [&]() -> int {
+        OFEA_HIP(hipMemcpyAsync(&Sh, B.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
+        OFEA_HIP(hipStreamSynchronize(st));
+        return OFEA_OK;
+    };
+    auto write_sc = [&]() -> int {
+        OFEA_HIP(hipMemcpyAsync(B.sc, &Sh, sizeof(DevScalars), hipMemcpyHostToDevice, st));
+        OFEA_HIP(hipStreamSynchronize(st));  // Sh is a stack value and changes next
+        return OFEA_OK;
+    };
+    const int PCm = method == OFEA_PCG_BLOCK_JACOBI ? 1 : 0;
+    auto replace = [&](int restart) {
+        if (PCm)
+            hipLaunchKernelGGL((k_replace<double, 1, false>), dim3(CG_NB), dim3(UP_WG), 0, st, nf, B.r64, v.r, v.z,
+                               restart, dinv, minv, B.P, B.co, B.sc);
+        else
+            hipLaunchKernelGGL((k_replace<double, 0, false>), dim3(CG_NB), dim3(UP_WG), 0, st, nf, B.r64, v.r, v.z,
+                               restart, dinv, minv, B.P, B.co, B.sc);
+    };
+    auto batch = [&]() {
+        if (PCm)
+            cg_batch<double, double, double, 1, false>(ctx, B, S, ctx->d_val, v, d_x, minv);
+        else
+            cg_batch<double, double, double, 0, false>(ctx, B, S, ctx->d_val, v, d_x, minv);
+    };
+    int rc = true_residual(ctx, B, S, d_b, d_x);  // r64 = b and ||b||^2
+    if (rc || (rc = read_sc())) return rc;
+    const double bb = Sh.s[S_BB];
+    if (bb == 0.0) {
+        if (iters) *iters = 0;
+        if (relres) *relres = 0.0;
+        return OFEA_OK;
+    }
+    // the same control flow as ofea_solve's FP64 methods (no reliable updates)
+    double tol2 = rtol * rtol * bb;
+    Sh.par0 = 0;
+    Sh.it0 = 0;
+    Sh.it = 0;
+    Sh.maxit = maxit;
+    Sh.done = 0;
+    Sh.ticket = 0;
+    Sh.s[S_TOL2] = tol2;
+    Sh.s[S_RU2] = -1.0;
+    if ((rc = write_sc())) return rc;
+    replace(1);
+    OFEA_HIP(hipGetLastError());
+    double rel = 1.0;
+    int tighten = 0;
+    for (;;) {
+        do {
+            batch();
+            OFEA_HIP(hipGetLastError());
+            if ((rc = read_sc())) return rc;
+        } while (!Sh.done);
+        const int done = Sh.done;
+        if (done == 3) return fail(OFEA_ERR_STATE, "PCG breakdown (matrix or preconditioner not SPD, or NaN)");
+        const int it_done = Sh.it, par_done = Sh.par0 ^ (Sh.bi_done & 1);
+        if ((rc = true_residual(ctx, B, S, d_b, d_x)) || (rc = read_sc())) return rc;
+        rel = std::sqrt(Sh.s[S_TRR] / bb);
+        if (rel <= rtol || done == 2) break;
+        if (done == 1 && ++tighten > 8) break;
+        if (done == 1) tol2 *= 0.25;
+        Sh.par0 = par_done;
+        Sh.it0 = it_done;
+        Sh.done = 0;
+        Sh.s[S_TOL2] = tol2;
+        if ((rc = write_sc())) return rc;
+        replace(0);
+        OFEA_HIP(hipGetLastError());
+    }
+    if (iters) *iters = Sh.it;
+    if (relres) *relres = rel;
+    return rel <= rtol ? OFEA_OK : OFEA_NOT_CONVERGED;
 }
 
 // plain streaming copy (16-byte vector loads/stores), grid-stride
