@@ -5,7 +5,12 @@
 // (src/Elements/C3D8_Gauss.cpp:80-108) and build_ele_stiff_mat_hammer
 // (src/Elements/C3D4_hammer.cpp:61-73).
 //
-// Design (column-owner gather, no atomics):
+// Two C3D4 paths: k_assemble_tet_chunks (the full linear assembly; each chunk
+// element is computed once per chunk, see its section) and k_assemble_tet
+// (the pair-gather kernel below: sparse plans of the CA update, and meshes
+// beyond the chunk plan's capacity).  C3D8 uses k_assemble_hex.
+//
+// Design of the pair-gather kernels (column-owner gather, no atomics):
 //   * one 256-thread workgroup owns a chunk of consecutive free nodes, i.e.
 //     the CSC columns 3r..3r+2 of each node r of the chunk;
 //   * phase 1 computes, for every (owned node r, incident element e) pair,
@@ -80,14 +85,17 @@ __device__ inline void gather_blocks(int b0, int b1, int t0, const int *__restri
 constexpr int TET_CT = 3 * TET_CAP;  // off-diagonal contributor entries of a chunk (<= 3 per pair)
 static_assert(TET_CT % 256 == 0, "whole rounds of the workgroup");
 
-// value of the neighbouring lane (lane ^ 1), through DPP quad_perm [1,0,3,2]
-__device__ inline double dpp_xor1(double x)
+// value of lane ^ 1 / lane ^ 2, through DPP quad_perm [1,0,3,2] / [2,3,0,1]
+template <int CTRL>
+__device__ inline double dpp_quad(double x)
 {
     const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, 0xB1, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0xB1, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
     return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
+__device__ inline double dpp_xor1(double x) { return dpp_quad<0xB1>(x); }
+__device__ inline double dpp_xor2(double x) { return dpp_quad<0x4E>(x); }
 
 // A phase-2 task: an off-diagonal block (r, k) or half of row c of the
 // diagonal block of r.  The global loads (the block's chunk-local node and
@@ -315,6 +323,165 @@ __global__ __launch_bounds__(ASM_WG) void k_assemble_tet(const double *__restric
     }
 }
 
+// ------------------------------------------------------------------ C3D4, chunk plan
+// The linear C3D4 path (TetChunkPlan, built by build_tet_chunks): one
+// workgroup per chunk of consecutive free nodes.
+//   phase 1: thread e computes the gradients of chunk element e and mu V into
+//            LDS rows (row 3 a + c = d N_a / d x_c, row 12 = mu V; the row
+//            stride is padded by two doubles so that the rows of one element
+//            fall in different banks);
+//   phase 2: one loop for every task: Q = sum (mu V g_col) (x) g_row over its
+//            contributors, K = rho Q + Q^T + I tr Q.  For a diagonal block the
+//            contributors are (e, a, a), so the same formula gives
+//            (rho + 1) sum mu V g (x) g + I sum mu V |g|^2; its four quarter
+//            tasks are added by a fixed quad reduction (DPP);
+//   output:  every task writes its block into an LDS image of the chunk's
+//            contiguous slice of the CSC values, then the slice goes to HBM
+//            with 16-byte stores.
+// Summation order: off-diagonal blocks ascending element order (that of
+// src/assemble.cpp:106-149); diagonal blocks the quarters in ascending
+// element order, then (q0 + q1) + (q2 + q3).  Every order is fixed: the values
+// are bitwise reproducible.
+constexpr int TC_RS = TC_ECAP + 2;    // LDS row stride in doubles
+constexpr int TC_ROWB = 8 * TC_RS;    // bytes per row
+constexpr int TC_WOFF = 12 * TC_ROWB; // byte offset of the mu V row
+
+__device__ inline double lds_f64(const char *base, unsigned off) { return *reinterpret_cast<const double *>(base + off); }
+
+__global__ __launch_bounds__(TC_WG) void k_assemble_tet_chunks(const double *__restrict__ xyz,
+                                                               const int *__restrict__ rec,
+                                                               const double *__restrict__ Eel, double E, double nu,
+                                                               const int *__restrict__ hdr,
+                                                               const int *__restrict__ elem,
+                                                               const uint16_t *__restrict__ ct,
+                                                               const uint32_t *__restrict__ task,
+                                                               const uint8_t *__restrict__ task_k,
+                                                               const int *__restrict__ node_blk,
+                                                               double *__restrict__ val, int accumulate, int nchunks)
+{
+    static_assert(TC_SLICE + 1 <= 13 * TC_RS + 2, "the slice image fits the element rows");
+    static_assert(TC_CTCAP % TC_WG == 0, "whole rounds of the workgroup");
+    __shared__ double s_g[13 * TC_RS + 2];  // element rows, then the slice image
+    __shared__ uint16_t s_ct[TC_CTCAP];
+    __shared__ int s_nb[TC_NCAP + 1];
+    const char *lb = reinterpret_cast<const char *>(s_g);
+    const int chunk = xcd_remap(blockIdx.x, nchunks);
+    const int *h = hdr + 16 * chunk;
+    const int n0 = h[0], nn = h[1], e0 = h[2], ne = h[3], c0 = h[4], nct = h[5], b0 = h[6], nblk = h[7];
+    const int t0 = h[8], ntask = h[9], ndq = h[10];
+    const int t = threadIdx.x;
+    // every global load below depends on the header only
+    int nbv = 0;
+    if (t <= nn) nbv = node_blk[n0 + t];
+    uint32_t ti = 0;
+    int k = 255;
+    if (t < ntask) {
+        ti = task[t0 + t];
+        k = task_k[t0 + t];
+    }
+    constexpr int CT_PER = TC_CTCAP / TC_WG;
+    uint16_t ctv[CT_PER];
+#pragma unroll
+    for (int u = 0; u < CT_PER; ++u) ctv[u] = (t + u * TC_WG < nct) ? ct[c0 + t + u * TC_WG] : (uint16_t)0;
+    double lam0, mu0;
+    lame<double>(E, nu, lam0, mu0);
+    const double rho = 2.0 * nu / (1.0 - 2.0 * nu);  // lambda / mu for every element (one nu per launch)
+    if (t < ne) {
+        const int4 cn = reinterpret_cast<const int4 *>(rec)[e0 + t];
+        const int nd[4] = {cn.x, cn.y, cn.z, cn.w};
+        const char *xb = reinterpret_cast<const char *>(xyz);
+        double X[12];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const unsigned o = (unsigned)nd[b] * 24u;  // 32-bit offsets from the array base
+            X[3 * b] = *reinterpret_cast<const double *>(xb + o);
+            X[3 * b + 1] = *reinterpret_cast<const double *>(xb + o + 8);
+            X[3 * b + 2] = *reinterpret_cast<const double *>(xb + o + 16);
+        }
+        double g[3][4];
+        const double detJ = tet_grads(X, g);
+        double mu = mu0, lam;
+        if (Eel) lame<double>(Eel[elem[e0 + t]], nu, lam, mu);
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s_g[(3 * b + c) * TC_RS + t] = g[c][b];
+        s_g[12 * TC_RS + t] = mu * (detJ / 6.0);  // signed V (src/Elements/C3D4.cpp:45-54 does not abs())
+    }
+#pragma unroll
+    for (int u = 0; u < CT_PER; ++u)
+        if (t + u * TC_WG < nct) s_ct[t + u * TC_WG] = ctv[u];
+    if (t <= nn) s_nb[t] = nbv - b0;
+    __syncthreads();
+    const int st = ti & 2047, cnt = (ti >> 11) & 127, quarter = (ti >> 18) & 1, rl = (ti >> 19) & 63;
+    double Q[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int l = st; l < st + cnt; ++l) {
+        const unsigned ci = s_ct[l];
+        const unsigned e8 = (ci & 255u) * 8u;
+        const unsigned ga = e8 + ((ci >> 8) & 3u) * (3u * TC_ROWB), gb = e8 + (ci >> 10) * (3u * TC_ROWB);
+        const double w = lds_f64(lb, e8 + TC_WOFF);
+        const double a0 = lds_f64(lb, ga), a1 = lds_f64(lb, ga + TC_ROWB), a2 = lds_f64(lb, ga + 2 * TC_ROWB);
+        const double q0 = lds_f64(lb, gb), q1 = lds_f64(lb, gb + TC_ROWB), q2 = lds_f64(lb, gb + 2 * TC_ROWB);
+        const double m0 = w * a0, m1 = w * a1, m2 = w * a2;
+        Q[0] += m0 * q0;
+        Q[1] += m0 * q1;
+        Q[2] += m0 * q2;
+        Q[3] += m1 * q0;
+        Q[4] += m1 * q1;
+        Q[5] += m1 * q2;
+        Q[6] += m2 * q0;
+        Q[7] += m2 * q1;
+        Q[8] += m2 * q2;
+    }
+    // quarters of the diagonal blocks occupy lanes [0, ndq) four per node, so a
+    // quad never mixes quarters with other tasks; only their waves reduce
+    if (__builtin_amdgcn_readfirstlane(t) < ndq) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            double sq = Q[i] + dpp_xor1(Q[i]);
+            sq += dpp_xor2(sq);
+            Q[i] = quarter ? sq : Q[i];
+        }
+    }
+    double K[9];
+    const double trQ = Q[0] + Q[4] + Q[8];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) K[3 * c + d] = rho * Q[3 * c + d] + Q[3 * d + c] + (c == d ? trQ : 0.0);
+    __syncthreads();  // the element rows are no longer read: they become the slice image
+    // slice value x sits at s_out[lead + x], so that aligned global pairs are
+    // aligned LDS pairs in the copy-out
+    const long long v0 = 9LL * b0;
+    const int lead = (int)(v0 & 1);
+    double *s_out = s_g + lead;
+    if (k != 255) {
+        const int nb = s_nb[rl], deg = s_nb[rl + 1] - nb;
+        double *o = s_out + 9 * nb + 3 * k;  // column 3r + c holds rows 3s + d at c (3 deg) + 3 k + d
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) o[c * 3 * deg + d] = K[3 * c + d];
+    }
+    __syncthreads();
+    const int len = 9 * nblk;
+    double *dst = val + v0;
+    if (lead && t == 0) dst[0] = accumulate ? dst[0] + s_out[0] : s_out[0];
+    const int npair = (len - lead) >> 1;
+    const double2 *s2 = reinterpret_cast<const double2 *>(s_g + 2 * lead);
+    double2 *d2 = reinterpret_cast<double2 *>(dst + lead);
+    for (int i = t; i < npair; i += TC_WG) {
+        double2 v = s2[i];
+        if (accumulate) {
+            const double2 old = d2[i];
+            v.x += old.x;
+            v.y += old.y;
+        }
+        d2[i] = v;
+    }
+    if (((len - lead) & 1) && t == 0) dst[len - 1] = accumulate ? dst[len - 1] + s_out[len - 1] : s_out[len - 1];
+}
+
 // ------------------------------------------------------------------ C3D8
 struct HexSource {
     const double (*blk)[9];
@@ -394,7 +561,13 @@ int launch_assemble_plan(ofea_ctx *ctx, int type, const DevPlan &dp, const doubl
                          double *d_out, int accumulate)
 {
     if (dp.nchunks == 0) return OFEA_OK;
-    if (type == OFEA_C3D4 && dp.blk_gid)
+    const DevTetChunks &tc = ctx->dtc;
+    if (type == OFEA_C3D4 && !dp.blk_gid && tc.nchunks > 0 && d_xyz == ctx->d_xyz) {
+        // the full linear C3D4 assembly of the context's own mesh: chunk plan
+        hipLaunchKernelGGL(k_assemble_tet_chunks, dim3(tc.nchunks), dim3(TC_WG), 0, ctx->stream, d_xyz, tc.rec, dp.E, E,
+                           nu, tc.hdr, tc.elem, tc.ct, tc.task, tc.task_k, ctx->d_node_blk, d_out, accumulate,
+                           tc.nchunks);
+    } else if (type == OFEA_C3D4 && dp.blk_gid)
         hipLaunchKernelGGL(k_assemble_tet<true>, dim3(dp.nchunks), dim3(ASM_WG), 0, ctx->stream, d_xyz, dp.conn, dp.E,
                            E, nu, dp.chunk_pair, dp.chunk_blk, dp.chunk_node, dp.node_pair, dp.pair, dp.blk_rl,
                            dp.off_ptr, dp.off_contrib, dp.blk_gid, ctx->d_node_blk, ctx->d_diag_k, d_out, accumulate,

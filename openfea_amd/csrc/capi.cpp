@@ -76,6 +76,13 @@ static void free_mesh(ofea_ctx *c)
     dev_free(c->d_u);
     dev_free(c->d_fint);
     for (auto &d : c->dp) free_plan(d);
+    dev_free(c->dtc.hdr);
+    dev_free(c->dtc.rec);
+    dev_free(c->dtc.elem);
+    dev_free(c->dtc.ct);
+    dev_free(c->dtc.task);
+    dev_free(c->dtc.task_k);
+    c->dtc = DevTetChunks();
     c->assembled = false;
     c->rhs_set = false;
 }
@@ -90,6 +97,7 @@ static void reset_mesh(ofea_ctx *c)
     c->resort.clear();
     c->E_elem.clear();
     c->P = Pattern();
+    c->TC = TetChunkPlan();
     for (int t = 0; t < 2; ++t) {
         c->G[t] = GatherPlan();
         c->has_type[t] = false;
@@ -112,6 +120,16 @@ static int upload_mesh(ofea_ctx *c)
     for (int t = 0; t < 2; ++t) {
         if (!c->has_type[t]) continue;
         if ((rc = upload_plan(c->G[t], c->dp[t]))) return rc;
+    }
+    if (c->TC.nchunks > 0) {
+        const TetChunkPlan &T = c->TC;
+        if ((rc = dev_upload(c->dtc.hdr, T.hdr.data(), T.hdr.size()))) return rc;
+        if ((rc = dev_upload(c->dtc.rec, T.rec.data(), T.rec.size()))) return rc;
+        if ((rc = dev_upload(c->dtc.elem, T.elem.data(), T.elem.size()))) return rc;
+        if ((rc = dev_upload(c->dtc.ct, T.ct.data(), std::max<size_t>(T.ct.size(), 1)))) return rc;
+        if ((rc = dev_upload(c->dtc.task, T.task.data(), std::max<size_t>(T.task.size(), 1)))) return rc;
+        if ((rc = dev_upload(c->dtc.task_k, T.task_k.data(), std::max<size_t>(T.task_k.size(), 1)))) return rc;
+        c->dtc.nchunks = T.nchunks;
     }
     return OFEA_OK;
 }
@@ -176,6 +194,7 @@ int ofea_set_mesh(ofea_ctx *c, int nd, const double *xyz, int ne, const int *con
     // context holds no mesh and every later call reports OFEA_ERR_STATE.
     Pattern P;
     GatherPlan G[2];
+    TetChunkPlan TC;
     bool has_type[2] = {false, false};
     int rc = build_pattern(nd, ne, conn, etype, resort, P);
     if (rc) return rc;
@@ -185,11 +204,15 @@ int ofea_set_mesh(ofea_ctx *c, int nd, const double *xyz, int ne, const int *con
         rc = build_gather_plan(nd, ne, conn, etype, resort, P, t, plan_caps(t), G[t]);
         if (rc) return rc;
     }
+    // chunk plan of the C3D4 assembly; a mesh beyond its capacity keeps the
+    // pair-gather kernel (the limit is not an error of set_mesh)
+    if (has_type[OFEA_C3D4] && build_tet_chunks(P, G[OFEA_C3D4], resort, TC) != OFEA_OK) TC = TetChunkPlan();
     c->xyz.assign(xyz, xyz + 3 * (size_t)nd);
     c->conn.assign(conn, conn + 8 * (size_t)ne);
     c->etype.assign(etype, etype + ne);
     c->resort.assign(resort, resort + nd);
     c->P = std::move(P);
+    c->TC = std::move(TC);
     for (int t = 0; t < 2; ++t) {
         c->G[t] = std::move(G[t]);
         c->has_type[t] = has_type[t];

@@ -426,6 +426,110 @@ int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const i
     return OFEA_OK;
 }
 
+int build_tet_chunks(const Pattern &P, const GatherPlan &G, const int *resort, TetChunkPlan &T)
+{
+    T = TetChunkPlan();
+    const int nf = P.nfree;
+    std::vector<int> loc(G.ne, -1);  // chunk element index of a type-local element, -1 = not in the chunk
+    struct Task {
+        int st, cnt, quarter, rl, k;
+    };
+    std::vector<Task> diag, off;
+    std::vector<int> els;
+    int r = 0;
+    while (r < nf) {
+        // grow the chunk node by node while its elements, tasks, slice and
+        // contributor entries fit
+        els.clear();
+        int nn = 0, nblk = 0, nct = 0;
+        const int r0 = r;
+        while (r < nf) {
+            const int p0 = G.node_pair[r], p1 = G.node_pair[r + 1];
+            int add = 0;
+            for (int p = p0; p < p1; ++p)
+                if (loc[G.pair[p] >> 3] < 0) ++add;
+            const int deg = P.node_blk[r + 1] - P.node_blk[r];
+            const int nct_r = 4 * (p1 - p0);  // bound: 3 off-diagonal and 1 diagonal entry per pair
+            const bool fits = (int)els.size() + add <= TC_ECAP && 4 * (nn + 1) + (nblk + deg) - (nn + 1) <= TC_WG &&
+                              9 * (nblk + deg) <= TC_SLICE && nn + 1 <= TC_NCAP && nct + nct_r <= TC_CTCAP;
+            if (!fits) {
+                if (nn == 0) {
+                    for (int el : els) loc[el] = -1;
+                    T = TetChunkPlan();
+                    return OFEA_ERR_LIMIT;  // no message: the caller falls back, set_mesh succeeds
+                }
+                break;
+            }
+            for (int p = p0; p < p1; ++p) {
+                const int el = G.pair[p] >> 3;
+                if (loc[el] < 0) {
+                    loc[el] = 0;
+                    els.push_back(el);
+                }
+            }
+            nn++;
+            nblk += deg;
+            nct += nct_r;
+            ++r;
+        }
+        std::sort(els.begin(), els.end());
+        for (int i = 0; i < (int)els.size(); ++i) loc[els[i]] = i;
+        const int e0 = (int)T.elem.size(), c0 = (int)T.ct.size();
+        T.elem.insert(T.elem.end(), els.begin(), els.end());
+        for (int el : els)
+            for (int b = 0; b < 4; ++b) T.rec.push_back(G.conn[4 * (size_t)el + b]);
+        diag.clear();
+        off.clear();
+        for (int q = r0; q < r; ++q) {
+            const int rl = q - r0;
+            const int p0 = G.node_pair[q], p1 = G.node_pair[q + 1];
+            const int nb = P.node_blk[q], deg = P.node_blk[q + 1] - nb;
+            for (int k = 0; k < deg; ++k) {
+                const int s = P.bcol[nb + k];
+                const int st = (int)T.ct.size() - c0;
+                if (s == q) {  // (e, a, a) for every element of the node, in four quarters
+                    const int cnt = p1 - p0;
+                    for (int p = p0; p < p1; ++p) {
+                        const int a = G.pair[p] & 7;
+                        T.ct.push_back((uint16_t)(loc[G.pair[p] >> 3] | (a << 8) | (a << 10)));
+                    }
+                    for (int qq = 0; qq < 4; ++qq)
+                        diag.push_back({st + (qq * cnt) / 4, ((qq + 1) * cnt) / 4 - (qq * cnt) / 4, 1, rl,
+                                        qq == 0 ? k : 255});
+                    continue;
+                }
+                int cnt = 0;
+                for (int p = p0; p < p1; ++p) {  // ascending element order
+                    const int el = G.pair[p] >> 3, a = G.pair[p] & 7;
+                    for (int b = 0; b < 4; ++b)
+                        if (resort[G.conn[4 * (size_t)el + b]] == s) {
+                            T.ct.push_back((uint16_t)(loc[el] | (a << 8) | (b << 10)));
+                            ++cnt;
+                        }
+                }
+                off.push_back({st, cnt, 0, rl, k});
+            }
+        }
+        std::stable_sort(off.begin(), off.end(), [](const Task &x, const Task &y) { return x.cnt > y.cnt; });
+        const int t0 = (int)T.task.size(), ntask = (int)(diag.size() + off.size());
+        for (const auto *v : {&diag, &off})
+            for (const Task &x : *v) {
+                if (x.st > 2047 || x.cnt > 127 || x.k > 255) {
+                    for (int el : els) loc[el] = -1;
+                    T = TetChunkPlan();
+                    return OFEA_ERR_LIMIT;
+                }
+                T.task.push_back((uint32_t)(x.st | (x.cnt << 11) | (x.quarter << 18) | (x.rl << 19)));
+                T.task_k.push_back((uint8_t)x.k);
+            }
+        T.hdr.insert(T.hdr.end(), {r0, r - r0, e0, (int)els.size(), c0, (int)T.ct.size() - c0, P.node_blk[r0],
+                                   P.node_blk[r] - P.node_blk[r0], t0, ntask, 4 * (r - r0), 0, 0, 0, 0, 0});
+        for (int el : els) loc[el] = -1;
+        T.nchunks++;
+    }
+    return OFEA_OK;
+}
+
 // ---------------------------------------------------------------- CA change file
 // ReadInfo::CA_read_change (src/data2cae.cpp:519-665): "*nodes_all, N" then
 // "x, y, z" lines; "*eles_change, M, type=T" then "kind, n1, ..., nk" lines

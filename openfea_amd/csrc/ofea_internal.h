@@ -74,6 +74,34 @@ int parse_inp_ext(const std::string &path, ofea_deck &D);
 int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const int *resort, const Pattern &P,
                       int type, int pair_cap, GatherPlan &G, bool sparse = false);
 
+// Chunk plan of the linear C3D4 assembly (k_assemble_tet_chunks).  A chunk is
+// a run of consecutive free nodes, i.e. one contiguous slice of the CSC value
+// array.  Its distinct elements (ascending) are computed once per chunk, one
+// per thread, into LDS; then one lane per task sums a 3x3 block from its
+// contributors (chunk element, slot of the column node, slot of the row node)
+// in ascending element order.  Off-diagonal blocks are one task each; a
+// diagonal block is four quarter tasks over the node's elements, summed by a
+// fixed quad reduction.  Tasks are ordered diagonal quarters first, then
+// off-diagonal blocks by decreasing contributor count, so that the lanes of a
+// wave run loops of equal length.
+constexpr int TC_WG = 256;         // threads per chunk workgroup
+constexpr int TC_ECAP = 256;       // distinct elements per chunk (one per thread)
+constexpr int TC_NCAP = 64;        // nodes per chunk
+constexpr int TC_CTCAP = 2048;     // contributor entries per chunk
+constexpr int TC_SLICE = 13 * 256; // values of the chunk's slice (its LDS image reuses the element rows)
+struct TetChunkPlan {
+    int nchunks = 0;
+    std::vector<int> hdr;           // 16 per chunk: n0 nn e0 ne c0 nct b0 nblk t0 ntask ndq (rest 0)
+    std::vector<int> rec;           // 4 node ids (0-based) per chunk element
+    std::vector<int> elem;          // type-local element index per chunk element (per-element modulus)
+    std::vector<uint16_t> ct;       // contributor: e | slot_col << 8 | slot_row << 10
+    std::vector<uint32_t> task;     // start | count << 11 | quarter << 18 | node-in-chunk << 19
+    std::vector<uint8_t> task_k;    // block slot k in the node's column, 255 = write nothing
+};
+// Builds the chunk plan; OFEA_ERR_LIMIT when a node exceeds a chunk's
+// capacity (the assembly then keeps the pair-gather kernel).
+int build_tet_chunks(const Pattern &P, const GatherPlan &G, const int *resort, TetChunkPlan &T);
+
 // Pairs per gather chunk, shared by every kernel that walks a GatherPlan:
 // C3D4 one pair per thread of a 256-thread workgroup, C3D8 one pair per
 // 8-lane Gauss-point group.
@@ -93,6 +121,13 @@ struct DevPlan {
     uint16_t *off_contrib = nullptr;
     uint8_t *blk_rl = nullptr;
     int *blk_gid = nullptr;  // null for full plans
+};
+struct DevTetChunks {
+    int nchunks = 0;
+    int *hdr = nullptr, *rec = nullptr, *elem = nullptr;
+    uint16_t *ct = nullptr;
+    uint32_t *task = nullptr;
+    uint8_t *task_k = nullptr;
 };
 
 }  // namespace ofea
@@ -137,6 +172,7 @@ struct ofea_ctx {
     std::vector<int> conn, etype, resort;
     ofea::Pattern P;
     ofea::GatherPlan G[2];
+    ofea::TetChunkPlan TC;  // linear C3D4 assembly; empty when the mesh exceeds its capacity
     bool has_type[2] = {false, false};
     std::vector<double> E_elem;  // empty = global E
     // device
@@ -145,6 +181,7 @@ struct ofea_ctx {
     int *d_resort = nullptr;
     double *d_val = nullptr;    // FP64 values, nnz
     ofea::DevPlan dp[2];
+    ofea::DevTetChunks dtc;
     bool assembled = false;
     int kver = 0;  // incremented by every assembly of K
     double asm_E = 0.0, asm_nu = 0.0;
