@@ -349,7 +349,7 @@ constexpr int TC_WOFF = 12 * TC_ROWB; // byte offset of the mu V row
 __device__ inline double lds_f64(const char *base, unsigned off) { return *reinterpret_cast<const double *>(base + off); }
 
 __global__ __launch_bounds__(TC_WG) void k_assemble_tet_chunks(const double *__restrict__ xyz,
-                                                               const int *__restrict__ rec,
+                                                               const int *__restrict__ conn,
                                                                const double *__restrict__ Eel, double E, double nu,
                                                                const int *__restrict__ hdr,
                                                                const int *__restrict__ elem,
@@ -387,7 +387,8 @@ __global__ __launch_bounds__(TC_WG) void k_assemble_tet_chunks(const double *__r
     lame<double>(E, nu, lam0, mu0);
     const double rho = 2.0 * nu / (1.0 - 2.0 * nu);  // lambda / mu for every element (one nu per launch)
     if (t < ne) {
-        const int4 cn = reinterpret_cast<const int4 *>(rec)[e0 + t];
+        const int el = elem[e0 + t];
+        const int4 cn = reinterpret_cast<const int4 *>(conn)[el];  // the type-local connectivity, 0-based
         const int nd[4] = {cn.x, cn.y, cn.z, cn.w};
         const char *xb = reinterpret_cast<const char *>(xyz);
         double X[12];
@@ -401,7 +402,7 @@ __global__ __launch_bounds__(TC_WG) void k_assemble_tet_chunks(const double *__r
         double g[3][4];
         const double detJ = tet_grads(X, g);
         double mu = mu0, lam;
-        if (Eel) lame<double>(Eel[elem[e0 + t]], nu, lam, mu);
+        if (Eel) lame<double>(Eel[el], nu, lam, mu);
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
@@ -564,7 +565,7 @@ int launch_assemble_plan(ofea_ctx *ctx, int type, const DevPlan &dp, const doubl
     const DevTetChunks &tc = ctx->dtc;
     if (type == OFEA_C3D4 && !dp.blk_gid && tc.nchunks > 0 && d_xyz == ctx->d_xyz) {
         // the full linear C3D4 assembly of the context's own mesh: chunk plan
-        hipLaunchKernelGGL(k_assemble_tet_chunks, dim3(tc.nchunks), dim3(TC_WG), 0, ctx->stream, d_xyz, tc.rec, dp.E, E,
+        hipLaunchKernelGGL(k_assemble_tet_chunks, dim3(tc.nchunks), dim3(TC_WG), 0, ctx->stream, d_xyz, dp.conn, dp.E, E,
                            nu, tc.hdr, tc.elem, tc.ct, tc.task, tc.task_k, ctx->d_node_blk, d_out, accumulate,
                            tc.nchunks);
     } else if (type == OFEA_C3D4 && dp.blk_gid)

@@ -77,7 +77,6 @@ static void free_mesh(ofea_ctx *c)
     dev_free(c->d_fint);
     for (auto &d : c->dp) free_plan(d);
     dev_free(c->dtc.hdr);
-    dev_free(c->dtc.rec);
     dev_free(c->dtc.elem);
     dev_free(c->dtc.ct);
     dev_free(c->dtc.task);
@@ -124,7 +123,6 @@ static int upload_mesh(ofea_ctx *c)
     if (c->TC.nchunks > 0) {
         const TetChunkPlan &T = c->TC;
         if ((rc = dev_upload(c->dtc.hdr, T.hdr.data(), T.hdr.size()))) return rc;
-        if ((rc = dev_upload(c->dtc.rec, T.rec.data(), T.rec.size()))) return rc;
         if ((rc = dev_upload(c->dtc.elem, T.elem.data(), T.elem.size()))) return rc;
         if ((rc = dev_upload(c->dtc.ct, T.ct.data(), std::max<size_t>(T.ct.size(), 1)))) return rc;
         if ((rc = dev_upload(c->dtc.task, T.task.data(), std::max<size_t>(T.task.size(), 1)))) return rc;

@@ -476,8 +476,6 @@ int build_tet_chunks(const Pattern &P, const GatherPlan &G, const int *resort, T
         for (int i = 0; i < (int)els.size(); ++i) loc[els[i]] = i;
         const int e0 = (int)T.elem.size(), c0 = (int)T.ct.size();
         T.elem.insert(T.elem.end(), els.begin(), els.end());
-        for (int el : els)
-            for (int b = 0; b < 4; ++b) T.rec.push_back(G.conn[4 * (size_t)el + b]);
         diag.clear();
         off.clear();
         for (int q = r0; q < r; ++q) {

@@ -92,8 +92,7 @@ constexpr int TC_SLICE = 13 * 256; // values of the chunk's slice (its LDS image
 struct TetChunkPlan {
     int nchunks = 0;
     std::vector<int> hdr;           // 16 per chunk: n0 nn e0 ne c0 nct b0 nblk t0 ntask ndq (rest 0)
-    std::vector<int> rec;           // 4 node ids (0-based) per chunk element
-    std::vector<int> elem;          // type-local element index per chunk element (per-element modulus)
+    std::vector<int> elem;          // type-local element index per chunk element, ascending
     std::vector<uint16_t> ct;       // contributor: e | slot_col << 8 | slot_row << 10
     std::vector<uint32_t> task;     // start | count << 11 | quarter << 18 | node-in-chunk << 19
     std::vector<uint8_t> task_k;    // block slot k in the node's column, 255 = write nothing
@@ -124,7 +123,7 @@ struct DevPlan {
 };
 struct DevTetChunks {
     int nchunks = 0;
-    int *hdr = nullptr, *rec = nullptr, *elem = nullptr;
+    int *hdr = nullptr, *elem = nullptr;
     uint16_t *ct = nullptr;
     uint32_t *task = nullptr;
     uint8_t *task_k = nullptr;

@@ -1,0 +1,164 @@
+"""GPU runs of configs 3 and 5 and of the reference's own sample decks,
+checked against the oracle over their whole course (not only a first step).
+
+  * config 3 (explicit C3D8, FP32 state): 1,000 central-difference steps on a
+    40x20x20 block (16,000 hexes, the largest whose oracle trajectory the test
+    budget holds; the bench's config-3 line states the same error at the full
+    100x50x50 size).  Reference trajectory: the oracle-pinned direct recurrence
+    with K from the oracle's fill_csc (the recurrence that
+    test_oracle_assembly.py proves equal to the oracle's incremental loop).
+    FP64 must agree to 1e-9, FP32 to the bound stated below;
+  * config 5 (TL-SVK Newton): the 10-increment ramp to a tip deflection of
+    0.2 L on a 20x10x10 block against oracle.nl_newton (scipy SuperLU inner
+    solves) to 1e-8;
+  * the reference's decks (tests/golden/ref_decks.npz, written from
+    sample_model/ by extract_decks.py): implicite_test_C3D4_C3D8.inp (mixed
+    C3D4 + C3D8), NLFEA.inp and CA_zongliang.inp through set_mesh / assemble /
+    solve: pattern bit-exact, values within 1e-12 of the oracle's, bitwise
+    repeatable, two-level PCG within 1e-8 of the direct solve of the oracle's
+    K; on NLFEA the TL tangent and internal force within 1e-12.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import GOLDEN, rel_max
+from oracle import oracle as O
+import openfea_amd as ofa
+from openfea_amd import _capi as A, meshgen as mg
+
+pytestmark = pytest.mark.gpu
+E, NU = mg.E_STEEL, mg.NU_STEEL
+FP32_CFG3_BOUND = 2e-5  # relative L2 displacement error of the FP32 state after 1,000 steps (measured 5.6e-6)
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def csc(cp, ri, v):
+    n = len(cp) - 1
+    return sp.csc_matrix((v, ri, cp), shape=(n, n))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = ofa.Context(0)
+    yield c
+    c.close()
+
+
+def explicit_reference(m, mass, v0, fixed, dt, nsteps):
+    """u after nsteps of the half-step central difference (zero external
+    force), on all DOFs with the clamped ones zeroed, K from the oracle."""
+    allfree = np.arange(m.nd, dtype=np.int32)
+    cp, ri = O.build_csc(m.conn, m.etype, allfree, 3 * m.nd)
+    K = csc(cp, ri, O.fill_csc(m.conn, m.etype, m.xyz, allfree, E, NU, cp, ri))
+    minv = 1.0 / np.repeat(mass, 3)
+    fx = np.repeat(fixed, 3).astype(bool)
+    v = v0.ravel().copy()
+    u = np.zeros(3 * m.nd)
+    for _ in range(nsteps):
+        u += v * dt
+        u[fx] = 0.0
+        v -= (K @ u) * minv * dt
+    return u
+
+
+def test_cfg3_thousand_steps_vs_oracle(ctx):
+    m = mg.hex_block(40, 20, 20)
+    nsteps = 1000
+    say(f"cfg3 explicit {m.ne} hexes, {nsteps} steps")
+    ctx.set_mesh(m.xyz, m.conn, m.etype, mg.free_index(m.nd, m.fix_set))
+    mass = ctx.lumped_mass(mg.RHO_STEEL)
+    assert rel_max(mass, O.lumped_mass(m.conn, m.etype, m.xyz, mg.RHO_STEEL)) < 1e-13
+    fixed = np.zeros(m.nd, dtype=np.int32)
+    fixed[m.fix_set - 1] = 1
+    v0 = mg.initial_velocity(m)
+    Ke = O.c3d8_ke(m.xyz[m.conn[0] - 1], E, NU)
+    me = O.c3d8_mass(m.xyz[m.conn[0] - 1], mg.RHO_STEEL)
+    dt = 0.9 * 2.0 / np.sqrt(np.linalg.eigvals(Ke / np.repeat(me, 3)[:, None]).real.max())
+    f = np.zeros((m.nd, 3))
+    u64 = ctx.explicit_run(E, NU, mass, f, v0, fixed, dt, nsteps, precision=A.FP64)
+    u32 = ctx.explicit_run(E, NU, mass, f, v0, fixed, dt, nsteps, precision=A.FP32)
+    ref = explicit_reference(m, mass, v0, fixed, dt, nsteps)
+    e64 = np.linalg.norm(u64 - ref) / np.linalg.norm(ref)
+    e32 = np.linalg.norm(u32 - ref) / np.linalg.norm(ref)
+    say(f"  after {nsteps} steps: FP64 {e64:.3e}, FP32 {e32:.3e} (relative L2 vs the oracle recurrence)")
+    assert e64 < 1e-9
+    assert e32 < FP32_CFG3_BOUND
+
+
+def test_cfg5_ramp_vs_oracle_newton(ctx):
+    m = mg.hex_block(20, 10, 10)
+    resort = mg.free_index(m.nd, m.fix_set)
+    ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
+    cp, ri = ctx.pattern()
+    K = ctx.assemble(E, NU)
+    F1 = mg.free_rhs(mg.face_traction(m, (0.0, 0.0, 1.0)), resort)
+    x1 = spla.spsolve(csc(cp, ri, K), F1)
+    tip = np.abs(m.xyz[:, 0] - 20.0) < 1e-9
+    p_star = 0.2 * 20.0 / ofa.reset_displacement(resort, x1).reshape(-1, 3)[tip, 2].mean()
+    F = p_star * F1
+    say(f"cfg5 ramp {m.ne} hexes, p* = {p_star:.2f} MPa, 10 increments")
+    u, nt, lt, rr, ok = ctx.nl_solve(E, NU, F, n_incr=10, tol=1e-10, max_newton=30, method=A.PCG_TWO_LEVEL,
+                                     inner_rtol=1e-12)
+    assert ok, (nt, rr)
+    uo, no = O.nl_newton(m.conn, m.etype, m.xyz, resort, E, NU, F, cp, ri, n_incr=10, tol=1e-10)
+    err = np.linalg.norm(u - uo) / np.linalg.norm(uo)
+    tipz = ofa.reset_displacement(resort, u).reshape(-1, 3)[tip, 2].mean()
+    say(f"  GPU Newton {nt} iterations ({lt} PCG), oracle {no}; |u - u_oracle| / |u_oracle| = {err:.3e}, "
+        f"tip u_z {tipz:.3f} mm")
+    assert err < 1e-8
+
+
+DECKS = ("mixed", "nlfea", "ca")
+# (PCG rtol, bound on the displacement error against the direct solve of the oracle's K)
+DECK_TOL = {"mixed": (1e-10, 1e-8), "nlfea": (1e-10, 1e-8), "ca": (1e-8, 1e-8)}
+
+
+def load_deck(key):
+    z = np.load(os.path.join(GOLDEN, "ref_decks.npz"))
+    conn = z[f"{key}_conn"]
+    if conn.shape[1] == 4:
+        conn = np.hstack([conn, np.zeros((len(conn), 4), np.int32)])
+    dof, val, Ed, nud = z[f"{key}_load"]
+    return (z[f"{key}_xyz"], conn.astype(np.int32), z[f"{key}_etype"].astype(np.int32), z[f"{key}_fix_set"],
+            z[f"{key}_load_set"], int(dof), float(val), float(Ed), float(nud))
+
+
+@pytest.mark.parametrize("key", DECKS)
+def test_reference_deck(ctx, key):
+    xyz, conn, et, fix, load, dof, lval, Ed, nud = load_deck(key)
+    resort, nfree = ofa.build_free_index(len(xyz), fix)
+    say(f"deck {key}: {len(xyz)} nodes, {len(et)} elements ({int((et == 1).sum())} C3D8)")
+    ctx.set_mesh(xyz, conn, et, resort)
+    cp, ri = ctx.pattern()
+    cpo, rio = O.build_csc(conn, et, resort, 3 * nfree)
+    assert np.array_equal(cp, cpo) and np.array_equal(ri, rio)
+    val = ctx.assemble(Ed, nud)
+    ref = O.fill_csc(conn, et, xyz, resort, Ed, nud, cp, ri)
+    err = rel_max(val, ref)
+    assert err < 1e-12, err
+    assert np.array_equal(ctx.assemble(Ed, nud), val)
+    F = ofa.build_single_load(3 * nfree, resort, load, dof, lval)
+    # A true FP64 residual cannot fall much below eps cond(K): CA_zongliang
+    # (E = 2.1e10, cond ~1e8) stalls near 1e-8 for every method, plain Jacobi
+    # included, so it is solved to 1e-8 (the time-to-solve tolerance); the
+    # other two decks to 1e-10.
+    rtol, bar = DECK_TOL[key]
+    u, it, rel, ok = ctx.solve(F, method=A.PCG_TWO_LEVEL, rtol=rtol, maxit=200000)
+    assert ok, (it, rel)
+    ud = spla.spsolve(csc(cp, ri, ref), F)
+    derr = np.linalg.norm(u - ud) / np.linalg.norm(ud)
+    say(f"  K within {err:.2e}; two-level PCG to {rtol:g}: {it} iterations, |u - u_direct| / |u_direct| = {derr:.2e}")
+    assert derr < bar
+    if key == "nlfea":  # the reference's geometric nonlinear sample: TL tangent at half the linear solution
+        KT, fint = ctx.nl_assemble(Ed, nud, 0.5 * u)
+        KTo, fo = O.nl_fill(conn, et, xyz, resort, Ed, nud, 0.5 * u, cp, ri)
+        assert rel_max(KT, KTo) < 1e-12 and rel_max(fint, fo) < 1e-12
