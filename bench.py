@@ -64,38 +64,48 @@ def cpu_baseline_solve(K, F):
     return (time.perf_counter() - t) * 1e3, it, rel
 
 
-def train_learned(workload, args, two_level=False):
-    """Trains the learned block-Jacobi correction on a smaller mesh of the same
-    family (different jitter seed) from that mesh's own PCG runs (block Jacobi,
-    or the two-level solver with coarse-corrected targets); the node-local
-    network then transfers to the benchmark mesh."""
-    import numpy as np
-
+def train_systems(specs, args):
+    """Trains the learned correction on the mean loss tr(G K G^T) / n of
+    several systems (one context each): specs are (label, mesh, resort, E_elem
+    or None).  Returns the parameters and a summary."""
     import openfea_amd as ofa
     from openfea_amd import learned as L, meshgen as mg
 
     t0 = time.perf_counter()
-    if workload == "cfg2":
-        m, resort, F = mg.cantilever_c3d4(24, 12, 12, seed=7)
-    else:
-        m, resort, F = mg.cantilever_c3d8(10, 5, 5)
-    ctx = ofa.Context(0)
-    ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
-    ctx.assemble(mg.E_STEEL, mg.NU_STEEL, copy=False)
-    rng = np.random.default_rng(11)
-    rhs = [F] + [ctx.spmv(rng.normal(size=ctx.n)) for _ in range(3)]
-    E, R = L.collect_pairs(ctx, rhs, snapshots=(8, 16, 32, 64, 128), two_level=two_level)
-    ln = L.Learned(ctx, seed=0)
-    ln.set_samples(E, R)
-    hist = ln.train(steps=args.train_steps, lr=2e-3)
-    params = ln.get_params()
-    ctx.close()
-    info = {"train_mesh": f"{m.ne} elements ({'C3D4 24x12x12, jitter seed 7' if workload == 'cfg2' else 'C3D8 10x5x5'})",
-            "pairs_from": "two-level PCG, coarse-corrected targets" if two_level else "block-Jacobi PCG",
-            "samples": int(len(E)), "steps": args.train_steps, "loss_first": float(hist[0]),
-            "loss_last": float(hist[-1]), "train_s": round(time.perf_counter() - t0, 3), "params": params}
-    log(f"learned: samples {len(E)} loss {hist[0]:.4e} -> {hist[-1]:.4e} in {info['train_s']}s")
-    return info
+    ctxs, lns = [], []
+    for label, m, resort, Ee in specs:
+        c = ofa.Context(0)
+        c.set_mesh(m.xyz, m.conn, m.etype, resort)
+        c.set_elem_E(Ee)
+        c.assemble(0.0 if Ee is not None else mg.E_STEEL, mg.NU_STEEL, copy=False)
+        ctxs.append(c)
+        lns.append(L.Learned(c, seed=0))
+    hist = L.train_multi(lns, steps=args.train_steps, lr=args.train_lr)
+    params = lns[0].get_params()
+    for c in ctxs:
+        c.close()
+    info = {"systems": [lbl for lbl, *_ in specs], "loss": "tr(G K G^T)/n (Kaporin, 1 = block Jacobi)",
+            "steps": args.train_steps, "lr": args.train_lr, "loss_first": float(hist[0]),
+            "loss_last": float(hist[-1]), "train_s": round(time.perf_counter() - t0, 3)}
+    log(f"learned: loss {hist[0]:.4f} -> {hist[-1]:.4f} in {info['train_s']}s on {len(specs)} systems")
+    return params, info
+
+
+def train_learned(workload, args):
+    """The learned correction for the static workloads, trained on smaller
+    meshes that the benchmark does not solve: two C3D4 blocks of other jitter
+    seeds and two C3D8 design variants."""
+    from openfea_amd import meshgen as mg
+
+    specs = []
+    for seed in (7, 8):
+        m, resort, _ = mg.cantilever_c3d4(24, 12, 12, seed=seed)
+        specs.append((f"C3D4 24x12x12 jitter seed {seed}", m, resort, None))
+    mh = mg.hex_block(16, 8, 8)
+    rh = mg.free_index(mh.nd, mh.fix_set)
+    for v in (0, 1):
+        specs.append((f"C3D8 16x8x8 design variant {v}", mh, rh, mg.design_field(mh, v)))
+    return train_systems(specs, args)
 
 
 def bench_static(args, workload):
@@ -146,16 +156,14 @@ def bench_static(args, workload):
                    "learned_two_level_mixed_fp32val_bf16net": A.PCG_LEARNED_TWO_LEVEL_MIXED}
     chosen = [] if args.no_solve else [m for m in args.solvers.split(",") if m]
     methods = [(m, all_methods[m]) for m in chosen]
-    learned_info, params = {}, {}
-    for tl in (False, True):
-        if any(m.startswith("learned") and (("two_level" in m) == tl) for m in chosen):
-            info = train_learned(workload, args, two_level=tl)
-            params[tl] = info.pop("params")
-            learned_info["two_level" if tl else "block_jacobi"] = info
-    ln = L.Learned(ctx, seed=0) if params else None
+    learned_info, params = {}, None
+    if any(m.startswith("learned") for m in chosen):
+        params, learned_info = train_learned(workload, args)
+    ln = L.Learned(ctx, seed=0) if params is not None else None
     infer = {}
     if ln is not None:  # network inference for the resident K: FP32 vs BF16 on the matrix cores
-        ln.set_params(next(iter(params.values())))
+        ln.set_params(params)
+        learned_info["loss_on_benchmark_K"] = float(ln.loss_grad()[0])
         for prec, tag in ((A.FP32, "fp32"), (A.BF16, "bf16_mfma")):
             ln.precision(prec)
             ts = []
@@ -170,7 +178,6 @@ def bench_static(args, workload):
     ctx.set_rhs(F)
     for name, method in methods:
         if name.startswith("learned"):
-            ln.set_params(params["two_level" in name])
             ln.precision(A.BF16 if name.endswith("bf16net") else A.FP32)
         ctx.sync()
         t = time.perf_counter()
@@ -501,24 +508,11 @@ def bench_newton(args, ctx, m, resort, E, nu, F, tip, L_):
     import openfea_amd as ofa
     from openfea_amd import _capi as A, learned as L, meshgen as mg
 
-    # learned correction trained on pairs from config-4 solves (40x20x20, design variant 0)
-    t0 = time.perf_counter()
+    # learned correction trained on the config-4 systems (40x20x20 C3D8, design variants 0..3)
     m4 = mg.hex_block(40, 20, 20)
     r4 = mg.free_index(m4.nd, m4.fix_set)
-    c4 = ofa.Context(0)
-    c4.set_mesh(m4.xyz, m4.conn, m4.etype, r4)
-    c4.set_elem_E(mg.design_field(m4, 0))
-    c4.assemble(0.0, nu, copy=False)
-    B4 = mg.batched_loads(m4, r4, 1, 4)[0]
-    Es, Rs = L.collect_pairs(c4, list(B4), snapshots=(8, 16, 32, 64, 128))
-    ln4 = L.Learned(c4, seed=0)
-    ln4.set_samples(Es, Rs)
-    hist = ln4.train(steps=args.train_steps, lr=2e-3)
-    params = ln4.get_params()
-    c4.close()
-    train = {"samples": int(len(Es)), "steps": args.train_steps, "loss_first": float(hist[0]),
-             "loss_last": float(hist[-1]), "train_s": round(time.perf_counter() - t0, 3)}
-    log(f"learned (cfg4 pairs): {train}")
+    params, train = train_systems([(f"C3D8 40x20x20 design variant {v}", m4, r4, mg.design_field(m4, v))
+                                   for v in range(4)], args)
     ln = L.Learned(ctx, seed=0)
     ln.set_params(params)
     runs = {}
@@ -811,6 +805,7 @@ def main():
                     "two_level_mixed_fp32val,learned_two_level_fp64,learned_two_level_mixed_fp32val,"
                     "learned_two_level_mixed_fp32val_bf16net")
     ap.add_argument("--train-steps", type=int, default=150)
+    ap.add_argument("--train-lr", type=float, default=1e-2)
     ap.add_argument("--ca-basis", type=int, default=4, help="CA basis size (ca workload)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -258,35 +258,49 @@ int ofea_nl_solve(ofea_ctx *ctx, double E, double nu, const double *F, int n_inc
                   int method, double inner_rtol, double *u_out, int *newton_iters, int *linear_iters,
                   double *relres);
 
-/* ---- learned block-Jacobi correction (no counterpart in the reference) ----
- * A 2-layer node-graph network (hidden 64) maps per-node features of the
- * resident K to SPD 3x3 preconditioner blocks M_r = R_r^-1 L_r L_r^T R_r^-T
- * (D_r = R_r^T R_r); L_r = I is block Jacobi.  OFEA_PCG_LEARNED uses them. */
+/* ---- learned factorized correction of block Jacobi (no counterpart in the
+ * reference) ----
+ * M = G^T G with G block lower-triangular on K's node-block pattern:
+ *   G_ii = L_i^-1 (D_i = L_i L_i^T, the 3x3 diagonal block of free node i),
+ *   G_is = -alpha_is L_i^-1 K_is D_s^-1 for each neighbour s < i,
+ *   alpha_is = 2 sigmoid(u_i + v_s) in (0, 2),
+ * where (u_i, v_i) are the two outputs of a 2-layer node-graph network
+ * (hidden 64, mean aggregation) over 9 features per node of the resident K.
+ * alpha = 0 is block Jacobi; M is SPD for any parameters.  OFEA_PCG_LEARNED*
+ * keep G in BF16 and apply z = G^T y with y = G r carried by its own
+ * recursion (one sweep over each half of G per iteration). */
 int ofea_learned_nparam(void);
-/* features of the resident K, deterministic initial parameters (head = 0,
- * i.e. block Jacobi), forward pass */
+/* features of the resident K, deterministic initial parameters (head zero:
+ * alpha = 1, the first-order rule), forward pass */
 int ofea_learned_init(ofea_ctx *ctx, unsigned long long seed);
 int ofea_learned_set_params(ofea_ctx *ctx, const float *params, int nparam);
 int ofea_learned_get_params(ofea_ctx *ctx, float *params, int nparam);
-/* training pairs (S samples, sample-major, n entries each): errors E and
- * residuals R = K E of the resident K */
-int ofea_learned_set_samples(ofea_ctx *ctx, const double *E, const double *R, int S);
-/* loss (1/S) sum_k |e_k - M r_k|_K^2 / (e_k^T r_k) and its gradient (nparam) */
+/* loss tr(G K G^T) / n for the resident K (n = free DOFs; 1 at alpha = 0;
+ * det(G K G^T) does not depend on alpha, so this is Kaporin's condition
+ * number of G K G^T up to a constant factor) and its gradient (nparam) */
 int ofea_learned_loss_grad(ofea_ctx *ctx, double *loss, float *grad);
-/* `steps` Adam steps at learning rate lr; loss_hist (steps, nullable) */
+/* `steps` Adam steps at learning rate lr on the resident K; loss_hist (steps, nullable) */
 int ofea_learned_train(ofea_ctx *ctx, int steps, float lr, double *loss_hist);
-/* current blocks (9 per free node, FP64) and features (9 per free node) */
+/* the same on the mean loss of nctx contexts (one resident system each, all on
+ * one device, each after ofea_learned_init): the parameters start from
+ * ctxs[0]'s and end in every context */
+int ofea_learned_train_multi(ofea_ctx *const *ctxs, int nctx, int steps, float lr, double *loss_hist);
+/* the factor G for the resident K as nnz values on K's layout (per free node
+ * i and block slot k a row-major 3x3: G_is for s < i, G_ii, (G_si)^T for
+ * s > i; stored BF16, returned widened to FP64) and the features (9 per free
+ * node); either pointer nullable */
 int ofea_learned_blocks(ofea_ctx *ctx, double *blocks, float *features);
-/* recompute features and blocks for a newly assembled K */
+/* the network's outputs (u_i, v_i) per free node (2 nfree), as they serve the solver */
+int ofea_learned_outputs(ofea_ctx *ctx, float *uv);
+/* recompute features and the factor for a newly assembled K */
 int ofea_learned_refresh(ofea_ctx *ctx);
 /* inference precision of the network's hidden layer when it produces the
- * solver's blocks: OFEA_FP32 (default) or OFEA_BF16 (BF16 operands, FP32
+ * solver's factor: OFEA_FP32 (default) or OFEA_BF16 (BF16 operands, FP32
  * accumulation on the matrix cores, v_mfma_f32_16x16x16_bf16); training stays
  * FP32.  Re-runs the network for the resident K. */
 int ofea_learned_precision(ofea_ctx *ctx, int precision);
 /* out = Z A_c^-1 Z^T r (n), the coarse correction of the two-level methods for
- * the resident K (built on first use).  Training pairs of the learned
- * two-level smoother use e - out as the target. */
+ * the resident K (built on first use). */
 int ofea_coarse_apply(ofea_ctx *ctx, const double *r, double *out);
 /* The aggregates of that coarse space (no counterpart in the reference):
  * agg (nfree = n/3) receives the aggregate of every free node; returns the

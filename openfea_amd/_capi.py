@@ -84,10 +84,11 @@ SIGNATURES = {
     "ofea_learned_init": (_i, [_vp, C.c_ulonglong]),
     "ofea_learned_set_params": (_i, [_vp, _vp, _i]),
     "ofea_learned_get_params": (_i, [_vp, _vp, _i]),
-    "ofea_learned_set_samples": (_i, [_vp, _vp, _vp, _i]),
     "ofea_learned_loss_grad": (_i, [_vp, _pd, _vp]),
     "ofea_learned_train": (_i, [_vp, _i, _f, _vp]),
+    "ofea_learned_train_multi": (_i, [_vp, _i, _i, _f, _vp]),
     "ofea_learned_blocks": (_i, [_vp, _vp, _vp]),
+    "ofea_learned_outputs": (_i, [_vp, _vp]),
     "ofea_learned_refresh": (_i, [_vp]),
     "ofea_learned_precision": (_i, [_vp, _i]),
     "ofea_coarse_apply": (_i, [_vp, _vp, _vp]),

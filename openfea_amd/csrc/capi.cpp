@@ -443,13 +443,6 @@ int ofea_learned_get_params(ofea_ctx *c, float *params, int nparam)
     return learned_get_params(c, params, nparam);
 }
 
-int ofea_learned_set_samples(ofea_ctx *c, const double *E, const double *R, int S)
-{
-    CHECK_CTX(c);
-    if (!E || !R || S <= 0) return fail(OFEA_ERR_ARG, "bad samples");
-    return learned_set_samples(c, E, R, S);
-}
-
 int ofea_learned_loss_grad(ofea_ctx *c, double *loss, float *grad)
 {
     CHECK_CTX(c);
@@ -460,13 +453,32 @@ int ofea_learned_train(ofea_ctx *c, int steps, float lr, double *loss_hist)
 {
     CHECK_CTX(c);
     if (steps < 0 || !(lr > 0.0f)) return fail(OFEA_ERR_ARG, "steps >= 0 and lr > 0 required");
-    return learned_train(c, steps, lr, loss_hist);
+    return learned_train(&c, 1, steps, lr, loss_hist);
+}
+
+int ofea_learned_train_multi(ofea_ctx *const *ctxs, int nctx, int steps, float lr, double *loss_hist)
+{
+    if (!ctxs || nctx < 1) return fail(OFEA_ERR_ARG, "need at least one context");
+    for (int k = 0; k < nctx; ++k) {
+        if (!ctxs[k]) return fail(OFEA_ERR_ARG, "null context");
+        if (ctxs[k]->device != ctxs[0]->device) return fail(OFEA_ERR_ARG, "contexts on different devices");
+    }
+    CHECK_CTX(ctxs[0]);
+    if (steps < 0 || !(lr > 0.0f)) return fail(OFEA_ERR_ARG, "steps >= 0 and lr > 0 required");
+    return learned_train(ctxs, nctx, steps, lr, loss_hist);
 }
 
 int ofea_learned_blocks(ofea_ctx *c, double *blocks, float *features)
 {
     CHECK_CTX(c);
     return learned_blocks(c, blocks, features);
+}
+
+int ofea_learned_outputs(ofea_ctx *c, float *uv)
+{
+    CHECK_CTX(c);
+    if (!uv) return fail(OFEA_ERR_ARG, "null output");
+    return learned_outputs(c, uv);
 }
 
 int ofea_learned_precision(ofea_ctx *c, int precision)

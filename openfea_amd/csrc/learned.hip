@@ -1,23 +1,33 @@
-// learned.hip -- learned correction of the block-Jacobi preconditioner.
+// learned.hip -- learned factorized correction of the block-Jacobi preconditioner.
 //
 // No counterpart in the reference (its solvers are direct, src/linear_solution.cpp).
-// A 2-layer node-graph network over the free-node adjacency of K maps local
-// features of each node to a lower-triangular L_r; the preconditioner block
-// is M_r = R_r^-1 (L_r L_r^T) R_r^-T with D_r = R_r^T R_r the node's 3x3
-// diagonal block of K.  M_r is SPD for any parameters, and L_r = I gives
-// block Jacobi exactly (the output head starts at zero).  PCG stays a valid
-// CG with a fixed SPD preconditioner; plain block-Jacobi PCG is the fallback.
+// With D_i = L_i L_i^T the 3x3 diagonal block of free node i (L_i lower
+// Cholesky), the preconditioner is M = G^T G for the block lower-triangular
+//   G_ii = L_i^-1,   G_is = -alpha_is L_i^-1 K_is D_s^-1   (s < i a neighbour of i),
+// the first step of a factorized sparse approximate inverse on K's node-block
+// pattern.  alpha = 0 is block Jacobi; a 2-layer node-graph network sets
+//   alpha_is = 2 sigmoid(u_i + v_s),  (u_i, v_i) = its two outputs at node i.
+// M is SPD for any parameters (G is triangular with nonsingular diagonal
+// blocks), so PCG stays a valid CG; plain block Jacobi is the fallback and
+// judge.  Applying M costs one pass over each half of G (BF16) per iteration
+// (the update kernel's y recursion and k_fac_upper in solver.hip).
 //
 //   features x_r (9): D_r / t_r (6 entries), log t_r - mean log t, deg_r / 27,
 //                     sum_{s!=r} ||K_rs||_F / ((deg_r - 1) ||D_r||_F);  t_r = tr(D_r)/3
 //   layer l:  h' = relu(W_self h_r + W_nbr mean_{s in N(r), s != r} h_s + b)   (9 -> 64 -> 64)
-//   head:     o_r = W_o h2_r + b_o (6);  L = [[e^o0,0,0],[o3,e^o1,0],[o5,o4,e^o2]]
+//   head:     (u_r, v_r) = W_o h2_r + b_o   (W_o, b_o start at zero: alpha = 1,
+//             the untrained first-order rule)
 //
-// Training loss over S pairs (e_k, r_k = K e_k) from the solver's own runs:
-//   loss = (1/S) sum_k (e_k - M r_k)^T K (e_k - M r_k) / (e_k^T r_k)
-// i.e. the energy-norm error left by one preconditioned Richardson step.
-// Its gradient needs one SpMV per sample; everything else is node-local or a
-// neighbour gather.  All training arithmetic is FP32 (M and the SpMV in FP64).
+// Training loss: tr(G K G^T) / n on the resident K.  G's diagonal blocks do
+// not depend on alpha, so det(G K G^T) is fixed and the loss is Kaporin's
+// condition number of G K G^T up to a constant factor, the quality measure of
+// factorized sparse approximate inverses; it is 1 at alpha = 0 and, unlike a
+// loss on sampled error vectors, it rises when alpha creates small
+// eigenvalues.  Row i contributes T_i = sum_{s,t <= i} tr(G_is K_st G_it^T);
+// with W_s = sum_{t <= i} K_st G_it^T, dT_i / dalpha_is = -2 tr(C_is W_s)
+// (C_is = L_i^-1 K_is D_s^-1), then dalpha/dx = alpha (1 - alpha / 2) and the
+// network's backward pass.  Network arithmetic FP32; G, the loss and the
+// per-pair derivatives FP64; the solver's copy of G BF16.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -31,7 +41,8 @@ namespace ofea {
 
 constexpr int LN_F = 9;     // input features
 constexpr int LN_H = 64;    // hidden width
-constexpr int LN_O = 6;     // head outputs
+constexpr int LN_O = 2;     // head outputs (u, v)
+constexpr float LN_HEAD_BIAS0 = 0.0f;  // initial u: alpha = 2 sigmoid(0) = 1, the first-order rule
 constexpr int LN_WG = 256;
 
 // parameter layout (floats)
@@ -51,18 +62,16 @@ struct LearnedWS {
     float *d_params = nullptr, *d_grad = nullptr;
     float *X = nullptr, *A0 = nullptr;              // N x 9
     float *Z1 = nullptr, *H1 = nullptr, *A1 = nullptr, *Z2 = nullptr, *H2 = nullptr;  // N x 64
-    float *O = nullptr;                             // N x 6
-    double *Rinv = nullptr;                         // N x 9 (upper-triangular R^-1, row-major)
-    float *dO = nullptr;                            // N x 6
+    float *O = nullptr;                             // N x 2
+    double *Rinv = nullptr;                         // N x 9 (upper-triangular R^-1, D = R^T R, row-major)
+    float *dO = nullptr;                            // N x 2
     float *dZ2 = nullptr, *dH1 = nullptr, *dZ1 = nullptr, *T1 = nullptr;  // N x 64
-    double *G = nullptr;                            // N x 9 gradient of the loss wrt M_r
     float *partial = nullptr;                       // weight-gradient partials
     int npartial_rows = 0;
-    // samples
-    int S = 0;
-    double *E = nullptr, *R = nullptr, *Dv = nullptr, *Q = nullptr;  // n x S each
-    double *wsmp = nullptr;                         // S: 1/(S e_k^T r_k)
-    double *loss_part = nullptr;
+    int gen = 0;                                    // incremented by every forward pass that serves the solver
+    double *Trow = nullptr;                         // N: T_i of the loss
+    double *dA = nullptr;                           // lower blocks of G: dloss/dalpha_is (0 on the diagonal)
+    double *tpart = nullptr;                        // partial sums of T
     int feat_kver = -1;  // K version the features were computed from
     bool bf16 = false;   // inference of layer 2 on the matrix cores in BF16 (training stays FP32)
 };
@@ -75,8 +84,8 @@ void learned_free(ofea_ctx *c)
     if (!w) return;
     for (void *p : {(void *)w->d_params, (void *)w->d_grad, (void *)w->X, (void *)w->A0, (void *)w->Z1, (void *)w->H1,
                     (void *)w->A1, (void *)w->Z2, (void *)w->H2, (void *)w->O, (void *)w->Rinv, (void *)w->dO,
-                    (void *)w->dZ2, (void *)w->dH1, (void *)w->dZ1, (void *)w->T1, (void *)w->G, (void *)w->partial,
-                    (void *)w->E, (void *)w->R, (void *)w->Dv, (void *)w->Q, (void *)w->wsmp, (void *)w->loss_part})
+                    (void *)w->dZ2, (void *)w->dH1, (void *)w->dZ1, (void *)w->T1, (void *)w->partial,
+                    (void *)w->Trow, (void *)w->dA, (void *)w->tpart})
         if (p) (void)hipFree(p);
     delete w;
     c->learned = nullptr;
@@ -256,126 +265,273 @@ __global__ __launch_bounds__(LN_MF_WG) void k_ln_dense_mfma(int nfree, const flo
         }
 }
 
-// head: o = Wo h2 + bo; M = A L L^T A^T (FP64).  Thread per node.
-__device__ inline void ln_Lmat(const float o[6], double L[9])
-{
-    L[0] = exp((double)o[0]);
-    L[1] = 0.0;
-    L[2] = 0.0;
-    L[3] = o[3];
-    L[4] = exp((double)o[1]);
-    L[5] = 0.0;
-    L[6] = o[5];
-    L[7] = o[4];
-    L[8] = exp((double)o[2]);
-}
-
+// head: (u, v) = Wo h2 + bo.  Thread per node.
 __global__ void k_ln_head(int nfree, const float *__restrict__ H2, const float *__restrict__ Wo,
-                          const float *__restrict__ bo, const double *__restrict__ Rinv, float *__restrict__ O,
-                          double *__restrict__ M)
+                          const float *__restrict__ bo, float *__restrict__ O)
 {
     const int r = blockIdx.x * LN_WG + threadIdx.x;
     if (r >= nfree) return;
-    float o[6];
     const float *h = H2 + (long long)r * LN_H;
     for (int j = 0; j < LN_O; ++j) {
         float acc = bo[j];
         for (int i = 0; i < LN_H; ++i) acc += Wo[j * LN_H + i] * h[i];
-        o[j] = acc;
         O[LN_O * r + j] = acc;
     }
-    double L[9], B[9];
-    ln_Lmat(o, L);
-    const double *A = Rinv + 9 * r;
-    // B = A L ; M = B B^T
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) B[3 * i + j] = A[3 * i] * L[j] + A[3 * i + 1] * L[3 + j] + A[3 * i + 2] * L[6 + j];
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j)
-            M[9 * r + 3 * i + j] = B[3 * i] * B[3 * j] + B[3 * i + 1] * B[3 * j + 1] + B[3 * i + 2] * B[3 * j + 2];
 }
 
-// d_k = e_k - M r_k (node-local), all samples.  Thread per (node, sample).
-__global__ void k_ln_residual(int nfree, int S, const double *__restrict__ M, const double *__restrict__ E,
-                              const double *__restrict__ R, double *__restrict__ Dv)
+__device__ inline double ln_alpha(float u, float v) { return 2.0 / (1.0 + exp(-(double)u - (double)v)); }
+
+// 3x3 products (row-major)
+__device__ inline void mm3(const double A[9], const double B[9], double C[9])
 {
-    const long long idx = (long long)blockIdx.x * LN_WG + threadIdx.x;
-    if (idx >= (long long)nfree * S) return;
-    const int k = (int)(idx / nfree), r = (int)(idx % nfree);
-    const long long base = (long long)k * 3 * nfree + 3 * r;
-    const double *m = M + 9 * r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+}
+
+// node data through an optional map (reordered rows: map[i] = the caller's node)
+struct FacNodes {
+    const double *Rinv;  // 9 per node, R^-1 with D = R^T R (L^-1 = (R^-1)^T)
+    const double *Binv;  // 9 per node, D^-1
+    const float *O;      // 2 per node, (u, v)
+    const int *map;
+    __device__ int at(int i) const { return map ? map[i] : i; }
+    __device__ void linv(int i, double L[9]) const
+    {
+        const double *A = Rinv + 9 * (size_t)at(i);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) L[3 * a + b] = A[3 * b + a];
+    }
+    __device__ void binv(int i, double B[9]) const
+    {
+        const double *A = Binv + 9 * (size_t)at(i);
+#pragma unroll
+        for (int e = 0; e < 9; ++e) B[e] = A[e];
+    }
+    __device__ double alpha(int i, int s) const { return ln_alpha(O[2 * at(i)], O[2 * at(s) + 1]); }
+};
+
+// K_is of node i's block slot k ([c][d] = K[3i + c][3s + d]) from K's node-block layout
+__device__ inline void k_block(const double *val, int nb0, int deg, int k, double Kb[9])
+{
+#pragma unroll
     for (int c = 0; c < 3; ++c)
-        Dv[base + c] = E[base + c] - (m[3 * c] * R[base] + m[3 * c + 1] * R[base + 1] + m[3 * c + 2] * R[base + 2]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) Kb[3 * c + d] = val[9 * (long long)nb0 + c * 3 * deg + 3 * k + d];
 }
 
-// loss partials: sum_k w_k d_k . q_k ; and the gradient wrt M_r:
-// G_r = -2 sum_k w_k q_{k,r} r_{k,r}^T.  Thread per node.
-__global__ void k_ln_lossgrad(int nfree, int S, const double *__restrict__ Dv, const double *__restrict__ Q,
-                              const double *__restrict__ R, const double *__restrict__ wsmp, double *__restrict__ G,
-                              double *__restrict__ loss_part)
+// G as the two half matrices of the solver (FacView in solver.hip, BF16): the
+// blocks of K's row i in K's order, s <= i into row i of lo, s >= i into row
+// i of up, each in the column-of-rows layout (entry (c, 3 k + d) of a row of
+// nb blocks at 9 ptr[i] + 3 nb c + 3 k + d):
+//   lo: s < i: G_is = -alpha_is L_i^-1 K_is D_s^-1;  s == i: L_i^-1;
+//   up: s > i: (G_si)^T = -alpha_si D_i^-1 K_is L_s^-T (K_si = K_is^T);  s == i: L_i^-T.
+// Thread per node (its blocks in a loop).
+__global__ void k_fac_build(int nfree, const int *__restrict__ node_blk, const int *__restrict__ bcol,
+                            const double *__restrict__ val, FacNodes nd, const int *__restrict__ lo_ptr,
+                            const int *__restrict__ up_ptr, unsigned short *__restrict__ lo,
+                            unsigned short *__restrict__ up)
 {
-    __shared__ double sm[LN_WG / 64];
-    const int r = blockIdx.x * LN_WG + threadIdx.x;
-    double l = 0.0;
-    if (r < nfree) {
-        double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int k = 0; k < S; ++k) {
-            const long long base = (long long)k * 3 * nfree + 3 * r;
-            const double w = wsmp[k];
-            l += w * (Dv[base] * Q[base] + Dv[base + 1] * Q[base + 1] + Dv[base + 2] * Q[base + 2]);
-            for (int c = 0; c < 3; ++c)
-                for (int d = 0; d < 3; ++d) g[3 * c + d] -= 2.0 * w * Q[base + c] * R[base + d];
+    const int i = blockIdx.x * LN_WG + threadIdx.x;
+    if (i >= nfree) return;
+    const int nb0 = node_blk[i], deg = node_blk[i + 1] - nb0;
+    const int l0 = lo_ptr[i], nl = lo_ptr[i + 1] - l0, u0 = up_ptr[i], nu = up_ptr[i + 1] - u0;
+    unsigned short *lrow = lo + 9 * (size_t)l0, *urow = up + 9 * (size_t)u0;
+    auto put = [](unsigned short *row, int nb, int kk, const double G[9]) {  // FP64 -> FP32 -> BF16
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) row[3 * nb * c + 3 * kk + d] = (unsigned short)to_bf16((float)G[3 * c + d]);
+    };
+    double Li[9], Bi[9];
+    nd.linv(i, Li);
+    nd.binv(i, Bi);
+    int kl = 0, ku = 0;
+    for (int k = 0; k < deg; ++k) {
+        const int s = bcol[nb0 + k];
+        double G[9];
+        if (s == i) {
+            put(lrow, nl, kl++, Li);
+#pragma unroll
+            for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+                for (int b2 = 0; b2 < 3; ++b2) G[3 * a2 + b2] = Li[3 * b2 + a2];
+            put(urow, nu, ku++, G);
+            continue;
         }
-        for (int i = 0; i < 9; ++i) G[9 * r + i] = g[i];
-    }
-    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
-    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = l;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int k = 0; k < LN_WG / 64; ++k) t += sm[k];
-        loss_part[blockIdx.x] = t;
+        double Kb[9], T[9], X[9];
+        k_block(val, nb0, deg, k, Kb);
+        if (s < i) {
+            double Bs[9];
+            nd.binv(s, Bs);
+            mm3(Li, Kb, T);
+            mm3(T, Bs, X);
+            const double a = nd.alpha(i, s);
+#pragma unroll
+            for (int e = 0; e < 9; ++e) G[e] = -a * X[e];
+            put(lrow, nl, kl++, G);
+        } else {
+            double Ls[9], LsT[9];
+            nd.linv(s, Ls);
+#pragma unroll
+            for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+                for (int b2 = 0; b2 < 3; ++b2) LsT[3 * a2 + b2] = Ls[3 * b2 + a2];
+            mm3(Bi, Kb, T);
+            mm3(T, LsT, X);
+            const double a = nd.alpha(s, i);
+#pragma unroll
+            for (int e = 0; e < 9; ++e) G[e] = -a * X[e];
+            put(urow, nu, ku++, G);
+        }
     }
 }
 
-// dO from G through M = A L L^T A^T and the exp diagonal.  Thread per node.
-__global__ void k_ln_head_back(int nfree, const double *__restrict__ G, const double *__restrict__ Rinv,
-                               const float *__restrict__ O, float *__restrict__ dO)
+// Loss rows, one wave per free node i of K's rows (sorted, so the blocks
+// s <= i of row i are its first np slots, the diagonal last): lane k < np
+// forms G_ik (and C_ik) of slot k into LDS, then W_k = sum_m K_{s_k s_m}
+// G_{i s_m}^T (K_{s_k s_m} found by binary search in row s_k; zero when the
+// two are not neighbours), T_i = sum_k tr(G_ik W_k) and
+// dA[lo_ptr[i] + k] = -2 tr(C_ik W_k) / n for s_k < i.  Sums over the lanes
+// in a fixed order.
+constexpr int KAP_MAXP = 64;  // lower blocks per row (checked on the host)
+__global__ __launch_bounds__(LN_WG) void k_kap_rows(int nfree, const int *__restrict__ node_blk,
+                                                   const int *__restrict__ bcol, const double *__restrict__ val,
+                                                   FacNodes nd, const int *__restrict__ lo_ptr, double inv_n,
+                                                   double *__restrict__ Trow, double *__restrict__ dA)
+{
+    __shared__ double sG[LN_WG / 64][KAP_MAXP][9];
+    __shared__ int sS[LN_WG / 64][KAP_MAXP];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * (LN_WG / 64) + wv;  // one row per wave, no loop: every wave reaches the barrier
+    const bool act = i < nfree;
+    int np = 0, nb0 = 0, deg = 0;
+    double C[9], G[9];
+    int s = -1;
+    if (act) {
+        np = lo_ptr[i + 1] - lo_ptr[i];
+        nb0 = node_blk[i];
+        deg = node_blk[i + 1] - nb0;
+        if (lane < np) {
+            s = bcol[nb0 + lane];
+            double Li[9];
+            nd.linv(i, Li);
+            if (s == i) {
+#pragma unroll
+                for (int e = 0; e < 9; ++e) G[e] = Li[e];
+            } else {
+                double Kb[9], T[9], Bs[9];
+                k_block(val, nb0, deg, lane, Kb);
+                nd.binv(s, Bs);
+                mm3(Li, Kb, T);
+                mm3(T, Bs, C);
+                const double a = nd.alpha(i, s);
+#pragma unroll
+                for (int e = 0; e < 9; ++e) G[e] = -a * C[e];
+            }
+#pragma unroll
+            for (int e = 0; e < 9; ++e) sG[wv][lane][e] = G[e];
+            sS[wv][lane] = s;
+        }
+    }
+    __syncthreads();
+    double t = 0.0;
+    if (act && lane < np) {
+        const int b0 = node_blk[s], dg = node_blk[s + 1] - b0;
+        double W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int m = 0; m < np; ++m) {
+            const int tm = sS[wv][m];
+            int lo = 0, hi = dg - 1, q = -1;  // slot of column tm in row s
+            while (lo <= hi) {
+                const int mid = (lo + hi) >> 1, cm = bcol[b0 + mid];
+                if (cm == tm) {
+                    q = mid;
+                    break;
+                }
+                if (cm < tm) lo = mid + 1;
+                else hi = mid - 1;
+            }
+            if (q < 0) continue;
+            double Kst[9];
+            k_block(val, b0, dg, q, Kst);
+            const double *Gm = sG[wv][m];
+#pragma unroll
+            for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+                for (int b2 = 0; b2 < 3; ++b2)  // W[a][b] += sum_c K_st[a][c] G_it[b][c]
+                    W[3 * a2 + b2] += Kst[3 * a2] * Gm[3 * b2] + Kst[3 * a2 + 1] * Gm[3 * b2 + 1] +
+                                      Kst[3 * a2 + 2] * Gm[3 * b2 + 2];
+        }
+        double gw = 0.0, cw = 0.0;
+#pragma unroll
+        for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+            for (int b2 = 0; b2 < 3; ++b2) {
+                gw += G[3 * a2 + b2] * W[3 * b2 + a2];
+                cw += (s == i ? 0.0 : C[3 * a2 + b2]) * W[3 * b2 + a2];
+            }
+        t = gw;
+        dA[lo_ptr[i] + lane] = s == i ? 0.0 : -2.0 * cw * inv_n;
+    }
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+    if (act && lane == 0) Trow[i] = t;
+}
+
+// dO = dloss/d(u, v): u_r collects its lower pairs (r, s), v_r the pairs
+// (i, r) of its upper neighbours i > r (their lower slot for r found by
+// binary search in row i), each times dalpha/dx = alpha (1 - alpha / 2).
+// Thread per node, fixed order.
+__global__ void k_kap_dO(int nfree, const int *__restrict__ lo_ptr, const int *__restrict__ lo_col,
+                         const int *__restrict__ up_ptr, const int *__restrict__ up_col, FacNodes nd,
+                         const double *__restrict__ dA, float *__restrict__ dO)
 {
     const int r = blockIdx.x * LN_WG + threadIdx.x;
     if (r >= nfree) return;
-    const double *A = Rinv + 9 * r, *Gr = G + 9 * r;
-    float o[6];
-    for (int j = 0; j < 6; ++j) o[j] = O[LN_O * r + j];
-    double L[9], H[9], T[9], dL[9];
-    ln_Lmat(o, L);
-    // H = A^T G A
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-            double acc = 0.0;
-            for (int a = 0; a < 3; ++a) acc += A[3 * a + i] * Gr[3 * a + j];
-            T[3 * i + j] = acc;
+    double du = 0.0, dv = 0.0;
+    for (int q = lo_ptr[r]; q < lo_ptr[r + 1]; ++q) {
+        const int s = lo_col[q];
+        if (s == r) continue;
+        const double a = nd.alpha(r, s);
+        du += dA[q] * a * (1.0 - 0.5 * a);
+    }
+    for (int q = up_ptr[r]; q < up_ptr[r + 1]; ++q) {
+        const int i = up_col[q];
+        if (i == r) continue;
+        int lo = lo_ptr[i], hi = lo_ptr[i + 1] - 1, k = -1;
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1, cm = lo_col[mid];
+            if (cm == r) {
+                k = mid;
+                break;
+            }
+            if (cm < r) lo = mid + 1;
+            else hi = mid - 1;
         }
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-            double acc = 0.0;
-            for (int a = 0; a < 3; ++a) acc += T[3 * i + a] * A[3 * a + j];
-            H[3 * i + j] = acc;
-        }
-    // dL = (H + H^T) L
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-            double acc = 0.0;
-            for (int a = 0; a < 3; ++a) acc += (H[3 * i + a] + H[3 * a + i]) * L[3 * a + j];
-            dL[3 * i + j] = acc;
-        }
-    float *d = dO + LN_O * r;
-    d[0] = (float)(dL[0] * L[0]);
-    d[1] = (float)(dL[4] * L[4]);
-    d[2] = (float)(dL[8] * L[8]);
-    d[3] = (float)dL[3];
-    d[4] = (float)dL[7];
-    d[5] = (float)dL[6];
+        if (k < 0) continue;  // the pattern is symmetric: not reached
+        const double a = nd.alpha(i, r);
+        dv += dA[k] * a * (1.0 - 0.5 * a);
+    }
+    dO[2 * r] = (float)du;
+    dO[2 * r + 1] = (float)dv;
+}
+
+// partial sums of T over fixed chunks (one workgroup each)
+__global__ void k_kap_sum(int nfree, const double *__restrict__ Trow, double *__restrict__ part)
+{
+    __shared__ double sm[LN_WG / 64];
+    double a = 0.0;
+    for (int i = blockIdx.x * LN_WG + threadIdx.x; i < nfree; i += gridDim.x * LN_WG) a += Trow[i];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < LN_WG / 64; ++w) t += sm[w];
+        part[blockIdx.x] = t;
+    }
 }
 
 // dH[r][i] = sum_o W[o][i] dZ[r][o]  (then optionally masked by relu'(Zmask))
@@ -460,16 +616,13 @@ static int ensure_lws(ofea_ctx *c)
     OFEA_HIP(hipMalloc((void **)&w->O, sizeof(float) * Nb * LN_O));
     OFEA_HIP(hipMalloc((void **)&w->dO, sizeof(float) * Nb * LN_O));
     OFEA_HIP(hipMalloc((void **)&w->Rinv, sizeof(double) * Nb * 9));
-    OFEA_HIP(hipMalloc((void **)&w->G, sizeof(double) * Nb * 9));
     const int nchunks = (N + LN_WG - 1) / LN_WG;
     w->npartial_rows = nchunks;
     OFEA_HIP(hipMalloc((void **)&w->partial, sizeof(float) * (size_t)std::max(nchunks, 1) * LN_H * (LN_H + 1)));
-    OFEA_HIP(hipMalloc((void **)&w->loss_part, sizeof(double) * (size_t)std::max(nchunks, 1)));
     return OFEA_OK;
 }
 
 int solver_prepare(ofea_ctx *ctx);
-int solver_spmv(ofea_ctx *ctx, const double *d_x, double *d_y);
 
 static int ln_features(ofea_ctx *c)
 {
@@ -489,13 +642,12 @@ static int ln_features(ofea_ctx *c)
     return OFEA_OK;
 }
 
-// forward pass with the device parameters; M blocks (FP64) into learned64.
-// infer: the network produces the solver's blocks (BF16 layer 2 when set);
-// training always runs FP32.
+// forward pass with the device parameters: the head outputs (u, v) per node.
+// infer: the outputs serve the solver (BF16 layer 2 when set; the generation
+// counter tells the solver to rebuild G); training always runs FP32.
 static int ln_forward(ofea_ctx *c, bool infer)
 {
     LearnedWS *w = lws(c);
-    ofea_solver_ws *s = c->ws;
     const int N = w->nfree;
     const float *P = w->d_params;
     hipLaunchKernelGGL(k_ln_dense, dim3(grid_for((long long)N * LN_H)), dim3(LN_WG), 0, c->stream, N, LN_F, LN_H, w->X,
@@ -508,9 +660,33 @@ static int ln_forward(ofea_ctx *c, bool infer)
     else
         hipLaunchKernelGGL(k_ln_dense, dim3(grid_for((long long)N * LN_H)), dim3(LN_WG), 0, c->stream, N, LN_H, LN_H,
                            w->H1, w->A1, P + OFF_W2S, P + OFF_W2N, P + OFF_B2, w->Z2, w->H2);
-    if (!s->learned64) OFEA_HIP(hipMalloc((void **)&s->learned64, sizeof(double) * 9 * (size_t)std::max(N, 1)));
-    hipLaunchKernelGGL(k_ln_head, dim3(grid_for(N)), dim3(LN_WG), 0, c->stream, N, w->H2, P + OFF_WO, P + OFF_BO,
-                       w->Rinv, w->O, s->learned64);
+    hipLaunchKernelGGL(k_ln_head, dim3(grid_for(N)), dim3(LN_WG), 0, c->stream, N, w->H2, P + OFF_WO, P + OFF_BO, w->O);
+    OFEA_HIP(hipGetLastError());
+    if (infer) w->gen++;
+    return OFEA_OK;
+}
+
+static FacNodes fac_nodes(ofea_ctx *c, const int *map)
+{
+    LearnedWS *w = lws(c);
+    return FacNodes{w->Rinv, c->ws->minv, w->O, map};
+}
+
+// G into the half matrices lo / up for the context's K (view 0: rows of K;
+// view 1: the reordered rows of the two-level methods, through perm)
+static int build_factor(ofea_ctx *c, int view, unsigned short *lo, unsigned short *up)
+{
+    ofea_solver_ws *s = c->ws;
+    const int N = lws(c)->nfree;
+    int rc = fac_prepare(c, view);
+    if (rc) return rc;
+    const FacMat &f = s->fac[view];
+    if (view == 1)
+        hipLaunchKernelGGL(k_fac_build, dim3(grid_for(N)), dim3(LN_WG), 0, c->stream, N, s->nb_p, s->bcol_p, s->val_p,
+                           fac_nodes(c, s->perm), f.lo_ptr, f.up_ptr, lo, up);
+    else
+        hipLaunchKernelGGL(k_fac_build, dim3(grid_for(N)), dim3(LN_WG), 0, c->stream, N, c->d_node_blk, c->d_bcol,
+                           c->d_val, fac_nodes(c, nullptr), f.lo_ptr, f.up_ptr, lo, up);
     OFEA_HIP(hipGetLastError());
     return OFEA_OK;
 }
@@ -538,21 +714,51 @@ static void wgrad(ofea_ctx *c, int Cout, int Cin, const float *dZ, const float *
                        Cout, Cin, w->partial, gW, gb);
 }
 
-// loss and gradient at the device parameters for the resident samples
+// Buffers of the loss for the resident pattern (view 0 of the factor's index
+// arrays): rows with more than KAP_MAXP lower blocks, or K rows that are not
+// sorted, are refused.
+static int kap_prepare(ofea_ctx *c)
+{
+    LearnedWS *w = lws(c);
+    if (w->Trow) return OFEA_OK;
+    int rc = fac_prepare(c, 0);
+    if (rc) return rc;
+    const int N = w->nfree;
+    const std::vector<int> &nb = c->P.node_blk, &bc = c->P.bcol;
+    for (int i = 0; i < N; ++i) {
+        int low = 0;
+        for (int q = nb[i]; q < nb[i + 1]; ++q) {
+            if (q > nb[i] && bc[q] <= bc[q - 1]) return fail(OFEA_ERR_STATE, "learned loss: unsorted pattern row");
+            low += bc[q] <= i;
+        }
+        if (low > KAP_MAXP) return fail(OFEA_ERR_LIMIT, "learned loss: a node has more than 64 lower neighbours");
+    }
+    const FacMat &f = c->ws->fac[0];
+    OFEA_HIP(hipMalloc((void **)&w->Trow, sizeof(double) * std::max(N, 1)));
+    OFEA_HIP(hipMalloc((void **)&w->dA, sizeof(double) * (size_t)std::max(f.nlo, 1LL)));
+    OFEA_HIP(hipMalloc((void **)&w->tpart, sizeof(double) * 64));
+    return OFEA_OK;
+}
+
+// loss tr(G K G^T) / n and its gradient (into d_grad) at the device parameters
 static int ln_loss_grad(ofea_ctx *c, double *loss)
 {
     LearnedWS *w = lws(c);
-    const int N = w->nfree, S = w->S, n = 3 * N;
-    int rc = ln_forward(c, false);
+    const int N = w->nfree;
+    int rc = kap_prepare(c);
     if (rc) return rc;
-    ofea_solver_ws *s = c->ws;
-    hipLaunchKernelGGL(k_ln_residual, dim3(grid_for((long long)N * S)), dim3(LN_WG), 0, c->stream, N, S, s->learned64,
-                       w->E, w->R, w->Dv);
-    for (int k = 0; k < S; ++k)
-        if ((rc = solver_spmv(c, w->Dv + (size_t)k * n, w->Q + (size_t)k * n))) return rc;
-    hipLaunchKernelGGL(k_ln_lossgrad, dim3(grid_for(N)), dim3(LN_WG), 0, c->stream, N, S, w->Dv, w->Q, w->R, w->wsmp,
-                       w->G, w->loss_part);
-    hipLaunchKernelGGL(k_ln_head_back, dim3(grid_for(N)), dim3(LN_WG), 0, c->stream, N, w->G, w->Rinv, w->O, w->dO);
+    if ((rc = ln_forward(c, false))) return rc;
+    const FacMat &f = c->ws->fac[0];
+    const double inv_n = 1.0 / (3.0 * std::max(N, 1));
+    const FacNodes nd = fac_nodes(c, nullptr);
+    hipLaunchKernelGGL(k_kap_rows, dim3((N + LN_WG / 64 - 1) / (LN_WG / 64)), dim3(LN_WG), 0, c->stream, N,
+                       c->d_node_blk, c->d_bcol, c->d_val, nd, f.lo_ptr, inv_n, w->Trow, w->dA);
+    const int nbs = std::max(1, std::min(64, (int)grid_for(N)));
+    hipLaunchKernelGGL(k_kap_sum, dim3(nbs), dim3(LN_WG), 0, c->stream, N, w->Trow, w->tpart);
+    hipLaunchKernelGGL(k_kap_dO, dim3(grid_for(N)), dim3(LN_WG), 0, c->stream, N, f.lo_ptr, f.lo_col, f.up_ptr,
+                       f.up_col, nd, w->dA, w->dO);
+    double tp[64];
+    OFEA_HIP(hipMemcpyAsync(tp, w->tpart, sizeof(double) * nbs, hipMemcpyDeviceToHost, c->stream));
     float *g = w->d_grad;
     const float *P = w->d_params;
     // head: dWo, dbo; dH2 -> dZ2 (relu mask)
@@ -574,15 +780,11 @@ static int ln_loss_grad(ofea_ctx *c, double *loss)
     wgrad(c, LN_H, LN_F, w->dH1, w->X, g + OFF_W1S, g + OFF_B1);
     wgrad(c, LN_H, LN_F, w->dH1, w->A0, g + OFF_W1N, nullptr);
     OFEA_HIP(hipGetLastError());
-    if (loss) {
-        std::vector<double> lp(std::max(w->npartial_rows, 1));
-        OFEA_HIP(hipMemcpyAsync(lp.data(), w->loss_part, sizeof(double) * w->npartial_rows, hipMemcpyDeviceToHost,
-                                c->stream));
-        OFEA_HIP(hipStreamSynchronize(c->stream));
-        double t = 0.0;
-        for (double v : lp) t += v;
-        *loss = t;
-    }
+    OFEA_HIP(hipStreamSynchronize(c->stream));
+    double L = 0.0;
+    for (int k = 0; k < nbs; ++k) L += tp[k];
+    if (!std::isfinite(L)) return fail(OFEA_ERR_STATE, "learned loss is not finite");
+    if (loss) *loss = L * inv_n;
     return OFEA_OK;
 }
 
@@ -605,7 +807,8 @@ static void init_params(std::vector<float> &p, unsigned long long seed)
     const double a1 = std::sqrt(3.0 / (2 * LN_F)), a2 = std::sqrt(3.0 / (2 * LN_H));
     for (int i = 0; i < 2 * LN_H * LN_F; ++i) p[OFF_W1S + i] = (float)(a1 * uni());
     for (int i = 0; i < 2 * LN_H * LN_H; ++i) p[OFF_W2S + i] = (float)(a2 * uni());
-    // head (OFF_WO .. end) stays zero: L = I, block Jacobi
+    // head zero: alpha = 2 sigmoid(0) = 1 at every pair, the first-order rule
+    p[OFF_BO] = LN_HEAD_BIAS0;
 }
 
 int learned_init(ofea_ctx *c, unsigned long long seed)
@@ -637,45 +840,10 @@ int learned_get_params(ofea_ctx *c, float *params, int nparam)
 
 int learned_nparam() { return LN_NPARAM; }
 
-// samples: E, R host arrays (S x n, sample-major); R = K E is not recomputed
-int learned_set_samples(ofea_ctx *c, const double *E, const double *R, int S)
-{
-    int rc = ln_prepare(c);
-    if (rc) return rc;
-    LearnedWS *w = lws(c);
-    const size_t n = 3 * (size_t)w->nfree;
-    // the buffers below are freed and refilled by blocking copies on the null
-    // stream: kernels of the context's stream must not still read them
-    OFEA_HIP(hipStreamSynchronize(c->stream));
-    for (double **p : {&w->E, &w->R, &w->Dv, &w->Q, &w->wsmp})
-        if (*p) {
-            (void)hipFree(*p);
-            *p = nullptr;
-        }
-    w->S = S;
-    if (S <= 0) return fail(OFEA_ERR_ARG, "need at least one sample");
-    OFEA_HIP(hipMalloc((void **)&w->E, sizeof(double) * n * S));
-    OFEA_HIP(hipMalloc((void **)&w->R, sizeof(double) * n * S));
-    OFEA_HIP(hipMalloc((void **)&w->Dv, sizeof(double) * n * S));
-    OFEA_HIP(hipMalloc((void **)&w->Q, sizeof(double) * n * S));
-    OFEA_HIP(hipMalloc((void **)&w->wsmp, sizeof(double) * S));
-    std::vector<double> wk(S);
-    for (int k = 0; k < S; ++k) {
-        double er = 0.0;
-        for (size_t i = 0; i < n; ++i) er += E[k * n + i] * R[k * n + i];
-        if (!(er > 0.0)) return fail(OFEA_ERR_ARG, "sample with e^T r <= 0 (r must be K e)");
-        wk[k] = 1.0 / (S * er);
-    }
-    OFEA_HIP(hipMemcpy(w->E, E, sizeof(double) * n * S, hipMemcpyHostToDevice));
-    OFEA_HIP(hipMemcpy(w->R, R, sizeof(double) * n * S, hipMemcpyHostToDevice));
-    OFEA_HIP(hipMemcpy(w->wsmp, wk.data(), sizeof(double) * S, hipMemcpyHostToDevice));
-    return OFEA_OK;
-}
-
 int learned_loss_grad(ofea_ctx *c, double *loss, float *grad)
 {
     LearnedWS *w = lws(c);
-    if (!w || w->S == 0) return fail(OFEA_ERR_STATE, "no training samples (ofea_learned_set_samples)");
+    if (!w) return fail(OFEA_ERR_STATE, "learned preconditioner not initialised");
     int rc = ln_loss_grad(c, loss);
     if (rc) return rc;
     if (grad) {
@@ -685,44 +853,120 @@ int learned_loss_grad(ofea_ctx *c, double *loss, float *grad)
     return OFEA_OK;
 }
 
-// Adam on the host over the tiny parameter vector; gradients from the device.
-int learned_train(ofea_ctx *c, int steps, float lr, double *loss_hist)
+// Adam on the host over the tiny parameter vector, on the mean loss of the
+// contexts' resident systems; gradients from each context's device.  The
+// parameters start from the first context's and end in every context.
+int learned_train(ofea_ctx *const *cs, int nctx, int steps, float lr, double *loss_hist)
 {
-    LearnedWS *w = lws(c);
-    if (!w || w->S == 0) return fail(OFEA_ERR_STATE, "no training samples (ofea_learned_set_samples)");
-    std::vector<float> g(LN_NPARAM), m(LN_NPARAM, 0.0f), v(LN_NPARAM, 0.0f);
+    for (int k = 0; k < nctx; ++k)
+        if (!lws(cs[k])) return fail(OFEA_ERR_STATE, "learned preconditioner not initialised (ofea_learned_init)");
+    std::vector<float> p = lws(cs[0])->params, g(LN_NPARAM), gs(LN_NPARAM), m(LN_NPARAM, 0.0f), v(LN_NPARAM, 0.0f);
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    auto put = [&](ofea_ctx *c, bool infer) -> int {
+        std::memcpy(lws(c)->params.data(), p.data(), sizeof(float) * LN_NPARAM);
+        int rc = upload_params(c);
+        if (!rc && infer) rc = ln_forward(c, true);
+        return rc;
+    };
     for (int t = 1; t <= steps; ++t) {
         double loss = 0.0;
-        int rc = learned_loss_grad(c, &loss, g.data());
-        if (rc) return rc;
+        std::fill(gs.begin(), gs.end(), 0.0f);
+        for (int k = 0; k < nctx; ++k) {
+            double lk = 0.0;
+            int rc = put(cs[k], false);
+            if (!rc) rc = learned_loss_grad(cs[k], &lk, g.data());
+            if (rc) return rc;
+            loss += lk / nctx;
+            for (int i = 0; i < LN_NPARAM; ++i) gs[i] += g[i] / nctx;
+        }
         if (loss_hist) loss_hist[t - 1] = loss;
         const float c1 = 1.0f - std::pow(b1, (float)t), c2 = 1.0f - std::pow(b2, (float)t);
         for (int i = 0; i < LN_NPARAM; ++i) {
-            if (!std::isfinite(g[i])) return fail(OFEA_ERR_STATE, "non-finite gradient during training");
-            m[i] = b1 * m[i] + (1.0f - b1) * g[i];
-            v[i] = b2 * v[i] + (1.0f - b2) * g[i] * g[i];
-            w->params[i] -= lr * (m[i] / c1) / (std::sqrt(v[i] / c2) + eps);
+            if (!std::isfinite(gs[i])) return fail(OFEA_ERR_STATE, "non-finite gradient during training");
+            m[i] = b1 * m[i] + (1.0f - b1) * gs[i];
+            v[i] = b2 * v[i] + (1.0f - b2) * gs[i] * gs[i];
+            p[i] -= lr * (m[i] / c1) / (std::sqrt(v[i] / c2) + eps);
         }
-        if ((rc = upload_params(c))) return rc;
     }
-    int rc = ln_forward(c, true);
-    if (rc) return rc;
+    for (int k = 0; k < nctx; ++k) {
+        int rc = put(cs[k], true);
+        if (rc) return rc;
+        OFEA_HIP(hipStreamSynchronize(cs[k]->stream));
+    }
+    return OFEA_OK;
+}
+
+// the factor G on K's node-block layout (nnz values, FP64 copies of the BF16
+// entries the solver applies; per block slot k of row i a row-major 3x3:
+// G_is for s < i, G_ii, (G_si)^T for s > i) and the node features
+int learned_blocks(ofea_ctx *c, double *blocks, float *features)
+{
+    LearnedWS *w = lws(c);
+    if (!w || !c->ws) return fail(OFEA_ERR_STATE, "learned preconditioner not initialised");
+    const size_t N = w->nfree;
+    if (blocks) {
+        int rc = learned_factor(c, false);
+        if (rc) return rc;
+        const FacMat &f = c->ws->fac[0];
+        std::vector<unsigned short> lo(9 * (size_t)f.nlo), up(9 * (size_t)f.nup);
+        std::vector<int> lp(N + 1), upp(N + 1);
+        OFEA_HIP(hipMemcpyAsync(lo.data(), f.lo, sizeof(unsigned short) * lo.size(), hipMemcpyDeviceToHost, c->stream));
+        OFEA_HIP(hipMemcpyAsync(up.data(), f.up, sizeof(unsigned short) * up.size(), hipMemcpyDeviceToHost, c->stream));
+        OFEA_HIP(hipMemcpyAsync(lp.data(), f.lo_ptr, sizeof(int) * (N + 1), hipMemcpyDeviceToHost, c->stream));
+        OFEA_HIP(hipMemcpyAsync(upp.data(), f.up_ptr, sizeof(int) * (N + 1), hipMemcpyDeviceToHost, c->stream));
+        OFEA_HIP(hipStreamSynchronize(c->stream));
+        const std::vector<int> &nb = c->P.node_blk, &bc = c->P.bcol;
+        for (size_t i = 0; i < N; ++i) {
+            const int nl = lp[i + 1] - lp[i], nu = upp[i + 1] - upp[i];
+            int kl = 0, ku = 0;
+            for (int q = nb[i]; q < nb[i + 1]; ++q) {
+                const size_t s = (size_t)bc[q];
+                // the s <= i half holds G_is (G_ii on the diagonal), the s > i half (G_si)^T
+                const bool low = s <= i;
+                const unsigned short *row = low ? &lo[9 * (size_t)lp[i]] : &up[9 * (size_t)upp[i]];
+                const int nbk = low ? nl : nu, kk = low ? kl++ : ku;
+                if (s >= i) ++ku;  // the diagonal also has a slot in the upper half
+                for (int cc = 0; cc < 3; ++cc)
+                    for (int d = 0; d < 3; ++d) {
+                        const unsigned bits = (unsigned)row[3 * nbk * cc + 3 * kk + d] << 16;
+                        float fv;
+                        std::memcpy(&fv, &bits, sizeof fv);
+                        blocks[9 * (size_t)q + 3 * cc + d] = fv;
+                    }
+            }
+        }
+    }
+    if (features) {
+        OFEA_HIP(hipMemcpyAsync(features, w->X, sizeof(float) * LN_F * N, hipMemcpyDeviceToHost, c->stream));
+        OFEA_HIP(hipStreamSynchronize(c->stream));
+    }
+    return OFEA_OK;
+}
+
+// the network's outputs (u, v) per free node, as they serve the solver
+int learned_outputs(ofea_ctx *c, float *uv)
+{
+    LearnedWS *w = lws(c);
+    if (!w || w->gen == 0) return fail(OFEA_ERR_STATE, "learned preconditioner not set (ofea_learned_*)");
+    OFEA_HIP(hipMemcpyAsync(uv, w->O, sizeof(float) * LN_O * (size_t)w->nfree, hipMemcpyDeviceToHost, c->stream));
     OFEA_HIP(hipStreamSynchronize(c->stream));
     return OFEA_OK;
 }
 
-// current preconditioner blocks (FP64, 9 per free node) and node features
-int learned_blocks(ofea_ctx *c, double *blocks, float *features)
+// the solver's G for the resident K (rebuilt when K or the outputs changed)
+int learned_factor(ofea_ctx *c, bool permuted)
 {
     LearnedWS *w = lws(c);
-    if (!w || !c->ws || !c->ws->learned64) return fail(OFEA_ERR_STATE, "learned preconditioner not initialised");
-    const size_t N = w->nfree;
-    if (blocks)
-        OFEA_HIP(hipMemcpyAsync(blocks, c->ws->learned64, sizeof(double) * 9 * N, hipMemcpyDeviceToHost, c->stream));
-    if (features) OFEA_HIP(hipMemcpyAsync(features, w->X, sizeof(float) * LN_F * N, hipMemcpyDeviceToHost, c->stream));
-    OFEA_HIP(hipStreamSynchronize(c->stream));
-    return OFEA_OK;
+    if (!w || w->gen == 0) return fail(OFEA_ERR_STATE, "learned preconditioner not set (ofea_learned_*)");
+    const int view = permuted ? 1 : 0;
+    int rc = fac_prepare(c, view);
+    if (rc) return rc;
+    FacMat &f = c->ws->fac[view];
+    if (f.kver == c->kver && f.gen == w->gen) return OFEA_OK;
+    if ((rc = build_factor(c, view, f.lo, f.up))) return rc;
+    f.kver = c->kver;
+    f.gen = w->gen;
+    return fac_tune(c, view);
 }
 
 int learned_precision(ofea_ctx *c, int precision)

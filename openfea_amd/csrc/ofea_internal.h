@@ -242,12 +242,21 @@ int learned_nparam();
 int learned_init(ofea_ctx *c, unsigned long long seed);
 int learned_set_params(ofea_ctx *c, const float *params, int nparam);
 int learned_get_params(ofea_ctx *c, float *params, int nparam);
-int learned_set_samples(ofea_ctx *c, const double *E, const double *R, int S);
 int learned_loss_grad(ofea_ctx *c, double *loss, float *grad);
-int learned_train(ofea_ctx *c, int steps, float lr, double *loss_hist);
+int learned_train(ofea_ctx *const *cs, int nctx, int steps, float lr, double *loss_hist);
 int learned_blocks(ofea_ctx *c, double *blocks, float *features);
+int learned_outputs(ofea_ctx *c, float *uv);
 int learned_refresh(ofea_ctx *c);
 int learned_precision(ofea_ctx *c, int precision);
+// G of the learned factorized correction for the resident K (reordered rows
+// of the two-level methods when permuted), rebuilt when K or the network's
+// outputs changed (learned.hip)
+int learned_factor(ofea_ctx *c, bool permuted);
+// index arrays of the factor's half matrices for view 0 (K's rows) or 1 (the
+// two-level order) and the iteration's y vectors (solver.hip)
+int fac_prepare(ofea_ctx *ctx, int view);
+// lanes per node of the factor passes of a view, timed once its values are built (solver.hip)
+int fac_tune(ofea_ctx *ctx, int view);
 // nonlinear (nl.hip)
 int nl_assemble(ofea_ctx *ctx, double E, double nu, const double *u, double *nz_val, double *f_int);
 int nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr, double tol, int max_newton, int method,

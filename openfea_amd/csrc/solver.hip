@@ -451,9 +451,11 @@ __device__ inline void apply_pc(int r, const T *rr, T *zz, const double *dinv, c
 {
     if (PC == 0) {
         for (int c = 0; c < 3; ++c) zz[c] = (T)((double)rr[c] * dinv[3 * r + c]);
-    } else {
+    } else if (PC == 1) {
         const T *m = minv + 9 * r;
         for (int c = 0; c < 3; ++c) zz[c] = m[3 * c] * rr[0] + m[3 * c + 1] * rr[1] + m[3 * c + 2] * rr[2];
+    } else {  // factorized correction: applied by the k_fac passes
+        for (int c = 0; c < 3; ++c) zz[c] = T(0);
     }
 }
 
@@ -499,25 +501,92 @@ __device__ inline void add_zt_r(const CoarseArgs &C, int i, const T rv[3], doubl
     g[5] += dx * r1 - dy * r0;
 }
 
+// The learned factor G of M = G^T G as two half block-matrices (rows of G
+// and rows of G^T) in the column-of-rows layout of K's values: row i's nb
+// blocks start at 9 ptr[i], entry (c, 3 k + d) at 9 ptr[i] + 3 nb c + 3 k + d.
+struct FacView {
+    const int *lo_ptr = nullptr, *lo_col = nullptr, *up_ptr = nullptr, *up_col = nullptr;
+    const unsigned short *lo = nullptr, *up = nullptr;  // BF16
+};
+
+__device__ inline double bf16_val(unsigned short h) { return (double)__uint_as_float((unsigned)h << 16); }
+
+// Row i of a half matrix times x, by the GL lanes of an aligned group (lane j
+// takes columns j, j + GL, ... of the row's 3 nb columns); every lane of the
+// group returns the three sums.  FU columns per lane and round: the column
+// indices, then the gathered x values, then the products, so that the loads
+// of a round are in flight together.
+template <class T, int GL>
+__device__ inline void fac_row(int i, int lane, const int *__restrict__ ptr, const int *__restrict__ col,
+                               const unsigned short *__restrict__ v, const T *__restrict__ x, double &a0, double &a1,
+                               double &a2)
+{
+    constexpr int FU = 4;
+    const int b0 = ptr[i], len = 3 * (ptr[i + 1] - b0);
+    const unsigned short *v0 = v + 9 * (size_t)b0;
+    a0 = a1 = a2 = 0.0;
+    for (int j0 = lane; j0 < len; j0 += FU * GL) {
+        int sx[FU];
+        double xv[FU];
+#pragma unroll
+        for (int u = 0; u < FU; ++u) {
+            const int j = j0 + u * GL, kk = j / 3;
+            sx[u] = j < len ? 3 * col[b0 + kk] + (j - 3 * kk) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < FU; ++u) xv[u] = sx[u] >= 0 ? (double)x[sx[u]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < FU; ++u) {
+            const int j = j0 + u * GL;
+            if (j < len) {
+                a0 += bf16_val(v0[j]) * xv[u];
+                a1 += bf16_val(v0[len + j]) * xv[u];
+                a2 += bf16_val(v0[2 * len + j]) * xv[u];
+            }
+        }
+    }
+    a0 = group_sum<GL>(a0);
+    a1 = group_sum<GL>(a1);
+    a2 = group_sum<GL>(a2);
+}
+
 // The update kernels run 512-thread workgroups on the CG grid: a node range
 // rarely exceeds 512 nodes, and the smaller workgroup keeps the whole grid
 // resident at up to 80 VGPRs.
 constexpr int UP_WG = 512;
 
+// PC 2 (factorized correction M = G^T G): the kernel also advances y = G r by
+// y -= alpha w, w = G q (q is complete after the SpMV kernel): w of each chunk
+// of up to UP_WG nodes of the range is gathered into LDS by groups of FG
+// lanes, before the reduction for the first chunk.  G^T y follows in k_fac_upper.
 template <class T, class XT, int PC, bool CO>
 __global__ __launch_bounds__(UP_WG) void k_cg_update(int bi, int nfree, XT *__restrict__ x, T *__restrict__ r,
                                                      T *__restrict__ z, const T *__restrict__ pA,
                                                      const T *__restrict__ pB, const T *__restrict__ q,
                                                      const double *__restrict__ dinv, const T *__restrict__ minv,
-                                                     CGParts P, CoarseArgs C, DevScalars *sc)
+                                                     T *__restrict__ gy, FacView F, CGParts P, CoarseArgs C,
+                                                     DevScalars *sc)
 {
     const int done0 = sc->done, par = sc->par0 ^ (bi & 1);
     const T *p = par ? pA : pB;  // p_new of this iteration
     int i0, i1;
     cg_range(nfree, i0, i1);
+    constexpr int FG = 8;
+    __shared__ double s_w[PC == 2 ? 3 : 1][PC == 2 ? UP_WG : 1];
+    auto fac_gather = [&](int c0) {  // w = G q for nodes c0 .. c0 + UP_WG - 1 of the range
+        if constexpr (PC == 2) {
+            const int lane = threadIdx.x % FG, cnt = min(UP_WG, i1 - c0);
+            for (int nn = (int)threadIdx.x / FG; nn < cnt; nn += UP_WG / FG) {
+                double a0, a1, a2;
+                fac_row<T, FG>(c0 + nn, lane, F.lo_ptr, F.lo_col, F.lo, q, a0, a1, a2);
+                if (lane < 3) s_w[lane][nn] = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
+            }
+        }
+    };
+    if (!done0) fac_gather(i0);
     // the node data do not depend on alpha: issue all loads before the reduction
     int i = i0 + (int)threadIdx.x;
-    T pv[3], qv[3], rv[3], mv[PC == 0 ? 3 : 9];
+    T pv[3], qv[3], rv[3], mv[PC == 0 ? 3 : 9], gyv[3];
     XT xv[3];
     float dv[3];
     auto load_node = [&](int ii) {
@@ -528,10 +597,13 @@ __global__ __launch_bounds__(UP_WG) void k_cg_update(int bi, int nfree, XT *__re
             rv[c] = r[3 * ii + c];
             xv[c] = x[3 * ii + c];
         }
+        if constexpr (PC == 2)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) gyv[c] = gy[3 * ii + c];
         if constexpr (PC == 0) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) mv[c] = (T)dinv[3 * ii + c];
-        } else {
+        } else if constexpr (PC == 1) {
 #pragma unroll
             for (int e = 0; e < 9; ++e) mv[e] = minv[9 * (size_t)ii + e];
         }
@@ -632,8 +704,18 @@ __global__ __launch_bounds__(UP_WG) void k_cg_update(int bi, int nfree, XT *__re
         }
     }
     double rzn = 0.0, rr2 = 0.0;
-    for (; i < i1; i += UP_WG) {
-        if (i >= i0 + UP_WG) load_node(i);  // ranges longer than one workgroup
+    // chunks of UP_WG nodes (ranges longer than one workgroup take several);
+    // every thread runs every chunk, for the barriers around the PC 2 gather
+    for (int c0 = i0; c0 < i1; c0 += UP_WG, i += UP_WG) {
+        if (c0 > i0) {
+            if constexpr (PC == 2) {
+                __syncthreads();  // s_w of the previous chunk has been read
+                fac_gather(c0);
+                __syncthreads();
+            }
+            if (i < i1) load_node(i);
+        }
+        if (i >= i1) continue;
         T zv[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -645,9 +727,15 @@ __global__ __launch_bounds__(UP_WG) void k_cg_update(int bi, int nfree, XT *__re
         if constexpr (PC == 0) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) zv[c] = rv[c] * mv[c];
-        } else {
+        } else if constexpr (PC == 1) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) zv[c] = mv[3 * c] * rv[0] + mv[3 * c + 1] * rv[1] + mv[3 * c + 2] * rv[2];
+        } else {  // factorized correction: y -= alpha G q here, z = G^T y in k_fac_upper
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                gy[3 * i + c] = (T)((double)gyv[c] - alpha * s_w[c][i - c0]);
+                zv[c] = T(0);
+            }
         }
         if constexpr (CO) {  // z_i += (Z y)_i = t + theta x d_i of this workgroup's aggregate
             const double d0 = dv[0], d1 = dv[1], d2 = dv[2];
@@ -658,7 +746,7 @@ __global__ __launch_bounds__(UP_WG) void k_cg_update(int bi, int nfree, XT *__re
         }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            z[3 * i + c] = zv[c];
+            if (PC != 2 || CO) z[3 * i + c] = zv[c];
             rzn += (double)rv[c] * (double)zv[c];
             rr2 += (double)rv[c] * (double)rv[c];
         }
@@ -670,7 +758,7 @@ __global__ __launch_bounds__(UP_WG) void k_cg_update(int bi, int nfree, XT *__re
     __syncthreads();
     if (w < 2) {
         const double t = col_sum<UP_WG>(s_col[w]);
-        if (threadIdx.x == 0) P.rz[(par ^ 1) * CG_NB + blockIdx.x] = t;
+        if (threadIdx.x == 0 && PC != 2) P.rz[(par ^ 1) * CG_NB + blockIdx.x] = t;  // PC 2: k_fac_upper
         if (threadIdx.x == 64) P.rr[blockIdx.x] = t;
     }
 }
@@ -777,14 +865,74 @@ __global__ __launch_bounds__(UP_WG) void k_replace(int nfree, const double *__re
         apply_pc<T, PC>(i, rv, zv, dinv, minv);
         for (int c = 0; c < 3; ++c) {
             r[3 * i + c] = rv[c];
-            z[3 * i + c] = zv[c];
+            if (PC != 2) z[3 * i + c] = zv[c];
             rz += (double)rv[c] * (double)zv[c];
             rr2 += (double)rv[c] * (double)rv[c];
         }
         if constexpr (CO) add_zt_r(C, i, rv, g);
     }
-    put_rz_rr_g<CO, UP_WG>(rz, rr2, g, P.rz + par * CG_NB, P.rr, C);
+    // PC 2: the <r, z> partials of this slot come from k_fac_upper (a scratch slot here)
+    put_rz_rr_g<CO, UP_WG>(rz, rr2, g, PC == 2 ? P.pq : P.rz + par * CG_NB, P.rr, C);
     if (restart && threadIdx.x == 0) P.rz[(par ^ 1) * CG_NB + blockIdx.x] = blockIdx.x == 0 ? INFINITY : 0.0;
+}
+
+// ---------------------------------------------------------------- factorized correction
+// k_fac_lower: y = G r (rows of the lower half matrix), at starts and residual
+// replacements; inside the iteration y follows y -= alpha G q in the update
+// kernel.  k_fac_upper: z (+)= G^T y (rows of the upper half matrix) and the
+// per-workgroup partial <r, z> into the CG slot the update kernel would have
+// written.  Both run on the CG grid with GL lanes per node and return at once
+// after convergence.
+template <class T, int GL>
+__global__ __launch_bounds__(CG_WG) void k_fac_lower(int nfree, FacView F, const T *__restrict__ r,
+                                                     T *__restrict__ y, DevScalars *sc)
+{
+    if (sc && sc->done) return;
+    constexpr int NPW = CG_WG / GL;
+    const int lane = threadIdx.x % GL;
+    int r0, r1;
+    cg_range(nfree, r0, r1);
+    for (int i = r0 + (int)threadIdx.x / GL; i < r1; i += NPW) {
+        double a0, a1, a2;
+        fac_row<T, GL>(i, lane, F.lo_ptr, F.lo_col, F.lo, r, a0, a1, a2);
+        if (lane < 3) y[3 * i + lane] = (T)(lane == 0 ? a0 : (lane == 1 ? a1 : a2));
+    }
+}
+
+// slot of the <r, z> partials: mode 0 after an update (iteration bi: the slot
+// the update kernel writes, par ^ 1), mode 1 after k_replace (par0); mode 2
+// writes no partials (training passes)
+template <class T, int GL>
+__global__ __launch_bounds__(CG_WG) void k_fac_upper(int bi, int mode, int add, int nfree, FacView F,
+                                                     const T *__restrict__ y, const T *__restrict__ r,
+                                                     T *__restrict__ z, CGParts P, DevScalars *sc)
+{
+    if (sc && sc->done) return;
+    constexpr int NPW = CG_WG / GL;
+    const int lane = threadIdx.x % GL;
+    int r0, r1;
+    cg_range(nfree, r0, r1);
+    double rz = 0.0;
+    for (int i = r0 + (int)threadIdx.x / GL; i < r1; i += NPW) {
+        double a0, a1, a2;
+        fac_row<T, GL>(i, lane, F.up_ptr, F.up_col, F.up, y, a0, a1, a2);
+        if (lane < 3) {
+            const int k = 3 * i + lane;
+            double zv = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
+            if (add) zv += (double)z[k];
+            const T zt = (T)zv;
+            z[k] = zt;
+            rz += (double)r[k] * (double)zt;
+        }
+    }
+    if (mode == 2) return;
+    double v[1] = {rz};
+    cg_block_sum<1>(v);
+    if (threadIdx.x == 0) {
+        const int par = sc->par0 ^ (bi & 1);
+        const int slot = mode == 0 ? (par ^ 1) : sc->par0;
+        P.rz[slot * CG_NB + blockIdx.x] = v[0];
+    }
 }
 
 // r64 = b - y, <r64,r64> -> S_TRR, <b,b> -> S_BB
@@ -932,14 +1080,17 @@ void solver_free(ofea_ctx *ctx)
     for (double *p : {w->part_rz, w->part_rr, w->part_pq, w->partials, w->v64.r, w->v64.z, w->v64.p, w->v64.p2, w->v64.q, w->r64, w->y64, w->dinv, w->minv,
                       w->dblk})
         if (p) (void)hipFree(p);
-    for (float *p : {w->v32.r, w->v32.z, w->v32.p, w->v32.p2, w->v32.q, w->minv32, w->val32, w->learned32})
+    for (float *p : {w->v32.r, w->v32.z, w->v32.p, w->v32.p2, w->v32.q, w->v32.y, w->minv32, w->val32})
         if (p) (void)hipFree(p);
-    if (w->learned64) (void)hipFree(w->learned64);
+    if (w->v64.y) (void)hipFree(w->v64.y);
+    for (FacMat &f : w->fac)
+        for (void *p : {(void *)f.lo_ptr, (void *)f.lo_col, (void *)f.up_ptr, (void *)f.up_col, (void *)f.lo, (void *)f.up})
+            if (p) (void)hipFree(p);
     for (void *p : {(void *)w->co.g, (void *)w->co.d, (void *)w->co_ainv, (void *)w->co.zq, (void *)w->co.ya})
         if (p) (void)hipFree(p);
     for (void *p : {(void *)w->perm, (void *)w->iperm, (void *)w->nb_p, (void *)w->bcol_p, (void *)w->val_p,
                     (void *)w->val32_p, (void *)w->dinv_p, (void *)w->minv_p, (void *)w->dblk_p,
-                    (void *)w->learned64_p, (void *)w->minv32_p, (void *)w->b_p, (void *)w->x_p, (void *)w->co_A,
+                    (void *)w->minv32_p, (void *)w->b_p, (void *)w->x_p, (void *)w->co_A,
                     (void *)w->co_inv, (void *)w->co_flag, (void *)w->sl_ptr, (void *)w->sl_part, (void *)w->sl_cptr,
                     (void *)w->ch_ptr, (void *)w->ch_beg, (void *)w->sl_blk})
         if (p) (void)hipFree(p);
@@ -1035,17 +1186,142 @@ struct SysView {
     const double *dinv;   // point-Jacobi inverse diagonal
     const double *b;      // right-hand side
     double *x;            // solution
+    FacView fac;          // learned factor G on this pattern (PC 2)
+    int fac_group;        // lanes per node of its passes
 };
+
+static FacView fac_view(const FacMat &f)
+{
+    return {f.lo_ptr, f.lo_col, f.up_ptr, f.up_col, f.lo, f.up};
+}
 
 static SysView ctx_view(ofea_ctx *ctx)
 {
-    return {ctx->d_node_blk, ctx->d_bcol, ctx->d_val, ctx->ws->dinv, ctx->d_rhs, ctx->d_x};
+    ofea_solver_ws *w = ctx->ws;
+    return {ctx->d_node_blk, ctx->d_bcol, ctx->d_val, w->dinv,          ctx->d_rhs,
+            ctx->d_x,        fac_view(w->fac[0]), w->fac[0].group ? w->fac[0].group : 8};
 }
 
 static SysView perm_view(ofea_ctx *ctx)
 {
     ofea_solver_ws *w = ctx->ws;
-    return {w->nb_p, w->bcol_p, w->val_p, w->dinv_p, w->b_p, w->x_p};
+    return {w->nb_p, w->bcol_p, w->val_p, w->dinv_p, w->b_p, w->x_p, fac_view(w->fac[1]),
+            w->fac[1].group ? w->fac[1].group : 8};
+}
+
+#define OFEA_GROUP_SWITCH(group, CASE) \
+    switch (group) {                   \
+        CASE(4)                        \
+        CASE(8)                        \
+        CASE(16)                       \
+        CASE(32)                       \
+        default:                       \
+        CASE(64)                       \
+    }
+
+// y = G r (k_fac_lower), z (+)= G^T y (k_fac_upper); mode as in k_fac_upper
+template <class T>
+static void launch_fac_lower(hipStream_t st, int group, int nfree, const FacView &F, const T *r, T *y,
+                             DevScalars *sc)
+{
+#define OFEA_FACL_CASE(GL)                                                                                 \
+    case GL:                                                                                               \
+        hipLaunchKernelGGL((k_fac_lower<T, GL>), dim3(CG_NB), dim3(CG_WG), 0, st, nfree, F, r, y, sc); \
+        break;
+    OFEA_GROUP_SWITCH(group, OFEA_FACL_CASE)
+#undef OFEA_FACL_CASE
+}
+
+template <class T>
+static void launch_fac_upper(hipStream_t st, int group, int bi, int mode, int add, int nfree, const FacView &F,
+                             const T *y, const T *r, T *z, CGParts P, DevScalars *sc)
+{
+#define OFEA_FACU_CASE(GL)                                                                                       \
+    case GL:                                                                                                     \
+        hipLaunchKernelGGL((k_fac_upper<T, GL>), dim3(CG_NB), dim3(CG_WG), 0, st, bi, mode, add, nfree, F, y, r, \
+                           z, P, sc);                                                                            \
+        break;
+    OFEA_GROUP_SWITCH(group, OFEA_FACU_CASE)
+#undef OFEA_FACU_CASE
+}
+
+// Index arrays of the learned factor's half matrices for view 0 (K's rows) or
+// 1 (the two-level order), from the host pattern; allocates the value arrays
+// and the y vectors of the iteration.  Built once per mesh.
+int fac_prepare(ofea_ctx *ctx, int view)
+{
+    ofea_solver_ws *w = ctx->ws;
+    FacMat &f = w->fac[view];
+    if (f.lo_ptr) return OFEA_OK;
+    const int nf = w->nfree;
+    const std::vector<int> &nb = ctx->P.node_blk;
+    const std::vector<int> &bc = ctx->P.bcol;
+    std::vector<int> perm;
+    if (view == 1) {  // row i of the two-level order is K's row perm[i], columns renumbered by iperm
+        perm.resize(nf);
+        for (int o = 0; o < nf; ++o) perm[w->iperm_host[o]] = o;
+    }
+    std::vector<int> lp(nf + 1, 0), up(nf + 1, 0), lc, uc;
+    lc.reserve((size_t)(nb[nf] + nf) / 2 + 1);
+    uc.reserve((size_t)(nb[nf] + nf) / 2 + 1);
+    for (int i = 0; i < nf; ++i) {
+        const int o = view == 1 ? perm[i] : i;
+        for (int q = nb[o]; q < nb[o + 1]; ++q) {
+            const int s = view == 1 ? w->iperm_host[bc[q]] : bc[q];
+            if (s <= i) lc.push_back(s);
+            if (s >= i) uc.push_back(s);
+        }
+        lp[i + 1] = (int)lc.size();
+        up[i + 1] = (int)uc.size();
+    }
+    f.nlo = (long long)lc.size();
+    f.nup = (long long)uc.size();
+    OFEA_HIP(hipMalloc((void **)&f.lo_ptr, sizeof(int) * (nf + 1)));
+    OFEA_HIP(hipMalloc((void **)&f.up_ptr, sizeof(int) * (nf + 1)));
+    OFEA_HIP(hipMalloc((void **)&f.lo_col, sizeof(int) * std::max<size_t>(lc.size(), 1)));
+    OFEA_HIP(hipMalloc((void **)&f.up_col, sizeof(int) * std::max<size_t>(uc.size(), 1)));
+    OFEA_HIP(hipMalloc((void **)&f.lo, sizeof(unsigned short) * 9 * std::max<size_t>(lc.size(), 1)));
+    OFEA_HIP(hipMalloc((void **)&f.up, sizeof(unsigned short) * 9 * std::max<size_t>(uc.size(), 1)));
+    OFEA_HIP(hipMemcpy(f.lo_ptr, lp.data(), sizeof(int) * (nf + 1), hipMemcpyHostToDevice));
+    OFEA_HIP(hipMemcpy(f.up_ptr, up.data(), sizeof(int) * (nf + 1), hipMemcpyHostToDevice));
+    if (!lc.empty()) OFEA_HIP(hipMemcpy(f.lo_col, lc.data(), sizeof(int) * lc.size(), hipMemcpyHostToDevice));
+    if (!uc.empty()) OFEA_HIP(hipMemcpy(f.up_col, uc.data(), sizeof(int) * uc.size(), hipMemcpyHostToDevice));
+    const size_t nbv = (size_t)std::max(w->n, 1);
+    if (!w->v64.y) OFEA_HIP(hipMalloc((void **)&w->v64.y, sizeof(double) * nbv));
+    if (!w->v32.y) OFEA_HIP(hipMalloc((void **)&w->v32.y, sizeof(float) * nbv));
+    f.kver = f.gen = -1;
+    return OFEA_OK;
+}
+
+// Lanes per node of the factor passes for view v: k_fac_upper timed for 4, 8
+// and 16 lanes on the built factor (y and z of the FP64 vector set as
+// scratch, before any solve uses them), the fastest kept.
+int fac_tune(ofea_ctx *ctx, int view)
+{
+    ofea_solver_ws *w = ctx->ws;
+    FacMat &f = w->fac[view];
+    if (f.group) return OFEA_OK;
+    const FacView F = fac_view(f);
+    const CGParts P{nullptr, nullptr, nullptr};
+    const unsigned gv = (unsigned)w->nblk_vec;
+    if (gv) hipLaunchKernelGGL((k_fill<double>), dim3(gv), dim3(RED_WG), 0, ctx->stream, w->n, w->v64.y, 1.0);
+    float best = -1.f;
+    for (int g : {4, 8, 16}) {
+        launch_fac_upper<double>(ctx->stream, g, 0, 2, 0, w->nfree, F, w->v64.y, w->v64.y, w->v64.z, P, nullptr);
+        OFEA_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+        for (int rep = 0; rep < 5; ++rep)
+            launch_fac_upper<double>(ctx->stream, g, 0, 2, 0, w->nfree, F, w->v64.y, w->v64.y, w->v64.z, P, nullptr);
+        OFEA_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+        OFEA_HIP(hipEventSynchronize(ctx->ev1));
+        OFEA_HIP(hipGetLastError());
+        float t = 0.f;
+        OFEA_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+        if (best < 0.f || t < best) {
+            best = t;
+            f.group = g;
+        }
+    }
+    return OFEA_OK;
 }
 
 int solver_spmv(ofea_ctx *ctx, const double *d_x, double *d_y)
@@ -1120,7 +1396,8 @@ int solver_prepare(ofea_ctx *ctx)
     return OFEA_OK;
 }
 
-// The graph of one batch: CG_BATCH iterations (2 kernels each) + batch end.
+// The graph of one batch: CG_BATCH iterations (2 kernels each, 3 with the
+// factorized correction) + batch end.
 template <class TV, class T, class XT, int PC, bool CO>
 static void cg_batch(ofea_ctx *ctx, const PcgBufs &w, const SysView &S, const TV *val, VecSet<T> &v, XT *x,
                      const T *minv)
@@ -1142,7 +1419,9 @@ static void cg_batch(ofea_ctx *ctx, const PcgBufs &w, const SysView &S, const TV
         }
 #undef OFEA_CG_CASE
         hipLaunchKernelGGL((k_cg_update<T, XT, PC, CO>), dim3(CG_NB), dim3(UP_WG), 0, ctx->stream, bi, w.nfree, x,
-                           v.r, v.z, v.p, v.p2, v.q, S.dinv, minv, P, w.co, w.sc);
+                           v.r, v.z, v.p, v.p2, v.q, S.dinv, minv, v.y, S.fac, P, w.co, w.sc);
+        if constexpr (PC == 2)
+            launch_fac_upper<T>(ctx->stream, S.fac_group, bi, 0, CO ? 1 : 0, w.nfree, S.fac, v.y, v.r, v.z, P, w.sc);
     }
     hipLaunchKernelGGL(k_cg_batch_end, dim3(1), dim3(64), 0, ctx->stream, w.sc, CG_BATCH);
 }
@@ -1213,6 +1492,10 @@ static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, Ve
     if ((rc = write_scalars(ctx))) return rc;
     hipLaunchKernelGGL((k_replace<T, PC, CO>), dim3(CG_NB), dim3(UP_WG), 0, ctx->stream, w->nfree, w->r64, v.r, v.z,
                        1, Sv.dinv, minv, P, w->co, w->sc);
+    if (PC == 2) {  // y = G r, z = G^T y and <r, z> of slot par0
+        launch_fac_lower<T>(ctx->stream, Sv.fac_group, w->nfree, Sv.fac, v.r, v.y, w->sc);
+        launch_fac_upper<T>(ctx->stream, Sv.fac_group, 0, 1, 0, w->nfree, Sv.fac, v.y, v.r, v.z, P, w->sc);
+    }
     if (CO)
         hipLaunchKernelGGL((k_coarse<T>), dim3(w->co_nagg), dim3(CG_WG), 0, ctx->stream, w->nfree, w->co_nagg,
                            w->co_ainv, v.r, v.z, w->part_rz, w->co, w->sc);
@@ -1276,6 +1559,10 @@ static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, Ve
         if ((rc = write_scalars(ctx))) return rc;
         hipLaunchKernelGGL((k_replace<T, PC, CO>), dim3(CG_NB), dim3(UP_WG), 0, ctx->stream, w->nfree, w->r64, v.r,
                            v.z, 0, Sv.dinv, minv, P, w->co, w->sc);
+        if (PC == 2) {
+            launch_fac_lower<T>(ctx->stream, Sv.fac_group, w->nfree, Sv.fac, v.r, v.y, w->sc);
+            launch_fac_upper<T>(ctx->stream, Sv.fac_group, 0, 1, 0, w->nfree, Sv.fac, v.y, v.r, v.z, P, w->sc);
+        }
         if (CO)
             hipLaunchKernelGGL((k_coarse<T>), dim3(w->co_nagg), dim3(CG_WG), 0, ctx->stream, w->nfree, w->co_nagg,
                                w->co_ainv, v.r, v.z, w->part_rz, w->co, w->sc);
@@ -1930,7 +2217,7 @@ static int coarse_structure(ofea_ctx *ctx)
         OFEA_HIP(hipMalloc((void **)&w->dinv_p, sizeof(double) * 3 * nfb));
         OFEA_HIP(hipMalloc((void **)&w->b_p, sizeof(double) * 3 * nfb));
         OFEA_HIP(hipMalloc((void **)&w->x_p, sizeof(double) * 3 * nfb));
-        for (double **p : {&w->minv_p, &w->dblk_p, &w->learned64_p})
+        for (double **p : {&w->minv_p, &w->dblk_p})
             OFEA_HIP(hipMalloc((void **)p, sizeof(double) * 9 * nfb));
         OFEA_HIP(hipMalloc((void **)&w->minv32_p, sizeof(float) * 9 * nfb));
         OFEA_HIP(hipMalloc((void **)&w->sl_ptr, sizeof(int) * (CG_NB / CG_AGG + 1)));
@@ -2098,30 +2385,38 @@ int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, 
     }
     const bool learned = method == OFEA_PCG_LEARNED || method == OFEA_PCG_LEARNED_MIXED ||
                          method == OFEA_PCG_LEARNED_TWO_LEVEL || method == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED;
-    if (learned && !w->learned64) return fail(OFEA_ERR_STATE, "learned preconditioner not set (ofea_learned_*)");
     if (method >= OFEA_PCG_TWO_LEVEL && method <= OFEA_PCG_LEARNED_TWO_LEVEL_MIXED) {
         // two-level: the reordered system (b in, x out through the permutation)
         if ((rc = coarse_prepare(ctx))) return rc;
+        if (learned && (rc = learned_factor(ctx, true))) return rc;  // G on the reordered rows
         const int nf = w->nfree;
-        const unsigned g3 = (unsigned)((3LL * nf + RED_WG - 1) / RED_WG), g9 = (unsigned)((9LL * nf + RED_WG - 1) / RED_WG);
+        const unsigned g3 = (unsigned)((3LL * nf + RED_WG - 1) / RED_WG);
         hipLaunchKernelGGL(k_gather_nodes<3>, dim3(g3), dim3(RED_WG), 0, ctx->stream, nf, w->perm, ctx->d_rhs, w->b_p);
-        if (learned)  // the learned blocks may have been refreshed since the last solve
-            hipLaunchKernelGGL(k_gather_nodes<9>, dim3(g9), dim3(RED_WG), 0, ctx->stream, nf, w->perm, w->learned64,
-                               w->learned64_p);
         OFEA_HIP(hipGetLastError());
         const SysView S = perm_view(ctx);
-        const double *mb = learned ? w->learned64_p : w->minv_p;
-        if (method == OFEA_PCG_TWO_LEVEL || method == OFEA_PCG_LEARNED_TWO_LEVEL)
-            rc = pcg_run<double, double, 1, true>(ctx, S, method == OFEA_PCG_TWO_LEVEL ? 6 : 8, w->val_p, w->v64, mb,
-                                                  rtol, maxit, -1.0, iters, relres);
-        else
-            rc = pcg_run<float, double, 1, true>(ctx, S, method == OFEA_PCG_TWO_LEVEL_MIXED ? 7 : 9, w->val32_p,
-                                                 w->v64, mb, rtol, maxit, 0.1, iters, relres);
+        switch (method) {
+        case OFEA_PCG_TWO_LEVEL:
+            rc = pcg_run<double, double, 1, true>(ctx, S, 6, w->val_p, w->v64, w->minv_p, rtol, maxit, -1.0, iters,
+                                                  relres);
+            break;
+        case OFEA_PCG_TWO_LEVEL_MIXED:
+            rc = pcg_run<float, double, 1, true>(ctx, S, 7, w->val32_p, w->v64, w->minv_p, rtol, maxit, 0.1, iters,
+                                                 relres);
+            break;
+        case OFEA_PCG_LEARNED_TWO_LEVEL:
+            rc = pcg_run<double, double, 2, true>(ctx, S, 8, w->val_p, w->v64, w->minv_p, rtol, maxit, -1.0, iters,
+                                                  relres);
+            break;
+        default:
+            rc = pcg_run<float, double, 2, true>(ctx, S, 9, w->val32_p, w->v64, w->minv_p, rtol, maxit, 0.1, iters,
+                                                 relres);
+        }
         if (rc != OFEA_OK && rc != OFEA_NOT_CONVERGED) return rc;
         hipLaunchKernelGGL(k_gather_nodes<3>, dim3(g3), dim3(RED_WG), 0, ctx->stream, nf, w->iperm, w->x_p, ctx->d_x);
         OFEA_HIP(hipGetLastError());
         return rc;
     }
+    if (learned && (rc = learned_factor(ctx, false))) return rc;
     const SysView S = ctx_view(ctx);
     switch (method) {
     case OFEA_PCG_JACOBI:
@@ -2134,10 +2429,10 @@ int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, 
         return pcg_run<float, double, 1, false>(ctx, S, 2, w->val32, w->v64, w->minv, rtol, maxit, 0.1, iters,
                                                 relres);
     case OFEA_PCG_LEARNED:
-        return pcg_run<double, double, 1, false>(ctx, S, 3, ctx->d_val, w->v64, w->learned64, rtol, maxit, -1.0,
-                                                 iters, relres);
+        return pcg_run<double, double, 2, false>(ctx, S, 3, ctx->d_val, w->v64, w->minv, rtol, maxit, -1.0, iters,
+                                                 relres);
     case OFEA_PCG_LEARNED_MIXED:
-        return pcg_run<float, double, 1, false>(ctx, S, 5, w->val32, w->v64, w->learned64, rtol, maxit, 0.1, iters,
+        return pcg_run<float, double, 2, false>(ctx, S, 5, w->val32, w->v64, w->minv, rtol, maxit, 0.1, iters,
                                                 relres);
     case OFEA_PCG_MIXED32:
         return pcg_run<float, float, 1, false>(ctx, S, 4, w->val32, w->v32, w->minv32, rtol, maxit, 0.1, iters,

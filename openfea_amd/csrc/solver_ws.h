@@ -23,9 +23,26 @@ struct DevScalars {
 // CG vectors; the search direction is double-buffered (p, p2) because the
 // SpMV kernel forms p_new = z + beta p_old on the fly while other workgroups
 // still read p_old.
+// y = G r of the factorized learned preconditioner (M = G^T G) is carried
+// by its own recursion (y -= alpha G q) next to r.
 template <class T>
 struct VecSet {
-    T *r = nullptr, *z = nullptr, *p = nullptr, *p2 = nullptr, *q = nullptr;
+    T *r = nullptr, *z = nullptr, *p = nullptr, *p2 = nullptr, *q = nullptr, *y = nullptr;
+};
+
+// The learned factor G (block lower-triangular on K's node-block pattern) as
+// two half block-matrices in the column-of-rows layout of K's values: row i
+// of lo holds the blocks G_is, s <= i; row i of up holds (G_si)^T, s >= i
+// (the rows of G^T), the diagonal transposed.  Row i's nb blocks start at
+// 9 ptr[i]; entry (c, 3 k + d) lies at 9 ptr[i] + 3 nb c + 3 k + d.  Blocks
+// keep the order of K's row.  Values in BF16 (the bits of a float's upper
+// half, rounded to nearest even).
+struct FacMat {
+    int *lo_ptr = nullptr, *lo_col = nullptr, *up_ptr = nullptr, *up_col = nullptr;
+    unsigned short *lo = nullptr, *up = nullptr;
+    long long nlo = 0, nup = 0;  // blocks
+    int kver = -1, gen = -1;     // K version and network generation of the values
+    int group = 0;               // lanes per node of the factor passes (timed on first use, 0 = not yet)
 };
 
 // captured graphs: jacobi, block, mixed, learned, mixed32, learned mixed, and
@@ -69,8 +86,10 @@ struct ofea_solver_ws {
     float *minv32 = nullptr;                // FP32 copy
     double *dblk = nullptr;                 // 3x3 diagonal blocks of K, 9 * nfree
     float *val32 = nullptr;                 // FP32 values, nnz
-    double *learned64 = nullptr;            // learned preconditioner blocks (FP64), 9 * nfree
-    float *learned32 = nullptr;             // FP32 copy used by the solver
+    // learned factorized correction M = G^T G for K's rows [0] and the
+    // reordered rows of the two-level methods [1]; values rebuilt when K or
+    // the network's outputs change
+    ofea::FacMat fac[2];
     bool precond_valid = false;
     // two-level coarse space
     ofea::CoarseArgs co;
@@ -86,7 +105,7 @@ struct ofea_solver_ws {
     int *nb_p = nullptr, *bcol_p = nullptr;  // node_blk and bcol in the new order
     double *val_p = nullptr;                 // K values, node rows in the new order
     float *val32_p = nullptr;
-    double *dinv_p = nullptr, *minv_p = nullptr, *dblk_p = nullptr, *learned64_p = nullptr;
+    double *dinv_p = nullptr, *minv_p = nullptr, *dblk_p = nullptr;
     float *minv32_p = nullptr;
     double *b_p = nullptr, *x_p = nullptr;  // right-hand side and solution in the new order
     std::vector<int> iperm_host;

@@ -1,14 +1,14 @@
-"""Learned block-Jacobi correction: training pairs and training driver.
+"""Learned factorized correction of block Jacobi: Python driver.
 
 The network, its forward/backward kernels and the Adam loop live in the native
-library (learned.hip, ofea_learned_*).  This module collects the training pairs
-from the method's own solves and drives the training:
-
-  * x* is solved to a tight tolerance with block-Jacobi PCG;
-  * the iterates x_k of a block-Jacobi PCG run after k = 8, 16, 32, ... steps give
-    errors e_k = x* - x_k and residuals r_k = K e_k (one SpMV each, on the GPU);
-  * ofea_learned_train minimises the energy-norm error left by one
-    preconditioned Richardson step over those pairs.
+library (learned.hip, ofea_learned_*).  The preconditioner is M = G^T G with
+G_ii = L_i^-1 (D_i = L_i L_i^T) and G_is = -alpha_is L_i^-1 K_is D_s^-1 for the
+lower neighbours s < i, alpha_is = 2 sigmoid(u_i + v_s) from the network.
+Training minimises tr(G K G^T) / n over one or several resident systems (for
+example the design variants of one mesh): with G's diagonal blocks fixed that
+is Kaporin's condition number of G K G^T up to a constant, and it tracks the
+PCG iteration count (it rises where a too-large alpha creates small
+eigenvalues that losses on sampled error vectors do not see).
 """
 from __future__ import annotations
 
@@ -22,30 +22,6 @@ from ._capi import check, lib, ptr
 
 def nparam() -> int:
     return lib().ofea_learned_nparam()
-
-
-def collect_pairs(ctx, rhs_list, snapshots=(8, 16, 32, 64, 128, 256), rtol_star=1e-11, two_level=False):
-    """(E, R) arrays (S x n) from the solver's own PCG iterates on the resident K.
-
-    Block-Jacobi PCG by default.  With two_level the iterates come from the
-    two-level solver and the targets are the errors left after its coarse
-    correction, e - Z A_c^-1 Z^T r: the network then learns the local part
-    of the two-level preconditioner."""
-    method = A.PCG_TWO_LEVEL if two_level else A.PCG_BLOCK_JACOBI
-    E, R = [], []
-    for b in rhs_list:
-        xs, _, _, _ = ctx.solve(b, method=method, rtol=rtol_star, maxit=200000)
-        for k in snapshots:
-            xk, it, _, ok = ctx.solve(b, method=method, rtol=1e-30, maxit=k)
-            e = xs - xk
-            if not np.any(e):
-                continue
-            r = ctx.spmv(e)
-            if two_level:
-                e = e - coarse_apply(ctx, r)
-            E.append(e)
-            R.append(r)
-    return np.ascontiguousarray(np.array(E)), np.ascontiguousarray(np.array(R))
 
 
 def coarse_apply(ctx, r):
@@ -81,13 +57,8 @@ class Learned:
         check(lib().ofea_learned_get_params(self.ctx._h, ptr(p), len(p)))
         return p
 
-    def set_samples(self, E, R):
-        E = np.ascontiguousarray(E, dtype=np.float64)
-        R = np.ascontiguousarray(R, dtype=np.float64)
-        check(lib().ofea_learned_set_samples(self.ctx._h, ptr(E), ptr(R), len(E)))
-        self._keep = (E, R)
-
     def loss_grad(self):
+        """tr(G K G^T) / n for the resident K and its gradient"""
         loss = C.c_double()
         g = np.zeros(nparam(), dtype=np.float32)
         check(lib().ofea_learned_loss_grad(self.ctx._h, C.byref(loss), ptr(g)))
@@ -98,12 +69,21 @@ class Learned:
         check(lib().ofea_learned_train(self.ctx._h, steps, lr, ptr(hist)))
         return hist[:steps]
 
-    def blocks(self):
+    def factor(self):
+        """G on K's node-block layout (nnz values: per free node i and block
+        slot k a row-major 3x3: G_is for s < i, G_ii, (G_si)^T for s > i) and
+        the node features (nfree x 9)."""
         n = self.ctx.n // 3
-        b = np.zeros(9 * n)
+        b = np.zeros(self.ctx.nnz)
         f = np.zeros(9 * n, dtype=np.float32)
         check(lib().ofea_learned_blocks(self.ctx._h, ptr(b), ptr(f)))
-        return b.reshape(n, 3, 3), f.reshape(n, 9)
+        return b, f.reshape(n, 9)
+
+    def outputs(self):
+        """the network's outputs (u, v) per free node (nfree x 2), as they serve the solver"""
+        uv = np.zeros(self.ctx.n // 3 * 2, dtype=np.float32)
+        check(lib().ofea_learned_outputs(self.ctx._h, ptr(uv)))
+        return uv.reshape(-1, 2)
 
     def precision(self, prec):
         """Inference precision of the hidden layer (A.FP32 or A.BF16 on the matrix cores)."""
@@ -111,3 +91,13 @@ class Learned:
 
     def refresh(self):
         check(lib().ofea_learned_refresh(self.ctx._h))
+
+
+def train_multi(learned, steps=200, lr=2e-3):
+    """Adam on the mean loss of several contexts' resident systems (Learned
+    handles on one device); the parameters start from the first handle's and
+    end in all of them.  Returns the loss history."""
+    hs = (C.c_void_p * len(learned))(*[ln.ctx._h for ln in learned])
+    hist = np.zeros(max(steps, 1))
+    check(lib().ofea_learned_train_multi(hs, len(learned), steps, lr, ptr(hist)))
+    return hist[:steps]

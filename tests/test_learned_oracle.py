@@ -1,12 +1,19 @@
-"""Pins the float64 oracle of the learned preconditioner (CPU only).
+"""Pins the float64 oracle of the learned factorized correction (CPU only).
 
-  * zero head => the blocks are exactly the block-Jacobi inverses D_r^-1;
-  * the blocks are SPD for arbitrary parameters;
-  * the analytic gradient equals central finite differences of the loss
+  * alpha -> 0 (very negative head bias) gives M = G^T G = block Jacobi D^-1
+    and the loss 1 (tr(L_i^-1 D_i L_i^-T) = 3 per node);
+  * M is SPD for arbitrary parameters; the upper slots of the stored factor
+    are the transposes of the lower blocks they mirror;
+  * the loss equals tr(G K G^T) / n from dense matrices;
+  * the per-pair derivative -2 tr(C_is W_s) / n matches central differences in
+    one alpha, and vanishes on a row whose alphas solve that row's normal
+    equations (built from the blocks, a route that shares no code with it);
+  * the parameter gradient equals central finite differences of the loss
     (every parameter group, random parameters with a non-zero head).
 """
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 from oracle import oracle as O
 from oracle import learned_oracle as LO
@@ -22,6 +29,15 @@ def graph():
     return LO.Graph(cp, ri, val)
 
 
+@pytest.fixture(scope="module")
+def tet_graph():
+    m = mg.kuhn_tets(3, 2, 2, jitter=0.2, seed=4)
+    resort = mg.free_index(m.nd, m.fix_set)
+    cp, ri = O.build_csc(m.conn, m.etype, resort, 3 * int(resort.max() + 1))
+    val = O.fill_csc(m.conn, m.etype, m.xyz, resort, 210000.0, 0.3, cp, ri)
+    return LO.Graph(cp, ri, val)
+
+
 def random_params(seed, head_scale=0.3):
     rng = np.random.default_rng(seed)
     p = rng.normal(0, 0.3, LO.NPARAM)
@@ -31,28 +47,65 @@ def random_params(seed, head_scale=0.3):
     return LO.pack(d)
 
 
-def test_zero_head_is_block_jacobi(graph):
+def test_zero_alpha_is_block_jacobi(graph):
     p = LO.unpack(random_params(1))
     p["Wo"][:] = 0
-    p["bo"][:] = 0
-    M, _ = LO.forward(graph, LO.pack(p))
-    assert np.allclose(M, np.linalg.inv(graph.D), rtol=1e-12, atol=0)
+    p["bo"][:] = [-60.0, 0.0]
+    slots, c = LO.forward(graph, LO.pack(p))
+    M = (c["G"].T @ c["G"]).toarray()
+    Dinv = sp.block_diag(list(np.linalg.inv(graph.D))).toarray()
+    assert np.abs(M - Dinv).max() <= 1e-12 * np.abs(Dinv).max()
+    assert abs(LO.loss_grad(graph, LO.pack(p))[0] - 1.0) < 1e-12
 
 
-def test_blocks_spd(graph):
-    M, _ = LO.forward(graph, random_params(2, head_scale=0.5))
-    assert np.allclose(M, M.transpose(0, 2, 1), rtol=1e-12)
+def test_spd_and_mirrored_slots(graph):
+    slots, c = LO.forward(graph, random_params(2, head_scale=0.5))
+    M = (c["G"].T @ c["G"]).toarray()
+    assert np.allclose(M, M.T, rtol=0, atol=1e-12 * np.abs(M).max())
     ev = np.linalg.eigvalsh(M)
-    assert np.all(ev.min(axis=1) > 1e-10 * ev.max(axis=1))
+    assert ev.min() > 1e-10 * ev.max()
+    G = c["G"].toarray()
+    for i in range(graph.N):
+        for k, s in enumerate(graph.nbrs[i]):
+            blk = G[3 * i:3 * i + 3, 3 * s:3 * s + 3] if s <= i else G[3 * s:3 * s + 3, 3 * i:3 * i + 3].T
+            assert np.allclose(slots[i][k], blk, rtol=0, atol=1e-13 * np.abs(G).max())
+
+
+def test_loss_is_dense_trace(graph):
+    p = random_params(3)
+    loss, _ = LO.loss_grad(graph, p)
+    slots, c = LO.forward(graph, p)
+    G = c["G"].toarray()
+    assert abs(loss - np.trace(G @ graph.K.toarray() @ G.T) / graph.n) < 1e-13
+
+
+@pytest.mark.parametrize("which", ["hex", "tet"])
+def test_pair_derivative_and_row_optimum(which, graph, tet_graph):
+    g = graph if which == "hex" else tet_graph
+    rng = np.random.default_rng(9)
+    alphas = {(i, s): rng.uniform(0.2, 1.5) for (i, k, s) in g.lower_edges()}
+    G, _ = LO.factor(g, alphas=alphas)
+    eg = LO.edge_grad(g, G)
+    for key in list(alphas)[::7]:
+        h = 1e-2  # the loss is quadratic in one alpha: central differences are exact up to rounding
+        ap, am = dict(alphas), dict(alphas)
+        ap[key] += h
+        am[key] -= h
+        fd = (LO.trace_loss(g, LO.factor(g, alphas=ap)[0]) - LO.trace_loss(g, LO.factor(g, alphas=am)[0])) / (2 * h)
+        assert abs(fd - eg[key]) <= 1e-9 * abs(fd) + 1e-13, (key, fd, eg[key])
+    i = max(range(g.N), key=lambda r: sum(1 for s in g.nbrs[r] if s < r))  # the row with most lower pairs
+    ss, a_opt = LO.row_optimal_alpha(g, i)
+    for s, a in zip(ss, a_opt):
+        alphas[(i, s)] = a
+    eg = LO.edge_grad(g, LO.factor(g, alphas=alphas)[0])
+    scale = max(abs(v) for v in eg.values())
+    assert max(abs(eg[(i, s)]) for s in ss) < 1e-10 * scale
 
 
 def test_gradient_finite_differences(graph):
     rng = np.random.default_rng(7)
-    E = rng.normal(size=(3, graph.n))
-    R = np.array([graph.K @ e for e in E])
     p = random_params(4)
-    loss, g = LO.loss_grad(graph, p, E, R)
-    assert loss > 0
+    loss, g = LO.loss_grad(graph, p)
     offs = np.cumsum([0] + [int(np.prod(s)) for _, s in LO.SIZES])
     for gi in range(len(LO.SIZES)):
         idx = rng.choice(np.arange(offs[gi], offs[gi + 1]), size=min(4, offs[gi + 1] - offs[gi]), replace=False)
@@ -61,7 +114,7 @@ def test_gradient_finite_differences(graph):
             pp, pm = p.copy(), p.copy()
             pp[i] += h
             pm[i] -= h
-            fd = (LO.loss_grad(graph, pp, E, R)[0] - LO.loss_grad(graph, pm, E, R)[0]) / (2 * h)
+            fd = (LO.loss_grad(graph, pp)[0] - LO.loss_grad(graph, pm)[0]) / (2 * h)
             assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-5 * np.abs(g).max(), (LO.SIZES[gi][0], i, fd, g[i])
 
 
