@@ -896,12 +896,7 @@ int autotune(ofea_ctx *c, int *best_group, float *times)
     }
     // the CG kernels instantiate 4..64; the multi-RHS kernels 8, 16, 32
     c->ws->spmv_group = best;
-    for (int s = 0; s < NSLOT; ++s) {
-        if (c->ws->exec[s]) (void)hipGraphExecDestroy(c->ws->exec[s]);
-        if (c->ws->graph[s]) (void)hipGraphDestroy(c->ws->graph[s]);
-        c->ws->exec[s] = nullptr;
-        c->ws->graph[s] = nullptr;
-    }
+    solver_drop_graphs(c);
     if (best_group) *best_group = best;
     return OFEA_OK;
 }

@@ -254,6 +254,7 @@ int learned_precision(ofea_ctx *c, int precision);
 int learned_factor(ofea_ctx *c, bool permuted);
 // index arrays of the factor's half matrices for view 0 (K's rows) or 1 (the
 // two-level order) and the iteration's y vectors (solver.hip)
+void solver_drop_graphs(ofea_ctx *ctx);
 int fac_prepare(ofea_ctx *ctx, int view);
 // lanes per node of the factor passes of a view, timed once its values are built (solver.hip)
 int fac_tune(ofea_ctx *ctx, int view);

@@ -42,6 +42,7 @@ namespace ofea {
 
 constexpr int RED_WG = 256;   // threads per workgroup of vector kernels
 constexpr int CG_BATCH = 16;  // iterations per graph launch
+constexpr int CG_TAIL = 4;    // iterations per graph launch near convergence
 
 // Cross-lane moves for the reductions, on the VALU (DPP) and the scalar
 // read-lane path: a shuffle through ds_bpermute costs LDS issue slots in every
@@ -845,6 +846,17 @@ __global__ void k_cg_batch_end(DevScalars *sc, int nbatch)
     if (threadIdx.x == 0 && blockIdx.x == 0 && !sc->done) sc->it0 += nbatch;
 }
 
+// stopping targets from ||b||^2 (written by k_residual): tol2 = rtol^2 ||b||^2,
+// reliable-update level ru^2 ||b||^2 (ru2 < 0: none); b = 0 ends the run at once
+__global__ void k_set_tol(DevScalars *sc, double rtol2, double ru2)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double bb = sc->s[S_BB];
+    sc->s[S_TOL2] = rtol2 * bb;
+    sc->s[S_RU2] = ru2 > 0.0 ? ru2 * bb : -1.0;
+    if (bb == 0.0) sc->done = 1;
+}
+
 // Residual replacement / start: r = (T) r64, z = M r; partials <r,z> into
 // the slot of parity par0 and <r,r>.  At a start (restart != 0) the other
 // slot is set to +inf, so the next iteration takes beta = 0 (p = z).  At a
@@ -1054,7 +1066,7 @@ static int ensure_ws(ofea_ctx *ctx)
     OFEA_HIP(hipMalloc((void **)&w->part_pq, sizeof(double) * CG_NB));
     OFEA_HIP(hipMalloc((void **)&w->sc, sizeof(DevScalars)));
     OFEA_HIP(hipMemset(w->sc, 0, sizeof(DevScalars)));
-    OFEA_HIP(hipHostMalloc((void **)&w->sc_host, 3 * sizeof(DevScalars), hipHostMallocDefault));  // [0] + 2 in flight
+    OFEA_HIP(hipHostMalloc((void **)&w->sc_host, (1 + NQ_PIPE) * sizeof(DevScalars), hipHostMallocDefault));
     for (auto &e : w->pipe_ev) OFEA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const size_t nb = (size_t)std::max(n, 1), nfb = (size_t)std::max(nf, 1);
     for (double **p : {&w->v64.r, &w->v64.z, &w->v64.p, &w->v64.p2, &w->v64.q, &w->r64, &w->y64})
@@ -1073,10 +1085,7 @@ void solver_free(ofea_ctx *ctx)
 {
     ofea_solver_ws *w = ctx->ws;
     if (!w) return;
-    for (int i = 0; i < NSLOT; ++i) {
-        if (w->exec[i]) (void)hipGraphExecDestroy(w->exec[i]);
-        if (w->graph[i]) (void)hipGraphDestroy(w->graph[i]);
-    }
+    solver_drop_graphs(ctx);
     for (double *p : {w->part_rz, w->part_rr, w->part_pq, w->partials, w->v64.r, w->v64.z, w->v64.p, w->v64.p2, w->v64.q, w->r64, w->y64, w->dinv, w->minv,
                       w->dblk})
         if (p) (void)hipFree(p);
@@ -1100,6 +1109,25 @@ void solver_free(ofea_ctx *ctx)
         if (e) (void)hipEventDestroy(e);
     delete w;
     ctx->ws = nullptr;
+}
+
+// the captured PCG graphs (both lengths), e.g. after the lane group changed
+void solver_drop_graphs(ofea_ctx *ctx)
+{
+    ofea_solver_ws *w = ctx->ws;
+    if (!w) return;
+    for (int i = 0; i < NSLOT; ++i) {
+        for (hipGraphExec_t *e : {&w->exec[i], &w->exec_tail[i]})
+            if (*e) {
+                (void)hipGraphExecDestroy(*e);
+                *e = nullptr;
+            }
+        for (hipGraph_t *g : {&w->graph[i], &w->graph_tail[i]})
+            if (*g) {
+                (void)hipGraphDestroy(*g);
+                *g = nullptr;
+            }
+    }
 }
 
 void solver_invalidate(ofea_ctx *ctx)
@@ -1396,14 +1424,14 @@ int solver_prepare(ofea_ctx *ctx)
     return OFEA_OK;
 }
 
-// The graph of one batch: CG_BATCH iterations (2 kernels each, 3 with the
-// factorized correction) + batch end.
+// The graph of one batch: nbatch (even) iterations (2 kernels each, 3 with
+// the factorized correction) + batch end.
 template <class TV, class T, class XT, int PC, bool CO>
 static void cg_batch(ofea_ctx *ctx, const PcgBufs &w, const SysView &S, const TV *val, VecSet<T> &v, XT *x,
-                     const T *minv)
+                     const T *minv, int nbatch = CG_BATCH)
 {
     const CGParts P = w.P;
-    for (int bi = 0; bi < CG_BATCH; ++bi) {
+    for (int bi = 0; bi < nbatch; ++bi) {
 #define OFEA_CG_CASE(G)                                                                                          \
     case G:                                                                                                      \
         hipLaunchKernelGGL((k_cg_spmv<TV, T, G, CO>), dim3(CG_NB), dim3(CG_WG), 0, ctx->stream, bi, w.nfree,  \
@@ -1423,7 +1451,7 @@ static void cg_batch(ofea_ctx *ctx, const PcgBufs &w, const SysView &S, const TV
         if constexpr (PC == 2)
             launch_fac_upper<T>(ctx->stream, S.fac_group, bi, 0, CO ? 1 : 0, w.nfree, S.fac, v.y, v.r, v.z, P, w.sc);
     }
-    hipLaunchKernelGGL(k_cg_batch_end, dim3(1), dim3(64), 0, ctx->stream, w.sc, CG_BATCH);
+    hipLaunchKernelGGL(k_cg_batch_end, dim3(1), dim3(64), 0, ctx->stream, w.sc, nbatch);
 }
 
 // true FP64 residual r64 = b - K64 x; ||r64||^2 -> S_TRR, ||b||^2 -> S_BB
@@ -1453,9 +1481,19 @@ static int write_scalars(ofea_ctx *ctx)
 }
 
 // PCG with reliable updates.  x (FP64, device) starts at 0.  TV/T: matrix and
-// vector precision of the iteration; PC: 0 point Jacobi, 1 block preconditioner.
-// ru_factor > 0 requests a residual replacement each time the recursive
-// residual has dropped by that factor since the last replacement.
+// vector precision of the iteration; PC: 0 point Jacobi, 1 block
+// preconditioner, 2 factorized correction.  ru_factor > 0 requests a
+// residual replacement each time the recursive residual has dropped by that
+// factor since the last replacement.
+//
+// Host loop: graphs of CG_BATCH iterations, up to two in flight; the host
+// reads each batch's scalars (pinned copy + event) while the next runs.  From
+// the recursive residuals of the completed batches it predicts the
+// iterations still needed; once they fit in what is in flight plus less than
+// a batch, it tops up with graphs of CG_TAIL iterations instead, so that
+// little work is queued past convergence.  The stopping targets are set on the
+// device from ||b||^2, so the start needs no round trip; at the end the true
+// residual is queued behind the drained batches and read with one sync.
 template <class TV, class T, int PC, bool CO>
 static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, VecSet<T> &v, const T *minv,
                    double rtol, int maxit, double ru_factor, int *iters, double *relres)
@@ -1471,101 +1509,115 @@ static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, Ve
     hipLaunchKernelGGL((k_fill<double>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, x, 0.0);
     hipLaunchKernelGGL((k_fill<T>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, v.p, T(0));
     hipLaunchKernelGGL((k_fill<T>), dim3(gv), dim3(RED_WG), 0, ctx->stream, n, v.p2, T(0));
+    S = DevScalars{};
+    S.maxit = maxit;
+    if (int rc = write_scalars(ctx)) return rc;
     int rc = true_residual(ctx, B, Sv, b, x);  // r64 = b and ||b||^2
     if (rc) return rc;
-    if ((rc = read_scalars(ctx))) return rc;
-    const double bb = S.s[S_BB];
-    if (bb == 0.0) {
-        if (iters) *iters = 0;
-        if (relres) *relres = 0.0;
-        return OFEA_OK;
-    }
-    double tol2 = rtol * rtol * bb;  // absolute target on the recursive ||r||^2
-    S.par0 = 0;
-    S.it0 = 0;
-    S.it = 0;
-    S.maxit = maxit;
-    S.done = 0;
-    S.ticket = 0;
-    S.s[S_TOL2] = tol2;
-    S.s[S_RU2] = ru_factor > 0.0 ? ru_factor * ru_factor * bb : -1.0;
-    if ((rc = write_scalars(ctx))) return rc;
-    hipLaunchKernelGGL((k_replace<T, PC, CO>), dim3(CG_NB), dim3(UP_WG), 0, ctx->stream, w->nfree, w->r64, v.r, v.z,
-                       1, Sv.dinv, minv, P, w->co, w->sc);
-    if (PC == 2) {  // y = G r, z = G^T y and <r, z> of slot par0
-        launch_fac_lower<T>(ctx->stream, Sv.fac_group, w->nfree, Sv.fac, v.r, v.y, w->sc);
-        launch_fac_upper<T>(ctx->stream, Sv.fac_group, 0, 1, 0, w->nfree, Sv.fac, v.y, v.r, v.z, P, w->sc);
-    }
-    if (CO)
-        hipLaunchKernelGGL((k_coarse<T>), dim3(w->co_nagg), dim3(CG_WG), 0, ctx->stream, w->nfree, w->co_nagg,
-                           w->co_ainv, v.r, v.z, w->part_rz, w->co, w->sc);
-    OFEA_HIP(hipGetLastError());
-    // OFEA_NO_GRAPH=1 launches the batches directly (for tools that cannot follow graph launches)
-    static const bool no_graph = std::getenv("OFEA_NO_GRAPH") && std::getenv("OFEA_NO_GRAPH")[0] == '1';
-    if (!w->exec[slot] && !no_graph) {
-        OFEA_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        cg_batch<TV, T, double, PC, CO>(ctx, B, Sv, val, v, x, minv);
-        OFEA_HIP(hipStreamEndCapture(ctx->stream, &w->graph[slot]));
-        OFEA_HIP(hipGraphInstantiate(&w->exec[slot], w->graph[slot], nullptr, nullptr, 0));
-    }
-    double rel = 1.0;
-    int tighten = 0;
-    // Up to two batches are kept in flight: the host checks the scalars of
-    // batch k (pinned copy + event) while batch k+1 runs.  After convergence
-    // the batch in flight returns at once from every kernel.
-    DevScalars *Hb = w->sc_host + 1;
-    auto launch_batch = [&](int hb) -> int {
-        if (no_graph) {
-            cg_batch<TV, T, double, PC, CO>(ctx, B, Sv, val, v, x, minv);
-            OFEA_HIP(hipGetLastError());
-        } else {
-            OFEA_HIP(hipGraphLaunch(w->exec[slot], ctx->stream));
-        }
-        OFEA_HIP(hipMemcpyAsync(&Hb[hb], w->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
-        OFEA_HIP(hipEventRecord(w->pipe_ev[hb], ctx->stream));
-        return OFEA_OK;
-    };
-    // with reliable updates (frequent stops) the batch in flight is mostly
-    // wasted work, so those methods keep one batch at a time
-    const int depth = ru_factor > 0.0 ? 1 : 2;
-    for (;;) {
-        for (int d = 0; d < depth; ++d)
-            if ((rc = launch_batch(d))) return rc;
-        int cur = 0;
-        for (;;) {
-            OFEA_HIP(hipEventSynchronize(w->pipe_ev[cur]));
-            if (Hb[cur].done) break;
-            if ((rc = launch_batch(cur))) return rc;
-            cur = (cur + 1) % depth;
-        }
-        OFEA_HIP(hipStreamSynchronize(ctx->stream));  // drain the batch in flight (it returned early)
-        if ((rc = read_scalars(ctx))) return rc;
-        const int done = S.done;
-        if (done == 3) return fail(OFEA_ERR_STATE, "PCG breakdown (matrix or preconditioner not SPD, or NaN)");
-        const int it_done = S.it, par_done = S.par0 ^ (S.bi_done & 1);
-        // true FP64 residual at convergence candidates, reliable-update points and maxit
-        if ((rc = true_residual(ctx, B, Sv, b, x))) return rc;
-        if ((rc = read_scalars(ctx))) return rc;
-        rel = std::sqrt(S.s[S_TRR] / bb);
-        if (rel <= rtol || done == 2) break;
-        if (done == 1 && ++tighten > 8) break;  // recursive and true residual disagree persistently
-        if (done == 1) tol2 *= 0.25;            // recursive residual converged but the true one did not
-        // replace the recursive residual, keep the search direction, resume at it_done
-        S.par0 = par_done;
-        S.it0 = it_done;
-        S.done = 0;
-        S.s[S_TOL2] = tol2;
-        S.s[S_RU2] = ru_factor > 0.0 ? ru_factor * ru_factor * S.s[S_TRR] : -1.0;
-        if ((rc = write_scalars(ctx))) return rc;
+    hipLaunchKernelGGL(k_set_tol, dim3(1), dim3(64), 0, ctx->stream, w->sc, rtol * rtol,
+                       ru_factor > 0.0 ? ru_factor * ru_factor : -1.0);
+    auto start = [&](int restart) {  // r from r64, z = M r and its partials
         hipLaunchKernelGGL((k_replace<T, PC, CO>), dim3(CG_NB), dim3(UP_WG), 0, ctx->stream, w->nfree, w->r64, v.r,
-                           v.z, 0, Sv.dinv, minv, P, w->co, w->sc);
-        if (PC == 2) {
+                           v.z, restart, Sv.dinv, minv, P, w->co, w->sc);
+        if (PC == 2) {  // y = G r, z = G^T y and <r, z> of slot par0
             launch_fac_lower<T>(ctx->stream, Sv.fac_group, w->nfree, Sv.fac, v.r, v.y, w->sc);
             launch_fac_upper<T>(ctx->stream, Sv.fac_group, 0, 1, 0, w->nfree, Sv.fac, v.y, v.r, v.z, P, w->sc);
         }
         if (CO)
             hipLaunchKernelGGL((k_coarse<T>), dim3(w->co_nagg), dim3(CG_WG), 0, ctx->stream, w->nfree, w->co_nagg,
                                w->co_ainv, v.r, v.z, w->part_rz, w->co, w->sc);
+    };
+    start(1);
+    OFEA_HIP(hipGetLastError());
+    // OFEA_NO_GRAPH=1 launches the batches directly (for tools that cannot follow graph launches)
+    static const bool no_graph = std::getenv("OFEA_NO_GRAPH") && std::getenv("OFEA_NO_GRAPH")[0] == '1';
+    if (!w->exec[slot] && !no_graph) {
+        OFEA_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        cg_batch<TV, T, double, PC, CO>(ctx, B, Sv, val, v, x, minv, CG_BATCH);
+        OFEA_HIP(hipStreamEndCapture(ctx->stream, &w->graph[slot]));
+        OFEA_HIP(hipGraphInstantiate(&w->exec[slot], w->graph[slot], nullptr, nullptr, 0));
+        OFEA_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        cg_batch<TV, T, double, PC, CO>(ctx, B, Sv, val, v, x, minv, CG_TAIL);
+        OFEA_HIP(hipStreamEndCapture(ctx->stream, &w->graph_tail[slot]));
+        OFEA_HIP(hipGraphInstantiate(&w->exec_tail[slot], w->graph_tail[slot], nullptr, nullptr, 0));
+    }
+    double rel = 1.0;
+    int tighten = 0;
+    DevScalars *Hb = w->sc_host + 1;
+    int qh[NQ_PIPE], qn[NQ_PIPE];  // ring of batches in flight: scalar slot, iterations
+    int q0 = 0, qc = 0, next = 0, inflight = 0;
+    auto launch_batch = [&](int nit) -> int {
+        const int hb = next;
+        next = (next + 1) % NQ_PIPE;
+        if (no_graph) {
+            cg_batch<TV, T, double, PC, CO>(ctx, B, Sv, val, v, x, minv, nit);
+            OFEA_HIP(hipGetLastError());
+        } else {
+            OFEA_HIP(hipGraphLaunch(nit == CG_BATCH ? w->exec[slot] : w->exec_tail[slot], ctx->stream));
+        }
+        OFEA_HIP(hipMemcpyAsync(&Hb[hb], w->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
+        OFEA_HIP(hipEventRecord(w->pipe_ev[hb], ctx->stream));
+        qh[(q0 + qc) % NQ_PIPE] = hb;
+        qn[(q0 + qc) % NQ_PIPE] = nit;
+        ++qc;
+        inflight += nit;
+        return OFEA_OK;
+    };
+    // with reliable updates (frequent stops) the work in flight is mostly
+    // wasted, so those methods keep one batch at a time
+    const int depth = ru_factor > 0.0 ? 1 : 2;
+    for (;;) {
+        double last_rr = -1.0, lr = 0.0;  // residual at the last completed batch; log decay per iteration
+        int last_it = 0;
+        for (int d = 0; d < depth; ++d)
+            if ((rc = launch_batch(CG_BATCH))) return rc;
+        for (;;) {
+            const int hb = qh[q0];
+            inflight -= qn[q0];
+            q0 = (q0 + 1) % NQ_PIPE;
+            --qc;
+            OFEA_HIP(hipEventSynchronize(w->pipe_ev[hb]));
+            const DevScalars &D = Hb[hb];
+            if (D.done) break;
+            const double rr = D.s[S_RR], tol2 = D.s[S_TOL2];
+            if (last_rr > 0.0 && rr > 0.0 && rr < last_rr && D.it0 > last_it)
+                lr = std::log(rr / last_rr) / (D.it0 - last_it);
+            last_rr = rr;
+            last_it = D.it0;
+            // iterations still needed from this point (unknown: a full pipeline)
+            const double need = lr < 0.0 && tol2 > 0.0 ? std::log(tol2 / rr) / lr + 1.0 : 1e30;
+            const int want = (int)std::min<double>(need, (double)depth * CG_BATCH);
+            while (inflight < want && qc < NQ_PIPE - 1)
+                if ((rc = launch_batch(want - inflight >= CG_BATCH ? CG_BATCH : CG_TAIL))) return rc;
+            if (qc == 0 && (rc = launch_batch(CG_TAIL))) return rc;  // keep going past a wrong prediction
+        }
+        // the batches still in flight return at once from every kernel; the
+        // true FP64 residual follows them in the stream
+        if ((rc = true_residual(ctx, B, Sv, b, x))) return rc;
+        OFEA_HIP(hipGetLastError());
+        if ((rc = read_scalars(ctx))) return rc;
+        q0 = qc = inflight = 0;
+        const int done = S.done;
+        const double bb = S.s[S_BB];
+        if (bb == 0.0) {  // b = 0: x = 0 is exact
+            if (iters) *iters = 0;
+            if (relres) *relres = 0.0;
+            return OFEA_OK;
+        }
+        if (done == 3) return fail(OFEA_ERR_STATE, "PCG breakdown (matrix or preconditioner not SPD, or NaN)");
+        const int it_done = S.it, par_done = S.par0 ^ (S.bi_done & 1);
+        rel = std::sqrt(S.s[S_TRR] / bb);
+        if (rel <= rtol || done == 2) break;
+        if (done == 1 && ++tighten > 8) break;  // recursive and true residual disagree persistently
+        if (done == 1) S.s[S_TOL2] *= 0.25;     // recursive residual converged but the true one did not
+        // replace the recursive residual, keep the search direction, resume at it_done
+        S.par0 = par_done;
+        S.it0 = it_done;
+        S.done = 0;
+        S.ticket = 0;
+        S.s[S_RU2] = ru_factor > 0.0 ? ru_factor * ru_factor * S.s[S_TRR] : -1.0;
+        if ((rc = write_scalars(ctx))) return rc;
+        start(0);
         OFEA_HIP(hipGetLastError());
     }
     if (iters) *iters = S.it;

@@ -48,6 +48,8 @@ struct FacMat {
 // captured graphs: jacobi, block, mixed, learned, mixed32, learned mixed, and
 // the four two-level variants (block, mixed, learned, learned mixed)
 constexpr int NSLOT = 10;
+// batches of the PCG host loop in flight at most, plus one
+constexpr int NQ_PIPE = 4;
 
 // Two-level coarse correction: z = M r + Z A_c^-1 Z^T r, Z = rigid-body modes
 // (3 translations, 3 rotations about the centroid) of node aggregates.  An
@@ -76,8 +78,8 @@ struct ofea_solver_ws {
     double *partials = nullptr;
     double *part_rz = nullptr, *part_rr = nullptr, *part_pq = nullptr;  // CG partials (fixed grid)
     ofea::DevScalars *sc = nullptr;
-    ofea::DevScalars *sc_host = nullptr;  // pinned, 3 entries: [0] and two batches in flight
-    hipEvent_t pipe_ev[2] = {nullptr, nullptr};
+    ofea::DevScalars *sc_host = nullptr;  // pinned, 1 + NQ_PIPE entries: [0] and the batches in flight
+    hipEvent_t pipe_ev[ofea::NQ_PIPE] = {};
     ofea::VecSet<double> v64;
     ofea::VecSet<float> v32;
     double *r64 = nullptr, *y64 = nullptr;  // true residual, K x
@@ -117,6 +119,6 @@ struct ofea_solver_ws {
     int *sl_blk = nullptr;
     double *co_A = nullptr, *co_inv = nullptr;  // A_c (then L and L^T) and its inverse, FP64
     int *co_flag = nullptr;
-    hipGraph_t graph[ofea::NSLOT] = {};
-    hipGraphExec_t exec[ofea::NSLOT] = {};
+    hipGraph_t graph[ofea::NSLOT] = {}, graph_tail[ofea::NSLOT] = {};  // CG_BATCH and CG_TAIL iterations
+    hipGraphExec_t exec[ofea::NSLOT] = {}, exec_tail[ofea::NSLOT] = {};
 };
