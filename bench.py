@@ -9,7 +9,11 @@ reference's CSC arrays with mesh, pattern and plans resident in HBM.
 value = assembled elements per second over the K timed steps.  The same
 line reports time-to-solve to a true relative residual of 1e-8 for plain
 FP64 Jacobi PCG and the mixed-precision/learned solvers, the roofline of the
-assembly kernel and the oracle's single-core CPU rate.
+assembly kernel and the CPU baseline: the reference's own element code
+(oracle/_ref) filling the same K on one host core and on all cores, with the
+oracle's single-core rate beside it.  A "configs" block adds compact lines
+for configs 1, 3, 4 and 5 (each with its time, roofline and CPU baseline);
+--no-configs leaves it out.
 
 Other workloads (--workload cfg1|cfg3|cfg4|cfg5|ca|post|sfem) print their own line.
 """
@@ -33,7 +37,8 @@ def log(msg):
 
 
 def load_traffic(kernel_tag):
-    """HBM bytes per launch measured by rocprofv3 --pmc (committed summary)."""
+    """HBM bytes per launch measured by rocprofv3 --pmc (committed summary)
+    and where the figure comes from."""
     path = os.path.join(REPO, "profiles", f"traffic_{kernel_tag}.json")
     if os.path.exists(path):
         with open(path) as fh:
@@ -41,17 +46,53 @@ def load_traffic(kernel_tag):
     return None
 
 
-def cpu_baseline_fill(m, resort, reps=3):
-    """Oracle fill rate (elems/s, one core) and the K it filled."""
+def traffic_source(kernel_tag):
+    return (f"profiles/traffic_{kernel_tag}.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of the same kernel "
+            f"and workload, scaled by the copy-kernel calibration (profiles/copy_calibration.md); not measured "
+            f"in this run")
+
+
+def cpu_threads():
+    n = os.cpu_count() or 1
+    return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n))))
+
+
+def cpu_fill_baseline(m, resort, what, E_elem=None, reps=1):
+    """K filled on the host by the reference's own element code (oracle/_ref:
+    build_ele_stiff_mat of src/Elements/C3D4.cpp / C3D8.cpp and the scatter of
+    fill_CSR_sparse_mat) on one core and on all cores, beside the oracle's
+    single-core rate (the calibration of the port).  Without oracle/_ref only
+    the oracle's rate (kind "port").  Returns the baseline and the K filled."""
     from oracle import oracle as O
 
     cp, ri = O.build_csc(m.conn, m.etype, resort, 3 * int(resort.max() + 1))
-    times = []
-    for _ in range(reps):
-        t = time.perf_counter()
-        val = O.fill_csc(m.conn, m.etype, m.xyz, resort, 210000.0, 0.3, cp, ri)
-        times.append(time.perf_counter() - t)
-    return m.ne / statistics.median(times), (cp, ri, val)
+
+    def best(fn):
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            r = fn()
+            ts.append(time.perf_counter() - t)
+        return min(ts), r
+
+    t_port, val = best(lambda: O.fill_csc(m.conn, m.etype, m.xyz, resort, 210000.0, 0.3, cp, ri, E_elem=E_elem))
+    out = {"unit": "elems/s", "nproc": os.cpu_count()}
+    if O.ref_lib() is None:
+        out.update({"value": m.ne / t_port, "cores": 1, "kind": "port",
+                    "sample": f"oracle fill_csc over {what} ({t_port:.2f}s); oracle/_ref not built here"})
+        return out, (cp, ri, val)
+    t1, (v1, _) = best(lambda: O.ref_fill_csc(m.conn, m.etype, m.xyz, resort, 210000.0, 0.3, cp, ri, threads=1,
+                                              E_elem=E_elem))
+    nt = cpu_threads()
+    tn, (vn, used) = best(lambda: O.ref_fill_csc(m.conn, m.etype, m.xyz, resort, 210000.0, 0.3, cp, ri, threads=nt,
+                                                 E_elem=E_elem))
+    out.update({"value": m.ne / t1, "cores": 1, "kind": "reference",
+                "sample": f"the reference's element stiffness + CSC scatter over {what}, pattern prebuilt "
+                          f"({t1:.2f}s on one core)",
+                "all_cores": {"value": m.ne / tn, "cores": used, "s": round(tn, 3)},
+                "port": {"value": m.ne / t_port, "cores": 1, "s": round(t_port, 3),
+                         "port_time_over_reference_time": round(t_port / t1, 3)}})
+    return out, (cp, ri, val)
 
 
 def cpu_baseline_solve(K, F):
@@ -198,17 +239,13 @@ def bench_static(args, workload):
 
     cpu = None
     if not args.no_cpu_baseline:
-        t = time.perf_counter()
-        rate, K = cpu_baseline_fill(m, resort)
-        cpu = {"value": rate, "unit": "elems/s", "cores": 1, "kind": "port",
-               "sample": f"oracle fill_csc (reference fill_CSR_sparse_mat algorithm) over the full {m.ne}-element "
-                         f"mesh, pattern prebuilt, median of 3 ({time.perf_counter() - t:.1f}s)"}
+        cpu, K = cpu_fill_baseline(m, resort, f"the full {m.ne}-element mesh")
         if workload == "cfg2" and not args.no_solve:
             ms, it, rel = cpu_baseline_solve(K, F)
-            cpu["time_to_solve"] = {"ms": round(ms, 1), "iters": it, "relres": rel,
+            cpu["time_to_solve"] = {"ms": round(ms, 1), "iters": it, "relres": rel, "cores": 1, "kind": "port",
                                     "method": "oracle Jacobi PCG, FP64, one core, same system and stopping rule "
-                                              "as plain_fp64_jacobi"}
-            log(f"cpu baseline: fill {rate:.3e} elems/s, Jacobi PCG {ms:.0f} ms ({it} it)")
+                                              "as plain_fp64_jacobi (the reference's own solver is PARDISO)"}
+        log(f"cpu baseline: {json.dumps(cpu)}")
     ctx.close()
     return {
         "metric": "assembled elems/s + time-to-solve at rel-resid 1e-8 (learned vs plain PCG)",
@@ -222,8 +259,8 @@ def bench_static(args, workload):
         "learned_inference_ms": infer,
         "setup_s": round(setup_s, 3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_assemble_tet" if nn == 4 else "k_assemble_hex",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source(tag),
+                     "kernel": "k_assemble_tet_chunks" if nn == 4 else "k_assemble_hex",
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }
@@ -269,6 +306,24 @@ def bench_explicit(args):
     u64 = ctx.explicit_run(mg.E_STEEL, mg.NU_STEEL, mass, fext, v0, fixed, dt, nsteps, precision=A.FP64)
     err = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
     log(f"explicit fp64 {ms64 * 1e3:.2f} us/step; fp32 vs fp64 at step {nsteps}: {err:.3e}")
+    # both states against the oracle's direct recurrence on the full mesh (K from the
+    # oracle's fill over all nodes; the recurrence equals the reference's incremental loop)
+    oracle_check = None
+    if not args.no_oracle_check:
+        t = time.perf_counter()
+        allfree = np.arange(m.nd, dtype=np.int32)
+        cpa, ria = O.build_csc(m.conn, m.etype, allfree, 3 * m.nd)
+        kv = O.fill_csc(m.conn, m.etype, m.xyz, allfree, mg.E_STEEL, mg.NU_STEEL, cpa, ria)
+        nt = cpu_threads()
+        uo = O.explicit_recurrence(cpa, ria, kv, 1.0 / np.repeat(mass, 3), np.repeat(fixed, 3).astype(np.int32),
+                                   v0.ravel(), dt, nsteps, threads=nt)
+        del kv, cpa, ria
+        no = np.linalg.norm(uo)
+        oracle_check = {"fp32_rel_l2_err": float(np.linalg.norm(u32.ravel() - uo) / no),
+                        "fp64_rel_l2_err": float(np.linalg.norm(u64.ravel() - uo) / no), "steps": nsteps,
+                        "reference": "oracle direct recurrence u += v dt, v -= M^-1 K u dt (K from or_fill_csc), "
+                                     f"FP64 on {nt} host threads", "s": round(time.perf_counter() - t, 1)}
+        log(f"explicit vs oracle after {nsteps} steps: {oracle_check}")
     alg_bytes = 32 * m.ne + 64 * m.nd
     flops = 5300 * m.ne  # per element-step, see the explicit roofline note
     ctx.close()
@@ -288,9 +343,10 @@ def bench_explicit(args):
         "warmup": 100, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": wl, "dt": dt, "elements": m.ne, "nodes": m.nd},
-        "fp64_ms_per_step": ms64, "fp32_vs_fp64_rel_err_final": err,
+        "fp64_ms_per_step": ms64, "fp32_vs_fp64_rel_err_final": err, "vs_oracle_final": oracle_check,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("explicit_step_cfg3"),
+                     "traffic_source": traffic_source("explicit_step_cfg3"),
                      "kernel": "k_force_hex<float> + k_node_update<float> (one step)",
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "fp32_flops_per_step": flops,
@@ -345,14 +401,20 @@ def bench_batched(args):
         batched_solve = bench_batched_solve(args, ctx, m, Ev, B, nu)
     cpu = None
     if not args.no_cpu_baseline:
-        cp, ri = O.build_csc(m.conn, m.etype, resort, 3 * int(resort.max() + 1))
-        k = 48
-        t = time.perf_counter()
+        k = 8
+        rates = []
         for v in range(k):
-            O.fill_csc(m.conn, m.etype, m.xyz, resort, 0.0, nu, cp, ri, E_elem=Ev[v])
-        el = time.perf_counter() - t
-        cpu = {"value": k * m.ne / el, "unit": "variant-elems/s", "cores": 1, "kind": "port",
-               "sample": f"oracle fill_csc with per-element E for {k} of the 64 variants ({el:.1f}s)"}
+            c, _ = cpu_fill_baseline(m, resort, f"design variant {v} (per-element E)", E_elem=Ev[v])
+            rates.append(c)
+        cpu = dict(rates[0])
+        cpu["unit"] = "variant-elems/s"
+        cpu["sample"] = (f"{k} of the 64 variants, each filled on the host: " + cpu["sample"].split(" over ")[0] +
+                         "; rates are the medians over the variants")
+        cpu["value"] = statistics.median(r["value"] for r in rates)
+        if "all_cores" in cpu:
+            cpu["all_cores"] = {"value": statistics.median(r["all_cores"]["value"] for r in rates),
+                                "cores": rates[0]["all_cores"]["cores"]}
+            cpu["port"] = {"value": statistics.median(r["port"]["value"] for r in rates), "cores": 1}
     return {
         "metric": "batched assembled variant-elems/s + multi-RHS time-to-solve at rel-resid 1e-8",
         "value": value, "unit": "variant-elems/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -364,6 +426,7 @@ def bench_batched(args):
         "batched_solve": batched_solve,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("assemble_hex_batch_cfg4"),
+                     "traffic_source": traffic_source("assemble_hex_batch_cfg4"),
                      "kernel": "k_assemble_hex_batch", "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }
@@ -497,6 +560,7 @@ def bench_nl(args):
         "newton": runs, "learned_training": train,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("nl_assemble_hex_cfg5"),
+                     "traffic_source": traffic_source("nl_assemble_hex_cfg5"),
                      "kernel": "k_nl_hex", "algorithmic_bytes_per_launch": alg_bytes,
                      "fp64_flops_per_launch": 185472 * m.ne,
                      "fp64_tflops": 185472 * m.ne / (ms_step * 1e-3) / 1e12, "fp64_peak_tflops": FP64_PEAK_TFLOPS},
@@ -721,6 +785,7 @@ def bench_post(args):
         "max_rel_err_vs_oracle": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("post_strain_cfg2"),
+                     "traffic_source": traffic_source("post_strain_cfg2"),
                      "kernel": "k_expand + k_post_elem + k_post_node (one call)",
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
@@ -779,9 +844,47 @@ def bench_sfem(args):
         "setup_s": round(setup_s, 3), "two_level_solve_ms": solve_ms, "solve_iters": it, "solve_converged": ok,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("sfem_assemble_cfg2"),
+                     "traffic_source": traffic_source("sfem_assemble_cfg2"),
                      "kernel": "k_sfem_grad + k_sfem_gather (one assembly)", "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }
+
+
+def compact_configs(args):
+    """Configs 1, 3, 4 and 5 in short form for the default line: each one's
+    timed figure, roofline (with the source of its traffic figure) and CPU
+    baseline, from the same functions their own --workload runs use."""
+    def sub(**kw):
+        a = argparse.Namespace(**vars(args))
+        a.steps, a.warmup, a.solve_reps = 10, 2, 3
+        for k, v in kw.items():
+            setattr(a, k, v)
+        return a
+
+    def pick(d, keys):
+        return {k: d[k] for k in keys if k in d}
+
+    keys = ["metric", "value", "unit", "ms_per_step", "dtype", "roofline", "cpu_baseline"]
+    out = {}
+    t = time.perf_counter()
+    c1 = bench_static(sub(solvers="plain_fp64_jacobi,fp64_block_jacobi,two_level_fp64"), "cfg1")
+    out["cfg1"] = dict(pick(c1, keys), workload=c1["config"]["workload"],
+                       time_to_solve={k: pick(v, ["ms", "iters", "converged"]) for k, v in c1["time_to_solve"].items()})
+    c3 = bench_explicit(sub(explicit_steps=1000))
+    out["cfg3"] = dict(pick(c3, keys + ["fp64_ms_per_step", "fp32_vs_fp64_rel_err_final", "vs_oracle_final"]),
+                       workload=c3["config"]["workload"])
+    c4 = bench_batched(sub(seq_methods="two_level_fp64"))
+    bs = c4.get("batched_solve") or {}
+    out["cfg4"] = dict(pick(c4, keys + ["sequential_assembly_ms_64_variants"]), workload=c4["config"]["workload"],
+                       batched_solve=pick(bs, ["systems", "s", "two_level_sequential_all_s", "iters_max", "converged"]))
+    c5 = bench_nl(sub(newton_methods="fp64_block_jacobi,two_level_fp64,learned_fp64"))
+    out["cfg5"] = dict(pick(c5, keys), workload=c5["config"]["workload"],
+                       newton={k: pick(v, ["s", "newton_iters", "pcg_iters", "converged"]) for k, v in
+                               (c5.get("newton") or {}).items()},
+                       learned_training=pick(c5.get("learned_training") or {}, ["loss_first", "loss_last", "train_s"]))
+    out["wall_s"] = round(time.perf_counter() - t, 1)
+    log(f"configs block: {out['wall_s']} s")
+    return out
 
 
 def main():
@@ -797,8 +900,10 @@ def main():
     ap.add_argument("--newton-methods", default="plain_fp64_jacobi,fp64_block_jacobi,learned_fp64,learned_fp64_bf16net,"
                     "two_level_fp64",
                     help="inner PCG methods of the Newton runs (cfg5)")
-    ap.add_argument("--solve-reps", type=int, default=3)
+    ap.add_argument("--solve-reps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-oracle-check", action="store_true", help="cfg3: skip the full-size oracle trajectory")
+    ap.add_argument("--no-configs", action="store_true", help="default run: leave out the configs 1/3/4/5 block")
     ap.add_argument("--no-solve", action="store_true", help="assembly only (for counter collection)")
     ap.add_argument("--solvers", default="plain_fp64_jacobi,fp64_block_jacobi,mixed_fp32val_block_jacobi,"
                     "mixed_fp32_all_block_jacobi,learned_fp64,learned_mixed_fp32val,two_level_fp64,"
@@ -813,6 +918,8 @@ def main():
         sys.exit(2)
     if args.workload in ("cfg1", "cfg2"):
         out = bench_static(args, args.workload)
+        if args.workload == "cfg2" and not args.no_configs:
+            out["configs"] = compact_configs(args)
     elif args.workload == "cfg3":
         out = bench_explicit(args)
     elif args.workload == "cfg4":

@@ -612,6 +612,49 @@ OR_API void or_lumped_mass(int nd, int ne, const int *conn, const int *etype, co
     }
 }
 
+/* The loop below in its direct form (the equality is tested against the
+ * incremental loop): InForce is never zeroed, so after each step it holds
+ * -K u with K from or_fill_csc over all nodes (symmetric: column i = row i).
+ * Per step with zero external force: u += v dt, u = 0 at clamped DOFs,
+ * v -= minv (K u) dt.  threads > 1 splits the rows of K u over OpenMP
+ * threads; each row is summed in column order, so the result does not depend
+ * on the thread count.  fixed (n) marks clamped DOFs, minv (n) = 1 / mass.
+ * u_out (n) receives u after nsteps steps. */
+OR_API void or_explicit_recurrence(int n, const int *col_ptr, const int *row_idx, const double *val,
+                                   const double *minv, const int *fixed, const double *v0, double dt, int nsteps,
+                                   double *u_out, int threads)
+{
+    double *v = (double *)malloc((size_t)n * sizeof(double));
+    double *ku = (double *)malloc((size_t)n * sizeof(double));
+    if (!v || !ku) {
+        free(v);
+        free(ku);
+        return;
+    }
+    for (int i = 0; i < n; ++i) {
+        v[i] = v0[i];
+        u_out[i] = 0.0;
+    }
+    if (threads < 1) threads = 1;
+    for (int step = 0; step < nsteps; ++step) {
+#pragma omp parallel for schedule(static) num_threads(threads)
+        for (int i = 0; i < n; ++i) {
+            double u = u_out[i] + v[i] * dt;
+            u_out[i] = fixed[i] ? 0.0 : u;
+        }
+#pragma omp parallel for schedule(static) num_threads(threads)
+        for (int i = 0; i < n; ++i) {
+            double acc = 0.0;
+            for (int k = col_ptr[i]; k < col_ptr[i + 1]; ++k) acc += val[k] * u_out[row_idx[k]];
+            ku[i] = acc;
+        }
+#pragma omp parallel for schedule(static) num_threads(threads)
+        for (int i = 0; i < n; ++i) v[i] -= ku[i] * minv[i] * dt;
+    }
+    free(v);
+    free(ku);
+}
+
 /* a9/a10: half-step central-difference loop of src/cae.cpp:420-574 with a
  * fixed dt and a fixed number of steps.  InForce accumulates -Ke*du_e on the
  * original coordinates each step (src/Elements/C3D8.cpp:158-193, InForce is

@@ -50,6 +50,7 @@ def lib():
         L.or_reset_displacement.argtypes = [_i, _pi, _pd, _pd]
         L.or_lumped_mass.argtypes = [_i, _i, _pi, _pi, _pd, _d, _pd]
         L.or_explicit_run.argtypes = [_i, _i, _pi, _pi, _pd, _d, _d, _pd, _pd, _vp, _pi, _d, _i, _pd]
+        L.or_explicit_recurrence.argtypes = [_i, _pi, _pi, _pd, _pd, _pi, _pd, _d, _i, _pd, _i]
         L.or_matfree_apply.argtypes = [_i, _pi, _pi, _pd, _pi, _d, _d, _vp, _pd, _pd, _i]
         L.or_ca_fill.argtypes = [_i, _pi, _pi, _pi, _pd, _pd, _pi, _d, _d, _pi, _pi, _ll, _pd]
         L.or_ca_fill.restype = _i
@@ -151,6 +152,39 @@ def fill_csc(conn, etype, xyz, resort, E, nu, col_ptr, row_idx, E_elem=None):
     return val
 
 
+_REF = None
+
+
+def ref_lib():
+    """oracle/_ref/libofea_ref.so (the reference's element code, built by
+    oracle/Makefile where the reference checkout exists), or None."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libofea_ref.so")
+        if not os.path.exists(path):
+            return None
+        L = C.CDLL(path)
+        L.ref_fill_csc.argtypes = [_i, _pi, _pi, _pd, _pi, _d, _vp, _d, _pi, _pi, _pd, _i, _i]
+        L.ref_fill_csc.restype = _i
+        _REF = L
+    return _REF
+
+
+def ref_fill_csc(conn, etype, xyz, resort, E, nu, col_ptr, row_idx, threads=1, E_elem=None):
+    """K values from the reference's own element stiffness and the scatter of
+    fill_CSR_sparse_mat (ref_harness.cpp), on `threads` cores; (values,
+    threads used), or None without oracle/_ref."""
+    L = ref_lib()
+    if L is None:
+        return None
+    val = np.zeros(len(row_idx))
+    Ee = None if E_elem is None else _f64(E_elem)
+    used = L.ref_fill_csc(len(etype), _i32(conn), _i32(etype), _f64(xyz).ravel(), _i32(resort), E,
+                          None if Ee is None else Ee.ctypes.data, nu, _i32(col_ptr), _i32(row_idx), val,
+                          len(col_ptr) - 1, int(threads))
+    return val, used
+
+
 def nl_fill(conn, etype, xyz, resort, E, nu, u_free, col_ptr, row_idx):
     val = np.zeros(len(row_idx))
     n = len(col_ptr) - 1
@@ -187,6 +221,16 @@ def lumped_mass(conn, etype, xyz, rho):
     m = np.zeros(nd)
     lib().or_lumped_mass(nd, len(etype), _i32(conn), _i32(etype), _f64(xyz).ravel(), rho, m)
     return m
+
+
+def explicit_recurrence(col_ptr, row_idx, val, minv, fixed, v0, dt, nsteps, threads=1):
+    """u after nsteps of the direct recurrence u += v dt (u = 0 at fixed DOFs),
+    v -= minv (K u) dt on all DOFs (K symmetric, CSC), on `threads` cores."""
+    n = len(col_ptr) - 1
+    u = np.zeros(n)
+    lib().or_explicit_recurrence(n, _i32(col_ptr), _i32(row_idx), _f64(val), _f64(minv), _i32(fixed), _f64(v0),
+                                 dt, nsteps, u, int(threads))
+    return u
 
 
 def explicit_run(conn, etype, xyz, E, nu, mass, f_ext, v0, fixed_mask, dt, nsteps):

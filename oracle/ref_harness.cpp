@@ -17,7 +17,11 @@
 #include "include/C3D8.h"
 #include "include/C3D4.h"
 
+#include <algorithm>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace {
 
@@ -118,9 +122,81 @@ double timestep(const double *X, double E, double nu, int nn, double dt)
     return dt;
 }
 
+// column-major lookup of row r in column j (rows sorted), as
+// fill_CSR_sparse_mat finds each entry (src/assemble.cpp:95-151)
+inline int csc_find(const int *col_ptr, const int *row_idx, int j, int r)
+{
+    const int *b = row_idx + col_ptr[j], *e = row_idx + col_ptr[j + 1];
+    const int *p = std::lower_bound(b, e, r);
+    return (p != e && *p == r) ? (int)(p - row_idx) : -1;
+}
+
 }  // namespace
 
 extern "C" {
+
+// K on a prebuilt CSC pattern from the reference's own element stiffness
+// (build_ele_stiff_mat of the registered element classes) and the scatter of
+// fill_CSR_sparse_mat (src/assemble.cpp:95-151): every element's Ke added, in
+// element order, at the free DOFs (resort >= 0, 1-based conn, 8 slots per
+// element, etype 0 = C3D4, 1 = C3D8; E_elem per element or NULL for E).
+// With threads > 1 (OpenMP) the element
+// matrices of a chunk are computed in parallel and the scatter is split by
+// column ranges, each thread adding the chunk's elements in order into its
+// own columns, so the sums do not depend on the thread count.  Returns the
+// threads used.
+int ref_fill_csc(int ne, const int *conn, const int *etype, const double *xyz, const int *resort, double E,
+                 const double *E_elem, double nu, const int *col_ptr, const int *row_idx, double *val, int ncol,
+                 int threads)
+{
+    const int T = threads > 0 ? threads : 1;
+    const int CH = 4096;
+    std::vector<double> buf((size_t)CH * 576);
+    std::vector<int> dofs((size_t)CH * 24);
+    for (int e0 = 0; e0 < ne; e0 += CH) {
+        const int e1 = std::min(ne, e0 + CH);
+#pragma omp parallel for schedule(static) num_threads(T)
+        for (int e = e0; e < e1; ++e) {
+            const int nn = etype[e] == 1 ? 8 : 4;
+            double X[24];
+            int *dof = &dofs[(size_t)(e - e0) * 24];
+            for (int a = 0; a < nn; ++a) {
+                const int node = conn[8 * e + a] - 1, f = resort[node];
+                for (int c = 0; c < 3; ++c) {
+                    X[3 * a + c] = xyz[3 * node + c];
+                    dof[3 * a + c] = f >= 0 ? 3 * f + c : -1;
+                }
+            }
+            double *Ke = &buf[(size_t)(e - e0) * 576];
+            const double Ee = E_elem ? E_elem[e] : E;
+            if (nn == 8) ke<CAE::hex_ele_elastic>(X, Ee, nu, 8, Ke);
+            else ke<CAE::tetra_ele_elastic>(X, Ee, nu, 4, Ke);
+        }
+#pragma omp parallel num_threads(T)
+        {
+            int t = 0, nt = 1;
+#ifdef _OPENMP
+            t = omp_get_thread_num();
+            nt = omp_get_num_threads();
+#endif
+            const int c0 = (int)((long long)ncol * t / nt), c1 = (int)((long long)ncol * (t + 1) / nt);
+            for (int e = e0; e < e1; ++e) {
+                const int nn = etype[e] == 1 ? 8 : 4;
+                const double *Ke = &buf[(size_t)(e - e0) * 576];
+                const int *dof = &dofs[(size_t)(e - e0) * 24];
+                for (int j = 0; j < 3 * nn; ++j) {
+                    if (dof[j] < c0 || dof[j] >= c1) continue;
+                    for (int i = 0; i < 3 * nn; ++i) {
+                        if (dof[i] < 0) continue;
+                        const int k = csc_find(col_ptr, row_idx, dof[j], dof[i]);
+                        if (k >= 0) val[k] += Ke[i * 3 * nn + j];
+                    }
+                }
+            }
+        }
+    }
+    return T;
+}
 
 // hex_ele_elastic::build_ele_stiff_mat (src/Elements/C3D8.cpp:45-49)
 void ref_c3d8_ke(const double *X, double E, double nu, double *Ke) { ke<CAE::hex_ele_elastic>(X, E, nu, 8, Ke); }

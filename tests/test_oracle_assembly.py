@@ -190,6 +190,14 @@ def test_explicit_incremental_equals_direct_and_conserves_energy():
         ud[fx] = 0.0
         v = v - (K @ ud) / M * dt
     assert rel_max(u, ud) < 1e-12
+    # the C form of the direct recurrence (used as the long-run reference):
+    # equal to the incremental loop, and the same bits on 1 and 4 threads
+    fixed3 = np.repeat(fixed, 3).astype(np.int32)
+    minv = 1.0 / M
+    kv = O.fill_csc(m.conn, m.etype, m.xyz, allfree, 210000.0, 0.3, cp, ri)
+    uc1 = O.explicit_recurrence(cp, ri, kv, minv, fixed3, v0.ravel(), dt, nsteps, threads=1)
+    uc4 = O.explicit_recurrence(cp, ri, kv, minv, fixed3, v0.ravel(), dt, nsteps, threads=4)
+    assert rel_max(uc1, u) < 1e-12 and np.array_equal(uc1, uc4)
     # shadow energy of central differences, free DOFs only: 1/2 v+ M v- + 1/2 u K u
     Kf = K[~fx][:, ~fx]
     Mf = M[~fx]
