@@ -205,7 +205,7 @@ def bench_static(args, workload):
     if ln is not None:  # network inference for the resident K: FP32 vs BF16 on the matrix cores
         ln.set_params(params)
         learned_info["loss_on_benchmark_K"] = float(ln.loss_grad()[0])
-        for prec, tag in ((A.FP32, "fp32"), (A.BF16, "bf16_mfma")):
+        for prec, ptag in ((A.FP32, "fp32"), (A.BF16, "bf16_mfma")):
             ln.precision(prec)
             ts = []
             for _ in range(5):
@@ -214,7 +214,7 @@ def bench_static(args, workload):
                 ln.refresh()
                 ctx.sync()
                 ts.append(time.perf_counter() - t)
-            infer[tag] = round(statistics.median(ts) * 1e3, 3)
+            infer[ptag] = round(statistics.median(ts) * 1e3, 3)
         log(f"learned inference ms: {infer}")
     ctx.set_rhs(F)
     for name, method in methods:

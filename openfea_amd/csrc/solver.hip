@@ -1958,6 +1958,146 @@ __global__ __launch_bounds__(CH_WG) void k_band_cholesky(int nc, int bw, double 
     if (tid == 0) *flag = 0;
 }
 
+// The same banded Cholesky split over kernels, one panel of CP_NB = 16
+// columns at a time (32-column panels were slower: the panel's serial part
+// grows with the square of the width): k_chol_lift lifts the diagonal (as
+// above) and clears the flag;
+// k_chol_panel (one workgroup) factors the panel's diagonal block and the
+// band rows below it in LDS exactly as k_band_cholesky does and writes them
+// back; k_chol_update applies the panel to the trailing band in 32 x 32
+// tiles of its lower triangle, one workgroup each, every entry updated once
+// with the sum over the panel's columns in ascending order.  Every kernel
+// returns at once when an earlier pivot failed (*flag != 0).
+__global__ __launch_bounds__(CH_WG) void k_chol_lift(int nc, double *__restrict__ A, int *__restrict__ flag)
+{
+    __shared__ double s_red[CH_WG / 64];
+    const int tid = threadIdx.x;
+    double tr = 0.0;
+    for (int i = tid; i < nc; i += CH_WG) tr += A[(size_t)i * nc + i];
+    for (int o = 32; o > 0; o >>= 1) tr += __shfl_xor(tr, o);
+    if ((tid & 63) == 0) s_red[tid >> 6] = tr;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < CH_WG / 64; ++w) t += s_red[w];
+    const double lift = 1e-12 * t / nc;
+    for (int i = tid; i < nc; i += CH_WG) A[(size_t)i * nc + i] += lift;
+    if (tid == 0) *flag = 0;
+}
+
+constexpr int CP_NB = 16;          // panel width of the split factorization
+constexpr int CP_LD = CP_NB + 1;    // odd LDS row stride
+__global__ __launch_bounds__(CH_WG) void k_chol_panel(int nc, int bw, int j0, double *__restrict__ A,
+                                                      int *__restrict__ flag)
+{
+    constexpr int CH_NB = CP_NB, CH_LD = CP_LD;  // this kernel's panel (LDS: CH_ROWS x CP_LD doubles)
+    __shared__ double s_p[CH_ROWS * CH_LD];
+    __shared__ int s_bad;
+    const int tid = threadIdx.x;
+    if (*flag) return;
+    if (tid == 0) s_bad = 0;
+    const int nb = min(CH_NB, nc - j0);
+    const int H = min(nc, j0 + nb + bw) - j0;
+    for (int t = tid; t < H * CH_NB; t += CH_WG) {
+        const int r = t / CH_NB, c = t % CH_NB;
+        s_p[r * CH_LD + c] = (r >= c && c < nb) ? A[(size_t)(j0 + r) * nc + j0 + c] : 0.0;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        double v[CH_NB];
+#pragma unroll
+        for (int c = 0; c < CH_NB; ++c) v[c] = (tid < nb && c <= tid) ? s_p[tid * CH_LD + c] : 0.0;
+#pragma unroll
+        for (int c = 0; c < CH_NB; ++c) {
+            if (c < nb) {
+                const double dc = read_lane(v[c], c);
+                const double piv = dc > 0.0 ? sqrt(dc) : 1.0;
+                if (tid == 0 && !(dc > 0.0) && !s_bad) s_bad = j0 + c + 1;
+                if (tid == c) v[c] = piv;
+                if (tid > c) v[c] /= piv;
+#pragma unroll
+                for (int c2 = c + 1; c2 < CH_NB; ++c2) {
+                    const double lc2 = read_lane(v[c], c2);
+                    if (tid >= c2 && c2 < nb) v[c2] -= v[c] * lc2;
+                }
+            }
+        }
+        if (tid < nb)
+#pragma unroll
+            for (int c = 0; c < CH_NB; ++c)
+                if (c <= tid) s_p[tid * CH_LD + c] = v[c];
+    }
+    __syncthreads();
+    if (s_bad) {
+        if (tid == 0) *flag = s_bad;
+        return;
+    }
+    __shared__ double s_rd[CH_NB];  // reciprocals of the panel's pivots
+    if (tid < nb) s_rd[tid] = 1.0 / s_p[tid * CH_LD + tid];
+    __syncthreads();
+    for (int r = nb + tid; r < H; r += CH_WG) {
+        double x[CH_NB];
+#pragma unroll
+        for (int c = 0; c < CH_NB; ++c) x[c] = s_p[r * CH_LD + c];
+#pragma unroll
+        for (int c = 0; c < CH_NB; ++c) {
+            if (c < nb) {
+                double t = x[c];
+#pragma unroll
+                for (int c2 = 0; c2 < c; ++c2) t -= x[c2] * s_p[c * CH_LD + c2];
+                x[c] = t * s_rd[c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CH_NB; ++c) s_p[r * CH_LD + c] = x[c];
+    }
+    __syncthreads();
+    for (int t = tid; t < H * CH_NB; t += CH_WG) {
+        const int r = t / CH_NB, c = t % CH_NB;
+        if (r >= c && c < nb) A[(size_t)(j0 + r) * nc + j0 + c] = s_p[r * CH_LD + c];
+    }
+}
+
+constexpr int CU_T = 32;  // trailing-update tile
+__global__ __launch_bounds__(256) void k_chol_update(int nc, int bw, int j0, double *__restrict__ A,
+                                                     const int *__restrict__ flag)
+{
+    constexpr int CH_NB = CP_NB;
+    __shared__ double s_i[CU_T][CH_NB + 1], s_k[CU_T][CH_NB + 1];
+    if (*flag) return;
+    const int nb = min(CH_NB, nc - j0);
+    const int w0 = j0 + nb, w1 = min(nc, j0 + nb + bw);  // trailing window rows/columns [w0, w1)
+    const int m = w1 - w0;
+    if (m <= 0) return;
+    const int nt = (m + CU_T - 1) / CU_T;
+    int ta = (int)((sqrt(8.0 * blockIdx.x + 1.0) - 1.0) * 0.5);
+    while ((ta + 1) * (ta + 2) / 2 <= (int)blockIdx.x) ++ta;
+    while (ta * (ta + 1) / 2 > (int)blockIdx.x) --ta;
+    const int tb = (int)blockIdx.x - ta * (ta + 1) / 2;
+    if (ta >= nt) return;  // the grid is sized for the widest window
+    const int i0 = w0 + ta * CU_T, k0 = w0 + tb * CU_T;
+    for (int t = threadIdx.x; t < CU_T * CH_NB; t += 256) {
+        const int r = t / CH_NB, c = t % CH_NB;
+        const int ri = i0 + r, rk = k0 + r;
+        s_i[r][c] = (ri < w1 && c < nb) ? A[(size_t)ri * nc + j0 + c] : 0.0;
+        s_k[r][c] = (rk < w1 && c < nb) ? A[(size_t)rk * nc + j0 + c] : 0.0;
+    }
+    __syncthreads();
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int a = 2 * ty + p, b = 2 * tx + q;
+            const int i = i0 + a, k = k0 + b;
+            if (i < w1 && k < w1 && k <= i) {
+                double acc = 0.0;
+#pragma unroll
+                for (int c = 0; c < CH_NB; ++c) acc += s_i[a][c] * s_k[b][c];
+                A[(size_t)i * nc + k] -= acc;
+            }
+        }
+}
+
 // the band below the diagonal := mean of both triangles (A_c is symmetric up
 // to rounding); one thread per band entry
 __global__ __launch_bounds__(RED_WG) void k_band_sym(int nc, int bw, double *__restrict__ A)
@@ -2098,6 +2238,148 @@ __global__ __launch_bounds__(64 * INV_WAVES) void k_band_inverse(int nc, int bw,
         }
     }
     for (int i = lane; i < nc; i += 64) inv[(size_t)e * nc + i] = y[i];
+}
+
+// Y = L^-1 for the band Cholesky factor L (lower triangle of the row-major
+// nc x nc array, half-bandwidth bw), one wave per column e, written as row e
+// of Yt (Yt[e][i] = Y[i][e]; zero for i < e).  The rows from e on are taken in
+// blocks of 64: lane l holds the accumulator of row r0 + l; the block's
+// unknowns are found in order (lane s's accumulator is final after step s - 1:
+// y_s = t_s / L_ss with the reciprocal formed beforehand, broadcast by
+// read_lane, and every later lane subtracts L[r0 + l][r0 + s] y_s, the row
+// entries of 16 steps loaded ahead).  Then the block's contribution to each of
+// the next bw rows, sum_s L[i][r0 + s] y_s, is one coalesced load of the row's
+// 64 entries (lane s holds y_s), LI_R rows at a time, summed over the wave by
+// one reduce-scatter; the
+// accumulators of those rows wait in a per-wave LDS ring.  Every sum has one
+// fixed order.  bw + 64 <= LI_RING (the host checks).
+constexpr int LI_WAVES = 4;
+constexpr int LI_RING = 512;
+constexpr int LI_R = 32;  // rows of the band update in flight (= 64 / 2: one row per lane pair)
+__global__ __launch_bounds__(64 * LI_WAVES) void k_band_linv(int nc, int bw, const double *__restrict__ L,
+                                                            double *__restrict__ Yt)
+{
+    __shared__ double s_acc[LI_WAVES][LI_RING];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int e = blockIdx.x * LI_WAVES + w;
+    if (e >= nc) return;  // whole waves; no workgroup barrier below
+    double *acc = s_acc[w];
+    for (int q = lane; q < LI_RING; q += 64) acc[q] = 0.0;
+    double *yrow = Yt + (size_t)e * nc;
+    for (int q = lane; q < e; q += 64) yrow[q] = 0.0;
+    wave_sync();
+    for (int r0 = e; r0 < nc; r0 += 64) {
+        const int nb = min(64, nc - r0);
+        const int i = r0 + lane;
+        const bool act = lane < nb;
+        double t = act ? acc[i % LI_RING] : 0.0;
+        if (i == e) t += 1.0;
+        const double rdg = act ? 1.0 / L[(size_t)i * nc + i] : 1.0;
+        wave_sync();
+        if (act) acc[i % LI_RING] = 0.0;  // the slot serves row i + LI_RING next
+        // the diagonal block, 16 steps per round with the row entries loaded first
+        double y = 0.0;
+        for (int s0 = 0; s0 < nb; s0 += 16) {
+            double lv[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int k = r0 + s0 + u;
+                const bool ok = act && s0 + u < lane && i - k <= bw;
+                const double v = L[ok ? (size_t)i * nc + k : 0];
+                lv[u] = ok ? v : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int sidx = s0 + u;
+                if (sidx < nb) {  // uniform
+                    const double ys = read_lane(t, sidx) * read_lane(rdg, sidx);
+                    if (lane == sidx) y = ys;
+                    t -= lv[u] * ys;  // zero for lanes <= sidx
+                }
+            }
+        }
+        if (act) yrow[i] = y;
+        // the block's contribution to the rows below it within the band
+        const int lo = r0 + nb, hi = min(nc - 1, r0 + nb - 1 + bw);
+        const int k = r0 + lane;
+        for (int m0 = lo; m0 <= hi; m0 += LI_R) {  // uniform
+            double part[LI_R];
+#pragma unroll
+            for (int u = 0; u < LI_R; ++u) {
+                const int i2 = m0 + u;
+                const bool ok = act && i2 <= hi && i2 - k <= bw;
+                const double v = L[ok ? (size_t)i2 * nc + k : 0];
+                part[u] = (ok ? v : 0.0) * y;
+            }
+            // reduce-scatter over the wave by recursive halving: at lane distance
+            // 32, 16, 8, 4, 2 a lane keeps half of its values (the half its lane
+            // bit selects) and adds its partner's; the last step adds the pair.
+            // Lanes 2u and 2u + 1 end with the total of row m0 + u.
+#pragma unroll
+            for (int d = 32, h = LI_R / 2; d >= 2; d >>= 1, h >>= 1) {
+                const bool up = (lane & d) != 0;
+#pragma unroll
+                for (int j = 0; j < h; ++j) {
+                    const double send = up ? part[j] : part[h + j];
+                    const double keep = up ? part[h + j] : part[j];
+                    part[j] = keep + __shfl_xor(send, d);
+                }
+            }
+            const double tot = part[0] + __shfl_xor(part[0], 1);
+            const int u = lane >> 1;
+            if ((lane & 1) == 0 && m0 + u <= hi) acc[(m0 + u) % LI_RING] -= tot;
+        }
+        wave_sync();
+    }
+}
+
+// A^-1 = Y^T Y (Y = L^-1, as rows of Yt) in FP32, symmetric: out[a][b] =
+// out[b][a] = sum_{i >= max(a, b)} Yt[a][i] Yt[b][i] in FP64, one 32 x 32
+// tile (b0 <= a0) per 256-thread workgroup, 2 x 2 outputs per thread, Yt
+// staged in LDS 32 columns at a time.
+constexpr int IG_T = 32, IG_K = 32;
+__global__ __launch_bounds__(256) void k_inv_gemm32(int nc, const double *__restrict__ Yt, float *__restrict__ out)
+{
+    __shared__ double sa[IG_K][IG_T + 1], sb[IG_K][IG_T + 1];
+    const int nt = (nc + IG_T - 1) / IG_T;
+    // tile (ta, tb) with tb <= ta from the linear block index
+    int ta = (int)((sqrt(8.0 * blockIdx.x + 1.0) - 1.0) * 0.5);
+    while ((ta + 1) * (ta + 2) / 2 <= (int)blockIdx.x) ++ta;
+    while (ta * (ta + 1) / 2 > (int)blockIdx.x) --ta;
+    const int tb = (int)blockIdx.x - ta * (ta + 1) / 2;
+    if (ta >= nt) return;
+    const int a0 = ta * IG_T, b0 = tb * IG_T;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;  // outputs a0 + 2 ty + p, b0 + 2 tx + q
+    double c[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int i0 = a0; i0 < nc; i0 += IG_K) {  // Yt[a][i] = 0 for i < a0 <= a, and b0 <= a0
+        __syncthreads();
+        for (int t = threadIdx.x; t < IG_K * IG_T; t += 256) {
+            const int r = t / IG_K, k = t % IG_K;  // row of the tile, column i0 + k
+            const int i = i0 + k, ra = a0 + r, rb = b0 + r;
+            sa[k][r] = (ra < nc && i < nc) ? Yt[(size_t)ra * nc + i] : 0.0;
+            sb[k][r] = (rb < nc && i < nc) ? Yt[(size_t)rb * nc + i] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int k = 0; k < IG_K; ++k) {
+            const double a_0 = sa[k][2 * ty], a_1 = sa[k][2 * ty + 1];
+            const double b_0 = sb[k][2 * tx], b_1 = sb[k][2 * tx + 1];
+            c[0][0] += a_0 * b_0;
+            c[0][1] += a_0 * b_1;
+            c[1][0] += a_1 * b_0;
+            c[1][1] += a_1 * b_1;
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int a = a0 + 2 * ty + p, b = b0 + 2 * tx + q;
+            if (a < nc && b < nc) {
+                out[(size_t)a * nc + b] = (float)c[p][q];
+                out[(size_t)b * nc + a] = (float)c[p][q];
+            }
+        }
 }
 
 // symmetric FP32 copy of the inverse
@@ -2340,16 +2622,35 @@ static int coarse_values(ofea_ctx *ctx)
                        w->sl_ptr, w->sl_part, w->sl_cptr, w->ch_ptr, w->ch_beg, w->sl_blk, w->co_A);
     const unsigned gb = (unsigned)(((long long)nc * w->co_bw + RED_WG - 1) / RED_WG);
     hipLaunchKernelGGL(k_band_sym, dim3(gb), dim3(RED_WG), 0, st, nc, w->co_bw, w->co_A);
-    hipLaunchKernelGGL(k_band_cholesky, dim3(1), dim3(CH_WG), 0, st, nc, w->co_bw, w->co_A, w->co_flag);
-    hipLaunchKernelGGL(k_band_upper, dim3(gb), dim3(RED_WG), 0, st, nc, w->co_bw, w->co_A);
+    if (w->co_bw + CP_NB <= CH_ROWS) {  // one panel kernel and one trailing-update kernel per CP_NB columns
+        const int ntu = (w->co_bw + CU_T - 1) / CU_T;
+        hipLaunchKernelGGL(k_chol_lift, dim3(1), dim3(CH_WG), 0, st, nc, w->co_A, w->co_flag);
+        for (int j0 = 0; j0 < nc; j0 += CP_NB) {
+            hipLaunchKernelGGL(k_chol_panel, dim3(1), dim3(CH_WG), 0, st, nc, w->co_bw, j0, w->co_A, w->co_flag);
+            if (j0 + CP_NB < nc)
+                hipLaunchKernelGGL(k_chol_update, dim3(ntu * (ntu + 1) / 2), dim3(256), 0, st, nc, w->co_bw, j0,
+                                   w->co_A, w->co_flag);
+        }
+    } else {
+        hipLaunchKernelGGL(k_band_cholesky, dim3(1), dim3(CH_WG), 0, st, nc, w->co_bw, w->co_A, w->co_flag);
+    }
+    const bool blocked = w->co_bw + 64 <= LI_RING;  // blocked inverse: L^-1 by 64-row blocks, then Y^T Y
+    if (!blocked) hipLaunchKernelGGL(k_band_upper, dim3(gb), dim3(RED_WG), 0, st, nc, w->co_bw, w->co_A);
     OFEA_HIP(hipGetLastError());
     int flag = 0;
     OFEA_HIP(hipMemcpyAsync(&flag, w->co_flag, sizeof(int), hipMemcpyDeviceToHost, st));
     OFEA_HIP(hipStreamSynchronize(st));
     if (flag) return fail(OFEA_ERR_STATE, "coarse matrix Z^T K Z is not positive definite");
-    hipLaunchKernelGGL(k_band_inverse, dim3((nc + INV_WAVES - 1) / INV_WAVES), dim3(64 * INV_WAVES), 0, st, nc,
-                       w->co_bw, w->co_A, w->co_inv);
-    hipLaunchKernelGGL(k_sym32, dim3(g2), dim3(RED_WG), 0, st, nc, w->co_inv, w->co_ainv);
+    if (blocked) {
+        hipLaunchKernelGGL(k_band_linv, dim3((nc + LI_WAVES - 1) / LI_WAVES), dim3(64 * LI_WAVES), 0, st, nc,
+                           w->co_bw, w->co_A, w->co_inv);
+        const int nt = (nc + IG_T - 1) / IG_T;
+        hipLaunchKernelGGL(k_inv_gemm32, dim3(nt * (nt + 1) / 2), dim3(256), 0, st, nc, w->co_inv, w->co_ainv);
+    } else {
+        hipLaunchKernelGGL(k_band_inverse, dim3((nc + INV_WAVES - 1) / INV_WAVES), dim3(64 * INV_WAVES), 0, st, nc,
+                           w->co_bw, w->co_A, w->co_inv);
+        hipLaunchKernelGGL(k_sym32, dim3(g2), dim3(RED_WG), 0, st, nc, w->co_inv, w->co_ainv);
+    }
     OFEA_HIP(hipGetLastError());
     w->co_ainv_host.clear();  // downloaded again by coarse_apply when needed
     return OFEA_OK;
@@ -2369,7 +2670,7 @@ static int coarse_prepare(ofea_ctx *ctx)
 }
 
 // out = Z A_c^-1 Z^T r on the host, with the FP32 offsets and inverse the
-// device applies (training targets of the learned two-level smoother)
+// device applies (a check of the coarse correction)
 int coarse_apply(ofea_ctx *ctx, const double *r, double *out)
 {
     int rc = solver_prepare(ctx);
