@@ -1480,6 +1480,10 @@ static int write_scalars(ofea_ctx *ctx)
     return OFEA_OK;
 }
 
+// Residual drop between two replacements of the recursive residual by the
+// true FP64 one in the methods with FP32 matrix values or vectors.
+constexpr double MIXED_RU = 0.01;
+
 // PCG with reliable updates.  x (FP64, device) starts at 0.  TV/T: matrix and
 // vector precision of the iteration; PC: 0 point Jacobi, 1 block
 // preconditioner, 2 factorized correction.  ru_factor > 0 requests a
@@ -1563,14 +1567,28 @@ static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, Ve
         inflight += nit;
         return OFEA_OK;
     };
-    // with reliable updates (frequent stops) the work in flight is mostly
-    // wasted, so those methods keep one batch at a time
-    const int depth = ru_factor > 0.0 ? 1 : 2;
+    // the prediction below aims at the next stop of either kind (convergence
+    // or a residual replacement), so the replacements of the mixed methods
+    // waste little of what is in flight either
+    const int depth = 2;
+    double lr = 0.0;         // log decay of ||r||^2 per iteration, kept across replacements
+    double rr_start = -1.0;  // ||r||^2 where a round starts, when the host knows it
+    auto top_up = [&](double need) -> int {  // batches until `need` iterations are in flight
+        const int want = (int)std::min<double>(need, (double)depth * CG_BATCH);
+        while (inflight < want && qc < NQ_PIPE - 1)
+            if (int r2 = launch_batch(want - inflight >= CG_BATCH ? CG_BATCH : CG_TAIL)) return r2;
+        return OFEA_OK;
+    };
+    int it_start = 0;  // iteration at which a round starts
     for (;;) {
-        double last_rr = -1.0, lr = 0.0;  // residual at the last completed batch; log decay per iteration
-        int last_it = 0;
-        for (int d = 0; d < depth; ++d)
-            if ((rc = launch_batch(CG_BATCH))) return rc;
+        double last_rr = rr_start;  // residual at the last completed batch (or the round's start)
+        int last_it = it_start;
+        {
+            const double tgt = std::max(S.s[S_TOL2], S.s[S_RU2]);
+            const double need = lr < 0.0 && rr_start > 0.0 && tgt > 0.0 ? std::log(tgt / rr_start) / lr + 1.0 : 1e30;
+            if ((rc = top_up(need))) return rc;
+            if (qc == 0 && (rc = launch_batch(CG_TAIL))) return rc;
+        }
         for (;;) {
             const int hb = qh[q0];
             inflight -= qn[q0];
@@ -1579,16 +1597,15 @@ static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, Ve
             OFEA_HIP(hipEventSynchronize(w->pipe_ev[hb]));
             const DevScalars &D = Hb[hb];
             if (D.done) break;
-            const double rr = D.s[S_RR], tol2 = D.s[S_TOL2];
+            const double rr = D.s[S_RR], tol2 = std::max(D.s[S_TOL2], D.s[S_RU2]);  // the nearer stop
+            if (last_rr < 0.0 && D.s[S_BB] > 0.0) last_rr = D.s[S_BB];  // first round: r_0 = b
             if (last_rr > 0.0 && rr > 0.0 && rr < last_rr && D.it0 > last_it)
                 lr = std::log(rr / last_rr) / (D.it0 - last_it);
             last_rr = rr;
             last_it = D.it0;
             // iterations still needed from this point (unknown: a full pipeline)
             const double need = lr < 0.0 && tol2 > 0.0 ? std::log(tol2 / rr) / lr + 1.0 : 1e30;
-            const int want = (int)std::min<double>(need, (double)depth * CG_BATCH);
-            while (inflight < want && qc < NQ_PIPE - 1)
-                if ((rc = launch_batch(want - inflight >= CG_BATCH ? CG_BATCH : CG_TAIL))) return rc;
+            if ((rc = top_up(need))) return rc;
             if (qc == 0 && (rc = launch_batch(CG_TAIL))) return rc;  // keep going past a wrong prediction
         }
         // the batches still in flight return at once from every kernel; the
@@ -1606,6 +1623,10 @@ static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, Ve
         }
         if (done == 3) return fail(OFEA_ERR_STATE, "PCG breakdown (matrix or preconditioner not SPD, or NaN)");
         const int it_done = S.it, par_done = S.par0 ^ (S.bi_done & 1);
+        {  // the round's own decay, for the next round's first launches
+            const double r0 = rr_start > 0.0 ? rr_start : bb;
+            if (S.s[S_RR] > 0.0 && S.s[S_RR] < r0 && it_done > it_start) lr = std::log(S.s[S_RR] / r0) / (it_done - it_start);
+        }
         rel = std::sqrt(S.s[S_TRR] / bb);
         if (rel <= rtol || done == 2) break;
         if (done == 1 && ++tighten > 8) break;  // recursive and true residual disagree persistently
@@ -1616,6 +1637,8 @@ static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, Ve
         S.done = 0;
         S.ticket = 0;
         S.s[S_RU2] = ru_factor > 0.0 ? ru_factor * ru_factor * S.s[S_TRR] : -1.0;
+        rr_start = S.s[S_TRR];
+        it_start = it_done;
         if ((rc = write_scalars(ctx))) return rc;
         start(0);
         OFEA_HIP(hipGetLastError());
@@ -2753,7 +2776,7 @@ int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, 
                                                   relres);
             break;
         case OFEA_PCG_TWO_LEVEL_MIXED:
-            rc = pcg_run<float, double, 1, true>(ctx, S, 7, w->val32_p, w->v64, w->minv_p, rtol, maxit, 0.1, iters,
+            rc = pcg_run<float, double, 1, true>(ctx, S, 7, w->val32_p, w->v64, w->minv_p, rtol, maxit, MIXED_RU, iters,
                                                  relres);
             break;
         case OFEA_PCG_LEARNED_TWO_LEVEL:
@@ -2761,7 +2784,7 @@ int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, 
                                                   relres);
             break;
         default:
-            rc = pcg_run<float, double, 2, true>(ctx, S, 9, w->val32_p, w->v64, w->minv_p, rtol, maxit, 0.1, iters,
+            rc = pcg_run<float, double, 2, true>(ctx, S, 9, w->val32_p, w->v64, w->minv_p, rtol, maxit, MIXED_RU, iters,
                                                  relres);
         }
         if (rc != OFEA_OK && rc != OFEA_NOT_CONVERGED) return rc;
@@ -2779,16 +2802,16 @@ int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, 
         return pcg_run<double, double, 1, false>(ctx, S, 1, ctx->d_val, w->v64, w->minv, rtol, maxit, -1.0, iters,
                                                  relres);
     case OFEA_PCG_MIXED:
-        return pcg_run<float, double, 1, false>(ctx, S, 2, w->val32, w->v64, w->minv, rtol, maxit, 0.1, iters,
+        return pcg_run<float, double, 1, false>(ctx, S, 2, w->val32, w->v64, w->minv, rtol, maxit, MIXED_RU, iters,
                                                 relres);
     case OFEA_PCG_LEARNED:
         return pcg_run<double, double, 2, false>(ctx, S, 3, ctx->d_val, w->v64, w->minv, rtol, maxit, -1.0, iters,
                                                  relres);
     case OFEA_PCG_LEARNED_MIXED:
-        return pcg_run<float, double, 2, false>(ctx, S, 5, w->val32, w->v64, w->minv, rtol, maxit, 0.1, iters,
+        return pcg_run<float, double, 2, false>(ctx, S, 5, w->val32, w->v64, w->minv, rtol, maxit, MIXED_RU, iters,
                                                 relres);
     case OFEA_PCG_MIXED32:
-        return pcg_run<float, float, 1, false>(ctx, S, 4, w->val32, w->v32, w->minv32, rtol, maxit, 0.1, iters,
+        return pcg_run<float, float, 1, false>(ctx, S, 4, w->val32, w->v32, w->minv32, rtol, maxit, MIXED_RU, iters,
                                                relres);
     default:
         return fail(OFEA_ERR_ARG, "unknown solver method");
