@@ -25,8 +25,13 @@
  *                         with GPU PCG methods beside "Pardiso_*"/"SuperLU_*"
  *   ofea_nl_assemble      assamble_nl_stiffness::fill_CSR_sparse_mat
  *                         (src/nl_assemble.cpp:94-166)
+ *   ofea_nl_solve         Newton loop of src/cae.cpp:209-253 with load increments
+ *   ofea_nl_solve_ref     the same loop with the reference's stopping rule
  *   ofea_lumped_mass      ele_base::build_ele_mass over all elements (src/cae.cpp:439-448)
- *   ofea_explicit_run     CAE_process::explicit_analysis time loop (src/cae.cpp:420-574)
+ *   ofea_explicit_run     CAE_process::explicit_analysis time loop (src/cae.cpp:420-574),
+ *                         fixed step count
+ *   ofea_explicit_analysis the same loop with its time control (auto step, CheckTime, outputs)
+ *   ofea_build_explicit_load OutFroce_ of src/cae.cpp:449-456 (int-truncated value)
  *   ofea_reset_displacement simulation_post::reset_displacement (src/post_process.cpp:18-40)
  *   ofea_ca_info_*        ReadInfo::CA_read_change (src/data2cae.cpp:519-665)
  *   ofea_ca_delta         ca_get_delt_stiffness / ca_fill_CSR_sparse_mat (src/ca_analysis.cpp:18-143)
@@ -238,6 +243,35 @@ int ofea_explicit_run(ofea_ctx *ctx, double E, double nu, const double *mass, co
                       const double *v0, const int *fixed_mask, double dt, int nsteps, int precision,
                       double *u_out);
 
+/* CAE_process::explicit_analysis (src/cae.cpp:420-574) with the reference's
+ * time control.  time_step 0 selects the automatic step, recomputed every
+ * step as 0.9 x the smallest 2 / sqrt(max row sum of |Ke|) over the elements
+ * at the current coordinates (UpdateTimeStep, src/explicit_tools.cpp:21-31;
+ * update_timestep, src/Elements/C3D8.cpp:195-219, src/Elements/C3D4.cpp:140-164);
+ * a given time_step is used as it is and, as in the reference, only lowered
+ * by CheckTime (src/explicit_tools.cpp:48-56), which clamps the step to land
+ * on each multiple of time_total / output_gap (the reference's output_gap_
+ * is 10) and on time_total.  The displacement (3nd) is saved at t = 0 and
+ * whenever the current time equals time_total or the current output time;
+ * the loop ends after the save at time_total or when the time passes it.
+ * Saves go to frames (max_frames x 3nd), with the output index of each
+ * (frame_index, the number of the reference's "<path>_<k>.vtk") and its time
+ * (frame_time); *n_frames counts the saves, *n_steps the loop iterations.
+ * Time control is FP64; precision selects the arithmetic of the element and
+ * node phases.  Inputs as ofea_explicit_run (f_ext from
+ * ofea_build_explicit_load for the reference's single load).  Returns
+ * OFEA_ERR_LIMIT when max_steps iterations end before time_total or the saves
+ * exceed max_frames (the first max_frames are returned). */
+int ofea_explicit_analysis(ofea_ctx *ctx, double E, double nu, const double *mass, const double *f_ext,
+                           const double *v0, const int *fixed_mask, double time_total, double time_step,
+                           int output_gap, int precision, int max_steps, int max_frames, double *frames,
+                           int *frame_index, double *frame_time, int *n_frames, int *n_steps);
+/* External force of the explicit loop (src/cae.cpp:449-456): f_ext (3nd)
+ * zero except f_ext[3 (node - 1) + dof - 1] = (int)value for every node of
+ * load_set (1-based) -- the reference stores the load value in an int, so
+ * its fraction is dropped. */
+int ofea_build_explicit_load(int nd, const int *load_set, int nload, int dof, double value, double *f_ext);
+
 /* ---- geometric nonlinear (total Lagrangian St.Venant-Kirchhoff) ---- */
 /* Tangent K_T (resident, copied to nz_val when non-NULL) and internal force
  * f_int (n, free DOFs) at the free-DOF displacement u (n). */
@@ -257,6 +291,18 @@ int ofea_nl_assemble(ofea_ctx *ctx, double E, double nu, const double *u, double
 int ofea_nl_solve(ofea_ctx *ctx, double E, double nu, const double *F, int n_incr, double tol, int max_newton,
                   int method, double inner_rtol, double *u_out, int *newton_iters, int *linear_iters,
                   double *relres);
+/* The Newton loop of CAE_process::implict_analysis exactly as the reference
+ * runs it (src/cae.cpp:209-253): res_norm = 1; while res_norm > res_limit
+ * (data_management::res_lmit, 1e-6 by default) and loop < max_loop (100 in
+ * the reference): K_T and f_int at u, r = F - f_int, K_T du = r, u += du,
+ * res_norm = ||r||_2 / n of that r (n = 3 x free nodes), loop++.  The full
+ * load at once; the du of the last residual test is applied.  The linear
+ * solves are GPU PCG (method, inner_rtol) in place of PARDISO.  u_out (n),
+ * counters nullable.  Returns OFEA_OK, or OFEA_NOT_CONVERGED when max_loop
+ * loops end with res_norm above res_limit. */
+int ofea_nl_solve_ref(ofea_ctx *ctx, double E, double nu, const double *F, double res_limit, int max_loop,
+                      int method, double inner_rtol, double *u_out, int *loops, int *linear_iters,
+                      double *res_norm);
 
 /* ---- learned factorized correction of block Jacobi (no counterpart in the
  * reference) ----

@@ -78,8 +78,12 @@ SIGNATURES = {
     "ofea_get_solution": (_i, [_vp, _vp]),
     "ofea_lumped_mass": (_i, [_vp, _d, _vp]),
     "ofea_explicit_run": (_i, [_vp, _d, _d, _vp, _vp, _vp, _vp, _d, _i, _i, _vp]),
+    "ofea_explicit_analysis": (_i, [_vp, _d, _d, _vp, _vp, _vp, _vp, _d, _d, _i, _i, _i, _i, _vp, _vp, _vp, _pi,
+                                    _pi]),
+    "ofea_build_explicit_load": (_i, [_i, _vp, _i, _i, _d, _vp]),
     "ofea_nl_assemble": (_i, [_vp, _d, _d, _vp, _vp, _vp]),
     "ofea_nl_solve": (_i, [_vp, _d, _d, _vp, _i, _d, _i, _i, _d, _vp, _pi, _pi, _pd]),
+    "ofea_nl_solve_ref": (_i, [_vp, _d, _d, _vp, _d, _i, _i, _d, _vp, _pi, _pi, _pd]),
     "ofea_learned_nparam": (_i, []),
     "ofea_learned_init": (_i, [_vp, C.c_ulonglong]),
     "ofea_learned_set_params": (_i, [_vp, _vp, _i]),

@@ -1,5 +1,6 @@
 // capi.cpp -- C ABI entry points of libofea_hip.so that own device state.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "ofea_internal.h"
@@ -365,6 +366,24 @@ int ofea_explicit_run(ofea_ctx *c, double E, double nu, const double *mass, cons
     return explicit_run(c, E, nu, mass, f_ext, v0, fixed_mask, dt, nsteps, precision, u_out);
 }
 
+int ofea_explicit_analysis(ofea_ctx *c, double E, double nu, const double *mass, const double *f_ext,
+                           const double *v0, const int *fixed_mask, double time_total, double time_step,
+                           int output_gap, int precision, int max_steps, int max_frames, double *frames,
+                           int *frame_index, double *frame_time, int *n_frames, int *n_steps)
+{
+    CHECK_CTX(c);
+    if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    if (!mass || !f_ext || !fixed_mask) return fail(OFEA_ERR_ARG, "null explicit input");
+    if (!std::isfinite(time_total) || !std::isfinite(time_step) || time_step < 0.0)
+        return fail(OFEA_ERR_ARG, "time_total must be finite and time_step finite and >= 0");
+    if (output_gap < 1 || max_steps < 1 || max_frames < 1)
+        return fail(OFEA_ERR_ARG, "output_gap, max_steps and max_frames must be >= 1");
+    if (precision != OFEA_FP32 && precision != OFEA_FP64) return fail(OFEA_ERR_ARG, "precision must be FP32 or FP64");
+    if ((double)max_frames * 3.0 * c->nd > 4.0e9) return fail(OFEA_ERR_LIMIT, "max_frames x 3 nd too large");
+    return explicit_analysis(c, E, nu, mass, f_ext, v0, fixed_mask, time_total, time_step, output_gap, precision,
+                             max_steps, max_frames, frames, frame_index, frame_time, n_frames, n_steps);
+}
+
 int ofea_nl_assemble(ofea_ctx *c, double E, double nu, const double *u, double *nz_val, double *f_int)
 {
     CHECK_CTX(c);
@@ -384,6 +403,17 @@ int ofea_nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr,
         return fail(OFEA_ERR_ARG, "bad Newton arguments");
     return nl_solve(c, E, nu, F, n_incr, tol, max_newton, method, inner_rtol, u_out, newton_iters, linear_iters,
                     relres);
+}
+
+int ofea_nl_solve_ref(ofea_ctx *c, double E, double nu, const double *F, double res_limit, int max_loop, int method,
+                      double inner_rtol, double *u_out, int *loops, int *linear_iters, double *res_norm)
+{
+    CHECK_CTX(c);
+    if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
+    NOT_SFEM(c);
+    if (!F || !(res_limit >= 0.0) || max_loop < 0 || !(inner_rtol > 0.0))
+        return fail(OFEA_ERR_ARG, "bad Newton arguments");
+    return nl_solve_ref(c, E, nu, F, res_limit, max_loop, method, inner_rtol, u_out, loops, linear_iters, res_norm);
 }
 
 int ofea_ca_delta(ofea_ctx *c, double E, double nu, int nmod, const int *kind, const int *conn_mod,

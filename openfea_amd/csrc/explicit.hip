@@ -28,6 +28,19 @@ namespace ofea {
 
 constexpr int EX_WG = 256;
 constexpr int EX_GRAPH_STEPS = 50;  // steps per captured graph (even)
+constexpr int DT_HEX = 32;          // hexes per workgroup of the critical-step kernel (8 lanes each)
+constexpr int DT_LDS = 201;         // doubles per hex in LDS: 8 points x (24 gradients + detJ), +1 pad
+
+// Time control of the reference's loop (src/cae.cpp:420-574), kept on the
+// device and advanced by one thread per step (k_time_ctl).  All times FP64.
+struct TimeCtl {
+    double t, dt, dt_old, total;  // current time, step (DBL_MAX before an automatic update), previous step
+    double cv, cu;                // this step's velocity factor (dt_old + dt)/2 and displacement step dt
+    int out, gap, auto_dt, steps, max_steps;
+    int done;                     // loop ended: kernels of later steps return at once
+    int node_go;                  // the node kernel of this step runs
+    int save, slot, nsaved, max_frames, overflow;
+};
 
 struct ExplicitWS {
     int nd = 0;
@@ -42,6 +55,17 @@ struct ExplicitWS {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     double g_E = 0, g_nu = 0, g_dt = 0;
+    // reference time control (ofea_explicit_analysis)
+    TimeCtl *tc = nullptr;                   // device state
+    double *dtpart = nullptr;                // per-workgroup critical steps
+    void *frames = nullptr;                  // saved displacements, run precision
+    int *fidx = nullptr;                     // output index of each save
+    double *ft = nullptr;                    // time of each save
+    int frames_cap = 0;
+    hipGraph_t tgraph = nullptr;
+    hipGraphExec_t texec = nullptr;
+    double tg_E = 0, tg_nu = 0;
+    int tg_auto = -1;
 };
 
 static ExplicitWS *ews(ofea_ctx *c) { return (ExplicitWS *)c->explicit_ws; }
@@ -52,8 +76,11 @@ void explicit_free(ofea_ctx *c)
     if (!w) return;
     if (w->exec) hipGraphExecDestroy(w->exec);
     if (w->graph) hipGraphDestroy(w->graph);
+    if (w->texec) hipGraphExecDestroy(w->texec);
+    if (w->tgraph) hipGraphDestroy(w->tgraph);
     for (void *p : {(void *)w->inc_ptr, (void *)w->inc, w->xyz_t, w->fe[0], w->fe[1], w->u[0], w->u[1], w->v, w->minv,
-                    w->fext, (void *)w->fixed})
+                    w->fext, (void *)w->fixed, (void *)w->tc, (void *)w->dtpart, w->frames, (void *)w->fidx,
+                    (void *)w->ft})
         if (p) hipFree(p);
     delete w;
     c->explicit_ws = nullptr;
@@ -180,8 +207,10 @@ int lumped_mass(ofea_ctx *c, double rho, double *mass)
 // ------------------------------------------------------------------ forces
 template <class T>
 __global__ __launch_bounds__(EX_WG) void k_force_hex(int ne, const int *__restrict__ conn, const T *__restrict__ xyz,
-                                                     const T *__restrict__ u, T lam, T mu, T *__restrict__ fe)
+                                                     const T *__restrict__ u, T lam, T mu, T *__restrict__ fe,
+                                                     const int *__restrict__ stop)
 {
+    if (stop && *stop) return;
     const int gid = blockIdx.x * EX_WG + threadIdx.x;
     const int e_raw = gid >> 3, gp = gid & 7;
     const int e = min(e_raw, ne - 1);  // whole 8-lane groups stay in the shuffles
@@ -231,8 +260,10 @@ __global__ __launch_bounds__(EX_WG) void k_force_hex(int ne, const int *__restri
 
 template <class T>
 __global__ __launch_bounds__(EX_WG) void k_force_tet(int ne, const int *__restrict__ conn, const T *__restrict__ xyz,
-                                                     const T *__restrict__ u, T lam, T mu, T *__restrict__ fe)
+                                                     const T *__restrict__ u, T lam, T mu, T *__restrict__ fe,
+                                                     const int *__restrict__ stop)
 {
+    if (stop && *stop) return;
     const int e = blockIdx.x * EX_WG + threadIdx.x;
     if (e >= ne) return;
     const int *cn = conn + 4 * e;
@@ -308,25 +339,43 @@ __global__ void k_explicit_init(int nd, T *__restrict__ v, T *__restrict__ u1, c
 }
 
 template <class T>
-static void launch_step(ofea_ctx *c, ExplicitWS *w, T lam, T mu, T dt, const T *ucur, T *unext)
+static void launch_forces(ofea_ctx *c, ExplicitWS *w, T lam, T mu, const T *ucur, const int *stop)
 {
     if (w->ne[1])
         hipLaunchKernelGGL(k_force_hex<T>, dim3((8 * w->ne[1] + EX_WG - 1) / EX_WG), dim3(EX_WG), 0, c->stream,
-                           w->ne[1], w->conn[1], (const T *)w->xyz_t, ucur, lam, mu, (T *)w->fe[1]);
+                           w->ne[1], w->conn[1], (const T *)w->xyz_t, ucur, lam, mu, (T *)w->fe[1], stop);
     if (w->ne[0])
         hipLaunchKernelGGL(k_force_tet<T>, dim3((w->ne[0] + EX_WG - 1) / EX_WG), dim3(EX_WG), 0, c->stream, w->ne[0],
-                           w->conn[0], (const T *)w->xyz_t, ucur, lam, mu, (T *)w->fe[0]);
+                           w->conn[0], (const T *)w->xyz_t, ucur, lam, mu, (T *)w->fe[0], stop);
+}
+
+template <class T>
+static void launch_step(ofea_ctx *c, ExplicitWS *w, T lam, T mu, T dt, const T *ucur, T *unext)
+{
+    launch_forces<T>(c, w, lam, mu, ucur, nullptr);
     hipLaunchKernelGGL(k_node_update<T>, dim3((w->nd + EX_WG - 1) / EX_WG), dim3(EX_WG), 0, c->stream, w->nd,
                        w->inc_ptr, w->inc, w->ne[0], w->ne[1], (const T *)w->fe[0], (const T *)w->fe[1], ucur, unext,
                        (T *)w->v,
                        (const T *)w->minv, (const T *)w->fext, w->fixed, dt);
 }
 
-template <class T>
-static int run_t(ofea_ctx *c, double E, double nu, const double *mass, const double *f_ext, const double *v0,
-                 const int *fixed_mask, double dt, int nsteps, double *u_out)
+static void drop_graphs(ExplicitWS *w)
 {
-    ExplicitWS *w = ews(c);
+    if (w->exec) hipGraphExecDestroy(w->exec);
+    if (w->graph) hipGraphDestroy(w->graph);
+    if (w->texec) hipGraphExecDestroy(w->texec);
+    if (w->tgraph) hipGraphDestroy(w->tgraph);
+    w->exec = nullptr;
+    w->graph = nullptr;
+    w->texec = nullptr;
+    w->tgraph = nullptr;
+}
+
+// Buffers in the run precision and the inputs converted to it.
+template <class T>
+static int upload_inputs(ofea_ctx *c, ExplicitWS *w, const double *mass, const double *f_ext, const double *v0,
+                         const int *fixed_mask)
+{
     const int nd = c->nd;
     const size_t n3 = 3 * (size_t)nd;
     for (int i = 0; i < nd; ++i)
@@ -336,15 +385,13 @@ static int run_t(ofea_ctx *c, double E, double nu, const double *mass, const dou
     // steps still read v, fext or minv must have finished first.
     OFEA_HIP(hipStreamSynchronize(c->stream));
     if (w->prec != (int)sizeof(T)) {
-        if (w->exec) hipGraphExecDestroy(w->exec);
-        if (w->graph) hipGraphDestroy(w->graph);
-        w->exec = nullptr;
-        w->graph = nullptr;
-        for (void **p : {&w->xyz_t, &w->fe[0], &w->fe[1], &w->u[0], &w->u[1], &w->v, &w->minv, &w->fext})
+        drop_graphs(w);
+        for (void **p : {&w->xyz_t, &w->fe[0], &w->fe[1], &w->u[0], &w->u[1], &w->v, &w->minv, &w->fext, &w->frames})
             if (*p) {
                 hipFree(*p);
                 *p = nullptr;
             }
+        w->frames_cap = 0;
         OFEA_HIP(hipMalloc(&w->xyz_t, sizeof(T) * n3));
         OFEA_HIP(hipMalloc(&w->fe[0], sizeof(T) * std::max(1, 12 * w->ne[0])));
         OFEA_HIP(hipMalloc(&w->fe[1], sizeof(T) * std::max(1, 24 * w->ne[1])));
@@ -368,6 +415,18 @@ static int run_t(ofea_ctx *c, double E, double nu, const double *mass, const dou
     for (int i = 0; i < nd; ++i) mi[i] = (T)(1.0 / mass[i]);
     OFEA_HIP(hipMemcpy(w->minv, mi.data(), sizeof(T) * nd, hipMemcpyHostToDevice));
     OFEA_HIP(hipMemcpy(w->fixed, fixed_mask, sizeof(int) * nd, hipMemcpyHostToDevice));
+    return OFEA_OK;
+}
+
+template <class T>
+static int run_t(ofea_ctx *c, double E, double nu, const double *mass, const double *f_ext, const double *v0,
+                 const int *fixed_mask, double dt, int nsteps, double *u_out)
+{
+    ExplicitWS *w = ews(c);
+    const int nd = c->nd;
+    const size_t n3 = 3 * (size_t)nd;
+    int rc = upload_inputs<T>(c, w, mass, f_ext, v0, fixed_mask);
+    if (rc) return rc;
     double lam_d, mu_d;
     lame<double>(E, nu, lam_d, mu_d);
     const T lam = (T)lam_d, mu = (T)mu_d, tdt = (T)dt;
@@ -407,11 +466,440 @@ static int run_t(ofea_ctx *c, double E, double nu, const double *mass, const dou
         if (nsteps == 0) {
             std::fill(u_out, u_out + n3, 0.0);
         } else {
+            std::vector<T> h(n3);
             OFEA_HIP(hipMemcpyAsync(h.data(), ures, sizeof(T) * n3, hipMemcpyDeviceToHost, c->stream));
             OFEA_HIP(hipStreamSynchronize(c->stream));
             for (size_t i = 0; i < n3; ++i) u_out[i] = (double)h[i];
         }
     }
+    return OFEA_OK;
+}
+
+// ------------------------------------------------------------------ reference time control
+// min in the form of update_timestep (src/Elements/C3D8.cpp:215-218): the
+// step is lowered only by a smaller value, so a NaN candidate is skipped.
+__device__ __forceinline__ double dt_lower(double dt, double c) { return dt > c ? c : dt; }
+
+// Critical step 2 / sqrt(max_v) of each hex at the current coordinates X + u
+// (update_timestep, src/Elements/C3D8.cpp:195-219: max_v = largest row sum of
+// |Ke|, starting at DBL_MIN), lowered over the workgroup's hexes into
+// part[block].  FP64 whatever the run precision.  Lanes (hex, Gauss point)
+// first put the point's gradients and detJ in LDS; then lane (hex, node a)
+// forms the rows of node a: with A_ab = sum_p detJ_p g_a(p) g_b(p)^T,
+// K_ab = lam A_ab + mu A_ab^T + mu tr(A_ab) I.
+template <class T>
+__global__ __launch_bounds__(EX_WG) void k_dt_hex(int ne, const int *__restrict__ conn, const double *__restrict__ xyz,
+                                                  const T *__restrict__ u, double lam, double mu,
+                                                  const TimeCtl *__restrict__ tc, double *__restrict__ part)
+{
+    if (tc->done) return;
+    __shared__ double s_g[DT_HEX * DT_LDS];
+    __shared__ double s_min[EX_WG / 64];
+    const int tid = threadIdx.x, el = tid >> 3, l = tid & 7;
+    const int e = blockIdx.x * DT_HEX + el;
+    const int ec = min(e, ne - 1);
+    double X[24];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const int node = conn[8 * ec + b];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) X[3 * b + k] = xyz[3 * node + k] + (double)u[3 * node + k];
+    }
+    {
+        double g[3][8];
+        const double detJ = hex_point<double>(X, l, g);
+        double *dst = s_g + el * DT_LDS + l * 25;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int b = 0; b < 8; ++b) dst[8 * k + b] = g[k][b];
+        dst[24] = detJ;
+    }
+    __syncthreads();
+    const double *src = s_g + el * DT_LDS;
+    const int a = l;
+    double wga[8][3];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+        const double w = src[25 * p + 24];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) wga[p][k] = w * src[25 * p + 8 * k + a];
+    }
+    double rs[3] = {0.0, 0.0, 0.0};
+    for (int b = 0; b < 8; ++b) {
+        double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+            const double gb0 = src[25 * p + b], gb1 = src[25 * p + 8 + b], gb2 = src[25 * p + 16 + b];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                A[i][0] += wga[p][i] * gb0;
+                A[i][1] += wga[p][i] * gb1;
+                A[i][2] += wga[p][i] * gb2;
+            }
+        }
+        const double tr = A[0][0] + A[1][1] + A[2][2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) rs[i] += fabs(lam * A[i][j] + mu * A[j][i] + (i == j ? mu * tr : 0.0));
+    }
+    double mv = 2.2250738585072014e-308;  // DBL_MIN
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (mv < rs[i]) mv = rs[i];
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+        const double other = __shfl_xor(mv, o);
+        if (mv < other) mv = other;
+    }
+    double dt = e < ne ? 2 / sqrt(mv) : 1.7976931348623157e308;
+#pragma unroll
+    for (int o = 8; o < 64; o <<= 1) dt = dt_lower(dt, __shfl_xor(dt, o));
+    if ((tid & 63) == 0) s_min[tid >> 6] = dt;
+    __syncthreads();
+    if (tid == 0) {
+        double m = s_min[0];
+        for (int k = 1; k < EX_WG / 64; ++k) m = dt_lower(m, s_min[k]);
+        part[blockIdx.x] = m;
+    }
+}
+
+// The same for tets (src/Elements/C3D4.cpp:140-164), one lane per element:
+// K_ab = V (lam g_a g_b^T + mu g_b g_a^T + mu (g_a . g_b) I), V = detJ / 6.
+template <class T>
+__global__ __launch_bounds__(EX_WG) void k_dt_tet(int ne, const int *__restrict__ conn, const double *__restrict__ xyz,
+                                                  const T *__restrict__ u, double lam, double mu,
+                                                  const TimeCtl *__restrict__ tc, double *__restrict__ part)
+{
+    if (tc->done) return;
+    __shared__ double s_min[EX_WG / 64];
+    const int tid = threadIdx.x, e = blockIdx.x * EX_WG + tid;
+    const int ec = min(e, ne - 1);
+    double X[12];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int node = conn[4 * ec + b];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) X[3 * b + k] = xyz[3 * node + k] + (double)u[3 * node + k];
+    }
+    double g[3][4];
+    const double V = tet_grads<double>(X, g) / 6.0;
+    double mv = 2.2250738585072014e-308;  // DBL_MIN
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        double rs[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const double dot = g[0][a] * g[0][b] + g[1][a] * g[1][b] + g[2][a] * g[2][b];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    rs[i] += fabs(V * (lam * g[i][a] * g[j][b] + mu * g[j][a] * g[i][b] + (i == j ? mu * dot : 0.0)));
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            if (mv < rs[i]) mv = rs[i];
+    }
+    double dt = e < ne ? 2 / sqrt(mv) : 1.7976931348623157e308;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) dt = dt_lower(dt, __shfl_xor(dt, o));
+    if ((tid & 63) == 0) s_min[tid >> 6] = dt;
+    __syncthreads();
+    if (tid == 0) {
+        double m = s_min[0];
+        for (int k = 1; k < EX_WG / 64; ++k) m = dt_lower(m, s_min[k]);
+        part[blockIdx.x] = m;
+    }
+}
+
+// CheckTime (src/explicit_tools.cpp:48-56)
+__device__ __forceinline__ bool check_time(double &dt, double limit)
+{
+    if (dt > limit) {
+        dt = limit;
+        return true;
+    }
+    return false;
+}
+
+// One step of the reference's time control, or with init the set-up before
+// its loop (src/cae.cpp:457-499).  Loop step (src/cae.cpp:501-570): leave when
+// t > total; the step is the automatic one (0.9 x the lowest critical step)
+// or the kept fixed one; CheckTime clamps it to the next output time
+// (counting the output) and to the total; the node kernel then updates with
+// v += a (dt_old + dt)/2, u += v dt; the displacement is saved when t is the
+// total or the current output time; after the save at the total the loop
+// ends, else t += dt and the step moves to dt_old.
+__global__ __launch_bounds__(EX_WG) void k_time_ctl(TimeCtl *__restrict__ tc, const double *__restrict__ part,
+                                                    int npart, int init, int *__restrict__ fidx,
+                                                    double *__restrict__ ft)
+{
+    __shared__ double s_min[EX_WG];
+    if (tc->done) {
+        if (threadIdx.x == 0) tc->node_go = 0;
+        return;
+    }
+    const int tid = threadIdx.x;
+    double m = 1.7976931348623157e308;
+    if (tc->auto_dt) {
+        for (int k = tid; k < npart; k += EX_WG) m = dt_lower(m, part[k]);
+        s_min[tid] = m;
+        __syncthreads();
+        for (int s = EX_WG / 2; s > 0; s >>= 1) {
+            if (tid < s) s_min[tid] = dt_lower(s_min[tid], s_min[tid + s]);
+            __syncthreads();
+        }
+        m = s_min[0];
+    }
+    if (tid != 0) return;
+    TimeCtl S = *tc;
+    if (init) {
+        double dt = S.dt;  // DBL_MAX for the automatic step, else the given one
+        if (S.auto_dt) dt = dt_lower(dt, m) * 0.9;
+        S.cv = (0.0 + dt) / 2;  // time_step_old is 0 before the loop
+        S.cu = dt;
+        S.out = 0;
+        S.steps = 0;
+        S.save = 0;
+        S.node_go = 0;
+        S.overflow = 0;
+        S.nsaved = 1;  // the undeformed state at t = 0 (output 0)
+        if (S.max_frames > 0) {
+            fidx[0] = 0;
+            ft[0] = 0.0;
+        }
+        S.t = 0.0 + dt;
+        S.dt_old = dt;
+        S.dt = S.auto_dt ? 1.7976931348623157e308 : dt;
+        *tc = S;
+        return;
+    }
+    if (!(S.t <= S.total) || S.steps >= S.max_steps) {
+        S.overflow = (S.t <= S.total) ? 1 : 0;
+        S.done = 1;
+        S.node_go = 0;
+        S.save = 0;
+        *tc = S;
+        return;
+    }
+    S.steps++;
+    double dt = S.dt;
+    if (S.auto_dt) dt = dt_lower(dt, m) * 0.9;
+    if (check_time(dt, (S.total / S.gap) * (S.out + 1) - S.t)) S.out++;
+    check_time(dt, S.total - S.t);
+    S.cv = (S.dt_old + dt) / 2;
+    S.cu = dt;
+    S.node_go = 1;
+    S.save = 0;
+    if (S.t == S.total || S.t == (S.total / S.gap) * (S.out)) {
+        if (S.nsaved < S.max_frames) {
+            fidx[S.nsaved] = S.out;
+            ft[S.nsaved] = S.t;
+            S.save = 1;
+            S.slot = S.nsaved;
+        }
+        S.nsaved++;
+    }
+    if (S.t == S.total) {
+        S.done = 1;
+    } else {
+        S.t += dt;
+        S.dt_old = dt;
+        S.dt = S.auto_dt ? 1.7976931348623157e308 : dt;
+    }
+    *tc = S;
+}
+
+// Node phase under the time control: f = -sum fe, v += (f + f_ext)/m cv,
+// u_next = u + v cu (0 at clamped nodes); u is saved to its frame slot first
+// when this step saves.
+template <class T>
+__global__ __launch_bounds__(EX_WG) void k_node_update_tc(int nd, const int *__restrict__ inc_ptr,
+                                                          const int *__restrict__ inc, int ne4, int ne8,
+                                                          const T *__restrict__ fe4, const T *__restrict__ fe8,
+                                                          const T *__restrict__ ucur, T *__restrict__ unext,
+                                                          T *__restrict__ v, const T *__restrict__ minv,
+                                                          const T *__restrict__ fext, const int *__restrict__ fixed,
+                                                          const TimeCtl *__restrict__ tc, T *__restrict__ frames)
+{
+    if (!tc->node_go) return;
+    const int i = blockIdx.x * EX_WG + threadIdx.x;
+    if (i >= nd) return;
+    const T cv = (T)tc->cv, cu = (T)tc->cu;
+    T f0 = T(0), f1 = T(0), f2 = T(0);
+    for (int t = inc_ptr[i]; t < inc_ptr[i + 1]; ++t) {
+        int code = inc[t], el = code >> 4, type = (code >> 3) & 1, a = code & 7;
+        const T *src = type ? fe8 + 3 * ((size_t)a * ne8 + el) : fe4 + 3 * ((size_t)a * ne4 + el);
+        f0 -= src[0];
+        f1 -= src[1];
+        f2 -= src[2];
+    }
+    const T u0 = ucur[3 * i], u1 = ucur[3 * i + 1], u2 = ucur[3 * i + 2];
+    if (tc->save) {
+        T *fr = frames + (size_t)tc->slot * 3 * nd + 3 * i;
+        fr[0] = u0;
+        fr[1] = u1;
+        fr[2] = u2;
+    }
+    const T mi = minv[i];
+    const T v0 = v[3 * i] + (f0 + fext[3 * i]) * mi * cv;
+    const T v1 = v[3 * i + 1] + (f1 + fext[3 * i + 1]) * mi * cv;
+    const T v2 = v[3 * i + 2] + (f2 + fext[3 * i + 2]) * mi * cv;
+    v[3 * i] = v0;
+    v[3 * i + 1] = v1;
+    v[3 * i + 2] = v2;
+    const bool fx = fixed[i] != 0;
+    unext[3 * i] = fx ? T(0) : u0 + v0 * cu;
+    unext[3 * i + 1] = fx ? T(0) : u1 + v1 * cu;
+    unext[3 * i + 2] = fx ? T(0) : u2 + v2 * cu;
+}
+
+// Before the loop: v_1/2 = v0 + (f_ext/m) (0 + dt)/2 (the internal force of
+// the undeformed state is 0), u_1 = v_1/2 dt (0 at clamped nodes), frame 0 = 0.
+template <class T>
+__global__ void k_explicit_init_tc(int nd, T *__restrict__ v, T *__restrict__ u1, const T *__restrict__ minv,
+                                   const T *__restrict__ fext, const int *__restrict__ fixed,
+                                   const TimeCtl *__restrict__ tc, T *__restrict__ frames)
+{
+    const int i = blockIdx.x * EX_WG + threadIdx.x;
+    if (i >= nd) return;
+    const T cv = (T)tc->cv, cu = (T)tc->cu;
+    const bool f0 = tc->max_frames > 0;
+    for (int c = 0; c < 3; ++c) {
+        T vv = v[3 * i + c] + fext[3 * i + c] * minv[i] * cv;
+        v[3 * i + c] = vv;
+        u1[3 * i + c] = fixed[i] ? T(0) : vv * cu;
+        if (f0) frames[3 * i + c] = T(0);
+    }
+}
+
+template <class T>
+static void launch_dt(ofea_ctx *c, ExplicitWS *w, double lam, double mu, const T *ucur)
+{
+    const int nb8 = (w->ne[1] + DT_HEX - 1) / DT_HEX;
+    if (w->ne[1])
+        hipLaunchKernelGGL(k_dt_hex<T>, dim3(nb8), dim3(EX_WG), 0, c->stream, w->ne[1], w->conn[1], c->d_xyz, ucur,
+                           lam, mu, (const TimeCtl *)w->tc, w->dtpart);
+    if (w->ne[0])
+        hipLaunchKernelGGL(k_dt_tet<T>, dim3((w->ne[0] + EX_WG - 1) / EX_WG), dim3(EX_WG), 0, c->stream, w->ne[0],
+                           w->conn[0], c->d_xyz, ucur, lam, mu, (const TimeCtl *)w->tc, w->dtpart + nb8);
+}
+
+static int dt_parts(const ExplicitWS *w)
+{
+    return (w->ne[1] + DT_HEX - 1) / DT_HEX + (w->ne[0] + EX_WG - 1) / EX_WG;
+}
+
+template <class T>
+static void launch_step_tc(ofea_ctx *c, ExplicitWS *w, T lam, T mu, double lam_d, double mu_d, bool auto_dt,
+                           const T *ucur, T *unext)
+{
+    launch_forces<T>(c, w, lam, mu, ucur, &w->tc->done);
+    if (auto_dt) launch_dt<T>(c, w, lam_d, mu_d, ucur);
+    hipLaunchKernelGGL(k_time_ctl, dim3(1), dim3(EX_WG), 0, c->stream, w->tc, (const double *)w->dtpart,
+                       dt_parts(w), 0, w->fidx, w->ft);
+    hipLaunchKernelGGL(k_node_update_tc<T>, dim3((w->nd + EX_WG - 1) / EX_WG), dim3(EX_WG), 0, c->stream, w->nd,
+                       w->inc_ptr, w->inc, w->ne[0], w->ne[1], (const T *)w->fe[0], (const T *)w->fe[1], ucur, unext,
+                       (T *)w->v, (const T *)w->minv, (const T *)w->fext, w->fixed, (const TimeCtl *)w->tc,
+                       (T *)w->frames);
+}
+
+template <class T>
+static int run_timed(ofea_ctx *c, double E, double nu, const double *mass, const double *f_ext, const double *v0,
+                     const int *fixed_mask, double time_total, double time_step, int output_gap, int max_steps,
+                     int max_frames, double *frames, int *frame_index, double *frame_time, int *n_frames,
+                     int *n_steps)
+{
+    ExplicitWS *w = ews(c);
+    const int nd = c->nd;
+    const size_t n3 = 3 * (size_t)nd;
+    int rc = upload_inputs<T>(c, w, mass, f_ext, v0, fixed_mask);
+    if (rc) return rc;
+    if (!w->tc) OFEA_HIP(hipMalloc((void **)&w->tc, sizeof(TimeCtl)));
+    if (!w->dtpart) OFEA_HIP(hipMalloc((void **)&w->dtpart, sizeof(double) * std::max(1, dt_parts(w))));
+    const int cap = std::max(1, max_frames);
+    if (w->frames_cap < cap) {
+        drop_graphs(w);  // the captured node kernels hold the frame buffer's address
+        for (void **p : {&w->frames, (void **)&w->fidx, (void **)&w->ft})
+            if (*p) {
+                hipFree(*p);
+                *p = nullptr;
+            }
+        OFEA_HIP(hipMalloc(&w->frames, sizeof(T) * n3 * cap));
+        OFEA_HIP(hipMalloc((void **)&w->fidx, sizeof(int) * cap));
+        OFEA_HIP(hipMalloc((void **)&w->ft, sizeof(double) * cap));
+        w->frames_cap = cap;
+    }
+    const bool auto_dt = time_step == 0.0;
+    TimeCtl S{};
+    S.dt = auto_dt ? 1.7976931348623157e308 : time_step;
+    S.total = time_total;
+    S.gap = output_gap;
+    S.auto_dt = auto_dt ? 1 : 0;
+    S.max_steps = max_steps;
+    S.max_frames = max_frames;
+    OFEA_HIP(hipMemcpy(w->tc, &S, sizeof(TimeCtl), hipMemcpyHostToDevice));
+    double lam_d, mu_d;
+    lame<double>(E, nu, lam_d, mu_d);
+    const T lam = (T)lam_d, mu = (T)mu_d;
+    // set-up before the loop: the automatic step of the undeformed mesh (u = 0)
+    OFEA_HIP(hipMemsetAsync(w->u[1], 0, sizeof(T) * n3, c->stream));
+    if (auto_dt) launch_dt<T>(c, w, lam_d, mu_d, (const T *)w->u[1]);
+    hipLaunchKernelGGL(k_time_ctl, dim3(1), dim3(EX_WG), 0, c->stream, w->tc, (const double *)w->dtpart,
+                       dt_parts(w), 1, w->fidx, w->ft);
+    hipLaunchKernelGGL(k_explicit_init_tc<T>, dim3((nd + EX_WG - 1) / EX_WG), dim3(EX_WG), 0, c->stream, nd,
+                       (T *)w->v, (T *)w->u[0], (const T *)w->minv, (const T *)w->fext, w->fixed,
+                       (const TimeCtl *)w->tc, (T *)w->frames);
+    OFEA_HIP(hipGetLastError());
+    if (w->texec && (w->tg_E != E || w->tg_nu != nu || w->tg_auto != (int)auto_dt)) {
+        hipGraphExecDestroy(w->texec);
+        hipGraphDestroy(w->tgraph);
+        w->texec = nullptr;
+        w->tgraph = nullptr;
+    }
+    if (!w->texec) {
+        OFEA_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int s = 0; s < EX_GRAPH_STEPS; ++s)
+            launch_step_tc<T>(c, w, lam, mu, lam_d, mu_d, auto_dt, (const T *)w->u[s & 1], (T *)w->u[(s + 1) & 1]);
+        OFEA_HIP(hipStreamEndCapture(c->stream, &w->tgraph));
+        OFEA_HIP(hipGraphInstantiate(&w->texec, w->tgraph, nullptr, nullptr, 0));
+        w->tg_E = E;
+        w->tg_nu = nu;
+        w->tg_auto = auto_dt ? 1 : 0;
+    }
+    // Graphs run until the device reports the end; steps after it return at
+    // once.  The host reads the state after each batch and sizes the next
+    // batch from the remaining time and the last step.
+    int batch = 1;
+    for (;;) {
+        for (int g = 0; g < batch; ++g) OFEA_HIP(hipGraphLaunch(w->texec, c->stream));
+        OFEA_HIP(hipMemcpyAsync(&S, w->tc, sizeof(TimeCtl), hipMemcpyDeviceToHost, c->stream));
+        OFEA_HIP(hipStreamSynchronize(c->stream));
+        if (S.done) break;
+        const double rem = S.dt_old > 0.0 ? (S.total - S.t) / S.dt_old : 0.0;
+        const double graphs = rem / EX_GRAPH_STEPS + output_gap / (double)EX_GRAPH_STEPS + 1.0;
+        batch = (int)std::min(64.0, std::max(1.0, std::ceil(graphs)));
+        const int left = (max_steps - S.steps + EX_GRAPH_STEPS - 1) / EX_GRAPH_STEPS + 1;
+        batch = std::min(batch, std::max(1, left));
+    }
+    if (n_steps) *n_steps = S.steps;
+    if (n_frames) *n_frames = S.nsaved;
+    const int keep = std::min(S.nsaved, max_frames);
+    if (keep > 0) {
+        std::vector<T> h(n3 * keep);
+        OFEA_HIP(hipMemcpyAsync(h.data(), w->frames, sizeof(T) * n3 * keep, hipMemcpyDeviceToHost, c->stream));
+        if (frame_index)
+            OFEA_HIP(hipMemcpyAsync(frame_index, w->fidx, sizeof(int) * keep, hipMemcpyDeviceToHost, c->stream));
+        if (frame_time)
+            OFEA_HIP(hipMemcpyAsync(frame_time, w->ft, sizeof(double) * keep, hipMemcpyDeviceToHost, c->stream));
+        OFEA_HIP(hipStreamSynchronize(c->stream));
+        if (frames)
+            for (size_t i = 0; i < n3 * keep; ++i) frames[i] = (double)h[i];
+    }
+    if (S.overflow) return fail(OFEA_ERR_LIMIT, "explicit time loop reached max_steps before time_total");
+    if (S.nsaved > max_frames) return fail(OFEA_ERR_LIMIT, "more outputs than max_frames (the first ones are kept)");
     return OFEA_OK;
 }
 
@@ -422,6 +910,20 @@ int explicit_run(ofea_ctx *c, double E, double nu, const double *mass, const dou
     if (rc) return rc;
     if (precision == OFEA_FP32) return run_t<float>(c, E, nu, mass, f_ext, v0, fixed_mask, dt, nsteps, u_out);
     return run_t<double>(c, E, nu, mass, f_ext, v0, fixed_mask, dt, nsteps, u_out);
+}
+
+int explicit_analysis(ofea_ctx *c, double E, double nu, const double *mass, const double *f_ext, const double *v0,
+                      const int *fixed_mask, double time_total, double time_step, int output_gap, int precision,
+                      int max_steps, int max_frames, double *frames, int *frame_index, double *frame_time,
+                      int *n_frames, int *n_steps)
+{
+    int rc = ensure_incidence(c);
+    if (rc) return rc;
+    if (precision == OFEA_FP32)
+        return run_timed<float>(c, E, nu, mass, f_ext, v0, fixed_mask, time_total, time_step, output_gap, max_steps,
+                                max_frames, frames, frame_index, frame_time, n_frames, n_steps);
+    return run_timed<double>(c, E, nu, mass, f_ext, v0, fixed_mask, time_total, time_step, output_gap, max_steps,
+                             max_frames, frames, frame_index, frame_time, n_frames, n_steps);
 }
 
 }  // namespace ofea

@@ -698,6 +698,21 @@ int ofea_build_single_load(int n, const int *resort, const int *load_set, int nl
     return OFEA_OK;
 }
 
+int ofea_build_explicit_load(int nd, const int *load_set, int nload, int dof, double value, double *f_ext)
+{
+    if (nd < 0 || !f_ext || nload < 0 || (nload > 0 && !load_set)) return fail(OFEA_ERR_ARG, "bad argument");
+    std::fill(f_ext, f_ext + 3 * (size_t)nd, 0.0);
+    if (dof < 1 || dof > 3) return fail(OFEA_ERR_ARG, "the DOF of the load node must be 1, 2 or 3");
+    if (!(value > -2147483649.0 && value < 2147483648.0)) return fail(OFEA_ERR_ARG, "load value outside the int range");
+    const int force_value = (int)value;  // int, as the reference holds it: the fraction is dropped
+    for (int i = 0; i < nload; ++i) {
+        const int node = load_set[i] - 1;
+        if (node < 0 || node >= nd) return fail(OFEA_ERR_ARG, "load node id out of range");
+        f_ext[3 * (size_t)node + dof - 1] = force_value;
+    }
+    return OFEA_OK;
+}
+
 int ofea_build_csr(int nd, int ne, const int *conn, const int *etype, const int *resort, int *n, long long *nnz,
                    int *col_ptr, int *row_idx)
 {

@@ -425,123 +425,151 @@ __global__ void k_axpy(int n, double a, const double *__restrict__ x, double *__
 
 int learned_refresh(ofea_ctx *c);
 
-// Load-stepped Newton on the total-Lagrangian St.Venant-Kirchhoff problem.
-// Reference loop: src/cae.cpp:209-253 (full load in one step, r = F - f_int,
-// K_T du = r, u += du, stop at ||r||_2/(3n) < 1e-6 or 100 loops).  Here the
-// load is ramped in n_incr increments lam_k = k/n_incr and each increment
-// iterates until ||lam F - f_int|| <= tol ||lam F|| (FP64 residual) or
-// max_newton iterations; the linear solves are GPU PCG (method, inner_rtol).
+// Linear solves of a Newton run, K_T du = r with the resident K_T and rhs.
 // A learned method is judged by the same solver with plain block Jacobi: on
 // the first linear solve the plain solver runs, then the learned one limited
 // to the plain iteration count; the learned method serves the rest of the run
 // only if it converged in fewer iterations, and the plain solver takes over
-// whenever it fails later.
+// whenever it fails later.  The solution is left in c->d_x.
+struct NewtonSolves {
+    int use;         // method in use
+    bool judged;     // the learned method has been judged (or none is used)
+    double *xs;      // plain solution kept while the learned one is judged (n)
+    int linear = 0;  // PCG iterations of every solve
+
+    static bool is_learned(int m)
+    {
+        return m == OFEA_PCG_LEARNED || m == OFEA_PCG_LEARNED_MIXED || m == OFEA_PCG_LEARNED_TWO_LEVEL ||
+               m == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED;
+    }
+    // the same solver with block Jacobi in place of the learned blocks
+    static int plain_of(int m)
+    {
+        return m == OFEA_PCG_LEARNED_MIXED             ? OFEA_PCG_MIXED
+               : m == OFEA_PCG_LEARNED_TWO_LEVEL       ? OFEA_PCG_TWO_LEVEL
+               : m == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED ? OFEA_PCG_TWO_LEVEL_MIXED
+                                                       : OFEA_PCG_BLOCK_JACOBI;
+    }
+    NewtonSolves(int method, double *buf) : use(method), judged(!is_learned(method)), xs(buf) {}
+
+    int solve(ofea_ctx *c, int n, double inner_rtol)
+    {
+        c->rhs_set = true;
+        int lit = 0;
+        double lrel = 0.0;
+        int m = use;
+        if (is_learned(m) && (!c->learned || learned_refresh(c) != OFEA_OK)) m = use = plain_of(m);
+        int src;
+        if (is_learned(m) && !judged) {
+            judged = true;
+            int pit = 0;
+            src = solver_solve(c, plain_of(m), inner_rtol, 20000, &pit, &lrel);
+            linear += pit;
+            if (src == OFEA_OK) {
+                if (hipMemcpyAsync(xs, c->d_x, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+                    return fail(OFEA_ERR_HIP, "solution copy failed");
+                const int lsrc = solver_solve(c, m, inner_rtol, pit, &lit, &lrel);
+                if (lsrc < 0 && lsrc != OFEA_ERR_STATE) {  // OFEA_ERR_STATE: breakdown of the learned PCG
+                    src = lsrc;
+                } else {
+                    linear += lit;
+                    if (lsrc != OFEA_OK || lit >= pit) {
+                        use = plain_of(m);
+                        if (hipMemcpyAsync(c->d_x, xs, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream) !=
+                            hipSuccess)
+                            return fail(OFEA_ERR_HIP, "solution copy failed");
+                    }
+                }
+            }
+        } else {
+            src = solver_solve(c, m, inner_rtol, 20000, &lit, &lrel);
+            linear += lit;
+            if (src != OFEA_OK && is_learned(m)) {  // learned path failed: the plain solver takes over
+                use = plain_of(m);
+                src = solver_solve(c, use, inner_rtol, 20000, &lit, &lrel);
+                linear += lit;
+            }
+        }
+        return src < 0 ? src : OFEA_OK;
+    }
+};
+
+// r = lam F - f_int into c->d_rhs; returns ||r||^2 and ||lam F||^2 (FP64,
+// partial sums added on the host in a fixed order).
+static int nl_residual(ofea_ctx *c, int n, double lam, const double *dF, double *part, std::vector<double> &hp,
+                       double &rr, double &ff)
+{
+    const int nb = (n + NLR_WG - 1) / NLR_WG;
+    hipLaunchKernelGGL(k_nl_residual, dim3(nb), dim3(NLR_WG), 0, c->stream, n, lam, dF, c->d_fint, c->d_rhs, part);
+    if (hipMemcpyAsync(hp.data(), part, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return fail(OFEA_ERR_HIP, "residual copy failed");
+    rr = 0.0;
+    ff = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        rr += hp[2 * b];
+        ff += hp[2 * b + 1];
+    }
+    return OFEA_OK;
+}
+
+// Device buffers of a Newton run: the load, residual partials, the kept plain solution.
+struct NewtonBuffers {
+    double *dF = nullptr, *part = nullptr, *xs = nullptr;
+    std::vector<double> hp;
+    int alloc(int n)
+    {
+        const int nb = (n + NLR_WG - 1) / NLR_WG;
+        hp.assign(2 * std::max(nb, 1), 0.0);
+        OFEA_HIP(hipMalloc((void **)&dF, sizeof(double) * std::max(n, 1)));
+        OFEA_HIP(hipMalloc((void **)&part, sizeof(double) * 2 * std::max(nb, 1)));
+        OFEA_HIP(hipMalloc((void **)&xs, sizeof(double) * std::max(n, 1)));
+        return OFEA_OK;
+    }
+    ~NewtonBuffers()
+    {
+        (void)hipFree(dF);
+        (void)hipFree(part);
+        (void)hipFree(xs);
+    }
+};
+
+// Load-stepped Newton on the total-Lagrangian St.Venant-Kirchhoff problem.
+// Reference loop: src/cae.cpp:209-253 (full load in one step, r = F - f_int,
+// K_T du = r, u += du, stop at ||r||_2/(3n) < 1e-6 or 100 loops; that rule is
+// nl_solve_ref below).  Here the load is ramped in n_incr increments
+// lam_k = k/n_incr and each increment iterates until
+// ||lam F - f_int|| <= tol ||lam F|| (FP64 residual) or max_newton
+// iterations; the linear solves are GPU PCG (method, inner_rtol; learned
+// methods judged as NewtonSolves describes).
 int nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr, double tol, int max_newton, int method,
              double inner_rtol, double *u_out, int *newton_total, int *linear_total, double *relres)
 {
     const int n = c->P.n;
     int rc = nl_buffers(c);
     if (rc) return rc;
-    double *dF = nullptr, *part = nullptr, *xs = nullptr;
+    NewtonBuffers B;
+    if ((rc = B.alloc(n))) return rc;
     const int nb = (n + NLR_WG - 1) / NLR_WG;
-    OFEA_HIP(hipMalloc((void **)&dF, sizeof(double) * std::max(n, 1)));
-    OFEA_HIP(hipMalloc((void **)&part, sizeof(double) * 2 * std::max(nb, 1)));
-    OFEA_HIP(hipMalloc((void **)&xs, sizeof(double) * std::max(n, 1)));  // plain solution while the learned one is judged
-    std::vector<double> hp(2 * std::max(nb, 1));
-    auto cleanup = [&]() {
-        (void)hipFree(dF);
-        (void)hipFree(part);
-        (void)hipFree(xs);
-    };
-    OFEA_HIP(hipMemcpyAsync(dF, F, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    OFEA_HIP(hipMemcpyAsync(B.dF, F, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     OFEA_HIP(hipMemsetAsync(c->d_u, 0, sizeof(double) * std::max(n, 1), c->stream));
-    auto is_learned = [](int m) {
-        return m == OFEA_PCG_LEARNED || m == OFEA_PCG_LEARNED_MIXED || m == OFEA_PCG_LEARNED_TWO_LEVEL ||
-               m == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED;
-    };
-    auto plain_of = [](int m) {  // the same solver with block Jacobi in place of the learned blocks
-        return m == OFEA_PCG_LEARNED_MIXED        ? OFEA_PCG_MIXED
-               : m == OFEA_PCG_LEARNED_TWO_LEVEL  ? OFEA_PCG_TWO_LEVEL
-               : m == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED ? OFEA_PCG_TWO_LEVEL_MIXED
-                                                       : OFEA_PCG_BLOCK_JACOBI;
-    };
-    int newton = 0, linear = 0, fallbacks = 0;
-    int use = method;
-    bool judged = !is_learned(method);
+    NewtonSolves S(method, B.xs);
+    int newton = 0;
     double rel = 0.0;
     for (int k = 1; k <= n_incr; ++k) {
         const double lam = (double)k / n_incr;
         bool conv = false;
         for (int it = 0; it <= max_newton; ++it) {
             if ((rc = nl_assemble_dev(c, E, nu))) break;
-            hipLaunchKernelGGL(k_nl_residual, dim3(nb), dim3(NLR_WG), 0, c->stream, n, lam, dF, c->d_fint, c->d_rhs,
-                               part);
-            if (hipMemcpyAsync(hp.data(), part, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->stream) !=
-                    hipSuccess ||
-                hipStreamSynchronize(c->stream) != hipSuccess) {
-                rc = fail(OFEA_ERR_HIP, "residual copy failed");
-                break;
-            }
             double rr = 0.0, ff = 0.0;
-            for (int b = 0; b < nb; ++b) {
-                rr += hp[2 * b];
-                ff += hp[2 * b + 1];
-            }
+            if ((rc = nl_residual(c, n, lam, B.dF, B.part, B.hp, rr, ff))) break;
             rel = ff > 0.0 ? std::sqrt(rr / ff) : std::sqrt(rr);
             if (rel <= tol) {
                 conv = true;
                 break;
             }
             if (it == max_newton) break;
-            c->rhs_set = true;
-            int lit = 0;
-            double lrel = 0.0;
-            int m = use;
-            if (is_learned(m) && (!c->learned || learned_refresh(c) != OFEA_OK)) m = use = plain_of(m);
-            int src;
-            if (is_learned(m) && !judged) {
-                // first solve: the plain solver, then the learned one with the
-                // plain iteration count as its limit; the learned method serves
-                // the rest of the run only if it converged in fewer iterations
-                judged = true;
-                int pit = 0;
-                src = solver_solve(c, plain_of(m), inner_rtol, 20000, &pit, &lrel);
-                linear += pit;
-                if (src == OFEA_OK) {
-                    if (hipMemcpyAsync(xs, c->d_x, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream) !=
-                        hipSuccess) {
-                        rc = fail(OFEA_ERR_HIP, "solution copy failed");
-                        break;
-                    }
-                    const int lsrc = solver_solve(c, m, inner_rtol, pit, &lit, &lrel);
-                    if (lsrc < 0 && lsrc != OFEA_ERR_STATE) {  // OFEA_ERR_STATE: breakdown of the learned PCG
-                        src = lsrc;
-                    } else {
-                        linear += lit;
-                        if (lsrc != OFEA_OK || lit >= pit) {
-                            use = plain_of(m);
-                            if (hipMemcpyAsync(c->d_x, xs, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                                               c->stream) != hipSuccess) {
-                                rc = fail(OFEA_ERR_HIP, "solution copy failed");
-                                break;
-                            }
-                        }
-                    }
-                }
-            } else {
-                src = solver_solve(c, m, inner_rtol, 20000, &lit, &lrel);
-                linear += lit;
-                if (src != OFEA_OK && is_learned(m)) {  // learned path failed: the plain solver takes over
-                    fallbacks++;
-                    use = plain_of(m);
-                    src = solver_solve(c, use, inner_rtol, 20000, &lit, &lrel);
-                    linear += lit;
-                }
-            }
-            if (src < 0) {
-                rc = src;
-                break;
-            }
+            if ((rc = S.solve(c, n, inner_rtol))) break;
             hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(NLR_WG), 0, c->stream, n, 1.0, c->d_x, c->d_u);
             newton++;
         }
@@ -551,20 +579,54 @@ int nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr, doub
             break;
         }
     }
-    if (rc < 0) {
-        cleanup();
-        return rc;
+    if (rc < 0) return rc;
+    if (u_out) {
+        OFEA_HIP(hipMemcpyAsync(u_out, c->d_u, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        OFEA_HIP(hipStreamSynchronize(c->stream));
+    }
+    if (newton_total) *newton_total = newton;
+    if (linear_total) *linear_total = S.linear;
+    if (relres) *relres = rel;
+    return rc;
+}
+
+// The reference's Newton loop as it stands (src/cae.cpp:209-253): res_norm
+// starts at 1; while res_norm > res_limit and loop < max_loop: K_T, f_int at
+// u; r = F - f_int; K_T du = r; u += du; res_norm = ||r||_2 / (3 nfree) of
+// that r (the residual before the update); loop++.  The full load at once,
+// no increments; the last du is applied after the residual test passes.
+// Linear solves: GPU PCG (method, inner_rtol) in place of PARDISO.
+int nl_solve_ref(ofea_ctx *c, double E, double nu, const double *F, double res_limit, int max_loop, int method,
+                 double inner_rtol, double *u_out, int *loops, int *linear_total, double *res_norm_out)
+{
+    const int n = c->P.n;
+    int rc = nl_buffers(c);
+    if (rc) return rc;
+    NewtonBuffers B;
+    if ((rc = B.alloc(n))) return rc;
+    const int nb = (n + NLR_WG - 1) / NLR_WG;
+    OFEA_HIP(hipMemcpyAsync(B.dF, F, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    OFEA_HIP(hipMemsetAsync(c->d_u, 0, sizeof(double) * std::max(n, 1), c->stream));
+    NewtonSolves S(method, B.xs);
+    double res_norm = 1.;
+    int loop = 0;
+    while (res_norm > res_limit && loop < max_loop) {
+        if ((rc = nl_assemble_dev(c, E, nu))) return rc;
+        double rr = 0.0, ff = 0.0;
+        if ((rc = nl_residual(c, n, 1.0, B.dF, B.part, B.hp, rr, ff))) return rc;
+        if ((rc = S.solve(c, n, inner_rtol))) return rc;
+        hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(NLR_WG), 0, c->stream, n, 1.0, c->d_x, c->d_u);
+        res_norm = std::sqrt(rr) / n;
+        loop++;
     }
     if (u_out) {
         OFEA_HIP(hipMemcpyAsync(u_out, c->d_u, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
         OFEA_HIP(hipStreamSynchronize(c->stream));
     }
-    cleanup();
-    if (newton_total) *newton_total = newton;
-    if (linear_total) *linear_total = linear;
-    if (relres) *relres = rel;
-    (void)fallbacks;
-    return rc;
+    if (loops) *loops = loop;
+    if (linear_total) *linear_total = S.linear;
+    if (res_norm_out) *res_norm_out = res_norm;
+    return res_norm > res_limit ? OFEA_NOT_CONVERGED : OFEA_OK;
 }
 
 }  // namespace ofea

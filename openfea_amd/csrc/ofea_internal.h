@@ -234,6 +234,10 @@ int stream_copy(ofea_ctx *ctx, long long bytes, int reps, float *ms);
 // explicit (explicit.hip)
 int explicit_run(ofea_ctx *ctx, double E, double nu, const double *mass, const double *f_ext, const double *v0,
                  const int *fixed_mask, double dt, int nsteps, int precision, double *u_out);
+int explicit_analysis(ofea_ctx *c, double E, double nu, const double *mass, const double *f_ext, const double *v0,
+                      const int *fixed_mask, double time_total, double time_step, int output_gap, int precision,
+                      int max_steps, int max_frames, double *frames, int *frame_index, double *frame_time,
+                      int *n_frames, int *n_steps);
 int lumped_mass(ofea_ctx *ctx, double rho, double *mass);
 void explicit_free(ofea_ctx *ctx);
 // learned preconditioner (learned.hip)
@@ -262,6 +266,8 @@ int fac_tune(ofea_ctx *ctx, int view);
 int nl_assemble(ofea_ctx *ctx, double E, double nu, const double *u, double *nz_val, double *f_int);
 int nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr, double tol, int max_newton, int method,
              double inner_rtol, double *u_out, int *newton_total, int *linear_total, double *relres);
+int nl_solve_ref(ofea_ctx *c, double E, double nu, const double *F, double res_limit, int max_loop, int method,
+                 double inner_rtol, double *u_out, int *loops, int *linear_total, double *res_norm_out);
 // batched variants (batched.hip)
 int assemble_batch(ofea_ctx *c, int nvar, const double *E_var, double nu, double *out);
 int solve_batch(ofea_ctx *c, int nvar, int nrhs, const double *B, double *X, double rtol, int maxit, int *iters,

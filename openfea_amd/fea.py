@@ -278,6 +278,23 @@ class Context:
                                       ptr(i32(fixed_mask)), dt, nsteps, precision, ptr(u)))
         return u
 
+    def explicit_analysis(self, E, nu, mass, f_ext, fixed_mask, time_total, time_step=0.0, output_gap=10,
+                          v0=None, precision=A.FP64, max_steps=1_000_000, max_frames=None):
+        """The reference's explicit loop with its time control (time_step 0 =
+        automatic).  Returns dict(frames (k x 3nd), index, time, steps)."""
+        max_frames = output_gap + 2 if max_frames is None else max_frames
+        frames = np.zeros((max_frames, 3 * self.nd))
+        idx = np.zeros(max_frames, dtype=np.int32)
+        tt = np.zeros(max_frames)
+        nf, ns = C.c_int(), C.c_int()
+        v0a = None if v0 is None else f64(v0).ravel()
+        check(lib().ofea_explicit_analysis(self._h, E, nu, ptr(f64(mass)), ptr(f64(f_ext).ravel()), ptr(v0a),
+                                           ptr(i32(fixed_mask)), time_total, time_step, output_gap, precision,
+                                           max_steps, max_frames, ptr(frames), ptr(idx), ptr(tt), C.byref(nf),
+                                           C.byref(ns)))
+        k = nf.value
+        return dict(frames=frames[:k], index=idx[:k], time=tt[:k], steps=ns.value)
+
     # ---- geometric nonlinear
     def nl_assemble(self, E, nu, u_free, copy=True):
         val = np.zeros(self.nnz) if copy else None
@@ -291,6 +308,16 @@ class Context:
         rc = check(lib().ofea_nl_solve(self._h, E, nu, ptr(f64(F)), n_incr, tol, max_newton, method, inner_rtol, ptr(u),
                                        C.byref(nt), C.byref(lt), C.byref(rr)), allow=(A.OFEA_OK, A.OFEA_NOT_CONVERGED))
         return u, nt.value, lt.value, rr.value, rc == A.OFEA_OK
+
+    def nl_solve_ref(self, E, nu, F, res_limit=1.0e-6, max_loop=100, method=A.PCG_TWO_LEVEL, inner_rtol=1e-10):
+        """Newton loop with the reference's stopping rule (src/cae.cpp:209-253).
+        Returns (u, loops, pcg iterations, res_norm, converged)."""
+        u = np.zeros(self.n)
+        lp, lt, rn = C.c_int(), C.c_int(), C.c_double()
+        rc = check(lib().ofea_nl_solve_ref(self._h, E, nu, ptr(f64(F)), res_limit, max_loop, method, inner_rtol,
+                                           ptr(u), C.byref(lp), C.byref(lt), C.byref(rn)),
+                   allow=(A.OFEA_OK, A.OFEA_NOT_CONVERGED))
+        return u, lp.value, lt.value, rn.value, rc == A.OFEA_OK
 
     # ---- batched variants / load cases
     def assemble_batch(self, E_var, nu, copy=True, nvar=None):
@@ -459,9 +486,12 @@ def CA_ReAnalysis(assam: AssambleStiffness, deck: Deck, info: CAInfo, deck_new: 
 
 
 def implict_analysis(deck: Deck, result_path: str | None = None, solver_type: str = "GPU_PCG_TWO_LEVEL",
-                     ctx: Context | None = None, E: float | None = None, nu: float | None = None, rtol: float = 1e-10):
-    """CAE_process::implict_analysis (src/cae.cpp:198-301) for a linear static
-    deck: clamp (Set-fix, plus nodes *Boundary fixes in all three DOFs),
+                     ctx: Context | None = None, E: float | None = None, nu: float | None = None, rtol: float = 1e-10,
+                     NLFEA: bool = False, res_lmit: float = 1.0e-6):
+    """CAE_process::implict_analysis (src/cae.cpp:198-301): with NLFEA the
+    reference's Newton loop on the total-Lagrangian tangent (full load,
+    ||r||_2 / (3 nfree) > res_lmit, at most 100 loops; ofea_nl_solve_ref),
+    else the linear static solve.  Clamp (Set-fix, plus nodes *Boundary fixes in all three DOFs),
     loads (every *Cload and *Dsload; on the reference's decks this equals
     build_single_load), material (E, nu of the deck, or per-element E of
     *Solid Section regions with one Poisson ratio), pattern + assembly,
@@ -495,16 +525,72 @@ def implict_analysis(deck: Deck, result_path: str | None = None, solver_type: st
             ctx.set_elem_E(deck.E_elem)
         else:
             ctx.set_elem_E(None)
-        assam.fill_CSR_sparse_mat(deck, E, nu)
-        if not solution_api(assam, deck, solver_type, rtol=rtol):
-            raise ValueError(f"unknown solver {solver_type}")
-        if not deck.solve_info["converged"]:
-            raise RuntimeError(f"solver did not converge: {deck.solve_info}")
+        if NLFEA:
+            if sec:
+                raise ValueError("the Newton loop takes one modulus: per-element section moduli are not supported")
+            if solver_type not in SOLVERS:
+                raise ValueError(f"unknown solver {solver_type}")
+            u, loops, pcg, res, ok = ctx.nl_solve_ref(E, nu, F, res_lmit, 100, SOLVERS[solver_type], rtol)
+            deck.single_dis_vec_ = u
+            deck.solve_info = {"loops": loops, "pcg_iters": pcg, "res_norm": res, "converged": ok}
+            if not ok:
+                raise RuntimeError(f"Newton loop did not reach res_lmit: {deck.solve_info}")
+        else:
+            assam.fill_CSR_sparse_mat(deck, E, nu)
+            if not solution_api(assam, deck, solver_type, rtol=rtol):
+                raise ValueError(f"unknown solver {solver_type}")
+            if not deck.solve_info["converged"]:
+                raise RuntimeError(f"solver did not converge: {deck.solve_info}")
         u_full = reset_displacement(deck.resort, deck.single_dis_vec_)
         fields = get_cauchy_stress_3d(ctx, deck, u_full, E, nu)
         if result_path:
             export_dis_2_vtk(result_path, deck, u_full, 1.0, fields)
         return u_full.reshape(-1, 3), fields
+    finally:
+        if own:
+            ctx.close()
+
+
+def build_explicit_load(nd: int, load_set, dof: int, value: float) -> np.ndarray:
+    """OutFroce_ of src/cae.cpp:449-456: (int)value at DOF dof of every load
+    node (the reference keeps the load value in an int)."""
+    f = np.zeros(3 * nd)
+    ls = i32(load_set)
+    check(lib().ofea_build_explicit_load(nd, ptr(ls), len(ls), dof, value, ptr(f)))
+    return f
+
+
+def explicit_analysis(deck: Deck, result_path: str | None = None, ctx: Context | None = None,
+                      E: float | None = None, nu: float | None = None, rho: float | None = None,
+                      time_total: float | None = None, time_step: float | None = None, output_gap: int = 10,
+                      precision: int = A.FP64, max_steps: int = 1_000_000):
+    """CAE_process::explicit_analysis (src/cae.cpp:420-574): lumped mass at
+    the deck's density, the single load of load_set / load_dof with its value
+    truncated to an int, clamped Set-fix nodes, the deck's time step (0 =
+    automatic) and total time (or the arguments), output_gap outputs; with
+    result_path every output k is written as "<result_path>_<k>.vtk"
+    (displacement fields, as the reference writes them).  Returns the dict of
+    Context.explicit_analysis (frames nd*3 each, their output index and time,
+    the step count)."""
+    E = deck.E if E is None else E
+    nu = deck.nu if nu is None else nu
+    rho = deck.rho if rho is None else rho
+    time_total = deck.t_total if time_total is None else time_total
+    time_step = deck.dt if time_step is None else time_step
+    resort, _ = build_free_index(deck.nd, deck.fix_set)
+    fixed = (resort < 0).astype(np.int32)
+    f_ext = build_explicit_load(deck.nd, deck.load_set, deck.load_dof, deck.load_value)
+    own = ctx is None
+    ctx = ctx or Context()
+    try:
+        ctx.set_mesh(deck.xyz, deck.conn, deck.etype, resort)
+        mass = ctx.lumped_mass(rho)
+        out = ctx.explicit_analysis(E, nu, mass, f_ext, fixed, time_total, time_step, output_gap,
+                                    precision=precision, max_steps=max_steps)
+        if result_path:
+            for k, fr in zip(out["index"], out["frames"]):
+                export_dis_2_vtk(f"{result_path}_{int(k)}.vtk", deck, fr, 1.0)
+        return out
     finally:
         if own:
             ctx.close()

@@ -730,6 +730,171 @@ OR_API void or_explicit_run(int nd, int ne, const int *conn, const int *etype, c
     free(fin);
 }
 
+/* a10: automatic time step of UpdateTimeStep (src/explicit_tools.cpp:21-31)
+ * over update_timestep of each element (src/Elements/C3D8.cpp:195-219,
+ * src/Elements/C3D4.cpp:140-164): per element, Ke at the coordinates xyz_cur
+ * (3nd, the current ones: X + u), max_v = largest row sum of |Ke| (start
+ * DBL_MIN), and dt = min(dt, 2 / sqrt(max_v)) in element order, starting from
+ * dt_in.  Returns dt (the caller scales it by 0.9, src/cae.cpp:463, 535). */
+OR_API double or_crit_timestep(int ne, const int *conn, const int *etype, const double *xyz_cur, double E, double nu,
+                               double dt_in)
+{
+    double dt = dt_in, Ke[576];
+    for (int e = 0; e < ne; ++e) {
+        const int nc = 3 * elem_nn(etype[e]);
+        elem_ke(e, conn, etype, xyz_cur, E, nu, Ke);
+        double max_v = 2.2250738585072014e-308; /* DBL_MIN */
+        for (int i = 0; i < nc; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < nc; ++j) s = s + fabs(Ke[i * nc + j]);
+            if (max_v < s) max_v = s;
+        }
+        const double c = 2 / sqrt(max_v);
+        if (dt > c) dt = c;
+    }
+    return dt;
+}
+
+/* CheckTime (src/explicit_tools.cpp:48-56) */
+static int check_time(double *timestep, double limit)
+{
+    if (*timestep > limit) {
+        *timestep = limit;
+        return 1;
+    }
+    return 0;
+}
+
+/* a10: the time loop of CAE_process::explicit_analysis (src/cae.cpp:420-574)
+ * with its own time control: time_step == 0 selects the automatic step
+ * (0.9 x or_crit_timestep at the current coordinates, recomputed every step),
+ * otherwise time_step is used and only ever lowered by CheckTime, which
+ * clamps the step to land on each multiple of time_total / output_gap and on
+ * time_total.  The displacement is saved at t = 0 and wherever the current
+ * time equals time_total or (time_total / output_gap) x output_count, and the
+ * loop ends after the save at time_total (or when the time passes it).
+ * InForce accumulates -Ke(X) du per element (never zeroed); u and du are
+ * zeroed at clamped nodes.  Extensions: f_ext (3nd) in place of the single
+ * load, v0 (3nd, nullable) as initial velocity.  Outputs: up to max_frames
+ * saved displacements (frames, 3nd each) with their output index
+ * (frame_idx: output_count at the save) and time (frame_t); *n_frames saves
+ * made, *n_steps loop iterations; dt_hist (nullable, max_steps) receives the
+ * step used by each iteration.  Returns 0, or 1 when max_steps iterations
+ * did not reach the end. */
+OR_API int or_explicit_timed(int nd, int ne, const int *conn, const int *etype, const double *xyz, double E, double nu,
+                             const double *mass, const double *f_ext, const double *v0, const int *fixed_mask,
+                             double time_total, double time_step_in, int output_gap, int max_steps, int max_frames,
+                             double *frames, int *frame_idx, double *frame_t, int *n_frames, int *n_steps,
+                             double *dt_hist)
+{
+    const size_t n3 = 3 * (size_t)nd;
+    double *u_prev = (double *)calloc(n3, sizeof(double));
+    double *u_cur = (double *)calloc(n3, sizeof(double));
+    double *du = (double *)calloc(n3, sizeof(double));
+    double *v_half = (double *)calloc(n3, sizeof(double));
+    double *v_next = (double *)calloc(n3, sizeof(double));
+    double *fin = (double *)calloc(n3, sizeof(double));
+    double *real = (double *)malloc(n3 * sizeof(double));
+    double *Ke0 = (double *)malloc((size_t)(ne > 0 ? ne : 1) * 576 * sizeof(double));
+    const double DMAX = 1.7976931348623157e308; /* DBL_MAX */
+    double d[24];
+    int saved = 0, rc = 1;
+    /* Ke on the original coordinates, as cal_in_force receives them (src/cae.cpp:527) */
+    for (int e = 0; e < ne; ++e) elem_ke(e, conn, etype, xyz, E, nu, Ke0 + (size_t)e * 576);
+#define OR_SAVE(src, idx, t)                                                         \
+    do {                                                                             \
+        if (saved < max_frames) {                                                    \
+            memcpy(frames + (size_t)saved * n3, (src), n3 * sizeof(double));         \
+            frame_idx[saved] = (idx);                                                \
+            frame_t[saved] = (t);                                                    \
+        }                                                                            \
+        saved++;                                                                     \
+    } while (0)
+    const int auto_time = time_step_in == 0.0;
+    double time_now = 0.0, time_step = DMAX, time_step_old = 0.0;
+    const double time_scale = 0.9;
+    if (auto_time) {
+        time_step = or_crit_timestep(ne, conn, etype, xyz, E, nu, time_step);
+        time_step *= time_scale;
+    } else {
+        time_step = time_step_in;
+    }
+    for (size_t i = 0; i < n3; ++i) {
+        double a0 = (fin[i] + f_ext[i]) / mass[i / 3];
+        v_next[i] = (v0 ? v0[i] : 0.0) + (a0 * (time_step_old + time_step) / 2);
+    }
+    int output_count = 0, step_count = 0;
+    OR_SAVE(u_cur, output_count, time_now); /* u_cur is zero: the undeformed state */
+    time_now += time_step;
+    /* SwapData (src/explicit_tools.cpp:33-46) */
+    memcpy(v_half, v_next, n3 * sizeof(double));
+    time_step_old = time_step;
+    if (auto_time) time_step = DMAX;
+    while (time_now <= time_total) {
+        if (step_count >= max_steps) goto out;
+        step_count++;
+        for (size_t i = 0; i < n3; ++i) {
+            du[i] = v_half[i] * time_step_old;
+            u_cur[i] = u_prev[i] + du[i];
+        }
+        for (int i = 0; i < nd; ++i)
+            if (fixed_mask[i])
+                for (int c = 0; c < 3; ++c) {
+                    du[3 * i + c] = 0.0;
+                    u_cur[3 * i + c] = 0.0;
+                }
+        for (int e = 0; e < ne; ++e) {
+            const int nn = elem_nn(etype[e]), nc = 3 * nn;
+            const double *Ke = Ke0 + (size_t)e * 576;
+            for (int a = 0; a < nn; ++a) {
+                int node = conn[8 * e + a] - 1;
+                for (int c = 0; c < 3; ++c) d[3 * a + c] = du[3 * node + c];
+            }
+            for (int m = 0; m < nc; ++m) {
+                double acc = 0.0;
+                for (int k = 0; k < nc; ++k) acc += Ke[m * nc + k] * d[k];
+                int node = conn[8 * e + m / 3] - 1;
+                fin[3 * node + m % 3] -= acc;
+            }
+        }
+        for (size_t i = 0; i < n3; ++i) real[i] = xyz[i] + u_cur[i];
+        if (auto_time) {
+            time_step = or_crit_timestep(ne, conn, etype, real, E, nu, time_step);
+            time_step *= time_scale;
+        }
+        if (check_time(&time_step, (time_total / output_gap) * (output_count + 1) - time_now)) output_count++;
+        check_time(&time_step, time_total - time_now);
+        if (dt_hist) dt_hist[step_count - 1] = time_step;
+        for (size_t i = 0; i < n3; ++i) {
+            double a = (fin[i] + f_ext[i]) / mass[i / 3];
+            v_next[i] = v_half[i] + (a * (time_step_old + time_step) / 2);
+        }
+        if (time_now == time_total || time_now == (time_total / output_gap) * (output_count)) {
+            OR_SAVE(u_cur, output_count, time_now);
+            if (time_now == time_total) break;
+        }
+        time_now += time_step;
+        memcpy(u_prev, u_cur, n3 * sizeof(double));
+        memcpy(v_half, v_next, n3 * sizeof(double));
+        time_step_old = time_step;
+        if (auto_time) time_step = DMAX;
+    }
+    rc = 0;
+out:
+#undef OR_SAVE
+    *n_frames = saved;
+    *n_steps = step_count;
+    free(u_prev);
+    free(u_cur);
+    free(du);
+    free(v_half);
+    free(v_next);
+    free(fin);
+    free(real);
+    free(Ke0);
+    return rc;
+}
+
 /* Element-by-element product y = sum_e Ke x_e over free DOFs (matrix-free
  * identity used to pin or_fill_csc: K x == sum_e Ke x_e). */
 OR_API void or_matfree_apply(int ne, const int *conn, const int *etype, const double *xyz, const int *resort, double E,

@@ -51,6 +51,11 @@ def lib():
         L.or_lumped_mass.argtypes = [_i, _i, _pi, _pi, _pd, _d, _pd]
         L.or_explicit_run.argtypes = [_i, _i, _pi, _pi, _pd, _d, _d, _pd, _pd, _vp, _pi, _d, _i, _pd]
         L.or_explicit_recurrence.argtypes = [_i, _pi, _pi, _pd, _pd, _pi, _pd, _d, _i, _pd, _i]
+        L.or_crit_timestep.argtypes = [_i, _pi, _pi, _pd, _d, _d, _d]
+        L.or_crit_timestep.restype = _d
+        L.or_explicit_timed.argtypes = [_i, _i, _pi, _pi, _pd, _d, _d, _pd, _pd, _vp, _pi, _d, _d, _i, _i, _i,
+                                        _pd, _pi, _pd, C.POINTER(_i), C.POINTER(_i), _vp]
+        L.or_explicit_timed.restype = _i
         L.or_matfree_apply.argtypes = [_i, _pi, _pi, _pd, _pi, _d, _d, _vp, _pd, _pd, _i]
         L.or_ca_fill.argtypes = [_i, _pi, _pi, _pi, _pd, _pd, _pi, _d, _d, _pi, _pi, _ll, _pd]
         L.or_ca_fill.restype = _i
@@ -166,6 +171,9 @@ def ref_lib():
         L = C.CDLL(path)
         L.ref_fill_csc.argtypes = [_i, _pi, _pi, _pd, _pi, _d, _vp, _d, _pi, _pi, _pd, _i, _i]
         L.ref_fill_csc.restype = _i
+        for nm in ("ref_c3d8_timestep", "ref_c3d4_timestep"):
+            getattr(L, nm).argtypes = [_pd, _d, _d, _d]
+            getattr(L, nm).restype = _d
         _REF = L
     return _REF
 
@@ -183,6 +191,22 @@ def ref_fill_csc(conn, etype, xyz, resort, E, nu, col_ptr, row_idx, threads=1, E
                           None if Ee is None else Ee.ctypes.data, nu, _i32(col_ptr), _i32(row_idx), val,
                           len(col_ptr) - 1, int(threads))
     return val, used
+
+
+def ref_crit_timestep(conn, etype, xyz_cur, E, nu, dt_in=np.finfo(np.float64).max):
+    """UpdateTimeStep's element loop over the reference's own update_timestep
+    (src/explicit_tools.cpp:21-31), or None without oracle/_ref."""
+    L = ref_lib()
+    if L is None:
+        return None
+    xyz_cur = _f64(xyz_cur).reshape(-1, 3)
+    conn = _i32(conn).reshape(-1, 8)
+    dt = dt_in
+    for e in range(len(etype)):
+        nn = 8 if etype[e] == 1 else 4
+        X = np.ascontiguousarray(xyz_cur[conn[e, :nn] - 1]).ravel()
+        dt = (L.ref_c3d8_timestep if nn == 8 else L.ref_c3d4_timestep)(X, E, nu, dt)
+    return dt
 
 
 def nl_fill(conn, etype, xyz, resort, E, nu, u_free, col_ptr, row_idx):
@@ -243,6 +267,35 @@ def explicit_run(conn, etype, xyz, E, nu, mass, f_ext, v0, fixed_mask, dt, nstep
     return u
 
 
+def crit_timestep(conn, etype, xyz_cur, E, nu, dt_in=np.finfo(np.float64).max):
+    """min over elements of 2 / sqrt(max row sum of |Ke(xyz_cur)|), from dt_in"""
+    return lib().or_crit_timestep(len(etype), _i32(conn), _i32(etype), _f64(xyz_cur).ravel(), E, nu, dt_in)
+
+
+def explicit_timed(conn, etype, xyz, E, nu, mass, f_ext, fixed_mask, time_total, time_step, output_gap=10,
+                   v0=None, max_steps=100000, max_frames=64, dt_history=False):
+    """Time loop of src/cae.cpp:420-574 with its time control (time_step 0 =
+    automatic).  Returns dict(frames (k x 3nd), idx, t, steps, complete
+    [, dt])."""
+    nd = len(xyz)
+    n3 = 3 * nd
+    frames = np.zeros((max_frames, n3))
+    idx = np.zeros(max_frames, dtype=np.int32)
+    tt = np.zeros(max_frames)
+    nf, ns = C.c_int(0), C.c_int(0)
+    hist = np.zeros(max_steps) if dt_history else None
+    v0a = None if v0 is None else _f64(v0).ravel()
+    rc = lib().or_explicit_timed(nd, len(etype), _i32(conn), _i32(etype), _f64(xyz).ravel(), E, nu, _f64(mass),
+                                 _f64(f_ext).ravel(), None if v0a is None else v0a.ctypes.data, _i32(fixed_mask),
+                                 time_total, time_step, output_gap, max_steps, max_frames, frames, idx, tt,
+                                 C.byref(nf), C.byref(ns), None if hist is None else hist.ctypes.data)
+    k = min(nf.value, max_frames)
+    out = dict(frames=frames[:k], idx=idx[:k], t=tt[:k], n_frames=nf.value, steps=ns.value, complete=rc == 0)
+    if dt_history:
+        out["dt"] = hist[:ns.value]
+    return out
+
+
 def matfree_apply(conn, etype, xyz, resort, E, nu, x, n, E_elem=None):
     y = np.zeros(n)
     Ee = None if E_elem is None else _f64(E_elem)
@@ -274,6 +327,30 @@ def nl_newton(conn, etype, xyz, resort, E, nu, F, col_ptr, row_idx, n_incr=1, to
         else:
             raise RuntimeError("oracle Newton did not converge")
     return u, iters
+
+
+def nl_newton_ref(conn, etype, xyz, resort, E, nu, F, col_ptr, row_idx, res_lmit=1.0e-6, max_loop=100):
+    """The Newton loop of src/cae.cpp:209-253 as it stands: res_norm = 1;
+    while res_norm > res_lmit and loop < max_loop: K_T, f_int at u,
+    r = F - f_int, K_T du = r (direct: scipy SuperLU in place of PARDISO),
+    u += du, res_norm = ||r||_2 / (3 nfree) of that r, loop += 1.
+    Returns (u, loops, res_norm, history of res_norm)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    n = len(col_ptr) - 1
+    u = np.zeros(n)
+    res_norm, loop, hist = 1.0, 0, []
+    F = np.asarray(F, dtype=np.float64)
+    while res_norm > res_lmit and loop < max_loop:
+        val, fint = nl_fill(conn, etype, xyz, resort, E, nu, u, col_ptr, row_idx)
+        r = F - fint
+        K = sp.csc_matrix((val, row_idx, col_ptr), shape=(n, n))
+        u = u + spla.spsolve(K, r)
+        res_norm = float(np.sqrt(np.sum(r * r))) / n
+        hist.append(res_norm)
+        loop += 1
+    return u, loop, res_norm, hist
 
 
 def ca_fill(kind, conn_mod, etype_mod, xyz_m, xyz_o, resort, E, nu, col_ptr, row_idx):
