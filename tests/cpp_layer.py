@@ -8,6 +8,7 @@ import numpy as np
 from conftest import REPO
 
 DRIVER = os.path.join(REPO, "tests", "cpp", "implicit_pipeline")
+CAE = os.path.join(REPO, "tests", "cpp", "cae_process")
 
 
 def driver():
@@ -15,6 +16,14 @@ def driver():
     if not os.path.exists(DRIVER):
         subprocess.run(["make", "-C", os.path.join(REPO, "tests", "cpp")], check=True)
     return DRIVER
+
+
+def run_cae(deck, prefix, *args, timeout=120):
+    """tests/cpp/cae_process (CAE_process: NLFEA Newton or explicit)."""
+    if not os.path.exists(CAE):
+        subprocess.run(["make", "-C", os.path.join(REPO, "tests", "cpp")], check=True)
+    cmd = [CAE, deck, str(prefix)] + [str(a) for a in args]
+    return subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
 
 
 def run(deck, prefix, mode=None, timeout=120):

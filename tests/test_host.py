@@ -126,6 +126,24 @@ def test_free_index_and_load_match_oracle():
     assert np.array_equal(ofa.reset_displacement(r1, u), O.reset_displacement(r1, u))
 
 
+def test_explicit_load_truncates_the_value_to_an_int():
+    """src/cae.cpp:449-456: int force_value = load_value_, assigned (not
+    added) at DOF load_dof of every load node, clamped nodes included."""
+    from openfea_amd import fea
+
+    f = fea.build_explicit_load(6, [2, 5, 5], 3, -10.9)
+    want = np.zeros(18)
+    want[[3 * 1 + 2, 3 * 4 + 2]] = -10.0
+    assert np.array_equal(f, want)
+    assert np.array_equal(fea.build_explicit_load(3, [1], 1, 2.999), np.array([2.0] + [0.0] * 8))
+    assert np.array_equal(fea.build_explicit_load(3, [3], 2, 0.7), np.zeros(9))
+    L = _capi.lib()
+    out = np.zeros(9)
+    assert L.ofea_build_explicit_load(3, _capi.ptr(_capi.i32([4])), 1, 1, 1.0, _capi.ptr(out)) == _capi.OFEA_ERR_ARG
+    assert L.ofea_build_explicit_load(3, _capi.ptr(_capi.i32([1])), 1, 4, 1.0, _capi.ptr(out)) == _capi.OFEA_ERR_ARG
+    assert L.ofea_build_explicit_load(3, _capi.ptr(_capi.i32([1])), 1, 1, 3e9, _capi.ptr(out)) == _capi.OFEA_ERR_ARG
+
+
 @pytest.mark.parametrize("mk", [lambda: mg.hex_block(5, 3, 2, jitter=0.2, seed=1),
                                 lambda: mg.kuhn_tets(4, 3, 3, jitter=0.2, seed=2),
                                 lambda: mg.hex_block(20, 10, 10)])

@@ -50,3 +50,14 @@ def test_device_path_fails_loudly_without_gpu(tmp_path):
         pytest.skip("a GPU is present")
     r = CL.run(os.path.join(GOLDEN, "mesh_o.inp"), tmp_path / "out")
     assert r.returncode == 1 and "build_CSR" in r.stderr
+
+
+@pytest.mark.parametrize("mode", [("nl",), ("explicit", "1e-3")])
+def test_cae_process_device_paths_fail_loudly_without_gpu(mode, tmp_path):
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    r = CL.run_cae(os.path.join(GOLDEN, "mesh_o.inp"), tmp_path / "out", *mode)
+    where = "build_CSR" if mode[0] == "nl" else "explicit_analysis"
+    assert r.returncode == 1 and f"openfea_amd: {where}" in r.stderr, r.stdout + r.stderr
