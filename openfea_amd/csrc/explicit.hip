@@ -8,15 +8,15 @@
 // which is the same quantity without the running-sum round-off.
 //
 // One step = two kernels:
-//   element phase  lane per (hex, Gauss point) [or thread per tet]: grad u,
-//                  sigma = lambda tr(eps) I + 2 mu eps, nodal forces
-//                  w detJ sigma g_a; an 8-lane reduce-scatter leaves each lane
-//                  the force of one element node -> fe[a][e] (local-node-major,
-//                  so that the node phase's gathers of consecutive nodes read
-//                  consecutive elements; no atomics);
-//   node phase     thread per node: f = -sum fe over incident elements in
-//                  ascending order, a = (f + f_ext)/m, v += a dt,
+//   element phase  one lane per element: the nodal forces Ke u_e (hexes in the
+//                  moment form of the trilinear element, k_force_hex) stored
+//                  local-node-major, fe[a][e], so that a wave's stores of one
+//                  local node fill whole lines and the node phase's gathers
+//                  of consecutive nodes read consecutive elements; no atomics;
+//   node phase     one lane per node: f = -sum fe over its incident elements
+//                  in ascending order, a = (f + f_ext)/m, v += a dt,
 //                  u_next = u + v dt (0 at clamped nodes).
+// Both launches map contiguous element / node ranges to one XCD each.
 // Runs in FP32 (production) or FP64 (parity) with the same kernels.
 #include <algorithm>
 #include <vector>
@@ -205,66 +205,166 @@ int lumped_mass(ofea_ctx *c, double rho, double *mass)
 }
 
 // ------------------------------------------------------------------ forces
+// Abaqus node of hex corner b = [x > 0] + 2 [y > 0] + 4 [z > 0].
+__host__ __device__ constexpr int hex_node_of_corner(int b)
+{
+    return (b & 4) + ((b & 3) == 0 ? 0 : (b & 3) == 1 ? 1 : (b & 3) == 3 ? 2 : 3);
+}
+
+// In place y_m = sum_b prod_{j in m} s_j(b) x_b over the 8 corners (s_j(b) =
+// +1 when bit j of b is set, else -1), NV components each: one butterfly
+// (lo + hi, hi - lo) per axis.
+template <class T, int NV>
+__device__ __forceinline__ void wht8(T x[8][NV])
+{
+#pragma unroll
+    for (int j = 1; j < 8; j <<= 1)
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            if (!(b & j))
+#pragma unroll
+                for (int k = 0; k < NV; ++k) {
+                    const T lo = x[b][k], hi = x[b | j][k];
+                    x[b][k] = lo + hi;
+                    x[b | j][k] = hi - lo;
+                }
+}
+
+// Its transpose, f_b = sum_m prod_{j in m} s_j(b) y_m: per axis (lo - hi, lo + hi).
+template <class T, int NV>
+__device__ __forceinline__ void wht8_t(T x[8][NV])
+{
+#pragma unroll
+    for (int j = 1; j < 8; j <<= 1)
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            if (!(b & j))
+#pragma unroll
+                for (int k = 0; k < NV; ++k) {
+                    const T lo = x[b][k], hi = x[b | j][k];
+                    x[b][k] = lo - hi;
+                    x[b | j][k] = lo + hi;
+                }
+}
+
+// Block b of a launch with 8 * q blocks, renumbered so that the blocks the
+// hardware places on one XCD (b mod 8) take q consecutive logical blocks:
+// each XCD works on one contiguous range of elements or nodes, whose
+// gathered neighbours then stay in that XCD's L2.
+__device__ __forceinline__ int xcd_block(int q) { return (int)(blockIdx.x & 7) * q + (int)(blockIdx.x >> 3); }
+
+// Hex forces f_a = Ke u_e, one lane per element, in the moment form of the
+// trilinear element: with c_m = 1/8 sum_a s_a^m X_a (s_a^m the product of
+// node a's corner signs over the axes in m), dX/dxi at a Gauss point is a
+// combination of 4 of the c_m, and the same for u; H' = adj(J) dU/dxi is
+// detJ grad u, S = mu (H' + H'^T) + lam tr(H') I is detJ sigma, and
+// P = S adj(J) / detJ.  The nodal forces sum_p P dN_a/dxi (weights 1) are
+// the inverse transform 1/8 sum_m s_a^m A_m of 7 moments A_m accumulated
+// over the points, which is Ke u (src/Elements/C3D8_Gauss.cpp:80-108,
+// src/Elements/C3D8.cpp:158-193).
 template <class T>
-__global__ __launch_bounds__(EX_WG) void k_force_hex(int ne, const int *__restrict__ conn, const T *__restrict__ xyz,
-                                                     const T *__restrict__ u, T lam, T mu, T *__restrict__ fe,
-                                                     const int *__restrict__ stop)
+__global__ __launch_bounds__(EX_WG) void k_force_hex(int ne, int q, const int *__restrict__ conn,
+                                                     const T *__restrict__ xyz, const T *__restrict__ u, T lam, T mu,
+                                                     T *__restrict__ fe, const int *__restrict__ stop)
 {
     if (stop && *stop) return;
-    const int gid = blockIdx.x * EX_WG + threadIdx.x;
-    const int e_raw = gid >> 3, gp = gid & 7;
-    const int e = min(e_raw, ne - 1);  // whole 8-lane groups stay in the shuffles
-    const int *cn = conn + 8 * e;
-    T X[24], U[24];
+    const int e = xcd_block(q) * EX_WG + threadIdx.x;
+    if (e >= ne) return;
+    const int4 n0 = *reinterpret_cast<const int4 *>(conn + 8 * (size_t)e);
+    const int4 n1 = *reinterpret_cast<const int4 *>(conn + 8 * (size_t)e + 4);
+    const int nodes[8] = {n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, n1.z, n1.w};
+    // coefficients by a 3-stage Walsh-Hadamard butterfly over the corner
+    // index b = [x > 0] + 2 [y > 0] + 4 [z > 0] (node hex_corner[b])
+    T cX[8][3], cU[8][3];
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-        int node = cn[b];
+        const int node = nodes[hex_node_of_corner(b)];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            X[3 * b + c] = xyz[3 * node + c];
-            U[3 * b + c] = u[3 * node + c];
+        for (int k = 0; k < 3; ++k) {
+            cX[b][k] = T(0.125) * xyz[3 * node + k];
+            cU[b][k] = T(0.125) * u[3 * node + k];
         }
     }
-    T g[3][8];
-    T detJ = hex_point<T>(X, gp, g);
-    // grad u: H[c][j] = sum_b u_b[c] g[j][b]
-    T H[3][3];
+    wht8<T, 3>(cX);
+    wht8<T, 3>(cU);
+    T Am[8][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+    for (int m = 0; m < 8; ++m)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            T acc = T(0);
+        for (int k = 0; k < 3; ++k) Am[m][k] = T(0);
+    const T g = T(0.57735026918962576451);  // 1/sqrt(3)
 #pragma unroll
-            for (int b = 0; b < 8; ++b) acc += U[3 * b + c] * g[j][b];
-            H[c][j] = acc;
+    for (int p = 0; p < 8; ++p) {
+        const T xi = T(hsgn(p, 0)) * g, eta = T(hsgn(p, 1)) * g, zeta = T(hsgn(p, 2)) * g;
+        T J[3][3], D[3][3];  // J[i][k] = dX_k/dxi_i, D[i][c] = du_c/dxi_i
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            J[0][k] = cX[1][k] + eta * cX[3][k] + zeta * cX[5][k] + (eta * zeta) * cX[7][k];
+            J[1][k] = cX[2][k] + xi * cX[3][k] + zeta * cX[6][k] + (xi * zeta) * cX[7][k];
+            J[2][k] = cX[4][k] + xi * cX[5][k] + eta * cX[6][k] + (xi * eta) * cX[7][k];
+            D[0][k] = cU[1][k] + eta * cU[3][k] + zeta * cU[5][k] + (eta * zeta) * cU[7][k];
+            D[1][k] = cU[2][k] + xi * cU[3][k] + zeta * cU[6][k] + (xi * zeta) * cU[7][k];
+            D[2][k] = cU[4][k] + xi * cU[5][k] + eta * cU[6][k] + (xi * eta) * cU[7][k];
         }
-    T tr = H[0][0] + H[1][1] + H[2][2];
-    T S[3][3];
+        T A[3][3];  // adj(J) = detJ J^-1
+        A[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        A[0][1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+        A[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+        A[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        A[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+        A[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+        A[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        A[2][1] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+        A[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        const T det = J[0][0] * A[0][0] + J[0][1] * A[1][0] + J[0][2] * A[2][0];
+        T Hp[3][3];  // detJ du_c/dx_j
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) S[c][j] = detJ * (mu * (H[c][j] + H[j][c]) + (c == j ? lam * tr : T(0)));
-    auto blk = [&](int a, T out[3]) {
+            for (int j = 0; j < 3; ++j) Hp[c][j] = A[j][0] * D[0][c] + A[j][1] * D[1][c] + A[j][2] * D[2][c];
+        const T ltr = lam * (Hp[0][0] + Hp[1][1] + Hp[2][2]);
+        T rdet;
+        if constexpr (sizeof(T) == 4)
+            rdet = __builtin_amdgcn_rcpf(det);  // v_rcp_f32, 1 ulp: within the FP32 path's rounding
+        else
+            rdet = T(1) / det;
+        T S[3][3];  // sigma = (detJ sigma) / detJ
 #pragma unroll
-        for (int c = 0; c < 3; ++c) out[c] = S[c][0] * g[0][a] + S[c][1] * g[1][a] + S[c][2] * g[2][a];
-    };
-    T f[3];
-    reduce_scatter8<3, T>(gp, blk, f);
-    if (e_raw < ne) {
-        T *dst = fe + 3 * ((size_t)gp * ne + e);
-        dst[0] = f[0];
-        dst[1] = f[1];
-        dst[2] = f[2];
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) S[c][j] = rdet * (mu * (Hp[c][j] + Hp[j][c]) + (c == j ? ltr : T(0)));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            T P[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) P[i] = S[c][0] * A[0][i] + S[c][1] * A[1][i] + S[c][2] * A[2][i];
+            Am[1][c] += P[0];
+            Am[2][c] += P[1];
+            Am[4][c] += P[2];
+            Am[3][c] += eta * P[0] + xi * P[1];
+            Am[5][c] += zeta * P[0] + xi * P[2];
+            Am[6][c] += zeta * P[1] + eta * P[2];
+            Am[7][c] += (eta * zeta) * P[0] + (xi * zeta) * P[1] + (xi * eta) * P[2];
+        }
+    }
+    // f at corner b = 1/8 sum_m s_b^m A_m: the transposed butterfly
+    wht8_t<T, 3>(Am);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        T *dst = fe + 3 * ((size_t)hex_node_of_corner(b) * ne + e);
+        dst[0] = T(0.125) * Am[b][0];
+        dst[1] = T(0.125) * Am[b][1];
+        dst[2] = T(0.125) * Am[b][2];
     }
 }
 
 template <class T>
-__global__ __launch_bounds__(EX_WG) void k_force_tet(int ne, const int *__restrict__ conn, const T *__restrict__ xyz,
-                                                     const T *__restrict__ u, T lam, T mu, T *__restrict__ fe,
-                                                     const int *__restrict__ stop)
+__global__ __launch_bounds__(EX_WG) void k_force_tet(int ne, int q, const int *__restrict__ conn,
+                                                     const T *__restrict__ xyz, const T *__restrict__ u, T lam, T mu,
+                                                     T *__restrict__ fe, const int *__restrict__ stop)
 {
     if (stop && *stop) return;
-    const int e = blockIdx.x * EX_WG + threadIdx.x;
+    const int e = xcd_block(q) * EX_WG + threadIdx.x;
     if (e >= ne) return;
     const int *cn = conn + 4 * e;
     T X[12], U[12];
@@ -294,23 +394,48 @@ __global__ __launch_bounds__(EX_WG) void k_force_tet(int ne, const int *__restri
         }
 }
 
+// f -= the forces of node i's elements, in ascending element order.  The
+// incidence codes are read 8 at a time and their force loads issued together
+// before the sums (absent entries add nothing).
 template <class T>
-__global__ __launch_bounds__(EX_WG) void k_node_update(int nd, const int *__restrict__ inc_ptr, const int *__restrict__ inc,
+__device__ __forceinline__ void gather_forces(int i, const int *__restrict__ inc_ptr, const int *__restrict__ inc,
+                                              int ne4, int ne8, const T *__restrict__ fe4,
+                                              const T *__restrict__ fe8, T &f0, T &f1, T &f2)
+{
+    const int t0 = inc_ptr[i], t1 = inc_ptr[i + 1];
+    for (int tb = t0; tb < t1; tb += 8) {
+        T g[8][3];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            g[k][0] = g[k][1] = g[k][2] = T(0);
+            if (tb + k < t1) {
+                const int code = inc[tb + k], el = code >> 4, type = (code >> 3) & 1, a = code & 7;
+                const T *src = type ? fe8 + 3 * ((size_t)a * ne8 + el) : fe4 + 3 * ((size_t)a * ne4 + el);
+                g[k][0] = src[0];
+                g[k][1] = src[1];
+                g[k][2] = src[2];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            f0 -= g[k][0];
+            f1 -= g[k][1];
+            f2 -= g[k][2];
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(EX_WG) void k_node_update(int nd, int q, const int *__restrict__ inc_ptr, const int *__restrict__ inc,
                                                        int ne4, int ne8, const T *__restrict__ fe4, const T *__restrict__ fe8,
                                                        const T *__restrict__ ucur, T *__restrict__ unext,
                                                        T *__restrict__ v, const T *__restrict__ minv,
                                                        const T *__restrict__ fext, const int *__restrict__ fixed, T dt)
 {
-    const int i = blockIdx.x * EX_WG + threadIdx.x;
+    const int i = xcd_block(q) * EX_WG + threadIdx.x;
     if (i >= nd) return;
     T f0 = T(0), f1 = T(0), f2 = T(0);
-    for (int t = inc_ptr[i]; t < inc_ptr[i + 1]; ++t) {
-        int code = inc[t], el = code >> 4, type = (code >> 3) & 1, a = code & 7;
-        const T *src = type ? fe8 + 3 * ((size_t)a * ne8 + el) : fe4 + 3 * ((size_t)a * ne4 + el);
-        f0 -= src[0];
-        f1 -= src[1];
-        f2 -= src[2];
-    }
+    gather_forces<T>(i, inc_ptr, inc, ne4, ne8, fe4, fe8, f0, f1, f2);
     const T mi = minv[i];
     T v0 = v[3 * i] + (f0 + fext[3 * i]) * mi * dt;
     T v1 = v[3 * i + 1] + (f1 + fext[3 * i + 1]) * mi * dt;
@@ -338,25 +463,31 @@ __global__ void k_explicit_init(int nd, T *__restrict__ v, T *__restrict__ u1, c
     }
 }
 
+// blocks per XCD for n items of EX_WG per block (the grid is 8 x that)
+static int xcd_q(int n) { return ((n + EX_WG - 1) / EX_WG + 7) / 8; }
+
 template <class T>
 static void launch_forces(ofea_ctx *c, ExplicitWS *w, T lam, T mu, const T *ucur, const int *stop)
 {
-    if (w->ne[1])
-        hipLaunchKernelGGL(k_force_hex<T>, dim3((8 * w->ne[1] + EX_WG - 1) / EX_WG), dim3(EX_WG), 0, c->stream,
-                           w->ne[1], w->conn[1], (const T *)w->xyz_t, ucur, lam, mu, (T *)w->fe[1], stop);
-    if (w->ne[0])
-        hipLaunchKernelGGL(k_force_tet<T>, dim3((w->ne[0] + EX_WG - 1) / EX_WG), dim3(EX_WG), 0, c->stream, w->ne[0],
-                           w->conn[0], (const T *)w->xyz_t, ucur, lam, mu, (T *)w->fe[0], stop);
+    if (w->ne[1]) {
+        const int q = xcd_q(w->ne[1]);
+        hipLaunchKernelGGL(k_force_hex<T>, dim3(8 * q), dim3(EX_WG), 0, c->stream, w->ne[1], q, w->conn[1],
+                           (const T *)w->xyz_t, ucur, lam, mu, (T *)w->fe[1], stop);
+    }
+    if (w->ne[0]) {
+        const int q = xcd_q(w->ne[0]);
+        hipLaunchKernelGGL(k_force_tet<T>, dim3(8 * q), dim3(EX_WG), 0, c->stream, w->ne[0], q, w->conn[0],
+                           (const T *)w->xyz_t, ucur, lam, mu, (T *)w->fe[0], stop);
+    }
 }
 
 template <class T>
 static void launch_step(ofea_ctx *c, ExplicitWS *w, T lam, T mu, T dt, const T *ucur, T *unext)
 {
     launch_forces<T>(c, w, lam, mu, ucur, nullptr);
-    hipLaunchKernelGGL(k_node_update<T>, dim3((w->nd + EX_WG - 1) / EX_WG), dim3(EX_WG), 0, c->stream, w->nd,
+    hipLaunchKernelGGL(k_node_update<T>, dim3(8 * xcd_q(w->nd)), dim3(EX_WG), 0, c->stream, w->nd, xcd_q(w->nd),
                        w->inc_ptr, w->inc, w->ne[0], w->ne[1], (const T *)w->fe[0], (const T *)w->fe[1], ucur, unext,
-                       (T *)w->v,
-                       (const T *)w->minv, (const T *)w->fext, w->fixed, dt);
+                       (T *)w->v, (const T *)w->minv, (const T *)w->fext, w->fixed, dt);
 }
 
 static void drop_graphs(ExplicitWS *w)
@@ -716,7 +847,7 @@ __global__ __launch_bounds__(EX_WG) void k_time_ctl(TimeCtl *__restrict__ tc, co
 // u_next = u + v cu (0 at clamped nodes); u is saved to its frame slot first
 // when this step saves.
 template <class T>
-__global__ __launch_bounds__(EX_WG) void k_node_update_tc(int nd, const int *__restrict__ inc_ptr,
+__global__ __launch_bounds__(EX_WG) void k_node_update_tc(int nd, int q, const int *__restrict__ inc_ptr,
                                                           const int *__restrict__ inc, int ne4, int ne8,
                                                           const T *__restrict__ fe4, const T *__restrict__ fe8,
                                                           const T *__restrict__ ucur, T *__restrict__ unext,
@@ -725,17 +856,11 @@ __global__ __launch_bounds__(EX_WG) void k_node_update_tc(int nd, const int *__r
                                                           const TimeCtl *__restrict__ tc, T *__restrict__ frames)
 {
     if (!tc->node_go) return;
-    const int i = blockIdx.x * EX_WG + threadIdx.x;
+    const int i = xcd_block(q) * EX_WG + threadIdx.x;
     if (i >= nd) return;
     const T cv = (T)tc->cv, cu = (T)tc->cu;
     T f0 = T(0), f1 = T(0), f2 = T(0);
-    for (int t = inc_ptr[i]; t < inc_ptr[i + 1]; ++t) {
-        int code = inc[t], el = code >> 4, type = (code >> 3) & 1, a = code & 7;
-        const T *src = type ? fe8 + 3 * ((size_t)a * ne8 + el) : fe4 + 3 * ((size_t)a * ne4 + el);
-        f0 -= src[0];
-        f1 -= src[1];
-        f2 -= src[2];
-    }
+    gather_forces<T>(i, inc_ptr, inc, ne4, ne8, fe4, fe8, f0, f1, f2);
     const T u0 = ucur[3 * i], u1 = ucur[3 * i + 1], u2 = ucur[3 * i + 2];
     if (tc->save) {
         T *fr = frames + (size_t)tc->slot * 3 * nd + 3 * i;
@@ -800,7 +925,7 @@ static void launch_step_tc(ofea_ctx *c, ExplicitWS *w, T lam, T mu, double lam_d
     if (auto_dt) launch_dt<T>(c, w, lam_d, mu_d, ucur);
     hipLaunchKernelGGL(k_time_ctl, dim3(1), dim3(EX_WG), 0, c->stream, w->tc, (const double *)w->dtpart,
                        dt_parts(w), 0, w->fidx, w->ft);
-    hipLaunchKernelGGL(k_node_update_tc<T>, dim3((w->nd + EX_WG - 1) / EX_WG), dim3(EX_WG), 0, c->stream, w->nd,
+    hipLaunchKernelGGL(k_node_update_tc<T>, dim3(8 * xcd_q(w->nd)), dim3(EX_WG), 0, c->stream, w->nd, xcd_q(w->nd),
                        w->inc_ptr, w->inc, w->ne[0], w->ne[1], (const T *)w->fe[0], (const T *)w->fe[1], ucur, unext,
                        (T *)w->v, (const T *)w->minv, (const T *)w->fext, w->fixed, (const TimeCtl *)w->tc,
                        (T *)w->frames);
