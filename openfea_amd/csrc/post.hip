@@ -22,6 +22,9 @@
 //     node's elements in ascending element order (the reference's loop
 //     order) and divide by the number of element incidences; a node in no
 //     element keeps 0.
+// On all-tet meshes k_post_tet_fused computes the nodal values in one pass
+// instead, recomputing each tet at its 4 nodes (no element rows in memory);
+// k_post_elem then runs only when the element values are asked for.
 #include <algorithm>
 #include <vector>
 
@@ -45,6 +48,7 @@ struct PostWs {
     double *ev = nullptr;    // ne x 8: the 7 element values of element e in row e
     double *ev7 = nullptr;   // 7 x ne, component-major copy for the caller (allocated on first request)
     double *nv = nullptr;    // 7 x nd
+    double *xu = nullptr;    // nd x 6 packed coordinates and displacement (all-tet meshes)
 };
 
 __device__ inline double mises_form(const double s[6])
@@ -154,6 +158,106 @@ __global__ __launch_bounds__(POST_WG) void k_post_elem(int ne, const int *__rest
     for (int i = 0; i < 4; ++i) row[i] = make_double2(out[2 * i], out[2 * i + 1]);
 }
 
+// One 48-byte record per node for the fused pass: x, y, z, ux, uy, uz (the
+// displacement of a clamped node is 0 in the resident form).
+template <bool RESIDENT>
+__global__ __launch_bounds__(POST_WG) void k_post_pack(int nd, const double *__restrict__ xyz,
+                                                       const double *__restrict__ u, const int *__restrict__ resort,
+                                                       double *__restrict__ xu)
+{
+    const int i = blockIdx.x * POST_WG + threadIdx.x;
+    if (i >= nd) return;
+    double r[6];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[c] = xyz[3 * (size_t)i + c];
+    if (RESIDENT) {
+        const int f = resort[i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r[3 + c] = f >= 0 ? u[3 * (size_t)f + c] : 0.0;
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r[3 + c] = u[3 * (size_t)i + c];
+    }
+    double2 *dst = reinterpret_cast<double2 *>(xu + 6 * (size_t)i);
+    dst[0] = make_double2(r[0], r[1]);
+    dst[1] = make_double2(r[2], r[3]);
+    dst[2] = make_double2(r[4], r[5]);
+}
+
+// The 7 values of tet e (strain or stress and the Mises form) from the
+// packed node records and the 16-byte tet connectivity rows (conn4, ne x 4,
+// 0-based): 1 + 4 x 3 sixteen-byte loads per tet.
+__device__ __forceinline__ void tet_values(int e, const int *__restrict__ conn4, const double *__restrict__ xu,
+                                           const double *__restrict__ Eel, double E, double nu, int quantity,
+                                           double out[7])
+{
+    const int4 nodes = *reinterpret_cast<const int4 *>(conn4 + 4 * (size_t)e);
+    const int nn[4] = {nodes.x, nodes.y, nodes.z, nodes.w};
+    double X[12], U[12];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const double2 *r = reinterpret_cast<const double2 *>(xu + 6 * (size_t)nn[a]);
+        const double2 p0 = r[0], p1 = r[1], p2 = r[2];
+        X[3 * a] = p0.x;
+        X[3 * a + 1] = p0.y;
+        X[3 * a + 2] = p1.x;
+        U[3 * a] = p1.y;
+        U[3 * a + 1] = p2.x;
+        U[3 * a + 2] = p2.y;
+    }
+    double g[3][4];
+    tet_grads(X, g);
+    double eps[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    strain_of<4>(g, U, 1.0, eps);
+    if (quantity == OFEA_POST_STRESS) {
+        double lam, mu;
+        lame<double>(Eel ? Eel[e] : E, nu, lam, mu);
+        const double tr = eps[0] + eps[1] + eps[2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) out[i] = lam * tr + 2.0 * mu * eps[i];
+#pragma unroll
+        for (int i = 3; i < 6; ++i) out[i] = mu * eps[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) out[i] = eps[i];
+    }
+    out[6] = mises_form(out);
+}
+
+// Nodal averages of an all-tet mesh in one pass, without element rows in
+// memory: eight lanes per node take its incidences k0 + j, k0 + j + 8, ...
+// and recompute those tets' values from the packed node records (each tet is
+// met by its 4 nodes); the lanes' sums are added in a fixed tree (lanes 4
+// apart, then 2, then 1) and lane 0 writes the 7 components [c][node].  Same
+// sums as k_post_elem + k_post_node up to the order of the additions.
+__global__ __launch_bounds__(POST_WG) void k_post_tet_fused(int nd, const int *__restrict__ inc_ptr,
+                                                            const int *__restrict__ inc,
+                                                            const int *__restrict__ conn4,
+                                                            const double *__restrict__ xu,
+                                                            const double *__restrict__ Eel, double E, double nu,
+                                                            int quantity, double *__restrict__ nv)
+{
+    const int t = xcd_remap(blockIdx.x, gridDim.x) * POST_WG + threadIdx.x;
+    const int i = t >> 3, j = t & 7;
+    if (i >= nd) return;  // whole 8-lane groups leave together
+    const int k0 = inc_ptr[i], k1 = inc_ptr[i + 1];
+    double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = k0 + j; k < k1; k += 8) {
+        double v[7];
+        tet_values(inc[k], conn4, xu, Eel, E, nu, quantity, v);
+#pragma unroll
+        for (int c = 0; c < 7; ++c) acc[c] += v[c];
+    }
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1)
+#pragma unroll
+        for (int c = 0; c < 7; ++c) acc[c] += __shfl_down(acc[c], o, 8);
+    if (j == 0) {
+#pragma unroll
+        for (int c = 0; c < 7; ++c) nv[(size_t)c * nd + i] = k1 > k0 ? acc[c] / (double)(k1 - k0) : 0.0;
+    }
+}
+
 // Eight lanes per node, lane c < 7 owns component c: the lanes of a node read
 // one 64-byte element row per incidence, in ascending element order.  Output
 // component-major [c][node] (the layout the caller receives).
@@ -193,6 +297,7 @@ void free_ws(PostWs *w)
     dev_free(w->ev);
     dev_free(w->ev7);
     dev_free(w->nv);
+    dev_free(w->xu);
     delete w;
 }
 
@@ -279,16 +384,34 @@ int post_nodal(ofea_ctx *c, double E, double nu, const double *u_full, int quant
     if (!c->E_elem.empty() && (rc = dev_upload(w->E, c->E_elem.data(), c->E_elem.size()))) return rc;
     const int nd = c->nd, ne = c->ne;
     const dim3 ge((ne + POST_WG - 1) / POST_WG), gn((8 * nd + POST_WG - 1) / POST_WG), blk(POST_WG);
-    if (u_full) {
-        OFEA_HIP(hipMemcpyAsync(w->u, u_full, sizeof(double) * 3 * nd, hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_post_elem<false>, ge, blk, 0, c->stream, ne, w->conn, w->rank8, w->etype, c->d_xyz, w->u,
-                           w->E, E, nu, quantity, w->ev);
-    } else {  // the context's resident free-DOF solution, read through the nodes' free ranks
-        hipLaunchKernelGGL(k_post_elem<true>, ge, blk, 0, c->stream, ne, w->conn, w->rank8, w->etype, c->d_xyz, c->d_x,
-                           w->E, E, nu, quantity, w->ev);
+    const bool resident = u_full == nullptr;  // else the context's free-DOF solution, read through the free ranks
+    const double *uu = resident ? c->d_x : w->u;
+    if (!resident) OFEA_HIP(hipMemcpyAsync(w->u, u_full, sizeof(double) * 3 * nd, hipMemcpyHostToDevice, c->stream));
+    // all-tet meshes: nodal values in one pass (element rows only when asked for);
+    // with hexes (8 Gauss points each) the element pass runs once and its rows are summed
+    const bool fused = !c->has_type[OFEA_C3D8];
+    if (!fused || elem_out) {
+        if (resident)
+            hipLaunchKernelGGL(k_post_elem<true>, ge, blk, 0, c->stream, ne, w->conn, w->rank8, w->etype, c->d_xyz,
+                               uu, w->E, E, nu, quantity, w->ev);
+        else
+            hipLaunchKernelGGL(k_post_elem<false>, ge, blk, 0, c->stream, ne, w->conn, w->rank8, w->etype, c->d_xyz,
+                               uu, w->E, E, nu, quantity, w->ev);
+        OFEA_HIP(hipGetLastError());
     }
-    OFEA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_post_node, gn, blk, 0, c->stream, nd, w->inc_ptr, w->inc, w->ev, w->nv);
+    if (fused) {
+        if (!w->xu && (rc = dev_upload<double>(w->xu, nullptr, 6 * (size_t)nd))) return rc;
+        const dim3 gp((nd + POST_WG - 1) / POST_WG);
+        if (resident)
+            hipLaunchKernelGGL(k_post_pack<true>, gp, blk, 0, c->stream, nd, c->d_xyz, uu, c->d_resort, w->xu);
+        else
+            hipLaunchKernelGGL(k_post_pack<false>, gp, blk, 0, c->stream, nd, c->d_xyz, uu, c->d_resort, w->xu);
+        // all tets: the type-local connectivity of the assembly plans is in global element order
+        hipLaunchKernelGGL(k_post_tet_fused, gn, blk, 0, c->stream, nd, w->inc_ptr, w->inc, c->dp[OFEA_C3D4].conn,
+                           w->xu, w->E, E, nu, quantity, w->nv);
+    } else {
+        hipLaunchKernelGGL(k_post_node, gn, blk, 0, c->stream, nd, w->inc_ptr, w->inc, w->ev, w->nv);
+    }
     OFEA_HIP(hipGetLastError());
     if (elem_out) {
         if (!w->ev7 && (rc = dev_upload<double>(w->ev7, nullptr, 7 * (size_t)ne))) return rc;
