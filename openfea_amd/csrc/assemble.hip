@@ -325,123 +325,172 @@ __global__ __launch_bounds__(ASM_WG) void k_assemble_tet(const double *__restric
 
 // ------------------------------------------------------------------ C3D4, chunk plan
 // The linear C3D4 path (TetChunkPlan, built by build_tet_chunks): one
-// workgroup per chunk of consecutive free nodes.
-//   phase 1: thread e computes the gradients of chunk element e and mu V into
-//            LDS rows (row 3 a + c = d N_a / d x_c, row 12 = mu V; the row
-//            stride is padded by two doubles so that the rows of one element
-//            fall in different banks);
-//   phase 2: one loop for every task: Q = sum (mu V g_col) (x) g_row over its
-//            contributors, K = rho Q + Q^T + I tr Q.  For a diagonal block the
-//            contributors are (e, a, a), so the same formula gives
-//            (rho + 1) sum mu V g (x) g + I sum mu V |g|^2; its four quarter
-//            tasks are added by a fixed quad reduction (DPP);
-//   output:  every task writes its block into an LDS image of the chunk's
-//            contiguous slice of the CSC values, then the slice goes to HBM
-//            with 16-byte stores.
-// Summation order: off-diagonal blocks ascending element order (that of
-// src/assemble.cpp:106-149); diagonal blocks the quarters in ascending
-// element order, then (q0 + q1) + (q2 + q3).  Every order is fixed: the values
-// are bitwise reproducible.
-constexpr int TC_RS = TC_ECAP + 2;    // LDS row stride in doubles
-constexpr int TC_ROWB = 8 * TC_RS;    // bytes per row
-constexpr int TC_WOFF = 12 * TC_ROWB; // byte offset of the mu V row
+// workgroup per chunk of consecutive free nodes.  Every per-chunk array has
+// a fixed stride, so all loads but the coordinate gather are issued at once.
+//   staging: lane i < TC_NTAB loads the coordinates of chunk node i into LDS;
+//   phase 1: lane e forms the cofactor gradients of chunk element e,
+//            c_1 = e1 x e2, c_2 = e2 x e0, c_3 = e0 x e1, c_0 = -(c_1 + c_2 + c_3)
+//            (e_i = x_(i+1) - x_0; grad N_a = c_a / det, det = e0 . c_1) and
+//            f = mu / (6 det) into LDS rows (row 3 a + c = c_a[c], row 12 = f);
+//            since w g_a (x) g_b = f c_a (x) c_b with w = mu det / 6 the
+//            element needs one division and no scaling of its gradients.
+//            The row stride TC_RS is a multiple of 16 doubles, so every row of
+//            a slot lies in the same bank and the plan orders each lane's
+//            contributors so that a quarter wave reads different slots modulo
+//            16 at each step; slot TC_ECAP is a zero element;
+//   phase 2: lane l sums Q = sum (f c_col) (x) c_row over its contributors
+//            (the wave's longest run; shorter runs read the zero element),
+//            the lanes of a group add their sums by a fixed DPP tree (only the
+//            steps the wave's largest group needs) and the first lane of the
+//            block forms K = rho Q + Q^T + I tr Q (a diagonal block has
+//            c_col = c_row);
+//   output:  every block goes to an LDS image of the chunk's contiguous slice
+//            of the CSC values, which is then copied out with 16-byte stores.
+// Summation order: each lane's run in the plan's order (its contributors are
+// those of src/assemble.cpp:106-149, reordered for the LDS banks), then the
+// group tree; every order is fixed, so the values are bitwise reproducible.
+constexpr int TC_RS = 256;              // LDS row stride in doubles (rows of slots 0..TC_ECAP)
+constexpr int TC_ROWB = 8 * TC_RS;      // bytes per row
+constexpr int TC_WOFF = 12 * TC_ROWB;   // byte offset of the f row
+static_assert(TC_ECAP <= 255, "element slots are 8-bit");
+static_assert(TC_RS > TC_ECAP && TC_RS % 16 == 0, "a multiple of 16 doubles: every row of a slot is in the same LDS bank");
+static_assert(9 * TC_WG + 1 <= 13 * TC_RS, "the slice image (at most one block per lane) fits the element rows");
+static_assert(3 * TC_NTAB <= 13 * TC_RS, "the staged coordinates fit the element rows");
 
 __device__ inline double lds_f64(const char *base, unsigned off) { return *reinterpret_cast<const double *>(base + off); }
 
+// x + (value of the partner lane) for lanes with do_it, x otherwise; the
+// partner is given by a DPP control that stays within a row of 16 lanes
+template <int CTRL>
+__device__ inline double tree_step(double x, bool do_it)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, true);
+    const double s = x + __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    return do_it ? s : x;
+}
+
+// one cross product: c = u x v
+__device__ inline void cross3(const double u[3], const double v[3], double c[3])
+{
+    c[0] = u[1] * v[2] - u[2] * v[1];
+    c[1] = u[2] * v[0] - u[0] * v[2];
+    c[2] = u[0] * v[1] - u[1] * v[0];
+}
+
 __global__ __launch_bounds__(TC_WG) void k_assemble_tet_chunks(const double *__restrict__ xyz,
-                                                               const int *__restrict__ conn,
-                                                               const double *__restrict__ Eel, double E, double nu,
+                                                               const double *__restrict__ Eel, double fsc,
+                                                               double fsc_per_E, double rho,
                                                                const int *__restrict__ hdr,
-                                                               const int *__restrict__ elem,
-                                                               const uint16_t *__restrict__ ct,
-                                                               const uint32_t *__restrict__ task,
-                                                               const uint8_t *__restrict__ task_k,
-                                                               const int *__restrict__ node_blk,
+                                                               const int *__restrict__ ntab,
+                                                               const uint32_t *__restrict__ eloc,
+                                                               const int *__restrict__ eid,
+                                                               const uint4 *__restrict__ task,
                                                                double *__restrict__ val, int accumulate, int nchunks)
 {
-    static_assert(TC_SLICE + 1 <= 13 * TC_RS + 2, "the slice image fits the element rows");
-    static_assert(TC_CTCAP % TC_WG == 0, "whole rounds of the workgroup");
-    __shared__ double s_g[13 * TC_RS + 2];  // element rows, then the slice image
-    __shared__ uint16_t s_ct[TC_CTCAP];
-    __shared__ int s_nb[TC_NCAP + 1];
+    __shared__ double s_g[13 * TC_RS];  // staged coordinates, then the element rows, then the slice image
     const char *lb = reinterpret_cast<const char *>(s_g);
     const int chunk = xcd_remap(blockIdx.x, nchunks);
-    const int *h = hdr + 16 * chunk;
-    const int n0 = h[0], nn = h[1], e0 = h[2], ne = h[3], c0 = h[4], nct = h[5], b0 = h[6], nblk = h[7];
-    const int t0 = h[8], ntask = h[9], ndq = h[10];
     const int t = threadIdx.x;
-    // every global load below depends on the header only
-    int nbv = 0;
-    if (t <= nn) nbv = node_blk[n0 + t];
-    uint32_t ti = 0;
-    int k = 255;
-    if (t < ntask) {
-        ti = task[t0 + t];
-        k = task_k[t0 + t];
+    // every load but the coordinates depends on the chunk index only
+    const int b0 = hdr[2 * chunk], nblk = hdr[2 * chunk + 1];
+    const int node = t < TC_NTAB ? ntab[(size_t)chunk * TC_NTAB + t] : -1;
+    const uint32_t el4 = t < TC_ECAP ? eloc[(size_t)chunk * TC_ECAP + t] : ~0u;
+    const int ei = (Eel && t < TC_ECAP) ? eid[(size_t)chunk * TC_ECAP + t] : -1;
+    const uint4 tk = task[(size_t)chunk * TC_WG + t];
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+    if (node >= 0) {
+        const double *p = xyz + 3 * (size_t)node;
+        x0 = p[0];
+        x1 = p[1];
+        x2 = p[2];
     }
-    constexpr int CT_PER = TC_CTCAP / TC_WG;
-    uint16_t ctv[CT_PER];
-#pragma unroll
-    for (int u = 0; u < CT_PER; ++u) ctv[u] = (t + u * TC_WG < nct) ? ct[c0 + t + u * TC_WG] : (uint16_t)0;
-    double lam0, mu0;
-    lame<double>(E, nu, lam0, mu0);
-    const double rho = 2.0 * nu / (1.0 - 2.0 * nu);  // lambda / mu for every element (one nu per launch)
-    if (t < ne) {
-        const int el = elem[e0 + t];
-        const int4 cn = reinterpret_cast<const int4 *>(conn)[el];  // the type-local connectivity, 0-based
-        const int nd[4] = {cn.x, cn.y, cn.z, cn.w};
-        const char *xb = reinterpret_cast<const char *>(xyz);
-        double X[12];
+    const double fs = ei >= 0 ? Eel[ei] * fsc_per_E : fsc;  // mu / 6
+    if (t < TC_NTAB) {
+        s_g[t] = x0;
+        s_g[TC_NTAB + t] = x1;
+        s_g[2 * TC_NTAB + t] = x2;
+    }
+    __syncthreads();
+    const bool has_el = el4 != ~0u;
+    double X[12];
+    if (has_el) {
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const unsigned o = (unsigned)nd[b] * 24u;  // 32-bit offsets from the array base
-            X[3 * b] = *reinterpret_cast<const double *>(xb + o);
-            X[3 * b + 1] = *reinterpret_cast<const double *>(xb + o + 8);
-            X[3 * b + 2] = *reinterpret_cast<const double *>(xb + o + 16);
+            const int sl = (el4 >> (8 * b)) & 255;
+            X[3 * b] = s_g[sl];
+            X[3 * b + 1] = s_g[TC_NTAB + sl];
+            X[3 * b + 2] = s_g[2 * TC_NTAB + sl];
         }
-        double g[3][4];
-        const double detJ = tet_grads(X, g);
-        double mu = mu0, lam;
-        if (Eel) lame<double>(Eel[el], nu, lam, mu);
+    }
+    __syncthreads();  // the coordinates are read: the rows overwrite them
+    if (has_el) {
+        double e[3][3], c[4][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) e[i][j] = X[3 * (i + 1) + j] - X[j];
+        cross3(e[1], e[2], c[1]);
+        cross3(e[2], e[0], c[2]);
+        cross3(e[0], e[1], c[3]);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) c[0][j] = -(c[1][j] + c[2][j] + c[3][j]);
+        // signed det (src/Elements/C3D4.cpp:45-54 does not abs() the volume)
+        const double det = e[0][0] * c[1][0] + e[0][1] * c[1][1] + e[0][2] * c[1][2];
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) s_g[(3 * b + c) * TC_RS + t] = g[c][b];
-        s_g[12 * TC_RS + t] = mu * (detJ / 6.0);  // signed V (src/Elements/C3D4.cpp:45-54 does not abs())
-    }
+            for (int j = 0; j < 3; ++j) s_g[(3 * b + j) * TC_RS + t] = c[b][j];
+        s_g[12 * TC_RS + t] = fs / det;
+    } else if (t == TC_ECAP) {
 #pragma unroll
-    for (int u = 0; u < CT_PER; ++u)
-        if (t + u * TC_WG < nct) s_ct[t + u * TC_WG] = ctv[u];
-    if (t <= nn) s_nb[t] = nbv - b0;
+        for (int j = 0; j < 13; ++j) s_g[j * TC_RS + t] = 0.0;
+    }
     __syncthreads();
-    const int st = ti & 2047, cnt = (ti >> 11) & 127, quarter = (ti >> 18) & 1, rl = (ti >> 19) & 63;
+    const uint32_t meta = tk.x;
+    const uint32_t cw[3] = {tk.y, tk.z, tk.w};
+    const int trip = __builtin_amdgcn_readfirstlane((int)((tk.y >> 12) & 7u));  // the wave's longest run
     double Q[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int l = st; l < st + cnt; ++l) {
-        const unsigned ci = s_ct[l];
-        const unsigned e8 = (ci & 255u) * 8u;
-        const unsigned ga = e8 + ((ci >> 8) & 3u) * (3u * TC_ROWB), gb = e8 + (ci >> 10) * (3u * TC_ROWB);
-        const double w = lds_f64(lb, e8 + TC_WOFF);
-        const double a0 = lds_f64(lb, ga), a1 = lds_f64(lb, ga + TC_ROWB), a2 = lds_f64(lb, ga + 2 * TC_ROWB);
-        const double q0 = lds_f64(lb, gb), q1 = lds_f64(lb, gb + TC_ROWB), q2 = lds_f64(lb, gb + 2 * TC_ROWB);
-        const double m0 = w * a0, m1 = w * a1, m2 = w * a2;
-        Q[0] += m0 * q0;
-        Q[1] += m0 * q1;
-        Q[2] += m0 * q2;
-        Q[3] += m1 * q0;
-        Q[4] += m1 * q1;
-        Q[5] += m1 * q2;
-        Q[6] += m2 * q0;
-        Q[7] += m2 * q1;
-        Q[8] += m2 * q2;
-    }
-    // quarters of the diagonal blocks occupy lanes [0, ndq) four per node, so a
-    // quad never mixes quarters with other tasks; only their waves reduce
-    if (__builtin_amdgcn_readfirstlane(t) < ndq) {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            double sq = Q[i] + dpp_xor1(Q[i]);
-            sq += dpp_xor2(sq);
-            Q[i] = quarter ? sq : Q[i];
+    for (int i = 0; i < TC_CPL; ++i) {
+        if (i < trip) {
+            const unsigned ci = (cw[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
+            const unsigned e8 = (ci & 255u) * 8u;
+            const unsigned ga = e8 + ((ci >> 8) & 3u) * (3u * TC_ROWB), gb = e8 + ((ci >> 10) & 3u) * (3u * TC_ROWB);
+            const double f = lds_f64(lb, e8 + TC_WOFF);
+            const double a0 = lds_f64(lb, ga), a1 = lds_f64(lb, ga + TC_ROWB), a2 = lds_f64(lb, ga + 2 * TC_ROWB);
+            const double q0 = lds_f64(lb, gb), q1 = lds_f64(lb, gb + TC_ROWB), q2 = lds_f64(lb, gb + 2 * TC_ROWB);
+            const double m0 = f * a0, m1 = f * a1, m2 = f * a2;
+            Q[0] += m0 * q0;
+            Q[1] += m0 * q1;
+            Q[2] += m0 * q2;
+            Q[3] += m1 * q0;
+            Q[4] += m1 * q1;
+            Q[5] += m1 * q2;
+            Q[6] += m2 * q0;
+            Q[7] += m2 * q1;
+            Q[8] += m2 * q2;
+        }
+    }
+    // lane groups: aligned, largest first, so lane 0 of a wave has the
+    // wave's largest group and the tree runs only the steps it needs
+    const int glog = (meta >> 24) & 7;
+    const int gmax = __builtin_amdgcn_readfirstlane(glog);
+    if (gmax >= 1) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Q[i] = tree_step<0xB1>(Q[i], glog >= 1);  // quad_perm [1,0,3,2]
+        if (gmax >= 2) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Q[i] = tree_step<0x4E>(Q[i], glog >= 2);  // quad_perm [2,3,0,1]
+            if (gmax >= 3) {
+#pragma unroll
+                for (int i = 0; i < 9; ++i) Q[i] = tree_step<0x141>(Q[i], glog >= 3);  // row_half_mirror
+                if (gmax >= 4) {
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) Q[i] = tree_step<0x140>(Q[i], glog >= 4);  // row_mirror
+                }
+            }
         }
     }
     double K[9];
@@ -456,9 +505,9 @@ __global__ __launch_bounds__(TC_WG) void k_assemble_tet_chunks(const double *__r
     const long long v0 = 9LL * b0;
     const int lead = (int)(v0 & 1);
     double *s_out = s_g + lead;
-    if (k != 255) {
-        const int nb = s_nb[rl], deg = s_nb[rl + 1] - nb;
-        double *o = s_out + 9 * nb + 3 * k;  // column 3r + c holds rows 3s + d at c (3 deg) + 3 k + d
+    if ((meta >> 27) & 1u) {
+        const int k = meta & 255, nbl = (meta >> 8) & 255, deg = (meta >> 16) & 255;
+        double *o = s_out + 9 * nbl + 3 * k;  // column 3r + c holds rows 3s + d at c (3 deg) + 3 k + d
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -565,9 +614,14 @@ int launch_assemble_plan(ofea_ctx *ctx, int type, const DevPlan &dp, const doubl
     const DevTetChunks &tc = ctx->dtc;
     if (type == OFEA_C3D4 && !dp.blk_gid && tc.nchunks > 0 && d_xyz == ctx->d_xyz) {
         // the full linear C3D4 assembly of the context's own mesh: chunk plan
-        hipLaunchKernelGGL(k_assemble_tet_chunks, dim3(tc.nchunks), dim3(TC_WG), 0, ctx->stream, d_xyz, dp.conn, dp.E, E,
-                           nu, tc.hdr, tc.elem, tc.ct, tc.task, tc.task_k, ctx->d_node_blk, d_out, accumulate,
-                           tc.nchunks);
+        // mu / 6 for the launch's modulus and per unit modulus (per-element E), and lambda / mu
+        double lam1, mu1;
+        lame<double>(1.0, nu, lam1, mu1);
+        double lamE, muE;
+        lame<double>(E, nu, lamE, muE);
+        hipLaunchKernelGGL(k_assemble_tet_chunks, dim3(tc.nchunks), dim3(TC_WG), 0, ctx->stream, d_xyz, dp.E,
+                           muE / 6.0, mu1 / 6.0, 2.0 * nu / (1.0 - 2.0 * nu), tc.hdr, tc.ntab, tc.eloc, tc.eid,
+                           reinterpret_cast<const uint4 *>(tc.task), d_out, accumulate, tc.nchunks);
     } else if (type == OFEA_C3D4 && dp.blk_gid)
         hipLaunchKernelGGL(k_assemble_tet<true>, dim3(dp.nchunks), dim3(ASM_WG), 0, ctx->stream, d_xyz, dp.conn, dp.E,
                            E, nu, dp.chunk_pair, dp.chunk_blk, dp.chunk_node, dp.node_pair, dp.pair, dp.blk_rl,

@@ -78,10 +78,10 @@ static void free_mesh(ofea_ctx *c)
     dev_free(c->d_fint);
     for (auto &d : c->dp) free_plan(d);
     dev_free(c->dtc.hdr);
-    dev_free(c->dtc.elem);
-    dev_free(c->dtc.ct);
+    dev_free(c->dtc.ntab);
+    dev_free(c->dtc.eloc);
+    dev_free(c->dtc.eid);
     dev_free(c->dtc.task);
-    dev_free(c->dtc.task_k);
     c->dtc = DevTetChunks();
     c->assembled = false;
     c->rhs_set = false;
@@ -124,10 +124,10 @@ static int upload_mesh(ofea_ctx *c)
     if (c->TC.nchunks > 0) {
         const TetChunkPlan &T = c->TC;
         if ((rc = dev_upload(c->dtc.hdr, T.hdr.data(), T.hdr.size()))) return rc;
-        if ((rc = dev_upload(c->dtc.elem, T.elem.data(), T.elem.size()))) return rc;
-        if ((rc = dev_upload(c->dtc.ct, T.ct.data(), std::max<size_t>(T.ct.size(), 1)))) return rc;
-        if ((rc = dev_upload(c->dtc.task, T.task.data(), std::max<size_t>(T.task.size(), 1)))) return rc;
-        if ((rc = dev_upload(c->dtc.task_k, T.task_k.data(), std::max<size_t>(T.task_k.size(), 1)))) return rc;
+        if ((rc = dev_upload(c->dtc.ntab, T.ntab.data(), T.ntab.size()))) return rc;
+        if ((rc = dev_upload(c->dtc.eloc, T.eloc.data(), T.eloc.size()))) return rc;
+        if ((rc = dev_upload(c->dtc.eid, T.eid.data(), T.eid.size()))) return rc;
+        if ((rc = dev_upload(c->dtc.task, T.task.data(), T.task.size()))) return rc;
         c->dtc.nchunks = T.nchunks;
     }
     return OFEA_OK;
@@ -205,7 +205,7 @@ int ofea_set_mesh(ofea_ctx *c, int nd, const double *xyz, int ne, const int *con
     }
     // chunk plan of the C3D4 assembly; a mesh beyond its capacity keeps the
     // pair-gather kernel (the limit is not an error of set_mesh)
-    if (has_type[OFEA_C3D4] && build_tet_chunks(P, G[OFEA_C3D4], resort, TC) != OFEA_OK) TC = TetChunkPlan();
+    if (has_type[OFEA_C3D4] && build_tet_chunks(nd, P, G[OFEA_C3D4], resort, TC) != OFEA_OK) TC = TetChunkPlan();
     c->xyz.assign(xyz, xyz + 3 * (size_t)nd);
     c->conn.assign(conn, conn + 8 * (size_t)ne);
     c->etype.assign(etype, etype + ne);

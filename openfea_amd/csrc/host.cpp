@@ -426,103 +426,191 @@ int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const i
     return OFEA_OK;
 }
 
-int build_tet_chunks(const Pattern &P, const GatherPlan &G, const int *resort, TetChunkPlan &T)
+int build_tet_chunks(int nd, const Pattern &P, const GatherPlan &G, const int *resort, TetChunkPlan &T)
 {
     T = TetChunkPlan();
     const int nf = P.nfree;
-    std::vector<int> loc(G.ne, -1);  // chunk element index of a type-local element, -1 = not in the chunk
-    struct Task {
-        int st, cnt, quarter, rl, k;
+    std::vector<int> eslot(G.ne, -1);  // chunk element slot while the chunk is built
+    std::vector<int> nslot(nd, -1);    // node-table slot while the chunk is built
+    // a block with c contributors takes the smallest power of two >= ceil(c / TC_CPL) lanes
+    auto group_of = [](int c) {
+        const int need = (c + TC_CPL - 1) / TC_CPL;
+        int g = 1;
+        while (g < need) g <<= 1;
+        return g;
     };
-    std::vector<Task> diag, off;
-    std::vector<int> els;
+    struct Block {
+        int q, k, g;
+        std::vector<uint32_t> ent;  // type-local element << 4 | slot_col << 2 | slot_row, ascending element
+    };
+    // the blocks of node q (slot k order), contributors in ascending element order
+    auto node_blocks = [&](int q, std::vector<Block> &out) {
+        const int p0 = G.node_pair[q], p1 = G.node_pair[q + 1];
+        const int nb = P.node_blk[q], deg = P.node_blk[q + 1] - nb;
+        const int *nbr = P.bcol.data() + nb;
+        out.assign(deg, Block());
+        for (int k = 0; k < deg; ++k) out[k].q = q, out[k].k = k;
+        for (int p = p0; p < p1; ++p) {  // pairs ascend by element
+            const int el = G.pair[p] >> 3, a = G.pair[p] & 7;
+            for (int b = 0; b < 4; ++b) {
+                const int s = resort[G.conn[4 * (size_t)el + b]];
+                if (s < 0) continue;
+                const int k = (int)(std::lower_bound(nbr, nbr + deg, s) - nbr);
+                out[k].ent.push_back((uint32_t)el << 4 | (uint32_t)a << 2 | (uint32_t)b);
+            }
+        }
+        for (auto &B : out) B.g = group_of((int)B.ent.size());
+    };
+    std::vector<Block> blocks, nodeb;
+    std::vector<int> els, nodes;
     int r = 0;
     while (r < nf) {
-        // grow the chunk node by node while its elements, tasks, slice and
-        // contributor entries fit
+        // grow the chunk node by node while its elements, nodes and lanes fit
+        blocks.clear();
         els.clear();
-        int nn = 0, nblk = 0, nct = 0;
+        nodes.clear();
+        int lanes = 0;
         const int r0 = r;
         while (r < nf) {
-            const int p0 = G.node_pair[r], p1 = G.node_pair[r + 1];
-            int add = 0;
-            for (int p = p0; p < p1; ++p)
-                if (loc[G.pair[p] >> 3] < 0) ++add;
-            const int deg = P.node_blk[r + 1] - P.node_blk[r];
-            const int nct_r = 4 * (p1 - p0);  // bound: 3 off-diagonal and 1 diagonal entry per pair
-            const bool fits = (int)els.size() + add <= TC_ECAP && 4 * (nn + 1) + (nblk + deg) - (nn + 1) <= TC_WG &&
-                              9 * (nblk + deg) <= TC_SLICE && nn + 1 <= TC_NCAP && nct + nct_r <= TC_CTCAP;
+            node_blocks(r, nodeb);
+            int lanes_r = 0, gmax = 1;
+            for (const auto &B : nodeb) lanes_r += B.g, gmax = std::max(gmax, B.g);
+            int add_e = 0, add_n = 0;
+            for (int p = G.node_pair[r]; p < G.node_pair[r + 1]; ++p) {
+                const int el = G.pair[p] >> 3;
+                if (eslot[el] >= 0) continue;
+                eslot[el] = -2;  // counted
+                ++add_e;
+                for (int b = 0; b < 4; ++b) {
+                    const int v = G.conn[4 * (size_t)el + b];
+                    if (nslot[v] == -1) nslot[v] = -2, ++add_n;
+                }
+            }
+            const bool fits = gmax <= TC_GMAX && (int)els.size() + add_e <= TC_ECAP &&
+                              (int)nodes.size() + add_n <= TC_NTAB && lanes + lanes_r <= TC_WG;
+            // take the counted marks back, then record what the node adds if it fits
+            for (int p = G.node_pair[r]; p < G.node_pair[r + 1]; ++p) {
+                const int el = G.pair[p] >> 3;
+                if (eslot[el] != -2) continue;
+                eslot[el] = fits ? 0 : -1;
+                if (fits) els.push_back(el);
+                for (int b = 0; b < 4; ++b) {
+                    const int v = G.conn[4 * (size_t)el + b];
+                    if (nslot[v] != -2) continue;
+                    nslot[v] = fits ? 0 : -1;
+                    if (fits) nodes.push_back(v);
+                }
+            }
             if (!fits) {
-                if (nn == 0) {
-                    for (int el : els) loc[el] = -1;
+                if (r == r0) {
+                    for (int el : els) eslot[el] = -1;
+                    for (int v : nodes) nslot[v] = -1;
                     T = TetChunkPlan();
                     return OFEA_ERR_LIMIT;  // no message: the caller falls back, set_mesh succeeds
                 }
                 break;
             }
-            for (int p = p0; p < p1; ++p) {
-                const int el = G.pair[p] >> 3;
-                if (loc[el] < 0) {
-                    loc[el] = 0;
-                    els.push_back(el);
-                }
-            }
-            nn++;
-            nblk += deg;
-            nct += nct_r;
+            for (auto &B : nodeb) blocks.push_back(std::move(B));
+            lanes += lanes_r;
             ++r;
         }
         std::sort(els.begin(), els.end());
-        for (int i = 0; i < (int)els.size(); ++i) loc[els[i]] = i;
-        const int e0 = (int)T.elem.size(), c0 = (int)T.ct.size();
-        T.elem.insert(T.elem.end(), els.begin(), els.end());
-        diag.clear();
-        off.clear();
-        for (int q = r0; q < r; ++q) {
-            const int rl = q - r0;
-            const int p0 = G.node_pair[q], p1 = G.node_pair[q + 1];
-            const int nb = P.node_blk[q], deg = P.node_blk[q + 1] - nb;
-            for (int k = 0; k < deg; ++k) {
-                const int s = P.bcol[nb + k];
-                const int st = (int)T.ct.size() - c0;
-                if (s == q) {  // (e, a, a) for every element of the node, in four quarters
-                    const int cnt = p1 - p0;
-                    for (int p = p0; p < p1; ++p) {
-                        const int a = G.pair[p] & 7;
-                        T.ct.push_back((uint16_t)(loc[G.pair[p] >> 3] | (a << 8) | (a << 10)));
-                    }
-                    for (int qq = 0; qq < 4; ++qq)
-                        diag.push_back({st + (qq * cnt) / 4, ((qq + 1) * cnt) / 4 - (qq * cnt) / 4, 1, rl,
-                                        qq == 0 ? k : 255});
-                    continue;
+        std::sort(nodes.begin(), nodes.end());
+        for (int i = 0; i < (int)els.size(); ++i) eslot[els[i]] = i;
+        for (int i = 0; i < (int)nodes.size(); ++i) nslot[nodes[i]] = i;
+        const size_t n_off = T.ntab.size(), e_off = T.eloc.size(), t_off = T.task.size();
+        T.ntab.resize(n_off + TC_NTAB, -1);
+        T.eloc.resize(e_off + TC_ECAP, ~0u);
+        T.eid.resize(e_off + TC_ECAP, -1);
+        for (int i = 0; i < (int)nodes.size(); ++i) T.ntab[n_off + i] = nodes[i];
+        for (int i = 0; i < (int)els.size(); ++i) {
+            uint32_t w = 0;
+            for (int b = 0; b < 4; ++b) w |= (uint32_t)nslot[G.conn[4 * (size_t)els[i] + b]] << (8 * b);
+            T.eloc[e_off + i] = w;
+            T.eid[e_off + i] = els[i];
+        }
+        // lanes: groups first, largest first, then single lanes by decreasing
+        // contributor count, so that lane 0 of a wave has the wave's longest run
+        std::stable_sort(blocks.begin(), blocks.end(), [](const Block &x, const Block &y) {
+            return x.g != y.g ? x.g > y.g : (x.g == 1 && x.ent.size() > y.ent.size());
+        });
+        const uint32_t pad = (uint32_t)TC_ECAP;  // the zero element, slots 0 / 0
+        T.task.resize(t_off + 4 * TC_WG);
+        for (int l = 0; l < TC_WG; ++l) {
+            T.task[t_off + 4 * l] = 0;
+            for (int w = 1; w < 4; ++w) T.task[t_off + 4 * l + w] = pad | pad << 16;
+        }
+        const int b0 = P.node_blk[r0];
+        int lane = 0;
+        for (const Block &B : blocks) {
+            const int c = (int)B.ent.size();
+            int glog = 0;
+            while ((1 << glog) < B.g) ++glog;
+            const int nbl = P.node_blk[B.q] - b0, deg = P.node_blk[B.q + 1] - P.node_blk[B.q];
+            for (int j = 0; j < B.g; ++j, ++lane) {
+                uint32_t *rec = &T.task[t_off + 4 * (size_t)lane];
+                const int i0 = (j * c) / B.g, i1 = ((j + 1) * c) / B.g;  // <= TC_CPL entries
+                rec[0] = (uint32_t)B.k | (uint32_t)nbl << 8 | (uint32_t)deg << 16 | (uint32_t)glog << 24 |
+                         (uint32_t)(j == 0) << 27 | (uint32_t)(i1 - i0) << 28 |
+                         (uint32_t)(P.bcol[P.node_blk[B.q] + B.k] == B.q) << 31;
+                for (int i = i0; i < i1; ++i) {
+                    const uint32_t e = B.ent[i];
+                    const uint32_t v = (uint32_t)eslot[e >> 4] | ((e >> 2) & 3u) << 8 | (e & 3u) << 10;
+                    const int u = i - i0;
+                    uint32_t &w = rec[1 + (u >> 1)];
+                    w = (u & 1) ? ((w & 0xFFFFu) | v << 16) : ((w & 0xFFFF0000u) | v);
                 }
-                int cnt = 0;
-                for (int p = p0; p < p1; ++p) {  // ascending element order
-                    const int el = G.pair[p] >> 3, a = G.pair[p] & 7;
-                    for (int b = 0; b < 4; ++b)
-                        if (resort[G.conn[4 * (size_t)el + b]] == s) {
-                            T.ct.push_back((uint16_t)(loc[el] | (a << 8) | (b << 10)));
-                            ++cnt;
-                        }
-                }
-                off.push_back({st, cnt, 0, rl, k});
             }
         }
-        std::stable_sort(off.begin(), off.end(), [](const Task &x, const Task &y) { return x.cnt > y.cnt; });
-        const int t0 = (int)T.task.size(), ntask = (int)(diag.size() + off.size());
-        for (const auto *v : {&diag, &off})
-            for (const Task &x : *v) {
-                if (x.st > 2047 || x.cnt > 127 || x.k > 255) {
-                    for (int el : els) loc[el] = -1;
-                    T = TetChunkPlan();
-                    return OFEA_ERR_LIMIT;
-                }
-                T.task.push_back((uint32_t)(x.st | (x.cnt << 11) | (x.quarter << 18) | (x.rl << 19)));
-                T.task_k.push_back((uint8_t)x.k);
+        // Per step, the lanes of a quarter wave read LDS together; order each
+        // lane's run so that they name different element slots modulo 16 (the
+        // LDS bank of every row of a slot when the row stride is a multiple
+        // of 16 doubles).  Greedy per step: each lane takes, of its remaining
+        // entries, the first one whose residue is least used at that step.
+        for (int q0 = 0; q0 < TC_WG; q0 += 16) {
+            std::vector<uint16_t> rem[16];
+            int steps = 0;
+            for (int l = 0; l < 16; ++l) {
+                const uint32_t *rec = &T.task[t_off + 4 * (size_t)(q0 + l)];
+                const int n = (rec[0] >> 28) & 7u;
+                for (int u = 0; u < n; ++u) rem[l].push_back((uint16_t)(rec[1 + (u >> 1)] >> (16 * (u & 1))));
+                steps = std::max(steps, n);
             }
-        T.hdr.insert(T.hdr.end(), {r0, r - r0, e0, (int)els.size(), c0, (int)T.ct.size() - c0, P.node_blk[r0],
-                                   P.node_blk[r] - P.node_blk[r0], t0, ntask, 4 * (r - r0), 0, 0, 0, 0, 0});
-        for (int el : els) loc[el] = -1;
+            std::vector<uint16_t> out[16];
+            for (int st = 0; st < steps; ++st) {
+                int used[16] = {0};
+                for (int l = 0; l < 16; ++l)
+                    if (rem[l].empty()) used[(int)pad & 15] = 1;  // the zero element: one address
+                for (int l = 0; l < 16; ++l) {
+                    if (rem[l].empty()) continue;
+                    size_t best = 0;
+                    for (size_t j = 1; j < rem[l].size(); ++j)
+                        if (used[rem[l][j] & 15] < used[rem[l][best] & 15]) best = j;
+                    used[rem[l][best] & 15]++;
+                    out[l].push_back(rem[l][best]);
+                    rem[l].erase(rem[l].begin() + (long)best);
+                }
+            }
+            for (int l = 0; l < 16; ++l) {
+                uint32_t *rec = &T.task[t_off + 4 * (size_t)(q0 + l)];
+                for (int w = 1; w < 4; ++w) rec[w] = pad | pad << 16;
+                for (size_t u = 0; u < out[l].size(); ++u) {
+                    uint32_t &w = rec[1 + (u >> 1)];
+                    w = (u & 1) ? ((w & 0xFFFFu) | (uint32_t)out[l][u] << 16) : ((w & 0xFFFF0000u) | out[l][u]);
+                }
+            }
+        }
+        // every lane of a wave carries the wave's longest run (bits 12-14 of
+        // its first contributor entry), the trip count of its summation loop
+        for (int w0 = 0; w0 < TC_WG; w0 += 64) {
+            uint32_t trip = 0;
+            for (int l = w0; l < w0 + 64; ++l) trip = std::max(trip, (T.task[t_off + 4 * (size_t)l] >> 28) & 7u);
+            for (int l = w0; l < w0 + 64; ++l) T.task[t_off + 4 * (size_t)l + 1] |= trip << 12;
+        }
+        T.hdr.push_back(b0);
+        T.hdr.push_back(P.node_blk[r] - b0);
+        for (int el : els) eslot[el] = -1;
+        for (int v : nodes) nslot[v] = -1;
         T.nchunks++;
     }
     return OFEA_OK;

@@ -76,30 +76,41 @@ int build_gather_plan(int nd, int ne, const int *conn, const int *etype, const i
 
 // Chunk plan of the linear C3D4 assembly (k_assemble_tet_chunks).  A chunk is
 // a run of consecutive free nodes, i.e. one contiguous slice of the CSC value
-// array.  Its distinct elements (ascending) are computed once per chunk, one
-// per thread, into LDS; then one lane per task sums a 3x3 block from its
-// contributors (chunk element, slot of the column node, slot of the row node)
-// in ascending element order.  Off-diagonal blocks are one task each; a
-// diagonal block is four quarter tasks over the node's elements, summed by a
-// fixed quad reduction.  Tasks are ordered diagonal quarters first, then
-// off-diagonal blocks by decreasing contributor count, so that the lanes of a
-// wave run loops of equal length.
-constexpr int TC_WG = 256;         // threads per chunk workgroup
-constexpr int TC_ECAP = 256;       // distinct elements per chunk (one per thread)
-constexpr int TC_NCAP = 64;        // nodes per chunk
-constexpr int TC_CTCAP = 2048;     // contributor entries per chunk
-constexpr int TC_SLICE = 13 * 256; // values of the chunk's slice (its LDS image reuses the element rows)
+// array.  Every per-chunk array has a fixed stride, so a workgroup finds its
+// data from its chunk index alone and the only dependent global load is the
+// coordinate gather:
+//   ntab  the distinct nodes of the chunk's elements (their coordinates are
+//         staged once per chunk in LDS), -1 padded;
+//   eloc  per distinct element (ascending) its 4 nodes as 8-bit ntab slots;
+//   task  per lane one 16-byte record: a meta word (block slot k in the node's
+//         column, the node's first block relative to the chunk, its degree,
+//         the lane group and the write flag) and up to TC_CPL contributors
+//         (chunk element, slot of the column node, slot of the row node),
+//         ordered so that the 16 lanes of a quarter wave read different
+//         element slots modulo 16 at each step (LDS banks).  A block with more contributors takes a
+//         group of 2, 4, 8 or 16 lanes, each a contiguous run of its list,
+//         summed by a fixed DPP tree; groups come first, largest first, so
+//         that they are aligned and a wave whose lane 0 has no group has none.
+//         Unused contributor entries name a zero element slot (TC_ECAP).
+constexpr int TC_WG = 256;    // lanes per chunk workgroup = task records per chunk
+constexpr int TC_ECAP = 248;  // distinct elements per chunk; slot TC_ECAP is the zero element
+constexpr int TC_NTAB = 160;  // distinct nodes per chunk
+constexpr int TC_CPL = 6;     // contributors per lane
+constexpr int TC_GMAX = 16;   // lanes per block
 struct TetChunkPlan {
     int nchunks = 0;
-    std::vector<int> hdr;           // 16 per chunk: n0 nn e0 ne c0 nct b0 nblk t0 ntask ndq (rest 0)
-    std::vector<int> elem;          // type-local element index per chunk element, ascending
-    std::vector<uint16_t> ct;       // contributor: e | slot_col << 8 | slot_row << 10
-    std::vector<uint32_t> task;     // start | count << 11 | quarter << 18 | node-in-chunk << 19
-    std::vector<uint8_t> task_k;    // block slot k in the node's column, 255 = write nothing
+    std::vector<int> hdr;         // 2 per chunk: first block of the chunk's slice, its block count
+    std::vector<int> ntab;        // TC_NTAB per chunk: 0-based node id, -1 = none
+    std::vector<uint32_t> eloc;   // TC_ECAP per chunk: ntab slots of nodes 0..3 (8 bits each), ~0u = none
+    std::vector<int> eid;         // TC_ECAP per chunk: type-local element index (per-element moduli), -1 = none
+    std::vector<uint32_t> task;   // 4 per lane, TC_WG lanes per chunk (meta word, then 6 x 16-bit contributors)
 };
+// task meta word: k | nbl << 8 | deg << 16 | log2(group) << 24 | write << 27 | count << 28 | diagonal << 31
+// contributor: chunk element | slot_col << 8 | slot_row << 10 (the first entry of a lane also
+// carries the longest count of the lane's wave in bits 12-14)
 // Builds the chunk plan; OFEA_ERR_LIMIT when a node exceeds a chunk's
 // capacity (the assembly then keeps the pair-gather kernel).
-int build_tet_chunks(const Pattern &P, const GatherPlan &G, const int *resort, TetChunkPlan &T);
+int build_tet_chunks(int nd, const Pattern &P, const GatherPlan &G, const int *resort, TetChunkPlan &T);
 
 // Pairs per gather chunk, shared by every kernel that walks a GatherPlan:
 // C3D4 one pair per thread of a 256-thread workgroup, C3D8 one pair per
@@ -123,10 +134,8 @@ struct DevPlan {
 };
 struct DevTetChunks {
     int nchunks = 0;
-    int *hdr = nullptr, *elem = nullptr;
-    uint16_t *ct = nullptr;
-    uint32_t *task = nullptr;
-    uint8_t *task_k = nullptr;
+    int *hdr = nullptr, *ntab = nullptr, *eid = nullptr;
+    uint32_t *eloc = nullptr, *task = nullptr;
 };
 
 }  // namespace ofea
