@@ -432,6 +432,13 @@ int ofea_assemble_batch(ofea_ctx *ctx, int nvar, const double *E_var, double nu,
  * OFEA_NOT_CONVERGED when a system misses rtol within maxit. */
 int ofea_solve_batch(ofea_ctx *ctx, int nvar, int nrhs, const double *B, double *X, double rtol, int maxit,
                      int *iters, double *relres);
+/* As ofea_solve_batch with the preconditioner chosen: OFEA_PCG_BLOCK_JACOBI
+ * (ofea_solve_batch's), or OFEA_PCG_TWO_LEVEL: block Jacobi plus the coarse
+ * correction Z A_c^-1 Z^T r of ofea_solve's two-level methods, A_c = Z^T K_v Z
+ * built on the device for every variant and shared by its nrhs systems (the
+ * systems run in the two-level node order; B and X keep the caller's). */
+int ofea_solve_batch_pc(ofea_ctx *ctx, int nvar, int nrhs, const double *B, double *X, double rtol, int maxit,
+                        int precond, int *iters, double *relres);
 /* Times the FP64 SpMV of the resident K with 4, 8, 16, 32 and 64 lanes per
  * node (times[5] in ms, nullable) and makes the fastest the lane-group size
  * of every later solve on this context. */

@@ -99,6 +99,7 @@ SIGNATURES = {
     "ofea_coarse_aggregates": (_i, [_vp, _vp]),
     "ofea_assemble_batch": (_i, [_vp, _i, _vp, _d, _vp]),
     "ofea_solve_batch": (_i, [_vp, _i, _i, _vp, _vp, _d, _i, _vp, _vp]),
+    "ofea_solve_batch_pc": (_i, [_vp, _i, _i, _vp, _vp, _d, _i, _i, _vp, _vp]),
     "ofea_autotune": (_i, [_vp, _pi, _vp]),
     "ofea_batch_tiles": (_i, [_vp, _i, _pi, _pf]),
     "ofea_timer_start": (_i, [_vp]),

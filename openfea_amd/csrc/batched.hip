@@ -252,9 +252,12 @@ struct BatchWS {
     void *ms = nullptr;        // MScalars (device)
     void *ms_host = nullptr;   // pinned
     const double **valp = nullptr;  // device slot with the current variant's values
+    // two-level: per-range partials of Z^T r, Z^T r per aggregate and y = A_c^-1 Z^T r (all systems)
+    double *part_co = nullptr, *gco = nullptr, *yco = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int graph_nrhs = 0, graph_group = 0;
+    int graph_nrhs = 0, graph_group = 0, graph_co = -1;
+    const void *graph_key = nullptr;  // pattern the captured graph reads
     int mb_group[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // tuned lane group per nrhs
     float mb_time[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 };
@@ -269,7 +272,7 @@ void batch_free(ofea_ctx *c)
     if (w->graph) (void)hipGraphDestroy(w->graph);
     for (void *p : {(void *)w->valb, (void *)w->Evar[0], (void *)w->Evar[1], (void *)w->Eraw, (void *)w->B, (void *)w->X, (void *)w->R,
                     (void *)w->Z, (void *)w->PA, (void *)w->PB, (void *)w->Q, (void *)w->Y, (void *)w->minv,
-                    (void *)w->part, w->ms, (void *)w->valp})
+                    (void *)w->part, w->ms, (void *)w->valp, (void *)w->part_co, (void *)w->gco, (void *)w->yco})
         if (p) (void)hipFree(p);
     if (w->ms_host) (void)hipHostFree(w->ms_host);
     delete w;
@@ -395,12 +398,47 @@ __device__ inline void mb_wg_sum(double v, double (*s_tmp)[MB_NRMAX], double *ds
     __syncthreads();
 }
 
-__device__ inline void mb_range(int n, int &i0, int &i1)
+__device__ inline void mb_range(int n, int &i0, int &i1, int *range = nullptr)
 {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
     i0 = min(b * per, n);
     i1 = min(i0 + per, n);
+    if (range) *range = b;
+}
+
+// per-system workgroup sums of NV values per thread (thread's system =
+// threadIdx.x % NR) into dst[q * NR + k]; one fixed order
+template <int NR, int NV>
+__device__ inline void mb_wg_sum_n(double (&v)[NV], double (*s_tmp)[NV][MB_NRMAX], double *dst)
+{
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+        for (int off = 32; off >= NR; off >>= 1) v[q] += __shfl_xor(v[q], off);
+    if ((threadIdx.x & 63) < NR)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) s_tmp[threadIdx.x >> 6][q][threadIdx.x & 63] = v[q];
+    __syncthreads();
+    if ((int)threadIdx.x < NV * NR) {
+        const int q = threadIdx.x / NR, k = threadIdx.x % NR;
+        double t = 0.0;
+        for (int w = 0; w < MB_NW; ++w) t += s_tmp[w][q][k];
+        dst[threadIdx.x] = t;
+    }
+    __syncthreads();
+}
+
+// Z^T r contribution of one node (3 translations, 3 rotations about the
+// aggregate centroid; d = node position minus centroid)
+__device__ inline void mb_ztr(const float *__restrict__ d, int i, const double r[3], double t[6])
+{
+    const double dx = d[3 * i], dy = d[3 * i + 1], dz = d[3 * i + 2];
+    t[0] = r[0];
+    t[1] = r[1];
+    t[2] = r[2];
+    t[3] = dy * r[2] - dz * r[1];
+    t[4] = dz * r[0] - dx * r[2];
+    t[5] = dx * r[1] - dy * r[0];
 }
 
 // Iteration start: per-system status from the previous update, then
@@ -494,19 +532,24 @@ __global__ __launch_bounds__(MB_WG) void k_mb_spmv(int bi, int nfree, const int 
     if (threadIdx.x < NR) mb_pq(part, threadIdx.x)[blockIdx.x] = s_dot[threadIdx.x];
 }
 
-// alpha_k = <r,z>/<p,q>; x += alpha p; r -= alpha q; z = M r (FP32 blocks);
-// partials of <r,z> and <r,r>.  Thread (node slot, system k = tid % NR).
-template <int NR>
+// alpha_k = <r,z>/<p,q>; x += alpha p; r -= alpha q; partials of <r,r>.
+// Block Jacobi (CO false): z = M r (FP32 blocks) and partials of <r,z>.
+// Two-level (CO true): per-range partials of Z^T r instead; k_mb_zapply
+// forms z after the coarse solve.  A finished system keeps x and r and gets
+// z = 0, so the next p is 0.  Thread (node slot, system k = tid % NR).
+template <int NR, bool CO>
 __global__ __launch_bounds__(MB_WG) void k_mb_update(int bi, int nfree, double *__restrict__ X, double *__restrict__ R,
                                                      double *__restrict__ Z, const double *__restrict__ PA,
                                                      const double *__restrict__ PB, const double *__restrict__ Q,
-                                                     const float *__restrict__ minv, double *part, MScalars *ms)
+                                                     const float *__restrict__ minv, const float *__restrict__ cd,
+                                                     double *__restrict__ part_co, double *part, MScalars *ms)
 {
     __shared__ double s_sc[2 * NR];
     __shared__ double s_alpha[NR];
+    __shared__ int s_act[NR];
     __shared__ int s_stop;
-    __shared__ double s_tmp[MB_NW][MB_NRMAX];
-    __shared__ double s_out[2 * NR];
+    __shared__ double s_tmp[MB_NW][6][MB_NRMAX];
+    __shared__ double s_out[6 * NR];
     if (threadIdx.x == 0) s_stop = ms->all_done;
     __syncthreads();
     if (s_stop) return;
@@ -515,48 +558,173 @@ __global__ __launch_bounds__(MB_WG) void k_mb_update(int bi, int nfree, double *
                     s_sc);
     if (threadIdx.x < NR) {
         const int k = threadIdx.x;
-        const bool act = !ms->done[k];
+        bool act = !ms->done[k];
         const double pq = s_sc[k];
         double alpha = 0.0;
         if (act) {
             if (!(pq > 0.0) || !isfinite(pq)) {
                 if (blockIdx.x == 0) ms->done[k] = 3;
+                act = false;
             } else {
                 alpha = s_sc[NR + k] / pq;
             }
         }
         s_alpha[k] = alpha;
+        s_act[k] = act;
     }
     __syncthreads();
     const int k = threadIdx.x % NR;
     const double alpha = s_alpha[k];
+    const bool act = s_act[k];
     const double *P = par ? PA : PB;
-    int i0, i1;
-    mb_range(nfree, i0, i1);
+    int i0, i1, rg;
+    mb_range(nfree, i0, i1, &rg);
     double rz = 0.0, rr = 0.0;
+    double g[6] = {0, 0, 0, 0, 0, 0};
     for (int i = i0 + (int)threadIdx.x / NR; i < i1; i += MB_WG / NR) {
-        const float *m = minv + 9 * i;
         double rv[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const long long o = (long long)(3 * i + c) * NR + k;
-            X[o] += alpha * P[o];
-            rv[c] = R[o] - alpha * Q[o];
-            R[o] = rv[c];
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const double zv = (double)m[3 * c] * rv[0] + (double)m[3 * c + 1] * rv[1] + (double)m[3 * c + 2] * rv[2];
-            Z[(long long)(3 * i + c) * NR + k] = zv;
-            rz += rv[c] * zv;
+            rv[c] = R[o];
+            if (act) {
+                X[o] += alpha * P[o];
+                rv[c] -= alpha * Q[o];
+                R[o] = rv[c];
+            }
             rr += rv[c] * rv[c];
         }
+        if constexpr (CO) {
+            double t[6];
+            mb_ztr(cd, i, rv, t);
+#pragma unroll
+            for (int m = 0; m < 6; ++m) g[m] += t[m];
+        } else {
+            const float *m = minv + 9 * i;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double zv = (double)m[3 * c] * rv[0] + (double)m[3 * c + 1] * rv[1] + (double)m[3 * c + 2] * rv[2];
+                zv = act ? zv : 0.0;
+                Z[(long long)(3 * i + c) * NR + k] = zv;
+                rz += rv[c] * zv;
+            }
+        }
     }
-    mb_wg_sum<NR>(rz, s_tmp, s_out);
-    mb_wg_sum<NR>(rr, s_tmp, s_out + NR);
+    if constexpr (CO) {
+        double v[6] = {g[0], g[1], g[2], g[3], g[4], g[5]};
+        mb_wg_sum_n<NR, 6>(v, s_tmp, s_out);
+        if ((int)threadIdx.x < 6 * NR) part_co[(size_t)rg * 6 * MB_NRMAX + threadIdx.x] = s_out[threadIdx.x];
+        double v1[1] = {rr};
+        mb_wg_sum_n<NR, 1>(v1, reinterpret_cast<double (*)[1][MB_NRMAX]>(&s_tmp[0][0][0]), s_out);
+        if (threadIdx.x < NR) mb_rr(part, threadIdx.x)[blockIdx.x] = s_out[threadIdx.x];
+    } else {
+        double v[2] = {rz, rr};
+        mb_wg_sum_n<NR, 2>(v, reinterpret_cast<double (*)[2][MB_NRMAX]>(&s_tmp[0][0][0]), s_out);
+        if (threadIdx.x < NR) {
+            mb_rz(part, par ^ 1, threadIdx.x)[blockIdx.x] = s_out[threadIdx.x];
+            mb_rr(part, threadIdx.x)[blockIdx.x] = s_out[NR + threadIdx.x];
+        }
+    }
+}
+
+// Two-level: Z^T r per aggregate from the per-range partials, in range order
+// (thread per (aggregate, mode, system)): g[a][m][k]
+__global__ void k_mb_cred(int nagg, int ra, int nranges, int nr, const double *__restrict__ part_co,
+                          double *__restrict__ gco, const MScalars *ms)
+{
+    if (ms->all_done) return;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nagg * 6 * nr) return;
+    const int a = t / (6 * nr), mk = t - a * 6 * nr, m = mk / nr, k = mk - m * nr;
+    double s = 0.0;
+    for (int b = a * ra; b < min((a + 1) * ra, nranges); ++b) s += part_co[(size_t)b * 6 * MB_NRMAX + m * nr + k];
+    gco[((size_t)a * 6 + m) * MB_NRMAX + k] = s;
+}
+
+// y = A_c^-1 g for every system: workgroup per CY_ROWS rows of A_c^-1 (FP32),
+// thread (system k, column slice); column sums in a fixed order
+constexpr int CY_ROWS = 12;
+template <int NR>
+__global__ __launch_bounds__(256) void k_mb_cy(int nc, const float *__restrict__ ainv, const double *__restrict__ gco,
+                                               double *__restrict__ yco, const MScalars *ms)
+{
+    constexpr int NS = 256 / NR;  // column slices
+    __shared__ double s_acc[NS][CY_ROWS][NR];
+    if (ms->all_done) return;
+    const int row0 = blockIdx.x * CY_ROWS;
+    const int k = threadIdx.x % NR, sl = threadIdx.x / NR;
+    double acc[CY_ROWS];
+#pragma unroll
+    for (int r = 0; r < CY_ROWS; ++r) acc[r] = 0.0;
+    for (int col = sl; col < nc; col += NS) {
+        const double gv = gco[(size_t)col * MB_NRMAX + k];
+#pragma unroll
+        for (int r = 0; r < CY_ROWS; ++r) {
+            const int row = min(row0 + r, nc - 1);
+            acc[r] += (double)ainv[(size_t)row * nc + col] * gv;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < CY_ROWS; ++r) s_acc[sl][r][k] = acc[r];
+    __syncthreads();
+    if ((int)threadIdx.x < CY_ROWS * NR) {
+        const int r = threadIdx.x / NR, kk = threadIdx.x % NR;
+        double t = 0.0;
+        for (int j = 0; j < NS; ++j) t += s_acc[j][r][kk];
+        if (row0 + r < nc) yco[(size_t)(row0 + r) * MB_NRMAX + kk] = t;
+    }
+}
+
+// Two-level: z = M r + Z y_a for the range's aggregate a; partials of <r,z>
+// into parity slot (par0 ^ (bi & 1)) ^ 1, or at the start (bi < 0) into
+// slot par0 with slot par0 ^ 1 summing to +inf (the first beta is 0).
+// Finished systems get z = 0.
+template <int NR>
+__global__ __launch_bounds__(MB_WG) void k_mb_zapply(int bi, int nfree, int ra, const double *__restrict__ R,
+                                                     double *__restrict__ Z, const float *__restrict__ minv,
+                                                     const float *__restrict__ cd, const double *__restrict__ yco,
+                                                     double *part, const MScalars *ms)
+{
+    __shared__ double s_y[6][NR];
+    __shared__ int s_act[NR];
+    __shared__ double s_tmp[MB_NW][1][MB_NRMAX];
+    __shared__ double s_out[NR];
+    if (ms->all_done) return;
+    int i0, i1, rg;
+    mb_range(nfree, i0, i1, &rg);
+    const int a = rg / ra;
+    if ((int)threadIdx.x < 6 * NR) s_y[threadIdx.x / NR][threadIdx.x % NR] = yco[((size_t)a * 6 + threadIdx.x / NR) * MB_NRMAX + threadIdx.x % NR];
+    if (threadIdx.x < NR) s_act[threadIdx.x] = !ms->done[threadIdx.x];
+    __syncthreads();
+    const int k = threadIdx.x % NR;
+    const bool act = s_act[k];
+    const double y0 = s_y[0][k], y1 = s_y[1][k], y2 = s_y[2][k], y3 = s_y[3][k], y4 = s_y[4][k], y5 = s_y[5][k];
+    double rz = 0.0;
+    for (int i = i0 + (int)threadIdx.x / NR; i < i1; i += MB_WG / NR) {
+        double rv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rv[c] = R[(long long)(3 * i + c) * NR + k];
+        const float *m = minv + 9 * i;
+        const double dx = cd[3 * i], dy = cd[3 * i + 1], dz = cd[3 * i + 2];
+        const double zc[3] = {y0 + y4 * dz - y5 * dy, y1 + y5 * dx - y3 * dz, y2 + y3 * dy - y4 * dx};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double zv = (double)m[3 * c] * rv[0] + (double)m[3 * c + 1] * rv[1] + (double)m[3 * c + 2] * rv[2] + zc[c];
+            zv = act ? zv : 0.0;
+            Z[(long long)(3 * i + c) * NR + k] = zv;
+            rz += rv[c] * zv;
+        }
+    }
+    double v[1] = {rz};
+    mb_wg_sum_n<NR, 1>(v, s_tmp, s_out);
     if (threadIdx.x < NR) {
-        mb_rz(part, par ^ 1, threadIdx.x)[blockIdx.x] = s_out[threadIdx.x];
-        mb_rr(part, threadIdx.x)[blockIdx.x] = s_out[NR + threadIdx.x];
+        const int p0 = ms->par0;
+        if (bi < 0) {
+            mb_rz(part, p0, threadIdx.x)[blockIdx.x] = s_out[threadIdx.x];
+            mb_rz(part, p0 ^ 1, threadIdx.x)[blockIdx.x] = blockIdx.x == 0 ? INFINITY : 0.0;
+        } else {
+            mb_rz(part, (p0 ^ (bi & 1)) ^ 1, threadIdx.x)[blockIdx.x] = s_out[threadIdx.x];
+        }
     }
 }
 
@@ -565,23 +733,25 @@ __global__ void k_mb_batch_end(MScalars *ms, int nbatch)
     if (threadIdx.x == 0 && blockIdx.x == 0 && !ms->all_done) ms->it0 += nbatch;
 }
 
-// start: x = 0, p = 0, r = b, z = M r, partials <r,z> (slot par0) and <r,r>;
-// slot par0^1 sums to +inf so the first beta is 0.
-template <int NR>
+// start: x = 0, p = 0, r = b, partials of <r,r>; block Jacobi: z = M r and
+// partials <r,z> (slot par0; slot par0^1 sums to +inf so the first beta is 0);
+// two-level: per-range partials of Z^T r (k_mb_zapply forms z).
+template <int NR, bool CO>
 __global__ __launch_bounds__(MB_WG) void k_mb_start(int nfree, const double *__restrict__ B, double *__restrict__ X,
                                                     double *__restrict__ R, double *__restrict__ Z,
                                                     double *__restrict__ PA, double *__restrict__ PB,
-                                                    const float *__restrict__ minv, double *part, const MScalars *ms)
+                                                    const float *__restrict__ minv, const float *__restrict__ cd,
+                                                    double *__restrict__ part_co, double *part, const MScalars *ms)
 {
-    __shared__ double s_tmp[MB_NW][MB_NRMAX];
-    __shared__ double s_out[2 * NR];
+    __shared__ double s_tmp[MB_NW][6][MB_NRMAX];
+    __shared__ double s_out[6 * NR];
     const int par = ms->par0;
     const int k = threadIdx.x % NR;
-    int i0, i1;
-    mb_range(nfree, i0, i1);
+    int i0, i1, rg;
+    mb_range(nfree, i0, i1, &rg);
     double rz = 0.0, rr = 0.0;
+    double g[6] = {0, 0, 0, 0, 0, 0};
     for (int i = i0 + (int)threadIdx.x / NR; i < i1; i += MB_WG / NR) {
-        const float *m = minv + 9 * i;
         double rv[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -591,21 +761,38 @@ __global__ __launch_bounds__(MB_WG) void k_mb_start(int nfree, const double *__r
             X[o] = 0.0;
             PA[o] = 0.0;
             PB[o] = 0.0;
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const double zv = (double)m[3 * c] * rv[0] + (double)m[3 * c + 1] * rv[1] + (double)m[3 * c + 2] * rv[2];
-            Z[(long long)(3 * i + c) * NR + k] = zv;
-            rz += rv[c] * zv;
             rr += rv[c] * rv[c];
         }
+        if constexpr (CO) {
+            double t[6];
+            mb_ztr(cd, i, rv, t);
+#pragma unroll
+            for (int m = 0; m < 6; ++m) g[m] += t[m];
+        } else {
+            const float *m = minv + 9 * i;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double zv = (double)m[3 * c] * rv[0] + (double)m[3 * c + 1] * rv[1] + (double)m[3 * c + 2] * rv[2];
+                Z[(long long)(3 * i + c) * NR + k] = zv;
+                rz += rv[c] * zv;
+            }
+        }
     }
-    mb_wg_sum<NR>(rz, s_tmp, s_out);
-    mb_wg_sum<NR>(rr, s_tmp, s_out + NR);
-    if (threadIdx.x < NR) {
-        mb_rz(part, par, threadIdx.x)[blockIdx.x] = s_out[threadIdx.x];
-        mb_rr(part, threadIdx.x)[blockIdx.x] = s_out[NR + threadIdx.x];
-        mb_rz(part, par ^ 1, threadIdx.x)[blockIdx.x] = blockIdx.x == 0 ? INFINITY : 0.0;
+    if constexpr (CO) {
+        double v[6] = {g[0], g[1], g[2], g[3], g[4], g[5]};
+        mb_wg_sum_n<NR, 6>(v, s_tmp, s_out);
+        if ((int)threadIdx.x < 6 * NR) part_co[(size_t)rg * 6 * MB_NRMAX + threadIdx.x] = s_out[threadIdx.x];
+        double v1[1] = {rr};
+        mb_wg_sum_n<NR, 1>(v1, reinterpret_cast<double (*)[1][MB_NRMAX]>(&s_tmp[0][0][0]), s_out);
+        if (threadIdx.x < NR) mb_rr(part, threadIdx.x)[blockIdx.x] = s_out[threadIdx.x];
+    } else {
+        double v[2] = {rz, rr};
+        mb_wg_sum_n<NR, 2>(v, reinterpret_cast<double (*)[2][MB_NRMAX]>(&s_tmp[0][0][0]), s_out);
+        if (threadIdx.x < NR) {
+            mb_rz(part, par, threadIdx.x)[blockIdx.x] = s_out[threadIdx.x];
+            mb_rr(part, threadIdx.x)[blockIdx.x] = s_out[NR + threadIdx.x];
+            mb_rz(part, par ^ 1, threadIdx.x)[blockIdx.x] = blockIdx.x == 0 ? INFINITY : 0.0;
+        }
     }
 }
 
@@ -665,34 +852,64 @@ __global__ void k_mb_precond(int nfree, const int *__restrict__ node_blk, const 
 
 constexpr int MB_GROUPS[3] = {16, 32, 64};
 
+// The system the multi-RHS solver works on: K's pattern in the caller's node
+// order (block Jacobi), or the two-level order with the variant's coarse
+// space (the values then come from coarse_for_values).
+struct MbSys {
+    const int *node_blk = nullptr, *bcol = nullptr;
+    const CoarseVariant *co = nullptr;
+};
+
 template <int NR>
-static void mb_launch_apply(ofea_ctx *c, BatchWS *w, int group, const double *x, double *y)
+static void mb_launch_apply(ofea_ctx *c, BatchWS *w, const MbSys &S, int group, const double *x, double *y)
 {
     const int nf = c->P.nfree;
     switch (group) {
     case 32:
-        hipLaunchKernelGGL((k_mb_apply<NR, 32>), dim3(MB_NB), dim3(MB_WG), 0, c->stream, nf, c->d_node_blk, c->d_bcol,
+        hipLaunchKernelGGL((k_mb_apply<NR, 32>), dim3(MB_NB), dim3(MB_WG), 0, c->stream, nf, S.node_blk, S.bcol,
                            w->valp, x, y);
         break;
     case 64:
-        hipLaunchKernelGGL((k_mb_apply<NR, 64>), dim3(MB_NB), dim3(MB_WG), 0, c->stream, nf, c->d_node_blk, c->d_bcol,
+        hipLaunchKernelGGL((k_mb_apply<NR, 64>), dim3(MB_NB), dim3(MB_WG), 0, c->stream, nf, S.node_blk, S.bcol,
                            w->valp, x, y);
         break;
     default:
-        hipLaunchKernelGGL((k_mb_apply<NR, 16>), dim3(MB_NB), dim3(MB_WG), 0, c->stream, nf, c->d_node_blk, c->d_bcol,
+        hipLaunchKernelGGL((k_mb_apply<NR, 16>), dim3(MB_NB), dim3(MB_WG), 0, c->stream, nf, S.node_blk, S.bcol,
                            w->valp, x, y);
     }
 }
 
+// two-level: Z^T r per aggregate, y = A_c^-1 Z^T r, z = M r + Z y (bi < 0: the start)
+template <int NR>
+static void mb_launch_coarse(ofea_ctx *c, BatchWS *w, const MbSys &S, int bi)
+{
+    const CoarseVariant &co = *S.co;
+    const int nf = c->P.nfree, nc = 6 * co.nagg;
+    const int nthr = co.nagg * 6 * NR;
+    hipLaunchKernelGGL(k_mb_cred, dim3((nthr + 255) / 256), dim3(256), 0, c->stream, co.nagg, co.ra, MB_NB, NR,
+                       (const double *)w->part_co, w->gco, (const MScalars *)w->ms);
+    hipLaunchKernelGGL((k_mb_cy<NR>), dim3((nc + CY_ROWS - 1) / CY_ROWS), dim3(256), 0, c->stream, nc, co.ainv,
+                       (const double *)w->gco, w->yco, (const MScalars *)w->ms);
+    hipLaunchKernelGGL((k_mb_zapply<NR>), dim3(MB_NB), dim3(MB_WG), 0, c->stream, bi, nf, co.ra, (const double *)w->R,
+                       w->Z, (const float *)w->minv, co.d, (const double *)w->yco, w->part, (const MScalars *)w->ms);
+}
+
 template <int NR, int G>
-static void mb_capture_batch(ofea_ctx *c, BatchWS *w)
+static void mb_capture_batch(ofea_ctx *c, BatchWS *w, const MbSys &S)
 {
     const int nf = c->P.nfree;
+    const float *cd = S.co ? S.co->d : nullptr;
     for (int bi = 0; bi < MB_BATCH; ++bi) {
-        hipLaunchKernelGGL((k_mb_spmv<NR, G>), dim3(MB_NB), dim3(MB_WG), 0, c->stream, bi, nf, c->d_node_blk, c->d_bcol,
+        hipLaunchKernelGGL((k_mb_spmv<NR, G>), dim3(MB_NB), dim3(MB_WG), 0, c->stream, bi, nf, S.node_blk, S.bcol,
                            w->valp, w->Z, w->PA, w->PB, w->Q, w->part, (MScalars *)w->ms);
-        hipLaunchKernelGGL((k_mb_update<NR>), dim3(MB_NB), dim3(MB_WG), 0, c->stream, bi, nf, w->X, w->R, w->Z, w->PA,
-                           w->PB, w->Q, w->minv, w->part, (MScalars *)w->ms);
+        if (S.co) {
+            hipLaunchKernelGGL((k_mb_update<NR, true>), dim3(MB_NB), dim3(MB_WG), 0, c->stream, bi, nf, w->X, w->R,
+                               w->Z, w->PA, w->PB, w->Q, w->minv, cd, w->part_co, w->part, (MScalars *)w->ms);
+            mb_launch_coarse<NR>(c, w, S, bi);
+        } else {
+            hipLaunchKernelGGL((k_mb_update<NR, false>), dim3(MB_NB), dim3(MB_WG), 0, c->stream, bi, nf, w->X, w->R,
+                               w->Z, w->PA, w->PB, w->Q, w->minv, cd, w->part_co, w->part, (MScalars *)w->ms);
+        }
     }
     hipLaunchKernelGGL(k_mb_batch_end, dim3(1), dim3(64), 0, c->stream, (MScalars *)w->ms, MB_BATCH);
 }
@@ -701,15 +918,15 @@ static void mb_capture_batch(ofea_ctx *c, BatchWS *w)
 // timed on the resident variant (5 launches after a warm-up) and the fastest
 // is kept for this context and NR.
 template <int NR>
-static int mb_tune(ofea_ctx *c, BatchWS *w)
+static int mb_tune(ofea_ctx *c, BatchWS *w, const MbSys &S)
 {
     if (w->mb_group[NR]) return OFEA_OK;
     float best_t = 1e30f;
     int best = 16;
     for (int g : MB_GROUPS) {
-        mb_launch_apply<NR>(c, w, g, w->B, w->Y);
+        mb_launch_apply<NR>(c, w, S, g, w->B, w->Y);
         OFEA_HIP(hipEventRecord(c->ev0, c->stream));
-        for (int i = 0; i < 5; ++i) mb_launch_apply<NR>(c, w, g, w->B, w->Y);
+        for (int i = 0; i < 5; ++i) mb_launch_apply<NR>(c, w, S, g, w->B, w->Y);
         OFEA_HIP(hipEventRecord(c->ev1, c->stream));
         OFEA_HIP(hipEventSynchronize(c->ev1));
         OFEA_HIP(hipGetLastError());
@@ -726,33 +943,42 @@ static int mb_tune(ofea_ctx *c, BatchWS *w)
 }
 
 template <int NR>
-static int mb_solve_variant(ofea_ctx *c, BatchWS *w, double rtol, int maxit, const std::vector<double> &bb, int *iters,
-                            double *relres)
+static int mb_solve_variant(ofea_ctx *c, BatchWS *w, const MbSys &S, double rtol, int maxit,
+                            const std::vector<double> &bb, int *iters, double *relres)
 {
     const int nf = c->P.nfree;
-    int rc = mb_tune<NR>(c, w);
+    int rc = mb_tune<NR>(c, w, S);
     if (rc) return rc;
     const int group = w->mb_group[NR];
-    MScalars &S = *(MScalars *)w->ms_host;
-    std::memset(&S, 0, sizeof(S));
-    for (int k = 0; k < NR; ++k) S.tol2[k] = 0.25 * rtol * rtol * bb[k];  // recursive target 0.5 rtol
-    S.maxit = maxit;
-    OFEA_HIP(hipMemcpyAsync(w->ms, &S, sizeof(S), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_mb_precond, dim3((nf + 255) / 256), dim3(256), 0, c->stream, nf, c->d_node_blk, c->d_bcol,
-                       w->valp, w->minv);
-    hipLaunchKernelGGL((k_mb_start<NR>), dim3(MB_NB), dim3(MB_WG), 0, c->stream, nf, w->B, w->X, w->R, w->Z, w->PA,
-                       w->PB, w->minv, w->part, (const MScalars *)w->ms);
+    MScalars &Sc = *(MScalars *)w->ms_host;
+    std::memset(&Sc, 0, sizeof(Sc));
+    for (int k = 0; k < NR; ++k) Sc.tol2[k] = 0.25 * rtol * rtol * bb[k];  // recursive target 0.5 rtol
+    Sc.maxit = maxit;
+    OFEA_HIP(hipMemcpyAsync(w->ms, &Sc, sizeof(Sc), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_mb_precond, dim3((nf + 255) / 256), dim3(256), 0, c->stream, nf, S.node_blk, S.bcol, w->valp,
+                       w->minv);
+    const float *cd = S.co ? S.co->d : nullptr;
+    if (S.co) {
+        hipLaunchKernelGGL((k_mb_start<NR, true>), dim3(MB_NB), dim3(MB_WG), 0, c->stream, nf, w->B, w->X, w->R, w->Z,
+                           w->PA, w->PB, w->minv, cd, w->part_co, w->part, (const MScalars *)w->ms);
+        mb_launch_coarse<NR>(c, w, S, -1);
+    } else {
+        hipLaunchKernelGGL((k_mb_start<NR, false>), dim3(MB_NB), dim3(MB_WG), 0, c->stream, nf, w->B, w->X, w->R,
+                           w->Z, w->PA, w->PB, w->minv, cd, w->part_co, w->part, (const MScalars *)w->ms);
+    }
     OFEA_HIP(hipGetLastError());
     auto direct_batch = [&]() {
         switch (group) {
-        case 32: mb_capture_batch<NR, 32>(c, w); break;
-        case 64: mb_capture_batch<NR, 64>(c, w); break;
-        default: mb_capture_batch<NR, 16>(c, w); break;
+        case 32: mb_capture_batch<NR, 32>(c, w, S); break;
+        case 64: mb_capture_batch<NR, 64>(c, w, S); break;
+        default: mb_capture_batch<NR, 16>(c, w, S); break;
         }
     };
     // OFEA_NO_GRAPH=1 launches the batches directly (for tools that cannot follow graph launches)
     static const bool no_graph = std::getenv("OFEA_NO_GRAPH") && std::getenv("OFEA_NO_GRAPH")[0] == '1';
-    if (!no_graph && (!w->exec || w->graph_nrhs != NR || w->graph_group != group)) {
+    const int co_key = S.co ? 6 * S.co->nagg : 0;
+    if (!no_graph && (!w->exec || w->graph_nrhs != NR || w->graph_group != group || w->graph_co != co_key ||
+                      w->graph_key != (const void *)S.node_blk)) {
         if (w->exec) (void)hipGraphExecDestroy(w->exec);
         if (w->graph) (void)hipGraphDestroy(w->graph);
         w->exec = nullptr;
@@ -763,6 +989,8 @@ static int mb_solve_variant(ofea_ctx *c, BatchWS *w, double rtol, int maxit, con
         OFEA_HIP(hipGraphInstantiate(&w->exec, w->graph, nullptr, nullptr, 0));
         w->graph_nrhs = NR;
         w->graph_group = group;
+        w->graph_co = co_key;
+        w->graph_key = S.node_blk;
     }
     for (;;) {
         if (no_graph) {
@@ -771,12 +999,12 @@ static int mb_solve_variant(ofea_ctx *c, BatchWS *w, double rtol, int maxit, con
         } else {
             OFEA_HIP(hipGraphLaunch(w->exec, c->stream));
         }
-        OFEA_HIP(hipMemcpyAsync(&S, w->ms, sizeof(S), hipMemcpyDeviceToHost, c->stream));
+        OFEA_HIP(hipMemcpyAsync(&Sc, w->ms, sizeof(Sc), hipMemcpyDeviceToHost, c->stream));
         OFEA_HIP(hipStreamSynchronize(c->stream));
-        if (S.all_done) break;
+        if (Sc.all_done) break;
     }
     // true FP64 residuals b - K x of every system
-    mb_launch_apply<NR>(c, w, group, w->X, w->Y);
+    mb_launch_apply<NR>(c, w, S, group, w->X, w->Y);
     OFEA_HIP(hipGetLastError());
     const size_t nn = (size_t)3 * nf * NR;
     std::vector<double> y(nn), b(nn);
@@ -792,19 +1020,22 @@ static int mb_solve_variant(ofea_ctx *c, BatchWS *w, double rtol, int maxit, con
         }
         const double rel = bb[k] > 0.0 ? std::sqrt(rr / bb[k]) : 0.0;
         if (relres) relres[k] = rel;
-        if (iters) iters[k] = S.done[k] ? S.it[k] : maxit;
-        if (S.done[k] == 3) return fail(OFEA_ERR_STATE, "PCG breakdown in a batched system");
+        if (iters) iters[k] = Sc.done[k] ? Sc.it[k] : maxit;
+        if (Sc.done[k] == 3) return fail(OFEA_ERR_STATE, "PCG breakdown in a batched system");
         all_ok = all_ok && rel <= rtol;
     }
     return all_ok ? OFEA_OK : OFEA_NOT_CONVERGED;
 }
 
-int solve_batch(ofea_ctx *c, int nvar, int nrhs, const double *B, double *X, double rtol, int maxit, int *iters,
-                double *relres)
+int solve_batch(ofea_ctx *c, int nvar, int nrhs, const double *B, double *X, double rtol, int maxit, int precond,
+                int *iters, double *relres)
 {
     BatchWS *w = bws(c);
     if (!w || !w->valb || w->nvar < nvar) return fail(OFEA_ERR_STATE, "ofea_assemble_batch first (with >= nvar variants)");
     if (nrhs != 1 && nrhs != 2 && nrhs != 4 && nrhs != 8) return fail(OFEA_ERR_ARG, "nrhs must be 1, 2, 4 or 8");
+    if (precond != OFEA_PCG_BLOCK_JACOBI && precond != OFEA_PCG_TWO_LEVEL)
+        return fail(OFEA_ERR_ARG, "batched preconditioner must be OFEA_PCG_BLOCK_JACOBI or OFEA_PCG_TWO_LEVEL");
+    const bool two = precond == OFEA_PCG_TWO_LEVEL;
     const int nf = c->P.nfree, n = c->P.n;
     const size_t nn = (size_t)n * nrhs;
     if (w->nrhs != nrhs) {
@@ -826,18 +1057,42 @@ int solve_batch(ofea_ctx *c, int nvar, int nrhs, const double *B, double *X, dou
     if (!w->ms) OFEA_HIP(hipMalloc(&w->ms, sizeof(MScalars)));
     if (!w->ms_host) OFEA_HIP(hipHostMalloc(&w->ms_host, sizeof(MScalars), hipHostMallocDefault));
     if (!w->valp) OFEA_HIP(hipMalloc((void **)&w->valp, sizeof(double *)));
+    if (two && !w->part_co) {
+        OFEA_HIP(hipMalloc((void **)&w->part_co, sizeof(double) * 6 * MB_NRMAX * MB_NB));
+        OFEA_HIP(hipMalloc((void **)&w->gco, sizeof(double) * CG_NC_MAX * MB_NRMAX));
+        OFEA_HIP(hipMalloc((void **)&w->yco, sizeof(double) * CG_NC_MAX * MB_NRMAX));
+    }
     std::vector<double> bi(nn), xo(nn), bb(nrhs);
     int status = OFEA_OK;
+    CoarseVariant co;
+    MbSys S;
+    S.node_blk = c->d_node_blk;
+    S.bcol = c->d_bcol;
     for (int v = 0; v < nvar; ++v) {
         const double *pv = w->valb + (size_t)v * w->nnz;
+        if (two) {
+            // the variant's coarse space; the solve runs on its reordered copy
+            int rc = coarse_for_values(c, pv, co);
+            if (rc) return rc;
+            if (6 * co.nagg > CG_NC_MAX) return fail(OFEA_ERR_LIMIT, "coarse space larger than the batched buffers");
+            S.node_blk = co.nb_p;
+            S.bcol = co.bcol_p;
+            S.co = &co;
+            pv = co.val_p;
+        }
         OFEA_HIP(hipMemcpyAsync(w->valp, &pv, sizeof(pv), hipMemcpyHostToDevice, c->stream));
-        // interleave the right-hand sides of variant v: [dof][system]
+        // interleave the right-hand sides of variant v: [dof][system], nodes in
+        // the order the solver works in
         for (int k = 0; k < nrhs; ++k) {
             const double *src = B + ((size_t)v * nrhs + k) * n;
             double s = 0.0;
-            for (int i = 0; i < n; ++i) {
-                bi[(size_t)i * nrhs + k] = src[i];
-                s += src[i] * src[i];
+            for (int i = 0; i < nf; ++i) {
+                const int j = two ? co.iperm[i] : i;
+                for (int d = 0; d < 3; ++d) {
+                    const double x = src[3 * i + d];
+                    bi[(size_t)(3 * j + d) * nrhs + k] = x;
+                    s += x * x;
+                }
             }
             bb[k] = s;
         }
@@ -847,10 +1102,10 @@ int solve_batch(ofea_ctx *c, int nvar, int nrhs, const double *B, double *X, dou
         int *itv = iters ? iters + (size_t)v * nrhs : nullptr;
         double *rv = relres ? relres + (size_t)v * nrhs : nullptr;
         switch (nrhs) {
-        case 1: rc = mb_solve_variant<1>(c, w, rtol, maxit, bb, itv, rv); break;
-        case 2: rc = mb_solve_variant<2>(c, w, rtol, maxit, bb, itv, rv); break;
-        case 4: rc = mb_solve_variant<4>(c, w, rtol, maxit, bb, itv, rv); break;
-        default: rc = mb_solve_variant<8>(c, w, rtol, maxit, bb, itv, rv); break;
+        case 1: rc = mb_solve_variant<1>(c, w, S, rtol, maxit, bb, itv, rv); break;
+        case 2: rc = mb_solve_variant<2>(c, w, S, rtol, maxit, bb, itv, rv); break;
+        case 4: rc = mb_solve_variant<4>(c, w, S, rtol, maxit, bb, itv, rv); break;
+        default: rc = mb_solve_variant<8>(c, w, S, rtol, maxit, bb, itv, rv); break;
         }
         if (rc < 0) return rc;
         if (rc != OFEA_OK) status = rc;
@@ -859,7 +1114,10 @@ int solve_batch(ofea_ctx *c, int nvar, int nrhs, const double *B, double *X, dou
             OFEA_HIP(hipStreamSynchronize(c->stream));
             for (int k = 0; k < nrhs; ++k) {
                 double *dst = X + ((size_t)v * nrhs + k) * n;
-                for (int i = 0; i < n; ++i) dst[i] = xo[(size_t)i * nrhs + k];
+                for (int i = 0; i < nf; ++i) {
+                    const int j = two ? co.iperm[i] : i;
+                    for (int d = 0; d < 3; ++d) dst[3 * i + d] = xo[(size_t)(3 * j + d) * nrhs + k];
+                }
             }
         }
     }

@@ -561,7 +561,15 @@ int ofea_solve_batch(ofea_ctx *c, int nvar, int nrhs, const double *B, double *X
 {
     CHECK_CTX(c);
     if (!B || nvar < 1 || maxit < 1 || !(rtol > 0.0)) return fail(OFEA_ERR_ARG, "bad B, nvar, maxit or rtol");
-    return solve_batch(c, nvar, nrhs, B, X, rtol, maxit, iters, relres);
+    return solve_batch(c, nvar, nrhs, B, X, rtol, maxit, OFEA_PCG_BLOCK_JACOBI, iters, relres);
+}
+
+int ofea_solve_batch_pc(ofea_ctx *c, int nvar, int nrhs, const double *B, double *X, double rtol, int maxit,
+                        int precond, int *iters, double *relres)
+{
+    CHECK_CTX(c);
+    if (!B || nvar < 1 || maxit < 1 || !(rtol > 0.0)) return fail(OFEA_ERR_ARG, "bad B, nvar, maxit or rtol");
+    return solve_batch(c, nvar, nrhs, B, X, rtol, maxit, precond, iters, relres);
 }
 
 int ofea_autotune(ofea_ctx *c, int *best_group, float *times)

@@ -279,8 +279,8 @@ int nl_solve_ref(ofea_ctx *c, double E, double nu, const double *F, double res_l
                  double inner_rtol, double *u_out, int *loops, int *linear_total, double *res_norm_out);
 // batched variants (batched.hip)
 int assemble_batch(ofea_ctx *c, int nvar, const double *E_var, double nu, double *out);
-int solve_batch(ofea_ctx *c, int nvar, int nrhs, const double *B, double *X, double rtol, int maxit, int *iters,
-                double *relres);
+int solve_batch(ofea_ctx *c, int nvar, int nrhs, const double *B, double *X, double rtol, int maxit, int precond,
+                int *iters, double *relres);
 int autotune(ofea_ctx *c, int *best_group, float *times);
 int batch_tiles(ofea_ctx *c, int nrhs, int *group, float *ms);
 void batch_free(ofea_ctx *c);
@@ -303,6 +303,20 @@ int solver_spmv_group_time(ofea_ctx *ctx, int group, int reps, float *ms);
 // two-level coarse correction Z A_c^-1 Z^T r on the host (solver.hip)
 int coarse_apply(ofea_ctx *ctx, const double *r, double *out);
 int coarse_aggregates(ofea_ctx *ctx, int *agg);
+// The two-level coarse space of a value array on K's pattern (batched
+// variants, batched.hip): views into the solver's reordered system.  Node i
+// of the caller's numbering is node iperm[i] of the reordered one; reordered
+// node j lies in CG range j / per and aggregate (j / per) / ra.
+struct CoarseVariant {
+    const int *nb_p = nullptr, *bcol_p = nullptr;  // reordered node_blk and bcol
+    const double *val_p = nullptr;                 // the values, node rows reordered
+    const double *minv_p = nullptr;                // their 3x3 block-Jacobi inverses, FP64
+    const float *d = nullptr;                      // 3 per node: position minus its aggregate centroid
+    const float *ainv = nullptr;                   // A_c^-1, (6 nagg)^2 row-major
+    int nagg = 0, per = 1, ra = 1;
+    const int *iperm = nullptr;                    // host: caller's node -> reordered node
+};
+int coarse_for_values(ofea_ctx *ctx, const double *vals, ofea::CoarseVariant &v);
 
 template <class T>
 int dev_upload(T *&d, const T *h, size_t count)

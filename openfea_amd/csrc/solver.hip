@@ -2622,7 +2622,10 @@ static int coarse_structure(ofea_ctx *ctx)
     return OFEA_OK;
 }
 
-static int coarse_values(ofea_ctx *ctx)
+// A_c^-1 (and the reordered values with their block Jacobi) of the values
+// vals: K (the two-level solves) or a batched variant; fp32 adds the FP32
+// copy of the reordered values the mixed two-level methods read.
+static int coarse_values(ofea_ctx *ctx, const double *vals, bool fp32)
 {
     ofea_solver_ws *w = ctx->ws;
     const int nf = w->nfree, nc = 6 * w->co_nagg;
@@ -2630,11 +2633,11 @@ static int coarse_values(ofea_ctx *ctx)
     hipStream_t st = ctx->stream;
     if (nf) {
         hipLaunchKernelGGL(k_perm_vals, dim3((nf + RED_WG / 64 - 1) / (RED_WG / 64)), dim3(RED_WG), 0, st, nf,
-                           w->perm, ctx->d_node_blk, w->nb_p, ctx->d_val, w->val_p);
+                           w->perm, ctx->d_node_blk, w->nb_p, vals, w->val_p);
         hipLaunchKernelGGL(k_precond_setup, dim3(w->nblk_node), dim3(RED_WG), 0, st, nf, w->nb_p, w->bcol_p, w->val_p,
                            w->dinv_p, w->minv_p, w->minv32_p, w->dblk_p);
     }
-    if (nnz)
+    if (nnz && fp32)
         hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((nnz + RED_WG - 1) / RED_WG)), dim3(RED_WG), 0,
                            st, nnz, w->val_p, w->val32_p);
     if (nc == 0) return OFEA_OK;
@@ -2686,9 +2689,35 @@ static int coarse_prepare(ofea_ctx *ctx)
     const auto t0 = std::chrono::steady_clock::now();
     int rc;
     if (!w->co_struct_valid && (rc = coarse_structure(ctx))) return rc;
-    if ((rc = coarse_values(ctx))) return rc;
+    if ((rc = coarse_values(ctx, ctx->d_val, true))) return rc;
     w->co_valid = true;
     w->co_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return OFEA_OK;
+}
+
+// The two-level coarse space of the values vals (a batched variant, same
+// pattern as K): the mesh's aggregate order and block lists (built once per
+// mesh), vals reordered into the solver's reordered value array with its
+// block Jacobi, and A_c^-1.  The single-system solver's own coarse space is
+// rebuilt from K on its next two-level solve.
+int coarse_for_values(ofea_ctx *ctx, const double *vals, CoarseVariant &v)
+{
+    int rc = solver_prepare(ctx);
+    if (rc) return rc;
+    ofea_solver_ws *w = ctx->ws;
+    if (!w->co_struct_valid && (rc = coarse_structure(ctx))) return rc;
+    w->co_valid = false;
+    if ((rc = coarse_values(ctx, vals, false))) return rc;
+    v.nb_p = w->nb_p;
+    v.bcol_p = w->bcol_p;
+    v.val_p = w->val_p;
+    v.minv_p = w->minv_p;
+    v.d = w->co.d;
+    v.ainv = w->co_ainv;
+    v.nagg = w->co_nagg;
+    v.per = w->co.per;
+    v.ra = w->co.ra;
+    v.iperm = w->iperm_host.data();
     return OFEA_OK;
 }
 

@@ -332,15 +332,20 @@ class Context:
         check(lib().ofea_assemble_batch(self._h, nv, ptr(E_var), nu, ptr(out)))
         return out
 
-    def solve_batch(self, B, rtol=1e-8, maxit=100000, copy=True):
-        """B: nvar x nrhs x n right-hand sides; returns X, iters, relres, converged."""
+    def solve_batch(self, B, rtol=1e-8, maxit=100000, copy=True, method=None):
+        """B: nvar x nrhs x n right-hand sides; returns X, iters, relres, converged.
+        method: None (ofea_solve_batch, block Jacobi) or A.PCG_BLOCK_JACOBI / A.PCG_TWO_LEVEL
+        (ofea_solve_batch_pc)."""
         B = f64(B)
         nvar, nrhs, n = B.shape
         X = np.zeros_like(B) if copy else None
         it = np.zeros((nvar, nrhs), dtype=np.int32)
         rr = np.zeros((nvar, nrhs))
-        rc = check(lib().ofea_solve_batch(self._h, nvar, nrhs, ptr(B), ptr(X), rtol, maxit, ptr(it), ptr(rr)),
-                   allow=(A.OFEA_OK, A.OFEA_NOT_CONVERGED))
+        if method is None:
+            rc = lib().ofea_solve_batch(self._h, nvar, nrhs, ptr(B), ptr(X), rtol, maxit, ptr(it), ptr(rr))
+        else:
+            rc = lib().ofea_solve_batch_pc(self._h, nvar, nrhs, ptr(B), ptr(X), rtol, maxit, method, ptr(it), ptr(rr))
+        rc = check(rc, allow=(A.OFEA_OK, A.OFEA_NOT_CONVERGED))
         return X, it, rr, rc == A.OFEA_OK
 
     def autotune(self):
