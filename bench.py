@@ -889,7 +889,8 @@ def compact_configs(args):
     c4 = bench_batched(sub(seq_methods="two_level_fp64"))
     bs = c4.get("batched_solve") or {}
     out["cfg4"] = dict(pick(c4, keys + ["sequential_assembly_ms_64_variants"]), workload=c4["config"]["workload"],
-                       batched_solve=pick(bs, ["systems", "s", "two_level_sequential_all_s", "iters_max", "converged"]))
+                       batched_solve=pick(bs, ["systems", "s", "two_level_sequential_all_s", "two_level_batched",
+                                               "iters_max", "converged"]))
     c5 = bench_nl(sub(newton_methods="fp64_block_jacobi,two_level_fp64,learned_fp64"))
     out["cfg5"] = dict(pick(c5, keys), workload=c5["config"]["workload"],
                        newton={k: pick(v, ["s", "newton_iters", "pcg_iters", "converged"]) for k, v in

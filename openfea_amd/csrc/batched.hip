@@ -352,6 +352,7 @@ constexpr int MB_WG = 1024;   // threads per workgroup
 constexpr int MB_NW = MB_WG / 64;
 constexpr int MB_NRMAX = 8;   // systems per variant
 constexpr int MB_BATCH = 16;  // iterations per captured graph
+constexpr int MB_UNR = 4;     // matrix entries per lane whose loads the SpMV issues together
 
 struct MScalars {
     double tol2[MB_NRMAX];
@@ -503,13 +504,34 @@ __global__ __launch_bounds__(MB_WG) void k_mb_spmv(int bi, int nfree, const int 
         const int nb0 = node_blk[r], len = 3 * (node_blk[r + 1] - nb0);
         const double *v0 = val + 9 * (long long)nb0;
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-        for (int j = jl; j < len; j += GJ) {
-            const int kk = j / 3, d = j - 3 * kk;
-            const long long xi = (long long)(3 * bcol[nb0 + kk] + d) * NR + k;
-            const double pv = act ? Z[xi] + beta * p_old[xi] : 0.0;
-            a0 += v0[j] * pv;
-            a1 += v0[len + j] * pv;
-            a2 += v0[2 * len + j] * pv;
+        // MB_UNR entries per pass with every load of the pass issued first
+        // (absent entries read entry jl and add nothing)
+        for (int jb = jl; jb < len; jb += MB_UNR * GJ) {
+            int col[MB_UNR];
+#pragma unroll
+            for (int u = 0; u < MB_UNR; ++u) {
+                const int j = jb + u * GJ < len ? jb + u * GJ : jl;
+                const int kk = j / 3;
+                col[u] = 3 * bcol[nb0 + kk] + (j - 3 * kk);
+            }
+            double zv[MB_UNR], pv[MB_UNR], m0[MB_UNR], m1[MB_UNR], m2[MB_UNR];
+#pragma unroll
+            for (int u = 0; u < MB_UNR; ++u) {
+                const int j = jb + u * GJ < len ? jb + u * GJ : jl;
+                const long long xi = (long long)col[u] * NR + k;
+                zv[u] = Z[xi];
+                pv[u] = p_old[xi];
+                m0[u] = v0[j];
+                m1[u] = v0[len + j];
+                m2[u] = v0[2 * len + j];
+            }
+#pragma unroll
+            for (int u = 0; u < MB_UNR; ++u) {
+                const double pe = (act && jb + u * GJ < len) ? zv[u] + beta * pv[u] : 0.0;
+                a0 += m0[u] * pe;
+                a1 += m1[u] * pe;
+                a2 += m2[u] * pe;
+            }
         }
         for (int off = G >> 1; off >= NR; off >>= 1) {
             a0 += __shfl_xor(a0, off);
@@ -643,7 +665,7 @@ __global__ void k_mb_cred(int nagg, int ra, int nranges, int nr, const double *_
 
 // y = A_c^-1 g for every system: workgroup per CY_ROWS rows of A_c^-1 (FP32),
 // thread (system k, column slice); column sums in a fixed order
-constexpr int CY_ROWS = 12;
+constexpr int CY_ROWS = 6;
 template <int NR>
 __global__ __launch_bounds__(256) void k_mb_cy(int nc, const float *__restrict__ ainv, const double *__restrict__ gco,
                                                double *__restrict__ yco, const MScalars *ms)
@@ -656,13 +678,21 @@ __global__ __launch_bounds__(256) void k_mb_cy(int nc, const float *__restrict__
     double acc[CY_ROWS];
 #pragma unroll
     for (int r = 0; r < CY_ROWS; ++r) acc[r] = 0.0;
-    for (int col = sl; col < nc; col += NS) {
-        const double gv = gco[(size_t)col * MB_NRMAX + k];
+    // 4 columns per pass, loads first (absent columns read column sl and add nothing)
+    for (int cb = sl; cb < nc; cb += 4 * NS) {
+        double gv[4];
+        float av[4][CY_ROWS];
 #pragma unroll
-        for (int r = 0; r < CY_ROWS; ++r) {
-            const int row = min(row0 + r, nc - 1);
-            acc[r] += (double)ainv[(size_t)row * nc + col] * gv;
+        for (int u = 0; u < 4; ++u) {
+            const int col = cb + u * NS < nc ? cb + u * NS : sl;
+            gv[u] = cb + u * NS < nc ? gco[(size_t)col * MB_NRMAX + k] : 0.0;
+#pragma unroll
+            for (int r = 0; r < CY_ROWS; ++r) av[u][r] = ainv[(size_t)min(row0 + r, nc - 1) * nc + col];
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int r = 0; r < CY_ROWS; ++r) acc[r] += (double)av[u][r] * gv[u];
     }
 #pragma unroll
     for (int r = 0; r < CY_ROWS; ++r) s_acc[sl][r][k] = acc[r];
