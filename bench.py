@@ -799,7 +799,7 @@ def bench_post(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("post_strain_cfg2"),
                      "traffic_source": traffic_source("post_strain_cfg2"),
-                     "kernel": "k_post_pack + k_post_tet_fused (one call)",
+                     "kernel": "k_post_chunks (one call)",
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }

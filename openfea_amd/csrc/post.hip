@@ -22,9 +22,11 @@
 //     node's elements in ascending element order (the reference's loop
 //     order) and divide by the number of element incidences; a node in no
 //     element keeps 0.
-// On all-tet meshes k_post_tet_fused computes the nodal values in one pass
-// instead, recomputing each tet at its 4 nodes (no element rows in memory);
-// k_post_elem then runs only when the element values are asked for.
+// On all-tet meshes k_post_chunks computes the nodal values in one pass
+// instead, over chunks of consecutive nodes whose distinct tets it evaluates
+// once per chunk (no element rows in memory); meshes beyond its chunk
+// capacity use k_post_pack + k_post_tet_fused (each tet evaluated at its 4
+// nodes); k_post_elem then runs only when the element values are asked for.
 #include <algorithm>
 #include <vector>
 
@@ -48,8 +50,24 @@ struct PostWs {
     double *ev = nullptr;    // ne x 8: the 7 element values of element e in row e
     double *ev7 = nullptr;   // 7 x ne, component-major copy for the caller (allocated on first request)
     double *nv = nullptr;    // 7 x nd
-    double *xu = nullptr;    // nd x 6 packed coordinates and displacement (all-tet meshes)
+    double *xu = nullptr;    // nd x 6 packed coordinates and displacement (all-tet meshes beyond the chunk plan)
+    // chunk plan of all-tet meshes (k_post_chunks): fixed strides per chunk
+    int nchunks = 0;
+    int *ch_hdr = nullptr;       // 4 per chunk: first owned node, owned nodes, distinct elements, incidence offset
+    int *ch_node = nullptr;      // PC_NTAB per chunk: node id (owned nodes first, in order), -1 = none
+    uint32_t *ch_eloc = nullptr; // PC_ECAP per chunk: 4 x 8-bit node slots per element, ascending element
+    int *ch_eid = nullptr;       // PC_ECAP per chunk: element index (per-element moduli)
+    uint16_t *ch_ist = nullptr;  // PC_NOWN + 1 per chunk: first incidence of each owned node, chunk-relative
+    uint8_t *ch_inc = nullptr;   // the owned nodes' elements as chunk slots, ascending, chunk after chunk
 };
+
+// Chunk plan of the fused nodal pass on all-tet meshes: a chunk owns a run of
+// consecutive nodes (all nodes, clamped ones too) and lists the distinct
+// elements of those nodes (computed once per chunk) and every node they touch.
+constexpr int PC_ECAP = 248;  // distinct elements per chunk (one per thread)
+constexpr int PC_NTAB = 160;  // nodes per chunk, owned and touched
+constexpr int PC_NOWN = 32;   // owned nodes per chunk (eight lanes each)
+constexpr int PC_ICAP = 1024; // incidences of the owned nodes per chunk
 
 __device__ inline double mises_form(const double s[6])
 {
@@ -258,6 +276,116 @@ __global__ __launch_bounds__(POST_WG) void k_post_tet_fused(int nd, const int *_
     }
 }
 
+// Nodal averages of an all-tet mesh over the chunk plan.  One workgroup per
+// chunk:
+//   staging: lane i < PC_NTAB gathers node i's coordinates and displacement
+//            (the resident free-DOF solution through the plan's free rank, 0
+//            at clamped nodes, or the caller's full vector) into LDS;
+//   elements: lane e computes the 7 values of chunk element e into LDS rows;
+//   nodes:   eight lanes per owned node (owned nodes are the first slots of
+//            the node table) take the node's incident elements (chunk slots,
+//            ascending, staged in LDS) j, j + 8, ..., and add the lanes' sums
+//            in a fixed tree; the average divides by the incidence count.
+// Each tet is computed once per chunk it touches; no per-node records in
+// memory.
+__global__ __launch_bounds__(POST_WG) void k_post_chunks(int nchunks, const int *__restrict__ hdr,
+                                                         const int *__restrict__ cnode, const int *__restrict__ resort,
+                                                         const uint32_t *__restrict__ celoc,
+                                                         const int *__restrict__ ceid,
+                                                         const uint16_t *__restrict__ cist,
+                                                         const uint8_t *__restrict__ cinc, const double *__restrict__ xyz,
+                                                         const double *__restrict__ u, int resident,
+                                                         const double *__restrict__ Eel, double E, double nu,
+                                                         int quantity, int nd, double *__restrict__ nv)
+{
+    __shared__ double s_n[6][PC_NTAB];        // x, y, z, ux, uy, uz of the chunk's nodes
+    __shared__ double s_v[7][PC_ECAP + 1];    // element values
+    __shared__ uint8_t s_inc[PC_ICAP];        // the owned nodes' incident elements (chunk slots)
+    __shared__ uint16_t s_ist[PC_NOWN + 1];
+    const int chunk = xcd_remap(blockIdx.x, nchunks);
+    const int t = threadIdx.x;
+    const int4 h = reinterpret_cast<const int4 *>(hdr)[chunk];
+    const int n0 = h.x, nown = h.y, nel = h.z;
+    const int node = t < PC_NTAB ? cnode[(size_t)chunk * PC_NTAB + t] : -1;
+    const uint32_t el4 = t < PC_ECAP ? celoc[(size_t)chunk * PC_ECAP + t] : ~0u;
+    const uint16_t ist = t <= PC_NOWN ? cist[(size_t)chunk * (PC_NOWN + 1) + t] : (uint16_t)0;
+    const int rank = (resident && node >= 0) ? resort[node] : -1;
+    const int ei = (Eel && t < nel) ? ceid[(size_t)chunk * PC_ECAP + t] : -1;
+    double r[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (node >= 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r[c] = xyz[3 * (size_t)node + c];
+        if (resident) {
+            if (rank >= 0)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) r[3 + c] = u[3 * (size_t)rank + c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) r[3 + c] = u[3 * (size_t)node + c];
+        }
+    }
+    double Ee = E;
+    if (ei >= 0) Ee = Eel[ei];
+    if (t < PC_NTAB)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) s_n[c][t] = r[c];
+    if (t <= PC_NOWN) s_ist[t] = ist;
+    __syncthreads();
+    const int ninc = s_ist[nown];
+    for (int k = t; k < ninc; k += POST_WG) s_inc[k] = cinc[(size_t)h.w + k];
+    if (t < nel) {
+        double X[12], U[12];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int sl = (el4 >> (8 * a)) & 255;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                X[3 * a + c] = s_n[c][sl];
+                U[3 * a + c] = s_n[3 + c][sl];
+            }
+        }
+        double g[3][4];
+        tet_grads(X, g);
+        double eps[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        strain_of<4>(g, U, 1.0, eps);
+        double out[7];
+        if (quantity == OFEA_POST_STRESS) {
+            double lam, mu;
+            lame<double>(Ee, nu, lam, mu);
+            const double tr = eps[0] + eps[1] + eps[2];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) out[i] = lam * tr + 2.0 * mu * eps[i];
+#pragma unroll
+            for (int i = 3; i < 6; ++i) out[i] = mu * eps[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) out[i] = eps[i];
+        }
+        out[6] = mises_form(out);
+#pragma unroll
+        for (int c = 0; c < 7; ++c) s_v[c][t] = out[c];
+    }
+    __syncthreads();
+    const int s = t >> 3, l = t & 7;  // owned node slot s, lane l
+    if (s >= nown) return;  // whole 8-lane groups leave together
+    const int k0 = s_ist[s], k1 = s_ist[s + 1], cnt = k1 - k0;
+    double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = k0 + l; k < k1; k += 8) {
+        const int j = s_inc[k];
+#pragma unroll
+        for (int c = 0; c < 7; ++c) acc[c] += s_v[c][j];
+    }
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1)
+#pragma unroll
+        for (int c = 0; c < 7; ++c) acc[c] += __shfl_down(acc[c], o, 8);
+    if (l == 0) {
+        const int i = n0 + s;
+#pragma unroll
+        for (int c = 0; c < 7; ++c) nv[(size_t)c * nd + i] = cnt > 0 ? acc[c] / (double)cnt : 0.0;
+    }
+}
+
 // Eight lanes per node, lane c < 7 owns component c: the lanes of a node read
 // one 64-byte element row per incidence, in ascending element order.  Output
 // component-major [c][node] (the layout the caller receives).
@@ -298,8 +426,17 @@ void free_ws(PostWs *w)
     dev_free(w->ev7);
     dev_free(w->nv);
     dev_free(w->xu);
+    dev_free(w->ch_hdr);
+    dev_free(w->ch_node);
+    dev_free(w->ch_ist);
+    dev_free(w->ch_inc);
+    dev_free(w->ch_eloc);
+    dev_free(w->ch_eid);
     delete w;
 }
+
+bool build_chunks(ofea_ctx *c, PostWs *w, const std::vector<int> &conn0, const std::vector<int> &ptr,
+                  const std::vector<int> &inc);
 
 int build(ofea_ctx *c, PostWs *w)
 {
@@ -341,7 +478,117 @@ int build(ofea_ctx *c, PostWs *w)
     if ((rc = dev_upload<double>(w->u, nullptr, 3 * (size_t)nd))) return rc;
     if ((rc = dev_upload<double>(w->ev, nullptr, 8 * (size_t)ne))) return rc;
     if ((rc = dev_upload<double>(w->nv, nullptr, 7 * (size_t)nd))) return rc;
+    w->nchunks = 0;
+    if (!c->has_type[OFEA_C3D8] && nd > 0 && !build_chunks(c, w, conn0, ptr, inc)) w->nchunks = 0;
     return OFEA_OK;
+}
+
+// Chunk plan of k_post_chunks (all-tet meshes): chunks of consecutive nodes
+// grown while their distinct elements, touched nodes and owned nodes fit.
+// Returns false (the fused per-node pass is kept) when one node alone does
+// not fit.
+bool build_chunks(ofea_ctx *c, PostWs *w, const std::vector<int> &conn0, const std::vector<int> &ptr,
+                  const std::vector<int> &inc)
+{
+    const int nd = c->nd;
+    std::vector<int> hdr, cnode, ceid;
+    std::vector<uint32_t> celoc;
+    std::vector<uint16_t> cist;
+    std::vector<uint8_t> cinc;
+    std::vector<int> eslot(c->ne, -1), nslot(nd, -1);
+    std::vector<int> els, others;
+    int i = 0;
+    while (i < nd) {
+        const int n0 = i;
+        els.clear();
+        others.clear();
+        int ntouched = 0;
+        int nincid = 0;
+        while (i < nd && i - n0 < PC_NOWN) {
+            // what node i adds: its elements not yet in the chunk and their nodes
+            int add_e = 0, add_n = nslot[i] < 0 ? 1 : 0;
+            std::vector<int> new_nodes;
+            for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+                const int e = inc[k];
+                if (eslot[e] >= 0) continue;
+                eslot[e] = -2;
+                ++add_e;
+                for (int a = 0; a < 4; ++a) {
+                    const int v = conn0[8 * (size_t)e + a];
+                    if (v != i && nslot[v] == -1) nslot[v] = -3, new_nodes.push_back(v);
+                }
+            }
+            const bool fits = (int)els.size() + add_e <= PC_ECAP &&
+                              ntouched + add_n + (int)new_nodes.size() <= PC_NTAB &&
+                              nincid + (ptr[i + 1] - ptr[i]) <= PC_ICAP;
+            for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+                const int e = inc[k];
+                if (eslot[e] != -2) continue;
+                eslot[e] = fits ? 0 : -1;
+                if (fits) els.push_back(e);
+            }
+            for (int v : new_nodes) nslot[v] = fits ? 0 : -1;
+            if (!fits) {
+                if (i == n0) {
+                    for (int e : els) eslot[e] = -1;
+                    for (int v = 0; v < nd; ++v) nslot[v] = -1;
+                    return false;
+                }
+                break;
+            }
+            if (nslot[i] < 0) ++ntouched;  // an owned node that no earlier element touched
+            nslot[i] = 0;
+            for (int v : new_nodes) others.push_back(v);
+            ntouched += (int)new_nodes.size();
+            nincid += ptr[i + 1] - ptr[i];
+            ++i;
+        }
+        // node table: owned nodes n0..i-1 first, then the other touched nodes ascending
+        std::sort(els.begin(), els.end());
+        std::vector<int> tab;
+        for (int v = n0; v < i; ++v) tab.push_back(v);
+        std::sort(others.begin(), others.end());
+        for (int v : others)
+            if (v < n0 || v >= i) tab.push_back(v);
+        if ((int)tab.size() > PC_NTAB) {  // cannot happen: counted above
+            return false;
+        }
+        for (int k = 0; k < (int)tab.size(); ++k) nslot[tab[k]] = k;
+        hdr.insert(hdr.end(), {n0, i - n0, (int)els.size(), (int)cinc.size()});
+        const size_t nb = cnode.size(), eb = celoc.size();
+        cnode.resize(nb + PC_NTAB, -1);
+        celoc.resize(eb + PC_ECAP, ~0u);
+        ceid.resize(eb + PC_ECAP, -1);
+        for (int k = 0; k < (int)tab.size(); ++k) cnode[nb + k] = tab[k];
+        for (int k = 0; k < (int)els.size(); ++k) eslot[els[k]] = k;
+        // owned nodes' incidences as chunk element slots (inc is ascending per node)
+        const size_t ib = cist.size();
+        cist.resize(ib + PC_NOWN + 1, 0);
+        int acc = 0;
+        for (int v = n0; v < i; ++v) {
+            cist[ib + (v - n0)] = (uint16_t)acc;
+            for (int k = ptr[v]; k < ptr[v + 1]; ++k) cinc.push_back((uint8_t)eslot[inc[k]]);
+            acc += ptr[v + 1] - ptr[v];
+        }
+        for (int v = i - n0; v <= PC_NOWN; ++v) cist[ib + v] = (uint16_t)acc;
+        for (int k = 0; k < (int)els.size(); ++k) {
+            uint32_t wv = 0;
+            for (int a = 0; a < 4; ++a) wv |= (uint32_t)nslot[conn0[8 * (size_t)els[k] + a]] << (8 * a);
+            celoc[eb + k] = wv;
+            ceid[eb + k] = els[k];
+        }
+        for (int e : els) eslot[e] = -1;
+        for (int v : tab) nslot[v] = -1;
+        w->nchunks++;
+    }
+    int rc;
+    if ((rc = dev_upload(w->ch_hdr, hdr.data(), hdr.size()))) return false;
+    if ((rc = dev_upload(w->ch_node, cnode.data(), cnode.size()))) return false;
+    if ((rc = dev_upload(w->ch_ist, cist.data(), cist.size()))) return false;
+    if ((rc = dev_upload(w->ch_inc, cinc.data(), std::max<size_t>(cinc.size(), 1)))) return false;
+    if ((rc = dev_upload(w->ch_eloc, celoc.data(), celoc.size()))) return false;
+    if ((rc = dev_upload(w->ch_eid, ceid.data(), ceid.size()))) return false;
+    return true;
 }
 
 int ensure(ofea_ctx *c, PostWs *&w)
@@ -399,7 +646,11 @@ int post_nodal(ofea_ctx *c, double E, double nu, const double *u_full, int quant
                                uu, w->E, E, nu, quantity, w->ev);
         OFEA_HIP(hipGetLastError());
     }
-    if (fused) {
+    if (fused && w->nchunks > 0) {
+        hipLaunchKernelGGL(k_post_chunks, dim3(w->nchunks), blk, 0, c->stream, w->nchunks, w->ch_hdr, w->ch_node,
+                           c->d_resort, w->ch_eloc, w->ch_eid, w->ch_ist, w->ch_inc, c->d_xyz, uu, resident ? 1 : 0,
+                           w->E, E, nu, quantity, nd, w->nv);
+    } else if (fused) {
         if (!w->xu && (rc = dev_upload<double>(w->xu, nullptr, 6 * (size_t)nd))) return rc;
         const dim3 gp((nd + POST_WG - 1) / POST_WG);
         if (resident)
