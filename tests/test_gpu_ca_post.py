@@ -174,3 +174,47 @@ def test_post_patch_test_gpu(ctx):
     for c in range(6):
         assert np.allclose(ev[c], eps[c], rtol=1e-10, atol=1e-16)
         assert np.allclose(nv[c], eps[c], rtol=1e-10, atol=1e-16)
+
+
+def star_mesh(npts=128):
+    """A node at the centre of the convex hull of npts points on a sphere, one
+    tet per hull triangle (2 npts - 4 tets share the centre): more elements
+    at one node than the chunk plans hold, within the pattern's limits."""
+    from scipy.spatial import ConvexHull
+
+    k = np.arange(npts) + 0.5
+    phi = np.arccos(1.0 - 2.0 * k / npts)
+    th = np.pi * (1.0 + 5.0 ** 0.5) * k
+    pts = np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], axis=1)
+    hull = ConvexHull(pts)
+    xyz = np.vstack([np.zeros(3), pts])
+    conn = np.zeros((len(hull.simplices), 8), dtype=np.int32)
+    for t, tri in enumerate(hull.simplices):
+        a, b, c = (pts[i] for i in tri)
+        if np.dot(np.cross(b - a, c - a), a + b + c) < 0:  # the centre-first tet positively oriented
+            tri = tri[[0, 2, 1]]
+        conn[t, :4] = [1, tri[0] + 2, tri[1] + 2, tri[2] + 2]
+    return xyz, conn, np.zeros(len(conn), dtype=np.int32)
+
+
+def test_high_valence_node_falls_back(ctx):
+    """One node in 252 tets: the C3D4 assembly takes the pair-gather kernel and
+    the strain recovery the per-node record pass; both match the oracle."""
+    xyz, conn, etype = star_mesh(128)
+    assert len(etype) == 252
+    fix = np.arange(2, 12)  # ten sphere nodes clamped (1-based ids)
+    resort = mg.free_index(len(xyz), fix)
+    say(f"high-valence node: {len(etype)} tets share node 1")
+    ctx.set_mesh(xyz, conn, etype, resort)
+    cp, ri = ctx.pattern()
+    K = ctx.assemble(E0, NU0)
+    Ko = O.fill_csc(conn, etype, xyz, resort, E0, NU0, cp, ri)
+    assert rel_max(K, Ko) < 1e-12
+    rng = np.random.default_rng(3)
+    u_full = rng.normal(size=3 * len(xyz)) * 1e-3
+    nv, ev = ctx.post_nodal(E0, NU0, u_full, A.POST_STRAIN, elem=True)
+    evo = O.post_elem(conn, etype, xyz, u_full, E0, NU0, A.POST_STRAIN, None)
+    nvo = O.post_node(conn, etype, len(xyz), evo)
+    for c in range(7):
+        assert rel_max(ev[c], evo[c]) < 1e-12, c
+        assert rel_max(nv[c], nvo[c]) < 1e-12, c
