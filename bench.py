@@ -349,7 +349,7 @@ def bench_explicit(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("explicit_step_cfg3"),
                      "traffic_source": traffic_source("explicit_step_cfg3"),
-                     "kernel": "k_force_hex<float> + k_node_update<float> (one step)",
+                     "kernel": "k_tile_forces<float> + k_tile_boundary<float> (one step)",
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "fp32_flops_per_step": flops,
                      "fp32_tflops": flops / (ms_step * 1e-3) / 1e12, "fp32_peak_tflops": FP32_PEAK_TFLOPS,

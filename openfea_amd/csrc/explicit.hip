@@ -7,9 +7,20 @@
 // InForce_k = -K0 u_k; here f_int = -K0 u_k is evaluated directly, matrix-free,
 // which is the same quantity without the running-sum round-off.
 //
-// One step = two kernels:
-//   element phase  one lane per element: the nodal forces Ke u_e (hexes in the
-//                  moment form of the trilinear element, k_force_hex) stored
+// One step = two kernels.  All-hex meshes, fixed step (ofea_explicit_run):
+//   k_tile_forces   one workgroup per compact tile of 256 hexes: the tile's
+//                   nodes staged in LDS, one lane per hex computes its nodal
+//                   forces Ke u_e (moment form of the trilinear element,
+//                   hex_moment_forces), the forces are summed in LDS in 8
+//                   conflict-free corner passes, and the nodes whose hexes
+//                   all lie in the tile get the central-difference update;
+//   k_tile_boundary the other nodes add their tiles' partial sums in tile
+//                   order and get the same update.
+//   The run keeps its node state in the tile order (boundary nodes, then each
+//   tile's interior nodes).
+// Meshes with tets, tiles that do not qualify, and the reference's time
+// control (ofea_explicit_analysis):
+//   element phase  one lane per element: the nodal forces Ke u_e stored
 //                  local-node-major, fe[a][e], so that a wave's stores of one
 //                  local node fill whole lines and the node phase's gathers
 //                  of consecutive nodes read consecutive elements; no atomics;
@@ -66,7 +77,25 @@ struct ExplicitWS {
     hipGraphExec_t texec = nullptr;
     double tg_E = 0, tg_nu = 0;
     int tg_auto = -1;
+    // element tiles of an all-hex mesh (k_tile_forces + k_tile_boundary); a
+    // tiled run keeps its node state in the tile order `perm` (boundary nodes
+    // first, then each tile's interior nodes)
+    int tiles = 0;                           // 0 not built, 1 built, -1 not applicable
+    int ntiles = 0, nbnode = 0, npart = 0;
+    int *t_hdr = nullptr;                    // 4 per tile: first interior node, interior count, boundary count, first partial
+    int *t_bnode = nullptr;                  // EXT_NT per tile: the tile's boundary nodes (tile order indices)
+    uint16_t *t_slot = nullptr;              // 8 per element lane, EX_WG per tile: node-table slots, ~0 = none
+    int *b_ptr = nullptr, *b_part = nullptr; // boundary node j: its partial slots, tile order
+    void *part = nullptr;                    // 3 per partial, run precision
+    std::vector<int> perm;                   // tile order index -> node id
 };
+
+// Tiles of the explicit step on all-hex meshes: EX_WG hexes per tile (one
+// lane each), compact by recursive bisection of the element centroids, at
+// most EXT_NT distinct nodes.  Within a tile no node is corner a of two
+// hexes, so the 8 passes (one per corner) of the LDS force sums have no
+// conflicts.
+constexpr int EXT_NT = 640;
 
 static ExplicitWS *ews(ofea_ctx *c) { return (ExplicitWS *)c->explicit_ws; }
 
@@ -80,7 +109,8 @@ void explicit_free(ofea_ctx *c)
     if (w->tgraph) hipGraphDestroy(w->tgraph);
     for (void *p : {(void *)w->inc_ptr, (void *)w->inc, w->xyz_t, w->fe[0], w->fe[1], w->u[0], w->u[1], w->v, w->minv,
                     w->fext, (void *)w->fixed, (void *)w->tc, (void *)w->dtpart, w->frames, (void *)w->fidx,
-                    (void *)w->ft})
+                    (void *)w->ft, (void *)w->t_hdr, (void *)w->t_bnode, (void *)w->t_slot, (void *)w->b_ptr,
+                    (void *)w->b_part, w->part})
         if (p) hipFree(p);
     delete w;
     c->explicit_ws = nullptr;
@@ -121,6 +151,141 @@ static int ensure_incidence(ofea_ctx *c)
         w->conn[t] = c->dp[t].conn;
         w->ne[t] = c->has_type[t] ? c->G[t].ne : 0;
     }
+    return OFEA_OK;
+}
+
+// Element tiles of an all-hex mesh for the tiled explicit step.  Leaves
+// w->tiles = -1 (the two-kernel step is kept) when the mesh has tets, a tile
+// touches more than EXT_NT nodes or a node is the same corner of two hexes of
+// one tile.  Tiles: the hex centroids cut by recursive bisection across the
+// longest extent (ties broken by the other axes, then the index) into pieces
+// of EX_WG hexes, all full but the last.  A node whose hexes all lie in one
+// tile is that tile's interior node; every other node (a boundary node) gets
+// one partial slot per tile that touches it, in tile order.  The run's node
+// state is kept in the tile order: the boundary nodes (ascending), then each
+// tile's interior nodes, so that both kernels read and write contiguous runs.
+static int build_tiles(ofea_ctx *c, ExplicitWS *w)
+{
+    if (w->tiles) return OFEA_OK;
+    w->tiles = -1;
+    if (c->has_type[OFEA_C3D4] || !c->has_type[OFEA_C3D8]) return OFEA_OK;
+    const GatherPlan &G = c->G[OFEA_C3D8];
+    const int ne = G.ne, nd = c->nd;
+    if (ne == 0) return OFEA_OK;
+    std::vector<double> cen(3 * (size_t)ne, 0.0);
+    for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < 8; ++a)
+            for (int k = 0; k < 3; ++k) cen[3 * (size_t)e + k] += 0.125 * c->xyz[3 * (size_t)G.conn[8 * (size_t)e + a] + k];
+    std::vector<int> ord(ne);
+    for (int e = 0; e < ne; ++e) ord[e] = e;
+    std::vector<std::pair<int, int>> stack{{0, ne}}, pieces;
+    while (!stack.empty()) {
+        const auto [lo, hi] = stack.back();
+        stack.pop_back();
+        if (hi - lo <= EX_WG) {
+            pieces.push_back({lo, hi});
+            continue;
+        }
+        double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+        for (int i = lo; i < hi; ++i)
+            for (int k = 0; k < 3; ++k) {
+                mn[k] = std::min(mn[k], cen[3 * (size_t)ord[i] + k]);
+                mx[k] = std::max(mx[k], cen[3 * (size_t)ord[i] + k]);
+            }
+        int ax = 0;
+        for (int k = 1; k < 3; ++k)
+            if (mx[k] - mn[k] > mx[ax] - mn[ax]) ax = k;
+        const int a1 = (ax + 1) % 3, a2 = (ax + 2) % 3;
+        const int mid = lo + ((hi - lo) / 2 + EX_WG - 1) / EX_WG * EX_WG;
+        std::nth_element(ord.begin() + lo, ord.begin() + mid, ord.begin() + hi, [&](int x, int y) {
+            const double *px = &cen[3 * (size_t)x], *py = &cen[3 * (size_t)y];
+            if (px[ax] != py[ax]) return px[ax] < py[ax];
+            if (px[a1] != py[a1]) return px[a1] < py[a1];
+            if (px[a2] != py[a2]) return px[a2] < py[a2];
+            return x < y;
+        });
+        stack.push_back({mid, hi});
+        stack.push_back({lo, mid});
+    }
+    const int nt = (int)pieces.size();
+    std::vector<int> total(nd, 0);
+    for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < 8; ++a) total[G.conn[8 * (size_t)e + a]]++;
+    // pass 1: each tile's hexes (ascending) and nodes, interior or boundary
+    std::vector<std::vector<int>> tel(nt), tint(nt), tbnd(nt);
+    std::vector<int> intile(nd, 0), corner_stamp(8 * (size_t)nd, -1), owner(nd, -1);
+    for (int t = 0; t < nt; ++t) {
+        std::vector<int> &els = tel[t];
+        els.assign(ord.begin() + pieces[t].first, ord.begin() + pieces[t].second);
+        std::sort(els.begin(), els.end());
+        std::vector<int> nodes;
+        for (int e : els)
+            for (int a = 0; a < 8; ++a) {
+                const int n = G.conn[8 * (size_t)e + a];
+                if (corner_stamp[8 * (size_t)n + a] == t) return OFEA_OK;  // a node at one corner of two hexes
+                corner_stamp[8 * (size_t)n + a] = t;
+                if (intile[n]++ == 0) nodes.push_back(n);
+            }
+        if ((int)nodes.size() > EXT_NT) return OFEA_OK;
+        std::sort(nodes.begin(), nodes.end());
+        for (int n : nodes) {
+            if (intile[n] == total[n]) {
+                tint[t].push_back(n);
+                owner[n] = t;
+            } else {
+                tbnd[t].push_back(n);
+            }
+            intile[n] = 0;
+        }
+    }
+    // pass 2: the tile order (boundary nodes ascending, then interior nodes tile by tile)
+    std::vector<int> perm, iperm(nd, -1);
+    for (int n = 0; n < nd; ++n)
+        if (owner[n] < 0) perm.push_back(n);
+    const int nb = (int)perm.size();
+    std::vector<int> hdr(4 * (size_t)nt);
+    int npart = 0;
+    for (int t = 0; t < nt; ++t) {
+        hdr[4 * t] = (int)perm.size();
+        hdr[4 * t + 1] = (int)tint[t].size();
+        hdr[4 * t + 2] = (int)tbnd[t].size();
+        hdr[4 * t + 3] = npart;
+        npart += (int)tbnd[t].size();
+        perm.insert(perm.end(), tint[t].begin(), tint[t].end());
+    }
+    for (int j = 0; j < nd; ++j) iperm[perm[j]] = j;
+    std::vector<int> t_bnode((size_t)nt * EXT_NT, -1), slot(nd, -1);
+    std::vector<uint16_t> t_slot((size_t)nt * EX_WG * 8, 0xFFFFu);
+    std::vector<std::vector<int>> parts_of(nb);
+    for (int t = 0; t < nt; ++t) {
+        const int ni = (int)tint[t].size();
+        for (int i = 0; i < ni; ++i) slot[tint[t][i]] = i;
+        for (int k = 0; k < (int)tbnd[t].size(); ++k) {
+            const int n = tbnd[t][k];
+            slot[n] = ni + k;
+            t_bnode[(size_t)t * EXT_NT + k] = iperm[n];
+            parts_of[iperm[n]].push_back(hdr[4 * t + 3] + k);
+        }
+        for (int l = 0; l < (int)tel[t].size(); ++l)
+            for (int a = 0; a < 8; ++a)
+                t_slot[((size_t)t * EX_WG + l) * 8 + a] = (uint16_t)slot[G.conn[8 * (size_t)tel[t][l] + a]];
+    }
+    std::vector<int> b_ptr{0}, b_part;
+    for (int j = 0; j < nb; ++j) {
+        b_part.insert(b_part.end(), parts_of[j].begin(), parts_of[j].end());
+        b_ptr.push_back((int)b_part.size());
+    }
+    int rc;
+    if ((rc = dev_upload(w->t_hdr, hdr.data(), hdr.size()))) return rc;
+    if ((rc = dev_upload(w->t_bnode, t_bnode.data(), t_bnode.size()))) return rc;
+    if ((rc = dev_upload(w->t_slot, t_slot.data(), t_slot.size()))) return rc;
+    if ((rc = dev_upload(w->b_ptr, b_ptr.data(), b_ptr.size()))) return rc;
+    if ((rc = dev_upload(w->b_part, b_part.data(), std::max<size_t>(b_part.size(), 1)))) return rc;
+    w->ntiles = nt;
+    w->nbnode = nb;
+    w->npart = npart;
+    w->perm = std::move(perm);
+    w->tiles = 1;
     return OFEA_OK;
 }
 
@@ -262,32 +427,14 @@ __device__ __forceinline__ int xcd_block(int q) { return (int)(blockIdx.x & 7) *
 // the inverse transform 1/8 sum_m s_a^m A_m of 7 moments A_m accumulated
 // over the points, which is Ke u (src/Elements/C3D8_Gauss.cpp:80-108,
 // src/Elements/C3D8.cpp:158-193).
+// The nodal forces of one hex from its corner-ordered coefficient inputs:
+// cX, cU hold 1/8 X and 1/8 u at corner b (node hex_node_of_corner(b)) and
+// are transformed in place; on return the force at corner b is 1/8 Am[b].
 template <class T>
-__global__ __launch_bounds__(EX_WG) void k_force_hex(int ne, int q, const int *__restrict__ conn,
-                                                     const T *__restrict__ xyz, const T *__restrict__ u, T lam, T mu,
-                                                     T *__restrict__ fe, const int *__restrict__ stop)
+__device__ __forceinline__ void hex_moment_forces(T cX[8][3], T cU[8][3], T lam, T mu, T Am[8][3])
 {
-    if (stop && *stop) return;
-    const int e = xcd_block(q) * EX_WG + threadIdx.x;
-    if (e >= ne) return;
-    const int4 n0 = *reinterpret_cast<const int4 *>(conn + 8 * (size_t)e);
-    const int4 n1 = *reinterpret_cast<const int4 *>(conn + 8 * (size_t)e + 4);
-    const int nodes[8] = {n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, n1.z, n1.w};
-    // coefficients by a 3-stage Walsh-Hadamard butterfly over the corner
-    // index b = [x > 0] + 2 [y > 0] + 4 [z > 0] (node hex_corner[b])
-    T cX[8][3], cU[8][3];
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-        const int node = nodes[hex_node_of_corner(b)];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            cX[b][k] = T(0.125) * xyz[3 * node + k];
-            cU[b][k] = T(0.125) * u[3 * node + k];
-        }
-    }
     wht8<T, 3>(cX);
     wht8<T, 3>(cU);
-    T Am[8][3];
 #pragma unroll
     for (int m = 0; m < 8; ++m)
 #pragma unroll
@@ -349,6 +496,34 @@ __global__ __launch_bounds__(EX_WG) void k_force_hex(int ne, int q, const int *_
     }
     // f at corner b = 1/8 sum_m s_b^m A_m: the transposed butterfly
     wht8_t<T, 3>(Am);
+}
+
+// One lane per hex: its nodal forces (hex_moment_forces) stored fe[a][e].
+template <class T>
+__global__ __launch_bounds__(EX_WG) void k_force_hex(int ne, int q, const int *__restrict__ conn,
+                                                     const T *__restrict__ xyz, const T *__restrict__ u, T lam, T mu,
+                                                     T *__restrict__ fe, const int *__restrict__ stop)
+{
+    if (stop && *stop) return;
+    const int e = xcd_block(q) * EX_WG + threadIdx.x;
+    if (e >= ne) return;
+    const int4 n0 = *reinterpret_cast<const int4 *>(conn + 8 * (size_t)e);
+    const int4 n1 = *reinterpret_cast<const int4 *>(conn + 8 * (size_t)e + 4);
+    const int nodes[8] = {n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, n1.z, n1.w};
+    // coefficients by a 3-stage Walsh-Hadamard butterfly over the corner
+    // index b = [x > 0] + 2 [y > 0] + 4 [z > 0] (node hex_corner[b])
+    T cX[8][3], cU[8][3];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const int node = nodes[hex_node_of_corner(b)];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            cX[b][k] = T(0.125) * xyz[3 * node + k];
+            cU[b][k] = T(0.125) * u[3 * node + k];
+        }
+    }
+    T Am[8][3];
+    hex_moment_forces<T>(cX, cU, lam, mu, Am);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
         T *dst = fe + 3 * ((size_t)hex_node_of_corner(b) * ne + e);
@@ -449,6 +624,125 @@ __global__ __launch_bounds__(EX_WG) void k_node_update(int nd, int q, const int 
     unext[3 * i + 2] = fx ? T(0) : ucur[3 * i + 2] + v2 * dt;
 }
 
+// Tiled step of an all-hex mesh (build_tiles), one workgroup per tile, node
+// state in the tile order:
+//   staging: the tile's nodes' X and u go to LDS (its interior nodes are one
+//            contiguous run, its boundary nodes are listed), force sums zeroed;
+//   forces:  lane l computes the nodal forces of hex l (hex_moment_forces);
+//   sums:    8 passes, one per corner b: every hex adds its force at corner b
+//            to that node's LDS sum (no node is corner b of two hexes of the
+//            tile, so a pass has no conflicts; the order is fixed);
+//   nodes:   an interior node is updated here, a = (f + f_ext)/m,
+//            v += a dt, u_next = u + v dt (0 at clamped nodes); a boundary
+//            node's sum goes to the tile's partial slot for it.
+// f = -K u: the sums hold K u and enter with a minus sign.
+template <class T>
+__device__ __forceinline__ void tile_node_update(int n, T f0, T f1, T f2, T u0, T u1, T u2, T *__restrict__ unext,
+                                                 T *__restrict__ v, const T *__restrict__ minv,
+                                                 const T *__restrict__ fext, const int *__restrict__ fixed, T dt)
+{
+    const T mi = minv[n];
+    const T v0 = v[3 * (size_t)n] + (f0 + fext[3 * (size_t)n]) * mi * dt;
+    const T v1 = v[3 * (size_t)n + 1] + (f1 + fext[3 * (size_t)n + 1]) * mi * dt;
+    const T v2 = v[3 * (size_t)n + 2] + (f2 + fext[3 * (size_t)n + 2]) * mi * dt;
+    v[3 * (size_t)n] = v0;
+    v[3 * (size_t)n + 1] = v1;
+    v[3 * (size_t)n + 2] = v2;
+    const bool fx = fixed[n] != 0;
+    unext[3 * (size_t)n] = fx ? T(0) : u0 + v0 * dt;
+    unext[3 * (size_t)n + 1] = fx ? T(0) : u1 + v1 * dt;
+    unext[3 * (size_t)n + 2] = fx ? T(0) : u2 + v2 * dt;
+}
+
+template <class T>
+__global__ __launch_bounds__(EX_WG) void k_tile_forces(int ntiles, const int *__restrict__ t_hdr,
+                                                       const int *__restrict__ t_bnode,
+                                                       const uint16_t *__restrict__ t_slot,
+                                                       const T *__restrict__ xyz, const T *__restrict__ ucur,
+                                                       T *__restrict__ unext, T *__restrict__ v,
+                                                       const T *__restrict__ minv, const T *__restrict__ fext,
+                                                       const int *__restrict__ fixed, T lam, T mu, T dt,
+                                                       T *__restrict__ part)
+{
+    __shared__ T s_x[3][EXT_NT], s_u[3][EXT_NT], s_f[3][EXT_NT];
+    const int tile = xcd_remap(blockIdx.x, ntiles);
+    const int t = threadIdx.x;
+    const int4 h = reinterpret_cast<const int4 *>(t_hdr)[tile];
+    const int ibase = h.x, ni = h.y, nn = h.y + h.z, pbase = h.w;
+    const uint4 sl = reinterpret_cast<const uint4 *>(t_slot)[(size_t)tile * EX_WG + t];  // 8 x 16-bit slots
+    for (int i = t; i < nn; i += EX_WG) {
+        const int n = i < ni ? ibase + i : t_bnode[(size_t)tile * EXT_NT + (i - ni)];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            s_x[k][i] = xyz[3 * (size_t)n + k];
+            s_u[k][i] = ucur[3 * (size_t)n + k];
+            s_f[k][i] = T(0);
+        }
+    }
+    __syncthreads();
+    const int slots[8] = {(int)(sl.x & 0xFFFFu), (int)(sl.x >> 16), (int)(sl.y & 0xFFFFu), (int)(sl.y >> 16),
+                          (int)(sl.z & 0xFFFFu), (int)(sl.z >> 16), (int)(sl.w & 0xFFFFu), (int)(sl.w >> 16)};
+    const bool has = slots[0] != 0xFFFF;
+    T Am[8][3];
+    if (has) {
+        T cX[8][3], cU[8][3];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int q = slots[hex_node_of_corner(b)];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                cX[b][k] = T(0.125) * s_x[k][q];
+                cU[b][k] = T(0.125) * s_u[k][q];
+            }
+        }
+        hex_moment_forces<T>(cX, cU, lam, mu, Am);
+    }
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        if (has) {
+            const int q = slots[hex_node_of_corner(b)];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) s_f[k][q] += T(0.125) * Am[b][k];
+        }
+        __syncthreads();
+    }
+    for (int i = t; i < nn; i += EX_WG) {
+        const T f0 = -s_f[0][i], f1 = -s_f[1][i], f2 = -s_f[2][i];
+        if (i < ni) {
+            tile_node_update<T>(ibase + i, f0, f1, f2, s_u[0][i], s_u[1][i], s_u[2][i], unext, v, minv, fext, fixed,
+                                dt);
+        } else {
+            T *p = part + 3 * (size_t)(pbase + i - ni);
+            p[0] = f0;
+            p[1] = f1;
+            p[2] = f2;
+        }
+    }
+}
+
+// Boundary nodes of the tiled step (the first nb nodes of the tile order):
+// f = the partial sums of the tiles that touch the node, added in tile order;
+// then the node update.
+template <class T>
+__global__ __launch_bounds__(EX_WG) void k_tile_boundary(int nb, const int *__restrict__ b_ptr,
+                                                         const int *__restrict__ b_part, const T *__restrict__ part,
+                                                         const T *__restrict__ ucur, T *__restrict__ unext,
+                                                         T *__restrict__ v, const T *__restrict__ minv,
+                                                         const T *__restrict__ fext, const int *__restrict__ fixed, T dt)
+{
+    const int n = blockIdx.x * EX_WG + threadIdx.x;
+    if (n >= nb) return;
+    T f0 = T(0), f1 = T(0), f2 = T(0);
+    for (int k = b_ptr[n]; k < b_ptr[n + 1]; ++k) {
+        const T *p = part + 3 * (size_t)b_part[k];
+        f0 += p[0];
+        f1 += p[1];
+        f2 += p[2];
+    }
+    tile_node_update<T>(n, f0, f1, f2, ucur[3 * (size_t)n], ucur[3 * (size_t)n + 1], ucur[3 * (size_t)n + 2], unext, v,
+                        minv, fext, fixed, dt);
+}
+
 // v_1/2 = v0 + (f_ext/m) dt/2 (f_int(u0 = 0) = 0, src/cae.cpp:471-482); u_1 = v_1/2 dt
 template <class T>
 __global__ void k_explicit_init(int nd, T *__restrict__ v, T *__restrict__ u1, const T *__restrict__ minv,
@@ -484,6 +778,16 @@ static void launch_forces(ofea_ctx *c, ExplicitWS *w, T lam, T mu, const T *ucur
 template <class T>
 static void launch_step(ofea_ctx *c, ExplicitWS *w, T lam, T mu, T dt, const T *ucur, T *unext)
 {
+    if (w->tiles == 1) {
+        hipLaunchKernelGGL(k_tile_forces<T>, dim3(w->ntiles), dim3(EX_WG), 0, c->stream, w->ntiles, w->t_hdr,
+                           w->t_bnode, w->t_slot, (const T *)w->xyz_t, ucur, unext, (T *)w->v, (const T *)w->minv,
+                           (const T *)w->fext, w->fixed, lam, mu, dt, (T *)w->part);
+        if (w->nbnode)
+            hipLaunchKernelGGL(k_tile_boundary<T>, dim3((w->nbnode + EX_WG - 1) / EX_WG), dim3(EX_WG), 0, c->stream,
+                               w->nbnode, w->b_ptr, w->b_part, (const T *)w->part, ucur, unext, (T *)w->v,
+                               (const T *)w->minv, (const T *)w->fext, w->fixed, dt);
+        return;
+    }
     launch_forces<T>(c, w, lam, mu, ucur, nullptr);
     hipLaunchKernelGGL(k_node_update<T>, dim3(8 * xcd_q(w->nd)), dim3(EX_WG), 0, c->stream, w->nd, xcd_q(w->nd),
                        w->inc_ptr, w->inc, w->ne[0], w->ne[1], (const T *)w->fe[0], (const T *)w->fe[1], ucur, unext,
@@ -503,9 +807,10 @@ static void drop_graphs(ExplicitWS *w)
 }
 
 // Buffers in the run precision and the inputs converted to it.
+// perm (nullable): the buffers hold node perm[j] at position j (the tiled run's order)
 template <class T>
 static int upload_inputs(ofea_ctx *c, ExplicitWS *w, const double *mass, const double *f_ext, const double *v0,
-                         const int *fixed_mask)
+                         const int *fixed_mask, const int *perm = nullptr)
 {
     const int nd = c->nd;
     const size_t n3 = 3 * (size_t)nd;
@@ -532,20 +837,31 @@ static int upload_inputs(ofea_ctx *c, ExplicitWS *w, const double *mass, const d
         OFEA_HIP(hipMalloc(&w->minv, sizeof(T) * nd));
         OFEA_HIP(hipMalloc(&w->fext, sizeof(T) * n3));
         if (!w->fixed) OFEA_HIP(hipMalloc((void **)&w->fixed, sizeof(int) * nd));
+        if (w->part) {
+            hipFree(w->part);
+            w->part = nullptr;
+        }
         w->prec = (int)sizeof(T);
     }
-    // host-side conversion of the inputs to the run precision
+    if (w->tiles == 1 && !w->part) OFEA_HIP(hipMalloc(&w->part, sizeof(T) * 3 * (size_t)std::max(w->npart, 1)));
+    // host-side conversion of the inputs to the run precision (and order)
+    auto src = [&](int j) { return perm ? perm[j] : j; };
     std::vector<T> h(n3);
-    for (size_t i = 0; i < n3; ++i) h[i] = (T)c->xyz[i];
+    for (int j = 0; j < nd; ++j)
+        for (int k = 0; k < 3; ++k) h[3 * (size_t)j + k] = (T)c->xyz[3 * (size_t)src(j) + k];
     OFEA_HIP(hipMemcpy(w->xyz_t, h.data(), sizeof(T) * n3, hipMemcpyHostToDevice));
-    for (size_t i = 0; i < n3; ++i) h[i] = (T)f_ext[i];
+    for (int j = 0; j < nd; ++j)
+        for (int k = 0; k < 3; ++k) h[3 * (size_t)j + k] = (T)f_ext[3 * (size_t)src(j) + k];
     OFEA_HIP(hipMemcpy(w->fext, h.data(), sizeof(T) * n3, hipMemcpyHostToDevice));
-    for (size_t i = 0; i < n3; ++i) h[i] = v0 ? (T)v0[i] : T(0);
+    for (int j = 0; j < nd; ++j)
+        for (int k = 0; k < 3; ++k) h[3 * (size_t)j + k] = v0 ? (T)v0[3 * (size_t)src(j) + k] : T(0);
     OFEA_HIP(hipMemcpy(w->v, h.data(), sizeof(T) * n3, hipMemcpyHostToDevice));
     std::vector<T> mi(nd);
-    for (int i = 0; i < nd; ++i) mi[i] = (T)(1.0 / mass[i]);
+    for (int j = 0; j < nd; ++j) mi[j] = (T)(1.0 / mass[src(j)]);
     OFEA_HIP(hipMemcpy(w->minv, mi.data(), sizeof(T) * nd, hipMemcpyHostToDevice));
-    OFEA_HIP(hipMemcpy(w->fixed, fixed_mask, sizeof(int) * nd, hipMemcpyHostToDevice));
+    std::vector<int> fx(nd);
+    for (int j = 0; j < nd; ++j) fx[j] = fixed_mask[src(j)];
+    OFEA_HIP(hipMemcpy(w->fixed, fx.data(), sizeof(int) * nd, hipMemcpyHostToDevice));
     return OFEA_OK;
 }
 
@@ -556,8 +872,10 @@ static int run_t(ofea_ctx *c, double E, double nu, const double *mass, const dou
     ExplicitWS *w = ews(c);
     const int nd = c->nd;
     const size_t n3 = 3 * (size_t)nd;
-    int rc = upload_inputs<T>(c, w, mass, f_ext, v0, fixed_mask);
+    int rc = build_tiles(c, w);
     if (rc) return rc;
+    const int *perm = w->tiles == 1 ? w->perm.data() : nullptr;
+    if ((rc = upload_inputs<T>(c, w, mass, f_ext, v0, fixed_mask, perm))) return rc;
     double lam_d, mu_d;
     lame<double>(E, nu, lam_d, mu_d);
     const T lam = (T)lam_d, mu = (T)mu_d, tdt = (T)dt;
@@ -600,7 +918,8 @@ static int run_t(ofea_ctx *c, double E, double nu, const double *mass, const dou
             std::vector<T> h(n3);
             OFEA_HIP(hipMemcpyAsync(h.data(), ures, sizeof(T) * n3, hipMemcpyDeviceToHost, c->stream));
             OFEA_HIP(hipStreamSynchronize(c->stream));
-            for (size_t i = 0; i < n3; ++i) u_out[i] = (double)h[i];
+            for (int j = 0; j < nd; ++j)
+                for (int k = 0; k < 3; ++k) u_out[3 * (size_t)(perm ? perm[j] : j) + k] = (double)h[3 * (size_t)j + k];
         }
     }
     return OFEA_OK;

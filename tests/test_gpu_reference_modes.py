@@ -99,7 +99,10 @@ def test_explicit_analysis_fixed_step(ctx):
     g = ctx.explicit_analysis(E, NU, mass, fext, fixed, nsteps * dt, dt, 1, v0=v0, precision=A.FP64)
     run = ctx.explicit_run(E, NU, mass, fext, v0, fixed, dt, nsteps, precision=A.FP64)
     assert g["steps"] == nsteps and list(g["index"]) == [0, 1] and list(g["time"]) == [0.0, nsteps * dt]
-    assert np.array_equal(g["frames"][1], run)
+    # the same loop; the tiled fixed-step run adds each node's element forces
+    # in another fixed order (corner passes, then tiles) than the time-controlled
+    # one (ascending elements), so the two agree to rounding
+    assert rel_max(g["frames"][1], run) < 1e-12
     ref = O.explicit_timed(m.conn, m.etype, m.xyz, E, NU, mass, fext, fixed, nsteps * dt, dt, output_gap=1, v0=v0)
     assert rel_max(g["frames"][1], ref["frames"][1]) < 1e-9
     # a step that is clamped onto the outputs (0.1 total / 1.3 dt is not whole)
