@@ -192,6 +192,7 @@ def bench_static(args, workload):
                    "mixed_fp32val_block_jacobi": A.PCG_MIXED, "mixed_fp32_all_block_jacobi": A.PCG_MIXED32,
                    "learned_fp64": A.PCG_LEARNED, "learned_mixed_fp32val": A.PCG_LEARNED_MIXED,
                    "two_level_fp64": A.PCG_TWO_LEVEL, "two_level_mixed_fp32val": A.PCG_TWO_LEVEL_MIXED,
+                   "two_level_mixed_fp32_all": A.PCG_TWO_LEVEL_MIXED32,
                    "learned_two_level_fp64": A.PCG_LEARNED_TWO_LEVEL,
                    "learned_two_level_mixed_fp32val": A.PCG_LEARNED_TWO_LEVEL_MIXED,
                    "learned_two_level_mixed_fp32val_bf16net": A.PCG_LEARNED_TWO_LEVEL_MIXED}
@@ -921,7 +922,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="assembly only (for counter collection)")
     ap.add_argument("--solvers", default="plain_fp64_jacobi,fp64_block_jacobi,mixed_fp32val_block_jacobi,"
                     "mixed_fp32_all_block_jacobi,learned_fp64,learned_mixed_fp32val,two_level_fp64,"
-                    "two_level_mixed_fp32val,learned_two_level_fp64,learned_two_level_mixed_fp32val,"
+                    "two_level_mixed_fp32val,two_level_mixed_fp32_all,learned_two_level_fp64,learned_two_level_mixed_fp32val,"
                     "learned_two_level_mixed_fp32val_bf16net")
     ap.add_argument("--train-steps", type=int, default=150)
     ap.add_argument("--train-lr", type=float, default=1e-2)

@@ -80,6 +80,7 @@ extern "C" {
 #define OFEA_PCG_TWO_LEVEL_MIXED 7          /* FP32 values, FP64 vectors, reliable updates    */
 #define OFEA_PCG_LEARNED_TWO_LEVEL 8        /* FP64 values, learned blocks + coarse           */
 #define OFEA_PCG_LEARNED_TWO_LEVEL_MIXED 9  /* FP32 values, learned blocks + coarse           */
+#define OFEA_PCG_TWO_LEVEL_MIXED32 10       /* FP32 values and vectors, block Jacobi + coarse */
 
 /* quantities of ofea_post_nodal */
 #define OFEA_POST_STRAIN 0  /* B u_e and its "Mises" value: what the reference writes as S_* */

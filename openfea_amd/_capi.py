@@ -35,6 +35,7 @@ PCG_TWO_LEVEL = 6
 PCG_TWO_LEVEL_MIXED = 7
 PCG_LEARNED_TWO_LEVEL = 8
 PCG_LEARNED_TWO_LEVEL_MIXED = 9
+PCG_TWO_LEVEL_MIXED32 = 10
 
 FP32 = 0
 FP64 = 1

@@ -2790,7 +2790,7 @@ int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, 
     }
     const bool learned = method == OFEA_PCG_LEARNED || method == OFEA_PCG_LEARNED_MIXED ||
                          method == OFEA_PCG_LEARNED_TWO_LEVEL || method == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED;
-    if (method >= OFEA_PCG_TWO_LEVEL && method <= OFEA_PCG_LEARNED_TWO_LEVEL_MIXED) {
+    if (method >= OFEA_PCG_TWO_LEVEL && method <= OFEA_PCG_TWO_LEVEL_MIXED32) {
         // two-level: the reordered system (b in, x out through the permutation)
         if ((rc = coarse_prepare(ctx))) return rc;
         if (learned && (rc = learned_factor(ctx, true))) return rc;  // G on the reordered rows
@@ -2811,6 +2811,10 @@ int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, 
         case OFEA_PCG_LEARNED_TWO_LEVEL:
             rc = pcg_run<double, double, 2, true>(ctx, S, 8, w->val_p, w->v64, w->minv_p, rtol, maxit, -1.0, iters,
                                                   relres);
+            break;
+        case OFEA_PCG_TWO_LEVEL_MIXED32:  // FP32 iteration vectors too: x and the true residual stay FP64
+            rc = pcg_run<float, float, 1, true>(ctx, S, 10, w->val32_p, w->v32, w->minv32_p, rtol, maxit, MIXED_RU,
+                                                iters, relres);
             break;
         default:
             rc = pcg_run<float, double, 2, true>(ctx, S, 9, w->val32_p, w->v64, w->minv_p, rtol, maxit, MIXED_RU, iters,

@@ -98,7 +98,7 @@ def test_spmv_matches_oracle(ctx):
 
 
 @pytest.mark.parametrize("method", [A.PCG_JACOBI, A.PCG_BLOCK_JACOBI, A.PCG_MIXED, A.PCG_MIXED32,
-                                    A.PCG_TWO_LEVEL, A.PCG_TWO_LEVEL_MIXED])
+                                    A.PCG_TWO_LEVEL, A.PCG_TWO_LEVEL_MIXED, A.PCG_TWO_LEVEL_MIXED32])
 def test_pcg_cfg1_matches_direct_solve(ctx, method):
     m, resort, F = mg.cantilever_c3d8()
     say(f"pcg method={method} cfg1 20x10x10")
@@ -144,7 +144,7 @@ def test_two_level_small_and_thin_meshes(ctx, name):
     val = ctx.assemble(mg.E_STEEL, mg.NU_STEEL)
     F = ctx.spmv(np.random.default_rng(8).normal(size=ctx.n))
     xd = spla.spsolve(csc(cp, ri, val), F)
-    for meth in (A.PCG_TWO_LEVEL, A.PCG_TWO_LEVEL_MIXED):
+    for meth in (A.PCG_TWO_LEVEL, A.PCG_TWO_LEVEL_MIXED, A.PCG_TWO_LEVEL_MIXED32):
         # rtol 1e-12: the thin block's condition number turns a 1e-10 residual into ~4e-8 of error
         x, it, rel, ok = ctx.solve(F, method=meth, rtol=1e-12, maxit=100000)
         assert ok and np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8, (meth, it, rel)
