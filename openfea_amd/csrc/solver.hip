@@ -1095,14 +1095,7 @@ void solver_free(ofea_ctx *ctx)
     for (FacMat &f : w->fac)
         for (void *p : {(void *)f.lo_ptr, (void *)f.lo_col, (void *)f.up_ptr, (void *)f.up_col, (void *)f.lo, (void *)f.up})
             if (p) (void)hipFree(p);
-    for (void *p : {(void *)w->co.g, (void *)w->co.d, (void *)w->co_ainv, (void *)w->co.zq, (void *)w->co.ya})
-        if (p) (void)hipFree(p);
-    for (void *p : {(void *)w->perm, (void *)w->iperm, (void *)w->nb_p, (void *)w->bcol_p, (void *)w->val_p,
-                    (void *)w->val32_p, (void *)w->dinv_p, (void *)w->minv_p, (void *)w->dblk_p,
-                    (void *)w->minv32_p, (void *)w->b_p, (void *)w->x_p, (void *)w->co_A,
-                    (void *)w->co_inv, (void *)w->co_flag, (void *)w->sl_ptr, (void *)w->sl_part, (void *)w->sl_cptr,
-                    (void *)w->ch_ptr, (void *)w->ch_beg, (void *)w->sl_blk})
-        if (p) (void)hipFree(p);
+    if (w->co_arena) (void)hipFree(w->co_arena);  // every two-level buffer
     if (w->sc) (void)hipFree(w->sc);
     if (w->sc_host) (void)hipHostFree(w->sc_host);
     for (auto &e : w->pipe_ev)
@@ -1117,6 +1110,7 @@ void solver_drop_graphs(ofea_ctx *ctx)
     ofea_solver_ws *w = ctx->ws;
     if (!w) return;
     for (int i = 0; i < NSLOT; ++i) {
+        w->runs[i] = 0;
         for (hipGraphExec_t *e : {&w->exec[i], &w->exec_tail[i]})
             if (*e) {
                 (void)hipGraphExecDestroy(*e);
@@ -1535,7 +1529,11 @@ static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, Ve
     OFEA_HIP(hipGetLastError());
     // OFEA_NO_GRAPH=1 launches the batches directly (for tools that cannot follow graph launches)
     static const bool no_graph = std::getenv("OFEA_NO_GRAPH") && std::getenv("OFEA_NO_GRAPH")[0] == '1';
-    if (!w->exec[slot] && !no_graph) {
+    // a slot's graphs are captured on its second solve: the first one (often
+    // the only solve of a new K) launches its batches directly and pays no
+    // capture and instantiation
+    const bool direct = no_graph || (!w->exec[slot] && w->runs[slot]++ == 0);
+    if (!w->exec[slot] && !direct) {
         OFEA_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         cg_batch<TV, T, double, PC, CO>(ctx, B, Sv, val, v, x, minv, CG_BATCH);
         OFEA_HIP(hipStreamEndCapture(ctx->stream, &w->graph[slot]));
@@ -1553,7 +1551,7 @@ static int pcg_run(ofea_ctx *ctx, const SysView &Sv, int slot, const TV *val, Ve
     auto launch_batch = [&](int nit) -> int {
         const int hb = next;
         next = (next + 1) % NQ_PIPE;
-        if (no_graph) {
+        if (direct) {
             cg_batch<TV, T, double, PC, CO>(ctx, B, Sv, val, v, x, minv, nit);
             OFEA_HIP(hipGetLastError());
         } else {
@@ -1690,65 +1688,61 @@ static void rcb_parts(int *b, int m, int L, const double *X, std::vector<AxisKey
     rcb_parts(b + lm, m - lm, L, X, key);
 }
 
-// Order of the free nodes for the two-level methods: compact parts of
-// exactly L nodes (the last part may be shorter), as slabs of whole parts
-// across the longest axis of the bounding box -- about as thick as a cubic
-// part, so A_c stays banded (a part couples only to parts of its own and the
-// neighbouring slabs) -- each slab cut into its parts by recursive coordinate
-// bisection.  perm[new] = old.
-static void coarse_order(int nf, const double *X, int L, std::vector<int> &perm)
+// runs fn(t) for t in [0, n) on n host threads (t = 0 on the caller's)
+template <class F>
+static void run_threads(int n, F &&fn)
 {
-    perm.resize(nf);
-    for (int i = 0; i < nf; ++i) perm[i] = i;
-    const int nparts = (nf + L - 1) / L;
-    if (nparts <= 1) return;
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int i = 0; i < nf; ++i)
-        for (int c = 0; c < 3; ++c) {
-            lo[c] = std::min(lo[c], X[3 * (size_t)i + c]);
-            hi[c] = std::max(hi[c], X[3 * (size_t)i + c]);
-        }
-    int ax = 0;
-    for (int c = 1; c < 3; ++c)
-        if (hi[c] - lo[c] > hi[ax] - lo[ax]) ax = c;
-    const double emax = hi[ax] - lo[ax];
-    double vol = 1.0;
-    for (int c = 0; c < 3; ++c) vol *= std::max(hi[c] - lo[c], 1e-3 * emax);
-    const double edge = std::cbrt(vol / nparts);
-    const int slabs = std::max(1, std::min(nparts, (int)std::lround(emax / std::max(edge, 1e-300))));
-    const int per_slab = (nparts + slabs - 1) / slabs;  // parts per slab
-    std::vector<AxisKey> key(nf);
-    for (int i = 0; i < nf; ++i) key[i] = {X[3 * (size_t)i + ax], i};
-    {
-        // sorted in up to 8 ranges on host threads, then merged pairwise
-        const int ns = nf < 16384 ? 1 : std::max(1, std::min(8, (int)std::max(1u, std::thread::hardware_concurrency())));
-        std::vector<size_t> cut(ns + 1);
-        for (int t = 0; t <= ns; ++t) cut[t] = (size_t)nf * t / ns;
-        std::vector<std::thread> pool;
-        for (int t = 1; t < ns; ++t)
-            pool.emplace_back([&key, &cut, t] { std::sort(key.begin() + cut[t], key.begin() + cut[t + 1]); });
-        std::sort(key.begin() + cut[0], key.begin() + cut[1]);
-        for (auto &th : pool) th.join();
-        for (int w = 1; w < ns; w *= 2)
-            for (int t = 0; t + w < ns; t += 2 * w)
-                std::inplace_merge(key.begin() + cut[t], key.begin() + cut[t + w], key.begin() + cut[std::min(ns, t + 2 * w)]);
-    }
-    for (int i = 0; i < nf; ++i) perm[i] = key[i].i;
-    // the slabs are cut independently, on up to 8 host threads
-    const long long slab_nodes = (long long)per_slab * L;
-    const int nslab = (int)((nf + slab_nodes - 1) / slab_nodes);
-    const int nthr = std::max(1, std::min({8, nslab, (int)std::max(1u, std::thread::hardware_concurrency())}));
-    auto cut = [&](int th, std::vector<AxisKey> &k) {
-        for (int sl = th; sl < nslab; sl += nthr) {
-            const long long s0 = sl * slab_nodes;
-            rcb_parts(perm.data() + s0, (int)std::min<long long>(slab_nodes, nf - s0), L, X, k);
-        }
-    };
     std::vector<std::thread> pool;
-    std::vector<std::vector<AxisKey>> keys(nthr - 1);
-    for (int th = 1; th < nthr; ++th) pool.emplace_back(cut, th, std::ref(keys[th - 1]));
-    cut(0, key);
-    for (auto &t : pool) t.join();
+    for (int t = 1; t < n; ++t) pool.emplace_back([&fn, t] { fn(t); });
+    fn(0);
+    for (auto &th : pool) th.join();
+}
+
+static int host_threads(int cap)
+{
+    return std::max(1, std::min(cap, std::min(16, (int)std::max(1u, std::thread::hardware_concurrency()))));
+}
+
+// slab[i] = the slab of free node i, slabs being runs of S nodes (the last may
+// be shorter) in the total order (X[3 i + ax], i).  The nodes are binned on
+// the axis (the bin is monotone in the coordinate), so a slab boundary falls
+// inside at most one bin and only the nodes of such bins are sorted.
+static void slab_of_nodes(int nf, const double *X, int ax, double lo, double hi, long long S, std::vector<int> &slab)
+{
+    constexpr int NBIN = 1 << 14;
+    const double scale = hi > lo ? NBIN / (hi - lo) : 0.0;
+    slab.resize(nf);
+    std::vector<int> cnt(NBIN + 1, 0), cut_of(NBIN, -1);
+    for (int i = 0; i < nf; ++i) {
+        const double t = (X[3 * (size_t)i + ax] - lo) * scale;
+        const int b = t >= NBIN ? NBIN - 1 : (t > 0.0 ? (int)t : 0);
+        slab[i] = b;  // the bin, for now
+        cnt[b + 1]++;
+    }
+    for (int b = 0; b < NBIN; ++b) cnt[b + 1] += cnt[b];  // cnt[b]: nodes before bin b
+    int ncut = 0;
+    for (long long k = S; k < nf; k += S) {
+        const int b = (int)(std::upper_bound(cnt.begin(), cnt.end(), (int)k) - cnt.begin()) - 1;
+        if (cnt[b] < k && cut_of[b] < 0) cut_of[b] = ncut++;
+    }
+    std::vector<std::vector<AxisKey>> inbin(ncut);
+    std::vector<int> cut_bin(ncut);
+    for (int b = 0; b < NBIN; ++b)
+        if (cut_of[b] >= 0) cut_bin[cut_of[b]] = b;
+    for (int i = 0; i < nf; ++i) {
+        const int b = slab[i];
+        if (cut_of[b] >= 0) {
+            const double x = X[3 * (size_t)i + ax];
+            inbin[cut_of[b]].push_back({x == 0.0 ? 0.0 : x, i});  // -0 and +0 compare equal, as for <
+        } else {
+            slab[i] = (int)(cnt[b] / S);  // the whole bin lies in one slab
+        }
+    }
+    for (int j = 0; j < ncut; ++j) {
+        std::vector<AxisKey> &k = inbin[j];
+        std::sort(k.begin(), k.end());
+        for (size_t r = 0; r < k.size(); ++r) slab[k[r].i] = (int)((cnt[cut_bin[j]] + (long long)r) / S);
+    }
 }
 
 // ---- coarse-space kernels (device set-up of A_c^-1 for each K) ----
@@ -1768,19 +1762,234 @@ __device__ inline void zcol(const float *dv, int q, double z[3])
 // (chunk, p, q) sums its entry over the chunk's blocks into LDS, then one per
 // (slot, p, q) adds the slot's chunk sums in order and writes A[6a+p][6b+q]
 // (fixed order, no atomics).
-constexpr int CA_MAXCH = 160;  // chunks per aggregate (host-enforced)
+constexpr int CA_MAXCH = 160;  // chunks, and coupled aggregates, per aggregate (k_co_lists checks)
+
+// ---- device part of the coarse structure (once per mesh) ----
+// free-node coordinates in the caller's free numbering
+__global__ __launch_bounds__(RED_WG) void k_co_xfree(int nd, const int *__restrict__ resort,
+                                                     const double *__restrict__ xyz, double *__restrict__ xf)
+{
+    const int n = blockIdx.x * RED_WG + (int)threadIdx.x;
+    if (n >= nd) return;
+    const int r = resort[n];
+    if (r < 0) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xf[3 * (size_t)r + c] = xyz[3 * (size_t)n + c];
+}
+
+// iperm, and the row lengths of the renumbered pattern in nb_p[i + 1]
+__global__ __launch_bounds__(RED_WG) void k_co_rows(int nf, const int *__restrict__ perm,
+                                                    const int *__restrict__ node_blk, int *__restrict__ iperm,
+                                                    int *__restrict__ nb_p)
+{
+    const int i = blockIdx.x * RED_WG + (int)threadIdx.x;
+    if (i == 0) nb_p[0] = 0;
+    if (i >= nf) return;
+    const int o = perm[i];
+    iperm[o] = i;
+    nb_p[i + 1] = node_blk[o + 1] - node_blk[o];
+}
+
+// in-place inclusive scan of a[0 .. n) in one workgroup: each thread sums a
+// contiguous segment, the segment sums are scanned in LDS, then each thread
+// rewrites its segment as running sums (integers: exact)
+constexpr int SC_WG = 1024;
+__global__ __launch_bounds__(SC_WG) void k_scan1(int n, int *__restrict__ a)
+{
+    __shared__ int s_sum[SC_WG];
+    const int t = threadIdx.x, seg = (n + SC_WG - 1) / SC_WG;
+    const int j0 = min(n, t * seg), j1 = min(n, j0 + seg);
+    int sum = 0;
+    for (int j = j0; j < j1; ++j) sum += a[j];
+    s_sum[t] = sum;
+    __syncthreads();
+    for (int d = 1; d < SC_WG; d <<= 1) {  // Hillis-Steele over the segment sums
+        const int v = t >= d ? s_sum[t - d] : 0;
+        __syncthreads();
+        s_sum[t] += v;
+        __syncthreads();
+    }
+    int run = s_sum[t] - sum;  // exclusive prefix of this segment
+    for (int j = j0; j < j1; ++j) {
+        run += a[j];
+        a[j] = run;
+    }
+}
+
+// per aggregate a: the centroid of its nodes (a fixed-order sum) and the
+// offsets d_i = x_i - centroid in FP32, as the kernels apply Z
+constexpr int CD_WG = 256;
+__global__ __launch_bounds__(CD_WG) void k_co_offsets(int nf, int L, const int *__restrict__ perm,
+                                                      const double *__restrict__ xf, float *__restrict__ d)
+{
+    __shared__ double s_c[3][CD_WG / 64];
+    const int a = blockIdx.x, i0 = a * L, i1 = min(nf, i0 + L), t = threadIdx.x;
+    double c3[3] = {0.0, 0.0, 0.0};
+    for (int i = i0 + t; i < i1; i += CD_WG)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) c3[c] += xf[3 * (size_t)perm[i] + c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double v = c3[c];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if ((t & 63) == 0) s_c[c][t >> 6] = v;
+    }
+    __syncthreads();
+    double cen[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double v = 0.0;
+        for (int w = 0; w < CD_WG / 64; ++w) v += s_c[c][w];
+        cen[c] = v / max(i1 - i0, 1);
+    }
+    for (int i = i0 + t; i < i1; i += CD_WG)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) d[3 * (size_t)i + c] = (float)(xf[3 * (size_t)perm[i] + c] - cen[c]);
+}
+
+// Block lists of A_c for aggregate b (one workgroup; the blocks of b's nodes
+// are [q0, q1) of the renumbered pattern): the coupled aggregates a
+// ("slots", ascending), the blocks grouped by the slot of their column's
+// aggregate -- in node and block order within a slot, written to b's own
+// range of sl_blk as (node << 8) | slot in the node's row -- and the slot
+// lists cut into at most CA_MAXCH chunks (chunk size 64 or more, doubled
+// until the chunks fit).  lst[b] = (slots, chunks, largest |a - b|); slots >
+// CA_MAXCH means the lists do not fit (the host reports it).
+constexpr int CL_WG = 1024;
+constexpr int CL_NAGG = CG_NB / CG_AGG;  // aggregates at most
+__global__ __launch_bounds__(CL_WG) void k_co_lists(int nf, int L, int nagg, const int *__restrict__ nb_p,
+                                                    const int *__restrict__ bcol_p, int *__restrict__ sl_blk,
+                                                    int *__restrict__ sl_part, int *__restrict__ sl_cptr,
+                                                    int *__restrict__ ch_beg, int4 *__restrict__ lst)
+{
+    __shared__ int s_flag[CL_NAGG], s_slot[CL_NAGG];
+    __shared__ int s_cnt[CA_MAXCH], s_base[CA_MAXCH + 1], s_run[CA_MAXCH];
+    __shared__ int s_wc[CL_WG / 64][CA_MAXCH];
+    __shared__ int s_ns, s_dmax;
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int i0 = b * L, i1 = min(nf, i0 + L), q0 = nb_p[i0], q1 = nb_p[i1];
+    for (int t = tid; t < CL_NAGG; t += CL_WG) s_flag[t] = 0;
+    for (int t = tid; t < CA_MAXCH; t += CL_WG) s_cnt[t] = 0;
+    __syncthreads();
+    for (int q = q0 + tid; q < q1; q += CL_WG) s_flag[bcol_p[q] / L] = 1;
+    __syncthreads();
+    if (wv == 0) {  // slots: ranks of the flagged aggregates, CL_NAGG / 64 per lane
+        constexpr int U = CL_NAGG / 64;
+        int f[U], c = 0, dmax = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int a = U * lane + u;
+            f[u] = a < nagg ? s_flag[a] : 0;
+            c += f[u];
+            if (f[u]) dmax = max(dmax, abs(a - b));
+        }
+        int incl = c;
+        for (int dd = 1; dd < 64; dd <<= 1) {
+            const int y = __shfl_up(incl, dd);
+            if (lane >= dd) incl += y;
+        }
+        for (int o = 32; o > 0; o >>= 1) dmax = max(dmax, __shfl_xor(dmax, o));
+        int base = incl - c;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int a = U * lane + u;
+            s_slot[a] = f[u] ? base : -1;
+            if (f[u]) {
+                if (base < CA_MAXCH) sl_part[b * CA_MAXCH + base] = a;
+                ++base;
+            }
+        }
+        if (lane == 63) {
+            s_ns = incl;
+            s_dmax = dmax;
+        }
+    }
+    __syncthreads();
+    const int ns = s_ns;
+    if (ns > CA_MAXCH) {
+        if (tid == 0) lst[b] = make_int4(ns, 0, s_dmax, 0);
+        return;
+    }
+    for (int q = q0 + tid; q < q1; q += CL_WG) atomicAdd(&s_cnt[s_slot[bcol_p[q] / L]], 1);  // counts: order-free
+    __syncthreads();
+    if (tid == 0) {  // slot starts, chunks (ns <= CA_MAXCH: a short serial loop)
+        int run = 0;
+        for (int t = 0; t < ns; ++t) {
+            s_base[t] = run;
+            s_run[t] = run;
+            run += s_cnt[t];
+        }
+        s_base[ns] = run;
+        int cs = max(64, (q1 - q0 + CA_MAXCH - 1) / CA_MAXCH);
+        for (;; cs *= 2) {
+            int nch = 0;
+            for (int t = 0; t < ns; ++t) nch += (s_cnt[t] + cs - 1) / cs;
+            if (nch <= CA_MAXCH) break;
+        }
+        const int cb = b * (CA_MAXCH + 1);
+        int c = 0;
+        for (int t = 0; t < ns; ++t) {
+            sl_cptr[cb + t] = c;
+            for (int e = s_base[t]; e < s_base[t + 1]; e += cs) ch_beg[cb + c++] = q0 + e;
+        }
+        sl_cptr[cb + ns] = c;
+        ch_beg[cb + c] = q1;
+        lst[b] = make_int4(ns, c, s_dmax, 0);
+    }
+    // stable scatter, CL_WG consecutive blocks at a time: a block's place in
+    // its slot = the slot's running base + the same-slot blocks of the
+    // earlier waves + those of the earlier lanes of its own wave
+    for (int c0 = q0; c0 < q1; c0 += CL_WG) {
+        for (int t = tid; t < (CL_WG / 64) * ns; t += CL_WG) s_wc[t / ns][t % ns] = 0;
+        __syncthreads();
+        const int q = c0 + tid;
+        const bool act = q < q1;
+        const int sl = act ? s_slot[bcol_p[q] / L] : -1;
+        int rank = 0;
+        unsigned long long todo = __ballot(act);
+        while (todo) {  // one pass per distinct slot of the wave
+            const int leader = __ffsll((long long)todo) - 1;
+            const int s1 = __shfl(sl, leader);
+            const unsigned long long m = __ballot(act && sl == s1);
+            if (act && sl == s1) rank = __popcll(m & ((1ull << lane) - 1ull));
+            if (lane == leader) s_wc[wv][s1] = __popcll(m);
+            todo &= ~m;
+        }
+        __syncthreads();
+        if (act) {
+            int pos = s_run[sl] + rank;
+            for (int w2 = 0; w2 < wv; ++w2) pos += s_wc[w2][sl];
+            int lo = i0, hi = i1 - 1;  // row node of block q: nb_p[r] <= q < nb_p[r + 1]
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (nb_p[mid] <= q) lo = mid;
+                else hi = mid - 1;
+            }
+            sl_blk[q0 + pos] = (lo << 8) | (q - nb_p[lo]);
+        }
+        __syncthreads();
+        for (int t = tid; t < ns; t += CL_WG) {
+            int add = 0;
+            for (int w2 = 0; w2 < CL_WG / 64; ++w2) add += s_wc[w2][t];
+            s_run[t] += add;
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(1024) void k_coarse_ac(int nc, const int *__restrict__ nb_p,
                                                     const int *__restrict__ bcol_p, const double *__restrict__ val_p,
-                                                    const float *__restrict__ d, const int *__restrict__ sl_ptr,
+                                                    const float *__restrict__ d, const int4 *__restrict__ lst,
                                                     const int *__restrict__ sl_part, const int *__restrict__ sl_cptr,
-                                                    const int *__restrict__ ch_ptr, const int *__restrict__ ch_beg,
-                                                    const int *__restrict__ sl_blk, double *__restrict__ A)
+                                                    const int *__restrict__ ch_beg, const int *__restrict__ sl_blk,
+                                                    double *__restrict__ A)
 {
     __shared__ double s_part[CA_MAXCH * 36];
     const int b = blockIdx.x;
-    const int c0 = ch_ptr[b], nch = ch_ptr[b + 1] - c0;
+    const int4 ls = lst[b];  // slots, chunks
+    const int cb = b * (CA_MAXCH + 1), nch = ls.y;
     for (int t = threadIdx.x; t < 36 * nch; t += blockDim.x) {
-        const int ch = c0 + t / 36, pq = t % 36, p = pq / 6, q = pq % 6;
+        const int ch = cb + t / 36, pq = t % 36, p = pq / 6, q = pq % 6;
         double acc = 0.0;
         for (int e = ch_beg[ch]; e < ch_beg[ch + 1]; ++e) {
             const int rk = sl_blk[e];  // (node << 8) | slot in the node's row
@@ -1798,12 +2007,11 @@ __global__ __launch_bounds__(1024) void k_coarse_ac(int nc, const int *__restric
         s_part[t] = acc;
     }
     __syncthreads();
-    const int s0 = sl_ptr[b], ns = sl_ptr[b + 1] - s0;
-    for (int t = threadIdx.x; t < 36 * ns; t += blockDim.x) {
-        const int sl = s0 + t / 36, pq = t % 36;
+    for (int t = threadIdx.x; t < 36 * ls.x; t += blockDim.x) {
+        const int sl = t / 36, pq = t % 36;
         double acc = 0.0;
-        for (int ch = sl_cptr[sl]; ch < sl_cptr[sl + 1]; ++ch) acc += s_part[(ch - c0) * 36 + pq];
-        A[(size_t)(6 * sl_part[sl] + pq / 6) * nc + 6 * b + pq % 6] = acc;
+        for (int ch = sl_cptr[cb + sl]; ch < sl_cptr[cb + sl + 1]; ++ch) acc += s_part[ch * 36 + pq];
+        A[(size_t)(6 * sl_part[b * CA_MAXCH + sl] + pq / 6) * nc + 6 * b + pq % 6] = acc;
     }
 }
 
@@ -1982,15 +2190,18 @@ __global__ __launch_bounds__(CH_WG) void k_band_cholesky(int nc, int bw, double 
 }
 
 // The same banded Cholesky split over kernels, one panel of CP_NB = 16
-// columns at a time (32-column panels were slower: the panel's serial part
-// grows with the square of the width): k_chol_lift lifts the diagonal (as
-// above) and clears the flag;
-// k_chol_panel (one workgroup) factors the panel's diagonal block and the
-// band rows below it in LDS exactly as k_band_cholesky does and writes them
-// back; k_chol_update applies the panel to the trailing band in 32 x 32
-// tiles of its lower triangle, one workgroup each, every entry updated once
-// with the sum over the panel's columns in ascending order.  Every kernel
-// returns at once when an earlier pivot failed (*flag != 0).
+// columns per step (32-column panels were slower: the panel's serial part
+// grows with the square of the width).  k_chol_lift lifts the diagonal (as
+// above) and clears the flag.  Step j (k_chol_step, first column j0) works
+// with look-ahead: workgroup 0 applies panel j - 1 to the columns of panel j
+// and factors panel j -- its diagonal block and the band rows below it, in
+// LDS exactly as k_band_cholesky does -- while the other workgroups apply
+// panel j - 1 to the rest of its trailing window (columns from j0 + CP_NB on)
+// in 64 x 64 tiles of the lower triangle.  So the critical path per panel is
+// one kernel, and the trailing update overlaps the next panel.  Every entry
+// receives each panel's update once, as the sum over the panel's columns in
+// ascending order, and the panels in order.  Every kernel returns at once
+// when an earlier pivot failed (*flag != 0).
 __global__ __launch_bounds__(CH_WG) void k_chol_lift(int nc, double *__restrict__ A, int *__restrict__ flag)
 {
     __shared__ double s_red[CH_WG / 64];
@@ -2009,28 +2220,98 @@ __global__ __launch_bounds__(CH_WG) void k_chol_lift(int nc, double *__restrict_
 
 constexpr int CP_NB = 16;          // panel width of the split factorization
 constexpr int CP_LD = CP_NB + 1;    // odd LDS row stride
-__global__ __launch_bounds__(CH_WG) void k_chol_panel(int nc, int bw, int j0, double *__restrict__ A,
-                                                      int *__restrict__ flag)
+constexpr int CS_T = 64;           // trailing-update tile (32 x 32 threads, 2 x 2 entries each)
+static_assert(2 * CS_T <= CH_ROWS, "tile rows in the panel buffer");
+__global__ __launch_bounds__(CH_WG) void k_chol_step(int nc, int bw, int j0, double *__restrict__ A,
+                                                     int *__restrict__ flag)
 {
-    constexpr int CH_NB = CP_NB, CH_LD = CP_LD;  // this kernel's panel (LDS: CH_ROWS x CP_LD doubles)
-    __shared__ double s_p[CH_ROWS * CH_LD];
+    constexpr int NB = CP_NB, LD = CP_LD;
+    __shared__ double s_p[CH_ROWS * LD];  // workgroup 0: panel j; the others: two row blocks of panel j - 1
+    __shared__ double s_q[NB][LD];       // workgroup 0: rows j0 .. j0 + NB - 1 of panel j - 1
+    __shared__ double s_rd[NB];          // reciprocals of the panel's pivots
     __shared__ int s_bad;
     const int tid = threadIdx.x;
     if (*flag) return;
-    if (tid == 0) s_bad = 0;
-    const int nb = min(CH_NB, nc - j0);
-    const int H = min(nc, j0 + nb + bw) - j0;
-    for (int t = tid; t < H * CH_NB; t += CH_WG) {
-        const int r = t / CH_NB, c = t % CH_NB;
-        s_p[r * CH_LD + c] = (r >= c && c < nb) ? A[(size_t)(j0 + r) * nc + j0 + c] : 0.0;
+    const int jp = j0 - NB;  // first column of panel j - 1 (< 0: none)
+    if (blockIdx.x > 0) {    // trailing tiles of panel j - 1, columns [j0 + NB, w1)
+        if (jp < 0) return;
+        const int w0 = j0 + NB, w1 = min(nc, j0 + bw);
+        const int m = w1 - w0;
+        if (m <= 0) return;
+        const int nt = (m + CS_T - 1) / CS_T, tix = (int)blockIdx.x - 1;
+        int ta = (int)((sqrt(8.0 * tix + 1.0) - 1.0) * 0.5);
+        while ((ta + 1) * (ta + 2) / 2 <= tix) ++ta;
+        while (ta * (ta + 1) / 2 > tix) --ta;
+        const int tb = tix - ta * (ta + 1) / 2;
+        if (ta >= nt) return;  // the grid is sized for the widest window
+        const int i0 = w0 + ta * CS_T, k0 = w0 + tb * CS_T;
+        double *s_i = s_p, *s_k = s_p + CS_T * LD;
+        for (int t = tid; t < CS_T * NB; t += CH_WG) {
+            const int r = t / NB, c = t % NB;
+            s_i[r * LD + c] = i0 + r < w1 ? A[(size_t)(i0 + r) * nc + jp + c] : 0.0;
+            s_k[r * LD + c] = k0 + r < w1 ? A[(size_t)(k0 + r) * nc + jp + c] : 0.0;
+        }
+        __syncthreads();
+        const int tx = tid & 31, ty = tid >> 5;
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int a = 2 * ty + p, bq = 2 * tx + q;
+                const int i = i0 + a, k = k0 + bq;
+                if (i < w1 && k < w1 && k <= i) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) acc += s_i[a * LD + c] * s_k[bq * LD + c];
+                    A[(size_t)i * nc + k] -= acc;
+                }
+            }
+        return;
     }
+    if (tid == 0) s_bad = 0;
+    const int nb = min(NB, nc - j0);
+    const int H = min(nc, j0 + nb + bw) - j0;  // panel rows (<= CH_ROWS)
+    // panel j - 1 reaches rows j0 .. j0 + hq - 1; thread t takes half of row
+    // t / 2, its row of panel j - 1 loaded with the panel
+    const int hq = jp >= 0 ? min(nc, j0 + bw) - j0 : 0;
+    static_assert(2 * (CH_ROWS - CP_NB) <= CH_WG, "one row half per thread");
+    const bool upd = tid < 2 * hq;
+    double lr[NB];
+    if (upd)
+#pragma unroll
+        for (int c = 0; c < NB; ++c) lr[c] = A[(size_t)(j0 + (tid >> 1)) * nc + jp + c];
+    for (int t = tid; t < H * NB; t += CH_WG) {
+        const int r = t / NB, c = t % NB;
+        s_p[r * LD + c] = (r >= c && c < nb) ? A[(size_t)(j0 + r) * nc + j0 + c] : 0.0;
+    }
+    if (jp >= 0)
+        for (int t = tid; t < NB * NB; t += CH_WG) {
+            const int r = t / NB, c = t % NB;
+            s_q[r][c] = j0 + r < nc ? A[(size_t)(j0 + r) * nc + jp + c] : 0.0;
+        }
     __syncthreads();
-    if (tid < 64) {
-        double v[CH_NB];
+    if (jp >= 0) {  // panel j - 1 on the columns of panel j
+        if (upd) {
+            const int r = tid >> 1, c0 = (tid & 1) * (NB / 2);
 #pragma unroll
-        for (int c = 0; c < CH_NB; ++c) v[c] = (tid < nb && c <= tid) ? s_p[tid * CH_LD + c] : 0.0;
+            for (int cc = 0; cc < NB / 2; ++cc) {
+                const int c = c0 + cc;
+                if (c <= r && c < nb) {
+                    double acc = 0.0;
 #pragma unroll
-        for (int c = 0; c < CH_NB; ++c) {
+                    for (int c2 = 0; c2 < NB; ++c2) acc += lr[c2] * s_q[c][c2];
+                    s_p[r * LD + c] -= acc;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < 64) {  // the nb x nb diagonal block by wave 0, lane l owning row l
+        double v[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) v[c] = (tid < nb && c <= tid) ? s_p[tid * LD + c] : 0.0;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
             if (c < nb) {
                 const double dc = read_lane(v[c], c);
                 const double piv = dc > 0.0 ? sqrt(dc) : 1.0;
@@ -2038,7 +2319,7 @@ __global__ __launch_bounds__(CH_WG) void k_chol_panel(int nc, int bw, int j0, do
                 if (tid == c) v[c] = piv;
                 if (tid > c) v[c] /= piv;
 #pragma unroll
-                for (int c2 = c + 1; c2 < CH_NB; ++c2) {
+                for (int c2 = c + 1; c2 < NB; ++c2) {
                     const double lc2 = read_lane(v[c], c2);
                     if (tid >= c2 && c2 < nb) v[c2] -= v[c] * lc2;
                 }
@@ -2046,79 +2327,37 @@ __global__ __launch_bounds__(CH_WG) void k_chol_panel(int nc, int bw, int j0, do
         }
         if (tid < nb)
 #pragma unroll
-            for (int c = 0; c < CH_NB; ++c)
-                if (c <= tid) s_p[tid * CH_LD + c] = v[c];
+            for (int c = 0; c < NB; ++c)
+                if (c <= tid) s_p[tid * LD + c] = v[c];
     }
     __syncthreads();
     if (s_bad) {
         if (tid == 0) *flag = s_bad;
         return;
     }
-    __shared__ double s_rd[CH_NB];  // reciprocals of the panel's pivots
-    if (tid < nb) s_rd[tid] = 1.0 / s_p[tid * CH_LD + tid];
+    if (tid < nb) s_rd[tid] = 1.0 / s_p[tid * LD + tid];
     __syncthreads();
-    for (int r = nb + tid; r < H; r += CH_WG) {
-        double x[CH_NB];
+    for (int r = nb + tid; r < H; r += CH_WG) {  // rows below the diagonal block: x L_diag^T = a
+        double x[NB];
 #pragma unroll
-        for (int c = 0; c < CH_NB; ++c) x[c] = s_p[r * CH_LD + c];
+        for (int c = 0; c < NB; ++c) x[c] = s_p[r * LD + c];
 #pragma unroll
-        for (int c = 0; c < CH_NB; ++c) {
+        for (int c = 0; c < NB; ++c) {
             if (c < nb) {
                 double t = x[c];
 #pragma unroll
-                for (int c2 = 0; c2 < c; ++c2) t -= x[c2] * s_p[c * CH_LD + c2];
+                for (int c2 = 0; c2 < c; ++c2) t -= x[c2] * s_p[c * LD + c2];
                 x[c] = t * s_rd[c];
             }
         }
 #pragma unroll
-        for (int c = 0; c < CH_NB; ++c) s_p[r * CH_LD + c] = x[c];
+        for (int c = 0; c < NB; ++c) s_p[r * LD + c] = x[c];
     }
     __syncthreads();
-    for (int t = tid; t < H * CH_NB; t += CH_WG) {
-        const int r = t / CH_NB, c = t % CH_NB;
-        if (r >= c && c < nb) A[(size_t)(j0 + r) * nc + j0 + c] = s_p[r * CH_LD + c];
+    for (int t = tid; t < H * NB; t += CH_WG) {
+        const int r = t / NB, c = t % NB;
+        if (r >= c && c < nb) A[(size_t)(j0 + r) * nc + j0 + c] = s_p[r * LD + c];
     }
-}
-
-constexpr int CU_T = 32;  // trailing-update tile
-__global__ __launch_bounds__(256) void k_chol_update(int nc, int bw, int j0, double *__restrict__ A,
-                                                     const int *__restrict__ flag)
-{
-    constexpr int CH_NB = CP_NB;
-    __shared__ double s_i[CU_T][CH_NB + 1], s_k[CU_T][CH_NB + 1];
-    if (*flag) return;
-    const int nb = min(CH_NB, nc - j0);
-    const int w0 = j0 + nb, w1 = min(nc, j0 + nb + bw);  // trailing window rows/columns [w0, w1)
-    const int m = w1 - w0;
-    if (m <= 0) return;
-    const int nt = (m + CU_T - 1) / CU_T;
-    int ta = (int)((sqrt(8.0 * blockIdx.x + 1.0) - 1.0) * 0.5);
-    while ((ta + 1) * (ta + 2) / 2 <= (int)blockIdx.x) ++ta;
-    while (ta * (ta + 1) / 2 > (int)blockIdx.x) --ta;
-    const int tb = (int)blockIdx.x - ta * (ta + 1) / 2;
-    if (ta >= nt) return;  // the grid is sized for the widest window
-    const int i0 = w0 + ta * CU_T, k0 = w0 + tb * CU_T;
-    for (int t = threadIdx.x; t < CU_T * CH_NB; t += 256) {
-        const int r = t / CH_NB, c = t % CH_NB;
-        const int ri = i0 + r, rk = k0 + r;
-        s_i[r][c] = (ri < w1 && c < nb) ? A[(size_t)ri * nc + j0 + c] : 0.0;
-        s_k[r][c] = (rk < w1 && c < nb) ? A[(size_t)rk * nc + j0 + c] : 0.0;
-    }
-    __syncthreads();
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-#pragma unroll
-    for (int p = 0; p < 2; ++p)
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int a = 2 * ty + p, b = 2 * tx + q;
-            const int i = i0 + a, k = k0 + b;
-            if (i < w1 && k < w1 && k <= i) {
-                double acc = 0.0;
-#pragma unroll
-                for (int c = 0; c < CH_NB; ++c) acc += s_i[a][c] * s_k[b][c];
-                A[(size_t)i * nc + k] -= acc;
-            }
-        }
 }
 
 // the band below the diagonal := mean of both triangles (A_c is symmetric up
@@ -2434,189 +2673,131 @@ static int coarse_structure(ofea_ctx *ctx)
     const int ra = std::max(CG_AGG, (64 + per - 1) / per);
     const int nagg = (nranges + ra - 1) / ra;
     const int L = per * ra;
-    // free-node coordinates, the new order and the aggregate centroids
-    std::vector<double> X((size_t)3 * nf), cen((size_t)3 * nagg, 0.0);
-    std::vector<int> cnt(nagg, 0);
+    const int nblk = ctx->P.node_blk[nf];
+    if (nf >= (1 << 23)) return fail(OFEA_ERR_LIMIT, "too many free nodes for the packed coarse block lists");
+    static_assert(CL_NAGG == CG_NB / CG_AGG, "aggregate bound");
+    // free-node coordinates and their bounding box
+    std::vector<double> X((size_t)3 * nf);
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int i = 0; i < ctx->nd; ++i) {
         const int r = ctx->resort[i];
         if (r < 0) continue;
-        for (int c = 0; c < 3; ++c) X[3 * (size_t)r + c] = ctx->xyz[3 * (size_t)i + c];
-    }
-    std::vector<int> perm, iperm(nf);
-    coarse_order(nf, X.data(), L, perm);
-    for (int i = 0; i < nf; ++i) iperm[perm[i]] = i;
-    for (int i = 0; i < nf; ++i) {
-        const int a = i / L;
-        cnt[a]++;
-        for (int c = 0; c < 3; ++c) cen[3 * a + c] += X[3 * (size_t)perm[i] + c];
-    }
-    for (int a = 0; a < nagg; ++a)
-        for (int c = 0; c < 3; ++c) cen[3 * a + c] /= std::max(cnt[a], 1);
-    std::vector<float> d((size_t)3 * nf);  // new order, FP32 as the device applies Z
-    for (int i = 0; i < nf; ++i)
-        for (int c = 0; c < 3; ++c) d[3 * (size_t)i + c] = (float)(X[3 * (size_t)perm[i] + c] - cen[3 * (i / L) + c]);
-    // renumbered pattern: node rows in the new order, each keeping its block order
-    // (bcol_p itself is built on the device by k_perm_bcol)
-    const int nblk = ctx->P.node_blk[nf];
-    if (nf >= (1 << 23)) return fail(OFEA_ERR_LIMIT, "too many free nodes for the packed coarse block lists");
-    std::vector<int> nb_p((size_t)nf + 1, 0);
-    for (int i = 0; i < nf; ++i) nb_p[i + 1] = nb_p[i] + ctx->P.node_blk[perm[i] + 1] - ctx->P.node_blk[perm[i]];
-    // per aggregate b: coupled aggregates a (ascending, "slots") and the
-    // blocks (r, k) of b's nodes with bcol in a, in node and block order,
-    // stored as (node, slot in its row) pairs.  b's blocks fill
-    // [nb_p[b L], nb_p[(b + 1) L]) of the list, so host threads over ranges of
-    // aggregates write disjoint parts; their slot and chunk lists are joined
-    // in aggregate order afterwards.
-    std::vector<int> sl_blk((size_t)std::max(nblk, 1));  // (node << 8) | slot in the node's row
-    struct AggLists {
-        std::vector<int> part, cptr, ch_beg, nslot, nchunk;  // cptr: into this range's ch_beg
-        int bw = 5;
-        bool too_many = false;
-    };
-    const int nthr = std::max(1, std::min({8, nagg, (int)std::max(1u, std::thread::hardware_concurrency())}));
-    std::vector<AggLists> lists(nthr);
-    auto build_range = [&](int th) {
-        AggLists &o = lists[th];
-        const int b_lo = (int)((long long)nagg * th / nthr), b_hi = (int)((long long)nagg * (th + 1) / nthr);
-        std::vector<int> slot_of(nagg, -1), scount, bstart, bp;
-        for (int b = b_lo; b < b_hi; ++b) {
-            const int i0 = b * L, i1 = std::min(nf, i0 + L);
-            const int q0 = nb_p[i0], q1 = nb_p[i1];
-            bp.resize(q1 - q0);  // aggregate of each block's row node
-            for (int i = i0; i < i1; ++i) {
-                const int o0 = ctx->P.node_blk[perm[i]];
-                for (int q = nb_p[i]; q < nb_p[i + 1]; ++q) bp[q - q0] = iperm[ctx->P.bcol[(size_t)o0 + q - nb_p[i]]] / L;
-            }
-            const int first = (int)o.part.size();
-            for (int q = 0; q < q1 - q0; ++q)
-                if (slot_of[bp[q]] < 0) {
-                    slot_of[bp[q]] = 0;
-                    o.part.push_back(bp[q]);
-                }
-            std::sort(o.part.begin() + first, o.part.end());
-            const int ns = (int)o.part.size() - first;
-            for (int t = 0; t < ns; ++t) slot_of[o.part[first + t]] = t;
-            scount.assign(ns + 1, 0);
-            for (int q = 0; q < q1 - q0; ++q) scount[slot_of[bp[q]] + 1]++;
-            for (int t = 0; t < ns; ++t) scount[t + 1] += scount[t];
-            bstart = scount;  // slot t: [q0 + bstart[t], q0 + bstart[t + 1])
-            for (int i = i0; i < i1; ++i)
-                for (int q = nb_p[i]; q < nb_p[i + 1]; ++q) {
-                    const size_t pos = (size_t)q0 + scount[slot_of[bp[q - q0]]]++;
-                    sl_blk[pos] = (i << 8) | (q - nb_p[i]);
-                }
-            for (int t = 0; t < ns; ++t) {
-                const int a = o.part[first + t];
-                o.bw = std::max(o.bw, 6 * std::abs(a - b) + 5);
-                slot_of[a] = -1;
-            }
-            if (ns > CA_MAXCH) {
-                o.too_many = true;
-                return;
-            }
-            // chunks of the slot lists: at most CA_MAXCH per aggregate
-            const int nbl = q1 - q0, ch0 = (int)o.ch_beg.size();
-            for (int cs = std::max(64, (nbl + CA_MAXCH - 1) / CA_MAXCH);; cs *= 2) {
-                int nch = 0;
-                for (int t = 0; t < ns; ++t) nch += (bstart[t + 1] - bstart[t] + cs - 1) / cs;
-                if (nch > CA_MAXCH) continue;
-                for (int t = 0; t < ns; ++t) {
-                    o.cptr.push_back((int)o.ch_beg.size());
-                    for (int e = q0 + bstart[t]; e < q0 + bstart[t + 1]; e += cs) o.ch_beg.push_back(e);
-                }
-                break;
-            }
-            o.nslot.push_back(ns);
-            o.nchunk.push_back((int)o.ch_beg.size() - ch0);
+        for (int c = 0; c < 3; ++c) {
+            const double x = ctx->xyz[3 * (size_t)i + c];
+            X[3 * (size_t)r + c] = x;
+            lo[c] = std::min(lo[c], x);
+            hi[c] = std::max(hi[c], x);
         }
-    };
-    {
-        std::vector<std::thread> pool;
-        for (int th = 1; th < nthr; ++th) pool.emplace_back(build_range, th);
-        build_range(0);
-        for (auto &t : pool) t.join();
     }
-    std::vector<int> sl_ptr(1, 0), sl_part, sl_cptr, ch_ptr(1, 0), ch_beg;
-    int bw = 5;
-    for (const AggLists &o : lists) {
-        if (o.too_many) return fail(OFEA_ERR_LIMIT, "an aggregate couples to too many aggregates");
-        bw = std::max(bw, o.bw);
-        const int coff = (int)ch_beg.size();
-        for (int ns : o.nslot) sl_ptr.push_back(sl_ptr.back() + ns);
-        for (int nch : o.nchunk) ch_ptr.push_back(ch_ptr.back() + nch);
-        sl_part.insert(sl_part.end(), o.part.begin(), o.part.end());
-        for (int c : o.cptr) sl_cptr.push_back(coff + c);
-        ch_beg.insert(ch_beg.end(), o.ch_beg.begin(), o.ch_beg.end());
+    // The order (host): parts of exactly L nodes (the last may be shorter),
+    // as slabs of whole parts across the longest axis of the bounding box --
+    // about as thick as a cubic part, so A_c stays banded (a part couples only
+    // to parts of its own and the neighbouring slabs) -- each slab cut into
+    // its parts by recursive coordinate bisection on its own host thread.
+    // perm[new] = old.
+    const int nparts = (nf + L - 1) / L;
+    int ax = 0;
+    for (int c = 1; c < 3; ++c)
+        if (hi[c] - lo[c] > hi[ax] - lo[ax]) ax = c;
+    const double emax = std::max(hi[ax] - lo[ax], 0.0);
+    double vol = 1.0;
+    for (int c = 0; c < 3; ++c) vol *= std::max(hi[c] - lo[c], 1e-3 * emax);
+    const double edge = std::cbrt(vol / std::max(nparts, 1));
+    const int slabs = std::max(1, std::min(nparts, (int)std::lround(emax / std::max(edge, 1e-300))));
+    const long long slab_nodes = (long long)((nparts + slabs - 1) / slabs) * L;
+    const int nslab = nf ? (int)((nf + slab_nodes - 1) / slab_nodes) : 0;
+    std::vector<int> perm(nf), iperm(nf);
+    if (nparts <= 1) {
+        for (int i = 0; i < nf; ++i) perm[i] = i;
+    } else {  // slab-major, ascending index inside each slab, then the parts of each slab
+        std::vector<int> slab;
+        slab_of_nodes(nf, X.data(), ax, lo[ax], hi[ax], slab_nodes, slab);
+        std::vector<long long> pos(nslab + 1, 0);
+        for (int i = 0; i < nf; ++i) pos[slab[i] + 1]++;
+        for (int sl = 0; sl < nslab; ++sl) pos[sl + 1] += pos[sl];
+        for (int i = 0; i < nf; ++i) perm[pos[slab[i]]++] = i;
+        const int nthr = nf < 16384 ? 1 : host_threads(nslab);
+        std::vector<std::vector<AxisKey>> keys(nthr);
+        run_threads(nthr, [&](int th) {
+            for (int sl = th; sl < nslab; sl += nthr) {
+                const long long s0 = sl * slab_nodes;
+                rcb_parts(perm.data() + s0, (int)std::min<long long>(slab_nodes, nf - s0), L, X.data(), keys[th]);
+            }
+        });
     }
-    sl_cptr.push_back((int)ch_beg.size());
-    ch_beg.push_back(nblk);
-    const int nc = 6 * nagg;
-    bw = std::min(bw, nc - 1);
-    // device buffers (sizes depend only on nfree, nnz and nagg <= CG_NB / CG_AGG)
+    for (int i = 0; i < nf; ++i) iperm[perm[i]] = i;
+    // device buffers (sizes depend only on nfree, nnz and the bounds)
     const size_t nfb = (size_t)std::max(nf, 1);
     const long long nnz = ctx->P.nnz;
     static_assert(CG_NC_MAX == 6 * (CG_NB / CG_AGG), "coarse dimension bound");
-    if (!w->co.g) {
-        OFEA_HIP(hipMalloc((void **)&w->co.g, sizeof(double) * 6 * CG_NB));
-        OFEA_HIP(hipMalloc((void **)&w->co.d, sizeof(float) * 3 * nfb));
-        OFEA_HIP(hipMalloc((void **)&w->co_ainv, sizeof(float) * CG_NC_MAX * CG_NC_MAX));
-        OFEA_HIP(hipMalloc((void **)&w->co.zq, sizeof(double) * 6 * CG_NB));
-        OFEA_HIP(hipMalloc((void **)&w->co.ya, sizeof(double) * 2 * CG_NC_MAX));
-        OFEA_HIP(hipMalloc((void **)&w->co_A, sizeof(double) * CG_NC_MAX * CG_NC_MAX));
-        OFEA_HIP(hipMalloc((void **)&w->co_inv, sizeof(double) * CG_NC_MAX * CG_NC_MAX));
-        OFEA_HIP(hipMalloc((void **)&w->co_flag, sizeof(int)));
-        OFEA_HIP(hipMalloc((void **)&w->perm, sizeof(int) * nfb));
-        OFEA_HIP(hipMalloc((void **)&w->iperm, sizeof(int) * nfb));
-        OFEA_HIP(hipMalloc((void **)&w->nb_p, sizeof(int) * (nfb + 1)));
-        OFEA_HIP(hipMalloc((void **)&w->bcol_p, sizeof(int) * (size_t)std::max(nblk, 1)));
-        OFEA_HIP(hipMalloc((void **)&w->val_p, sizeof(double) * (size_t)std::max(nnz, 1LL)));
-        OFEA_HIP(hipMalloc((void **)&w->val32_p, sizeof(float) * (size_t)std::max(nnz, 1LL)));
-        OFEA_HIP(hipMalloc((void **)&w->dinv_p, sizeof(double) * 3 * nfb));
-        OFEA_HIP(hipMalloc((void **)&w->b_p, sizeof(double) * 3 * nfb));
-        OFEA_HIP(hipMalloc((void **)&w->x_p, sizeof(double) * 3 * nfb));
-        for (double **p : {&w->minv_p, &w->dblk_p})
-            OFEA_HIP(hipMalloc((void **)p, sizeof(double) * 9 * nfb));
-        OFEA_HIP(hipMalloc((void **)&w->minv32_p, sizeof(float) * 9 * nfb));
-        OFEA_HIP(hipMalloc((void **)&w->sl_ptr, sizeof(int) * (CG_NB / CG_AGG + 1)));
-        OFEA_HIP(hipMalloc((void **)&w->ch_ptr, sizeof(int) * (CG_NB / CG_AGG + 1)));
-        OFEA_HIP(hipMalloc((void **)&w->sl_blk, sizeof(int) * (size_t)std::max(nblk, 1)));
+    if (!w->co_arena) {  // one allocation holds every two-level buffer
+        const size_t nbk = (size_t)std::max(nblk, 1), nz = (size_t)std::max(nnz, 1LL);
+        const size_t ncm2 = (size_t)CG_NC_MAX * CG_NC_MAX, lch = (size_t)CL_NAGG * (CA_MAXCH + 1);
+        struct Part {
+            void **p;
+            size_t bytes;
+        } parts[] = {
+            {(void **)&w->co.g, sizeof(double) * 6 * CG_NB},    {(void **)&w->co.zq, sizeof(double) * 6 * CG_NB},
+            {(void **)&w->co.ya, sizeof(double) * 2 * CG_NC_MAX}, {(void **)&w->co.d, sizeof(float) * 3 * nfb},
+            {(void **)&w->co_ainv, sizeof(float) * ncm2},       {(void **)&w->co_A, sizeof(double) * ncm2},
+            {(void **)&w->co_inv, sizeof(double) * ncm2},       {(void **)&w->co_flag, sizeof(int)},
+            {(void **)&w->perm, sizeof(int) * nfb},             {(void **)&w->iperm, sizeof(int) * nfb},
+            {(void **)&w->nb_p, sizeof(int) * (nfb + 1)},       {(void **)&w->bcol_p, sizeof(int) * nbk},
+            {(void **)&w->val_p, sizeof(double) * nz},          {(void **)&w->val32_p, sizeof(float) * nz},
+            {(void **)&w->dinv_p, sizeof(double) * 3 * nfb},    {(void **)&w->b_p, sizeof(double) * 3 * nfb},
+            {(void **)&w->x_p, sizeof(double) * 3 * nfb},       {(void **)&w->minv_p, sizeof(double) * 9 * nfb},
+            {(void **)&w->dblk_p, sizeof(double) * 9 * nfb},    {(void **)&w->minv32_p, sizeof(float) * 9 * nfb},
+            {(void **)&w->sl_blk, sizeof(int) * nbk},           {(void **)&w->sl_part, sizeof(int) * CL_NAGG * CA_MAXCH},
+            {(void **)&w->sl_cptr, sizeof(int) * lch},          {(void **)&w->ch_beg, sizeof(int) * lch},
+            {(void **)&w->co_lst, sizeof(int4) * CL_NAGG},      {(void **)&w->co_xf, sizeof(double) * 3 * nfb}};
+        size_t tot = 0;
+        for (const Part &q : parts) tot += (q.bytes + 255) & ~(size_t)255;
+        char *base = nullptr;
+        OFEA_HIP(hipMalloc((void **)&base, tot));
+        w->co_arena = base;
+        for (const Part &q : parts) {
+            *q.p = base;
+            base += (q.bytes + 255) & ~(size_t)255;
+        }
     }
-    for (int **p : {&w->sl_part, &w->sl_cptr, &w->ch_beg}) {
-        if (*p) (void)hipFree(*p);
-        *p = nullptr;
-    }
-    OFEA_HIP(hipMalloc((void **)&w->sl_part, sizeof(int) * std::max<size_t>(sl_part.size(), 1)));
-    OFEA_HIP(hipMalloc((void **)&w->sl_cptr, sizeof(int) * sl_cptr.size()));
-    OFEA_HIP(hipMalloc((void **)&w->ch_beg, sizeof(int) * ch_beg.size()));
+    // The rest on the device: iperm and the renumbered pattern (nb_p by a
+    // scan of the row lengths, bcol_p), the offsets d from the aggregate
+    // centroids, and the block lists of A_c.
     hipStream_t st = ctx->stream;
-    OFEA_HIP(hipMemsetAsync(w->co.g, 0, sizeof(double) * 6 * CG_NB, st));
-    OFEA_HIP(hipMemsetAsync(w->co.zq, 0, sizeof(double) * 6 * CG_NB, st));
-    OFEA_HIP(hipMemsetAsync(w->co.ya, 0, sizeof(double) * 2 * CG_NC_MAX, st));
-    OFEA_HIP(hipMemcpyAsync((void *)w->co.d, d.data(), sizeof(float) * d.size(), hipMemcpyHostToDevice, st));
+    const unsigned gz6 = (6 * CG_NB + RED_WG - 1) / RED_WG;
+    hipLaunchKernelGGL((k_fill<double>), dim3(gz6), dim3(RED_WG), 0, st, 6 * CG_NB, w->co.g, 0.0);
+    hipLaunchKernelGGL((k_fill<double>), dim3(gz6), dim3(RED_WG), 0, st, 6 * CG_NB, w->co.zq, 0.0);
+    hipLaunchKernelGGL((k_fill<double>), dim3((2 * CG_NC_MAX + RED_WG - 1) / RED_WG), dim3(RED_WG), 0, st,
+                       2 * CG_NC_MAX, w->co.ya, 0.0);
+    std::vector<int4> lst(nagg);
     if (nf) {
         OFEA_HIP(hipMemcpyAsync(w->perm, perm.data(), sizeof(int) * nf, hipMemcpyHostToDevice, st));
-        OFEA_HIP(hipMemcpyAsync(w->iperm, iperm.data(), sizeof(int) * nf, hipMemcpyHostToDevice, st));
-        OFEA_HIP(hipMemcpyAsync(w->nb_p, nb_p.data(), sizeof(int) * (nf + 1), hipMemcpyHostToDevice, st));
-        OFEA_HIP(hipMemcpyAsync(w->sl_blk, sl_blk.data(), sizeof(int) * (size_t)std::max(nblk, 1),
-                                hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_perm_bcol, dim3((nf + RED_WG - 1) / RED_WG), dim3(RED_WG), 0, st, nf, w->perm, w->iperm,
-                           ctx->d_node_blk, ctx->d_bcol, w->nb_p, w->bcol_p);
+        const unsigned gn = (unsigned)((ctx->nd + RED_WG - 1) / RED_WG), gf = (unsigned)((nf + RED_WG - 1) / RED_WG);
+        hipLaunchKernelGGL(k_co_xfree, dim3(gn), dim3(RED_WG), 0, st, ctx->nd, ctx->d_resort, ctx->d_xyz, w->co_xf);
+        hipLaunchKernelGGL(k_co_rows, dim3(gf), dim3(RED_WG), 0, st, nf, w->perm, ctx->d_node_blk, w->iperm, w->nb_p);
+        hipLaunchKernelGGL(k_scan1, dim3(1), dim3(SC_WG), 0, st, nf, w->nb_p + 1);
+        hipLaunchKernelGGL(k_co_offsets, dim3(nagg), dim3(CD_WG), 0, st, nf, L, w->perm, w->co_xf, (float *)w->co.d);
+        hipLaunchKernelGGL(k_perm_bcol, dim3(gf), dim3(RED_WG), 0, st, nf, w->perm, w->iperm, ctx->d_node_blk,
+                           ctx->d_bcol, w->nb_p, w->bcol_p);
+        hipLaunchKernelGGL(k_co_lists, dim3(nagg), dim3(CL_WG), 0, st, nf, L, nagg, w->nb_p, w->bcol_p, w->sl_blk,
+                           w->sl_part, w->sl_cptr, w->ch_beg, w->co_lst);
         OFEA_HIP(hipGetLastError());
+        OFEA_HIP(hipMemcpyAsync(lst.data(), w->co_lst, sizeof(int4) * nagg, hipMemcpyDeviceToHost, st));
     }
-    OFEA_HIP(hipMemcpyAsync(w->sl_ptr, sl_ptr.data(), sizeof(int) * sl_ptr.size(), hipMemcpyHostToDevice, st));
-    if (!sl_part.empty())
-        OFEA_HIP(hipMemcpyAsync(w->sl_part, sl_part.data(), sizeof(int) * sl_part.size(), hipMemcpyHostToDevice, st));
-    OFEA_HIP(hipMemcpyAsync(w->sl_cptr, sl_cptr.data(), sizeof(int) * sl_cptr.size(), hipMemcpyHostToDevice, st));
-    OFEA_HIP(hipMemcpyAsync(w->ch_ptr, ch_ptr.data(), sizeof(int) * ch_ptr.size(), hipMemcpyHostToDevice, st));
-    OFEA_HIP(hipMemcpyAsync(w->ch_beg, ch_beg.data(), sizeof(int) * ch_beg.size(), hipMemcpyHostToDevice, st));
-    OFEA_HIP(hipStreamSynchronize(st));  // the host vectors end with this function
+    OFEA_HIP(hipStreamSynchronize(st));
+    int bw = 5;
+    for (const int4 &l : lst) {
+        if (l.x > CA_MAXCH) return fail(OFEA_ERR_LIMIT, "an aggregate couples to too many aggregates");
+        bw = std::max(bw, 6 * l.z + 5);
+    }
+    bw = std::min(bw, 6 * nagg - 1);
     w->co.per = per;
     w->co.ra = ra;
     w->co_nagg = nagg;
     w->co.nagg = nagg;
     w->co.ainv = w->co_ainv;
     w->co_bw = bw;
-    w->co_d_host = std::move(d);
+    w->co_d_host.clear();  // downloaded by coarse_apply when needed
     w->iperm_host = std::move(iperm);
     w->co_struct_valid = true;
     return OFEA_OK;
@@ -2645,18 +2826,15 @@ static int coarse_values(ofea_ctx *ctx, const double *vals, bool fp32)
     const unsigned g2 = (unsigned)((nc2 + RED_WG - 1) / RED_WG);
     hipLaunchKernelGGL((k_fill<double>), dim3(g2), dim3(RED_WG), 0, st, (int)nc2, w->co_A, 0.0);
     hipLaunchKernelGGL(k_coarse_ac, dim3(w->co_nagg), dim3(1024), 0, st, nc, w->nb_p, w->bcol_p, w->val_p, w->co.d,
-                       w->sl_ptr, w->sl_part, w->sl_cptr, w->ch_ptr, w->ch_beg, w->sl_blk, w->co_A);
+                       w->co_lst, w->sl_part, w->sl_cptr, w->ch_beg, w->sl_blk, w->co_A);
     const unsigned gb = (unsigned)(((long long)nc * w->co_bw + RED_WG - 1) / RED_WG);
     hipLaunchKernelGGL(k_band_sym, dim3(gb), dim3(RED_WG), 0, st, nc, w->co_bw, w->co_A);
-    if (w->co_bw + CP_NB <= CH_ROWS) {  // one panel kernel and one trailing-update kernel per CP_NB columns
-        const int ntu = (w->co_bw + CU_T - 1) / CU_T;
+    if (w->co_bw + CP_NB <= CH_ROWS) {  // one kernel per CP_NB columns
+        const int ntu = (std::max(w->co_bw - CP_NB, 1) + CS_T - 1) / CS_T;
         hipLaunchKernelGGL(k_chol_lift, dim3(1), dim3(CH_WG), 0, st, nc, w->co_A, w->co_flag);
-        for (int j0 = 0; j0 < nc; j0 += CP_NB) {
-            hipLaunchKernelGGL(k_chol_panel, dim3(1), dim3(CH_WG), 0, st, nc, w->co_bw, j0, w->co_A, w->co_flag);
-            if (j0 + CP_NB < nc)
-                hipLaunchKernelGGL(k_chol_update, dim3(ntu * (ntu + 1) / 2), dim3(256), 0, st, nc, w->co_bw, j0,
-                                   w->co_A, w->co_flag);
-        }
+        for (int j0 = 0; j0 < nc; j0 += CP_NB)
+            hipLaunchKernelGGL(k_chol_step, dim3(j0 == 0 ? 1 : 1 + ntu * (ntu + 1) / 2), dim3(CH_WG), 0, st, nc,
+                               w->co_bw, j0, w->co_A, w->co_flag);
     } else {
         hipLaunchKernelGGL(k_band_cholesky, dim3(1), dim3(CH_WG), 0, st, nc, w->co_bw, w->co_A, w->co_flag);
     }
@@ -2734,6 +2912,13 @@ int coarse_apply(ofea_ctx *ctx, const double *r, double *out)
         w->co_ainv_host.resize((size_t)nc * nc);
         OFEA_HIP(hipMemcpyAsync(w->co_ainv_host.data(), w->co_ainv, sizeof(float) * nc * nc, hipMemcpyDeviceToHost,
                                 ctx->stream));
+        OFEA_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    if (w->co_d_host.size() != 3 * (size_t)nf) {
+        w->co_d_host.resize(3 * (size_t)nf);
+        if (nf)
+            OFEA_HIP(hipMemcpyAsync(w->co_d_host.data(), w->co.d, sizeof(float) * 3 * (size_t)nf,
+                                    hipMemcpyDeviceToHost, ctx->stream));
         OFEA_HIP(hipStreamSynchronize(ctx->stream));
     }
     const float *d = w->co_d_host.data();  // offsets in the two-level node order

@@ -113,12 +113,20 @@ struct ofea_solver_ws {
     std::vector<int> iperm_host;
     bool co_struct_valid = false;  // order, pattern and block lists built for this mesh
     int co_bw = 0;                 // half-bandwidth of A_c
-    // A_c block lists: slots (coupled aggregates) per aggregate, chunks per
-    // slot and per aggregate, chunk starts, blocks ((node << 8) | slot in its row)
-    int *sl_ptr = nullptr, *sl_part = nullptr, *sl_cptr = nullptr, *ch_ptr = nullptr, *ch_beg = nullptr;
+    // A_c block lists, built on the device per aggregate b: its slots (coupled
+    // aggregates, ascending) at sl_part[b CA_MAXCH], chunk starts of each slot
+    // at sl_cptr[b (CA_MAXCH + 1)], chunk starts at ch_beg[b (CA_MAXCH + 1)]
+    // (ending with the aggregate's last block), the blocks ((node << 8) | slot
+    // in its row) in b's range of sl_blk, and co_lst[b] = (slots, chunks,
+    // largest |a - b| of a coupled aggregate a)
+    int *sl_part = nullptr, *sl_cptr = nullptr, *ch_beg = nullptr;
     int *sl_blk = nullptr;
+    int4 *co_lst = nullptr;
+    double *co_xf = nullptr;   // free-node coordinates, caller's free numbering (3 nfree)
+    void *co_arena = nullptr;  // device allocation holding every two-level buffer
     double *co_A = nullptr, *co_inv = nullptr;  // A_c (then L and L^T) and its inverse, FP64
     int *co_flag = nullptr;
     hipGraph_t graph[ofea::NSLOT] = {}, graph_tail[ofea::NSLOT] = {};  // CG_BATCH and CG_TAIL iterations
     hipGraphExec_t exec[ofea::NSLOT] = {}, exec_tail[ofea::NSLOT] = {};
+    int runs[ofea::NSLOT] = {};  // solves of each slot since its graphs were last dropped
 };
