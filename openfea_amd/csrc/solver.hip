@@ -1066,7 +1066,15 @@ static int ensure_ws(ofea_ctx *ctx)
     OFEA_HIP(hipMalloc((void **)&w->part_pq, sizeof(double) * CG_NB));
     OFEA_HIP(hipMalloc((void **)&w->sc, sizeof(DevScalars)));
     OFEA_HIP(hipMemset(w->sc, 0, sizeof(DevScalars)));
-    OFEA_HIP(hipHostMalloc((void **)&w->sc_host, (1 + NQ_PIPE) * sizeof(DevScalars), hipHostMallocDefault));
+    // pinned: the scalar blocks, the coarse factorization's flag, the
+    // per-aggregate results of the coarse block lists
+    OFEA_HIP(hipHostMalloc((void **)&w->sc_host,
+                           (1 + NQ_PIPE) * sizeof(DevScalars) + 64 + sizeof(int4) * (CG_NB / CG_AGG),
+                           hipHostMallocDefault));
+    w->co_flag_host = reinterpret_cast<int *>(w->sc_host + 1 + NQ_PIPE);
+    *w->co_flag_host = 0;
+    w->co_lst_host = reinterpret_cast<int4 *>(reinterpret_cast<char *>(w->co_flag_host) + 64);
+    OFEA_HIP(hipEventCreateWithFlags(&w->co_ev, hipEventDisableTiming));
     for (auto &e : w->pipe_ev) OFEA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const size_t nb = (size_t)std::max(n, 1), nfb = (size_t)std::max(nf, 1);
     for (double **p : {&w->v64.r, &w->v64.z, &w->v64.p, &w->v64.p2, &w->v64.q, &w->r64, &w->y64})
@@ -1098,6 +1106,7 @@ void solver_free(ofea_ctx *ctx)
     if (w->co_arena) (void)hipFree(w->co_arena);  // every two-level buffer
     if (w->sc) (void)hipFree(w->sc);
     if (w->sc_host) (void)hipHostFree(w->sc_host);
+    if (w->co_ev) (void)hipEventDestroy(w->co_ev);
     for (auto &e : w->pipe_ev)
         if (e) (void)hipEventDestroy(e);
     delete w;
@@ -1756,12 +1765,6 @@ __device__ inline void zcol(const float *dv, int q, double z[3])
     z[2] = q == 2 ? 1.0 : q == 3 ? y : q == 4 ? -x : 0.0;
 }
 
-// A_c = Z^T K Z on the reordered K: workgroup b owns the column blocks of
-// aggregate b.  The blocks (r, k) of b's nodes are listed per coupled
-// aggregate a ("slot") and the lists are cut into chunks; one thread per
-// (chunk, p, q) sums its entry over the chunk's blocks into LDS, then one per
-// (slot, p, q) adds the slot's chunk sums in order and writes A[6a+p][6b+q]
-// (fixed order, no atomics).
 constexpr int CA_MAXCH = 160;  // chunks, and coupled aggregates, per aggregate (k_co_lists checks)
 
 // ---- device part of the coarse structure (once per mesh) ----
@@ -1790,30 +1793,57 @@ __global__ __launch_bounds__(RED_WG) void k_co_rows(int nf, const int *__restric
     nb_p[i + 1] = node_blk[o + 1] - node_blk[o];
 }
 
-// in-place inclusive scan of a[0 .. n) in one workgroup: each thread sums a
-// contiguous segment, the segment sums are scanned in LDS, then each thread
-// rewrites its segment as running sums (integers: exact)
-constexpr int SC_WG = 1024;
-__global__ __launch_bounds__(SC_WG) void k_scan1(int n, int *__restrict__ a)
+// In-place inclusive scan of a[0 .. n) (integers, exact): k_scan_tiles scans
+// each tile of SC_TILE entries (rounds of SC_WG: wave scans, then the wave
+// totals) and stores the tile's total; k_scan_add adds the totals of the
+// earlier tiles to each tile.
+constexpr int SC_WG = 1024, SC_TILE = 4 * SC_WG;
+__device__ inline int block_scan_incl(int v, int *s_w)
 {
-    __shared__ int s_sum[SC_WG];
-    const int t = threadIdx.x, seg = (n + SC_WG - 1) / SC_WG;
-    const int j0 = min(n, t * seg), j1 = min(n, j0 + seg);
-    int sum = 0;
-    for (int j = j0; j < j1; ++j) sum += a[j];
-    s_sum[t] = sum;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        const int y = __shfl_up(v, dd);
+        if (lane >= dd) v += y;
+    }
+    if (lane == 63) s_w[wv] = v;
     __syncthreads();
-    for (int d = 1; d < SC_WG; d <<= 1) {  // Hillis-Steele over the segment sums
-        const int v = t >= d ? s_sum[t - d] : 0;
-        __syncthreads();
-        s_sum[t] += v;
-        __syncthreads();
+    int off = 0;
+    for (int w2 = 0; w2 < wv; ++w2) off += s_w[w2];
+    __syncthreads();  // s_w is reused by the next call
+    return v + off;
+}
+
+__global__ __launch_bounds__(SC_WG) void k_scan_tiles(int n, int *__restrict__ a, int *__restrict__ tot)
+{
+    __shared__ int s_w[SC_WG / 64];
+    __shared__ int s_carry;
+    const int t0 = blockIdx.x * SC_TILE;
+    if (threadIdx.x == 0) s_carry = 0;
+    for (int u = 0; u < SC_TILE / SC_WG; ++u) {
+        const int j = t0 + u * SC_WG + (int)threadIdx.x;
+        const int v = j < n ? a[j] : 0;
+        __syncthreads();  // s_carry of the previous round
+        const int carry = s_carry;
+        const int incl = block_scan_incl(v, s_w);
+        if (j < n) a[j] = carry + incl;
+        if (threadIdx.x == SC_WG - 1) s_carry = carry + incl;
     }
-    int run = s_sum[t] - sum;  // exclusive prefix of this segment
-    for (int j = j0; j < j1; ++j) {
-        run += a[j];
-        a[j] = run;
+    __syncthreads();
+    if (threadIdx.x == 0) tot[blockIdx.x] = s_carry;
+}
+
+__global__ __launch_bounds__(SC_WG) void k_scan_add(int n, int *__restrict__ a, const int *__restrict__ tot)
+{
+    __shared__ int s_off;
+    if (threadIdx.x == 0) {
+        int off = 0;
+        for (int t = 0; t < (int)blockIdx.x; ++t) off += tot[t];
+        s_off = off;
     }
+    __syncthreads();
+    const int off = s_off;
+    for (int j = blockIdx.x * SC_TILE + (int)threadIdx.x; j < min(n, (int)(blockIdx.x + 1) * SC_TILE); j += SC_WG)
+        a[j] += off;
 }
 
 // per aggregate a: the centroid of its nodes (a fixed-order sum) and the
@@ -1977,37 +2007,77 @@ __global__ __launch_bounds__(CL_WG) void k_co_lists(int nf, int L, int nagg, con
     }
 }
 
-__global__ __launch_bounds__(1024) void k_coarse_ac(int nc, const int *__restrict__ nb_p,
-                                                    const int *__restrict__ bcol_p, const double *__restrict__ val_p,
-                                                    const float *__restrict__ d, const int4 *__restrict__ lst,
-                                                    const int *__restrict__ sl_part, const int *__restrict__ sl_cptr,
-                                                    const int *__restrict__ ch_beg, const int *__restrict__ sl_blk,
-                                                    double *__restrict__ A)
+// A_c = Z^T K Z on the reordered K: workgroup b owns the column blocks of
+// aggregate b.  The blocks (r, k) of b's nodes are listed per coupled
+// aggregate a ("slot") and the lists are cut into chunks.  One wave per
+// chunk: lane l takes the chunk's blocks l, l + 64, ..., forms Z_s^T K_sr Z_r
+// (6 x 6; s the block's column node in a, r its row node in b) and sums them,
+// and the 36 sums are added over the wave by a fixed butterfly into LDS.  Then
+// one thread per (slot, p, q) adds the slot's chunk sums in order and writes
+// A[6a+p][6b+q] (fixed order, no atomics).
+constexpr int CA_WG = 512;  // 256 VGPRs per lane: the 36 sums and the block stay in registers
+__global__ __launch_bounds__(CA_WG) void k_coarse_ac(int nc, const int *__restrict__ nb_p,
+                                                     const int *__restrict__ bcol_p, const double *__restrict__ val_p,
+                                                     const float *__restrict__ d, const int4 *__restrict__ lst,
+                                                     const int *__restrict__ sl_part, const int *__restrict__ sl_cptr,
+                                                     const int *__restrict__ ch_beg, const int *__restrict__ sl_blk,
+                                                     double *__restrict__ A)
 {
     __shared__ double s_part[CA_MAXCH * 36];
     const int b = blockIdx.x;
     const int4 ls = lst[b];  // slots, chunks
+    if (ls.x > CA_MAXCH) return;  // lists that do not fit: the host reports it
     const int cb = b * (CA_MAXCH + 1), nch = ls.y;
-    for (int t = threadIdx.x; t < 36 * nch; t += blockDim.x) {
-        const int ch = cb + t / 36, pq = t % 36, p = pq / 6, q = pq % 6;
-        double acc = 0.0;
-        for (int e = ch_beg[ch]; e < ch_beg[ch + 1]; ++e) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int ch = wv; ch < nch; ch += CA_WG / 64) {
+        double acc[36];
+#pragma unroll
+        for (int t = 0; t < 36; ++t) acc[t] = 0.0;
+        for (int e = ch_beg[cb + ch] + lane; e < ch_beg[cb + ch + 1]; e += 64) {
             const int rk = sl_blk[e];  // (node << 8) | slot in the node's row
             const int r = rk >> 8, k = rk & 255;
             const int nb0 = nb_p[r], deg = nb_p[r + 1] - nb0;
-            double zs[3], zr[3];
-            zcol(d + 3 * (size_t)bcol_p[nb0 + k], p, zs);
-            zcol(d + 3 * (size_t)r, q, zr);
+            const int sn = bcol_p[nb0 + k];
             const double *blk = val_p + 9LL * nb0 + 3 * k;  // blk[c * 3 deg + m] = K(3s + m, 3r + c)
+            double kb[3][3];
 #pragma unroll
             for (int c = 0; c < 3; ++c)
 #pragma unroll
-                for (int m = 0; m < 3; ++m) acc += zs[m] * blk[c * 3 * deg + m] * zr[c];
+                for (int m = 0; m < 3; ++m) kb[m][c] = blk[c * 3 * deg + m];
+            const double rx = d[3 * (size_t)r], ry = d[3 * (size_t)r + 1], rz = d[3 * (size_t)r + 2];
+            const double sx = d[3 * (size_t)sn], sy = d[3 * (size_t)sn + 1], sz = d[3 * (size_t)sn + 2];
+            // T = K_sr Z_r (3 x 6): translations, then the rotation columns
+            // (0, -z, y), (z, 0, -x), (-y, x, 0) of the row node's offset
+            double T[3][6];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                T[m][0] = kb[m][0];
+                T[m][1] = kb[m][1];
+                T[m][2] = kb[m][2];
+                T[m][3] = -rz * kb[m][1] + ry * kb[m][2];
+                T[m][4] = rz * kb[m][0] - rx * kb[m][2];
+                T[m][5] = -ry * kb[m][0] + rx * kb[m][1];
+            }
+            // rows of Z_s^T: the same columns of the column node's offset
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                acc[q] += T[0][q];
+                acc[6 + q] += T[1][q];
+                acc[12 + q] += T[2][q];
+                acc[18 + q] += -sz * T[1][q] + sy * T[2][q];
+                acc[24 + q] += sz * T[0][q] - sx * T[2][q];
+                acc[30 + q] += -sy * T[0][q] + sx * T[1][q];
+            }
         }
-        s_part[t] = acc;
+#pragma unroll
+        for (int t = 0; t < 36; ++t) {
+            double v = acc[t];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if (lane == 0) s_part[ch * 36 + t] = v;
+        }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < 36 * ls.x; t += blockDim.x) {
+    for (int t = threadIdx.x; t < 36 * ls.x; t += CA_WG) {
         const int sl = t / 36, pq = t % 36;
         double acc = 0.0;
         for (int ch = sl_cptr[cb + sl]; ch < sl_cptr[cb + sl + 1]; ++ch) acc += s_part[ch * 36 + pq];
@@ -2749,7 +2819,8 @@ static int coarse_structure(ofea_ctx *ctx)
             {(void **)&w->dblk_p, sizeof(double) * 9 * nfb},    {(void **)&w->minv32_p, sizeof(float) * 9 * nfb},
             {(void **)&w->sl_blk, sizeof(int) * nbk},           {(void **)&w->sl_part, sizeof(int) * CL_NAGG * CA_MAXCH},
             {(void **)&w->sl_cptr, sizeof(int) * lch},          {(void **)&w->ch_beg, sizeof(int) * lch},
-            {(void **)&w->co_lst, sizeof(int4) * CL_NAGG},      {(void **)&w->co_xf, sizeof(double) * 3 * nfb}};
+            {(void **)&w->co_lst, sizeof(int4) * CL_NAGG},      {(void **)&w->co_xf, sizeof(double) * 3 * nfb},
+            {(void **)&w->co_scan, sizeof(int) * (nfb / SC_TILE + 1)}};
         size_t tot = 0;
         for (const Part &q : parts) tot += (q.bytes + 255) & ~(size_t)255;
         char *base = nullptr;
@@ -2769,44 +2840,54 @@ static int coarse_structure(ofea_ctx *ctx)
     hipLaunchKernelGGL((k_fill<double>), dim3(gz6), dim3(RED_WG), 0, st, 6 * CG_NB, w->co.zq, 0.0);
     hipLaunchKernelGGL((k_fill<double>), dim3((2 * CG_NC_MAX + RED_WG - 1) / RED_WG), dim3(RED_WG), 0, st,
                        2 * CG_NC_MAX, w->co.ya, 0.0);
-    std::vector<int4> lst(nagg);
     if (nf) {
         OFEA_HIP(hipMemcpyAsync(w->perm, perm.data(), sizeof(int) * nf, hipMemcpyHostToDevice, st));
         const unsigned gn = (unsigned)((ctx->nd + RED_WG - 1) / RED_WG), gf = (unsigned)((nf + RED_WG - 1) / RED_WG);
         hipLaunchKernelGGL(k_co_xfree, dim3(gn), dim3(RED_WG), 0, st, ctx->nd, ctx->d_resort, ctx->d_xyz, w->co_xf);
         hipLaunchKernelGGL(k_co_rows, dim3(gf), dim3(RED_WG), 0, st, nf, w->perm, ctx->d_node_blk, w->iperm, w->nb_p);
-        hipLaunchKernelGGL(k_scan1, dim3(1), dim3(SC_WG), 0, st, nf, w->nb_p + 1);
+        const unsigned ntile = (unsigned)((nf + SC_TILE - 1) / SC_TILE);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(SC_WG), 0, st, nf, w->nb_p + 1, w->co_scan);
+        if (ntile > 1) hipLaunchKernelGGL(k_scan_add, dim3(ntile), dim3(SC_WG), 0, st, nf, w->nb_p + 1, w->co_scan);
         hipLaunchKernelGGL(k_co_offsets, dim3(nagg), dim3(CD_WG), 0, st, nf, L, w->perm, w->co_xf, (float *)w->co.d);
         hipLaunchKernelGGL(k_perm_bcol, dim3(gf), dim3(RED_WG), 0, st, nf, w->perm, w->iperm, ctx->d_node_blk,
                            ctx->d_bcol, w->nb_p, w->bcol_p);
         hipLaunchKernelGGL(k_co_lists, dim3(nagg), dim3(CL_WG), 0, st, nf, L, nagg, w->nb_p, w->bcol_p, w->sl_blk,
                            w->sl_part, w->sl_cptr, w->ch_beg, w->co_lst);
         OFEA_HIP(hipGetLastError());
-        OFEA_HIP(hipMemcpyAsync(lst.data(), w->co_lst, sizeof(int4) * nagg, hipMemcpyDeviceToHost, st));
+        OFEA_HIP(hipMemcpyAsync(w->co_lst_host, w->co_lst, sizeof(int4) * nagg, hipMemcpyDeviceToHost, st));
     }
-    OFEA_HIP(hipStreamSynchronize(st));
-    int bw = 5;
-    for (const int4 &l : lst) {
-        if (l.x > CA_MAXCH) return fail(OFEA_ERR_LIMIT, "an aggregate couples to too many aggregates");
-        bw = std::max(bw, 6 * l.z + 5);
-    }
-    bw = std::min(bw, 6 * nagg - 1);
+    // the lists' results (bandwidth of A_c, limit) are read by coarse_values
+    // once its first kernels are queued
+    OFEA_HIP(hipEventRecord(w->co_ev, st));
+    w->co_lst_pending = true;
     w->co.per = per;
     w->co.ra = ra;
     w->co_nagg = nagg;
     w->co.nagg = nagg;
     w->co.ainv = w->co_ainv;
-    w->co_bw = bw;
     w->co_d_host.clear();  // downloaded by coarse_apply when needed
+    w->co_perm_host = std::move(perm);  // the source of a queued copy
     w->iperm_host = std::move(iperm);
     w->co_struct_valid = true;
+    return OFEA_OK;
+}
+
+// the coarse factorization's flag, copied to pinned memory after the set-up
+static int coarse_check(ofea_ctx *ctx)
+{
+    ofea_solver_ws *w = ctx->ws;
+    OFEA_HIP(hipStreamSynchronize(ctx->stream));
+    if (*w->co_flag_host) {
+        w->co_valid = false;
+        return fail(OFEA_ERR_STATE, "coarse matrix Z^T K Z is not positive definite");
+    }
     return OFEA_OK;
 }
 
 // A_c^-1 (and the reordered values with their block Jacobi) of the values
 // vals: K (the two-level solves) or a batched variant; fp32 adds the FP32
 // copy of the reordered values the mixed two-level methods read.
-static int coarse_values(ofea_ctx *ctx, const double *vals, bool fp32)
+static int coarse_values(ofea_ctx *ctx, const double *vals, bool fp32, bool defer_check = false)
 {
     ofea_solver_ws *w = ctx->ws;
     const int nf = w->nfree, nc = 6 * w->co_nagg;
@@ -2825,8 +2906,22 @@ static int coarse_values(ofea_ctx *ctx, const double *vals, bool fp32)
     const long long nc2 = (long long)nc * nc;
     const unsigned g2 = (unsigned)((nc2 + RED_WG - 1) / RED_WG);
     hipLaunchKernelGGL((k_fill<double>), dim3(g2), dim3(RED_WG), 0, st, (int)nc2, w->co_A, 0.0);
-    hipLaunchKernelGGL(k_coarse_ac, dim3(w->co_nagg), dim3(1024), 0, st, nc, w->nb_p, w->bcol_p, w->val_p, w->co.d,
+    hipLaunchKernelGGL(k_coarse_ac, dim3(w->co_nagg), dim3(CA_WG), 0, st, nc, w->nb_p, w->bcol_p, w->val_p, w->co.d,
                        w->co_lst, w->sl_part, w->sl_cptr, w->ch_beg, w->sl_blk, w->co_A);
+    if (w->co_lst_pending) {  // the block lists' results, copied while the kernels above run
+        OFEA_HIP(hipEventSynchronize(w->co_ev));
+        w->co_lst_pending = false;
+        int bw = 5;
+        for (int b = 0; b < w->co_nagg; ++b) {
+            const int4 l = w->co_lst_host[b];
+            if (l.x > CA_MAXCH) {
+                w->co_struct_valid = false;
+                return fail(OFEA_ERR_LIMIT, "an aggregate couples to too many aggregates");
+            }
+            bw = std::max(bw, 6 * l.z + 5);
+        }
+        w->co_bw = std::min(bw, nc - 1);
+    }
     const unsigned gb = (unsigned)(((long long)nc * w->co_bw + RED_WG - 1) / RED_WG);
     hipLaunchKernelGGL(k_band_sym, dim3(gb), dim3(RED_WG), 0, st, nc, w->co_bw, w->co_A);
     if (w->co_bw + CP_NB <= CH_ROWS) {  // one kernel per CP_NB columns
@@ -2841,10 +2936,6 @@ static int coarse_values(ofea_ctx *ctx, const double *vals, bool fp32)
     const bool blocked = w->co_bw + 64 <= LI_RING;  // blocked inverse: L^-1 by 64-row blocks, then Y^T Y
     if (!blocked) hipLaunchKernelGGL(k_band_upper, dim3(gb), dim3(RED_WG), 0, st, nc, w->co_bw, w->co_A);
     OFEA_HIP(hipGetLastError());
-    int flag = 0;
-    OFEA_HIP(hipMemcpyAsync(&flag, w->co_flag, sizeof(int), hipMemcpyDeviceToHost, st));
-    OFEA_HIP(hipStreamSynchronize(st));
-    if (flag) return fail(OFEA_ERR_STATE, "coarse matrix Z^T K Z is not positive definite");
     if (blocked) {
         hipLaunchKernelGGL(k_band_linv, dim3((nc + LI_WAVES - 1) / LI_WAVES), dim3(64 * LI_WAVES), 0, st, nc,
                            w->co_bw, w->co_A, w->co_inv);
@@ -2857,19 +2948,23 @@ static int coarse_values(ofea_ctx *ctx, const double *vals, bool fp32)
     }
     OFEA_HIP(hipGetLastError());
     w->co_ainv_host.clear();  // downloaded again by coarse_apply when needed
+    // the factorization's flag (a failed pivot leaves non-finite values in
+    // the inverse, nothing else): read here, or by the caller once the stream
+    // has drained (defer_check)
+    OFEA_HIP(hipMemcpyAsync(w->co_flag_host, w->co_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (!defer_check) return coarse_check(ctx);
     return OFEA_OK;
 }
 
-static int coarse_prepare(ofea_ctx *ctx)
+// defer_check: the caller calls coarse_check once the stream has drained
+static int coarse_prepare(ofea_ctx *ctx, bool defer_check = false)
 {
     ofea_solver_ws *w = ctx->ws;
     if (w->co_valid) return OFEA_OK;
-    const auto t0 = std::chrono::steady_clock::now();
     int rc;
     if (!w->co_struct_valid && (rc = coarse_structure(ctx))) return rc;
-    if ((rc = coarse_values(ctx, ctx->d_val, true))) return rc;
+    if ((rc = coarse_values(ctx, ctx->d_val, true, defer_check))) return rc;
     w->co_valid = true;
-    w->co_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return OFEA_OK;
 }
 
@@ -2977,8 +3072,12 @@ int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, 
                          method == OFEA_PCG_LEARNED_TWO_LEVEL || method == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED;
     if (method >= OFEA_PCG_TWO_LEVEL && method <= OFEA_PCG_TWO_LEVEL_MIXED32) {
         // two-level: the reordered system (b in, x out through the permutation)
-        if ((rc = coarse_prepare(ctx))) return rc;
-        if (learned && (rc = learned_factor(ctx, true))) return rc;  // G on the reordered rows
+        // the set-up is queued without a wait; its flag is checked after the solve
+        if ((rc = coarse_prepare(ctx, true))) return rc;
+        if (learned && (rc = learned_factor(ctx, true))) {  // G on the reordered rows
+            (void)coarse_check(ctx);
+            return rc;
+        }
         const int nf = w->nfree;
         const unsigned g3 = (unsigned)((3LL * nf + RED_WG - 1) / RED_WG);
         hipLaunchKernelGGL(k_gather_nodes<3>, dim3(g3), dim3(RED_WG), 0, ctx->stream, nf, w->perm, ctx->d_rhs, w->b_p);
@@ -3005,6 +3104,7 @@ int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, 
             rc = pcg_run<float, double, 2, true>(ctx, S, 9, w->val32_p, w->v64, w->minv_p, rtol, maxit, MIXED_RU, iters,
                                                  relres);
         }
+        if (int rc2 = coarse_check(ctx)) return rc2;
         if (rc != OFEA_OK && rc != OFEA_NOT_CONVERGED) return rc;
         hipLaunchKernelGGL(k_gather_nodes<3>, dim3(g3), dim3(RED_WG), 0, ctx->stream, nf, w->iperm, w->x_p, ctx->d_x);
         OFEA_HIP(hipGetLastError());

@@ -98,7 +98,11 @@ struct ofea_solver_ws {
     int co_nagg = 0;
     float *co_ainv = nullptr;  // (6 nagg)^2, row-major, FP32
     bool co_valid = false;
-    double co_setup_ms = 0.0;
+    int *co_flag_host = nullptr;  // pinned (inside the sc_host allocation): the factorization's flag
+    int4 *co_lst_host = nullptr;  // pinned (same allocation): per-aggregate results of the block lists
+    hipEvent_t co_ev = nullptr;   // the copy of those results has landed
+    bool co_lst_pending = false;  // the structure was rebuilt and co_lst_host not read yet
+    std::vector<int> co_perm_host;
     std::vector<float> co_d_host, co_ainv_host;  // host copies for ofea_coarse_apply
     // The two-level methods run on the free nodes reordered into compact parts
     // (one part = one aggregate = ra consecutive CG ranges of the new order),
@@ -123,6 +127,7 @@ struct ofea_solver_ws {
     int *sl_blk = nullptr;
     int4 *co_lst = nullptr;
     double *co_xf = nullptr;   // free-node coordinates, caller's free numbering (3 nfree)
+    int *co_scan = nullptr;    // tile totals of the row-length scan
     void *co_arena = nullptr;  // device allocation holding every two-level buffer
     double *co_A = nullptr, *co_inv = nullptr;  // A_c (then L and L^T) and its inverse, FP64
     int *co_flag = nullptr;
