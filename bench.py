@@ -612,6 +612,21 @@ def bench_newton(args, ctx, m, resort, E, nu, F, tip, L_):
     base = runs.get("plain_fp64_jacobi", {}).get("s")
     for v in runs.values():
         v["speedup_vs_plain"] = round(base / v["s"], 3) if base else None
+    tl = [k for k in runs if "two_level" in k]
+    if tl:  # the coarse set-up each new tangent costs the two-level inner solves (A_c, Cholesky, inverse)
+        ts = []
+        for _ in range(3):
+            ctx.nl_assemble(E, nu, u, copy=False)  # a new K: the coarse space is rebuilt on its next use
+            ctx.sync()
+            t = time.perf_counter()
+            L.coarse_aggregates(ctx)
+            ts.append(time.perf_counter() - t)
+        setup_ms = statistics.median(ts) * 1e3
+        for k in tl:
+            v = runs[k]
+            v["coarse_setup_ms_per_tangent"] = round(setup_ms, 3)
+            v["coarse_setup_share"] = round(v["newton_iters"] * setup_ms / (v["s"] * 1e3), 3)
+        log(f"newton: coarse set-up per tangent {setup_ms:.3f} ms")
     return runs, train
 
 
@@ -894,7 +909,8 @@ def compact_configs(args):
                                                "iters_max", "converged"]))
     c5 = bench_nl(sub(newton_methods="fp64_block_jacobi,two_level_fp64,learned_fp64,learned_two_level_fp64"))
     out["cfg5"] = dict(pick(c5, keys), workload=c5["config"]["workload"],
-                       newton={k: pick(v, ["s", "newton_iters", "pcg_iters", "converged"]) for k, v in
+                       newton={k: pick(v, ["s", "newton_iters", "pcg_iters", "converged", "coarse_setup_ms_per_tangent",
+                                           "coarse_setup_share"]) for k, v in
                                (c5.get("newton") or {}).items()},
                        learned_training=pick(c5.get("learned_training") or {}, ["loss_first", "loss_last", "train_s"]))
     out["wall_s"] = round(time.perf_counter() - t, 1)
