@@ -32,6 +32,12 @@
 //                 contributors (domains containing both r and s, ascending
 //                 domain: the reference's summation order) and writes the 9
 //                 CSC values once; no atomics.
+//   k_sfem_gather_chunks  the same sums over a chunk plan: the rows of the
+//                 domains around a chunk of spatially compact columns are
+//                 staged in LDS once (streaming loads, so they do not evict
+//                 the partly written value lines from L2), then one lane per
+//                 block reads its contributor pairs from LDS; the plain
+//                 gather serves meshes whose rings exceed the LDS.
 #include <algorithm>
 #include <cmath>
 #include <unordered_map>
@@ -66,6 +72,11 @@ struct SfemWs {
     int2 *contrib = nullptr;        // (row of r, row of s) per contributor
     long long ncontrib = 0;
     int *blk_r = nullptr;           // column node of each block
+    // chunk plan of k_sfem_gather_chunks (gnchunks = 0: the plain gather)
+    int gnchunks = 0;
+    int *g_col_ptr = nullptr, *g_cols = nullptr, *g_dom_ptr = nullptr, *g_blk_ptr = nullptr, *g_cptr = nullptr;
+    int2 *g_doms = nullptr;
+    uint32_t *g_con = nullptr;
 };
 
 __device__ inline void d_cross(const double a[3], const double b[3], double o[3])
@@ -256,11 +267,11 @@ __global__ __launch_bounds__(256) void k_sfem_gather(int nblk, const int *__rest
     const int r = blk_r[b];
     const int k = b - node_blk[r];
     double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int t = cptr[b]; t < cptr[b + 1]; ++t) {
-        const int2 pr = contrib[t];
+    // contributors two at a time: both pairs and then all four rows are
+    // loaded before the sums, which keep the ascending order
+    auto load_rows = [&](int2 pr, double x[12], double y[12]) {
         const double2 *x2 = reinterpret_cast<const double2 *>(rows + 12 * (size_t)pr.x);
         const double2 *y2 = reinterpret_cast<const double2 *>(rows + 12 * (size_t)pr.y);
-        double x[12], y[12];
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
             const double2 u = x2[i], v = y2[i];
@@ -269,12 +280,29 @@ __global__ __launch_bounds__(256) void k_sfem_gather(int nblk, const int *__rest
             y[2 * i] = v.x;
             y[2 * i + 1] = v.y;
         }
+    };
+    auto add = [&](const double x[12], const double y[12]) {
 #pragma unroll
         for (int v = 0; v < 4; ++v)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
 #pragma unroll
                 for (int d = 0; d < 3; ++d) M[3 * c + d] += x[3 * v + c] * y[3 * v + d];
+    };
+    int t = cptr[b];
+    const int t1 = cptr[b + 1];
+    for (; t + 1 < t1; t += 2) {
+        const int2 p0 = contrib[t], p1 = contrib[t + 1];
+        double x0[12], y0[12], x1[12], y1[12];
+        load_rows(p0, x0, y0);
+        load_rows(p1, x1, y1);
+        add(x0, y0);
+        add(x1, y1);
+    }
+    if (t < t1) {
+        double x0[12], y0[12];
+        load_rows(contrib[t], x0, y0);
+        add(x0, y0);
     }
     const double tr = M[0] + M[4] + M[8];
     const int base = col_ptr[3 * r], stride = col_ptr[3 * r + 1] - base;
@@ -283,6 +311,102 @@ __global__ __launch_bounds__(256) void k_sfem_gather(int nblk, const int *__rest
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int d = 0; d < 3; ++d) dst[c * stride + d] = lam * M[3 * c + d] + mu * M[3 * d + c] + (c == d ? mu * tr : 0.0);
+}
+
+// The same block sums over a chunk plan (built by sfem_setup): one workgroup
+// per chunk of spatially compact columns.  The rows of every domain that
+// touches a chunk column are staged in LDS first (each domain's rows are
+// contiguous in memory: one wave copies a domain at a time), then one lane
+// per block of the chunk's columns sums its contributors from LDS -- the
+// same pairs in the same ascending-domain order as k_sfem_gather, so the
+// values are the same bits.
+constexpr int SG_WG = 1024;
+constexpr int SG_ROWS = 1600;  // staged rows per chunk (12 doubles each)
+constexpr int SG_COLS = 64;   // columns per chunk
+__global__ __launch_bounds__(SG_WG) void k_sfem_gather_chunks(int nchunks, const int *__restrict__ ch_col_ptr,
+                                                              const int *__restrict__ ch_cols,
+                                                              const int *__restrict__ ch_dom_ptr,
+                                                              const int2 *__restrict__ ch_doms,
+                                                              const int *__restrict__ ch_blk_ptr,
+                                                              const int *__restrict__ gcptr,
+                                                              const uint32_t *__restrict__ gcon,
+                                                              const int *__restrict__ dom_mptr,
+                                                              const int *__restrict__ node_blk,
+                                                              const int *__restrict__ col_ptr,
+                                                              const double *__restrict__ rows, double lam, double mu,
+                                                              double *__restrict__ val)
+{
+    __shared__ __align__(16) double s_rows[SG_ROWS * 12];
+    __shared__ int s_grow[SG_ROWS];  // global row of each staged row
+    __shared__ int s_cb[SG_COLS + 1], s_col[SG_COLS], s_nrow;
+    const int ch = xcd_remap(blockIdx.x, nchunks), t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int c0 = ch_col_ptr[ch], ncol = ch_col_ptr[ch + 1] - c0;
+    if (t == 0) {
+        s_cb[0] = 0;
+        for (int i = 0; i < ncol; ++i) {
+            const int r = ch_cols[c0 + i];
+            s_col[i] = r;
+            s_cb[i + 1] = s_cb[i] + node_blk[r + 1] - node_blk[r];
+        }
+    }
+    // the global row of every staged row (one thread per domain), then one
+    // flat copy of 16-byte pieces, eight loads issued before their stores
+    const int d0 = ch_dom_ptr[ch], ndom = ch_dom_ptr[ch + 1] - d0;
+    for (int i = t; i < ndom; i += SG_WG) {
+        const int2 e = ch_doms[d0 + i];  // domain, first LDS row
+        const int g0 = dom_mptr[e.x], cnt = dom_mptr[e.x + 1] - g0;
+        for (int j = 0; j < cnt; ++j) s_grow[e.y + j] = g0 + j;
+        if (i == ndom - 1) s_nrow = e.y + cnt;
+    }
+    __syncthreads();
+    {
+        const int npc = 6 * s_nrow;  // 16-byte pieces
+        double2 *dst = reinterpret_cast<double2 *>(s_rows);
+        constexpr int U = 8;
+        for (int i0 = 0; i0 < npc; i0 += U * SG_WG) {
+            double2 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * SG_WG + t;
+                if (i < npc) {
+                    const double *q = rows + 12 * (size_t)s_grow[i / 6] + 2 * (i % 6);
+                    v[u].x = __builtin_nontemporal_load(q);
+                    v[u].y = __builtin_nontemporal_load(q + 1);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * SG_WG + t;
+                if (i < npc) dst[i] = v[u];
+            }
+        }
+    }
+    __syncthreads();
+    const int nb = s_cb[ncol], gb0 = ch_blk_ptr[ch];
+    for (int bl = t; bl < nb; bl += SG_WG) {
+        int ci = 0;
+        while (s_cb[ci + 1] <= bl) ++ci;
+        const int r = s_col[ci], k = bl - s_cb[ci];
+        double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int q = gcptr[gb0 + bl]; q < gcptr[gb0 + bl + 1]; ++q) {
+            const uint32_t u = gcon[q];
+            const double *x = s_rows + 12 * (u & 0xFFFFu), *y = s_rows + 12 * (u >> 16);
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) M[3 * c + d] += x[3 * v + c] * y[3 * v + d];
+        }
+        const double tr = M[0] + M[4] + M[8];
+        const int base = col_ptr[3 * r], stride = col_ptr[3 * r + 1] - base;
+        double *dst = val + base + 3 * k;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                dst[c * stride + d] = lam * M[3 * c + d] + mu * M[3 * d + c] + (c == d ? mu * tr : 0.0);
+    }
 }
 
 void free_ws(SfemWs *w)
@@ -302,6 +426,13 @@ void free_ws(SfemWs *w)
     dev_free(w->chunk_dom);
     dev_free(w->mdom);
     dev_free(w->pdom);
+    dev_free(w->g_col_ptr);
+    dev_free(w->g_cols);
+    dev_free(w->g_dom_ptr);
+    dev_free(w->g_blk_ptr);
+    dev_free(w->g_cptr);
+    dev_free(w->g_doms);
+    dev_free(w->g_con);
     delete w;
 }
 
@@ -443,6 +574,117 @@ void sfem_free(ofea_ctx *c)
     c->sfem_ws = nullptr;
 }
 
+// Chunk plan of k_sfem_gather_chunks: the free columns in Morton order of
+// their coordinates, cut greedily into chunks of at most SG_COLS columns whose
+// domains (the ring of each column's node) hold at most SG_ROWS rows; per
+// chunk its columns, its domains (ascending, with their first LDS row) and,
+// per block of its columns, the contributors as (LDS row of r, LDS row of s)
+// in the order of the plain gather's list.  Returns false when one column's
+// domains alone exceed SG_ROWS (the plain gather then serves the mesh).
+struct GatherPlan2 {
+    std::vector<int> col_ptr{0}, cols, dom_ptr{0}, blk_ptr{0}, cptr{0};
+    std::vector<int2> doms;
+    std::vector<uint32_t> con;
+};
+
+static bool sfem_chunk_plan(int nd, const double *xyz, const SfemHost &H, const int *resort, const Pattern &P,
+                            const std::vector<int> &cnt, const std::vector<int2> &contrib, GatherPlan2 &G)
+{
+    const int nf = P.nfree;
+    std::vector<int> node_of(nf, -1);
+    for (int i = 0; i < nd; ++i)
+        if (resort[i] >= 0) node_of[resort[i]] = i;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int r = 0; r < nf; ++r)
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = std::min(lo[c], xyz[3 * (size_t)node_of[r] + c]);
+            hi[c] = std::max(hi[c], xyz[3 * (size_t)node_of[r] + c]);
+        }
+    auto spread = [](unsigned v) {  // 10 bits -> every third bit
+        v &= 1023u;
+        v = (v | (v << 16)) & 0x030000FFu;
+        v = (v | (v << 8)) & 0x0300F00Fu;
+        v = (v | (v << 4)) & 0x030C30C3u;
+        v = (v | (v << 2)) & 0x09249249u;
+        return v;
+    };
+    std::vector<std::pair<unsigned, int>> key(nf);
+    for (int r = 0; r < nf; ++r) {
+        unsigned q[3];
+        for (int c = 0; c < 3; ++c) {
+            const double ext = hi[c] - lo[c];
+            const double u = ext > 0.0 ? (xyz[3 * (size_t)node_of[r] + c] - lo[c]) / ext : 0.0;
+            q[c] = (unsigned)std::min(1023.0, std::max(0.0, u * 1023.0));
+        }
+        key[r] = {spread(q[0]) | (spread(q[1]) << 1) | (spread(q[2]) << 2), r};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> dom_mark(nd, -1), chunk_doms, chunk_cols, row_dom(H.member.size());
+    for (int k = 0; k < nd; ++k)
+        for (int a = H.mptr[k]; a < H.mptr[k + 1]; ++a) row_dom[a] = k;
+    auto rows_of = [&](int k) { return H.mptr[k + 1] - H.mptr[k]; };
+    auto flush = [&]() {
+        if (chunk_cols.empty()) return;
+        std::sort(chunk_doms.begin(), chunk_doms.end());
+        std::vector<int> lrow0(chunk_doms.size());
+        int lr = 0;
+        for (size_t i = 0; i < chunk_doms.size(); ++i) {
+            lrow0[i] = lr;
+            G.doms.push_back(make_int2(chunk_doms[i], lr));
+            dom_mark[chunk_doms[i]] = (int)i;  // position in the chunk's sorted list
+            lr += rows_of(chunk_doms[i]);
+        }
+        for (int r : chunk_cols) {
+            G.cols.push_back(r);
+            for (int b = P.node_blk[r]; b < P.node_blk[r + 1]; ++b) {
+                for (int q = cnt[b]; q < cnt[b + 1]; ++q) {
+                    const int2 pr = contrib[q];  // rows (k, r), (k, s): the same domain k
+                    const int kdom = row_dom[pr.x];
+                    const int base = lrow0[dom_mark[kdom]] - H.mptr[kdom];
+                    G.con.push_back((uint32_t)(base + pr.x) | ((uint32_t)(base + pr.y) << 16));
+                }
+                G.cptr.push_back((int)G.con.size());
+            }
+        }
+        for (int k : chunk_doms) dom_mark[k] = -1;
+        G.col_ptr.push_back((int)G.cols.size());
+        G.dom_ptr.push_back((int)G.doms.size());
+        G.blk_ptr.push_back((int)G.cptr.size() - 1);
+        chunk_doms.clear();
+        chunk_cols.clear();
+    };
+    int rows = 0;
+    std::vector<int> add;
+    for (const auto &kr : key) {
+        const int r = kr.second, n = node_of[r];
+        add.clear();
+        int extra = 0;
+        for (int a = H.mptr[n]; a < H.mptr[n + 1]; ++a) {  // the domains holding n: its ring
+            const int k = H.member[a];
+            if (dom_mark[k] == -2) continue;
+            dom_mark[k] = -2;
+            add.push_back(k);
+            extra += rows_of(k);
+        }
+        for (int k : add) dom_mark[k] = -1;
+        int fresh = 0;  // domains already in the chunk do not count again
+        for (int k : add)
+            if (std::find(chunk_doms.begin(), chunk_doms.end(), k) == chunk_doms.end()) fresh += rows_of(k);
+        if (extra > SG_ROWS) return false;
+        if (!chunk_cols.empty() && ((int)chunk_cols.size() >= SG_COLS || rows + fresh > SG_ROWS)) {
+            flush();
+            rows = 0;
+            fresh = extra;
+        }
+        for (int k : add)
+            if (std::find(chunk_doms.begin(), chunk_doms.end(), k) == chunk_doms.end()) chunk_doms.push_back(k);
+        chunk_cols.push_back(r);
+        rows += fresh;
+    }
+    flush();
+    return G.con.size() < (1ull << 31);
+}
+
 int sfem_setup(ofea_ctx *c)
 {
     if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
@@ -541,6 +783,17 @@ int sfem_setup(ofea_ctx *c)
     int *d_blk_r = nullptr;
     if ((rc = dev_upload(d_blk_r, blk_r.data(), std::max<size_t>(blk_r.size(), 1)))) return rc;
     w->blk_r = d_blk_r;
+    GatherPlan2 G;
+    if (sfem_chunk_plan(c->nd, c->xyz.data(), H, c->resort.data(), c->P, cnt, contrib, G)) {
+        w->gnchunks = (int)G.col_ptr.size() - 1;
+        if ((rc = dev_upload(w->g_col_ptr, G.col_ptr.data(), G.col_ptr.size()))) return rc;
+        if ((rc = dev_upload(w->g_cols, G.cols.data(), std::max<size_t>(G.cols.size(), 1)))) return rc;
+        if ((rc = dev_upload(w->g_dom_ptr, G.dom_ptr.data(), G.dom_ptr.size()))) return rc;
+        if ((rc = dev_upload(w->g_doms, G.doms.data(), std::max<size_t>(G.doms.size(), 1)))) return rc;
+        if ((rc = dev_upload(w->g_blk_ptr, G.blk_ptr.data(), G.blk_ptr.size()))) return rc;
+        if ((rc = dev_upload(w->g_cptr, G.cptr.data(), G.cptr.size()))) return rc;
+        if ((rc = dev_upload(w->g_con, G.con.data(), std::max<size_t>(G.con.size(), 1)))) return rc;
+    }
     return OFEA_OK;
 }
 
@@ -557,7 +810,11 @@ int sfem_assemble(ofea_ctx *c, double E, double nu, double *nz_val)
                            w->pdom, w->vol, w->rows);
     OFEA_HIP(hipGetLastError());
     const int nblk = c->P.node_blk[c->P.nfree];
-    if (nblk > 0)
+    if (w->gnchunks > 0)
+        hipLaunchKernelGGL(k_sfem_gather_chunks, dim3(w->gnchunks), dim3(SG_WG), 0, c->stream, w->gnchunks,
+                           w->g_col_ptr, w->g_cols, w->g_dom_ptr, w->g_doms, w->g_blk_ptr, w->g_cptr, w->g_con,
+                           w->dom_mptr, c->d_node_blk, c->d_col_ptr, w->rows, lam, mu, c->d_val);
+    else if (nblk > 0)
         hipLaunchKernelGGL(k_sfem_gather, dim3((nblk + 255) / 256), dim3(256), 0, c->stream, nblk, w->blk_r,
                            c->d_node_blk, c->d_col_ptr, w->cptr, w->contrib, w->rows, lam, mu, c->d_val);
     OFEA_HIP(hipGetLastError());
