@@ -874,7 +874,7 @@ def bench_sfem(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("sfem_assemble_cfg2"),
                      "traffic_source": traffic_source("sfem_assemble_cfg2"),
-                     "kernel": "k_sfem_grad + k_sfem_gather (one assembly)", "algorithmic_bytes_per_launch": alg_bytes},
+                     "kernel": "k_sfem_grad + k_sfem_gather_chunks (one assembly)", "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }
 
