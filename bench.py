@@ -195,7 +195,8 @@ def bench_static(args, workload):
                    "two_level_mixed_fp32_all": A.PCG_TWO_LEVEL_MIXED32,
                    "learned_two_level_fp64": A.PCG_LEARNED_TWO_LEVEL,
                    "learned_two_level_mixed_fp32val": A.PCG_LEARNED_TWO_LEVEL_MIXED,
-                   "learned_two_level_mixed_fp32val_bf16net": A.PCG_LEARNED_TWO_LEVEL_MIXED}
+                   "learned_two_level_mixed_fp32val_bf16net": A.PCG_LEARNED_TWO_LEVEL_MIXED,
+                   "learned_two_level_mixed_fp32_all": A.PCG_LEARNED_TWO_LEVEL_MIXED32}
     chosen = [] if args.no_solve else [m for m in args.solvers.split(",") if m]
     methods = [(m, all_methods[m]) for m in chosen]
     learned_info, params = {}, None
@@ -939,6 +940,7 @@ def main():
     ap.add_argument("--solvers", default="plain_fp64_jacobi,fp64_block_jacobi,mixed_fp32val_block_jacobi,"
                     "mixed_fp32_all_block_jacobi,learned_fp64,learned_mixed_fp32val,two_level_fp64,"
                     "two_level_mixed_fp32val,two_level_mixed_fp32_all,learned_two_level_fp64,learned_two_level_mixed_fp32val,"
+                    "learned_two_level_mixed_fp32_all,"
                     "learned_two_level_mixed_fp32val_bf16net")
     ap.add_argument("--train-steps", type=int, default=150)
     ap.add_argument("--train-lr", type=float, default=1e-2)

@@ -81,6 +81,7 @@ extern "C" {
 #define OFEA_PCG_LEARNED_TWO_LEVEL 8        /* FP64 values, learned blocks + coarse           */
 #define OFEA_PCG_LEARNED_TWO_LEVEL_MIXED 9  /* FP32 values, learned blocks + coarse           */
 #define OFEA_PCG_TWO_LEVEL_MIXED32 10       /* FP32 values and vectors, block Jacobi + coarse */
+#define OFEA_PCG_LEARNED_TWO_LEVEL_MIXED32 11 /* FP32 values and vectors, learned blocks + coarse */
 
 /* quantities of ofea_post_nodal */
 #define OFEA_POST_STRAIN 0  /* B u_e and its "Mises" value: what the reference writes as S_* */

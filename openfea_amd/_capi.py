@@ -36,6 +36,7 @@ PCG_TWO_LEVEL_MIXED = 7
 PCG_LEARNED_TWO_LEVEL = 8
 PCG_LEARNED_TWO_LEVEL_MIXED = 9
 PCG_TWO_LEVEL_MIXED32 = 10
+PCG_LEARNED_TWO_LEVEL_MIXED32 = 11
 
 FP32 = 0
 FP64 = 1

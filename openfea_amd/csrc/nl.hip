@@ -440,7 +440,7 @@ struct NewtonSolves {
     static bool is_learned(int m)
     {
         return m == OFEA_PCG_LEARNED || m == OFEA_PCG_LEARNED_MIXED || m == OFEA_PCG_LEARNED_TWO_LEVEL ||
-               m == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED;
+               m == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED || m == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED32;
     }
     // the same solver with block Jacobi in place of the learned blocks
     static int plain_of(int m)
@@ -448,6 +448,7 @@ struct NewtonSolves {
         return m == OFEA_PCG_LEARNED_MIXED             ? OFEA_PCG_MIXED
                : m == OFEA_PCG_LEARNED_TWO_LEVEL       ? OFEA_PCG_TWO_LEVEL
                : m == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED ? OFEA_PCG_TWO_LEVEL_MIXED
+               : m == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED32 ? OFEA_PCG_TWO_LEVEL_MIXED32
                                                        : OFEA_PCG_BLOCK_JACOBI;
     }
     NewtonSolves(int method, double *buf) : use(method), judged(!is_learned(method)), xs(buf) {}

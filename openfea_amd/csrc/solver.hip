@@ -3069,8 +3069,9 @@ int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, 
         return OFEA_OK;
     }
     const bool learned = method == OFEA_PCG_LEARNED || method == OFEA_PCG_LEARNED_MIXED ||
-                         method == OFEA_PCG_LEARNED_TWO_LEVEL || method == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED;
-    if (method >= OFEA_PCG_TWO_LEVEL && method <= OFEA_PCG_TWO_LEVEL_MIXED32) {
+                         method == OFEA_PCG_LEARNED_TWO_LEVEL || method == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED ||
+                         method == OFEA_PCG_LEARNED_TWO_LEVEL_MIXED32;
+    if (method >= OFEA_PCG_TWO_LEVEL && method <= OFEA_PCG_LEARNED_TWO_LEVEL_MIXED32) {
         // two-level: the reordered system (b in, x out through the permutation)
         // the set-up is queued without a wait; its flag is checked after the solve
         if ((rc = coarse_prepare(ctx, true))) return rc;
@@ -3098,6 +3099,10 @@ int solver_solve(ofea_ctx *ctx, int method, double rtol, int maxit, int *iters, 
             break;
         case OFEA_PCG_TWO_LEVEL_MIXED32:  // FP32 iteration vectors too: x and the true residual stay FP64
             rc = pcg_run<float, float, 1, true>(ctx, S, 10, w->val32_p, w->v32, w->minv32_p, rtol, maxit, MIXED_RU,
+                                                iters, relres);
+            break;
+        case OFEA_PCG_LEARNED_TWO_LEVEL_MIXED32:
+            rc = pcg_run<float, float, 2, true>(ctx, S, 11, w->val32_p, w->v32, w->minv32_p, rtol, maxit, MIXED_RU,
                                                 iters, relres);
             break;
         default:

@@ -47,7 +47,7 @@ struct FacMat {
 
 // captured graphs: jacobi, block, mixed, learned, mixed32, learned mixed, and
 // the four two-level variants (block, mixed, learned, learned mixed)
-constexpr int NSLOT = 11;
+constexpr int NSLOT = 12;
 // batches of the PCG host loop in flight at most, plus one
 constexpr int NQ_PIPE = 4;
 

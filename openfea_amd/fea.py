@@ -446,7 +446,8 @@ SOLVERS = {"GPU_PCG": A.PCG_JACOBI, "GPU_PCG_BLOCK": A.PCG_BLOCK_JACOBI, "GPU_PC
            "GPU_PCG_LEARNED_MIXED": A.PCG_LEARNED_MIXED, "GPU_PCG_TWO_LEVEL": A.PCG_TWO_LEVEL,
            "GPU_PCG_TWO_LEVEL_MIXED": A.PCG_TWO_LEVEL_MIXED, "GPU_PCG_LEARNED_TWO_LEVEL": A.PCG_LEARNED_TWO_LEVEL,
            "GPU_PCG_LEARNED_TWO_LEVEL_MIXED": A.PCG_LEARNED_TWO_LEVEL_MIXED,
-           "GPU_PCG_TWO_LEVEL_MIXED32": A.PCG_TWO_LEVEL_MIXED32}
+           "GPU_PCG_TWO_LEVEL_MIXED32": A.PCG_TWO_LEVEL_MIXED32,
+           "GPU_PCG_LEARNED_TWO_LEVEL_MIXED32": A.PCG_LEARNED_TWO_LEVEL_MIXED32}
 
 
 def solution_api(assam: AssambleStiffness, deck: Deck, solver_type: str = "GPU_PCG", rtol: float = 1e-10,

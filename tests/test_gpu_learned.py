@@ -141,6 +141,23 @@ def test_bf16_inference_on_matrix_cores(setup):
     assert np.array_equal(ln.outputs(), o32)
 
 
+@pytest.mark.parametrize("method", [A.PCG_LEARNED, A.PCG_LEARNED_MIXED, A.PCG_LEARNED_TWO_LEVEL,
+                                    A.PCG_LEARNED_TWO_LEVEL_MIXED, A.PCG_LEARNED_TWO_LEVEL_MIXED32])
+def test_learned_methods_match_direct(setup, method):
+    """Every learned method (FP64, FP32 values, two-level, FP32 values and
+    vectors with the coarse space) with a non-trivial network reaches the
+    direct solution within 1e-8 at rtol 1e-10 (true FP64 residual)."""
+    ctx, g, F, cp, ri, val = setup
+    ln = L.Learned(ctx, seed=4)
+    ln.set_params(random_params(6))
+    ctx.set_rhs(F)
+    x, it, rel, ok = ctx.solve(F, method=method, rtol=1e-10, maxit=100000)
+    xd = spla.spsolve(sp.csc_matrix((val, ri, cp), shape=(len(cp) - 1,) * 2), F)
+    say(f"learned method {method}: {it} iterations, relres {rel:.2e}")
+    assert ok and rel <= 1e-10
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
+
+
 def timed_solve(ctx, F, method, reps=3):
     best = None
     for _ in range(reps):
