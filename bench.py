@@ -601,6 +601,7 @@ def bench_newton(args, ctx, m, resort, E, nu, F, tip, L_):
     for name in [x for x in args.newton_methods.split(",") if x]:
         meth = newton_all[name]
         ln.precision(A.BF16 if name.endswith("bf16net") else A.FP32)
+        b0 = L.coarse_builds(ctx)
         ctx.sync()
         t = time.perf_counter()
         u, nt, lt, rel, ok = ctx.nl_solve(E, nu, F, n_incr=10, tol=1e-8, max_newton=25, method=meth,
@@ -608,7 +609,7 @@ def bench_newton(args, ctx, m, resort, E, nu, F, tip, L_):
         el = time.perf_counter() - t
         uz = ofa.reset_displacement(resort, u).reshape(-1, 3)[tip, 2].mean()
         runs[name] = {"s": round(el, 3), "newton_iters": nt, "pcg_iters": lt, "relres": rel, "converged": ok,
-                      "tip_uz_over_L": float(uz / L_)}
+                      "tip_uz_over_L": float(uz / L_), "coarse_builds": L.coarse_builds(ctx) - b0}
         log(f"newton {name}: {runs[name]}")
     base = runs.get("plain_fp64_jacobi", {}).get("s")
     for v in runs.values():
@@ -625,9 +626,9 @@ def bench_newton(args, ctx, m, resort, E, nu, F, tip, L_):
         setup_ms = statistics.median(ts) * 1e3
         for k in tl:
             v = runs[k]
-            v["coarse_setup_ms_per_tangent"] = round(setup_ms, 3)
-            v["coarse_setup_share"] = round(v["newton_iters"] * setup_ms / (v["s"] * 1e3), 3)
-        log(f"newton: coarse set-up per tangent {setup_ms:.3f} ms")
+            v["coarse_setup_ms_per_build"] = round(setup_ms, 3)
+            v["coarse_setup_share"] = round(v["coarse_builds"] * setup_ms / (v["s"] * 1e3), 3)
+        log(f"newton: coarse set-up per build {setup_ms:.3f} ms")
     return runs, train
 
 
@@ -910,8 +911,8 @@ def compact_configs(args):
                                                "iters_max", "converged"]))
     c5 = bench_nl(sub(newton_methods="fp64_block_jacobi,two_level_fp64,learned_fp64,learned_two_level_fp64"))
     out["cfg5"] = dict(pick(c5, keys), workload=c5["config"]["workload"],
-                       newton={k: pick(v, ["s", "newton_iters", "pcg_iters", "converged", "coarse_setup_ms_per_tangent",
-                                           "coarse_setup_share"]) for k, v in
+                       newton={k: pick(v, ["s", "newton_iters", "pcg_iters", "converged", "coarse_builds",
+                                           "coarse_setup_ms_per_build", "coarse_setup_share"]) for k, v in
                                (c5.get("newton") or {}).items()},
                        learned_training=pick(c5.get("learned_training") or {}, ["loss_first", "loss_last", "train_s"]))
     out["wall_s"] = round(time.perf_counter() - t, 1)

@@ -356,6 +356,10 @@ int ofea_coarse_apply(ofea_ctx *ctx, const double *r, double *out);
  * the free nodes (slabs across the longest axis, cut by recursive coordinate
  * bisection), all of the same node count except the last. */
 int ofea_coarse_aggregates(ofea_ctx *ctx, int *agg);
+/* Coarse inverses A_c^-1 built since the mesh was set (no counterpart in the
+ * reference): every two-level solve after a new assembly builds one, the
+ * Newton drivers only when a tangent's solve slows down (>= 0, or an error). */
+int ofea_coarse_builds(ofea_ctx *ctx);
 
 /* ---- combined-approximation (CA) reanalysis on the resident K0 ----
  * ofea_ca_delta: Delta K on the pattern of the resident K (nnz values, copied

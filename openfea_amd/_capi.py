@@ -99,6 +99,7 @@ SIGNATURES = {
     "ofea_learned_precision": (_i, [_vp, _i]),
     "ofea_coarse_apply": (_i, [_vp, _vp, _vp]),
     "ofea_coarse_aggregates": (_i, [_vp, _vp]),
+    "ofea_coarse_builds": (_i, [_vp]),
     "ofea_assemble_batch": (_i, [_vp, _i, _vp, _d, _vp]),
     "ofea_solve_batch": (_i, [_vp, _i, _i, _vp, _vp, _d, _i, _vp, _vp]),
     "ofea_solve_batch_pc": (_i, [_vp, _i, _i, _vp, _vp, _d, _i, _i, _vp, _vp]),

@@ -600,6 +600,12 @@ int ofea_coarse_aggregates(ofea_ctx *c, int *agg)
     return coarse_aggregates(c, agg);
 }
 
+int ofea_coarse_builds(ofea_ctx *c)
+{
+    CHECK_CTX(c);
+    return coarse_builds(c);
+}
+
 int ofea_sync(ofea_ctx *c)
 {
     CHECK_CTX(c);

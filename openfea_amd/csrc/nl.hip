@@ -334,7 +334,9 @@ __global__ __launch_bounds__(NL_WG) void k_nl_tet(const double *__restrict__ xyz
 
 int nl_plan_cap(int type) { return type == OFEA_C3D8 ? NL_HEX_CAP : NL_TET_CAP; }
 
-static int nl_assemble_dev(ofea_ctx *c, double E, double nu)
+// lag_coarse: a Newton tangent whose two-level solve may keep the coarse
+// inverse of the previous tangent (solver_invalidate_values)
+static int nl_assemble_dev(ofea_ctx *c, double E, double nu, bool lag_coarse = false)
 {
     const int n = c->P.n;
     const bool mixed = c->has_type[0] && c->has_type[1];
@@ -356,7 +358,8 @@ static int nl_assemble_dev(ofea_ctx *c, double E, double nu)
     }
     c->assembled = true;
     c->kver++;
-    solver_invalidate(c);
+    if (lag_coarse) solver_invalidate_values(c);
+    else solver_invalidate(c);
     return OFEA_OK;
 }
 
@@ -452,6 +455,17 @@ struct NewtonSolves {
                                                        : OFEA_PCG_BLOCK_JACOBI;
     }
     NewtonSolves(int method, double *buf) : use(method), judged(!is_learned(method)), xs(buf) {}
+    // The two-level coarse inverse is rebuilt for a tangent only when the
+    // last solve took 25% more iterations than the first solve after the
+    // previous rebuild; otherwise the next tangent keeps it (its values, block
+    // preconditioners and learned factor are always the tangent's own).
+    int base_it = -1, last_it = 0;
+    bool lag_coarse()
+    {
+        if (base_it >= 0 && 4 * last_it <= 5 * base_it) return true;
+        base_it = -1;  // the next solve, on a rebuilt coarse space, sets the base
+        return false;
+    }
 
     int solve(ofea_ctx *c, int n, double inner_rtol)
     {
@@ -491,6 +505,8 @@ struct NewtonSolves {
                 linear += lit;
             }
         }
+        last_it = lit;
+        if (base_it < 0) base_it = lit;
         return src < 0 ? src : OFEA_OK;
     }
 };
@@ -535,6 +551,13 @@ struct NewtonBuffers {
     }
 };
 
+// The Newton drivers let tangents keep an earlier coarse inverse; on the way
+// out (any path) the resident K_T gets its own on the next solve.
+struct FullInvalidateOnExit {
+    ofea_ctx *c;
+    ~FullInvalidateOnExit() { solver_invalidate(c); }
+};
+
 // Load-stepped Newton on the total-Lagrangian St.Venant-Kirchhoff problem.
 // Reference loop: src/cae.cpp:209-253 (full load in one step, r = F - f_int,
 // K_T du = r, u += du, stop at ||r||_2/(3n) < 1e-6 or 100 loops; that rule is
@@ -555,13 +578,14 @@ int nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr, doub
     OFEA_HIP(hipMemcpyAsync(B.dF, F, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     OFEA_HIP(hipMemsetAsync(c->d_u, 0, sizeof(double) * std::max(n, 1), c->stream));
     NewtonSolves S(method, B.xs);
+    FullInvalidateOnExit guard{c};
     int newton = 0;
     double rel = 0.0;
     for (int k = 1; k <= n_incr; ++k) {
         const double lam = (double)k / n_incr;
         bool conv = false;
         for (int it = 0; it <= max_newton; ++it) {
-            if ((rc = nl_assemble_dev(c, E, nu))) break;
+            if ((rc = nl_assemble_dev(c, E, nu, S.lag_coarse()))) break;
             double rr = 0.0, ff = 0.0;
             if ((rc = nl_residual(c, n, lam, B.dF, B.part, B.hp, rr, ff))) break;
             rel = ff > 0.0 ? std::sqrt(rr / ff) : std::sqrt(rr);
@@ -609,10 +633,11 @@ int nl_solve_ref(ofea_ctx *c, double E, double nu, const double *F, double res_l
     OFEA_HIP(hipMemcpyAsync(B.dF, F, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     OFEA_HIP(hipMemsetAsync(c->d_u, 0, sizeof(double) * std::max(n, 1), c->stream));
     NewtonSolves S(method, B.xs);
+    FullInvalidateOnExit guard{c};
     double res_norm = 1.;
     int loop = 0;
     while (res_norm > res_limit && loop < max_loop) {
-        if ((rc = nl_assemble_dev(c, E, nu))) return rc;
+        if ((rc = nl_assemble_dev(c, E, nu, S.lag_coarse()))) return rc;
         double rr = 0.0, ff = 0.0;
         if ((rc = nl_residual(c, n, 1.0, B.dF, B.part, B.hp, rr, ff))) return rc;
         if ((rc = S.solve(c, n, inner_rtol))) return rc;

@@ -239,6 +239,11 @@ size_t noalloc_bytes();
 int solve_noalloc(ofea_ctx *ctx, int method, const double *d_b, double *d_x, double rtol, int maxit, int *iters,
                   double *relres);
 void solver_invalidate(ofea_ctx *ctx);  // K changed: preconditioner and FP32 copy are stale
+// K changed a little (the next Newton tangent): the reordered values and the
+// block preconditioners are rebuilt on the next two-level solve, the coarse
+// inverse A_c^-1 of the earlier K is kept (still an SPD coarse correction)
+void solver_invalidate_values(ofea_ctx *ctx);
+int coarse_builds(ofea_ctx *ctx);  // A_c^-1 built since the solver work space was created
 int stream_copy(ofea_ctx *ctx, long long bytes, int reps, float *ms);
 // explicit (explicit.hip)
 int explicit_run(ofea_ctx *ctx, double E, double nu, const double *mass, const double *f_ext, const double *v0,

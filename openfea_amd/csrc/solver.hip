@@ -1141,6 +1141,14 @@ void solver_invalidate(ofea_ctx *ctx)
     }
 }
 
+void solver_invalidate_values(ofea_ctx *ctx)
+{
+    if (ctx->ws) {
+        ctx->ws->precond_valid = false;
+        if (ctx->ws->co_valid) ctx->ws->co_vals_stale = true;
+    }
+}
+
 // The buffers one PCG run works in: the context's solver work space, or the
 // static scratch of ofea_solve_noalloc.
 struct PcgBufs {
@@ -2872,6 +2880,25 @@ static int coarse_structure(ofea_ctx *ctx)
     return OFEA_OK;
 }
 
+// vals reordered into the two-level order (val_p), its block-Jacobi inverses
+// and, with fp32, the FP32 copy the mixed two-level methods read
+static void coarse_refresh_values(ofea_ctx *ctx, const double *vals, bool fp32)
+{
+    ofea_solver_ws *w = ctx->ws;
+    const int nf = w->nfree;
+    const long long nnz = ctx->P.nnz;
+    hipStream_t st = ctx->stream;
+    if (nf) {
+        hipLaunchKernelGGL(k_perm_vals, dim3((nf + RED_WG / 64 - 1) / (RED_WG / 64)), dim3(RED_WG), 0, st, nf,
+                           w->perm, ctx->d_node_blk, w->nb_p, vals, w->val_p);
+        hipLaunchKernelGGL(k_precond_setup, dim3(w->nblk_node), dim3(RED_WG), 0, st, nf, w->nb_p, w->bcol_p, w->val_p,
+                           w->dinv_p, w->minv_p, w->minv32_p, w->dblk_p);
+    }
+    if (nnz && fp32)
+        hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((nnz + RED_WG - 1) / RED_WG)), dim3(RED_WG), 0,
+                           st, nnz, w->val_p, w->val32_p);
+}
+
 // the coarse factorization's flag, copied to pinned memory after the set-up
 static int coarse_check(ofea_ctx *ctx)
 {
@@ -2890,18 +2917,9 @@ static int coarse_check(ofea_ctx *ctx)
 static int coarse_values(ofea_ctx *ctx, const double *vals, bool fp32, bool defer_check = false)
 {
     ofea_solver_ws *w = ctx->ws;
-    const int nf = w->nfree, nc = 6 * w->co_nagg;
-    const long long nnz = ctx->P.nnz;
+    const int nc = 6 * w->co_nagg;
     hipStream_t st = ctx->stream;
-    if (nf) {
-        hipLaunchKernelGGL(k_perm_vals, dim3((nf + RED_WG / 64 - 1) / (RED_WG / 64)), dim3(RED_WG), 0, st, nf,
-                           w->perm, ctx->d_node_blk, w->nb_p, vals, w->val_p);
-        hipLaunchKernelGGL(k_precond_setup, dim3(w->nblk_node), dim3(RED_WG), 0, st, nf, w->nb_p, w->bcol_p, w->val_p,
-                           w->dinv_p, w->minv_p, w->minv32_p, w->dblk_p);
-    }
-    if (nnz && fp32)
-        hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((nnz + RED_WG - 1) / RED_WG)), dim3(RED_WG), 0,
-                           st, nnz, w->val_p, w->val32_p);
+    coarse_refresh_values(ctx, vals, fp32);
     if (nc == 0) return OFEA_OK;
     const long long nc2 = (long long)nc * nc;
     const unsigned g2 = (unsigned)((nc2 + RED_WG - 1) / RED_WG);
@@ -2960,13 +2978,24 @@ static int coarse_values(ofea_ctx *ctx, const double *vals, bool fp32, bool defe
 static int coarse_prepare(ofea_ctx *ctx, bool defer_check = false)
 {
     ofea_solver_ws *w = ctx->ws;
-    if (w->co_valid) return OFEA_OK;
+    if (w->co_valid) {
+        if (w->co_vals_stale) {  // K's new values with the coarse inverse of the earlier K
+            coarse_refresh_values(ctx, ctx->d_val, true);
+            OFEA_HIP(hipGetLastError());
+            w->co_vals_stale = false;
+        }
+        return OFEA_OK;
+    }
+    w->co_vals_stale = false;
     int rc;
     if (!w->co_struct_valid && (rc = coarse_structure(ctx))) return rc;
     if ((rc = coarse_values(ctx, ctx->d_val, true, defer_check))) return rc;
     w->co_valid = true;
+    w->co_builds++;
     return OFEA_OK;
 }
+
+int coarse_builds(ofea_ctx *ctx) { return ctx->ws ? ctx->ws->co_builds : 0; }
 
 // The two-level coarse space of the values vals (a batched variant, same
 // pattern as K): the mesh's aggregate order and block lists (built once per
@@ -2981,6 +3010,7 @@ int coarse_for_values(ofea_ctx *ctx, const double *vals, CoarseVariant &v)
     if (!w->co_struct_valid && (rc = coarse_structure(ctx))) return rc;
     w->co_valid = false;
     if ((rc = coarse_values(ctx, vals, false))) return rc;
+    w->co_builds++;
     v.nb_p = w->nb_p;
     v.bcol_p = w->bcol_p;
     v.val_p = w->val_p;

@@ -98,6 +98,8 @@ struct ofea_solver_ws {
     int co_nagg = 0;
     float *co_ainv = nullptr;  // (6 nagg)^2, row-major, FP32
     bool co_valid = false;
+    bool co_vals_stale = false;  // co_valid, but K's values changed since (solver_invalidate_values)
+    int co_builds = 0;           // coarse inverses built
     int *co_flag_host = nullptr;  // pinned (inside the sc_host allocation): the factorization's flag
     int4 *co_lst_host = nullptr;  // pinned (same allocation): per-aggregate results of the block lists
     hipEvent_t co_ev = nullptr;   // the copy of those results has landed

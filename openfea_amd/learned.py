@@ -41,6 +41,14 @@ def coarse_aggregates(ctx):
     return agg, nagg
 
 
+def coarse_builds(ctx):
+    """Coarse inverses A_c^-1 built since the mesh was set."""
+    n = lib().ofea_coarse_builds(ctx._h)
+    if n < 0:
+        check(n)
+    return n
+
+
 class Learned:
     """Handle on the learned preconditioner of one context."""
 

@@ -29,7 +29,7 @@ import scipy.sparse.linalg as spla
 from conftest import GOLDEN, rel_max
 from oracle import oracle as O
 import openfea_amd as ofa
-from openfea_amd import _capi as A, meshgen as mg
+from openfea_amd import _capi as A, learned as L, meshgen as mg
 
 pytestmark = pytest.mark.gpu
 E, NU = mg.E_STEEL, mg.NU_STEEL
@@ -106,9 +106,19 @@ def test_cfg5_ramp_vs_oracle_newton(ctx):
     p_star = 0.2 * 20.0 / ofa.reset_displacement(resort, x1).reshape(-1, 3)[tip, 2].mean()
     F = p_star * F1
     say(f"cfg5 ramp {m.ne} hexes, p* = {p_star:.2f} MPa, 10 increments")
+    b0 = L.coarse_builds(ctx)
     u, nt, lt, rr, ok = ctx.nl_solve(E, NU, F, n_incr=10, tol=1e-10, max_newton=30, method=A.PCG_TWO_LEVEL,
                                      inner_rtol=1e-12)
+    builds = L.coarse_builds(ctx) - b0
     assert ok, (nt, rr)
+    # tangents keep the coarse inverse of an earlier one until a solve slows down
+    say(f"  coarse inverses built for {nt} tangent solves: {builds}")
+    assert 1 <= builds < nt
+    # afterwards the resident K_T gets its own coarse space and solves to the true residual
+    b = np.random.default_rng(3).normal(size=ctx.n)
+    xk, itk, relk, okk = ctx.solve(b, method=A.PCG_TWO_LEVEL, rtol=1e-10)
+    assert okk and np.linalg.norm(b - ctx.spmv(xk)) <= 1e-10 * np.linalg.norm(b) * 1.01
+    assert L.coarse_builds(ctx) == b0 + builds + 1
     uo, no = O.nl_newton(m.conn, m.etype, m.xyz, resort, E, NU, F, cp, ri, n_incr=10, tol=1e-10)
     err = np.linalg.norm(u - uo) / np.linalg.norm(uo)
     tipz = ofa.reset_displacement(resort, u).reshape(-1, 3)[tip, 2].mean()
