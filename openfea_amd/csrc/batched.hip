@@ -1098,11 +1098,18 @@ int solve_batch(ofea_ctx *c, int nvar, int nrhs, const double *B, double *X, dou
     MbSys S;
     S.node_blk = c->d_node_blk;
     S.bcol = c->d_bcol;
+    // A variant keeps the coarse inverse built for an earlier one until its
+    // predecessor's slowest system took 25% more iterations than the slowest
+    // of the first variant after the last build.
+    int base_it = -1, last_it = 0;
+    std::vector<int> itl(nrhs);
     for (int v = 0; v < nvar; ++v) {
         const double *pv = w->valb + (size_t)v * w->nnz;
         if (two) {
             // the variant's coarse space; the solve runs on its reordered copy
-            int rc = coarse_for_values(c, pv, co);
+            const bool keep = base_it >= 0 && 4 * last_it <= 5 * base_it;
+            if (!keep) base_it = -1;
+            int rc = coarse_for_values(c, pv, co, keep);
             if (rc) return rc;
             if (6 * co.nagg > CG_NC_MAX) return fail(OFEA_ERR_LIMIT, "coarse space larger than the batched buffers");
             S.node_blk = co.nb_p;
@@ -1129,7 +1136,7 @@ int solve_batch(ofea_ctx *c, int nvar, int nrhs, const double *B, double *X, dou
         OFEA_HIP(hipMemcpyAsync(w->B, bi.data(), sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
         OFEA_HIP(hipStreamSynchronize(c->stream));  // pv and bi are pageable host memory reused next variant
         int rc;
-        int *itv = iters ? iters + (size_t)v * nrhs : nullptr;
+        int *itv = iters ? iters + (size_t)v * nrhs : itl.data();
         double *rv = relres ? relres + (size_t)v * nrhs : nullptr;
         switch (nrhs) {
         case 1: rc = mb_solve_variant<1>(c, w, S, rtol, maxit, bb, itv, rv); break;
@@ -1139,6 +1146,8 @@ int solve_batch(ofea_ctx *c, int nvar, int nrhs, const double *B, double *X, dou
         }
         if (rc < 0) return rc;
         if (rc != OFEA_OK) status = rc;
+        last_it = *std::max_element(itv, itv + nrhs);
+        if (base_it < 0) base_it = last_it;
         if (X) {
             OFEA_HIP(hipMemcpyAsync(xo.data(), w->X, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
             OFEA_HIP(hipStreamSynchronize(c->stream));

@@ -321,7 +321,7 @@ struct CoarseVariant {
     int nagg = 0, per = 1, ra = 1;
     const int *iperm = nullptr;                    // host: caller's node -> reordered node
 };
-int coarse_for_values(ofea_ctx *ctx, const double *vals, ofea::CoarseVariant &v);
+int coarse_for_values(ofea_ctx *ctx, const double *vals, ofea::CoarseVariant &v, bool keep_inverse = false);
 
 template <class T>
 int dev_upload(T *&d, const T *h, size_t count)

@@ -2868,6 +2868,7 @@ static int coarse_structure(ofea_ctx *ctx)
     // once its first kernels are queued
     OFEA_HIP(hipEventRecord(w->co_ev, st));
     w->co_lst_pending = true;
+    w->co_ainv_ready = false;
     w->co.per = per;
     w->co.ra = ra;
     w->co_nagg = nagg;
@@ -2906,6 +2907,7 @@ static int coarse_check(ofea_ctx *ctx)
     OFEA_HIP(hipStreamSynchronize(ctx->stream));
     if (*w->co_flag_host) {
         w->co_valid = false;
+        w->co_ainv_ready = false;
         return fail(OFEA_ERR_STATE, "coarse matrix Z^T K Z is not positive definite");
     }
     return OFEA_OK;
@@ -2989,8 +2991,10 @@ static int coarse_prepare(ofea_ctx *ctx, bool defer_check = false)
     w->co_vals_stale = false;
     int rc;
     if (!w->co_struct_valid && (rc = coarse_structure(ctx))) return rc;
+    w->co_ainv_ready = false;
     if ((rc = coarse_values(ctx, ctx->d_val, true, defer_check))) return rc;
     w->co_valid = true;
+    w->co_ainv_ready = true;  // the deferred check clears it again on failure
     w->co_builds++;
     return OFEA_OK;
 }
@@ -3000,17 +3004,26 @@ int coarse_builds(ofea_ctx *ctx) { return ctx->ws ? ctx->ws->co_builds : 0; }
 // The two-level coarse space of the values vals (a batched variant, same
 // pattern as K): the mesh's aggregate order and block lists (built once per
 // mesh), vals reordered into the solver's reordered value array with its
-// block Jacobi, and A_c^-1.  The single-system solver's own coarse space is
-// rebuilt from K on its next two-level solve.
-int coarse_for_values(ofea_ctx *ctx, const double *vals, CoarseVariant &v)
+// block Jacobi, and A_c^-1 -- or, with keep_inverse, the inverse built last
+// on this mesh (of another variant or of K: still an SPD coarse correction).
+// The single-system solver's own coarse space is rebuilt from K on its next
+// two-level solve.
+int coarse_for_values(ofea_ctx *ctx, const double *vals, CoarseVariant &v, bool keep_inverse)
 {
     int rc = solver_prepare(ctx);
     if (rc) return rc;
     ofea_solver_ws *w = ctx->ws;
     if (!w->co_struct_valid && (rc = coarse_structure(ctx))) return rc;
     w->co_valid = false;
-    if ((rc = coarse_values(ctx, vals, false))) return rc;
-    w->co_builds++;
+    if (keep_inverse && w->co_ainv_ready) {  // the values of vals, the inverse built last on this mesh
+        coarse_refresh_values(ctx, vals, false);
+        OFEA_HIP(hipGetLastError());
+    } else {
+        w->co_ainv_ready = false;
+        if ((rc = coarse_values(ctx, vals, false))) return rc;
+        w->co_ainv_ready = true;
+        w->co_builds++;
+    }
     v.nb_p = w->nb_p;
     v.bcol_p = w->bcol_p;
     v.val_p = w->val_p;

@@ -100,6 +100,7 @@ struct ofea_solver_ws {
     bool co_valid = false;
     bool co_vals_stale = false;  // co_valid, but K's values changed since (solver_invalidate_values)
     int co_builds = 0;           // coarse inverses built
+    bool co_ainv_ready = false;  // co_ainv holds an inverse built on this mesh (of K or a batched variant)
     int *co_flag_host = nullptr;  // pinned (inside the sc_host allocation): the factorization's flag
     int4 *co_lst_host = nullptr;  // pinned (same allocation): per-aggregate results of the block lists
     hipEvent_t co_ev = nullptr;   // the copy of those results has landed
