@@ -597,15 +597,18 @@ def bench_newton(args, ctx, m, resort, E, nu, F, tip, L_):
     runs = {}
     newton_all = {"plain_fp64_jacobi": A.PCG_JACOBI, "fp64_block_jacobi": A.PCG_BLOCK_JACOBI,
                   "learned_fp64": A.PCG_LEARNED, "learned_fp64_bf16net": A.PCG_LEARNED,
-                  "two_level_fp64": A.PCG_TWO_LEVEL, "learned_two_level_fp64": A.PCG_LEARNED_TWO_LEVEL}
+                  "two_level_fp64": A.PCG_TWO_LEVEL, "learned_two_level_fp64": A.PCG_LEARNED_TWO_LEVEL,
+                  "two_level_fp64_adaptive_inner": A.PCG_TWO_LEVEL,
+                  "learned_two_level_fp64_adaptive_inner": A.PCG_LEARNED_TWO_LEVEL}
     for name in [x for x in args.newton_methods.split(",") if x]:
         meth = newton_all[name]
         ln.precision(A.BF16 if name.endswith("bf16net") else A.FP32)
         b0 = L.coarse_builds(ctx)
         ctx.sync()
         t = time.perf_counter()
-        u, nt, lt, rel, ok = ctx.nl_solve(E, nu, F, n_incr=10, tol=1e-8, max_newton=25, method=meth,
-                                          inner_rtol=args.inner_rtol)
+        # "_adaptive_inner": Eisenstat-Walker forcing terms, no tighter than --inner-rtol
+        irt = -args.inner_rtol if name.endswith("_adaptive_inner") else args.inner_rtol
+        u, nt, lt, rel, ok = ctx.nl_solve(E, nu, F, n_incr=10, tol=1e-8, max_newton=25, method=meth, inner_rtol=irt)
         el = time.perf_counter() - t
         uz = ofa.reset_displacement(resort, u).reshape(-1, 3)[tip, 2].mean()
         runs[name] = {"s": round(el, 3), "newton_iters": nt, "pcg_iters": lt, "relres": rel, "converged": ok,
@@ -909,7 +912,8 @@ def compact_configs(args):
     out["cfg4"] = dict(pick(c4, keys + ["sequential_assembly_ms_64_variants"]), workload=c4["config"]["workload"],
                        batched_solve=pick(bs, ["systems", "s", "two_level_sequential_all_s", "two_level_batched",
                                                "iters_max", "converged"]))
-    c5 = bench_nl(sub(newton_methods="fp64_block_jacobi,two_level_fp64,learned_fp64,learned_two_level_fp64"))
+    c5 = bench_nl(sub(newton_methods="fp64_block_jacobi,two_level_fp64,learned_fp64,learned_two_level_fp64,"
+                                     "two_level_fp64_adaptive_inner"))
     out["cfg5"] = dict(pick(c5, keys), workload=c5["config"]["workload"],
                        newton={k: pick(v, ["s", "newton_iters", "pcg_iters", "converged", "coarse_builds",
                                            "coarse_setup_ms_per_build", "coarse_setup_share"]) for k, v in
@@ -931,7 +935,8 @@ def main():
     ap.add_argument("--seq-methods", default="plain_fp64_jacobi,fp64_block_jacobi,two_level_fp64",
                     help="one-system-at-a-time solvers compared with the batched solve (cfg4)")
     ap.add_argument("--newton-methods", default="plain_fp64_jacobi,fp64_block_jacobi,learned_fp64,learned_fp64_bf16net,"
-                    "two_level_fp64,learned_two_level_fp64",
+                    "two_level_fp64,learned_two_level_fp64,two_level_fp64_adaptive_inner,"
+                    "learned_two_level_fp64_adaptive_inner",
                     help="inner PCG methods of the Newton runs (cfg5)")
     ap.add_argument("--solve-reps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")

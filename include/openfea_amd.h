@@ -282,7 +282,11 @@ int ofea_nl_assemble(ofea_ctx *ctx, double E, double nu, const double *u, double
  * extended with load stepping: the load F (n) is ramped in n_incr equal
  * increments; each increment iterates K_T du = lam F - f_int, u += du until
  * ||lam F - f_int||_2 <= tol ||lam F||_2 (FP64) or max_newton iterations.
- * Linear solves: ofea_solve method with inner_rtol.  A learned method is
+ * Linear solves: ofea_solve method with inner_rtol; a negative inner_rtol
+ * asks for inexact Newton instead: the Eisenstat-Walker forcing terms
+ * (eta_k = 0.9 (|r_k| / |r_k-1|)^2, safeguarded by 0.9 eta_k-1^2 when that
+ * exceeds 0.1, at most 0.1) bounded below by -inner_rtol and by half of what
+ * the outer tolerance needs.  A learned method is
  * judged by the same solver with plain block Jacobi: on the first linear
  * solve the plain solver runs, then the learned one limited to the plain
  * iteration count, and the learned method serves the rest of the run only if

@@ -399,7 +399,7 @@ int ofea_nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr,
     CHECK_CTX(c);
     if (c->nd == 0) return fail(OFEA_ERR_STATE, "ofea_set_mesh first");
     NOT_SFEM(c);
-    if (!F || n_incr < 1 || !(tol > 0.0) || max_newton < 1 || !(inner_rtol > 0.0))
+    if (!F || n_incr < 1 || !(tol > 0.0) || max_newton < 1 || !(inner_rtol != 0.0) || !std::isfinite(inner_rtol))
         return fail(OFEA_ERR_ARG, "bad Newton arguments");
     return nl_solve(c, E, nu, F, n_incr, tol, max_newton, method, inner_rtol, u_out, newton_iters, linear_iters,
                     relres);

@@ -456,14 +456,15 @@ struct NewtonSolves {
     }
     NewtonSolves(int method, double *buf) : use(method), judged(!is_learned(method)), xs(buf) {}
     // The two-level coarse inverse is rebuilt for a tangent only when the
-    // last solve took 25% more iterations than the first solve after the
-    // previous rebuild; otherwise the next tangent keeps it (its values, block
-    // preconditioners and learned factor are always the tangent's own).
-    int base_it = -1, last_it = 0;
+    // last solve needed 25% more iterations per decade of residual reduction
+    // than the first solve after the previous rebuild; otherwise the next
+    // tangent keeps it (its values, block preconditioners and learned factor
+    // are always the tangent's own).
+    double base_rate = -1.0, last_rate = 0.0;
     bool lag_coarse()
     {
-        if (base_it >= 0 && 4 * last_it <= 5 * base_it) return true;
-        base_it = -1;  // the next solve, on a rebuilt coarse space, sets the base
+        if (base_rate >= 0.0 && last_rate <= 1.25 * base_rate) return true;
+        base_rate = -1.0;  // the next solve, on a rebuilt coarse space, sets the base
         return false;
     }
 
@@ -505,8 +506,8 @@ struct NewtonSolves {
                 linear += lit;
             }
         }
-        last_it = lit;
-        if (base_it < 0) base_it = lit;
+        last_rate = lit / std::max(0.5, -std::log10(inner_rtol));
+        if (base_rate < 0.0) base_rate = last_rate;
         return src < 0 ? src : OFEA_OK;
     }
 };
@@ -581,9 +582,16 @@ int nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr, doub
     FullInvalidateOnExit guard{c};
     int newton = 0;
     double rel = 0.0;
+    // inner_rtol < 0: inexact Newton with the forcing terms of Eisenstat and
+    // Walker (choice 2: eta_k = 0.9 (|r_k| / |r_k-1|)^2, kept at least
+    // 0.9 eta_k-1^2 when that exceeds 0.1), at most 0.1, never tighter than
+    // half of what the outer tolerance needs nor than -inner_rtol
+    const bool adaptive = inner_rtol < 0.0;
+    const double eta_min = adaptive ? -inner_rtol : inner_rtol;
     for (int k = 1; k <= n_incr; ++k) {
         const double lam = (double)k / n_incr;
         bool conv = false;
+        double eta_prev = 0.0, rn_prev = 0.0;
         for (int it = 0; it <= max_newton; ++it) {
             if ((rc = nl_assemble_dev(c, E, nu, S.lag_coarse()))) break;
             double rr = 0.0, ff = 0.0;
@@ -594,7 +602,23 @@ int nl_solve(ofea_ctx *c, double E, double nu, const double *F, int n_incr, doub
                 break;
             }
             if (it == max_newton) break;
-            if ((rc = S.solve(c, n, inner_rtol))) break;
+            double eta = eta_min;
+            if (adaptive) {
+                const double rn = std::sqrt(rr);
+                eta = 0.1;
+                if (it > 0 && rn_prev > 0.0) {
+                    const double q = rn / rn_prev;
+                    eta = 0.9 * q * q;
+                    const double sg = 0.9 * eta_prev * eta_prev;
+                    if (sg > 0.1) eta = std::max(eta, sg);
+                }
+                eta = std::min(eta, 0.1);
+                if (rn > 0.0 && ff > 0.0) eta = std::max(eta, 0.5 * tol * std::sqrt(ff) / rn);
+                eta = std::max(eta, eta_min);
+                eta_prev = eta;
+                rn_prev = rn;
+            }
+            if ((rc = S.solve(c, n, eta))) break;
             hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(NLR_WG), 0, c->stream, n, 1.0, c->d_x, c->d_u);
             newton++;
         }

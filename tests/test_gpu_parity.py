@@ -296,14 +296,18 @@ def test_tl_tangent_and_force_match_oracle(ctx, kind):
     assert rel_max(f, fo) < 1e-12
 
 
-def test_newton_matches_oracle(ctx):
+@pytest.mark.parametrize("method,inner", [(A.PCG_BLOCK_JACOBI, 1e-11), (A.PCG_BLOCK_JACOBI, -1e-11),
+                                          (A.PCG_TWO_LEVEL, -1e-11)])
+def test_newton_matches_oracle(ctx, method, inner):
+    """Load-stepped Newton against the oracle's (direct inner solves): with a
+    fixed inner tolerance and with the adaptive (Eisenstat-Walker) one."""
     m, resort, F = mg.cantilever_c3d8(6, 3, 3)
-    say("newton tl 6x3x3")
+    say(f"newton tl 6x3x3 method {method} inner {inner}")
     ctx.set_mesh(m.xyz, m.conn, m.etype, resort)
     cp, ri = ctx.pattern()
     load = 40 * F
     u, nt, lt, rel, ok = ctx.nl_solve(mg.E_STEEL, mg.NU_STEEL, load, n_incr=3, tol=1e-10, max_newton=20,
-                                      method=A.PCG_BLOCK_JACOBI, inner_rtol=1e-11)
+                                      method=method, inner_rtol=inner)
     uo, it_o = O.nl_newton(m.conn, m.etype, m.xyz, resort, mg.E_STEEL, mg.NU_STEEL, load, cp, ri, n_incr=3, tol=1e-10)
     say(f"newton: gpu {nt} iterations ({lt} PCG), oracle {it_o}; rel {rel:.2e}")
     assert ok and rel <= 1e-10
